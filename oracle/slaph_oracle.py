@@ -1,0 +1,886 @@
+"""CPU oracle for the sLapH contraction hot path.
+
+TEST INFRASTRUCTURE ONLY.  Only `tests/`, `__graft_entry__.smoke()` and the
+`cpu_baseline` leg of `bench.py` may import this module, and only as the
+checker or as the timed CPU baseline.  The product path (the HIP library under
+`slaph-contractions_amd/`) never calls it.
+
+This is a plain numpy restatement of the reference algorithm
+(pittlerf/sLapH-contractions, C++11 + Eigen), each function citing the
+reference file:line it follows.  complex128 throughout, native (non-Kahan)
+accumulation like the reference's default build (CMakeLists.txt:163-170).
+
+Pins (each checked in tests/test_oracle.py):
+  * gamma tables     - closed form tmLQCD basis and products
+                       (tests/gtest/gamma.cpp:18-104)
+  * momentum phases  - create_momenta == create_momenta_new
+                       (tests/gtest/create_momenta.cpp:5-20) + closed form
+  * Kahan            - tests/gtest/KahanAccumulatorTest.cpp:9-59 known answers
+  * dilution scheme  - tests/dilution-iterator/control.txt, byte-exact
+  * VdaggerV         - invariants (p=0 -> 1 for orthonormal V, VdV(-p) = VdV(p)^+)
+                       and brute-force triple loop on a tiny lattice
+  * everything else  - end to end: the 18 golden HDF5 files of
+                       tests/integration-L4 (2,307 datasets) at 1e-12 absolute,
+                       the reference's own gate (tests/integration-L4/compare:16)
+Parity unpinned: displaced operators (gauge links) are not restated here.
+"""
+from __future__ import annotations
+
+import math
+import os
+import re
+from collections import OrderedDict
+
+import numpy as np
+
+# ---------------------------------------------------------------------------
+# Gamma matrices (src/Gamma.cpp:18-246).  The sparse tables are derived from
+# the dense tmLQCD basis and its products, the same closed form the reference's
+# gtest (tests/gtest/gamma.cpp:18-104) checks them against.
+# ---------------------------------------------------------------------------
+_I = 1j
+_G0 = np.array([[0, 0, 1, 0], [0, 0, 0, 1], [1, 0, 0, 0], [0, 1, 0, 0]], complex)
+_G1 = np.array([[0, 0, 0, _I], [0, 0, _I, 0], [0, -_I, 0, 0], [-_I, 0, 0, 0]], complex)
+_G2 = np.array([[0, 0, 0, 1], [0, 0, -1, 0], [0, -1, 0, 0], [1, 0, 0, 0]], complex)
+_G3 = np.array([[0, 0, _I, 0], [0, 0, 0, -_I], [-_I, 0, 0, 0], [0, _I, 0, 0]], complex)
+
+
+def dense_gammas():
+    g5 = _G0 @ _G1 @ _G2 @ _G3
+    one = np.eye(4, dtype=complex)
+    return [_G0, _G1, _G2, _G3, one, g5,
+            _G0 @ g5, _G1 @ g5, _G2 @ g5, _G3 @ g5,
+            _G0 @ _G1, _G0 @ _G2, _G0 @ _G3,
+            _G0 @ _G1 @ g5, _G0 @ _G2 @ g5, _G0 @ _G3 @ g5]
+
+
+def make_gamma():
+    """Per gamma: (row[4], value[4]); column k has value[k] at row[k]."""
+    out = []
+    for g in dense_gammas():
+        rows, vals = [], []
+        for k in range(4):
+            nz = np.nonzero(g[:, k])[0]
+            assert len(nz) == 1
+            rows.append(int(nz[0]))
+            vals.append(complex(g[nz[0], k]))
+        out.append((rows, vals))
+    return out
+
+
+GAMMA = make_gamma()
+
+# ---------------------------------------------------------------------------
+# Accumulators (include/KahanAccumulator.hpp:14-78)
+# ---------------------------------------------------------------------------
+
+
+class KahanAccumulator:
+    """Neumaier variant comparing magnitudes (KahanAccumulator.hpp:23-32)."""
+
+    def __init__(self, zero=0.0):
+        self.s = zero
+        self.c = zero
+
+    def add(self, x):
+        t = self.s + x
+        if abs(self.s) >= abs(x):
+            self.c += (self.s - t) + x
+        else:
+            self.c += (x - t) + self.s
+        self.s = t
+        return self
+
+    def merge(self, other):  # KahanAccumulator.hpp:40-44
+        self.s += other.s
+        self.c += other.c
+        return self
+
+    def value(self):
+        return self.s + self.c
+
+
+# ---------------------------------------------------------------------------
+# Dilution scheme (include/dilution-iterator.hpp:21-356), block type only is
+# used by contract (Correlators.cpp:65); interlace kept for the fixture.
+# ---------------------------------------------------------------------------
+
+
+def block_pairs(num_slice, block_size):
+    """DilutionScheme::operator[] order (dilution-iterator.hpp:315-328)."""
+    nb = num_slice // block_size
+    out = []
+    for bso in range(nb):
+        for bsi in range(bso, nb):
+            out.append((bso, bsi))
+    return out
+
+
+def slice_pairs(bso, bsi, num_slice, num_block, interlace=False):
+    """BlockIterator enumeration (dilution-iterator.hpp:50-124, 232-282)."""
+    bs = num_slice // num_block
+    out = []
+    passes = [(bso, bsi)] if bso == bsi else [(bso, bsi), (bsi, bso)]
+    for a, b in passes:
+        if not interlace:
+            srcs = range(a * bs, (a + 1) * bs)
+            snks = range(b * bs, (b + 1) * bs)
+        else:
+            srcs = range(a, num_slice, bs)
+            snks = range(b, num_slice, bs)
+        for s in srcs:
+            for k in snks:
+                out.append((s, k))
+    return out
+
+
+def dilution_iterator_text(num_slice, num_block, interlace):
+    """Restates test_dilution_scheme (src/dilution-iterator.cpp:17-36)."""
+    name = "I" if interlace else "B"
+    bs = num_slice // num_block
+    lines = ["T = %d, T%s%d (Morningstar), T%s%d (Other):" % (num_slice, name, num_block, name, bs), ""]
+    for bso, bsi in block_pairs(num_slice, bs):
+        lines.append("%2d => %2d" % (bso, bsi))
+        for s, k in slice_pairs(bso, bsi, num_slice, num_block, interlace):
+            lines.append("  %2d -> %2d" % (s, k))
+    return "\n".join(lines) + "\n\n\n"
+
+
+def time_delta(t_source, t_sink, Lt):
+    """get_time_delta (Correlators.cpp:21-23): |(sink - source - Lt) % Lt|, C++ '%'."""
+    v = t_sink - t_source - Lt
+    r = int(math.fmod(v, Lt))
+    return abs(r)
+
+
+# ---------------------------------------------------------------------------
+# Infile grammar (src/global_data_input_handling_utils.cpp:54-347)
+# ---------------------------------------------------------------------------
+
+
+def momentum_combinations(p):
+    """create_all_momentum_combinations (..._utils.cpp:54-66)."""
+    out = []
+    for p1 in range(-p, p + 1):
+        for p2 in range(-p, p + 1):
+            for p3 in range(-p, p + 1):
+                if p1 * p1 + p2 * p2 + p3 * p3 == p:
+                    out.append((p1, p2, p3))
+    return out
+
+
+def make_quark(s, qid):
+    t = s.split(":")
+    if len(t) != 9:
+        raise ValueError("invalid quark " + s)
+    return dict(type=t[0], nrnd=int(t[1]), dilT_type=t[2], dilT=int(t[3]),
+                dilE_type=t[4], dilE=int(t[5]), dilD_type=t[6], dilD=int(t[7]),
+                id=qid, path=t[8])
+
+
+def make_operator_list(s):
+    """make_operator_list (..._utils.cpp:215-275). Returns list of
+    (gamma tuple, displacement tuple, momentum tuple)."""
+    ops = []
+    for op_t in s.split(":"):
+        gammas, disps, moms = [], [], []
+        for tok in op_t.split("."):
+            if tok.startswith("g"):
+                gammas.append(int(tok[1:]))
+            elif tok.startswith("d"):
+                if tok[1] == "0":
+                    disps.append(())
+                elif tok[1] in "<>":
+                    for part in tok[1:].split(","):
+                        disps.append(tuple((d[0], d[1]) for d in part.split("|")))
+                else:
+                    raise ValueError("bad displacement")
+            elif tok.startswith("p"):
+                if tok[1] == "(":
+                    moms = [[tuple(int(x) for x in tok[2:-1].split(","))]]
+                else:
+                    moms = [momentum_combinations(int(x)) for x in tok[1:].split(",")]
+            else:
+                raise ValueError("bad operator token")
+        for group in moms:
+            for mom in group:
+                for d in disps:
+                    ops.append((tuple(gammas), d, mom))
+    return ops
+
+
+def make_correlator(s):
+    """make_correlator (..._utils.cpp:302-347)."""
+    typ, qn, on, tot = None, [], [], []
+    for tok in s.split(":"):
+        if tok.startswith("C"):
+            typ = tok
+        elif tok.startswith("Q"):
+            qn.append(int(tok[1:]))
+        elif tok.startswith("Op"):
+            on.append(int(tok[2:]))
+        elif tok.startswith("G"):
+            pass
+        elif tok.startswith("P"):
+            if tok[1] == "(":
+                tot.append(tuple(int(x) for x in tok[2:-1].split(",")))
+            else:
+                for x in tok[1:].split(","):
+                    tot.extend(momentum_combinations(int(x)))
+        else:
+            raise ValueError("bad correlator token")
+    return dict(type=typ, quarks=qn, ops=on, tot_mom=tot)
+
+
+def parse_infile(text):
+    """Minimal INI reader for the keys read_parameters uses (global_data.cpp:61-249)."""
+    cfg = dict(quarks=[], operator_list=[], correlator_list=[],
+               momentum_cutoff={0: 4, 1: 5, 2: 6, 3: 7, 4: 4})
+    section = ""
+    for raw in text.splitlines():
+        line = raw.split("#")[0].strip()
+        if not line:
+            continue
+        if line.startswith("["):
+            section = line.strip("[]").strip()
+            continue
+        k, v = [x.strip() for x in line.split("=", 1)]
+        key = section + "." + k if section else k
+        if key == "quarks.quark":
+            cfg["quarks"].append(make_quark(v, len(cfg["quarks"])))
+        elif key == "operator_lists.operator_list":
+            cfg["operator_list"].append(make_operator_list(v))
+        elif key == "correlator_lists.correlator_list":
+            cfg["correlator_list"].append(make_correlator(v))
+        elif k == "momentum_cutoff_0":
+            cfg["momentum_cutoff"][0] = int(v)
+        else:
+            cfg[k] = v
+    for k in ("Lt", "Lx", "Ly", "Lz", "number_of_eigen_vec"):
+        cfg[k] = int(cfg[k])
+    return cfg
+
+
+# ---------------------------------------------------------------------------
+# Diagram specs (src/DiagramSpec.cpp:3-160): vertices (source, sink) and
+# traces as lists of (quark line name, q1, q2).
+# ---------------------------------------------------------------------------
+DIAGRAM_SPECS = {
+    "C1": (([0], []), [[("Q1", 0, 0)]]),
+    "C20": (([0], [1]), [[("Q1", 0, 1), ("Q1", 1, 0)]]),
+    "C2c": (([0], [1]), [[("Q0", 0, 1), ("Q2", 1, 0)]]),
+    "C20V": (([0], [1]), [[("Q1", 0, 0)], [("Q1", 1, 1)]]),
+    "C30": (([0, 2], [1]), [[("Q1", 0, 1), ("Q1", 1, 2), ("Q1", 2, 0)]]),
+    "C3c": (([0, 2], [1]), [[("Q1", 0, 1), ("Q0", 1, 2), ("Q2", 2, 0)]]),
+    "C30V": (([0, 1], [2]), [[("Q1", 0, 1), ("Q1", 1, 0)], [("Q1", 2, 2)]]),
+    "C40B": (([0, 3], [1, 2]), [[("Q1", 3, 0), ("Q1", 0, 1), ("Q1", 1, 2), ("Q1", 2, 3)]]),
+    "C4cB": (([0, 3], [1, 2]), [[("Q2", 3, 0), ("Q0", 0, 1), ("Q2", 1, 2), ("Q0", 2, 3)]]),
+    "C40C": (([0, 2], [1, 3]), [[("Q1", 3, 0), ("Q1", 0, 1), ("Q1", 1, 2), ("Q1", 2, 3)]]),
+    "C4cC": (([0, 2], [1, 3]), [[("Q2", 3, 0), ("Q0", 0, 1), ("Q2", 1, 2), ("Q0", 2, 3)]]),
+    "C40D": (([0, 2], [1, 3]), [[("Q1", 0, 1), ("Q1", 1, 0)], [("Q1", 2, 3), ("Q1", 3, 2)]]),
+    "C4cD": (([0, 2], [1, 3]), [[("Q0", 0, 1), ("Q2", 1, 0)], [("Q0", 2, 3), ("Q2", 3, 2)]]),
+    "C40V": (([0, 1], [2, 3]), [[("Q1", 0, 1), ("Q1", 1, 0)], [("Q1", 2, 3), ("Q1", 3, 2)]]),
+    "C4cV": (([0, 1], [2, 3]), [[("Q0", 0, 1), ("Q2", 1, 0)], [("Q0", 2, 3), ("Q2", 3, 2)]]),
+    "C6cC": (([0, 2, 4], [1, 3, 5]), [[("Q2", 5, 0), ("Q0", 0, 1), ("Q2", 1, 2),
+                                        ("Q0", 2, 3), ("Q2", 3, 4), ("Q0", 4, 5)]]),
+    "C6cD": (([0, 2, 4], [1, 3, 5]), [[("Q0", 0, 1), ("Q2", 1, 0)], [("Q0", 2, 3), ("Q2", 3, 2)],
+                                       [("Q0", 4, 5), ("Q2", 5, 4)]]),
+    "C6cCD": (([0, 2, 4], [1, 3, 5]), [[("Q2", 3, 0), ("Q0", 0, 1), ("Q2", 1, 2), ("Q0", 2, 3)],
+                                        [("Q0", 4, 5), ("Q2", 5, 4)]]),
+}
+
+TRACE_NAMES = ["trQ0Q2", "trQ1", "trQ1Q0Q2", "trQ1Q1", "trQ1Q1Q1", "trQ1Q1Q1Q1",
+               "trQ2Q0Q2Q0", "trQ2Q0Q2Q0Q2Q0"]
+
+
+def _cartesian(lengths):
+    """CartesianProduct (include/CartesianProduct.hpp:25-73): index 0 fastest."""
+    total = 1
+    for n in lengths:
+        total *= n
+    for i in range(total):
+        comb, ii = [], i
+        for n in lengths:
+            comb.append(ii % n)
+            ii //= n
+        yield comb
+
+
+def _unique_push_back(vec, elem, key=lambda x: x):
+    """unique_push_back (init_lookup_tables.cpp:337-346)."""
+    k = key(elem)
+    for i, e in enumerate(vec):
+        if key(e) == k:
+            return i
+    vec.append(elem)
+    return len(vec) - 1
+
+
+def build_quantum_numbers(corr, operator_list, cutoff):
+    """build_quantum_numbers_from_correlator_list (init_lookup_tables.cpp:42-163),
+    including the quirk that the sink Cartesian product uses the *source*
+    vertex list lengths (:78-82)."""
+    qn_op = [operator_list[n] for n in corr["ops"]]
+    (src, snk), _ = DIAGRAM_SPECS[corr["type"]]
+    lengths_source = [len(qn_op[i]) for i in src]
+    lengths_sink = [len(qn_op[i]) for i in src]
+    nvert = len(src) + len(snk)
+    out = []
+    for isrc in _cartesian(lengths_source):
+        qn_all = [None] * nvert
+        qn_source = []
+        for j, v in enumerate(src):
+            q = qn_op[v][isrc[j]]
+            qn_source.append(q)
+            qn_all[v] = q
+        p_so = tuple(sum(q[2][a] for q in qn_source) for a in range(3))
+        sum_norm = sum(sum(x * x for x in q[2]) for q in qn_source)
+        if corr["tot_mom"] and p_so not in corr["tot_mom"]:
+            continue
+        psq = sum(x * x for x in p_so)
+        if psq not in cutoff:
+            raise IndexError("momentum_cutoff.at(%d)" % psq)
+        if sum_norm > cutoff[psq]:
+            continue
+        if not snk:
+            out.append(list(qn_source))
+            continue
+        for isnk in _cartesian(lengths_sink):
+            qn_sink = []
+            for j, v in enumerate(snk):
+                q = qn_op[v][isnk[j]]
+                qn_sink.append(q)
+                qn_all[v] = q
+            p_si = tuple(sum(q[2][a] for q in qn_sink) for a in range(3))
+            if p_so == tuple(-x for x in p_si):
+                out.append(list(qn_all))
+    return out
+
+
+def dataset_name(ctype, quark_types, qns):
+    """build_hdf5_dataset_name (init_lookup_tables.cpp:196-211)."""
+    s = ctype + "_" + "".join(quark_types)
+    for g, d, p in qns:
+        s += "_p" + "".join(str(x) for x in p)
+        s += ".d" + ("000" if not d else "".join(a + b for a, b in d))
+        s += ".g" + "".join(str(x) for x in g)
+    return s
+
+
+def create_rnd_vec_id(quarks, q1, q2, is_loop):
+    """create_rnd_vec_id (init_lookup_tables.cpp:294-335)."""
+    s1 = sum(quarks[i]["nrnd"] for i in range(q1))
+    s2 = sum(quarks[i]["nrnd"] for i in range(q2))
+    e1 = s1 + quarks[q1]["nrnd"]
+    e2 = s2 + quarks[q2]["nrnd"]
+    if quarks[q1]["nrnd"] < 1 or quarks[q2]["nrnd"] < 1 or (q1 == q2 and quarks[q1]["nrnd"] < 2):
+        raise SystemExit("There are not enough random vectors for charged correlators")
+    if not is_loop:
+        return [(i, j) for i in range(s1, e1) for j in range(s2, e2) if i != j]
+    return [(i, i) for i in range(s1, e1)]
+
+
+def init_lookup_tables(cfg):
+    """init_lookup_tables (init_lookup_tables.cpp:470-556)."""
+    vdv_lookup = []  # list of (id, momentum, displacement)
+    ql_lookup = {"Q0": [], "Q1": [], "Q2": []}
+    trace_indices = {n: [] for n in TRACE_NAMES}
+    requests = {}
+    for corr in cfg["correlator_list"]:
+        qns = build_quantum_numbers(corr, cfg["operator_list"], cfg["momentum_cutoff"])
+        quark_types = [cfg["quarks"][i]["type"] for i in corr["quarks"]]
+        # build_VdaggerV_lookup (init_lookup_tables.cpp:235-264)
+        vdv_indices = []
+        for qn_vec in qns:
+            row = []
+            for g, d, p in qn_vec:
+                mp = tuple(-x for x in p)
+                hit = None
+                for k, (vid, vmom, vdisp) in enumerate(vdv_lookup):
+                    if vdisp == d and (vmom == p or vmom == mp):
+                        hit = k
+                        break
+                if hit is None:
+                    vdv_lookup.append((len(vdv_lookup), p, d))
+                    row.append((len(vdv_lookup) - 1, False))
+                else:
+                    dag = p != (0, 0, 0) and vdv_lookup[hit][1] == mp
+                    row.append((hit, dag))
+            vdv_indices.append(row)
+        (src, snk), traces = DIAGRAM_SPECS[corr["type"]]
+        nql = sum(len(t) for t in traces)
+        for d_i, qn_vec in enumerate(qns):
+            ql_ids = [None] * nql
+            for trace in traces:
+                for name, q1, q2 in trace:
+                    ric = create_rnd_vec_id(cfg["quarks"], corr["quarks"][q1], corr["quarks"][q2], q1 == q2)
+                    # build_Quarkline_lookup_one_qn (init_lookup_tables.cpp:348-370): keyed by q2
+                    qn = qn_vec[q2]
+                    vid, dag = vdv_indices[d_i][q2]
+                    cand = (vid, dag, tuple(qn[0]), tuple(ric))
+                    ql_ids[q2] = _unique_push_back(ql_lookup[name], cand)
+            dsname = dataset_name(corr["type"], quark_types, qn_vec)
+            # make_trace_request_factories (init_lookup_tables.cpp:418-462)
+            trs = []
+            for trace in traces:
+                name = "tr" + "".join(q[0] for q in trace)
+                verts = [q[1] for q in trace]
+                locs = tuple("source" if v in src else "sink" for v in verts)
+                ids = tuple(ql_ids[v] for v in verts)
+                tr_id = _unique_push_back(trace_indices[name], ids)
+                trs.append((name, tr_id, locs))
+            reqs = requests.setdefault(corr["type"], [])
+            _unique_push_back(reqs, (trs, dsname), key=lambda r: r[0])
+    unity = -1
+    for vid, vmom, vdisp in vdv_lookup:
+        if vmom == (0, 0, 0) and not vdisp:
+            unity = vid
+    return dict(vdv_lookup=vdv_lookup, index_of_unity=unity, ql_lookup=ql_lookup,
+                trace_indices=trace_indices, requests=dict(sorted(requests.items())))
+
+
+# ---------------------------------------------------------------------------
+# Inputs (Perambulator.cpp:12-71, RandomVector.hpp:38-40, EigenVector.cpp:35-76)
+# and file names (global_data_build_IO_names.cpp:7-85)
+# ---------------------------------------------------------------------------
+
+
+def perambulator_filename(q, cfgno, r, Lt, Vs):
+    return ("%s/cnfg%04d/rnd_vec_%02d/perambulator.rndvecnb%02d.%s.Tso%c%04d.Vso%c%04d."
+            "Dso%c%1d.TsiF%04d.SsiF%04d.DsiF4.CsiF3.smeared0.%05d") % (
+        q["path"], cfgno, r, r, q["type"], q["dilT_type"][-1], Lt // q["dilT"],
+        q["dilE_type"][-1], q["dilE"], q["dilD_type"][-1], q["dilD"], Lt, Vs, cfgno)
+
+
+def randomvector_filename(q, cfgno, r, nev):
+    return "%s/cnfg%04d/rnd_vec_%02d/randomvector.rndvecnb%02d.%s.nbev%04d.%04d" % (
+        q["path"], cfgno, r, r, q["type"], nev, cfgno)
+
+
+def resort_perambulator(raw, Lt, nev, dilT, dilE, dilD):
+    """Perambulator::read_perambulator re-sort (Perambulator.cpp:48-64): file row
+    4N*t+4v+d, column dilD*dilE*b+dilD*e+d' -> rows 4N*t+N*d+v, columns
+    dilE*dilD*b+dilE*d'+e."""
+    nb = Lt // dilT
+    a = raw.reshape(Lt, nev, 4, nb, dilE, dilD)
+    a = a.transpose(0, 2, 1, 3, 5, 4)
+    return np.ascontiguousarray(a.reshape(4 * nev * Lt, nb * dilE * dilD))
+
+
+def read_inputs(cfg, base, cfgno):
+    Lt, nev = cfg["Lt"], cfg["number_of_eigen_vec"]
+    Vs = cfg["Lx"] * cfg["Ly"] * cfg["Lz"]
+    perams, rnds = [], []
+    for q in cfg["quarks"]:
+        qq = dict(q)
+        qq["path"] = os.path.join(base, q["path"])
+        for r in range(q["nrnd"]):
+            raw = np.fromfile(perambulator_filename(qq, cfgno, r, Lt, Vs), dtype=np.complex128)
+            perams.append(resort_perambulator(raw, Lt, nev, q["dilT"], q["dilE"], q["dilD"]))
+            rnds.append(np.fromfile(randomvector_filename(qq, cfgno, r, nev), dtype=np.complex128))
+    ev = []
+    prefix = "%s/%s.%04d." % (os.path.join(base, cfg["path_eigenvectors"]), cfg["name_eigenvectors"], cfgno)
+    for t in range(Lt):
+        raw = np.fromfile(prefix + "%03d" % t, dtype=np.complex128)
+        ev.append(raw.reshape(nev, 3 * Vs).T.copy())  # column-major K x N
+    return perams, rnds, ev
+
+
+# ---------------------------------------------------------------------------
+# Operators (src/OperatorFactory.cpp:64-140, 220-326)
+# ---------------------------------------------------------------------------
+
+
+def create_momenta(Lx, Ly, Lz, mom):
+    """create_momenta (OperatorFactory.cpp:64-95): exp(-i 2pi p.x/L), site=(x*Ly+y)*Lz+z."""
+    x = np.arange(Lx)[:, None, None]
+    y = np.arange(Ly)[None, :, None]
+    z = np.arange(Lz)[None, None, :]
+    ph = (mom[0] * 2.0 * np.pi / Lx) * x + (mom[1] * 2.0 * np.pi / Ly) * y + (mom[2] * 2.0 * np.pi / Lz) * z
+    return np.exp(-1j * ph).reshape(-1)
+
+
+def create_momenta_new(Lx, Ly, Lz, mom):
+    """create_momenta_new (OperatorFactory.cpp:97-140): products of per-axis roots."""
+    def roots(p, L):
+        return np.exp(-1j * (p * 2.0 * np.pi / L) * np.arange(L))
+    rx, ry, rz = roots(mom[0], Lx), roots(mom[1], Ly), roots(mom[2], Lz)
+    return ((rx[:, None] * ry[None, :])[:, :, None] * rz[None, None, :]).reshape(-1)
+
+
+def vdaggerv(V, phase):
+    """VdaggerV = V^+ diag(phase per site, repeated over colour) V
+    (OperatorFactory.cpp:294-300)."""
+    mom = np.repeat(phase, 3)
+    return V.conj().T @ (mom[:, None] * V)
+
+
+def build_vdaggerv(cfg, lut, eigvecs):
+    Lx, Ly, Lz = cfg["Lx"], cfg["Ly"], cfg["Lz"]
+    nev = cfg["number_of_eigen_vec"]
+    out = {}
+    for vid, mom, disp in lut["vdv_lookup"]:
+        if disp:
+            raise NotImplementedError("displaced operators are not restated in the oracle")
+        ph = create_momenta(Lx, Ly, Lz, mom)
+        for t, V in enumerate(eigvecs):
+            if vid == lut["index_of_unity"]:
+                out[(vid, t)] = np.eye(nev, dtype=complex)
+            else:
+                out[(vid, t)] = vdaggerv(V, ph)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Diluted factors (src/DilutedFactorFactory.cpp:33-253)
+# A factor list entry: (matrix D x D, (r1, r2), used mask)
+# ---------------------------------------------------------------------------
+
+
+class Factories:
+    def __init__(self, cfg, lut, vdv, perams, rnds, dilT, dilE):
+        self.cfg, self.lut, self.vdv = cfg, lut, vdv
+        self.perams, self.rnds = perams, rnds
+        self.Lt = cfg["Lt"]
+        self.nev = cfg["number_of_eigen_vec"]
+        self.dilT, self.dilE = dilT, dilE
+        self.D = 4 * dilE
+        self.cache = {"Q0": {}, "Q1": {}, "Q2": {}}
+
+    def clear(self):
+        self.cache = {"Q0": {}, "Q1": {}, "Q2": {}}
+
+    def _vdv(self, entry, t):
+        m = self.vdv[(entry[0], t)]
+        return m.conj().T if entry[1] else m
+
+    def _rho(self, r, t):
+        return self.rnds[r].reshape(self.Lt, self.nev, 4)[t]  # RandomVector.hpp:38-40
+
+    def _dil(self, rho_col):
+        """N x dilE matrix: rho_col[v] at column v % dilE."""
+        N, dE = self.nev, self.dilE
+        out = np.zeros((N, dE), complex)
+        out[np.arange(N), np.arange(N) % dE] = rho_col
+        return out
+
+    def get(self, qt, key):
+        key = tuple(key)
+        c = self.cache[qt]
+        if key not in c:
+            c[key] = getattr(self, "_build_" + qt)(*key)
+        return c[key]
+
+    def _build_Q0(self, t):
+        """DilutedFactorFactory<Q0>::build (DilutedFactorFactory.cpp:33-101)."""
+        dE, D = self.dilE, self.D
+        res = []
+        for entry in self.lut["ql_lookup"]["Q0"]:
+            vdv = self._vdv(entry, t)
+            rows, vals = GAMMA[entry[2][0]]
+            lst = []
+            for r1, r2 in entry[3]:
+                rho1, rho2 = self._rho(r1, t), self._rho(r2, t)
+                M = np.concatenate([vdv @ self._dil(rho2[:, d]) for d in range(4)], axis=1)
+                Q = np.zeros((D, D), complex)
+                for d in range(4):
+                    g = rows[d]
+                    Q[g * dE:(g + 1) * dE, d * dE:(d + 1) * dE] += vals[d] * (
+                        self._dil(rho1[:, g]).conj().T @ M[:, d * dE:(d + 1) * dE])
+                lst.append((Q, (r1, r2), 0))
+            res.append(lst)
+        return res
+
+    def _build_Q1(self, t, b):
+        """DilutedFactorFactory<Q1>::build (DilutedFactorFactory.cpp:103-170)."""
+        dE, D, N = self.dilE, self.D, self.nev
+        res = []
+        for entry in self.lut["ql_lookup"]["Q1"]:
+            vdv = self._vdv(entry, t)
+            rows, vals = GAMMA[entry[2][0]]
+            lst = []
+            for r1, r2 in entry[3]:
+                rho1 = self._rho(r1, t)
+                P = self.perams[r2]
+                rv = np.concatenate([self._dil(rho1[:, d]).conj().T @ vdv for d in range(4)], axis=0)
+                Q = np.zeros((D, D), complex)
+                for row in range(4):
+                    for col in range(4):
+                        pb = P[(t * 4 + rows[row]) * N:(t * 4 + rows[row] + 1) * N,
+                               (b * 4 + col) * dE:(b * 4 + col + 1) * dE]
+                        Q[row * dE:(row + 1) * dE, col * dE:(col + 1) * dE] = vals[row] * (rv[row * dE:(row + 1) * dE] @ pb)
+                lst.append((Q, (r1, r2), 0))
+            res.append(lst)
+        return res
+
+    def _build_Q2(self, b1, t, b2):
+        """DilutedFactorFactory<Q2>::build (DilutedFactorFactory.cpp:172-253),
+        gamma_5 sign s(row,col) = -1 iff exactly one index is in {2,3} (:215-216)."""
+        dE, D, N = self.dilE, self.D, self.nev
+        res = []
+        for entry in self.lut["ql_lookup"]["Q2"]:
+            vdv = self._vdv(entry, t)
+            rows, vals = GAMMA[entry[2][0]]
+            lst = []
+            for r1, r2 in entry[3]:
+                P1, P2 = self.perams[r1], self.perams[r2]
+                M = np.zeros((D, 4 * N), complex)
+                for row in range(4):
+                    for col in range(4):
+                        pb = P1[(t * 4 + row) * N:(t * 4 + row + 1) * N, (b1 * 4 + col) * dE:(b1 * 4 + col + 1) * dE]
+                        blk = pb.conj().T @ vdv
+                        if (row + col) == 3 or abs(row - col) > 1:
+                            blk = -blk
+                        M[col * dE:(col + 1) * dE, row * N:(row + 1) * N] = blk
+                Q = np.zeros((D, D), complex)
+                for bd in range(4):
+                    g = rows[bd]
+                    for row in range(4):
+                        for col in range(4):
+                            pb = P2[(t * 4 + g) * N:(t * 4 + g + 1) * N, (b2 * 4 + col) * dE:(b2 * 4 + col + 1) * dE]
+                            Q[row * dE:(row + 1) * dE, col * dE:(col + 1) * dE] += vals[bd] * (
+                                M[row * dE:(row + 1) * dE, bd * N:(bd + 1) * N] @ pb)
+                lst.append((Q, (r1, r2), 0))
+            res.append(lst)
+        return res
+
+
+def factor_product(left, right):
+    """operator*(vector<DilutedFactor>, vector<DilutedFactor>) (DilutedFactor.cpp:6-51)."""
+    out = []
+    for lm, lric, lu in left:
+        for rm, rric, ru in right:
+            if not (lric[1] == rric[0] and lric[0] != rric[1]):
+                continue
+            if lu & ru:
+                continue
+            out.append((lm @ rm, (lric[0], rric[1]), lu | ru | (1 << lric[1])))
+    if not out:
+        raise RuntimeError("vector<DilutedFactor> operator* has an empty result.")
+    return out
+
+
+def factor_to_trace2(left, right):
+    """factor_to_trace(L, R) (DilutedTrace.cpp:84-138)."""
+    out = []
+    for lm, lric, lu in left:
+        for rm, rric, ru in right:
+            if not (lric[1] == rric[0] and lric[0] == rric[1]):
+                continue
+            if lu & ru:
+                continue
+            used = lu | ru | (1 << lric[1]) | (1 << lric[0])
+            out.append((complex(np.trace(lm @ rm)), used))
+    if not out:
+        raise RuntimeError("factor_to_trace(L,R) has an empty result. Not enough random vectors?")
+    return out
+
+
+def factor_to_trace1(vec):
+    """factor_to_trace(v) (DilutedTrace.cpp:140-165)."""
+    out = [(complex(np.trace(m)), u | (1 << ric[0])) for m, ric, u in vec if ric[0] == ric[1]]
+    if not out:
+        raise RuntimeError("factor_to_trace(v) has an empty result.")
+    return out
+
+
+def inner_product2(left, right):
+    """inner_product(DT, DT) (DilutedTrace.cpp:5-44)."""
+    n = 0
+    res = 0j
+    for lv, lu in left:
+        rs = 0j
+        for rv, ru in right:
+            if lu & ru:
+                continue
+            rs += rv
+            n += 1
+        res += lv * rs
+    if n == 0:
+        raise RuntimeError("inner_product has an empty result. Not enough random vectors?")
+    return res / n
+
+
+def inner_product3(left, middle, right):
+    """inner_product(DT, DT, DT) (DilutedTrace.cpp:46-82): rejects only masks
+    shared by all three (:63)."""
+    n = 0
+    res = 0j
+    for lv, lu in left:
+        for mv, mu in middle:
+            rs = 0j
+            for rv, ru in right:
+                if lu & mu & ru:
+                    continue
+                rs += rv
+                n += 1
+            res += lv * mv * rs
+    if n == 0:
+        raise RuntimeError("inner_product has an empty result. Not enough random vectors?")
+    return res / n
+
+
+class Traces:
+    """DilutedTrace{1,2,3,4,6}Factory::build (DilutedTraceFactory.cpp:13-238)."""
+
+    def __init__(self, fac, lut, dilT):
+        self.f, self.lut, self.dilT = fac, lut, dilT
+        self.cache = {}
+
+    def clear(self):
+        # DilutedTrace1Factory::clear is a no-op (DilutedTraceFactory.hpp:61)
+        self.cache = {k: v for k, v in self.cache.items() if k[0] == "trQ1"}
+
+    def blk(self, t):
+        return t // self.dilT  # DilutionScheme::time_to_block, block type
+
+    def get(self, name, key, tr_id):
+        ck = (name, tuple(key))
+        if ck not in self.cache:
+            self.cache[ck] = self._build(name, key)
+        return self.cache[ck][tr_id]
+
+    def _build(self, name, key):
+        f, B = self.f, self.blk
+        idx = self.lut["trace_indices"][name]
+        out = []
+        if name == "trQ1":
+            t, = key
+            for c in idx:
+                out.append(factor_to_trace1(f.get("Q1", (t, B(t)))[c[0]]))
+        elif name == "trQ1Q1":
+            t1, t2 = key
+            for c in idx:
+                out.append(factor_to_trace2(f.get("Q1", (t1, B(t2)))[c[0]], f.get("Q1", (t2, B(t1)))[c[1]]))
+        elif name == "trQ0Q2":
+            t1, t2 = key
+            b2 = B(t2)
+            for c in idx:
+                out.append(factor_to_trace2(f.get("Q0", (t2,))[c[1]], f.get("Q2", (b2, t1, b2))[c[0]]))
+        elif name == "trQ1Q1Q1":
+            t1, t2, t3 = key
+            L1 = {}
+            for c in idx:
+                k = (c[0], c[1])
+                if k not in L1:
+                    L1[k] = factor_product(f.get("Q1", (t1, B(t2)))[c[0]], f.get("Q1", (t2, B(t3)))[c[1]])
+            for c in idx:
+                out.append(factor_to_trace2(L1[(c[0], c[1])], f.get("Q1", (t3, B(t1)))[c[2]]))
+        elif name == "trQ1Q0Q2":
+            t1, t2, t3 = key
+            L1 = {}
+            for c in idx:
+                k = (c[2], c[0])
+                if k not in L1:
+                    L1[k] = factor_product(f.get("Q0", (t1,))[c[2]], f.get("Q2", (B(t1), t1, B(t2)))[c[0]])
+            for c in idx:
+                out.append(factor_to_trace2(L1[(c[2], c[0])], f.get("Q1", (t2, B(t3)))[c[1]]))
+        elif name == "trQ1Q1Q1Q1":
+            t0, t1, t2, t3 = key
+            L1, L2 = {}, {}
+            for c in idx:
+                k1, k2 = (c[0], c[1]), (c[2], c[3])
+                if k1 not in L1:
+                    L1[k1] = factor_product(f.get("Q1", (t0, B(t1)))[c[0]], f.get("Q1", (t1, B(t2)))[c[1]])
+                if k2 not in L2:
+                    L2[k2] = factor_product(f.get("Q1", (t2, B(t3)))[c[2]], f.get("Q1", (t3, B(t0)))[c[3]])
+            for c in idx:
+                out.append(factor_to_trace2(L1[(c[0], c[1])], L2[(c[2], c[3])]))
+        elif name == "trQ2Q0Q2Q0":
+            t0, t1, t2, t3 = key
+            b0, b2 = B(t0), B(t2)
+            L1, L2 = {}, {}
+            for c in idx:
+                k1, k2 = (c[0], c[1]), (c[2], c[3])
+                if k1 not in L1:
+                    L1[k1] = factor_product(f.get("Q0", (t0,))[c[0]], f.get("Q2", (b0, t1, b2))[c[1]])
+                if k2 not in L2:
+                    L2[k2] = factor_product(f.get("Q0", (t2,))[c[2]], f.get("Q2", (b2, t3, b0))[c[3]])
+            for c in idx:
+                out.append(factor_to_trace2(L1[(c[0], c[1])], L2[(c[2], c[3])]))
+        elif name == "trQ2Q0Q2Q0Q2Q0":
+            t0, t1, t2, t3, t4, t5 = key
+            b1, b3, b5 = B(t1), B(t3), B(t5)
+            for c in idx:
+                f0 = f.get("Q2", (b5, t0, b1))[c[1]]
+                f1 = f.get("Q0", (t1,))[c[2]]
+                f2 = f.get("Q2", (b1, t2, b3))[c[3]]
+                f3 = f.get("Q0", (t3,))[c[4]]
+                f4 = f.get("Q2", (b3, t4, b5))[c[5]]
+                f5 = f.get("Q0", (t5,))[c[0]]
+                chain = factor_product(factor_product(factor_product(factor_product(f0, f1), f2), f3), f4)
+                out.append(factor_to_trace2(chain, f5))
+        else:
+            raise KeyError(name)
+        return out
+
+
+def resolve_request(trs, traces, t_so, t_si):
+    """resolve_request{1,2,3} (Diagram.cpp:9-94)."""
+    vecs = []
+    for name, tr_id, locs in trs:
+        key = [t_so if l == "source" else t_si for l in locs]
+        vecs.append(traces.get(name, key, tr_id))
+    if len(vecs) == 1:
+        x = vecs[0]
+        s = 0j
+        for v, _ in x:
+            s += v
+        return s / len(x)
+    if len(vecs) == 2:
+        return inner_product2(vecs[0], vecs[1])
+    if len(vecs) == 3:
+        return inner_product3(vecs[0], vecs[1], vecs[2])
+    raise RuntimeError("This many traces are not implemented yet.")
+
+
+def contract(cfg, lut, vdv, perams, rnds, single_block_pair=-1, kahan=False):
+    """contract (Correlators.cpp:40-127) + Diagram::assemble/write
+    (Diagram.cpp:96-147).  Returns {diagram: [(dataset_name, complex[Lt])]}.
+    Block pairs run sequentially; caches are cleared per block pair (:116)."""
+    Lt = cfg["Lt"]
+    q0 = cfg["quarks"][0]
+    dilT, dilE = q0["dilT"], q0["dilE"]
+    fac = Factories(cfg, lut, vdv, perams, rnds, dilT, dilE)
+    tr = Traces(fac, lut, dilT)
+    acc = {name: [[KahanAccumulator(0j) if kahan else 0j for _ in reqs] for _ in range(Lt)]
+           for name, reqs in lut["requests"].items() if reqs}
+    nb = Lt // dilT
+    for b, (bso, bsi) in enumerate(block_pairs(Lt, dilT)):
+        if single_block_pair >= 0 and b != single_block_pair:
+            continue
+        for name, reqs in lut["requests"].items():
+            if not reqs:
+                continue
+            for t_so, t_si in slice_pairs(bso, bsi, Lt, nb):
+                t = time_delta(t_so, t_si, Lt)
+                for i, (trs, _) in enumerate(reqs):
+                    v = resolve_request(trs, tr, t_so, t_si)
+                    if kahan:
+                        acc[name][t][i].add(v)
+                    else:
+                        acc[name][t][i] += v
+        fac.clear()
+        tr.clear()
+    out = {}
+    for name, reqs in lut["requests"].items():
+        if not reqs:
+            continue
+        lst = []
+        for i, (_, dsn) in enumerate(reqs):
+            vals = np.array([(acc[name][t][i].value() if kahan else acc[name][t][i]) / Lt for t in range(Lt)])
+            lst.append((dsn, vals))
+        out[name] = lst
+    return out
+
+
+def run_infile(path, cfgno=1000, kahan=False):
+    """The `contract -i <infile> --start_config n --end_config n` flow
+    (main/contract.cpp:28-107) for one configuration."""
+    with open(path) as fh:
+        cfg = parse_infile(fh.read())
+    lut = init_lookup_tables(cfg)
+    base = os.path.dirname(os.path.abspath(path))
+    perams, rnds, ev = read_inputs(cfg, base, cfgno)
+    vdv = build_vdaggerv(cfg, lut, ev)
+    return contract(cfg, lut, vdv, perams, rnds, kahan=kahan)
