@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Seconds per gauge configuration of the sLapH contraction path on one MI355X.
+
+One step = VdaggerV for every momentum and timeslice + all pipi I=2 diagrams
+(C2c, C4cD, C4cB, C4cC over |P|^2 <= 4 and C6cD at P = 0) over all dilution
+block pairs, on synthetic inputs already resident in HBM:
+L = 32, T = 64, N_ev = 220, 27 momenta |p|^2 <= 3, 5 random vectors,
+dilution TB16 / EI4 / DF4.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pipi32|pipi24|small]
+
+Prints one JSON line.  The roofline object is the VdaggerV kernel (FP64
+matrix cores), timed with HIP events on the stream it runs on; cpu_baseline is
+the numpy oracle timed on a bounded sample on this host and extrapolated.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "slaph-contractions_amd"))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    # headline: 32^3 x 64, N_ev = 220
+    "pipi32": dict(L=32, T=64, N=220, nrnd=5, dilT=16, dilE=4, pmax="0,1,2,3"),
+    # 24^3 x 48, N_ev = 120 (production size of the reference's pipi_I2.in lattice)
+    "pipi24": dict(L=24, T=48, N=120, nrnd=5, dilT=4, dilE=8, pmax="0,1,2,3"),
+    # tiny CPU-checkable case
+    "small": dict(L=4, T=8, N=8, nrnd=4, dilT=2, dilE=4, pmax="0,1"),
+}
+
+PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+PEAK_HBM = 8000.0      # GB/s
+
+
+def infile(w):
+    return """
+Lt = {T}
+Lx = {L}
+Ly = {L}
+Lz = {L}
+number_of_eigen_vec = {N}
+[quarks]
+quark = u:{nrnd}:TB:{dilT}:EI:{dilE}:DF:4:synthetic
+[operator_lists]
+operator_list = g5.d0.p{pmax}
+[correlator_lists]
+correlator_list = C2c:Q0:Op0:Q0:Op0:P0,1,2,3,4
+correlator_list = C4cD:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1,2,3,4
+correlator_list = C4cB:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1,2,3,4
+correlator_list = C4cC:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1,2,3,4
+correlator_list = C6cD:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0
+""".format(**w)
+
+
+def cpu_baseline(w, text, max_seconds=30.0):
+    """Numpy oracle on a bounded sample of the same workload, extrapolated to
+    one configuration: one VdaggerV product, Q0 of one timeslice, Q2 of one
+    key, and one slice pair of every diagram on a subset of its correlators."""
+    import numpy as np
+    from oracle import slaph_oracle as o
+    rng = np.random.default_rng(7)
+    cfg = o.parse_infile(text)
+    lut = o.init_lookup_tables(cfg)
+    L, T, N = w["L"], w["T"], w["N"]
+    K = 3 * L ** 3
+    V = np.linalg.qr(rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N)))[0]
+    ph = o.create_momenta(L, L, L, (1, 0, 0))
+    t0 = time.perf_counter()
+    o.vdaggerv(V, ph)
+    t_vdv = time.perf_counter() - t0
+    n_vdv = sum(1 for vid, _, _ in lut["vdv_lookup"] if vid != lut["index_of_unity"])
+    # factors and one slice pair on random operators (values do not change the cost)
+    nb = T // w["dilT"]
+    cols = nb * w["dilE"] * 4
+    perams = [rng.standard_normal((4 * N * T, cols)) + 1j * rng.standard_normal((4 * N * T, cols))
+              for _ in range(w["nrnd"])]
+    rnds = [(rng.integers(0, 2, T * N * 4) * 2 - 1 + 1j * (rng.integers(0, 2, T * N * 4) * 2 - 1)) / np.sqrt(2)
+            for _ in range(w["nrnd"])]
+
+    class LazyVdv(dict):
+        def __missing__(self, key):
+            m = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+            self[key] = m
+            return m
+    fac = o.Factories(cfg, lut, LazyVdv(), perams, rnds, w["dilT"], w["dilE"])
+    tr = o.Traces(fac, lut, w["dilT"])
+    t_so, t_si = 0, w["dilT"]  # an off-diagonal block pair
+    t0 = time.perf_counter()
+    fac.get("Q0", (t_si,))
+    t_q0 = time.perf_counter() - t0
+    b_so, b_si = t_so // w["dilT"], t_si // w["dilT"]
+    t0 = time.perf_counter()
+    fac.get("Q2", (b_si, t_so, b_si))
+    t_q2 = time.perf_counter() - t0
+    # the remaining keys this slice pair needs are built untimed
+    fac.get("Q0", (t_so,))
+    for key in ((b_so, t_so, b_si), (b_si, t_si, b_so), (b_si, t_si, b_si), (b_so, t_so, b_so)):
+        fac.get("Q2", key)
+    q2keys = set()
+    for a in range(nb):
+        for t in range(T):
+            q2keys.add((a, t, a))
+            q2keys.add((t // w["dilT"], t, a))
+    per_pair = 0.0
+    budget = max_seconds
+    subset = {}
+    for d, reqs in lut["requests"].items():
+        n_sub = 0
+        t0 = time.perf_counter()
+        for trs, _ in reqs:
+            o.resolve_request(trs, tr, t_so, t_si)
+            n_sub += 1
+            if time.perf_counter() - t0 > budget / max(1, len(lut["requests"])):
+                break
+        dt = time.perf_counter() - t0
+        per_pair += dt * len(reqs) / n_sub
+        subset[d] = "%d/%d" % (n_sub, len(reqs))
+    total = t_vdv * n_vdv * T + t_q0 * T + t_q2 * len(q2keys) + per_pair * T * T
+    sample = ("numpy oracle, extrapolated: VdaggerV 1 of %d (p,t) products [%.2f s each]; Q0 of 1 of %d "
+              "timeslices; Q2 of 1 of %d keys; 1 of %d slice pairs with correlator subsets %s" %
+              (n_vdv * T, t_vdv, T, len(q2keys), T * T, json.dumps(subset)))
+    threads = os.environ.get("OMP_NUM_THREADS") or str(os.cpu_count())
+    return {"value": total, "unit": "s", "cores": int(threads), "kind": "port", "sample": sample,
+            "note": "VdaggerV leg uses BLAS with the listed threads; the contraction leg is single-threaded Python"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="pipi32", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--chunk-gib", type=float, default=24.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(see DESIGN.md for how the work would shard)", file=sys.stderr)
+        return 2
+    import slaph
+    w = WORKLOADS[args.workload]
+    text = infile(w)
+    ctx = slaph.Context(0)
+    ctx.configure(text)
+    ctx.set_chunk_budget(int(args.chunk_gib * (1 << 30)))
+    t0 = time.perf_counter()
+    ctx.synthetic_inputs(20261203, True)
+    print("# inputs generated in %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
+    ops = slaph.OperatorFactory(ctx)
+    for i in range(args.warmup):
+        ops.create_operators()
+        slaph.contract(ctx)
+        print("# warmup %d done" % i, file=sys.stderr, flush=True)
+    vdv_ms, stage = [], {"vdaggerv": 0.0, "factors": 0.0, "contract": 0.0}
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ops.create_operators()
+        vdv_ms.append(ctx.stage_time("vdaggerv_kernel"))
+        slaph.contract(ctx)
+        for k in stage:
+            stage[k] += ctx.stage_time(k) / args.steps
+        print("# step %d done" % i, file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    ms = elapsed / args.steps * 1000.0
+    L, T, N = w["L"], w["T"], w["N"]
+    K = 3 * L ** 3
+    n_vdv = ctx.query("n_vdaggerv") - (1 if ctx.query("index_of_unity") >= 0 else 0)
+    flops = 8.0 * N * N * K * n_vdv * T
+    kern_ms = sum(vdv_ms) / len(vdv_ms)
+    achieved = flops / (kern_ms / 1e3) / 1e12
+    line = {
+        "metric": "s per gauge config (VdV+all diagrams) L=32 T=64 Nev=220: mixed vs FP64 @1e-10",
+        "value": ms / 1000.0,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded, device-generated; orthonormalised eigenvectors)",
+        "config": {"workload": "%s: L=%d T=%d Nev=%d, %d rnd vecs, TB%d/EI%d/DF4, g5 p^2<=%s, "
+                               "C2c C4cD C4cB C4cC (P^2<=4) + C6cD (P=0), plain FP64 path"
+                               % (args.workload, L, T, N, w["nrnd"], w["dilT"], w["dilE"], w["pmax"][-1]),
+                   "precision": "fp64"},
+        "stages_ms": {"vdaggerv": stage["vdaggerv"], "factors": stage["factors"],
+                      "contract_incl_factors": stage["contract"]},
+        "roofline": {"bound": "mfma", "kernel": "k_ahb (VdaggerV)", "achieved": achieved,
+                     "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA,
+                     "traffic": None, "algorithmic_flops_per_launch": flops, "avg_launch_ms": kern_ms},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(w, text)
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

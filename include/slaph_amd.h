@@ -1,0 +1,143 @@
+/*
+ * slaph_amd.h -- C ABI of the MI355X contraction path of sLapH-contractions.
+ *
+ * The reference (pittlerf/sLapH-contractions) has no FFI: its boundary is the
+ * C++ class layer driven by main/contract.cpp.  Each entry point below
+ * replaces one step of that flow, cited as file:line of the reference.  Host
+ * callers keep the reference semantics: inputs are read-only, correlators are
+ * sum_{slice pairs} c(t_source, t_sink) / Lt at t = (t_source - t_sink) mod Lt.
+ *
+ * Conventions
+ *  - complex numbers are interleaved (re, im) doubles, matrices column-major;
+ *  - every function returns 0 on success and a negative code on failure;
+ *    slc_last_error() then holds the message (the reference's
+ *    std::runtime_error text where one exists);
+ *  - one context per host thread; a context is bound to one HIP device.
+ */
+#ifndef SLAPH_AMD_H
+#define SLAPH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct slc_ctx slc_ctx;
+
+enum {
+  SLC_OK = 0,
+  SLC_ERR_ARG = -1,     /* invalid argument or state */
+  SLC_ERR_RUNTIME = -2, /* reference runtime_error (e.g. "Not enough random vectors?") */
+  SLC_ERR_DEVICE = -3   /* HIP error */
+};
+
+/* Context with its HIP stream and device buffers.  No reference counterpart
+ * (the reference keeps its state in GlobalData and the factory objects). */
+int slc_ctx_create(int device, slc_ctx **out);
+void slc_ctx_destroy(slc_ctx *ctx);
+const char *slc_last_error(const slc_ctx *ctx);
+
+/* read_parameters + init_lookup_tables from the text of an infile
+ * (reference: src/global_data.cpp:61-390, src/init_lookup_tables.cpp:470-556).
+ * Allocates the device input buffers for the lattice it describes. */
+int slc_configure(slc_ctx *ctx, const char *infile_text);
+
+/* Sizes after slc_configure.  `what` is one of "Lt", "Lx", "Ly", "Lz", "nev",
+ * "dilT", "dilE", "n_rnd", "peram_rows", "peram_cols", "n_vdaggerv",
+ * "index_of_unity", "n_Q0", "n_Q1", "n_Q2". */
+int slc_query(const slc_ctx *ctx, const char *what, long long *out);
+
+/* EigenVector::read_eigen_vector for one timeslice (reference:
+ * src/EigenVector.cpp:35-76): V is (3 Lx Ly Lz) x nev, column-major. */
+int slc_set_eigenvectors(slc_ctx *ctx, int t, const double *V);
+
+/* Perambulator in the reference's in-memory layout after the re-sort of
+ * Perambulator::read_perambulator (src/Perambulator.cpp:48-64): rows
+ * 4 nev t + nev d + v, columns dilE 4 b + dilE d' + e, column-major. */
+int slc_set_perambulator(slc_ctx *ctx, int entity, const double *P);
+
+/* RandomVector entity (include/RandomVector.hpp:38-40): entry 4(nev t + v) + d. */
+int slc_set_random_vector(slc_ctx *ctx, int entity, const double *rho);
+
+/* Read perambulators, random vectors and eigenvectors of configuration
+ * `config` from the reference's file layout, paths relative to base_dir
+ * (main/contract.cpp:83-93, src/global_data_build_IO_names.cpp:7-85). */
+int slc_read_inputs(slc_ctx *ctx, const char *base_dir, int config);
+
+/* Seeded synthetic inputs generated on the device: complex Gaussian
+ * eigenvectors (orthonormalised by Cholesky-QR when orthonormalise != 0),
+ * complex Gaussian perambulators, Z2xZ2 random vectors.  Replaces the
+ * `mock` fills of the reference (EigenVector.cpp:73-75, Perambulator.cpp:65-68). */
+int slc_synthetic_inputs(slc_ctx *ctx, unsigned long long seed, int orthonormalise);
+
+/* Copy one resident input back to the host in the layouts above:
+ * kind 0 = eigenvectors of timeslice `index`, 1 = perambulator `index`,
+ * 2 = random vector `index`.  No reference counterpart (the reference reads
+ * its inputs from files only); used to check device-generated inputs. */
+int slc_get_input(slc_ctx *ctx, int kind, int index, double *out);
+
+/* OperatorFactory::create_operators with handling_vdaggerv = build
+ * (src/OperatorFactory.cpp:220-326, 507-520): VdaggerV(p, t) =
+ * V(t)^+ diag(exp(-i p.x)) V(t) for every lookup entry, on the GPU. */
+int slc_create_operators(slc_ctx *ctx);
+
+/* OperatorFactory::return_vdaggerv (include/OperatorsForMesons.hpp:66-69):
+ * copies the nev x nev matrix of lookup entry id at timeslice t to out. */
+int slc_return_vdaggerv(slc_ctx *ctx, int id, int t, double *out);
+
+/* DilutedFactorFactory<Q>::operator[](key).at({op}) (include/
+ * DilutedFactorFactory.hpp:43-49, src/DilutedFactorFactory.cpp:33-253):
+ * qtype 0/1/2 = Q0 {t} / Q1 {t, b} / Q2 {b1, t, b2}.  Writes the D x D
+ * matrices (D = 4 dilE) of every random-vector pair to out (capacity `cap`
+ * matrices), the pairs to ric (2 ints each) and returns their count in *n. */
+int slc_diluted_factor(slc_ctx *ctx, int qtype, const int *key, int op, double *out, int *ric, int cap, int *n);
+
+/* contract() (src/Correlators.cpp:40-127): every diagram of the infile over
+ * all dilution block pairs (or only block pair `single_time_slice_combination`
+ * when >= 0, global_data.cpp:84-87).  kahan != 0 accumulates the slice pairs
+ * with Neumaier compensation (include/KahanAccumulator.hpp:14-48). */
+int slc_contract(slc_ctx *ctx, int single_time_slice_combination, int kahan);
+
+/* Results of the last slc_contract, per diagram (Diagram::write,
+ * src/Diagram.cpp:132-147): names, counts, and n_corr x Lt complex values in
+ * the order of the reference's correlator requests. */
+int slc_diagram_count(const slc_ctx *ctx, int *n);
+int slc_diagram_name(const slc_ctx *ctx, int i, char *buf, int len);
+int slc_correlator_count(const slc_ctx *ctx, const char *diagram, int *n);
+int slc_correlator_name(const slc_ctx *ctx, const char *diagram, int i, char *buf, int len);
+int slc_fetch_correlators(const slc_ctx *ctx, const char *diagram, double *out);
+
+/* Stage timings of the last create_operators / contract (milliseconds, HIP
+ * events): what = "vdaggerv", "vdaggerv_kernel", "factors", "contract". */
+int slc_stage_time(const slc_ctx *ctx, const char *what, double *ms);
+
+/* Upper bound on the device memory the contraction may use for per-chunk
+ * products and traces (bytes); sets the number of slice pairs per launch. */
+int slc_set_chunk_budget(slc_ctx *ctx, long long bytes);
+
+/* C = A^H diag(phase[k/3]) B on the device (batch of `batch` products with
+ * element strides sA, sB, sC): the kernel behind VdaggerV
+ * (OperatorFactory.cpp:294-300).  A, B, C, phase are device pointers; phase
+ * may be NULL. */
+int slc_ahb_device(slc_ctx *ctx, const double *A, const double *B, double *C, const double *phase, int M, int N,
+                   int K, long long lda, long long ldb, long long ldc, int batch, long long sA, long long sB,
+                   long long sC);
+
+/* Host-only (no device needed): parse an infile and dump the lookup tables
+ * init_lookup_tables builds (src/init_lookup_tables.cpp:470-556) as text --
+ * VdaggerV lookup, Q0/Q1/Q2 DilutedFactorIndex lists, trace index lists and
+ * every correlator request with its dataset name.  Returns the bytes needed
+ * including the NUL (writes at most len), or a negative code. */
+int slc_lookup_dump(const char *infile_text, char *buf, int len);
+
+/* Dilution scheme text of test_dilution_scheme (src/dilution-iterator.cpp:17-36),
+ * block (interlace = 0) or interlace; returns the number of bytes needed
+ * including the terminating NUL (writes at most len). */
+int slc_dilution_text(int num_slice, int num_block, int interlace, char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLAPH_AMD_H */

@@ -386,6 +386,7 @@ def init_lookup_tables(cfg):
     ql_lookup = {"Q0": [], "Q1": [], "Q2": []}
     trace_indices = {n: [] for n in TRACE_NAMES}
     requests = {}
+    tr_index, req_index = {}, {}  # hash indexes giving unique_push_back's first-seen order
     for corr in cfg["correlator_list"]:
         qns = build_quantum_numbers(corr, cfg["operator_list"], cfg["momentum_cutoff"])
         quark_types = [cfg["quarks"][i]["type"] for i in corr["quarks"]]
@@ -427,10 +428,17 @@ def init_lookup_tables(cfg):
                 verts = [q[1] for q in trace]
                 locs = tuple("source" if v in src else "sink" for v in verts)
                 ids = tuple(ql_ids[v] for v in verts)
-                tr_id = _unique_push_back(trace_indices[name], ids)
+                tr_id = tr_index.setdefault(name, {}).get(ids)
+                if tr_id is None:
+                    tr_id = _unique_push_back(trace_indices[name], ids)
+                    tr_index[name][ids] = tr_id
                 trs.append((name, tr_id, locs))
             reqs = requests.setdefault(corr["type"], [])
-            _unique_push_back(reqs, (trs, dsname), key=lambda r: r[0])
+            rkey = tuple(trs)  # equality ignores the dataset name (global_data.hpp:35-37)
+            seen = req_index.setdefault(corr["type"], set())
+            if rkey not in seen:
+                seen.add(rkey)
+                reqs.append((trs, dsname))
     unity = -1
     for vid, vmom, vdisp in vdv_lookup:
         if vmom == (0, 0, 0) and not vdisp:

@@ -1,0 +1,228 @@
+// C ABI of the contraction path (declared in include/slaph_amd.h).  No C++
+// exception crosses this boundary: each entry converts them to error codes
+// and keeps the message for slc_last_error().
+#include "../../include/slaph_amd.h"
+
+#include "engine.hpp"
+
+#include <cstring>
+#include <new>
+#include <string>
+
+struct slc_ctx {
+  std::unique_ptr<slaph::Engine> eng;
+  std::string err;
+};
+
+namespace {
+
+template <typename F>
+int guarded(slc_ctx *ctx, F &&f) {
+  if (!ctx) return SLC_ERR_ARG;
+  try {
+    ctx->err.clear();
+    f();
+    return SLC_OK;
+  } catch (std::out_of_range const &e) {
+    ctx->err = e.what();
+    return SLC_ERR_ARG;
+  } catch (std::runtime_error const &e) {
+    ctx->err = e.what();
+    return std::strncmp(e.what(), "HIP error", 9) == 0 ? SLC_ERR_DEVICE : SLC_ERR_RUNTIME;
+  } catch (std::exception const &e) {
+    ctx->err = e.what();
+    return SLC_ERR_RUNTIME;
+  }
+}
+
+int copy_string(std::string const &s, char *buf, int len) {
+  if (buf && len > 0) {
+    size_t n = std::min<size_t>(s.size(), size_t(len) - 1);
+    std::memcpy(buf, s.data(), n);
+    buf[n] = 0;
+  }
+  return int(s.size()) + 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int slc_ctx_create(int device, slc_ctx **out) {
+  if (!out) return SLC_ERR_ARG;
+  *out = nullptr;
+  slc_ctx *c = new (std::nothrow) slc_ctx;
+  if (!c) return SLC_ERR_RUNTIME;
+  try {
+    c->eng.reset(new slaph::Engine(device));
+  } catch (std::exception const &e) {
+    delete c;
+    return SLC_ERR_DEVICE;
+  }
+  *out = c;
+  return SLC_OK;
+}
+
+void slc_ctx_destroy(slc_ctx *ctx) { delete ctx; }
+
+const char *slc_last_error(const slc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int slc_configure(slc_ctx *ctx, const char *text) {
+  return guarded(ctx, [&] {
+    if (!text) throw std::out_of_range("null infile text");
+    ctx->eng->configure(text);
+  });
+}
+
+int slc_query(const slc_ctx *cctx, const char *what, long long *out) {
+  slc_ctx *ctx = const_cast<slc_ctx *>(cctx);
+  return guarded(ctx, [&] {
+    if (!what || !out) throw std::out_of_range("null argument");
+    auto const &gd = ctx->eng->gd();
+    std::string w = what;
+    if (gd.quarks.empty()) throw std::out_of_range("context is not configured");
+    if (w == "Lt") *out = gd.Lt;
+    else if (w == "Lx") *out = gd.Lx;
+    else if (w == "Ly") *out = gd.Ly;
+    else if (w == "Lz") *out = gd.Lz;
+    else if (w == "nev") *out = gd.nev;
+    else if (w == "dilT") *out = gd.dilT();
+    else if (w == "dilE") *out = gd.dilE();
+    else if (w == "n_rnd") *out = gd.n_rnd_total();
+    else if (w == "peram_rows") *out = gd.peram_rows();
+    else if (w == "peram_cols") *out = gd.peram_cols();
+    else if (w == "n_vdaggerv") *out = (long long)gd.vdaggerv_lookup.size();
+    else if (w == "index_of_unity") *out = gd.index_of_unity;
+    else if (w == "n_Q0") *out = (long long)gd.quarkline_lookuptable.at("Q0").size();
+    else if (w == "n_Q1") *out = (long long)gd.quarkline_lookuptable.at("Q1").size();
+    else if (w == "n_Q2") *out = (long long)gd.quarkline_lookuptable.at("Q2").size();
+    else throw std::out_of_range("unknown query " + w);
+  });
+}
+
+int slc_set_eigenvectors(slc_ctx *ctx, int t, const double *V) {
+  return guarded(ctx, [&] { ctx->eng->set_eigenvectors(t, reinterpret_cast<const slaph::cplx *>(V)); });
+}
+
+int slc_set_perambulator(slc_ctx *ctx, int r, const double *P) {
+  return guarded(ctx, [&] { ctx->eng->set_perambulator(r, reinterpret_cast<const slaph::cplx *>(P)); });
+}
+
+int slc_set_random_vector(slc_ctx *ctx, int r, const double *rho) {
+  return guarded(ctx, [&] { ctx->eng->set_random_vector(r, reinterpret_cast<const slaph::cplx *>(rho)); });
+}
+
+int slc_read_inputs(slc_ctx *ctx, const char *base, int config) {
+  return guarded(ctx, [&] { ctx->eng->read_inputs(base ? base : ".", config); });
+}
+
+int slc_synthetic_inputs(slc_ctx *ctx, unsigned long long seed, int orth) {
+  return guarded(ctx, [&] { ctx->eng->synthetic_inputs(seed, orth != 0); });
+}
+
+int slc_get_input(slc_ctx *ctx, int kind, int index, double *out) {
+  return guarded(ctx, [&] {
+    if (!out) throw std::out_of_range("null output");
+    ctx->eng->get_input(kind, index, reinterpret_cast<slaph::cplx *>(out));
+  });
+}
+
+int slc_create_operators(slc_ctx *ctx) {
+  return guarded(ctx, [&] { ctx->eng->create_operators(); });
+}
+
+int slc_return_vdaggerv(slc_ctx *ctx, int id, int t, double *out) {
+  return guarded(ctx, [&] { ctx->eng->return_vdaggerv(id, t, reinterpret_cast<slaph::cplx *>(out)); });
+}
+
+int slc_diluted_factor(slc_ctx *ctx, int qtype, const int *key, int op, double *out, int *ric, int cap, int *n) {
+  return guarded(ctx, [&] {
+    if (!key || !out || !ric || !n) throw std::out_of_range("null argument");
+    *n = ctx->eng->diluted_factor(qtype, key, op, reinterpret_cast<slaph::cplx *>(out), ric, cap);
+  });
+}
+
+int slc_contract(slc_ctx *ctx, int single, int kahan) {
+  return guarded(ctx, [&] { ctx->eng->contract(single, kahan); });
+}
+
+int slc_diagram_count(const slc_ctx *cctx, int *n) {
+  slc_ctx *ctx = const_cast<slc_ctx *>(cctx);
+  return guarded(ctx, [&] { *n = int(ctx->eng->diagrams().size()); });
+}
+
+int slc_diagram_name(const slc_ctx *cctx, int i, char *buf, int len) {
+  slc_ctx *ctx = const_cast<slc_ctx *>(cctx);
+  int need = 0;
+  int rc = guarded(ctx, [&] { need = copy_string(ctx->eng->diagrams().at(i), buf, len); });
+  return rc == SLC_OK ? need : rc;
+}
+
+int slc_correlator_count(const slc_ctx *cctx, const char *d, int *n) {
+  slc_ctx *ctx = const_cast<slc_ctx *>(cctx);
+  return guarded(ctx, [&] { *n = ctx->eng->correlator_count(d); });
+}
+
+int slc_correlator_name(const slc_ctx *cctx, const char *d, int i, char *buf, int len) {
+  slc_ctx *ctx = const_cast<slc_ctx *>(cctx);
+  int need = 0;
+  int rc = guarded(ctx, [&] { need = copy_string(ctx->eng->correlator_name(d, i), buf, len); });
+  return rc == SLC_OK ? need : rc;
+}
+
+int slc_fetch_correlators(const slc_ctx *cctx, const char *d, double *out) {
+  slc_ctx *ctx = const_cast<slc_ctx *>(cctx);
+  return guarded(ctx, [&] { ctx->eng->fetch_correlators(d, reinterpret_cast<slaph::cplx *>(out)); });
+}
+
+int slc_stage_time(const slc_ctx *cctx, const char *what, double *ms) {
+  slc_ctx *ctx = const_cast<slc_ctx *>(cctx);
+  return guarded(ctx, [&] {
+    std::string w = what ? what : "";
+    auto const &t = ctx->eng->times();
+    if (w == "vdaggerv") *ms = t.vdaggerv_ms;
+    else if (w == "vdaggerv_kernel") *ms = t.vdaggerv_kernel_ms;
+    else if (w == "factors") *ms = t.factors_ms;
+    else if (w == "contract") *ms = t.contract_ms;
+    else throw std::out_of_range("unknown stage " + w);
+  });
+}
+
+int slc_set_chunk_budget(slc_ctx *ctx, long long bytes) {
+  return guarded(ctx, [&] {
+    if (bytes <= 0) throw std::out_of_range("budget must be positive");
+    ctx->eng->set_chunk_budget(size_t(bytes));
+  });
+}
+
+int slc_ahb_device(slc_ctx *ctx, const double *A, const double *B, double *C, const double *phase, int M, int N,
+                   int K, long long lda, long long ldb, long long ldc, int batch, long long sA, long long sB,
+                   long long sC) {
+  return guarded(ctx, [&] {
+    if (M <= 0 || N <= 0 || K <= 0 || batch <= 0 || lda < K || ldb < K || ldc < M)
+      throw std::out_of_range("invalid matrix shape");
+    ctx->eng->ahb_raw(reinterpret_cast<const double2 *>(A), reinterpret_cast<const double2 *>(B),
+                      reinterpret_cast<double2 *>(C), reinterpret_cast<const double2 *>(phase), M, N, K, lda, ldb, ldc,
+                      batch, sA, sB, sC);
+  });
+}
+
+int slc_lookup_dump(const char *text, char *buf, int len) {
+  if (!text) return SLC_ERR_ARG;
+  try {
+    slaph::GlobalData gd;
+    slaph::parse_infile(gd, text);
+    slaph::init_lookup_tables(gd);
+    return copy_string(slaph::lookup_dump(gd), buf, len);
+  } catch (std::exception const &e) {
+    copy_string(std::string("error: ") + e.what(), buf, len);
+    return SLC_ERR_RUNTIME;
+  }
+}
+
+int slc_dilution_text(int num_slice, int num_block, int interlace, char *buf, int len) {
+  if (num_slice <= 0 || num_block <= 0 || num_slice % num_block) return SLC_ERR_ARG;
+  return copy_string(slaph::dilution_iterator_text(num_slice, num_block, interlace != 0), buf, len);
+}
+
+}  // extern "C"

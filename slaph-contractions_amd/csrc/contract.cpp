@@ -1,0 +1,657 @@
+// contract(): the correlator assembly of the reference (Correlators.cpp:40-127,
+// Diagram.cpp:9-147, DilutedTraceFactory.cpp:13-238) as a host planner plus
+// batched device launches.
+//
+// Planning splits every index into a time part and a quantum-number part:
+//  * the quantum-number part (which operator entries, which random-vector
+//    combinations, which combinations survive the used-index masks) does not
+//    depend on time and is enumerated once per contract() call;
+//  * the time part (which factor slab a leaf reads, which product instance a
+//    slice pair shares) is a handful of integers per slice pair.
+// The device kernels compose the two through small index tables, so host work
+// per chunk of slice pairs is O(slice pairs + trace ids).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <set>
+#include <unordered_map>
+
+namespace slaph {
+
+namespace {
+
+// ----- shapes of the trace factories ----------------------------------------
+// A leaf reads factor type q (0,1,2) for operator entry c_look[cidx] at a time
+// key built from the trace times: Q0 (t[a]); Q1 (t[a], B(t[b]));
+// Q2 (B(t[a]), t[b], B(t[c])).
+struct LeafSpec {
+  int q, cidx, a, b, c;
+};
+enum NodeKind { kLeaf = 0, kProd = 1, kTrace2 = 2, kTrace1 = 3 };
+struct NodeSpec {
+  NodeKind kind;
+  int leaf;         // for kLeaf
+  int left, right;  // node indices
+};
+struct Shape {
+  int ntimes;
+  std::vector<LeafSpec> leaves;
+  std::vector<NodeSpec> nodes;  // topological; last is the trace
+};
+
+Shape make_shape(std::string const &name) {
+  Shape s;
+  auto leaf = [&s](int q, int cidx, int a, int b = 0, int c = 0) {
+    s.leaves.push_back({q, cidx, a, b, c});
+    s.nodes.push_back({kLeaf, int(s.leaves.size()) - 1, -1, -1});
+    return int(s.nodes.size()) - 1;
+  };
+  auto prod = [&s](int l, int r) {
+    s.nodes.push_back({kProd, -1, l, r});
+    return int(s.nodes.size()) - 1;
+  };
+  auto tr2 = [&s](int l, int r) { s.nodes.push_back({kTrace2, -1, l, r}); };
+  if (name == "trQ1") {  // DilutedTraceFactory.cpp:13-25
+    s.ntimes = 1;
+    int x = leaf(1, 0, 0, 0);
+    s.nodes.push_back({kTrace1, -1, x, -1});
+  } else if (name == "trQ1Q1") {  // :29-43
+    s.ntimes = 2;
+    int x = leaf(1, 0, 0, 1), y = leaf(1, 1, 1, 0);
+    tr2(x, y);
+  } else if (name == "trQ0Q2") {  // :47-61
+    s.ntimes = 2;
+    int x = leaf(0, 1, 1), y = leaf(2, 0, 1, 0, 1);
+    tr2(x, y);
+  } else if (name == "trQ1Q1Q1") {  // :65-89
+    s.ntimes = 3;
+    int a = leaf(1, 0, 0, 1), b = leaf(1, 1, 1, 2), p = prod(a, b), c = leaf(1, 2, 2, 0);
+    tr2(p, c);
+  } else if (name == "trQ1Q0Q2") {  // :95-119
+    s.ntimes = 3;
+    int a = leaf(0, 2, 0), b = leaf(2, 0, 0, 0, 1), p = prod(a, b), c = leaf(1, 1, 1, 2);
+    tr2(p, c);
+  } else if (name == "trQ1Q1Q1Q1") {  // :125-155
+    s.ntimes = 4;
+    int a = leaf(1, 0, 0, 1), b = leaf(1, 1, 1, 2), p = prod(a, b);
+    int c = leaf(1, 2, 2, 3), d = leaf(1, 3, 3, 0), q = prod(c, d);
+    tr2(p, q);
+  } else if (name == "trQ2Q0Q2Q0") {  // :161-190
+    s.ntimes = 4;
+    int a = leaf(0, 0, 0), b = leaf(2, 1, 0, 1, 2), p = prod(a, b);
+    int c = leaf(0, 2, 2), d = leaf(2, 3, 2, 3, 0), q = prod(c, d);
+    tr2(p, q);
+  } else if (name == "trQ2Q0Q2Q0Q2Q0") {  // :197-238
+    s.ntimes = 6;
+    int f0 = leaf(2, 1, 5, 0, 1), f1 = leaf(0, 2, 1), p = prod(f0, f1);
+    int f2 = leaf(2, 3, 1, 2, 3);
+    p = prod(p, f2);
+    int f3 = leaf(0, 4, 3);
+    p = prod(p, f3);
+    int f4 = leaf(2, 5, 3, 4, 5);
+    p = prod(p, f4);
+    int f5 = leaf(0, 0, 5);
+    tr2(p, f5);
+  } else {
+    throw std::runtime_error("unknown trace " + name);
+  }
+  return s;
+}
+
+// ----- random-vector templates ---------------------------------------------
+struct FactorTmpl {
+  std::vector<std::pair<int, int>> ric;
+  std::vector<uint16_t> used;
+};
+struct PairTmpl {
+  std::vector<std::array<int, 2>> combos;
+  std::vector<uint16_t> masks;  // traces only
+  int out = -1;                 // products only: output factor template
+};
+
+class Templates {
+ public:
+  int leaf(std::vector<std::pair<int, int>> const &rnd) {
+    auto it = leaf_ids_.find(rnd);
+    if (it != leaf_ids_.end()) return it->second;
+    FactorTmpl t;
+    for (auto const &p : rnd) {
+      if (p.first > 15 || p.second > 15) throw std::runtime_error("at most 16 random vectors are supported");
+      t.ric.push_back(p);
+      t.used.push_back(0);
+    }
+    f_.push_back(t);
+    leaf_ids_[rnd] = int(f_.size()) - 1;
+    return int(f_.size()) - 1;
+  }
+  FactorTmpl const &factor(int id) const { return f_.at(id); }
+
+  // operator*(vector<DilutedFactor>, vector<DilutedFactor>) (DilutedFactor.cpp:6-51)
+  PairTmpl const &product(int a, int b) {
+    auto key = std::make_pair(a, b);
+    auto it = prod_.find(key);
+    if (it != prod_.end()) return it->second;
+    PairTmpl p;
+    FactorTmpl out;
+    auto const &L = f_.at(a), &R = f_.at(b);
+    for (size_t i = 0; i < L.ric.size(); ++i)
+      for (size_t j = 0; j < R.ric.size(); ++j) {
+        if (!(L.ric[i].second == R.ric[j].first && L.ric[i].first != R.ric[j].second)) continue;
+        if (L.used[i] & R.used[j]) continue;
+        p.combos.push_back({int(i), int(j)});
+        out.ric.push_back({L.ric[i].first, R.ric[j].second});
+        out.used.push_back(uint16_t(L.used[i] | R.used[j] | (1u << L.ric[i].second)));
+      }
+    if (p.combos.empty())
+      throw std::runtime_error(
+          "vector<DilutedFactor> operator*(vector<DilutedFactor>, vector<DilutedFactor>) has an empty result.");
+    f_.push_back(out);
+    p.out = int(f_.size()) - 1;
+    return prod_.emplace(key, p).first->second;
+  }
+
+  // factor_to_trace(L, R) (DilutedTrace.cpp:84-138)
+  PairTmpl const &trace2(int a, int b) {
+    auto key = std::make_pair(a, b);
+    auto it = tr2_.find(key);
+    if (it != tr2_.end()) return it->second;
+    PairTmpl p;
+    auto const &L = f_.at(a), &R = f_.at(b);
+    for (size_t i = 0; i < L.ric.size(); ++i)
+      for (size_t j = 0; j < R.ric.size(); ++j) {
+        if (!(L.ric[i].second == R.ric[j].first && L.ric[i].first == R.ric[j].second)) continue;
+        if (L.used[i] & R.used[j]) continue;
+        p.combos.push_back({int(i), int(j)});
+        p.masks.push_back(
+            uint16_t(L.used[i] | R.used[j] | (1u << L.ric[i].second) | (1u << L.ric[i].first)));
+      }
+    if (p.combos.empty())
+      throw std::runtime_error(
+          "vector<DilutedTrace> factor_to_trace(vector<DilutedFactor>, vector<DilutedFactor>) has an empty result. "
+          "Not enough random vectors?");
+    return tr2_.emplace(key, p).first->second;
+  }
+
+  // factor_to_trace(v) (DilutedTrace.cpp:140-165)
+  PairTmpl const &trace1(int a) {
+    auto it = tr1_.find(a);
+    if (it != tr1_.end()) return it->second;
+    PairTmpl p;
+    auto const &L = f_.at(a);
+    for (size_t i = 0; i < L.ric.size(); ++i) {
+      if (L.ric[i].first != L.ric[i].second) continue;
+      p.combos.push_back({int(i), 0});
+      p.masks.push_back(uint16_t(L.used[i] | (1u << L.ric[i].first)));
+    }
+    if (p.combos.empty())
+      throw std::runtime_error("vector<DilutedTrace> factor_to_trace(vector<DilutedFactor>) has an empty result.");
+    return tr1_.emplace(a, p).first->second;
+  }
+
+ private:
+  std::vector<FactorTmpl> f_;
+  std::map<std::vector<std::pair<int, int>>, int> leaf_ids_;
+  std::map<std::pair<int, int>, PairTmpl> prod_, tr2_;
+  std::map<int, PairTmpl> tr1_;
+};
+
+// ----- plans -------------------------------------------------------------------
+struct NodePlan {
+  // quantum-number part (per contract)
+  std::map<std::vector<int>, int> qn_index;
+  std::vector<std::vector<int>> qn_key;  // leaf entries of the subtree
+  std::vector<int> qn_left, qn_right;    // child qn (leaf child: operator entry)
+  std::vector<int> qn_tmpl;              // output factor template (leaf/product)
+  std::vector<long long> qn_off;         // output offset of each qn
+  long long stride = 0;
+  std::vector<ComboProd> combos;
+  long long cmaxX = 0, cmaxY = 0, cmaxO = 0;  // largest qn offset + combo index per operand
+  std::vector<int> trl_qn;  // trl -> qn
+  DevBuf d_combos, d_Xq, d_Yq;
+  // time part (per chunk)
+  std::map<std::vector<long long>, int> time_index;
+  std::vector<int> time_left, time_right;  // child time index (product children)
+  std::vector<long long> time_leafbase_l, time_leafbase_r;
+  std::vector<int> sp_time;  // slice pair -> time index
+  long long base = 0;        // offset in the chunk's product/trace buffer
+  DevBuf d_Xt, d_Yt;
+  std::vector<long long> h_Xt, h_Yt;
+};
+
+struct Family {
+  std::string name;
+  std::vector<Loc> locs;
+  Shape shape;
+  std::vector<int> tr_ids;  // trl -> tr_id
+  std::map<int, int> trl_of;
+  std::vector<NodePlan> nodes;
+  std::vector<int> mask_off;  // trl -> offset into the global mask array
+  std::vector<int> ncomb;     // trl -> number of trace values
+};
+
+struct CorrPlan {
+  std::string name;
+  std::vector<std::vector<std::pair<int, int>>> reqs;  // correlator -> [(family, trl)]
+  std::vector<std::string> datasets;
+};
+
+// Every (factor type, time key) a leaf of this family reads at slice pair
+// (t_so, t_si); the slab index is returned through `base`.
+std::vector<int> trace_times(Family const &f, int tso, int tsi) {
+  std::vector<int> tt(f.locs.size());
+  for (size_t i = 0; i < f.locs.size(); ++i) tt[i] = f.locs[i] == Loc::source ? tso : tsi;
+  return tt;
+}
+
+}  // namespace
+
+void Engine::contract(int single_block_pair, int kahan) {
+  require_inputs();
+  if (!have_ops_) throw std::runtime_error("operators have not been created");
+  SLAPH_HIP(hipSetDevice(device_));
+  auto t_start = std::chrono::steady_clock::now();
+  const int dilT = gd_.dilT();
+  auto B = [dilT](int t) { return t / dilT; };  // DilutionScheme::time_to_block, block type
+
+  // ---- families and correlators (time independent) ----
+  std::vector<Family> fams;
+  std::map<std::pair<std::string, std::vector<Loc>>, int> fam_of;
+  std::vector<CorrPlan> corrs;
+  for (auto const &kv : gd_.correlator_requests_map) {
+    if (kv.second.empty()) continue;
+    CorrPlan cp;
+    cp.name = kv.first;
+    for (auto const &req : kv.second) {
+      if (req.traces.empty() || req.traces.size() > 3) throw std::runtime_error("This many traces are not implemented yet.");
+      std::vector<std::pair<int, int>> refs;
+      for (auto const &tr : req.traces) {
+        auto key = std::make_pair(tr.name, tr.locs);
+        auto it = fam_of.find(key);
+        if (it == fam_of.end()) {
+          Family f;
+          f.name = tr.name;
+          f.locs = tr.locs;
+          f.shape = make_shape(tr.name);
+          if (int(f.locs.size()) != f.shape.ntimes) throw std::logic_error("trace time count mismatch");
+          fams.push_back(std::move(f));
+          it = fam_of.emplace(key, int(fams.size()) - 1).first;
+        }
+        Family &f = fams[it->second];
+        auto jt = f.trl_of.find(tr.tr_id);
+        if (jt == f.trl_of.end()) {
+          f.tr_ids.push_back(tr.tr_id);
+          jt = f.trl_of.emplace(tr.tr_id, int(f.tr_ids.size()) - 1).first;
+        }
+        refs.push_back({it->second, jt->second});
+      }
+      cp.reqs.push_back(refs);
+      cp.datasets.push_back(req.dataset);
+    }
+    corrs.push_back(std::move(cp));
+  }
+
+  // ---- quantum-number planning per family ----
+  compute_q_offsets();
+  static const char *qname[3] = {"Q0", "Q1", "Q2"};
+  Templates tm;
+  std::vector<uint16_t> masks;
+  for (auto &f : fams) {
+    auto const &cl = gd_.trace_indices_map.at(f.name);
+    f.nodes.resize(f.shape.nodes.size());
+    for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+      auto const &ns = f.shape.nodes[n];
+      NodePlan &np = f.nodes[n];
+      bool root = n + 1 == f.shape.nodes.size();
+      np.trl_qn.resize(f.tr_ids.size());
+      for (size_t trl = 0; trl < f.tr_ids.size(); ++trl) {
+        auto const &c = cl.at(f.tr_ids[trl]);
+        if (ns.kind == kLeaf) {
+          auto const &ls = f.shape.leaves[ns.leaf];
+          np.trl_qn[trl] = c.at(ls.cidx);  // leaf qn = operator entry
+          continue;
+        }
+        std::vector<int> key;
+        auto append = [&](int child) {
+          auto const &cs = f.shape.nodes[child];
+          if (cs.kind == kLeaf) key.push_back(c.at(f.shape.leaves[cs.leaf].cidx));
+          else {
+            auto const &k2 = f.nodes[child].qn_key[f.nodes[child].trl_qn[trl]];
+            key.insert(key.end(), k2.begin(), k2.end());
+          }
+        };
+        append(ns.left);
+        if (ns.right >= 0) append(ns.right);
+        if (root) key.push_back(int(trl));  // every trl is its own trace output
+        auto it = np.qn_index.find(key);
+        if (it == np.qn_index.end()) {
+          it = np.qn_index.emplace(key, int(np.qn_key.size())).first;
+          np.qn_key.push_back(key);
+          np.qn_left.push_back(f.nodes[ns.left].trl_qn[trl]);
+          np.qn_right.push_back(ns.right >= 0 ? f.nodes[ns.right].trl_qn[trl] : 0);
+        }
+        np.trl_qn[trl] = it->second;
+      }
+      if (ns.kind == kLeaf) continue;
+      // templates and combos
+      auto tmpl_of = [&](int child, int cqn) {
+        auto const &cs = f.shape.nodes[child];
+        if (cs.kind == kLeaf) {
+          auto const &ls = f.shape.leaves[cs.leaf];
+          return tm.leaf(gd_.quarkline_lookuptable.at(qname[ls.q]).at(cqn).rnd);
+        }
+        return f.nodes[child].qn_tmpl[cqn];
+      };
+      np.qn_off.assign(np.qn_key.size() + 1, 0);
+      np.qn_tmpl.assign(np.qn_key.size(), -1);
+      if (root) {
+        f.mask_off.assign(f.tr_ids.size(), 0);
+        f.ncomb.assign(f.tr_ids.size(), 0);
+      }
+      for (size_t q = 0; q < np.qn_key.size(); ++q) {
+        int lt = tmpl_of(ns.left, np.qn_left[q]);
+        PairTmpl const *pt;
+        if (ns.kind == kProd) {
+          pt = &tm.product(lt, tmpl_of(ns.right, np.qn_right[q]));
+          np.qn_tmpl[q] = pt->out;
+        } else if (ns.kind == kTrace2) {
+          pt = &tm.trace2(lt, tmpl_of(ns.right, np.qn_right[q]));
+        } else {
+          pt = &tm.trace1(lt);
+        }
+        for (size_t k = 0; k < pt->combos.size(); ++k)
+          np.combos.push_back({int(q), pt->combos[k][0], pt->combos[k][1], np.qn_off[q] + (long long)k});
+        np.qn_off[q + 1] = np.qn_off[q] + (long long)pt->combos.size();
+        if (root) {
+          int trl = np.qn_key[q].back();
+          f.mask_off[trl] = int(masks.size());
+          f.ncomb[trl] = int(pt->combos.size());
+          masks.insert(masks.end(), pt->masks.begin(), pt->masks.end());
+        }
+      }
+      np.stride = np.qn_off.back();
+      // operand qn offsets: leaf child -> first pair of the operator entry;
+      // product child -> its qn offset
+      std::vector<long long> Xq(np.qn_key.size()), Yq(np.qn_key.size(), 0);
+      auto off_of = [&](int child, int cqn) -> long long {
+        auto const &cs = f.shape.nodes[child];
+        if (cs.kind == kLeaf) return q_off_[f.shape.leaves[cs.leaf].q].at(cqn);
+        return f.nodes[child].qn_off[cqn];
+      };
+      for (size_t q = 0; q < np.qn_key.size(); ++q) {
+        Xq[q] = off_of(ns.left, np.qn_left[q]);
+        if (ns.right >= 0) Yq[q] = off_of(ns.right, np.qn_right[q]);
+      }
+      for (auto const &cb : np.combos) {
+        np.cmaxX = std::max(np.cmaxX, Xq[cb.qn] + cb.i);
+        np.cmaxY = std::max(np.cmaxY, Yq[cb.qn] + cb.j);
+        np.cmaxO = std::max(np.cmaxO, cb.o);
+      }
+      upload(np.d_Xq, Xq);
+      upload(np.d_Yq, Yq);
+      upload(np.d_combos, np.combos);
+    }
+  }
+
+  // ---- chunks of slice pairs and the factor keys they need ----
+  auto bps = block_pairs(Lt_, dilT);
+  std::vector<std::vector<SlicePair>> bp_slices;
+  for (size_t b = 0; b < bps.size(); ++b) {
+    if (single_block_pair >= 0 && int(b) != single_block_pair) continue;
+    bp_slices.push_back(slice_pairs(bps[b].first, bps[b].second, Lt_, dilT));
+  }
+  std::set<std::vector<int>> need1, need2;
+  for (auto const &sl : bp_slices)
+    for (auto const &sp : sl)
+      for (auto const &f : fams) {
+        auto tt = trace_times(f, sp.source, sp.sink);
+        for (auto const &ls : f.shape.leaves) {
+          if (ls.q == 1) need1.insert({tt[ls.a], B(tt[ls.b])});
+          if (ls.q == 2) need2.insert({B(tt[ls.a]), tt[ls.b], B(tt[ls.c])});
+        }
+      }
+  {
+    auto t0 = std::chrono::steady_clock::now();
+    build_factors(std::vector<std::vector<int>>(need1.begin(), need1.end()),
+                  std::vector<std::vector<int>>(need2.begin(), need2.end()));
+    times_.factors_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  DevBuf d_masks;
+  upload(d_masks, masks);
+
+  // Slices per chunk from the memory budget (worst case: every product node
+  // has a distinct time key per slice pair).
+  const long long DD = (long long)D_ * D_;
+  double per_sp = 0;
+  for (auto const &f : fams)
+    for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+      auto const &ns = f.shape.nodes[n];
+      if (ns.kind == kProd) per_sp += double(f.nodes[n].stride) * DD * sizeof(double2);
+      if (n + 1 == f.shape.nodes.size()) per_sp += double(f.nodes[n].stride) * sizeof(double2);
+    }
+  for (auto const &c : corrs) per_sp += double(c.reqs.size()) * sizeof(double2);
+  int smax = int(std::max(1.0, std::min(65535.0, double(chunk_budget_) / std::max(per_sp, 1.0))));
+
+  // ---- accumulators ----
+  std::vector<DevBuf> d_sum(corrs.size()), d_comp(corrs.size());
+  for (size_t c = 0; c < corrs.size(); ++c) {
+    size_t n = (size_t)Lt_ * corrs[c].reqs.size();
+    d_sum[c].alloc(sizeof(double2) * n);
+    d_comp[c].alloc(sizeof(double2) * n);
+    SLAPH_HIP(hipMemsetAsync(d_sum[c].as<void>(), 0, sizeof(double2) * n, stream_));
+    SLAPH_HIP(hipMemsetAsync(d_comp[c].as<void>(), 0, sizeof(double2) * n, stream_));
+  }
+  DevBuf d_prod, d_tr, d_cval, d_td;
+  std::vector<DevBuf> d_jobs_c(corrs.size());
+  size_t max_corr = 1;
+  for (auto const &c : corrs) max_corr = std::max(max_corr, c.reqs.size());
+  d_cval.alloc(sizeof(double2) * max_corr * (size_t)std::min<size_t>(smax, (size_t)Lt_ * Lt_));
+
+  auto slab_of = [&](int q) -> const double2 * {
+    return q == 0 ? q0_.as<double2>() : (q == 1 ? q1_.as<double2>() : q2_.as<double2>());
+  };
+  auto leaf_base = [&](LeafSpec const &ls, std::vector<int> const &tt) -> long long {
+    if (ls.q == 0) return (long long)tt[ls.a] * q_total_[0];
+    if (ls.q == 1) return ((long long)tt[ls.a] * nb_ + B(tt[ls.b])) * q_total_[1];
+    int slot = q2slot_.at(((size_t)B(tt[ls.a]) * Lt_ + tt[ls.b]) * nb_ + B(tt[ls.c]));
+    if (slot < 0) throw std::logic_error("Q2 key was not built");
+    return (long long)slot * q_total_[2];
+  };
+
+  EventTimer ctimer(stream_);
+  const bool progress = std::getenv("SLAPH_PROGRESS") != nullptr;
+  int bp_done = 0;
+  for (auto const &sl : bp_slices) {
+    if (progress) {
+      // like the reference's per-block-pair message (Correlators.cpp:92-97)
+      std::fprintf(stderr, "[slaph] block pair %d of %zu (%zu slice pairs, %d per launch)\n", bp_done++,
+                   bp_slices.size(), sl.size(), smax);
+      std::fflush(stderr);
+    }
+    for (size_t c0 = 0; c0 < sl.size(); c0 += smax) {
+      const int S = int(std::min<size_t>(smax, sl.size() - c0));
+      // time planning
+      long long prod_total = 0, tr_total = 0;
+      for (auto &f : fams) {
+        std::vector<std::vector<int>> tts(S);
+        for (int s = 0; s < S; ++s) tts[s] = trace_times(f, sl[c0 + s].source, sl[c0 + s].sink);
+        // subtree leaf bases per node and slice pair
+        std::vector<std::vector<std::vector<long long>>> keys(f.shape.nodes.size(),
+                                                              std::vector<std::vector<long long>>(S));
+        for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+          auto const &ns = f.shape.nodes[n];
+          NodePlan &np = f.nodes[n];
+          bool root = n + 1 == f.shape.nodes.size();
+          np.time_index.clear();
+          np.time_left.clear();
+          np.time_right.clear();
+          np.time_leafbase_l.clear();
+          np.time_leafbase_r.clear();
+          np.sp_time.assign(S, 0);
+          for (int s = 0; s < S; ++s) {
+            if (ns.kind == kLeaf) {
+              keys[n][s] = {leaf_base(f.shape.leaves[ns.leaf], tts[s])};
+              continue;
+            }
+            std::vector<long long> key = keys[ns.left][s];
+            if (ns.right >= 0) key.insert(key.end(), keys[ns.right][s].begin(), keys[ns.right][s].end());
+            keys[n][s] = key;
+            int ti;
+            if (root) {
+              ti = s;
+            } else {
+              auto it = np.time_index.find(key);
+              if (it != np.time_index.end()) {
+                np.sp_time[s] = it->second;
+                continue;
+              }
+              ti = int(np.time_index.size());
+              np.time_index.emplace(key, ti);
+            }
+            np.sp_time[s] = ti;
+            auto child_t = [&](int child, std::vector<int> &ct, std::vector<long long> &lb) {
+              if (child < 0) {
+                ct.push_back(0);
+                lb.push_back(0);
+                return;
+              }
+              if (f.shape.nodes[child].kind == kLeaf) {
+                ct.push_back(0);
+                lb.push_back(keys[child][s][0]);
+              } else {
+                ct.push_back(f.nodes[child].sp_time[s]);
+                lb.push_back(0);
+              }
+            };
+            child_t(ns.left, np.time_left, np.time_leafbase_l);
+            child_t(ns.right, np.time_right, np.time_leafbase_r);
+          }
+          if (ns.kind == kLeaf) continue;
+          int ntime = root ? S : int(np.time_index.size());
+          if (root) {
+            np.base = tr_total;
+            tr_total += (long long)S * np.stride;
+          } else {
+            np.base = prod_total;
+            prod_total += (long long)ntime * np.stride;
+          }
+          // operand time tables: leaf -> slab base; product -> base + ti * stride
+          std::vector<long long> Xt(ntime), Yt(ntime, 0);
+          for (int ti = 0; ti < ntime; ++ti) {
+            auto const &l = f.shape.nodes[ns.left];
+            Xt[ti] = l.kind == kLeaf ? np.time_leafbase_l[ti]
+                                     : f.nodes[ns.left].base + np.time_left[ti] * f.nodes[ns.left].stride;
+            if (ns.right >= 0) {
+              auto const &r = f.shape.nodes[ns.right];
+              Yt[ti] = r.kind == kLeaf ? np.time_leafbase_r[ti]
+                                       : f.nodes[ns.right].base + np.time_right[ti] * f.nodes[ns.right].stride;
+            }
+          }
+          upload(np.d_Xt, Xt);
+          upload(np.d_Yt, Yt);
+          np.h_Xt = std::move(Xt);
+          np.h_Yt = std::move(Yt);
+        }
+      }
+      d_prod.ensure(sizeof(double2) * std::max<long long>(1, prod_total * DD));
+      d_tr.ensure(sizeof(double2) * std::max<long long>(1, tr_total));
+      // launches: products in topological order, then traces
+      for (auto &f : fams)
+        for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+          auto const &ns = f.shape.nodes[n];
+          NodePlan &np = f.nodes[n];
+          if (ns.kind == kLeaf) continue;
+          bool root = n + 1 == f.shape.nodes.size();
+          auto src = [&](int child) -> const double2 * {
+            if (child < 0) return nullptr;
+            auto const &cs = f.shape.nodes[child];
+            if (cs.kind == kLeaf) return slab_of(f.shape.leaves[cs.leaf].q);
+            return d_prod.as<double2>();
+          };
+          int ntime = root ? S : int(np.time_index.size());
+          // Conservative bounds check of every operand and output index the
+          // launch can form (largest time base + largest qn/combo offset).
+          auto slab_count = [&](int child) -> long long {
+            auto const &cs = f.shape.nodes[child];
+            if (cs.kind != kLeaf) return prod_total;
+            int q = f.shape.leaves[cs.leaf].q;
+            DevBuf const &b = q == 0 ? q0_ : (q == 1 ? q1_ : q2_);
+            return (long long)(b.bytes() / (sizeof(double2) * DD));
+          };
+          auto max_of = [](DevBuf const &, std::vector<long long> const &v) {
+            long long m = 0;
+            for (auto x : v) m = std::max(m, x);
+            return m;
+          };
+          if (max_of(np.d_Xt, np.h_Xt) + np.cmaxX >= slab_count(ns.left) ||
+              (ns.right >= 0 && max_of(np.d_Yt, np.h_Yt) + np.cmaxY >= slab_count(ns.right)) ||
+              np.base + (long long)(ntime - 1) * np.stride + np.cmaxO >= (root ? tr_total : prod_total))
+            throw std::logic_error("contraction plan indexes outside its buffers (" + f.name + ")");
+          if (!root) {
+            launch_prod(src(ns.left), np.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
+                        np.d_Yt.as<long long>(), np.d_Yq.as<long long>(), d_prod.as<double2>(), np.base, np.stride,
+                        np.d_combos.as<ComboProd>(), int(np.combos.size()), ntime, D_, stream_);
+          } else {
+            launch_trace(src(ns.left), np.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
+                         np.d_Yt.as<long long>(), np.d_Yq.as<long long>(), d_tr.as<double2>(), np.base, np.stride,
+                         np.d_combos.as<ComboProd>(), int(np.combos.size()), ntime, D_, stream_);
+          }
+        }
+      // resolve and accumulate per diagram
+      std::vector<int> td(S);
+      for (int s = 0; s < S; ++s) td[s] = time_delta(sl[c0 + s].source, sl[c0 + s].sink, Lt_);
+      upload(d_td, td);
+      for (size_t c = 0; c < corrs.size(); ++c) {
+        auto const &cp = corrs[c];
+        std::vector<CorrJob> jobs(cp.reqs.size());
+        for (size_t i = 0; i < cp.reqs.size(); ++i) {
+          jobs[i].ntr = int(cp.reqs[i].size());
+          for (size_t k = 0; k < cp.reqs[i].size(); ++k) {
+            auto const &f = fams[cp.reqs[i][k].first];
+            int trl = cp.reqs[i][k].second;
+            NodePlan const &root = f.nodes.back();
+            jobs[i].tr[k].base = root.base + root.qn_off[root.trl_qn[trl]];
+            jobs[i].tr[k].stride = root.stride;
+            jobs[i].tr[k].n = f.ncomb[trl];
+            jobs[i].tr[k].mask_off = f.mask_off[trl];
+            if (jobs[i].tr[k].base + (long long)(S - 1) * root.stride + f.ncomb[trl] > tr_total ||
+                f.mask_off[trl] + f.ncomb[trl] > (long long)masks.size())
+              throw std::logic_error("correlator request indexes outside the trace buffer");
+          }
+        }
+        upload(d_jobs_c[c], jobs);
+        launch_resolve(d_tr.as<double2>(), d_masks.as<uint16_t>(), d_jobs_c[c].as<CorrJob>(), int(cp.reqs.size()), S,
+                       d_cval.as<double2>(), stream_);
+        launch_accumulate(d_cval.as<double2>(), d_td.as<int>(), S, int(cp.reqs.size()), d_sum[c].as<double2>(),
+                          d_comp[c].as<double2>(), kahan, stream_);
+      }
+      // host tables are re-filled for the next chunk: wait for this one
+      SLAPH_HIP(hipStreamSynchronize(stream_));
+    }
+  }
+  double kms = ctimer.stop();
+  (void)kms;
+
+  // ---- results: correlator_[t][i].value() / Lt (Diagram.cpp:132-147) ----
+  results_.clear();
+  for (size_t c = 0; c < corrs.size(); ++c) {
+    size_t n = (size_t)Lt_ * corrs[c].reqs.size();
+    std::vector<cplx> sum(n), comp(n);
+    SLAPH_HIP(hipMemcpy(sum.data(), d_sum[c].as<void>(), sizeof(cplx) * n, hipMemcpyDeviceToHost));
+    SLAPH_HIP(hipMemcpy(comp.data(), d_comp[c].as<void>(), sizeof(cplx) * n, hipMemcpyDeviceToHost));
+    DiagramResult r;
+    r.name = corrs[c].name;
+    r.names = corrs[c].datasets;
+    size_t nc = corrs[c].reqs.size();
+    r.values.resize(n);
+    for (size_t i = 0; i < nc; ++i)
+      for (int t = 0; t < Lt_; ++t) r.values[i * Lt_ + t] = (sum[t * nc + i] + comp[t * nc + i]) / double(Lt_);
+    results_.push_back(std::move(r));
+  }
+  times_.contract_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+}
+
+}  // namespace slaph

@@ -1,0 +1,555 @@
+// Device-resident contraction engine; see engine.hpp.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <set>
+#include <unordered_map>
+
+namespace slaph {
+
+// ---------------------------------------------------------------------------
+// DevBuf
+// ---------------------------------------------------------------------------
+void DevBuf::reset() {
+  if (p_) (void)hipFree(p_);
+  p_ = nullptr;
+  bytes_ = 0;
+}
+
+void DevBuf::alloc(size_t bytes) {
+  reset();
+  if (bytes == 0) return;
+  SLAPH_HIP(hipMalloc(&p_, bytes));
+  bytes_ = bytes;
+}
+
+void DevBuf::ensure(size_t bytes) {
+  if (bytes <= bytes_) return;
+  alloc(bytes);
+}
+
+EventTimer::EventTimer(hipStream_t s) : s_(s) {
+  SLAPH_HIP(hipEventCreate(&a_));
+  SLAPH_HIP(hipEventCreate(&b_));
+  SLAPH_HIP(hipEventRecord(a_, s_));
+}
+
+EventTimer::~EventTimer() {
+  (void)hipEventDestroy(a_);
+  (void)hipEventDestroy(b_);
+}
+
+double EventTimer::stop() {
+  SLAPH_HIP(hipEventRecord(b_, s_));
+  SLAPH_HIP(hipEventSynchronize(b_));
+  float ms = 0;
+  SLAPH_HIP(hipEventElapsedTime(&ms, a_, b_));
+  return ms;
+}
+
+// ---------------------------------------------------------------------------
+// Engine: configuration and inputs
+// ---------------------------------------------------------------------------
+Engine::Engine(int device) : device_(device) {
+  int n = 0;
+  SLAPH_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) throw std::runtime_error("invalid HIP device " + std::to_string(device));
+  SLAPH_HIP(hipSetDevice(device));
+  SLAPH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+}
+
+Engine::~Engine() {
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::configure(std::string const &text) {
+  GlobalData gd;
+  parse_infile(gd, text);
+  init_lookup_tables(gd);
+  for (auto const &q : gd.quarks)
+    if (q.dilT != gd.quarks[0].dilT || q.dilE != gd.quarks[0].dilE || q.dilD != 4)
+      throw std::runtime_error("all quarks must share the dilution scheme of quark 0 with full Dirac dilution");
+  if (gd.Lt % gd.quarks[0].dilT != 0) throw std::runtime_error("Lt must be a multiple of the time dilution block");
+  if (gd.need_gaugefield) throw std::runtime_error("displaced operators are not implemented on this path");
+  gd_ = std::move(gd);
+  configured_ = true;
+  K_ = 3LL * gd_.Lx * gd_.Ly * gd_.Lz;
+  N_ = gd_.nev;
+  D_ = 4 * gd_.dilE();
+  Lt_ = gd_.Lt;
+  nb_ = gd_.n_blocks();
+  nrnd_ = gd_.n_rnd_total();
+  if (D_ > 32) throw std::runtime_error("4*dilE above 32 is not supported");
+  have_ev_ = have_peram_ = have_rnd_ = have_ops_ = false;
+  results_.clear();
+  SLAPH_HIP(hipSetDevice(device_));
+  alloc_inputs();
+}
+
+void Engine::require_configured() const {
+  if (!configured_) throw std::runtime_error("context is not configured");
+}
+
+void Engine::require_inputs() const {
+  require_configured();
+  if (!have_ev_ || !have_peram_ || !have_rnd_) throw std::runtime_error("inputs are not set");
+}
+
+void Engine::alloc_inputs() {
+  V_.alloc(sizeof(double2) * K_ * N_ * Lt_);
+  rho_.alloc(sizeof(double2) * (size_t)nrnd_ * Lt_ * N_ * 4);
+  peram_.alloc(sizeof(double2) * (size_t)nrnd_ * gd_.peram_rows() * gd_.peram_cols());
+}
+
+void Engine::set_eigenvectors(int t, const cplx *V) {
+  require_configured();
+  if (t < 0 || t >= Lt_) throw std::runtime_error("timeslice out of range");
+  SLAPH_HIP(hipSetDevice(device_));
+  SLAPH_HIP(hipMemcpy(V_.as<double2>() + (size_t)t * K_ * N_, V, sizeof(double2) * K_ * N_, hipMemcpyHostToDevice));
+  have_ev_ = true;
+}
+
+void Engine::set_perambulator(int r, const cplx *P) {
+  require_configured();
+  if (r < 0 || r >= nrnd_) throw std::runtime_error("perambulator index out of range");
+  size_t n = gd_.peram_rows() * gd_.peram_cols();
+  SLAPH_HIP(hipSetDevice(device_));
+  SLAPH_HIP(hipMemcpy(peram_.as<double2>() + r * n, P, sizeof(double2) * n, hipMemcpyHostToDevice));
+  have_peram_ = true;
+}
+
+void Engine::set_random_vector(int r, const cplx *rho) {
+  require_configured();
+  if (r < 0 || r >= nrnd_) throw std::runtime_error("random vector index out of range");
+  size_t n = (size_t)Lt_ * N_ * 4;
+  SLAPH_HIP(hipSetDevice(device_));
+  SLAPH_HIP(hipMemcpy(rho_.as<double2>() + r * n, rho, sizeof(double2) * n, hipMemcpyHostToDevice));
+  have_rnd_ = true;
+}
+
+void Engine::read_inputs(std::string const &base, int config) {
+  // Perambulator::read_perambulators_from_separate_files (Perambulator.cpp:82-97),
+  // RandomVector::read_random_vectors_from_separate_files (RandomVector.cpp:104-113),
+  // EigenVector::read_eigen_vector (EigenVector.cpp:35-76).
+  require_configured();
+  int Vs = gd_.Lx * gd_.Ly * gd_.Lz;
+  long long prow = gd_.peram_rows(), pcol = gd_.peram_cols();
+  std::vector<cplx> file(prow * pcol), sorted(prow * pcol), rnd((size_t)Lt_ * N_ * 4);
+  int r = 0;
+  for (auto const &q0 : gd_.quarks) {
+    Quark q = q0;
+    if (!q.path.empty() && q.path[0] != '/') q.path = base + "/" + q.path;
+    for (int i = 0; i < q.nrnd; ++i, ++r) {
+      read_binary(perambulator_filename(q, config, i, Lt_, Vs), file.data(), sizeof(cplx) * file.size());
+      resort_perambulator(file.data(), sorted.data(), Lt_, N_, q.dilT, q.dilE, q.dilD);
+      set_perambulator(r, sorted.data());
+      read_binary(randomvector_filename(q, config, i, N_), rnd.data(), sizeof(cplx) * rnd.size());
+      set_random_vector(r, rnd.data());
+    }
+  }
+  GlobalData g2 = gd_;
+  if (!g2.path_eigenvectors.empty() && g2.path_eigenvectors[0] != '/') g2.path_eigenvectors = base + "/" + g2.path_eigenvectors;
+  std::string prefix = eigenvector_prefix(g2, config);
+  std::vector<cplx> V(K_ * N_);
+  for (int t = 0; t < Lt_; ++t) {
+    char suffix[16];
+    std::snprintf(suffix, sizeof suffix, "%03d", t);
+    // file: N vectors of K complex numbers each = column-major K x N
+    read_binary(prefix + suffix, V.data(), sizeof(cplx) * V.size());
+    set_eigenvectors(t, V.data());
+  }
+}
+
+namespace {
+
+// Cholesky G = L L^H of a Hermitian positive-definite N x N (column-major);
+// returns R^{-1} with R = L^H upper triangular, so that V R^{-1} has
+// orthonormal columns when G = V^H V.
+std::vector<cplx> chol_inverse_upper(std::vector<cplx> const &G, int N) {
+  std::vector<cplx> L(N * N, 0.0);
+  for (int j = 0; j < N; ++j) {
+    double d = G[j * N + j].real();
+    for (int k = 0; k < j; ++k) d -= std::norm(L[k * N + j]);
+    if (!(d > 0)) throw std::runtime_error("orthonormalisation: Gram matrix is not positive definite");
+    double ljj = std::sqrt(d);
+    L[j * N + j] = ljj;
+    for (int i = j + 1; i < N; ++i) {
+      cplx s = G[j * N + i];
+      for (int k = 0; k < j; ++k) s -= L[k * N + i] * std::conj(L[k * N + j]);
+      L[j * N + i] = s / ljj;
+    }
+  }
+  // R = L^H (upper). Invert R column by column: R X = I.
+  std::vector<cplx> X(N * N, 0.0);
+  for (int c = 0; c < N; ++c) {
+    for (int i = c; i >= 0; --i) {
+      cplx s = (i == c) ? 1.0 : 0.0;
+      for (int k = i + 1; k <= c; ++k) s -= std::conj(L[i * N + k]) * X[c * N + k];  // R[i,k] = conj(L[k,i])
+      X[c * N + i] = s / std::conj(L[i * N + i]);
+    }
+  }
+  return X;
+}
+
+unsigned long long seed_of(unsigned long long seed, unsigned long long kind, unsigned long long idx) {
+  unsigned long long z = seed * 0x9E3779B97F4A7C15ULL + kind * 0xD1B54A32D192ED03ULL + idx * 0x8CB92BA72F3D8DD7ULL;
+  z = (z ^ (z >> 33)) * 0xff51afd7ed558ccdULL;
+  return z ^ (z >> 29);
+}
+
+}  // namespace
+
+void Engine::synthetic_inputs(unsigned long long seed, bool orthonormalise) {
+  // Seeded complex Gaussian eigenvectors (optionally orthonormalised by two
+  // passes of Cholesky-QR), complex Gaussian perambulators and Z2xZ2 random
+  // vectors with entries (+-1 +-i)/sqrt(2); everything generated on the device.
+  require_configured();
+  SLAPH_HIP(hipSetDevice(device_));
+  size_t np = gd_.peram_rows() * gd_.peram_cols();
+  for (int r = 0; r < nrnd_; ++r) {
+    launch_fill_gauss(peram_.as<double2>() + r * np, np, seed_of(seed, 1, r), stream_);
+    launch_fill_z2z2(rho_.as<double2>() + (size_t)r * Lt_ * N_ * 4, (long long)Lt_ * N_ * 4, seed_of(seed, 2, r),
+                     stream_);
+  }
+  DevBuf tmp, G, R, jobs;
+  if (orthonormalise) {
+    tmp.alloc(sizeof(double2) * K_ * N_);
+    G.alloc(sizeof(double2) * N_ * N_);
+    R.alloc(sizeof(double2) * N_ * N_);
+  }
+  std::vector<cplx> hG(N_ * N_);
+  for (int t = 0; t < Lt_; ++t) {
+    double2 *Vt = V_.as<double2>() + (size_t)t * K_ * N_;
+    launch_fill_gauss(Vt, K_ * N_, seed_of(seed, 3, t), stream_);
+    if (!orthonormalise) continue;
+    for (int pass = 0; pass < 2; ++pass) {
+      ahb_raw(Vt, Vt, G.as<double2>(), nullptr, N_, N_, (int)K_, K_, K_, N_, 1, 0, 0, 0);
+      SLAPH_HIP(hipMemcpyAsync(hG.data(), G.as<void>(), sizeof(cplx) * hG.size(), hipMemcpyDeviceToHost, stream_));
+      SLAPH_HIP(hipStreamSynchronize(stream_));
+      auto Rinv = chol_inverse_upper(hG, N_);
+      SLAPH_HIP(hipMemcpyAsync(R.as<void>(), Rinv.data(), sizeof(cplx) * Rinv.size(), hipMemcpyHostToDevice, stream_));
+      launch_gemm_nn_small(Vt, R.as<double2>(), tmp.as<double2>(), K_, N_, stream_);
+      SLAPH_HIP(hipMemcpyAsync(Vt, tmp.as<void>(), sizeof(double2) * K_ * N_, hipMemcpyDeviceToDevice, stream_));
+    }
+  }
+  SLAPH_HIP(hipStreamSynchronize(stream_));
+  have_ev_ = have_peram_ = have_rnd_ = true;
+}
+
+void Engine::get_input(int kind, int index, cplx *out) const {
+  require_inputs();
+  const double2 *src = nullptr;
+  size_t n = 0;
+  if (kind == 0 && index >= 0 && index < Lt_) {
+    n = (size_t)K_ * N_;
+    src = V_.as<double2>() + index * n;
+  } else if (kind == 1 && index >= 0 && index < nrnd_) {
+    n = gd_.peram_rows() * gd_.peram_cols();
+    src = peram_.as<double2>() + index * n;
+  } else if (kind == 2 && index >= 0 && index < nrnd_) {
+    n = (size_t)Lt_ * N_ * 4;
+    src = rho_.as<double2>() + index * n;
+  } else {
+    throw std::out_of_range("get_input: bad kind or index");
+  }
+  SLAPH_HIP(hipSetDevice(device_));
+  SLAPH_HIP(hipMemcpy(out, src, sizeof(double2) * n, hipMemcpyDeviceToHost));
+}
+
+void Engine::ahb_raw(const double2 *A, const double2 *B, double2 *C, const double2 *phase, int M, int N, int K,
+                     long long lda, long long ldb, long long ldc, int batch, long long sA, long long sB, long long sC) {
+  std::vector<AhBJob> jobs(batch);
+  for (int i = 0; i < batch; ++i) jobs[i] = {A + i * sA, B + i * sB, C + i * sC, phase, nullptr, lda, ldb, ldc};
+  DevBuf dj;
+  upload(dj, jobs);
+  launch_ahb(dj.as<AhBJob>(), batch, M, N, K, stream_);
+  SLAPH_HIP(hipStreamSynchronize(stream_));
+}
+
+// ---------------------------------------------------------------------------
+// OperatorFactory::create_operators / build_vdaggerv (OperatorFactory.cpp:220-326)
+// ---------------------------------------------------------------------------
+void Engine::create_operators() {
+  require_inputs();
+  if (!gd_.handling_vdaggerv.empty() && gd_.handling_vdaggerv != "build")
+    throw std::runtime_error("handling_vdaggerv '" + gd_.handling_vdaggerv + "' is not supported (only build)");
+  SLAPH_HIP(hipSetDevice(device_));
+  int nvdv = int(gd_.vdaggerv_lookup.size());
+  int Vs = gd_.Lx * gd_.Ly * gd_.Lz;
+  vdv_.alloc(sizeof(double2) * std::max(1, nvdv) * Lt_ * N_ * N_);
+  phase_.alloc(sizeof(double2) * std::max(1, nvdv) * Vs);
+  // Momentum phases exp(-i 2 pi p.x / L), site = (x Ly + y) Lz + z
+  // (create_momenta, OperatorFactory.cpp:64-95).
+  std::vector<cplx> ph((size_t)std::max(1, nvdv) * Vs);
+  for (auto const &op : gd_.vdaggerv_lookup) {
+    double ipx = op.mom[0] * 2.0 * M_PI / gd_.Lx, ipy = op.mom[1] * 2.0 * M_PI / gd_.Ly,
+           ipz = op.mom[2] * 2.0 * M_PI / gd_.Lz;
+    for (int x = 0; x < gd_.Lx; ++x)
+      for (int y = 0; y < gd_.Ly; ++y)
+        for (int z = 0; z < gd_.Lz; ++z)
+          ph[(size_t)op.id * Vs + (x * gd_.Ly + y) * gd_.Lz + z] =
+              std::exp(cplx(0, -1) * (ipx * x + ipy * y + ipz * z));
+  }
+  SLAPH_HIP(hipMemcpyAsync(phase_.as<void>(), ph.data(), sizeof(cplx) * ph.size(), hipMemcpyHostToDevice, stream_));
+  std::vector<AhBJob> jobs;
+  for (auto const &op : gd_.vdaggerv_lookup) {
+    double2 *out = vdv_.as<double2>() + (size_t)op.id * Lt_ * N_ * N_;
+    if (op.id == gd_.index_of_unity) {
+      launch_identity(out, N_, Lt_, stream_);
+      continue;
+    }
+    for (int t = 0; t < Lt_; ++t) {
+      const double2 *Vt = V_.as<double2>() + (size_t)t * K_ * N_;
+      jobs.push_back({Vt, Vt, out + (size_t)t * N_ * N_, phase_.as<double2>() + (size_t)op.id * Vs, nullptr, K_, K_, N_});
+    }
+  }
+  DevBuf dj;
+  upload(dj, jobs);
+  EventTimer tm(stream_);
+  for (size_t i = 0; i < jobs.size(); i += 65535)
+    launch_ahb(dj.as<AhBJob>() + i, int(std::min<size_t>(65535, jobs.size() - i)), N_, N_, (int)K_, stream_);
+  times_.vdaggerv_kernel_ms = tm.stop();
+  times_.vdaggerv_launches = int((jobs.size() + 65534) / 65535);
+  times_.vdaggerv_ms = times_.vdaggerv_kernel_ms;
+  // Daggered copies of the operators that some factor uses with -p
+  // (build_VdaggerV_lookup's dagger flag, init_lookup_tables.cpp:257-259).
+  dag_slot_.assign(std::max(1, nvdv), -1);
+  int ndag = 0;
+  for (auto const &kv : gd_.quarkline_lookuptable)
+    for (auto const &e : kv.second)
+      if (e.dagger && dag_slot_[e.id_vdaggerv] < 0) dag_slot_[e.id_vdaggerv] = ndag++;
+  vdvdag_.alloc(sizeof(double2) * std::max(1, ndag) * Lt_ * N_ * N_);
+  for (int id = 0; id < nvdv; ++id)
+    if (dag_slot_[id] >= 0)
+      launch_dagger(vdv_.as<double2>() + (size_t)id * Lt_ * N_ * N_,
+                    vdvdag_.as<double2>() + (size_t)dag_slot_[id] * Lt_ * N_ * N_, N_, Lt_, stream_);
+  SLAPH_HIP(hipStreamSynchronize(stream_));
+  have_ops_ = true;
+}
+
+const double2 *Engine::vdv_ptr(int id, bool dagger, int t) const {
+  if (dagger) return vdvdag_.as<double2>() + ((size_t)dag_slot_.at(id) * Lt_ + t) * N_ * N_;
+  return vdv_.as<double2>() + ((size_t)id * Lt_ + t) * N_ * N_;
+}
+
+void Engine::return_vdaggerv(int id, int t, cplx *out) {
+  if (!have_ops_) throw std::runtime_error("operators have not been created");
+  if (id < 0 || id >= int(gd_.vdaggerv_lookup.size()) || t < 0 || t >= Lt_)
+    throw std::runtime_error("return_vdaggerv: index out of range");
+  SLAPH_HIP(hipSetDevice(device_));
+  SLAPH_HIP(hipMemcpy(out, vdv_ptr(id, false, t), sizeof(cplx) * N_ * N_, hipMemcpyDeviceToHost));
+}
+
+// ---------------------------------------------------------------------------
+// Diluted factors on the device (DilutedFactorFactory.cpp:33-253)
+// ---------------------------------------------------------------------------
+void Engine::compute_q_offsets() {
+  static const char *qn[3] = {"Q0", "Q1", "Q2"};
+  for (int q = 0; q < 3; ++q) {
+    auto const &lst = gd_.quarkline_lookuptable.at(qn[q]);
+    q_off_[q].assign(lst.size() + 1, 0);
+    for (size_t e = 0; e < lst.size(); ++e) q_off_[q][e + 1] = q_off_[q][e] + (long long)lst[e].rnd.size();
+    q_total_[q] = q_off_[q].back();
+  }
+}
+
+void Engine::build_factors(std::vector<std::vector<int>> const &q1keys, std::vector<std::vector<int>> const &q2keys) {
+  compute_q_offsets();
+  const long long DD = (long long)D_ * D_;
+  const long long prow = gd_.peram_rows();
+  const size_t rsz = (size_t)Lt_ * N_ * 4, psz = gd_.peram_rows() * gd_.peram_cols();
+  const double2 *rho = rho_.as<double2>();
+  const double2 *peram = peram_.as<double2>();
+  DevBuf dj;
+  // Q0 for every timeslice.
+  {
+    auto const &lst = gd_.quarkline_lookuptable.at("Q0");
+    q0_.alloc(sizeof(double2) * std::max<long long>(1, Lt_ * q_total_[0] * DD));
+    std::vector<Q0Job> jobs;
+    for (int t = 0; t < Lt_; ++t)
+      for (size_t e = 0; e < lst.size(); ++e)
+        for (size_t j = 0; j < lst[e].rnd.size(); ++j) {
+          auto const &p = lst[e].rnd[j];
+          jobs.push_back({vdv_ptr(lst[e].id_vdaggerv, lst[e].dagger, t), rho + p.first * rsz + (size_t)t * N_ * 4,
+                          rho + p.second * rsz + (size_t)t * N_ * 4,
+                          q0_.as<double2>() + ((long long)t * q_total_[0] + q_off_[0][e] + (long long)j) * DD,
+                          lst[e].gamma.at(0)});
+        }
+    upload(dj, jobs);
+    launch_q0(dj.as<Q0Job>(), int(jobs.size()), N_, gd_.dilE(), stream_);
+    SLAPH_HIP(hipStreamSynchronize(stream_));
+  }
+  // Q1 for the needed (t, b).
+  {
+    auto const &lst = gd_.quarkline_lookuptable.at("Q1");
+    q1need_.assign((size_t)Lt_ * nb_, 0);
+    for (auto const &k : q1keys) q1need_[k[0] * nb_ + k[1]] = 1;
+    q1_.alloc(sizeof(double2) * std::max<long long>(1, (long long)Lt_ * nb_ * q_total_[1] * DD));
+    std::vector<Q1Job> jobs;
+    for (int t = 0; t < Lt_; ++t)
+      for (int b = 0; b < nb_; ++b) {
+        if (!q1need_[t * nb_ + b]) continue;
+        for (size_t e = 0; e < lst.size(); ++e)
+          for (size_t j = 0; j < lst[e].rnd.size(); ++j) {
+            auto const &p = lst[e].rnd[j];
+            jobs.push_back({vdv_ptr(lst[e].id_vdaggerv, lst[e].dagger, t), rho + p.first * rsz + (size_t)t * N_ * 4,
+                            peram + p.second * psz,
+                            q1_.as<double2>() + (((long long)t * nb_ + b) * q_total_[1] + q_off_[1][e] + (long long)j) * DD,
+                            t, b, lst[e].gamma.at(0)});
+          }
+      }
+    upload(dj, jobs);
+    launch_q1(dj.as<Q1Job>(), int(jobs.size()), N_, gd_.dilE(), prow, stream_);
+    SLAPH_HIP(hipStreamSynchronize(stream_));
+  }
+  // Q2 for the needed (b1, t, b2): M' = gval(bd) s(bd,c) P1_bd^H VdV per
+  // (b1, t, operator, r1), then Q2 = M' * stacked P2 blocks.
+  {
+    auto const &lst = gd_.quarkline_lookuptable.at("Q2");
+    q2slot_.assign((size_t)nb_ * Lt_ * nb_, -1);
+    int nslot = 0;
+    for (auto const &k : q2keys) {
+      int &s = q2slot_[((size_t)k[0] * Lt_ + k[1]) * nb_ + k[2]];
+      if (s < 0) s = nslot++;
+    }
+    q2_.alloc(sizeof(double2) * std::max<long long>(1, nslot * q_total_[2] * DD));
+    // rowscale[g][bd][row] = gval_g(bd) * s(bd, row / dilE); s = -1 iff
+    // exactly one of bd, c lies in {2,3} (DilutedFactorFactory.cpp:215-216).
+    auto const &gt = gamma_table();
+    std::vector<cplx> rs(16 * 4 * D_);
+    for (int g = 0; g < 16; ++g)
+      for (int bd = 0; bd < 4; ++bd)
+        for (int row = 0; row < D_; ++row) {
+          int c = row / gd_.dilE();
+          double sgn = ((bd + c) == 3 || std::abs(bd - c) > 1) ? -1.0 : 1.0;
+          rs[(g * 4 + bd) * D_ + row] = gt[g].value[bd] * sgn;
+        }
+    DevBuf drs, Mbuf, djb;
+    upload(drs, rs);
+    for (int t = 0; t < Lt_; ++t) {
+      // M' jobs for this timeslice, keyed by (b1, entry, r1).
+      std::map<std::array<int, 3>, int> mjob;
+      std::vector<AhBJob> ajobs;
+      std::vector<Q2BJob> bjobs;
+      std::vector<int> bjob_m;
+      for (int b1 = 0; b1 < nb_; ++b1)
+        for (int b2 = 0; b2 < nb_; ++b2) {
+          int slot = q2slot_[((size_t)b1 * Lt_ + t) * nb_ + b2];
+          if (slot < 0) continue;
+          for (size_t e = 0; e < lst.size(); ++e)
+            for (size_t j = 0; j < lst[e].rnd.size(); ++j) {
+              auto const &p = lst[e].rnd[j];
+              std::array<int, 3> key{b1, int(e), p.first};
+              auto it = mjob.find(key);
+              if (it == mjob.end()) it = mjob.emplace(key, int(mjob.size())).first;
+              bjobs.push_back({nullptr, peram + p.second * psz,
+                               q2_.as<double2>() + ((long long)slot * q_total_[2] + q_off_[2][e] + (long long)j) * DD, t,
+                               b2, lst[e].gamma.at(0)});
+              bjob_m.push_back(it->second);
+            }
+        }
+      if (mjob.empty()) continue;
+      const long long msz = (long long)D_ * 4 * N_;
+      Mbuf.ensure(sizeof(double2) * msz * mjob.size());
+      for (auto const &kv : mjob) {
+        int b1 = kv.first[0], e = kv.first[1], r1 = kv.first[2];
+        for (int bd = 0; bd < 4; ++bd) {
+          AhBJob a;
+          a.A = peram + r1 * psz + (long long)(b1 * D_) * prow + (long long)(t * 4 + bd) * N_;
+          a.lda = prow;
+          a.B = vdv_ptr(lst[e].id_vdaggerv, lst[e].dagger, t);
+          a.ldb = N_;
+          a.C = Mbuf.as<double2>() + kv.second * msz + (long long)bd * N_ * D_;
+          a.ldc = D_;
+          a.phase = nullptr;
+          a.rowscale = drs.as<double2>() + (lst[e].gamma.at(0) * 4 + bd) * D_;
+          ajobs.push_back(a);
+        }
+      }
+      for (size_t i = 0; i < bjobs.size(); ++i) bjobs[i].Mp = Mbuf.as<double2>() + (long long)bjob_m[i] * msz;
+      upload(dj, ajobs);
+      for (size_t i = 0; i < ajobs.size(); i += 65535)
+        launch_ahb(dj.as<AhBJob>() + i, int(std::min<size_t>(65535, ajobs.size() - i)), D_, N_, N_, stream_);
+      upload(djb, bjobs);
+      launch_q2b(djb.as<Q2BJob>(), int(bjobs.size()), N_, gd_.dilE(), prow, stream_);
+      SLAPH_HIP(hipStreamSynchronize(stream_));
+    }
+  }
+}
+
+int Engine::diluted_factor(int qtype, const int *key, int op, cplx *out, int *ric, int cap) {
+  require_inputs();
+  if (!have_ops_) throw std::runtime_error("operators have not been created");
+  static const char *qn[3] = {"Q0", "Q1", "Q2"};
+  if (qtype < 0 || qtype > 2) throw std::runtime_error("qtype must be 0, 1 or 2");
+  auto const &lst = gd_.quarkline_lookuptable.at(qn[qtype]);
+  if (op < 0 || op >= int(lst.size())) throw std::runtime_error("operator index out of range");
+  std::vector<std::vector<int>> k1, k2;
+  long long base = 0;
+  if (qtype == 1) {
+    if (key[0] < 0 || key[0] >= Lt_ || key[1] < 0 || key[1] >= nb_) throw std::runtime_error("bad Q1 key");
+    k1.push_back({key[0], key[1]});
+  } else if (qtype == 2) {
+    if (key[0] < 0 || key[0] >= nb_ || key[1] < 0 || key[1] >= Lt_ || key[2] < 0 || key[2] >= nb_)
+      throw std::runtime_error("bad Q2 key");
+    k2.push_back({key[0], key[1], key[2]});
+  } else if (key[0] < 0 || key[0] >= Lt_) {
+    throw std::runtime_error("bad Q0 key");
+  }
+  build_factors(k1, k2);
+  const long long DD = (long long)D_ * D_;
+  const double2 *src;
+  if (qtype == 0) {
+    base = (long long)key[0] * q_total_[0];
+    src = q0_.as<double2>();
+  } else if (qtype == 1) {
+    base = ((long long)key[0] * nb_ + key[1]) * q_total_[1];
+    src = q1_.as<double2>();
+  } else {
+    base = (long long)q2slot_[((size_t)key[0] * Lt_ + key[1]) * nb_ + key[2]] * q_total_[2];
+    src = q2_.as<double2>();
+  }
+  int n = int(lst[op].rnd.size());
+  if (n > cap) throw std::runtime_error("output capacity too small");
+  SLAPH_HIP(hipMemcpy(out, src + (base + q_off_[qtype][op]) * DD, sizeof(cplx) * DD * n, hipMemcpyDeviceToHost));
+  for (int j = 0; j < n; ++j) {
+    ric[2 * j] = lst[op].rnd[j].first;
+    ric[2 * j + 1] = lst[op].rnd[j].second;
+  }
+  return n;
+}
+
+std::vector<std::string> Engine::diagrams() const {
+  std::vector<std::string> out;
+  for (auto const &r : results_) out.push_back(r.name);
+  return out;
+}
+
+int Engine::correlator_count(std::string const &d) const {
+  for (auto const &r : results_)
+    if (r.name == d) return int(r.names.size());
+  throw std::runtime_error("no results for diagram " + d);
+}
+
+std::string Engine::correlator_name(std::string const &d, int i) const {
+  for (auto const &r : results_)
+    if (r.name == d) return r.names.at(i);
+  throw std::runtime_error("no results for diagram " + d);
+}
+
+void Engine::fetch_correlators(std::string const &d, cplx *out) const {
+  for (auto const &r : results_)
+    if (r.name == d) {
+      std::copy(r.values.begin(), r.values.end(), out);
+      return;
+    }
+  throw std::runtime_error("no results for diagram " + d);
+}
+
+}  // namespace slaph

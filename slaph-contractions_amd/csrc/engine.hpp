@@ -1,0 +1,155 @@
+// The device-resident contraction engine behind the C ABI: one per context.
+//
+// It mirrors the reference's data flow (main/contract.cpp:28-107):
+// configure (read_parameters + init_lookup_tables) -> inputs (perambulators,
+// random vectors, eigenvectors) -> create_operators (VdaggerV) -> contract.
+// Every floating-point stage runs on the GPU; the host only plans.
+#pragma once
+
+#include "host.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace slaph {
+
+// Owning device allocation.
+class DevBuf {
+ public:
+  DevBuf() = default;
+  ~DevBuf() { reset(); }
+  DevBuf(DevBuf const &) = delete;
+  DevBuf &operator=(DevBuf const &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p_(o.p_), bytes_(o.bytes_) {
+    o.p_ = nullptr;
+    o.bytes_ = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      reset();
+      p_ = o.p_;
+      bytes_ = o.bytes_;
+      o.p_ = nullptr;
+      o.bytes_ = 0;
+    }
+    return *this;
+  }
+  void reset();
+  void ensure(size_t bytes);  // grow-only
+  void alloc(size_t bytes);   // exact (re)allocation
+  template <typename T>
+  T *as() const {
+    return static_cast<T *>(p_);
+  }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void *p_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+// Elapsed device time between construction and stop() on one stream.
+class EventTimer {
+ public:
+  explicit EventTimer(hipStream_t s);
+  ~EventTimer();
+  double stop();
+
+ private:
+  hipStream_t s_;
+  hipEvent_t a_, b_;
+};
+
+struct StageTimes {
+  double vdaggerv_ms = 0, factors_ms = 0, contract_ms = 0;
+  double vdaggerv_kernel_ms = 0;  // k_ahb only
+  int vdaggerv_launches = 0;
+};
+
+// Blocking upload of a host vector into a grow-only device buffer.
+template <typename T>
+void upload(DevBuf &buf, std::vector<T> const &v) {
+  buf.ensure(std::max<size_t>(sizeof(T) * v.size(), 16));
+  if (!v.empty()) SLAPH_HIP(hipMemcpy(buf.as<void>(), v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+}
+
+class Engine {
+ public:
+  explicit Engine(int device);
+  ~Engine();
+
+  void configure(std::string const &infile_text);
+  GlobalData const &gd() const { return gd_; }
+
+  // Inputs (host buffers, reference in-memory layouts).
+  void set_eigenvectors(int t, const cplx *V);     // K x N column-major, K = 3 Lx Ly Lz
+  void set_perambulator(int r, const cplx *P);     // resorted column-major
+  void set_random_vector(int r, const cplx *rho);  // Lt * N * 4, entry 4(N t + v) + d
+  void read_inputs(std::string const &base_dir, int config);
+  void synthetic_inputs(unsigned long long seed, bool orthonormalise);
+  void get_input(int kind, int index, cplx *out) const;
+
+  // OperatorFactory::create_operators (build) / return_vdaggerv.
+  void create_operators();
+  void return_vdaggerv(int id, int t, cplx *out);
+
+  // DilutedFactorFactory<Q>::operator[](key) for one operator index: copies
+  // the D x D matrices (in random-pair order) to out, pairs to ric.
+  int diluted_factor(int qtype, const int *key, int op, cplx *out, int *ric, int cap);
+
+  // contract (Correlators.cpp:40-127); results kept on the host.
+  void contract(int single_block_pair, int kahan);
+  std::vector<std::string> diagrams() const;
+  int correlator_count(std::string const &diagram) const;
+  std::string correlator_name(std::string const &diagram, int i) const;
+  void fetch_correlators(std::string const &diagram, cplx *out) const;  // n_corr x Lt
+
+  void set_chunk_budget(size_t bytes) { chunk_budget_ = bytes; }
+  StageTimes const &times() const { return times_; }
+  hipStream_t stream() const { return stream_; }
+
+  // Raw entry: C = A^H diag(phase) B for device buffers (tests and tools).
+  void ahb_raw(const double2 *A, const double2 *B, double2 *C, const double2 *phase, int M, int N, int K,
+               long long lda, long long ldb, long long ldc, int batch, long long sA, long long sB, long long sC);
+
+ private:
+  void require_configured() const;
+  void require_inputs() const;
+  void alloc_inputs();
+  const double2 *vdv_ptr(int id, bool dagger, int t) const;
+  void compute_q_offsets();
+  void build_factors(std::vector<std::vector<int>> const &q1keys, std::vector<std::vector<int>> const &q2keys);
+
+  int device_;
+  hipStream_t stream_ = nullptr;
+  GlobalData gd_;
+  bool configured_ = false;
+  bool have_ev_ = false, have_peram_ = false, have_rnd_ = false, have_ops_ = false;
+  long long K_ = 0;
+  int N_ = 0, D_ = 0, Lt_ = 0, nb_ = 0, nrnd_ = 0;
+
+  DevBuf V_, rho_, peram_, vdv_, vdvdag_, phase_;
+  std::vector<int> dag_slot_;  // vdaggerv id -> slot in vdvdag_, or -1
+
+  // factor slabs
+  DevBuf q0_, q1_, q2_;
+  std::vector<long long> q_off_[3];  // per Q type: entry -> first pair index
+  long long q_total_[3] = {0, 0, 0};
+  std::vector<int> q2slot_;          // (b1*Lt + t)*nb + b2 -> slot, -1 if absent
+  std::vector<char> q1need_;         // t*nb + b
+
+  struct DiagramResult {
+    std::string name;
+    std::vector<std::string> names;
+    std::vector<cplx> values;  // n_corr x Lt
+  };
+  std::vector<DiagramResult> results_;
+
+  size_t chunk_budget_ = size_t(24) << 30;
+  StageTimes times_;
+};
+
+}  // namespace slaph

@@ -1,0 +1,560 @@
+// Device kernels of the sLapH contraction path (gfx950 / CDNA4).
+//
+// VdaggerV and the first Q2 stage are dense complex products A^H B with a long
+// contraction dimension; they run on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64) with the momentum phase applied while the B tile is
+// staged into LDS.  The diluted-factor and trace products are many small
+// D x D matrices and run one wavefront per matrix, driven by index tables
+// that the host planner builds.
+#include "kernels.hpp"
+
+namespace slaph {
+
+// Defined in host.cpp: the sparse gamma tables as plain arrays.
+void host_gamma_tables(int (*rows)[4], double (*vals)[4][2]);
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // conj(a) * b
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+}
+
+// Sparse gamma tables in constant memory (same closed form as the host's
+// gamma_table(); uploaded once per process).
+__constant__ int c_gamma_row[16][4];
+__constant__ double2 c_gamma_val[16][4];
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// A^H diag(phase) B on FP64 MFMA.  Workgroup: 4 waves, 64 x 64 complex output
+// tile, each wave a 32 x 32 block = 2 x 2 MFMA 16x16 blocks for the real and
+// the imaginary part.  K is staged in chunks of 16 through LDS.
+// MFMA 16x16x4 f64 operand layout: A[m=lane%16][k=lane/16],
+// B[k=lane/16][n=lane%16]; result element i of lane l is row 4*(l/16)+i,
+// column l%16.
+// ---------------------------------------------------------------------------
+constexpr int AHB_TM = 64, AHB_TN = 64, AHB_KC = 16;
+
+__global__ __launch_bounds__(256) void k_ahb(const AhBJob *__restrict__ jobs, int M, int N, int K) {
+  const AhBJob J = jobs[blockIdx.z];
+  const int m0 = blockIdx.x * AHB_TM, n0 = blockIdx.y * AHB_TN;
+  __shared__ double2 As[AHB_TM][AHB_KC + 1];
+  __shared__ double2 Bs[AHB_TN][AHB_KC + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  d4 cr[2][2], ci[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      cr[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+      ci[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+  for (int k0 = 0; k0 < K; k0 += AHB_KC) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int idx = tid + r * 256;
+      const int kk = idx % AHB_KC, mm = idx / AHB_KC;
+      const int k = k0 + kk;
+      const int m = m0 + mm, n = n0 + mm;
+      double2 av = make_double2(0.0, 0.0), bv = make_double2(0.0, 0.0);
+      if (k < K && m < M) av = J.A[(long long)m * J.lda + k];
+      if (k < K && n < N) {
+        bv = J.B[(long long)n * J.ldb + k];
+        if (J.phase) bv = cmul(J.phase[k / 3], bv);
+      }
+      As[mm][kk] = av;
+      Bs[mm][kk] = bv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kq = 0; kq < AHB_KC; kq += 4) {
+      const int kr = kq + (lane >> 4);
+      const double2 a0 = As[wm * 32 + (lane & 15)][kr];
+      const double2 a1 = As[wm * 32 + 16 + (lane & 15)][kr];
+      const double2 b0 = Bs[wn * 32 + (lane & 15)][kr];
+      const double2 b1 = Bs[wn * 32 + 16 + (lane & 15)][kr];
+      const double2 av[2] = {a0, a1}, bv[2] = {b0, b1};
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) {
+          // conj(a) b = (ax bx + ay by) + i (ax by - ay bx)
+          cr[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi].x, bv[ni].x, cr[mi][ni], 0, 0, 0);
+          cr[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi].y, bv[ni].y, cr[mi][ni], 0, 0, 0);
+          ci[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi].x, bv[ni].y, ci[mi][ni], 0, 0, 0);
+          ci[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[mi].y, bv[ni].x, ci[mi][ni], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = m0 + wm * 32 + mi * 16 + 4 * (lane >> 4) + i;
+        const int col = n0 + wn * 32 + ni * 16 + (lane & 15);
+        if (row < M && col < N) {
+          double2 v = make_double2(cr[mi][ni][i], ci[mi][ni][i]);
+          if (J.rowscale) v = cmul(J.rowscale[row], v);
+          J.C[(long long)col * J.ldc + row] = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Q0[(e1 + dilE g(d)), (e2 + dilE d)] =
+//   gval(d) sum_{v=e1 mod dilE} conj rho1(v, g(d)) sum_{w=e2 mod dilE} VdV[v,w] rho2(w,d)
+// (reference: DilutedFactorFactory<Q0>::build, DilutedFactorFactory.cpp:33-101)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_q0(const Q0Job *__restrict__ jobs, int N, int dilE) {
+  extern __shared__ double2 X[];  // [v][d][e2] : N * 4 * dilE
+  const Q0Job J = jobs[blockIdx.x];
+  const int D = 4 * dilE;
+  for (int idx = threadIdx.x; idx < N * dilE; idx += blockDim.x) {
+    const int v = idx % N, e2 = idx / N;
+    double2 acc[4];
+    for (int d = 0; d < 4; ++d) acc[d] = make_double2(0.0, 0.0);
+    for (int w = e2; w < N; w += dilE) {
+      const double2 x = J.vdv[(long long)w * N + v];
+#pragma unroll
+      for (int d = 0; d < 4; ++d) cfma(acc[d], x, J.rho2[4 * w + d]);
+    }
+    for (int d = 0; d < 4; ++d) X[(v * 4 + d) * dilE + e2] = acc[d];
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < D * D; o += blockDim.x) {
+    const int row = o % D, col = o / D;
+    const int d = col / dilE, e2 = col % dilE;
+    const int g = c_gamma_row[J.gamma][d];
+    double2 s = make_double2(0.0, 0.0);
+    if (row / dilE == g) {
+      const int e1 = row % dilE;
+      for (int v = e1; v < N; v += dilE) cfma(s, make_double2(J.rho1[4 * v + g].x, -J.rho1[4 * v + g].y),
+                                              X[(v * 4 + d) * dilE + e2]);
+      s = cmul(c_gamma_val[J.gamma][d], s);
+    }
+    J.out[o] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Q1[(r,e1),(c,e2)] = gval(r) sum_{v=e1} conj rho1(v,r) sum_w VdV[v,w]
+//                      P2[(t,g(r),w),(b,c,e2)]
+// (reference: DilutedFactorFactory<Q1>::build, DilutedFactorFactory.cpp:103-170)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_q1(const Q1Job *__restrict__ jobs, int N, int dilE, long long prow) {
+  extern __shared__ double2 RV[];  // [w][d][e1]
+  const Q1Job J = jobs[blockIdx.x];
+  const int D = 4 * dilE;
+  for (int idx = threadIdx.x; idx < N * dilE; idx += blockDim.x) {
+    const int e1 = idx % dilE, w = idx / dilE;
+    double2 acc[4];
+    for (int d = 0; d < 4; ++d) acc[d] = make_double2(0.0, 0.0);
+    for (int v = e1; v < N; v += dilE) {
+      const double2 x = J.vdv[(long long)w * N + v];
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        const double2 r = J.rho1[4 * v + d];
+        cfma(acc[d], make_double2(r.x, -r.y), x);
+      }
+    }
+    for (int d = 0; d < 4; ++d) RV[(w * 4 + d) * dilE + e1] = acc[d];
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < D * D; o += blockDim.x) {
+    const int row = o % D, col = o / D;
+    const int r = row / dilE, e1 = row % dilE;
+    const int g = c_gamma_row[J.gamma][r];
+    const double2 *pcol = J.peram2 + (long long)(J.b * D + col) * prow + (long long)(J.t * 4 + g) * N;
+    double2 s = make_double2(0.0, 0.0);
+    for (int w = 0; w < N; ++w) cfma(s, RV[(w * 4 + r) * dilE + e1], pcol[w]);
+    J.out[o] = cmul(c_gamma_val[J.gamma][r], s);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Q2 second stage: Q2[(c,e),(c2,e2)] = sum_bd sum_w M'[(c,e),(bd,w)]
+//   P2[(t, g(bd), w), (b2, c2, e2)], M' already carries gval(bd) and the
+//   gamma_5 signs (reference: DilutedFactorFactory.cpp:221-243).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_q2b(const Q2BJob *__restrict__ jobs, int N, int dilE, long long prow) {
+  const Q2BJob J = jobs[blockIdx.x];
+  const int D = 4 * dilE;
+  for (int o = threadIdx.x; o < D * D; o += blockDim.x) {
+    const int row = o % D, col = o / D;
+    double2 s = make_double2(0.0, 0.0);
+    for (int bd = 0; bd < 4; ++bd) {
+      const int g = c_gamma_row[J.gamma][bd];
+      const double2 *pcol = J.peram2 + (long long)(J.b2 * D + col) * prow + (long long)(J.t * 4 + g) * N;
+      const double2 *mrow = J.Mp + (long long)bd * N * D + row;
+      for (int w = 0; w < N; ++w) cfma(s, mrow[(long long)w * D], pcol[w]);
+    }
+    J.out[o] = s;
+  }
+}
+
+__global__ void k_dagger(const double2 *__restrict__ in, double2 *__restrict__ out, int N, int count) {
+  const long long total = (long long)N * N * count;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long mat = i / ((long long)N * N);
+    const int r = int(i % N), c = int((i / N) % N);
+    const double2 x = in[mat * N * N + (long long)r * N + c];  // in[c, r]
+    out[i] = make_double2(x.x, -x.y);
+  }
+}
+
+__global__ void k_identity(double2 *out, int N, int count) {
+  const long long total = (long long)N * N * count;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int r = int(i % N), c = int((i / N) % N);
+    out[i] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched D x D factor product (reference: operator*, DilutedFactor.cpp:6-51):
+// one wavefront per output matrix, operands staged in LDS.
+// ---------------------------------------------------------------------------
+constexpr int kMaxD = 32;
+
+__global__ __launch_bounds__(64) void k_prod(const double2 *__restrict__ sX, const long long *__restrict__ Xt,
+                                             const long long *__restrict__ Xq, const double2 *__restrict__ sY,
+                                             const long long *__restrict__ Yt, const long long *__restrict__ Yq,
+                                             double2 *__restrict__ out, long long obase, long long ostride,
+                                             const ComboProd *__restrict__ combos, int D) {
+  __shared__ double2 A[kMaxD * kMaxD], B[kMaxD * kMaxD];
+  const ComboProd c = combos[blockIdx.x];
+  const int ti = blockIdx.y;
+  const long long DD = (long long)D * D;
+  const double2 *x = sX + (Xt[ti] + Xq[c.qn] + c.i) * DD;
+  const double2 *y = sY + (Yt[ti] + Yq[c.qn] + c.j) * DD;
+  for (int e = threadIdx.x; e < D * D; e += 64) {
+    A[e] = x[e];
+    B[e] = y[e];
+  }
+  __syncthreads();
+  double2 *o = out + (obase + ti * ostride + c.o) * DD;
+  for (int e = threadIdx.x; e < D * D; e += 64) {
+    const int r = e % D, col = e / D;
+    double2 s = make_double2(0.0, 0.0);
+    for (int k = 0; k < D; ++k) cfma(s, A[k * D + r], B[col * D + k]);
+    o[e] = s;
+  }
+}
+
+// tr(X Y) = sum_{a,b} X[a,b] Y[b,a] (reference: factor_to_trace(L, R),
+// DilutedTrace.cpp:84-138, which forms the full product first), or tr(X)
+// when no right operand is given (DilutedTrace.cpp:140-165).
+__global__ __launch_bounds__(64) void k_trace(const double2 *__restrict__ sX, const long long *__restrict__ Xt,
+                                              const long long *__restrict__ Xq, const double2 *__restrict__ sY,
+                                              const long long *__restrict__ Yt, const long long *__restrict__ Yq,
+                                              double2 *__restrict__ out, long long obase, long long ostride,
+                                              const ComboProd *__restrict__ combos, int D) {
+  const ComboProd c = combos[blockIdx.x];
+  const int ti = blockIdx.y;
+  const long long DD = (long long)D * D;
+  const double2 *x = sX + (Xt[ti] + Xq[c.qn] + c.i) * DD;
+  double re = 0.0, im = 0.0;
+  if (sY) {
+    const double2 *y = sY + (Yt[ti] + Yq[c.qn] + c.j) * DD;
+    for (int e = threadIdx.x; e < D * D; e += 64) {
+      const int a = e % D, b = e / D;
+      const double2 p = cmul(x[e], y[a * D + b]);
+      re += p.x;
+      im += p.y;
+    }
+  } else {
+    for (int a = threadIdx.x; a < D; a += 64) {
+      re += x[a * D + a].x;
+      im += x[a * D + a].y;
+    }
+  }
+  re = wave_sum(re);
+  im = wave_sum(im);
+  if (threadIdx.x == 0) out[obase + ti * ostride + c.o] = make_double2(re, im);
+}
+
+// ---------------------------------------------------------------------------
+// resolve_request{1,2,3} (reference: Diagram.cpp:9-94, DilutedTrace.cpp:5-82):
+// one thread per (slice pair, correlator).
+// ---------------------------------------------------------------------------
+__global__ void k_resolve(const double2 *__restrict__ tv, const uint16_t *__restrict__ masks,
+                          const CorrJob *__restrict__ jobs, int ncorr, int nsp, double2 *__restrict__ cval) {
+  const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (id >= (long long)ncorr * nsp) return;
+  const int corr = int(id % ncorr), s = int(id / ncorr);
+  const CorrJob J = jobs[corr];
+  const double2 *v0 = tv + J.tr[0].base + s * J.tr[0].stride;
+  double2 res = make_double2(0.0, 0.0);
+  if (J.ntr == 1) {
+    for (int i = 0; i < J.tr[0].n; ++i) res = cadd(res, v0[i]);
+    res.x /= J.tr[0].n;
+    res.y /= J.tr[0].n;
+  } else if (J.ntr == 2) {
+    const double2 *v1 = tv + J.tr[1].base + s * J.tr[1].stride;
+    const uint16_t *m0 = masks + J.tr[0].mask_off, *m1 = masks + J.tr[1].mask_off;
+    long long n = 0;
+    for (int i = 0; i < J.tr[0].n; ++i) {
+      double2 rs = make_double2(0.0, 0.0);
+      for (int j = 0; j < J.tr[1].n; ++j) {
+        if (m0[i] & m1[j]) continue;
+        rs = cadd(rs, v1[j]);
+        ++n;
+      }
+      res = cadd(res, cmul(v0[i], rs));
+    }
+    res.x /= double(n);
+    res.y /= double(n);
+  } else {
+    const double2 *v1 = tv + J.tr[1].base + s * J.tr[1].stride;
+    const double2 *v2 = tv + J.tr[2].base + s * J.tr[2].stride;
+    const uint16_t *m0 = masks + J.tr[0].mask_off, *m1 = masks + J.tr[1].mask_off, *m2 = masks + J.tr[2].mask_off;
+    long long n = 0;
+    for (int i = 0; i < J.tr[0].n; ++i)
+      for (int j = 0; j < J.tr[1].n; ++j) {
+        double2 rs = make_double2(0.0, 0.0);
+        const uint16_t mij = m0[i] & m1[j];
+        for (int k = 0; k < J.tr[2].n; ++k) {
+          if (mij & m2[k]) continue;
+          rs = cadd(rs, v2[k]);
+          ++n;
+        }
+        res = cadd(res, cmul(cmul(v0[i], v1[j]), rs));
+      }
+    res.x /= double(n);
+    res.y /= double(n);
+  }
+  cval[(long long)s * ncorr + corr] = res;
+}
+
+// correlator_[t][i] += c (reference: Diagram::assemble, Diagram.cpp:96-114);
+// Neumaier compensation (KahanAccumulator.hpp:23-32) when kahan != 0.
+__global__ void k_accumulate(const double2 *__restrict__ cval, const int *__restrict__ tdelta, int nsp, int ncorr,
+                             double2 *__restrict__ sum, double2 *__restrict__ comp, int kahan) {
+  const int corr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (corr >= ncorr) return;
+  for (int s = 0; s < nsp; ++s) {
+    const long long at = (long long)tdelta[s] * ncorr + corr;
+    const double2 x = cval[(long long)s * ncorr + corr];
+    double2 acc = sum[at];
+    if (!kahan) {
+      sum[at] = cadd(acc, x);
+    } else {
+      const double2 t = cadd(acc, x);
+      const double na = hypot(acc.x, acc.y), nx = hypot(x.x, x.y);
+      double2 c = comp[at];
+      if (na >= nx) {
+        c.x += (acc.x - t.x) + x.x;
+        c.y += (acc.y - t.y) + x.y;
+      } else {
+        c.x += (x.x - t.x) + acc.x;
+        c.y += (x.y - t.y) + acc.y;
+      }
+      comp[at] = c;
+      sum[at] = t;
+    }
+  }
+}
+
+__global__ void k_scale(double2 *x, long long n, double f) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    x[i].x *= f;
+    x[i].y *= f;
+  }
+}
+
+// splitmix64 finaliser as a counter-based generator: value i of stream `seed`.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double u01(unsigned long long z) { return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+
+__global__ void k_fill_gauss(double2 *x, long long n, unsigned long long seed) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const unsigned long long h = mix64(seed ^ mix64((unsigned long long)i));
+    const double u1 = u01(h), u2 = u01(mix64(h));
+    const double r = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincos(6.283185307179586 * u2, &sn, &cs);
+    x[i] = make_double2(r * cs * 0.7071067811865476, r * sn * 0.7071067811865476);
+  }
+}
+
+__global__ void k_fill_z2z2(double2 *x, long long n, unsigned long long seed) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const unsigned long long h = mix64(seed ^ mix64((unsigned long long)i));
+    const double a = (h & 1) ? 0.7071067811865476 : -0.7071067811865476;
+    const double b = (h & 2) ? 0.7071067811865476 : -0.7071067811865476;
+    x[i] = make_double2(a, b);
+  }
+}
+
+// C = A R with A: K x N, R: N x N small, one thread per row segment.
+__global__ __launch_bounds__(256) void k_gemm_nn_small(const double2 *__restrict__ A, const double2 *__restrict__ R,
+                                                       double2 *__restrict__ C, long long K, int N) {
+  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (k >= K) return;
+  double2 s = make_double2(0.0, 0.0);
+  for (int j = 0; j < N; ++j) cfma(s, A[(long long)j * K + k], R[(long long)c * N + j]);
+  C[(long long)c * K + k] = s;
+}
+
+bool g_const_ready[64] = {false};
+
+void upload_gamma_constants() {
+  int dev = 0;
+  SLAPH_HIP(hipGetDevice(&dev));
+  if (dev < 64 && g_const_ready[dev]) return;
+  int rows[16][4];
+  double2 vals[16][4];
+  // Columns k of gamma g: value at row rows[g][k] (tmLQCD basis products,
+  // same derivation as the host's gamma_table()).
+  double tmp[16][4][2];
+  host_gamma_tables(rows, tmp);
+  for (int g = 0; g < 16; ++g)
+    for (int k = 0; k < 4; ++k) vals[g][k] = make_double2(tmp[g][k][0], tmp[g][k][1]);
+  SLAPH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gamma_row), rows, sizeof rows));
+  SLAPH_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gamma_val), vals, sizeof vals));
+  if (dev < 64) g_const_ready[dev] = true;
+}
+
+dim3 grid1d(long long n, int block) {
+  long long g = (n + block - 1) / block;
+  if (g > 65535LL * 16) g = 65535LL * 16;
+  if (g < 1) g = 1;
+  return dim3(unsigned(g));
+}
+
+}  // namespace
+
+void launch_ahb(const AhBJob *d_jobs, int njobs, int M, int N, int K, hipStream_t s) {
+  if (njobs <= 0) return;
+  if (njobs > 65535) throw std::runtime_error("launch_ahb: too many jobs in one launch");
+  dim3 grid((M + AHB_TM - 1) / AHB_TM, (N + AHB_TN - 1) / AHB_TN, njobs);
+  hipLaunchKernelGGL(k_ahb, grid, dim3(256), 0, s, d_jobs, M, N, K);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_q0(const Q0Job *d_jobs, int njobs, int N, int dilE, hipStream_t s) {
+  if (njobs <= 0) return;
+  upload_gamma_constants();
+  size_t lds = sizeof(double2) * N * 4 * dilE;
+  if (lds > 160 * 1024) throw std::runtime_error("Q0: N*4*dilE too large for LDS");
+  hipLaunchKernelGGL(k_q0, dim3(njobs), dim3(256), lds, s, d_jobs, N, dilE);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_q1(const Q1Job *d_jobs, int njobs, int N, int dilE, long long prow, hipStream_t s) {
+  if (njobs <= 0) return;
+  upload_gamma_constants();
+  size_t lds = sizeof(double2) * N * 4 * dilE;
+  if (lds > 160 * 1024) throw std::runtime_error("Q1: N*4*dilE too large for LDS");
+  hipLaunchKernelGGL(k_q1, dim3(njobs), dim3(256), lds, s, d_jobs, N, dilE, prow);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_q2b(const Q2BJob *d_jobs, int njobs, int N, int dilE, long long prow, hipStream_t s) {
+  if (njobs <= 0) return;
+  upload_gamma_constants();
+  hipLaunchKernelGGL(k_q2b, dim3(njobs), dim3(256), 0, s, d_jobs, N, dilE, prow);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_dagger(const double2 *in, double2 *out, int N, int count, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_dagger, grid1d((long long)N * N * count, 256), dim3(256), 0, s, in, out, N, count);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_identity(double2 *out, int N, int count, hipStream_t s) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_identity, grid1d((long long)N * N * count, 256), dim3(256), 0, s, out, N, count);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_prod(const double2 *sX, const long long *Xt, const long long *Xq, const double2 *sY, const long long *Yt,
+                 const long long *Yq, double2 *out, long long obase, long long ostride, const ComboProd *combos,
+                 int ncombo, int ntime, int D, hipStream_t s) {
+  if (ncombo <= 0 || ntime <= 0) return;
+  if (D > kMaxD) throw std::runtime_error("factor dimension above 32 is not supported");
+  if (ntime > 65535) throw std::runtime_error("launch_prod: too many time keys");
+  hipLaunchKernelGGL(k_prod, dim3(ncombo, ntime), dim3(64), 0, s, sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos,
+                     D);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_trace(const double2 *sX, const long long *Xt, const long long *Xq, const double2 *sY, const long long *Yt,
+                  const long long *Yq, double2 *out, long long obase, long long ostride, const ComboProd *combos,
+                  int ncombo, int ntime, int D, hipStream_t s) {
+  if (ncombo <= 0 || ntime <= 0) return;
+  if (ntime > 65535) throw std::runtime_error("launch_trace: too many slice pairs");
+  hipLaunchKernelGGL(k_trace, dim3(ncombo, ntime), dim3(64), 0, s, sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos,
+                     D);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_resolve(const double2 *tv, const uint16_t *masks, const CorrJob *jobs, int ncorr, int nsp, double2 *cval,
+                    hipStream_t s) {
+  long long n = (long long)ncorr * nsp;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_resolve, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, tv, masks, jobs, ncorr, nsp, cval);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_accumulate(const double2 *cval, const int *tdelta, int nsp, int ncorr, double2 *sum, double2 *comp,
+                       int kahan, hipStream_t s) {
+  if (ncorr <= 0 || nsp <= 0) return;
+  hipLaunchKernelGGL(k_accumulate, dim3((ncorr + 255) / 256), dim3(256), 0, s, cval, tdelta, nsp, ncorr, sum, comp,
+                     kahan);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_scale(double2 *x, long long n, double f, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scale, grid1d(n, 256), dim3(256), 0, s, x, n, f);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_fill_gauss(double2 *x, long long n, unsigned long long seed, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_gauss, grid1d(n, 256), dim3(256), 0, s, x, n, seed);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_fill_z2z2(double2 *x, long long n, unsigned long long seed, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_z2z2, grid1d(n, 256), dim3(256), 0, s, x, n, seed);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_gemm_nn_small(const double2 *A, const double2 *R, double2 *C, long long K, int N, hipStream_t s) {
+  if (K <= 0 || N <= 0) return;
+  dim3 grid(unsigned((K + 255) / 256), N);
+  hipLaunchKernelGGL(k_gemm_nn_small, grid, dim3(256), 0, s, A, R, C, K, N);
+  SLAPH_HIP(hipGetLastError());
+}
+
+}  // namespace slaph

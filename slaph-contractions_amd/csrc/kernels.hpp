@@ -1,0 +1,102 @@
+// Device kernels of the contraction path and their host launchers.
+// All matrices are column-major complex<double> stored as double2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace slaph {
+
+#define SLAPH_HIP(call)                                                                       \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " +   \
+                               __FILE__ + ":" + std::to_string(__LINE__) + ": " #call);       \
+  } while (0)
+
+// One matrix product C = rowscale .* (A^H diag(phase[k/3]) B) of a batch.
+// A: K x M (lda), B: K x N (ldb), C: M x N (ldc); phase/rowscale may be null.
+struct AhBJob {
+  const double2 *A;
+  const double2 *B;
+  double2 *C;
+  const double2 *phase;
+  const double2 *rowscale;
+  long long lda, ldb, ldc;
+};
+void launch_ahb(const AhBJob *d_jobs, int njobs, int M, int N, int K, hipStream_t s);
+
+// Q0 of one (t, operator, random pair): D x D.
+struct Q0Job {
+  const double2 *vdv;   // N x N (daggered if needed)
+  const double2 *rho1;  // random vector r1 at timeslice t: N x 4 (index 4v+d)
+  const double2 *rho2;
+  double2 *out;
+  int gamma;
+};
+void launch_q0(const Q0Job *d_jobs, int njobs, int N, int dilE, hipStream_t s);
+
+// Q1 of one (t, b, operator, random pair): D x D.
+struct Q1Job {
+  const double2 *vdv;
+  const double2 *rho1;
+  const double2 *peram2;  // column-major perambulator of r2
+  double2 *out;
+  int t, b, gamma;
+};
+void launch_q1(const Q1Job *d_jobs, int njobs, int N, int dilE, long long peram_rows, hipStream_t s);
+
+// Second stage of Q2: out = M' (D x 4N) * stacked perambulator rows.
+struct Q2BJob {
+  const double2 *Mp;      // D x 4N, column (bd*N + w)
+  const double2 *peram2;
+  double2 *out;
+  int t, b2, gamma;
+};
+void launch_q2b(const Q2BJob *d_jobs, int njobs, int N, int dilE, long long peram_rows, hipStream_t s);
+
+void launch_dagger(const double2 *in, double2 *out, int N, int count, hipStream_t s);
+void launch_identity(double2 *out, int N, int count, hipStream_t s);
+
+// Table-driven batched factor product / trace kernels.
+// Operand matrix index = Xt[ti] + Xq[qn] + i; output index = obase + ti*ostride + o.
+struct ComboProd {
+  int qn, i, j;
+  long long o;
+};
+void launch_prod(const double2 *slabX, const long long *Xt, const long long *Xq, const double2 *slabY,
+                 const long long *Yt, const long long *Yq, double2 *out, long long obase, long long ostride,
+                 const ComboProd *combos, int ncombo, int ntime, int D, hipStream_t s);
+// out scalar index = obase + ti*ostride + o; right slab null -> trace of a single matrix.
+void launch_trace(const double2 *slabX, const long long *Xt, const long long *Xq, const double2 *slabY,
+                  const long long *Yt, const long long *Yq, double2 *out, long long obase, long long ostride,
+                  const ComboProd *combos, int ncombo, int ntime, int D, hipStream_t s);
+
+// Correlator resolution: mean / 2-way / 3-way masked inner products.
+struct TraceRef {
+  long long base;    // family base + tr offset (add s*stride)
+  long long stride;  // family stride per slice pair
+  int n;             // number of random-vector combinations
+  int mask_off;      // into masks[]
+};
+struct CorrJob {
+  int ntr;
+  TraceRef tr[3];
+};
+void launch_resolve(const double2 *tvals, const uint16_t *masks, const CorrJob *jobs, int ncorr, int nsp,
+                    double2 *cval, hipStream_t s);
+void launch_accumulate(const double2 *cval, const int *tdelta, int nsp, int ncorr, double2 *sum, double2 *comp,
+                       int kahan, hipStream_t s);
+void launch_scale(double2 *x, long long n, double f, hipStream_t s);
+
+// Synthetic inputs: seeded counter-based Gaussian / Z2xZ2 fills.
+void launch_fill_gauss(double2 *x, long long n, unsigned long long seed, hipStream_t s);
+void launch_fill_z2z2(double2 *x, long long n, unsigned long long seed, hipStream_t s);
+// C (K x N) = A (K x N) * R (N x N), R column-major, all on device.
+void launch_gemm_nn_small(const double2 *A, const double2 *R, double2 *C, long long K, int N, hipStream_t s);
+
+}  // namespace slaph

@@ -1,0 +1,252 @@
+"""Python binding of the MI355X contraction path of sLapH-contractions.
+
+Thin ctypes layer over the C ABI in include/slaph_amd.h, shaped like the
+reference's host flow (main/contract.cpp:28-107):
+
+    ctx = Context(device=0)
+    ctx.configure(open("test_all.ini").read())   # read_parameters + init_lookup_tables
+    ctx.read_inputs(base_dir, config=1000)       # Perambulator / RandomVector / EigenVector
+    ops = OperatorFactory(ctx); ops.create_operators()
+    contract(ctx)                                # Correlators.cpp:40-127
+    ctx.correlators("C2c")                       # {dataset name: complex[Lt]}
+
+All arithmetic runs in the HIP library `libslaph_amd.so` built next to this
+package; importing this module fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libslaph_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "slaph_amd.h")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libslaph_amd.so is not built: run `python slaph-contractions_amd/build.py` "
+                      "(or __graft_entry__.build()) first")
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+_c = ctypes
+_P = _c.c_void_p
+_D = _c.POINTER(_c.c_double)
+_I = _c.POINTER(_c.c_int)
+
+_SIGS = {
+    "slc_ctx_create": (_c.c_int, [_c.c_int, _c.POINTER(_P)]),
+    "slc_ctx_destroy": (None, [_P]),
+    "slc_last_error": (_c.c_char_p, [_P]),
+    "slc_configure": (_c.c_int, [_P, _c.c_char_p]),
+    "slc_query": (_c.c_int, [_P, _c.c_char_p, _c.POINTER(_c.c_longlong)]),
+    "slc_set_eigenvectors": (_c.c_int, [_P, _c.c_int, _D]),
+    "slc_set_perambulator": (_c.c_int, [_P, _c.c_int, _D]),
+    "slc_set_random_vector": (_c.c_int, [_P, _c.c_int, _D]),
+    "slc_read_inputs": (_c.c_int, [_P, _c.c_char_p, _c.c_int]),
+    "slc_synthetic_inputs": (_c.c_int, [_P, _c.c_ulonglong, _c.c_int]),
+    "slc_get_input": (_c.c_int, [_P, _c.c_int, _c.c_int, _D]),
+    "slc_create_operators": (_c.c_int, [_P]),
+    "slc_return_vdaggerv": (_c.c_int, [_P, _c.c_int, _c.c_int, _D]),
+    "slc_diluted_factor": (_c.c_int, [_P, _c.c_int, _I, _c.c_int, _D, _I, _c.c_int, _I]),
+    "slc_contract": (_c.c_int, [_P, _c.c_int, _c.c_int]),
+    "slc_diagram_count": (_c.c_int, [_P, _I]),
+    "slc_diagram_name": (_c.c_int, [_P, _c.c_int, _c.c_char_p, _c.c_int]),
+    "slc_correlator_count": (_c.c_int, [_P, _c.c_char_p, _I]),
+    "slc_correlator_name": (_c.c_int, [_P, _c.c_char_p, _c.c_int, _c.c_char_p, _c.c_int]),
+    "slc_fetch_correlators": (_c.c_int, [_P, _c.c_char_p, _D]),
+    "slc_stage_time": (_c.c_int, [_P, _c.c_char_p, _D]),
+    "slc_set_chunk_budget": (_c.c_int, [_P, _c.c_longlong]),
+    "slc_ahb_device": (_c.c_int, [_P, _P, _P, _P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_longlong,
+                                  _c.c_longlong, _c.c_longlong, _c.c_int, _c.c_longlong, _c.c_longlong,
+                                  _c.c_longlong]),
+    "slc_lookup_dump": (_c.c_int, [_c.c_char_p, _c.c_char_p, _c.c_int]),
+    "slc_dilution_text": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.c_char_p, _c.c_int]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(_lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+class SlaphError(RuntimeError):
+    """Error reported through the C ABI (message of the reference's runtime_error)."""
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+def lookup_dump(infile_text: str) -> str:
+    """init_lookup_tables as text; host only (no device needed)."""
+    need = _lib.slc_lookup_dump(infile_text.encode(), None, 0)
+    buf = _c.create_string_buffer(abs(need) + 1 if need > 0 else 1 << 20)
+    rc = _lib.slc_lookup_dump(infile_text.encode(), buf, len(buf))
+    if rc < 0:
+        raise SlaphError(buf.value.decode())
+    return buf.value.decode()
+
+
+def dilution_text(num_slice: int, num_block: int, interlace: bool) -> str:
+    need = _lib.slc_dilution_text(num_slice, num_block, int(interlace), None, 0)
+    if need < 0:
+        raise SlaphError("invalid dilution scheme")
+    buf = _c.create_string_buffer(need)
+    _lib.slc_dilution_text(num_slice, num_block, int(interlace), buf, need)
+    return buf.value.decode()
+
+
+class Context:
+    """One device context (C ABI slc_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = _P()
+        rc = _lib.slc_ctx_create(device, _c.byref(h))
+        if rc != 0:
+            raise SlaphError("cannot create a context on HIP device %d (code %d)" % (device, rc))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.slc_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc < 0:
+            raise SlaphError(_lib.slc_last_error(self._h).decode())
+        return rc
+
+    # GlobalData ---------------------------------------------------------
+    def configure(self, infile_text: str):
+        self._check(_lib.slc_configure(self._h, infile_text.encode()))
+
+    def query(self, what: str) -> int:
+        v = _c.c_longlong()
+        self._check(_lib.slc_query(self._h, what.encode(), _c.byref(v)))
+        return v.value
+
+    # inputs ---------------------------------------------------------------
+    def set_eigenvectors(self, t: int, V: np.ndarray):
+        V = np.asfortranarray(V, dtype=np.complex128)
+        self._check(_lib.slc_set_eigenvectors(self._h, t, _dptr(V.reshape(-1, order="F"))))
+
+    def set_perambulator(self, r: int, P: np.ndarray):
+        P = np.asfortranarray(P, dtype=np.complex128)
+        self._check(_lib.slc_set_perambulator(self._h, r, _dptr(P.reshape(-1, order="F"))))
+
+    def set_random_vector(self, r: int, rho: np.ndarray):
+        rho = np.ascontiguousarray(rho, dtype=np.complex128)
+        self._check(_lib.slc_set_random_vector(self._h, r, _dptr(rho)))
+
+    def read_inputs(self, base_dir: str, config: int):
+        self._check(_lib.slc_read_inputs(self._h, base_dir.encode(), config))
+
+    def synthetic_inputs(self, seed: int, orthonormalise: bool = True):
+        self._check(_lib.slc_synthetic_inputs(self._h, seed, int(orthonormalise)))
+
+    def get_input(self, kind: str, index: int) -> np.ndarray:
+        """Resident input back on the host: 'eigenvectors' (K x N), 'perambulator'
+        (rows x cols), 'random_vector' (Lt*N*4)."""
+        k = {"eigenvectors": 0, "perambulator": 1, "random_vector": 2}[kind]
+        Lt, N = self.query("Lt"), self.query("nev")
+        if k == 0:
+            shape = (3 * self.query("Lx") * self.query("Ly") * self.query("Lz"), N)
+        elif k == 1:
+            shape = (self.query("peram_rows"), self.query("peram_cols"))
+        else:
+            shape = (Lt * N * 4,)
+        out = np.zeros(int(np.prod(shape)), dtype=np.complex128)
+        self._check(_lib.slc_get_input(self._h, k, index, _dptr(out)))
+        return out.reshape(shape, order="F") if len(shape) == 2 else out
+
+    # results ----------------------------------------------------------------
+    def diagrams(self):
+        n = _c.c_int()
+        self._check(_lib.slc_diagram_count(self._h, _c.byref(n)))
+        out = []
+        for i in range(n.value):
+            buf = _c.create_string_buffer(256)
+            self._check(_lib.slc_diagram_name(self._h, i, buf, 256))
+            out.append(buf.value.decode())
+        return out
+
+    def correlators(self, diagram: str):
+        n = _c.c_int()
+        d = diagram.encode()
+        self._check(_lib.slc_correlator_count(self._h, d, _c.byref(n)))
+        Lt = self.query("Lt")
+        vals = np.zeros(n.value * Lt, dtype=np.complex128)
+        self._check(_lib.slc_fetch_correlators(self._h, d, _dptr(vals)))
+        vals = vals.reshape(n.value, Lt)
+        out = {}
+        for i in range(n.value):
+            need = self._check(_lib.slc_correlator_name(self._h, d, i, None, 0))
+            buf = _c.create_string_buffer(need)
+            self._check(_lib.slc_correlator_name(self._h, d, i, buf, need))
+            out[buf.value.decode()] = vals[i]
+        return out
+
+    def stage_time(self, what: str) -> float:
+        v = _c.c_double()
+        self._check(_lib.slc_stage_time(self._h, what.encode(), _c.byref(v)))
+        return v.value
+
+    def set_chunk_budget(self, nbytes: int):
+        self._check(_lib.slc_set_chunk_budget(self._h, int(nbytes)))
+
+    def ahb_device(self, A_ptr, B_ptr, C_ptr, phase_ptr, M, N, K, lda, ldb, ldc, batch=1, sA=0, sB=0, sC=0):
+        """C = A^H diag(phase[k/3]) B on device pointers (ints)."""
+        self._check(_lib.slc_ahb_device(self._h, A_ptr, B_ptr, C_ptr, phase_ptr, M, N, K, lda, ldb, ldc,
+                                        batch, sA, sB, sC))
+
+
+class OperatorFactory:
+    """OperatorFactory (include/OperatorsForMesons.hpp:39-69), handling_vdaggerv=build."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+
+    def create_operators(self):
+        self.ctx._check(_lib.slc_create_operators(self.ctx._h))
+
+    def return_vdaggerv(self, index: int, t: int) -> np.ndarray:
+        N = self.ctx.query("nev")
+        out = np.zeros(N * N, dtype=np.complex128)
+        self.ctx._check(_lib.slc_return_vdaggerv(self.ctx._h, index, t, _dptr(out)))
+        return out.reshape(N, N, order="F")
+
+
+class DilutedFactorFactory:
+    """DilutedFactorFactory<Q0|Q1|Q2> (include/DilutedFactorFactory.hpp:28-65):
+    factory[key][op] -> list of (matrix D x D, (r1, r2))."""
+
+    _QT = {"Q0": 0, "Q1": 1, "Q2": 2}
+
+    def __init__(self, ctx: Context, qtype: str):
+        self.ctx = ctx
+        self.q = self._QT[qtype]
+
+    def get(self, key, op: int):
+        key = (_c.c_int * len(key))(*key)
+        D = 4 * self.ctx.query("dilE")
+        cap = 256
+        out = np.zeros(cap * D * D, dtype=np.complex128)
+        ric = np.zeros(2 * cap, dtype=np.int32)
+        n = _c.c_int()
+        self.ctx._check(_lib.slc_diluted_factor(self.ctx._h, self.q, key, op, _dptr(out),
+                                                ric.ctypes.data_as(_I), cap, _c.byref(n)))
+        mats = out[: n.value * D * D].reshape(n.value, D, D).transpose(0, 2, 1)  # column-major
+        return [(mats[i], (int(ric[2 * i]), int(ric[2 * i + 1]))) for i in range(n.value)]
+
+
+def contract(ctx: Context, single_time_slice_combination: int = -1, kahan: bool = False):
+    """contract() of Correlators.cpp:40-127 on the device."""
+    ctx._check(_lib.slc_contract(ctx._h, single_time_slice_combination, int(kahan)))

@@ -1,0 +1,232 @@
+"""GPU parity of the HIP path, called through the C ABI, against the oracle
+and the reference's golden L4 correlators."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import slaph_oracle as o
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+L4 = os.path.join(GOLD, "l4")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def slaph():
+    import slaph as s
+    return s
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def _rand(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize("M,N,K,phase", [(16, 16, 4, False), (220, 220, 3 * 512, True), (16, 220, 220, False),
+                                         (7, 5, 13, True), (64, 129, 100, True)])
+def test_ahb_kernel_matches_numpy(slaph, torch_cuda, M, N, K, phase):
+    """C = A^H diag(phase[k/3]) B on FP64 MFMA (checks the MFMA lane layout too)."""
+    torch = torch_cuda
+    say("ahb", M, N, K, phase)
+    rng = np.random.default_rng(M * 1000 + N + K)
+    A = _rand(rng, K, M)
+    Bm = _rand(rng, K, N)
+    ph = np.exp(-1j * rng.uniform(0, 2 * np.pi, size=(K + 2) // 3))
+    dA = torch.from_numpy(np.asfortranarray(A).reshape(-1, order="F").view(np.float64).copy()).cuda()
+    dB = torch.from_numpy(np.asfortranarray(Bm).reshape(-1, order="F").view(np.float64).copy()).cuda()
+    dP = torch.from_numpy(ph.view(np.float64).copy()).cuda()
+    dC = torch.zeros(2 * M * N, dtype=torch.float64, device="cuda")
+    ctx = slaph.Context(0)
+    ctx.ahb_device(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), dP.data_ptr() if phase else None, M, N, K, K, K, M)
+    torch.cuda.synchronize()
+    C = dC.cpu().numpy().view(np.complex128).reshape(N, M).T
+    want = A.conj().T @ ((np.repeat(ph, 3)[:K, None] if phase else 1.0) * Bm)
+    assert np.max(np.abs(C - want)) <= 1e-12 * max(1.0, np.max(np.abs(want))) * 4
+
+
+@pytest.fixture(scope="module")
+def l4_ctx(slaph):
+    say("l4: configure + read inputs")
+    ctx = slaph.Context(0)
+    ctx.configure(open(os.path.join(L4, "test_all.ini")).read())
+    ctx.read_inputs(L4, 1000)
+    ops = slaph.OperatorFactory(ctx)
+    ops.create_operators()
+    return ctx, ops
+
+
+@pytest.fixture(scope="module")
+def l4_oracle():
+    cfg = o.parse_infile(open(os.path.join(L4, "test_all.ini")).read())
+    lut = o.init_lookup_tables(cfg)
+    perams, rnds, ev = o.read_inputs(cfg, L4, 1000)
+    vdv = o.build_vdaggerv(cfg, lut, ev)
+    return cfg, lut, perams, rnds, vdv
+
+
+def test_l4_vdaggerv_matches_oracle(l4_ctx, l4_oracle):
+    ctx, ops = l4_ctx
+    cfg, lut, perams, rnds, vdv = l4_oracle
+    say("l4: vdaggerv")
+    for vid, mom, _ in lut["vdv_lookup"]:
+        for t in range(cfg["Lt"]):
+            got = ops.return_vdaggerv(vid, t)
+            assert np.max(np.abs(got - vdv[(vid, t)])) < 1e-13, (vid, t)
+
+
+def test_l4_diluted_factors_match_oracle(slaph, l4_ctx, l4_oracle):
+    ctx, ops = l4_ctx
+    cfg, lut, perams, rnds, vdv = l4_oracle
+    say("l4: factors")
+    fac = o.Factories(cfg, lut, vdv, perams, rnds, cfg["quarks"][0]["dilT"], cfg["quarks"][0]["dilE"])
+    cases = [("Q0", (1,)), ("Q0", (3,)), ("Q1", (2, 0)), ("Q1", (0, 1)), ("Q2", (1, 2, 0)), ("Q2", (0, 0, 0))]
+    for qt, key in cases:
+        dev = slaph.DilutedFactorFactory(ctx, qt)
+        want = fac.get(qt, key)
+        for op in range(len(lut["ql_lookup"][qt])):
+            got = dev.get(key, op)
+            assert len(got) == len(want[op])
+            for (gm, gric), (wm, wric, _) in zip(got, want[op]):
+                assert gric == wric
+                assert np.max(np.abs(gm - wm)) < 1e-12, (qt, key, op)
+
+
+@pytest.mark.parametrize("kahan", [False, True])
+def test_l4_golden_correlators(slaph, l4_ctx, kahan):
+    """All 18 diagrams, 2,307 datasets, against the reference's golden files at
+    1e-12 absolute (the reference gate, tests/integration-L4/compare:16)."""
+    ctx, ops = l4_ctx
+    say("l4: contract kahan=%d" % kahan)
+    slaph.contract(ctx, kahan=kahan)
+    gold = np.load(os.path.join(GOLD, "l4_correlators.npz"))
+    seen = set()
+    worst = 0.0
+    for d in ctx.diagrams():
+        for name, vals in ctx.correlators(d).items():
+            key = d + "/" + name
+            seen.add(key)
+            err = np.max(np.abs(vals - gold[key]))
+            worst = max(worst, err)
+            assert err <= 1e-12, (key, err)
+    assert seen == set(gold.files)
+    say("l4 worst abs error", worst)
+
+
+def test_l4_single_block_pair_and_chunking(slaph, l4_ctx, l4_oracle):
+    """single_time_slice_combination and a tiny chunk budget (one slice pair
+    per launch) give the oracle's numbers."""
+    ctx, ops = l4_ctx
+    cfg, lut, perams, rnds, vdv = l4_oracle
+    say("l4: single block pair 1, chunk budget 1 byte")
+    ctx.set_chunk_budget(1)
+    try:
+        slaph.contract(ctx, single_time_slice_combination=1)
+    finally:
+        ctx.set_chunk_budget(24 << 30)
+    want = o.contract(cfg, lut, vdv, perams, rnds, single_block_pair=1)
+    for d, lst in want.items():
+        got = ctx.correlators(d)
+        for name, vals in lst:
+            assert np.max(np.abs(got[name] - vals)) <= 1e-12, (d, name)
+
+
+SMALL = """
+Lt = 8
+Lx = 4
+Ly = 4
+Lz = 4
+number_of_eigen_vec = 8
+[quarks]
+quark = u:4:TB:2:EI:4:DF:4:synthetic
+[operator_lists]
+operator_list = g5.d0.p0,1
+[correlator_lists]
+correlator_list = C2c:Q0:Op0:Q0:Op0:P0,1
+correlator_list = C4cD:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1
+correlator_list = C4cB:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0
+correlator_list = C4cC:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0
+correlator_list = C6cD:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0
+"""
+
+
+def test_host_inputs_small_pipi_matches_oracle(slaph):
+    """Config-1-like lattice (4^3 x 8, N=8, dilE 4, 4 random vectors) with
+    seeded host inputs: every correlator against the oracle at 1e-12 norm-wise."""
+    say("small pipi")
+    cfg = o.parse_infile(SMALL)
+    lut = o.init_lookup_tables(cfg)
+    rng = np.random.default_rng(11)
+    Lt, N, K = cfg["Lt"], cfg["number_of_eigen_vec"], 3 * 64
+    ev = [np.linalg.qr(_rand(rng, K, N))[0] for _ in range(Lt)]
+    nrnd = 4
+    nb = Lt // 2
+    perams = [_rand(rng, 4 * N * Lt, nb * 4 * 4) for _ in range(nrnd)]
+    rnds = [((rng.integers(0, 2, Lt * N * 4) * 2 - 1) + 1j * (rng.integers(0, 2, Lt * N * 4) * 2 - 1)) / np.sqrt(2)
+            for _ in range(nrnd)]
+    ctx = slaph.Context(0)
+    ctx.configure(SMALL)
+    for t in range(Lt):
+        ctx.set_eigenvectors(t, ev[t])
+    for r in range(nrnd):
+        ctx.set_perambulator(r, perams[r])
+        ctx.set_random_vector(r, rnds[r])
+    slaph.OperatorFactory(ctx).create_operators()
+    slaph.contract(ctx)
+    vdv = o.build_vdaggerv(cfg, lut, ev)
+    want = o.contract(cfg, lut, vdv, perams, rnds)
+    for d, lst in want.items():
+        got = ctx.correlators(d)
+        for name, vals in lst:
+            scale = max(np.max(np.abs(vals)), 1e-300)
+            assert np.max(np.abs(got[name] - vals)) / scale <= 1e-12, (d, name)
+
+
+def test_synthetic_inputs_match_oracle(slaph):
+    """Device-generated inputs: eigenvectors orthonormal (V^H V = 1), random
+    vectors in {(+-1 +-i)/sqrt 2}, and every correlator of the small pipi
+    workload equal to the oracle's on the downloaded inputs."""
+    say("synthetic inputs")
+    cfg = o.parse_infile(SMALL)
+    lut = o.init_lookup_tables(cfg)
+    ctx = slaph.Context(0)
+    ctx.configure(SMALL)
+    ctx.synthetic_inputs(1234, True)
+    Lt, nrnd = cfg["Lt"], 4
+    ev = [ctx.get_input("eigenvectors", t) for t in range(Lt)]
+    perams = [ctx.get_input("perambulator", r) for r in range(nrnd)]
+    rnds = [ctx.get_input("random_vector", r) for r in range(nrnd)]
+    for V in ev:
+        assert np.max(np.abs(V.conj().T @ V - np.eye(V.shape[1]))) < 1e-13
+    for rho in rnds:
+        assert np.allclose(np.abs(rho.real), 2 ** -0.5) and np.allclose(np.abs(rho.imag), 2 ** -0.5)
+    slaph.OperatorFactory(ctx).create_operators()
+    slaph.contract(ctx)
+    vdv = o.build_vdaggerv(cfg, lut, ev)
+    want = o.contract(cfg, lut, vdv, perams, rnds)
+    for d, lst in want.items():
+        got = ctx.correlators(d)
+        for name, vals in lst:
+            scale = max(np.max(np.abs(vals)), 1e-300)
+            assert np.max(np.abs(got[name] - vals)) / scale <= 1e-12, (d, name)
+
+
+def test_not_enough_random_vectors_raises(slaph):
+    ctx = slaph.Context(0)
+    with pytest.raises(slaph.SlaphError, match="not enough random vectors"):
+        ctx.configure(SMALL.replace("u:4:", "u:1:"))
