@@ -36,6 +36,8 @@ enum {
  * (the reference keeps its state in GlobalData and the factory objects). */
 int slc_ctx_create(int device, slc_ctx **out);
 void slc_ctx_destroy(slc_ctx *ctx);
+/* Message of the last failure on ctx; with ctx == NULL, of the last failed
+ * slc_ctx_create on this thread. */
 const char *slc_last_error(const slc_ctx *ctx);
 
 /* read_parameters + init_lookup_tables from the text of an infile
@@ -123,6 +125,11 @@ int slc_set_chunk_budget(slc_ctx *ctx, long long bytes);
 int slc_ahb_device(slc_ctx *ctx, const double *A, const double *B, double *C, const double *phase, int M, int N,
                    int K, long long lda, long long ldb, long long ldc, int batch, long long sA, long long sB,
                    long long sC);
+
+/* The same product on host buffers (A: K x M, B: K x N, C: M x N, phase of
+ * length ceil(K/3) or NULL); copies through the device.  For checks. */
+int slc_ahb_host(slc_ctx *ctx, const double *A, const double *B, double *C, const double *phase, int M, int N,
+                 int K);
 
 /* Host-only (no device needed): parse an infile and dump the lookup tables
  * init_lookup_tables builds (src/init_lookup_tables.cpp:470-556) as text --

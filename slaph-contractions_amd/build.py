@@ -20,16 +20,13 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-I" + SRC]
 # Packed FP32 instructions stay off in device code (toolchain notes).
-DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-munsafe-fp-atomics"]
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["kernels.hip", "host.cpp", "engine.cpp", "contract.cpp", "capi.cpp"]
 
 
 def compile_one(src):
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC] + COMMON + DEVICE + ["-c", os.path.join(SRC, src), "-o", out]
-    if not src.endswith(".hip"):
-        cmd[1:1] = ["-x", "hip"] if False else []
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))

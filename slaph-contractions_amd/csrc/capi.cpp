@@ -16,6 +16,10 @@ struct slc_ctx {
 
 namespace {
 
+// Message of the last failed slc_ctx_create on this thread (no context exists
+// to hold it); returned by slc_last_error(NULL).
+thread_local std::string g_create_error;
+
 template <typename F>
 int guarded(slc_ctx *ctx, F &&f) {
   if (!ctx) return SLC_ERR_ARG;
@@ -56,6 +60,7 @@ int slc_ctx_create(int device, slc_ctx **out) {
   try {
     c->eng.reset(new slaph::Engine(device));
   } catch (std::exception const &e) {
+    g_create_error = e.what();
     delete c;
     return SLC_ERR_DEVICE;
   }
@@ -65,7 +70,7 @@ int slc_ctx_create(int device, slc_ctx **out) {
 
 void slc_ctx_destroy(slc_ctx *ctx) { delete ctx; }
 
-const char *slc_last_error(const slc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char *slc_last_error(const slc_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
 
 int slc_configure(slc_ctx *ctx, const char *text) {
   return guarded(ctx, [&] {
@@ -204,6 +209,26 @@ int slc_ahb_device(slc_ctx *ctx, const double *A, const double *B, double *C, co
     ctx->eng->ahb_raw(reinterpret_cast<const double2 *>(A), reinterpret_cast<const double2 *>(B),
                       reinterpret_cast<double2 *>(C), reinterpret_cast<const double2 *>(phase), M, N, K, lda, ldb, ldc,
                       batch, sA, sB, sC);
+  });
+}
+
+int slc_ahb_host(slc_ctx *ctx, const double *A, const double *B, double *C, const double *phase, int M, int N,
+                 int K) {
+  return guarded(ctx, [&] {
+    if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) throw std::out_of_range("invalid matrix shape");
+    slaph::DevBuf dA, dB, dC, dP;
+    dA.alloc(sizeof(double2) * (size_t)K * M);
+    dB.alloc(sizeof(double2) * (size_t)K * N);
+    dC.alloc(sizeof(double2) * (size_t)M * N);
+    SLAPH_HIP(hipMemcpy(dA.as<void>(), A, dA.bytes(), hipMemcpyHostToDevice));
+    SLAPH_HIP(hipMemcpy(dB.as<void>(), B, dB.bytes(), hipMemcpyHostToDevice));
+    if (phase) {
+      dP.alloc(sizeof(double2) * (size_t)((K + 2) / 3));
+      SLAPH_HIP(hipMemcpy(dP.as<void>(), phase, dP.bytes(), hipMemcpyHostToDevice));
+    }
+    ctx->eng->ahb_raw(dA.as<double2>(), dB.as<double2>(), dC.as<double2>(), phase ? dP.as<double2>() : nullptr, M,
+                      N, K, K, K, M, 1, 0, 0, 0);
+    SLAPH_HIP(hipMemcpy(C, dC.as<void>(), dC.bytes(), hipMemcpyDeviceToHost));
   });
 }
 

@@ -46,8 +46,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 // tile, each wave a 32 x 32 block = 2 x 2 MFMA 16x16 blocks for the real and
 // the imaginary part.  K is staged in chunks of 16 through LDS.
 // MFMA 16x16x4 f64 operand layout: A[m=lane%16][k=lane/16],
-// B[k=lane/16][n=lane%16]; result element i of lane l is row 4*(l/16)+i,
-// column l%16.
+// B[k=lane/16][n=lane%16]; result element i of lane l is row l/16 + 4*i,
+// column l%16 (measured: tests/test_gpu_parity.py::test_ahb_kernel_matches_numpy).
 // ---------------------------------------------------------------------------
 constexpr int AHB_TM = 64, AHB_TN = 64, AHB_KC = 16;
 
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void k_ahb(const AhBJob *__restrict__ jobs, in
     for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int row = m0 + wm * 32 + mi * 16 + 4 * (lane >> 4) + i;
+        const int row = m0 + wm * 32 + mi * 16 + (lane >> 4) + 4 * i;
         const int col = n0 + wn * 32 + ni * 16 + (lane & 15);
         if (row < M && col < N) {
           double2 v = make_double2(cr[mi][ni][i], ci[mi][ni][i]);

@@ -61,6 +61,7 @@ _SIGS = {
     "slc_ahb_device": (_c.c_int, [_P, _P, _P, _P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_longlong,
                                   _c.c_longlong, _c.c_longlong, _c.c_int, _c.c_longlong, _c.c_longlong,
                                   _c.c_longlong]),
+    "slc_ahb_host": (_c.c_int, [_P, _D, _D, _D, _D, _c.c_int, _c.c_int, _c.c_int]),
     "slc_lookup_dump": (_c.c_int, [_c.c_char_p, _c.c_char_p, _c.c_int]),
     "slc_dilution_text": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.c_char_p, _c.c_int]),
 }
@@ -108,13 +109,18 @@ class Context:
         h = _P()
         rc = _lib.slc_ctx_create(device, _c.byref(h))
         if rc != 0:
-            raise SlaphError("cannot create a context on HIP device %d (code %d)" % (device, rc))
+            raise SlaphError("cannot create a context on HIP device %d (code %d): %s"
+                             % (device, rc, _lib.slc_last_error(None).decode()))
         self._h = h
 
     def close(self):
-        if getattr(self, "_h", None):
-            _lib.slc_ctx_destroy(self._h)
-            self._h = None
+        h = getattr(self, "_h", None)
+        self._h = None
+        if h and _lib is not None:
+            try:
+                _lib.slc_ctx_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
 
     def __del__(self):
         self.close()
@@ -201,6 +207,17 @@ class Context:
 
     def set_chunk_budget(self, nbytes: int):
         self._check(_lib.slc_set_chunk_budget(self._h, int(nbytes)))
+
+    def ahb_host(self, A: np.ndarray, B: np.ndarray, phase=None) -> np.ndarray:
+        """A^H diag(phase[k/3]) B on the device for host matrices A (K x M), B (K x N)."""
+        K, M = A.shape
+        N = B.shape[1]
+        a = np.asfortranarray(A, dtype=np.complex128).reshape(-1, order="F")
+        b = np.asfortranarray(B, dtype=np.complex128).reshape(-1, order="F")
+        c = np.zeros(M * N, dtype=np.complex128)
+        p = None if phase is None else np.ascontiguousarray(phase, dtype=np.complex128)
+        self._check(_lib.slc_ahb_host(self._h, _dptr(a), _dptr(b), _dptr(c), None if p is None else _dptr(p), M, N, K))
+        return c.reshape(M, N, order="F")
 
     def ahb_device(self, A_ptr, B_ptr, C_ptr, phase_ptr, M, N, K, lda, ldb, ldc, batch=1, sA=0, sB=0, sC=0):
         """C = A^H diag(phase[k/3]) B on device pointers (ints)."""

@@ -26,36 +26,21 @@ def slaph():
     return s
 
 
-@pytest.fixture(scope="module")
-def torch_cuda():
-    torch = pytest.importorskip("torch")
-    if not torch.cuda.is_available():
-        pytest.skip("no HIP device")
-    return torch
-
-
 def _rand(rng, *shape):
     return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
 
 
 @pytest.mark.parametrize("M,N,K,phase", [(16, 16, 4, False), (220, 220, 3 * 512, True), (16, 220, 220, False),
                                          (7, 5, 13, True), (64, 129, 100, True)])
-def test_ahb_kernel_matches_numpy(slaph, torch_cuda, M, N, K, phase):
+def test_ahb_kernel_matches_numpy(slaph, M, N, K, phase):
     """C = A^H diag(phase[k/3]) B on FP64 MFMA (checks the MFMA lane layout too)."""
-    torch = torch_cuda
     say("ahb", M, N, K, phase)
     rng = np.random.default_rng(M * 1000 + N + K)
     A = _rand(rng, K, M)
     Bm = _rand(rng, K, N)
     ph = np.exp(-1j * rng.uniform(0, 2 * np.pi, size=(K + 2) // 3))
-    dA = torch.from_numpy(np.asfortranarray(A).reshape(-1, order="F").view(np.float64).copy()).cuda()
-    dB = torch.from_numpy(np.asfortranarray(Bm).reshape(-1, order="F").view(np.float64).copy()).cuda()
-    dP = torch.from_numpy(ph.view(np.float64).copy()).cuda()
-    dC = torch.zeros(2 * M * N, dtype=torch.float64, device="cuda")
     ctx = slaph.Context(0)
-    ctx.ahb_device(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), dP.data_ptr() if phase else None, M, N, K, K, K, M)
-    torch.cuda.synchronize()
-    C = dC.cpu().numpy().view(np.complex128).reshape(N, M).T
+    C = ctx.ahb_host(A, Bm, ph if phase else None)
     want = A.conj().T @ ((np.repeat(ph, 3)[:K, None] if phase else 1.0) * Bm)
     assert np.max(np.abs(C - want)) <= 1e-12 * max(1.0, np.max(np.abs(want))) * 4
 
