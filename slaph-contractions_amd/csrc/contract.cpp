@@ -210,7 +210,8 @@ struct NodePlan {
   long long stride = 0;
   std::vector<ComboProd> combos;
   long long cmaxX = 0, cmaxY = 0, cmaxO = 0;  // largest qn offset + combo index per operand
-  std::vector<int> trl_qn;  // trl -> qn
+  std::vector<int> trl_qn;  // trl -> qn (in the owner's qn space)
+  int owner = -1;           // node whose product table this node reads
   DevBuf d_combos, d_Xq, d_Yq;
   // time part (per chunk)
   std::map<std::vector<long long>, int> time_index;
@@ -231,12 +232,21 @@ struct Family {
   std::vector<NodePlan> nodes;
   std::vector<int> mask_off;  // trl -> offset into the global mask array
   std::vector<int> ncomb;     // trl -> number of trace values
+  // mask groups of each trace vector (first-seen order of the masks)
+  std::vector<int> ngroup, goff, gmask_off;  // trl -> count, offset in the family's group sums, global mask offset
+  std::vector<std::vector<int>> gcount;      // trl -> combinations per group
+  std::vector<GroupItem> gitems;
+  std::vector<int2> gpairs;  // (left, right) combination indices, grouped
+  long long gstride = 0;     // group sums per slice pair
+  long long gbase = 0;       // offset of this family in the chunk's group-sum buffer
+  DevBuf d_gitems, d_gpairs;
 };
 
 struct CorrPlan {
   std::string name;
   std::vector<std::vector<std::pair<int, int>>> reqs;  // correlator -> [(family, trl)]
   std::vector<std::string> datasets;
+  std::vector<double> inv_n;  // 1 / admitted combinations per correlator
 };
 
 // Every (factor type, time key) a leaf of this family reads at slice pair
@@ -302,16 +312,35 @@ void Engine::contract(int single_block_pair, int kahan) {
   for (auto &f : fams) {
     auto const &cl = gd_.trace_indices_map.at(f.name);
     f.nodes.resize(f.shape.nodes.size());
+    // Product nodes with the same subtree shape (same factor types in the same
+    // places) share one product table: an alias keeps its own time/qn index
+    // maps into its owner's storage, so an instance (time key, operator
+    // entries) is computed once however many nodes read it.
+    std::vector<std::string> sig(f.shape.nodes.size());
     for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
       auto const &ns = f.shape.nodes[n];
-      NodePlan &np = f.nodes[n];
+      if (ns.kind == kLeaf) sig[n] = "L" + std::to_string(f.shape.leaves[ns.leaf].q);
+      else sig[n] = "P(" + sig[ns.left] + "," + (ns.right >= 0 ? sig[ns.right] : "") + ")";
+      f.nodes[n].owner = int(n);
+      if (ns.kind == kProd)
+        for (size_t m = 0; m < n; ++m)
+          if (f.shape.nodes[m].kind == kProd && sig[m] == sig[n]) {
+            f.nodes[n].owner = f.nodes[m].owner;
+            break;
+          }
+    }
+    for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+      auto const &ns = f.shape.nodes[n];
+      NodePlan &self = f.nodes[n];
+      NodePlan &np = f.nodes[self.owner];
       bool root = n + 1 == f.shape.nodes.size();
-      np.trl_qn.resize(f.tr_ids.size());
+      self.trl_qn.resize(f.tr_ids.size());
+      size_t nqn_before = np.qn_key.size();
       for (size_t trl = 0; trl < f.tr_ids.size(); ++trl) {
         auto const &c = cl.at(f.tr_ids[trl]);
         if (ns.kind == kLeaf) {
           auto const &ls = f.shape.leaves[ns.leaf];
-          np.trl_qn[trl] = c.at(ls.cidx);  // leaf qn = operator entry
+          self.trl_qn[trl] = c.at(ls.cidx);  // leaf qn = operator entry
           continue;
         }
         std::vector<int> key;
@@ -319,7 +348,8 @@ void Engine::contract(int single_block_pair, int kahan) {
           auto const &cs = f.shape.nodes[child];
           if (cs.kind == kLeaf) key.push_back(c.at(f.shape.leaves[cs.leaf].cidx));
           else {
-            auto const &k2 = f.nodes[child].qn_key[f.nodes[child].trl_qn[trl]];
+            NodePlan const &co = f.nodes[f.nodes[child].owner];
+            auto const &k2 = co.qn_key[f.nodes[child].trl_qn[trl]];
             key.insert(key.end(), k2.begin(), k2.end());
           }
         };
@@ -333,32 +363,36 @@ void Engine::contract(int single_block_pair, int kahan) {
           np.qn_left.push_back(f.nodes[ns.left].trl_qn[trl]);
           np.qn_right.push_back(ns.right >= 0 ? f.nodes[ns.right].trl_qn[trl] : 0);
         }
-        np.trl_qn[trl] = it->second;
+        self.trl_qn[trl] = it->second;
       }
       if (ns.kind == kLeaf) continue;
-      // templates and combos
+      if (self.owner != int(n) && np.qn_key.size() == nqn_before) continue;  // alias, nothing new
+      // templates and combos (of the owner, over all its qn entries)
+      auto const &os = f.shape.nodes[self.owner];
       auto tmpl_of = [&](int child, int cqn) {
         auto const &cs = f.shape.nodes[child];
         if (cs.kind == kLeaf) {
           auto const &ls = f.shape.leaves[cs.leaf];
           return tm.leaf(gd_.quarkline_lookuptable.at(qname[ls.q]).at(cqn).rnd);
         }
-        return f.nodes[child].qn_tmpl[cqn];
+        return f.nodes[f.nodes[child].owner].qn_tmpl[cqn];
       };
       np.qn_off.assign(np.qn_key.size() + 1, 0);
       np.qn_tmpl.assign(np.qn_key.size(), -1);
+      np.combos.clear();
+      np.cmaxX = np.cmaxY = np.cmaxO = 0;
       if (root) {
         f.mask_off.assign(f.tr_ids.size(), 0);
         f.ncomb.assign(f.tr_ids.size(), 0);
       }
       for (size_t q = 0; q < np.qn_key.size(); ++q) {
-        int lt = tmpl_of(ns.left, np.qn_left[q]);
+        int lt = tmpl_of(os.left, np.qn_left[q]);
         PairTmpl const *pt;
         if (ns.kind == kProd) {
-          pt = &tm.product(lt, tmpl_of(ns.right, np.qn_right[q]));
+          pt = &tm.product(lt, tmpl_of(os.right, np.qn_right[q]));
           np.qn_tmpl[q] = pt->out;
         } else if (ns.kind == kTrace2) {
-          pt = &tm.trace2(lt, tmpl_of(ns.right, np.qn_right[q]));
+          pt = &tm.trace2(lt, tmpl_of(os.right, np.qn_right[q]));
         } else {
           pt = &tm.trace1(lt);
         }
@@ -379,11 +413,11 @@ void Engine::contract(int single_block_pair, int kahan) {
       auto off_of = [&](int child, int cqn) -> long long {
         auto const &cs = f.shape.nodes[child];
         if (cs.kind == kLeaf) return q_off_[f.shape.leaves[cs.leaf].q].at(cqn);
-        return f.nodes[child].qn_off[cqn];
+        return f.nodes[f.nodes[child].owner].qn_off[cqn];
       };
       for (size_t q = 0; q < np.qn_key.size(); ++q) {
-        Xq[q] = off_of(ns.left, np.qn_left[q]);
-        if (ns.right >= 0) Yq[q] = off_of(ns.right, np.qn_right[q]);
+        Xq[q] = off_of(os.left, np.qn_left[q]);
+        if (os.right >= 0) Yq[q] = off_of(os.right, np.qn_right[q]);
       }
       for (auto const &cb : np.combos) {
         np.cmaxX = std::max(np.cmaxX, Xq[cb.qn] + cb.i);
@@ -419,8 +453,80 @@ void Engine::contract(int single_block_pair, int kahan) {
                   std::vector<std::vector<int>>(need2.begin(), need2.end()));
     times_.factors_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
-  DevBuf d_masks;
-  upload(d_masks, masks);
+  // ---- mask groups per trace vector and the exact normalisation of each
+  // correlator (the num_summands of DilutedTrace.cpp:5-82, Diagram.cpp:9-27) ----
+  std::vector<uint16_t> gmasks;
+  for (auto &f : fams) {
+    NodePlan const &root = f.nodes.back();
+    size_t ntrl = f.tr_ids.size();
+    f.ngroup.assign(ntrl, 0);
+    f.goff.assign(ntrl, 0);
+    f.gmask_off.assign(ntrl, 0);
+    f.gcount.assign(ntrl, {});
+    f.gitems.clear();
+    f.gpairs.clear();
+    long long off = 0;
+    for (size_t trl = 0; trl < ntrl; ++trl) {
+      std::vector<uint16_t> gm;
+      std::vector<std::vector<int>> members;
+      for (int c = 0; c < f.ncomb[trl]; ++c) {
+        uint16_t m = masks[f.mask_off[trl] + c];
+        size_t g = std::find(gm.begin(), gm.end(), m) - gm.begin();
+        if (g == gm.size()) {
+          gm.push_back(m);
+          members.emplace_back();
+        }
+        members[g].push_back(c);
+      }
+      f.ngroup[trl] = int(gm.size());
+      f.goff[trl] = int(off);
+      f.gmask_off[trl] = int(gmasks.size());
+      gmasks.insert(gmasks.end(), gm.begin(), gm.end());
+      int q = root.trl_qn[trl];
+      for (size_t g = 0; g < gm.size(); ++g) {
+        f.gitems.push_back({q, int(f.gpairs.size()), int(members[g].size()), off + (long long)g});
+        for (int c : members[g]) {
+          auto const &cb = root.combos.at(root.qn_off[q] + c);  // combos of qn q are stored at qn_off[q]..
+          if (cb.qn != q || cb.o != root.qn_off[q] + c) throw std::logic_error("trace combo table out of order");
+          f.gpairs.push_back(make_int2(cb.i, cb.j));
+        }
+        f.gcount[trl].push_back(int(members[g].size()));
+      }
+      off += (long long)gm.size();
+    }
+    f.gstride = off;
+    upload(f.d_gitems, f.gitems);
+    upload(f.d_gpairs, f.gpairs);
+  }
+  for (auto &cp : corrs) {
+    cp.inv_n.resize(cp.reqs.size());
+    for (size_t i = 0; i < cp.reqs.size(); ++i) {
+      auto const &r = cp.reqs[i];
+      auto gmk = [&](int k, int g) { return gmasks[fams[r[k].first].gmask_off[r[k].second] + g]; };
+      auto cnt = [&](int k, int g) { return (long long)fams[r[k].first].gcount[r[k].second][g]; };
+      auto ng = [&](int k) { return fams[r[k].first].ngroup[r[k].second]; };
+      long long n = 0;
+      if (r.size() == 1) {
+        n = fams[r[0].first].ncomb[r[0].second];
+      } else if (r.size() == 2) {
+        for (int g = 0; g < ng(0); ++g)
+          for (int h = 0; h < ng(1); ++h)
+            if (!(gmk(0, g) & gmk(1, h))) n += cnt(0, g) * cnt(1, h);
+      } else {
+        for (int g = 0; g < ng(0); ++g)
+          for (int h = 0; h < ng(1); ++h)
+            for (int k = 0; k < ng(2); ++k)
+              if (!(gmk(0, g) & gmk(1, h) & gmk(2, k))) n += cnt(0, g) * cnt(1, h) * cnt(2, k);
+      }
+      if (n == 0)
+        throw std::runtime_error(
+            "compcomp_t inner_product(vector<DilutedTrace>, vector<DilutedTrace>) has an empty result. Not enough "
+            "random vectors?");
+      cp.inv_n[i] = 1.0 / double(n);
+    }
+  }
+  DevBuf d_gmasks;
+  upload(d_gmasks, gmasks);
 
   // Slices per chunk from the memory budget (worst case: every product node
   // has a distinct time key per slice pair).
@@ -430,7 +536,7 @@ void Engine::contract(int single_block_pair, int kahan) {
     for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
       auto const &ns = f.shape.nodes[n];
       if (ns.kind == kProd) per_sp += double(f.nodes[n].stride) * DD * sizeof(double2);
-      if (n + 1 == f.shape.nodes.size()) per_sp += double(f.nodes[n].stride) * sizeof(double2);
+      if (n + 1 == f.shape.nodes.size()) per_sp += double(f.gstride) * sizeof(double2);
     }
   for (auto const &c : corrs) per_sp += double(c.reqs.size()) * sizeof(double2);
   int smax = int(std::max(1.0, std::min(65535.0, double(chunk_budget_) / std::max(per_sp, 1.0))));
@@ -444,7 +550,7 @@ void Engine::contract(int single_block_pair, int kahan) {
     SLAPH_HIP(hipMemsetAsync(d_sum[c].as<void>(), 0, sizeof(double2) * n, stream_));
     SLAPH_HIP(hipMemsetAsync(d_comp[c].as<void>(), 0, sizeof(double2) * n, stream_));
   }
-  DevBuf d_prod, d_tr, d_cval, d_td;
+  DevBuf d_prod, d_cval, d_td, d_gv;
   std::vector<DevBuf> d_jobs_c(corrs.size());
   size_t max_corr = 1;
   for (auto const &c : corrs) max_corr = std::max(max_corr, c.reqs.size());
@@ -474,23 +580,27 @@ void Engine::contract(int single_block_pair, int kahan) {
     for (size_t c0 = 0; c0 < sl.size(); c0 += smax) {
       const int S = int(std::min<size_t>(smax, sl.size() - c0));
       // time planning
-      long long prod_total = 0, tr_total = 0;
+      long long prod_total = 0;
       for (auto &f : fams) {
         std::vector<std::vector<int>> tts(S);
         for (int s = 0; s < S; ++s) tts[s] = trace_times(f, sl[c0 + s].source, sl[c0 + s].sink);
         // subtree leaf bases per node and slice pair
         std::vector<std::vector<std::vector<long long>>> keys(f.shape.nodes.size(),
                                                               std::vector<std::vector<long long>>(S));
-        for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
-          auto const &ns = f.shape.nodes[n];
-          NodePlan &np = f.nodes[n];
-          bool root = n + 1 == f.shape.nodes.size();
+        for (auto &np : f.nodes) {
           np.time_index.clear();
           np.time_left.clear();
           np.time_right.clear();
           np.time_leafbase_l.clear();
           np.time_leafbase_r.clear();
           np.sp_time.assign(S, 0);
+        }
+        // (a) time keys; aliases index into their owner's time table
+        for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+          auto const &ns = f.shape.nodes[n];
+          NodePlan &self = f.nodes[n];
+          NodePlan &np = f.nodes[self.owner];
+          bool root = n + 1 == f.shape.nodes.size();
           for (int s = 0; s < S; ++s) {
             if (ns.kind == kLeaf) {
               keys[n][s] = {leaf_base(f.shape.leaves[ns.leaf], tts[s])};
@@ -505,13 +615,13 @@ void Engine::contract(int single_block_pair, int kahan) {
             } else {
               auto it = np.time_index.find(key);
               if (it != np.time_index.end()) {
-                np.sp_time[s] = it->second;
+                self.sp_time[s] = it->second;
                 continue;
               }
               ti = int(np.time_index.size());
               np.time_index.emplace(key, ti);
             }
-            np.sp_time[s] = ti;
+            self.sp_time[s] = ti;
             auto child_t = [&](int child, std::vector<int> &ct, std::vector<long long> &lb) {
               if (child < 0) {
                 ct.push_back(0);
@@ -529,25 +639,38 @@ void Engine::contract(int single_block_pair, int kahan) {
             child_t(ns.left, np.time_left, np.time_leafbase_l);
             child_t(ns.right, np.time_right, np.time_leafbase_r);
           }
-          if (ns.kind == kLeaf) continue;
-          int ntime = root ? S : int(np.time_index.size());
-          if (root) {
-            np.base = tr_total;
-            tr_total += (long long)S * np.stride;
-          } else {
-            np.base = prod_total;
-            prod_total += (long long)ntime * np.stride;
+        }
+        // (b) storage of the owners' product tables
+        for (size_t n = 0; n + 1 < f.shape.nodes.size(); ++n) {
+          NodePlan &np = f.nodes[n];
+          if (f.shape.nodes[n].kind != kProd || np.owner != int(n)) continue;
+          np.base = prod_total;
+          prod_total += (long long)np.time_index.size() * np.stride;
+        }
+        for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+          NodePlan &np = f.nodes[n];
+          if (np.owner != int(n)) {
+            np.base = f.nodes[np.owner].base;
+            np.stride = f.nodes[np.owner].stride;
           }
-          // operand time tables: leaf -> slab base; product -> base + ti * stride
+        }
+        // (c) operand time tables of the owners: leaf -> slab base; product ->
+        // base + ti * stride of the child's owner
+        for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+          auto const &ns = f.shape.nodes[n];
+          NodePlan &np = f.nodes[n];
+          if (ns.kind == kLeaf || np.owner != int(n)) continue;
+          bool root = n + 1 == f.shape.nodes.size();
+          int ntime = root ? S : int(np.time_index.size());
           std::vector<long long> Xt(ntime), Yt(ntime, 0);
           for (int ti = 0; ti < ntime; ++ti) {
             auto const &l = f.shape.nodes[ns.left];
-            Xt[ti] = l.kind == kLeaf ? np.time_leafbase_l[ti]
-                                     : f.nodes[ns.left].base + np.time_left[ti] * f.nodes[ns.left].stride;
+            NodePlan const &lo = f.nodes[f.nodes[ns.left].owner];
+            Xt[ti] = l.kind == kLeaf ? np.time_leafbase_l[ti] : lo.base + np.time_left[ti] * lo.stride;
             if (ns.right >= 0) {
               auto const &r = f.shape.nodes[ns.right];
-              Yt[ti] = r.kind == kLeaf ? np.time_leafbase_r[ti]
-                                       : f.nodes[ns.right].base + np.time_right[ti] * f.nodes[ns.right].stride;
+              NodePlan const &ro = f.nodes[f.nodes[ns.right].owner];
+              Yt[ti] = r.kind == kLeaf ? np.time_leafbase_r[ti] : ro.base + np.time_right[ti] * ro.stride;
             }
           }
           upload(np.d_Xt, Xt);
@@ -557,13 +680,19 @@ void Engine::contract(int single_block_pair, int kahan) {
         }
       }
       d_prod.ensure(sizeof(double2) * std::max<long long>(1, prod_total * DD));
-      d_tr.ensure(sizeof(double2) * std::max<long long>(1, tr_total));
+      // trace values go straight into mask-group sums
+      long long g_total = 0;
+      for (auto &f : fams) {
+        f.gbase = g_total;
+        g_total += (long long)S * f.gstride;
+      }
+      d_gv.ensure(sizeof(double2) * std::max<long long>(1, g_total));
       // launches: products in topological order, then traces
       for (auto &f : fams)
         for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
           auto const &ns = f.shape.nodes[n];
           NodePlan &np = f.nodes[n];
-          if (ns.kind == kLeaf) continue;
+          if (ns.kind == kLeaf || np.owner != int(n)) continue;
           bool root = n + 1 == f.shape.nodes.size();
           auto src = [&](int child) -> const double2 * {
             if (child < 0) return nullptr;
@@ -588,16 +717,18 @@ void Engine::contract(int single_block_pair, int kahan) {
           };
           if (max_of(np.d_Xt, np.h_Xt) + np.cmaxX >= slab_count(ns.left) ||
               (ns.right >= 0 && max_of(np.d_Yt, np.h_Yt) + np.cmaxY >= slab_count(ns.right)) ||
-              np.base + (long long)(ntime - 1) * np.stride + np.cmaxO >= (root ? tr_total : prod_total))
+              (root ? f.gbase + (long long)(ntime - 1) * f.gstride + f.gstride > g_total
+                    : np.base + (long long)(ntime - 1) * np.stride + np.cmaxO >= prod_total))
             throw std::logic_error("contraction plan indexes outside its buffers (" + f.name + ")");
           if (!root) {
             launch_prod(src(ns.left), np.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
                         np.d_Yt.as<long long>(), np.d_Yq.as<long long>(), d_prod.as<double2>(), np.base, np.stride,
                         np.d_combos.as<ComboProd>(), int(np.combos.size()), ntime, D_, stream_);
           } else {
-            launch_trace(src(ns.left), np.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
-                         np.d_Yt.as<long long>(), np.d_Yq.as<long long>(), d_tr.as<double2>(), np.base, np.stride,
-                         np.d_combos.as<ComboProd>(), int(np.combos.size()), ntime, D_, stream_);
+            launch_trace_groups(src(ns.left), np.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
+                                np.d_Yt.as<long long>(), np.d_Yq.as<long long>(), f.d_gitems.as<GroupItem>(),
+                                int(f.gitems.size()), f.d_gpairs.as<int2>(), d_gv.as<double2>(), f.gbase, f.gstride,
+                                ntime, D_, stream_);
           }
         }
       // resolve and accumulate per diagram
@@ -609,21 +740,21 @@ void Engine::contract(int single_block_pair, int kahan) {
         std::vector<CorrJob> jobs(cp.reqs.size());
         for (size_t i = 0; i < cp.reqs.size(); ++i) {
           jobs[i].ntr = int(cp.reqs[i].size());
+          jobs[i].inv_n = cp.inv_n[i];
           for (size_t k = 0; k < cp.reqs[i].size(); ++k) {
             auto const &f = fams[cp.reqs[i][k].first];
             int trl = cp.reqs[i][k].second;
-            NodePlan const &root = f.nodes.back();
-            jobs[i].tr[k].base = root.base + root.qn_off[root.trl_qn[trl]];
-            jobs[i].tr[k].stride = root.stride;
-            jobs[i].tr[k].n = f.ncomb[trl];
-            jobs[i].tr[k].mask_off = f.mask_off[trl];
-            if (jobs[i].tr[k].base + (long long)(S - 1) * root.stride + f.ncomb[trl] > tr_total ||
-                f.mask_off[trl] + f.ncomb[trl] > (long long)masks.size())
-              throw std::logic_error("correlator request indexes outside the trace buffer");
+            jobs[i].tr[k].base = f.gbase + f.goff[trl];
+            jobs[i].tr[k].stride = f.gstride;
+            jobs[i].tr[k].ngroup = f.ngroup[trl];
+            jobs[i].tr[k].gmask_off = f.gmask_off[trl];
+            if (jobs[i].tr[k].base + (long long)(S - 1) * f.gstride + f.ngroup[trl] > g_total ||
+                f.gmask_off[trl] + f.ngroup[trl] > (long long)gmasks.size())
+              throw std::logic_error("correlator request indexes outside the group-sum buffer");
           }
         }
         upload(d_jobs_c[c], jobs);
-        launch_resolve(d_tr.as<double2>(), d_masks.as<uint16_t>(), d_jobs_c[c].as<CorrJob>(), int(cp.reqs.size()), S,
+        launch_resolve(d_gv.as<double2>(), d_gmasks.as<uint16_t>(), d_jobs_c[c].as<CorrJob>(), int(cp.reqs.size()), S,
                        d_cval.as<double2>(), stream_);
         launch_accumulate(d_cval.as<double2>(), d_td.as<int>(), S, int(cp.reqs.size()), d_sum[c].as<double2>(),
                           d_comp[c].as<double2>(), kahan, stream_);

@@ -294,55 +294,91 @@ __global__ __launch_bounds__(64) void k_trace(const double2 *__restrict__ sX, co
 }
 
 // ---------------------------------------------------------------------------
-// resolve_request{1,2,3} (reference: Diagram.cpp:9-94, DilutedTrace.cpp:5-82):
-// one thread per (slice pair, correlator).
+// Traces fused with their mask-group sums (reference: factor_to_trace,
+// DilutedTrace.cpp:84-165, followed by the sums inside inner_product /
+// resolve_request1): one wavefront per (time, trace id, mask group); each lane
+// accumulates its share of sum_combos sum_{a,b} X[a,b] Y[b,a] over all
+// combinations of the group, then one wave reduction.
 // ---------------------------------------------------------------------------
-__global__ void k_resolve(const double2 *__restrict__ tv, const uint16_t *__restrict__ masks,
+__global__ __launch_bounds__(64) void k_trace_groups(
+    const double2 *__restrict__ sX, const long long *__restrict__ Xt, const long long *__restrict__ Xq,
+    const double2 *__restrict__ sY, const long long *__restrict__ Yt, const long long *__restrict__ Yq,
+    const GroupItem *__restrict__ items, const int2 *__restrict__ pairs, double2 *__restrict__ out, long long obase,
+    long long ostride, int D) {
+  const GroupItem g = items[blockIdx.x];
+  const int ti = blockIdx.y;
+  const long long DD = (long long)D * D;
+  const double2 *xb = sX + (Xt[ti] + Xq[g.qn]) * DD;
+  double re = 0.0, im = 0.0;
+  if (sY) {
+    const double2 *yb = sY + (Yt[ti] + Yq[g.qn]) * DD;
+    for (int c = 0; c < g.n; ++c) {
+      const int2 p = pairs[g.list_off + c];
+      const double2 *x = xb + p.x * DD;
+      const double2 *y = yb + p.y * DD;
+      for (int e = threadIdx.x; e < D * D; e += 64) {
+        const int a = e % D, b = e / D;
+        const double2 u = x[e], v = y[a * D + b];
+        re = fma(u.x, v.x, re);
+        re = fma(-u.y, v.y, re);
+        im = fma(u.x, v.y, im);
+        im = fma(u.y, v.x, im);
+      }
+    }
+  } else {
+    for (int c = 0; c < g.n; ++c) {
+      const double2 *x = xb + pairs[g.list_off + c].x * DD;
+      for (int a = threadIdx.x; a < D; a += 64) {
+        re += x[a * D + a].x;
+        im += x[a * D + a].y;
+      }
+    }
+  }
+  re = wave_sum(re);
+  im = wave_sum(im);
+  if (threadIdx.x == 0) out[obase + ti * ostride + g.dst] = make_double2(re, im);
+}
+
+// ---------------------------------------------------------------------------
+// resolve_request{1,2,3} (reference: Diagram.cpp:9-94, DilutedTrace.cpp:5-82)
+// on mask-group sums: mean, sum_g L_g sum_{h: m_g & m_h = 0} R_h, and the
+// 3-way form that rejects only masks shared by all three traces
+// (DilutedTrace.cpp:63).  One thread per (slice pair, correlator).
+// ---------------------------------------------------------------------------
+__global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__restrict__ gm,
                           const CorrJob *__restrict__ jobs, int ncorr, int nsp, double2 *__restrict__ cval) {
   const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (id >= (long long)ncorr * nsp) return;
   const int corr = int(id % ncorr), s = int(id / ncorr);
   const CorrJob J = jobs[corr];
-  const double2 *v0 = tv + J.tr[0].base + s * J.tr[0].stride;
+  const double2 *v0 = gv + J.tr[0].base + s * J.tr[0].stride;
   double2 res = make_double2(0.0, 0.0);
   if (J.ntr == 1) {
-    for (int i = 0; i < J.tr[0].n; ++i) res = cadd(res, v0[i]);
-    res.x /= J.tr[0].n;
-    res.y /= J.tr[0].n;
+    for (int g = 0; g < J.tr[0].ngroup; ++g) res = cadd(res, v0[g]);
   } else if (J.ntr == 2) {
-    const double2 *v1 = tv + J.tr[1].base + s * J.tr[1].stride;
-    const uint16_t *m0 = masks + J.tr[0].mask_off, *m1 = masks + J.tr[1].mask_off;
-    long long n = 0;
-    for (int i = 0; i < J.tr[0].n; ++i) {
+    const double2 *v1 = gv + J.tr[1].base + s * J.tr[1].stride;
+    const uint16_t *m0 = gm + J.tr[0].gmask_off, *m1 = gm + J.tr[1].gmask_off;
+    for (int g = 0; g < J.tr[0].ngroup; ++g) {
       double2 rs = make_double2(0.0, 0.0);
-      for (int j = 0; j < J.tr[1].n; ++j) {
-        if (m0[i] & m1[j]) continue;
-        rs = cadd(rs, v1[j]);
-        ++n;
-      }
-      res = cadd(res, cmul(v0[i], rs));
+      for (int h = 0; h < J.tr[1].ngroup; ++h)
+        if (!(m0[g] & m1[h])) rs = cadd(rs, v1[h]);
+      res = cadd(res, cmul(v0[g], rs));
     }
-    res.x /= double(n);
-    res.y /= double(n);
   } else {
-    const double2 *v1 = tv + J.tr[1].base + s * J.tr[1].stride;
-    const double2 *v2 = tv + J.tr[2].base + s * J.tr[2].stride;
-    const uint16_t *m0 = masks + J.tr[0].mask_off, *m1 = masks + J.tr[1].mask_off, *m2 = masks + J.tr[2].mask_off;
-    long long n = 0;
-    for (int i = 0; i < J.tr[0].n; ++i)
-      for (int j = 0; j < J.tr[1].n; ++j) {
+    const double2 *v1 = gv + J.tr[1].base + s * J.tr[1].stride;
+    const double2 *v2 = gv + J.tr[2].base + s * J.tr[2].stride;
+    const uint16_t *m0 = gm + J.tr[0].gmask_off, *m1 = gm + J.tr[1].gmask_off, *m2 = gm + J.tr[2].gmask_off;
+    for (int g = 0; g < J.tr[0].ngroup; ++g)
+      for (int h = 0; h < J.tr[1].ngroup; ++h) {
+        const uint16_t mgh = m0[g] & m1[h];
         double2 rs = make_double2(0.0, 0.0);
-        const uint16_t mij = m0[i] & m1[j];
-        for (int k = 0; k < J.tr[2].n; ++k) {
-          if (mij & m2[k]) continue;
-          rs = cadd(rs, v2[k]);
-          ++n;
-        }
-        res = cadd(res, cmul(cmul(v0[i], v1[j]), rs));
+        for (int k = 0; k < J.tr[2].ngroup; ++k)
+          if (!(mgh & m2[k])) rs = cadd(rs, v2[k]);
+        res = cadd(res, cmul(cmul(v0[g], v1[h]), rs));
       }
-    res.x /= double(n);
-    res.y /= double(n);
   }
+  res.x *= J.inv_n;
+  res.y *= J.inv_n;
   cval[(long long)s * ncorr + corr] = res;
 }
 
@@ -516,11 +552,21 @@ void launch_trace(const double2 *sX, const long long *Xt, const long long *Xq, c
   SLAPH_HIP(hipGetLastError());
 }
 
-void launch_resolve(const double2 *tv, const uint16_t *masks, const CorrJob *jobs, int ncorr, int nsp, double2 *cval,
+void launch_trace_groups(const double2 *sX, const long long *Xt, const long long *Xq, const double2 *sY,
+                         const long long *Yt, const long long *Yq, const GroupItem *items, int nitems, const int2 *pairs,
+                         double2 *out, long long obase, long long ostride, int ntime, int D, hipStream_t s) {
+  if (nitems <= 0 || ntime <= 0) return;
+  if (ntime > 65535) throw std::runtime_error("launch_trace_groups: too many slice pairs");
+  hipLaunchKernelGGL(k_trace_groups, dim3(nitems, ntime), dim3(64), 0, s, sX, Xt, Xq, sY, Yt, Yq, items, pairs, out,
+                     obase, ostride, D);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_resolve(const double2 *gv, const uint16_t *gm, const CorrJob *jobs, int ncorr, int nsp, double2 *cval,
                     hipStream_t s) {
   long long n = (long long)ncorr * nsp;
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_resolve, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, tv, masks, jobs, ncorr, nsp, cval);
+  hipLaunchKernelGGL(k_resolve, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, gv, gm, jobs, ncorr, nsp, cval);
   SLAPH_HIP(hipGetLastError());
 }
 

@@ -77,17 +77,32 @@ void launch_trace(const double2 *slabX, const long long *Xt, const long long *Xq
                   const ComboProd *combos, int ncombo, int ntime, int D, hipStream_t s);
 
 // Correlator resolution: mean / 2-way / 3-way masked inner products.
+// A trace vector seen through its used-index mask groups: the trace values of
+// all random-vector combinations with the same mask are summed first
+// (group sums are exact re-associations of the reference's sums).
 struct TraceRef {
-  long long base;    // family base + tr offset (add s*stride)
-  long long stride;  // family stride per slice pair
-  int n;             // number of random-vector combinations
-  int mask_off;      // into masks[]
+  long long base;    // offset of the group sums at slice pair 0 (add s*stride)
+  long long stride;  // group-sum stride per slice pair
+  int ngroup;        // number of mask groups
+  int gmask_off;     // into group masks[]
 };
 struct CorrJob {
   int ntr;
+  double inv_n;  // 1 / number of admitted combinations (host-computed, exact)
   TraceRef tr[3];
 };
-void launch_resolve(const double2 *tvals, const uint16_t *masks, const CorrJob *jobs, int ncorr, int nsp,
+// Trace group sums: out[obase + ti*ostride + dst] = sum over the item's
+// combinations (i, j) of tr(X[Xt[ti] + Xq[qn] + i] * Y[Yt[ti] + Yq[qn] + j])
+// (or tr(X[...]) when Y is null).
+struct GroupItem {
+  int qn;
+  int list_off, n;
+  long long dst;
+};
+void launch_trace_groups(const double2 *slabX, const long long *Xt, const long long *Xq, const double2 *slabY,
+                         const long long *Yt, const long long *Yq, const GroupItem *items, int nitems, const int2 *pairs,
+                         double2 *out, long long obase, long long ostride, int ntime, int D, hipStream_t s);
+void launch_resolve(const double2 *gvals, const uint16_t *gmasks, const CorrJob *jobs, int ncorr, int nsp,
                     double2 *cval, hipStream_t s);
 void launch_accumulate(const double2 *cval, const int *tdelta, int nsp, int ncorr, double2 *sum, double2 *comp,
                        int kahan, hipStream_t s);
