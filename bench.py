@@ -191,7 +191,7 @@ def main():
                    "precision": "fp64"},
         "stages_ms": {"vdaggerv": stage["vdaggerv"], "factors": stage["factors"],
                       "contract_incl_factors": stage["contract"]},
-        "roofline": {"bound": "mfma", "kernel": "k_ahb (VdaggerV)", "achieved": achieved,
+        "roofline": {"bound": "mfma", "kernel": "k_vdv (VdaggerV)", "achieved": achieved,
                      "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA,
                      "traffic": None, "algorithmic_flops_per_launch": flops, "avg_launch_ms": kern_ms},
     }

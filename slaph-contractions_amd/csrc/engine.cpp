@@ -217,25 +217,37 @@ void Engine::synthetic_inputs(unsigned long long seed, bool orthonormalise) {
     launch_fill_z2z2(rho_.as<double2>() + (size_t)r * Lt_ * N_ * 4, (long long)Lt_ * N_ * 4, seed_of(seed, 2, r),
                      stream_);
   }
-  DevBuf tmp, G, R, jobs;
+  for (int t = 0; t < Lt_; ++t)
+    launch_fill_gauss(V_.as<double2>() + (size_t)t * K_ * N_, K_ * N_, seed_of(seed, 3, t), stream_);
   if (orthonormalise) {
+    // Two passes of Cholesky-QR: Gram matrices of all timeslices in one
+    // batched launch, Cholesky on the host, V <- V R^{-1} on the device.
+    DevBuf tmp, G, R, dj;
     tmp.alloc(sizeof(double2) * K_ * N_);
-    G.alloc(sizeof(double2) * N_ * N_);
+    G.alloc(sizeof(double2) * (size_t)Lt_ * N_ * N_);
     R.alloc(sizeof(double2) * N_ * N_);
-  }
-  std::vector<cplx> hG(N_ * N_);
-  for (int t = 0; t < Lt_; ++t) {
-    double2 *Vt = V_.as<double2>() + (size_t)t * K_ * N_;
-    launch_fill_gauss(Vt, K_ * N_, seed_of(seed, 3, t), stream_);
-    if (!orthonormalise) continue;
+    std::vector<VdvJob> jobs(Lt_);
+    for (int t = 0; t < Lt_; ++t) {
+      jobs[t] = VdvJob{};
+      jobs[t].V = V_.as<double2>() + (size_t)t * K_ * N_;
+      jobs[t].np = 1;
+      jobs[t].C[0] = G.as<double2>() + (size_t)t * N_ * N_;
+    }
+    upload(dj, jobs);
+    std::vector<cplx> hG((size_t)Lt_ * N_ * N_);
     for (int pass = 0; pass < 2; ++pass) {
-      ahb_raw(Vt, Vt, G.as<double2>(), nullptr, N_, N_, (int)K_, K_, K_, N_, 1, 0, 0, 0);
+      launch_vdv(dj.as<VdvJob>(), Lt_, N_, K_, stream_);
       SLAPH_HIP(hipMemcpyAsync(hG.data(), G.as<void>(), sizeof(cplx) * hG.size(), hipMemcpyDeviceToHost, stream_));
       SLAPH_HIP(hipStreamSynchronize(stream_));
-      auto Rinv = chol_inverse_upper(hG, N_);
-      SLAPH_HIP(hipMemcpyAsync(R.as<void>(), Rinv.data(), sizeof(cplx) * Rinv.size(), hipMemcpyHostToDevice, stream_));
-      launch_gemm_nn_small(Vt, R.as<double2>(), tmp.as<double2>(), K_, N_, stream_);
-      SLAPH_HIP(hipMemcpyAsync(Vt, tmp.as<void>(), sizeof(double2) * K_ * N_, hipMemcpyDeviceToDevice, stream_));
+      for (int t = 0; t < Lt_; ++t) {
+        std::vector<cplx> Gt(hG.begin() + (size_t)t * N_ * N_, hG.begin() + (size_t)(t + 1) * N_ * N_);
+        auto Rinv = chol_inverse_upper(Gt, N_);
+        double2 *Vt = V_.as<double2>() + (size_t)t * K_ * N_;
+        SLAPH_HIP(hipMemcpy(R.as<void>(), Rinv.data(), sizeof(cplx) * Rinv.size(), hipMemcpyHostToDevice));
+        launch_gemm_nn_small(Vt, R.as<double2>(), tmp.as<double2>(), K_, N_, stream_);
+        SLAPH_HIP(hipMemcpyAsync(Vt, tmp.as<void>(), sizeof(double2) * K_ * N_, hipMemcpyDeviceToDevice, stream_));
+        SLAPH_HIP(hipStreamSynchronize(stream_));
+      }
     }
   }
   SLAPH_HIP(hipStreamSynchronize(stream_));
@@ -297,23 +309,37 @@ void Engine::create_operators() {
               std::exp(cplx(0, -1) * (ipx * x + ipy * y + ipz * z));
   }
   SLAPH_HIP(hipMemcpyAsync(phase_.as<void>(), ph.data(), sizeof(cplx) * ph.size(), hipMemcpyHostToDevice, stream_));
-  std::vector<AhBJob> jobs;
+  // Non-trivial momenta in balanced groups of at most kVdvGroup: each
+  // workgroup of the kernel serves a whole group from one V(t) panel load.
+  std::vector<int> ids;
   for (auto const &op : gd_.vdaggerv_lookup) {
-    double2 *out = vdv_.as<double2>() + (size_t)op.id * Lt_ * N_ * N_;
-    if (op.id == gd_.index_of_unity) {
-      launch_identity(out, N_, Lt_, stream_);
-      continue;
-    }
-    for (int t = 0; t < Lt_; ++t) {
-      const double2 *Vt = V_.as<double2>() + (size_t)t * K_ * N_;
-      jobs.push_back({Vt, Vt, out + (size_t)t * N_ * N_, phase_.as<double2>() + (size_t)op.id * Vs, nullptr, K_, K_, N_});
+    if (op.id == gd_.index_of_unity)
+      launch_identity(vdv_.as<double2>() + (size_t)op.id * Lt_ * N_ * N_, N_, Lt_, stream_);
+    else
+      ids.push_back(op.id);
+  }
+  int ngroups = int((ids.size() + kVdvGroup - 1) / kVdvGroup);
+  std::vector<VdvJob> jobs;
+  for (int t = 0; t < Lt_; ++t) {
+    size_t at = 0;
+    for (int g = 0; g < ngroups; ++g) {
+      size_t len = ids.size() / ngroups + (size_t(g) < ids.size() % ngroups ? 1 : 0);
+      VdvJob j{};
+      j.V = V_.as<double2>() + (size_t)t * K_ * N_;
+      j.np = int(len);
+      for (size_t k = 0; k < len; ++k, ++at) {
+        int id = ids[at];
+        j.phase[k] = phase_.as<double2>() + (size_t)id * Vs;
+        j.C[k] = vdv_.as<double2>() + ((size_t)id * Lt_ + t) * N_ * N_;
+      }
+      jobs.push_back(j);
     }
   }
   DevBuf dj;
   upload(dj, jobs);
   EventTimer tm(stream_);
   for (size_t i = 0; i < jobs.size(); i += 65535)
-    launch_ahb(dj.as<AhBJob>() + i, int(std::min<size_t>(65535, jobs.size() - i)), N_, N_, (int)K_, stream_);
+    launch_vdv(dj.as<VdvJob>() + i, int(std::min<size_t>(65535, jobs.size() - i)), N_, K_, stream_);
   times_.vdaggerv_kernel_ms = tm.stop();
   times_.vdaggerv_launches = int((jobs.size() + 65534) / 65535);
   times_.vdaggerv_ms = times_.vdaggerv_kernel_ms;

@@ -119,6 +119,102 @@ __global__ __launch_bounds__(256) void k_ahb(const AhBJob *__restrict__ jobs, in
 }
 
 // ---------------------------------------------------------------------------
+// VdaggerV for a group of momenta (reference: OperatorFactory.cpp:294-300).
+// One workgroup = one 32 x 32 complex output tile of every momentum of the
+// group; 4 waves, each a 16 x 16 block.  V is streamed in K-chunks of 32 rows
+// through double-buffered LDS (the A and B panels are columns of the same
+// V(t)); the momentum phases of the chunk's sites are staged next to them and
+// applied to the B fragment in registers, so every V element loaded feeds
+// 4 * np MFMAs.
+// ---------------------------------------------------------------------------
+constexpr int VDV_T = 32, VDV_KC = 32, VDV_SITES = VDV_KC / 3 + 2;
+
+__global__ __launch_bounds__(256) void k_vdv(const VdvJob *__restrict__ jobs, int N, long long K, int tiles_n) {
+  const VdvJob &J = jobs[blockIdx.y];
+  const int np = J.np;
+  const int m0 = (blockIdx.x / tiles_n) * VDV_T, n0 = (blockIdx.x % tiles_n) * VDV_T;
+  __shared__ double2 As[2][VDV_T][VDV_KC + 1];
+  __shared__ double2 Bs[2][VDV_T][VDV_KC + 1];
+  __shared__ double2 Ph[2][kVdvGroup][VDV_SITES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const double2 *__restrict__ V = J.V;
+  d4 cr[kVdvGroup], ci[kVdvGroup];
+#pragma unroll
+  for (int p = 0; p < kVdvGroup; ++p) {
+    cr[p] = d4{0.0, 0.0, 0.0, 0.0};
+    ci[p] = d4{0.0, 0.0, 0.0, 0.0};
+  }
+  double2 ra[4], rb[4], rp = make_double2(1.0, 0.0);
+  const int nk = int((K + VDV_KC - 1) / VDV_KC);
+  auto load = [&](int kc) {
+    const long long k0 = (long long)kc * VDV_KC;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int idx = tid + r * 256;
+      const int kk = idx % VDV_KC, c = idx / VDV_KC;
+      const long long k = k0 + kk;
+      ra[r] = (k < K && m0 + c < N) ? V[(long long)(m0 + c) * K + k] : make_double2(0.0, 0.0);
+      rb[r] = (k < K && n0 + c < N) ? V[(long long)(n0 + c) * K + k] : make_double2(0.0, 0.0);
+    }
+    if (tid < kVdvGroup * VDV_SITES) {
+      const int p = tid / VDV_SITES, j = tid % VDV_SITES;
+      const long long site = k0 / 3 + j;
+      rp = make_double2(1.0, 0.0);
+      if (p < np && J.phase[p] && site * 3 < K) rp = J.phase[p][site];
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int idx = tid + r * 256;
+      As[buf][idx / VDV_KC][idx % VDV_KC] = ra[r];
+      Bs[buf][idx / VDV_KC][idx % VDV_KC] = rb[r];
+    }
+    if (tid < kVdvGroup * VDV_SITES) Ph[buf][tid / VDV_SITES][tid % VDV_SITES] = rp;
+  };
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kc = 0; kc < nk; ++kc) {
+    const int buf = kc & 1;
+    if (kc + 1 < nk) load(kc + 1);
+    const long long k0 = (long long)kc * VDV_KC, s0 = k0 / 3;
+#pragma unroll
+    for (int kq = 0; kq < VDV_KC; kq += 4) {
+      const int kr = kq + (lane >> 4);
+      const double2 a = As[buf][wm * 16 + (lane & 15)][kr];
+      const double2 v = Bs[buf][wn * 16 + (lane & 15)][kr];
+      const int sl = int((k0 + kr) / 3 - s0);
+#pragma unroll
+      for (int p = 0; p < kVdvGroup; ++p) {
+        if (p < np) {
+          const double2 b = cmul(Ph[buf][p][sl], v);
+          // conj(a) b = (ax bx + ay by) + i (ax by - ay bx)
+          cr[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, cr[p], 0, 0, 0);
+          cr[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, cr[p], 0, 0, 0);
+          ci[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, ci[p], 0, 0, 0);
+          ci[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.x, ci[p], 0, 0, 0);
+        }
+      }
+    }
+    if (kc + 1 < nk) store(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int p = 0; p < kVdvGroup; ++p) {
+    if (p >= np) continue;
+    double2 *C = J.C[p];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = m0 + wm * 16 + (lane >> 4) + 4 * i;
+      const int col = n0 + wn * 16 + (lane & 15);
+      if (row < N && col < N) C[(long long)col * N + row] = make_double2(cr[p][i], ci[p][i]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Q0[(e1 + dilE g(d)), (e2 + dilE d)] =
 //   gval(d) sum_{v=e1 mod dilE} conj rho1(v, g(d)) sum_{w=e2 mod dilE} VdV[v,w] rho2(w,d)
 // (reference: DilutedFactorFactory<Q0>::build, DilutedFactorFactory.cpp:33-101)
@@ -491,6 +587,14 @@ void launch_ahb(const AhBJob *d_jobs, int njobs, int M, int N, int K, hipStream_
   if (njobs > 65535) throw std::runtime_error("launch_ahb: too many jobs in one launch");
   dim3 grid((M + AHB_TM - 1) / AHB_TM, (N + AHB_TN - 1) / AHB_TN, njobs);
   hipLaunchKernelGGL(k_ahb, grid, dim3(256), 0, s, d_jobs, M, N, K);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, hipStream_t s) {
+  if (njobs <= 0) return;
+  if (njobs > 65535) throw std::runtime_error("launch_vdv: too many jobs in one launch");
+  const int tiles = (N + VDV_T - 1) / VDV_T;
+  hipLaunchKernelGGL(k_vdv, dim3(tiles * tiles, njobs), dim3(256), 0, s, d_jobs, N, K, tiles);
   SLAPH_HIP(hipGetLastError());
 }
 

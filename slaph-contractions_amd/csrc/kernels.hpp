@@ -30,6 +30,18 @@ struct AhBJob {
 };
 void launch_ahb(const AhBJob *d_jobs, int njobs, int M, int N, int K, hipStream_t s);
 
+// VdaggerV of one timeslice for a group of up to kVdvGroup momenta:
+// C[p] = V^H diag(phase[p][k/3]) V, V: K x N column-major (ld = K), C: N x N.
+// phase[p] may be null (unit phase, e.g. for a Gram matrix).
+constexpr int kVdvGroup = 8;
+struct VdvJob {
+  const double2 *V;
+  const double2 *phase[kVdvGroup];
+  double2 *C[kVdvGroup];
+  int np;
+};
+void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, hipStream_t s);
+
 // Q0 of one (t, operator, random pair): D x D.
 struct Q0Job {
   const double2 *vdv;   // N x N (daggered if needed)
