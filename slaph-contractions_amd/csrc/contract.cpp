@@ -239,7 +239,7 @@ struct Family {
   std::vector<int2> gpairs;  // (left, right) combination indices, grouped
   long long gstride = 0;     // group sums per slice pair
   long long gbase = 0;       // offset of this family in the chunk's group-sum buffer
-  DevBuf d_gitems, d_gpairs;
+  DevBuf d_gitems, d_gpairs, d_tile;
 };
 
 struct CorrPlan {
@@ -248,6 +248,38 @@ struct CorrPlan {
   std::vector<std::string> datasets;
   std::vector<double> inv_n;  // 1 / admitted combinations per correlator
 };
+
+// Is the chunk a rectangular grid in the operands' time bases?  True when the
+// slice pairs map one-to-one onto (distinct left base) x (distinct right base)
+// with at most kGridT of each; fills the tile tables then.
+bool grid_tile(std::vector<long long> const &Xt, std::vector<long long> const &Yt, GridTile &tile) {
+  std::vector<long long> L, R;
+  for (auto x : Xt)
+    if (std::find(L.begin(), L.end(), x) == L.end()) {
+      if (L.size() == size_t(kGridT)) return false;
+      L.push_back(x);
+    }
+  for (auto y : Yt)
+    if (std::find(R.begin(), R.end(), y) == R.end()) {
+      if (R.size() == size_t(kGridT)) return false;
+      R.push_back(y);
+    }
+  if (L.size() * R.size() != Xt.size()) return false;
+  tile = GridTile{};
+  for (auto &row : tile.sidx)
+    for (auto &v : row) v = -1;
+  for (size_t s = 0; s < Xt.size(); ++s) {
+    int a = int(std::find(L.begin(), L.end(), Xt[s]) - L.begin());
+    int b = int(std::find(R.begin(), R.end(), Yt[s]) - R.begin());
+    if (tile.sidx[a][b] >= 0) return false;
+    tile.sidx[a][b] = int(s);
+  }
+  tile.nL = int(L.size());
+  tile.nR = int(R.size());
+  for (size_t a = 0; a < L.size(); ++a) tile.Lt[a] = L[a];
+  for (size_t b = 0; b < R.size(); ++b) tile.Rt[b] = R[b];
+  return true;
+}
 
 // Every (factor type, time key) a leaf of this family reads at slice pair
 // (t_so, t_si); the slab index is returned through `base`.
@@ -577,8 +609,15 @@ void Engine::contract(int single_block_pair, int kahan) {
                    bp_slices.size(), sl.size(), smax);
       std::fflush(stderr);
     }
-    for (size_t c0 = 0; c0 < sl.size(); c0 += smax) {
-      const int S = int(std::min<size_t>(smax, sl.size() - c0));
+    // Chunks are whole rows (one source time, all sink times of the block)
+    // of one direction of the block pair, so that time-separable traces see
+    // a rectangular tile of slice pairs.
+    const size_t pass_len = (size_t)dilT * dilT;
+    const size_t chunk_rows = std::max<size_t>(1, std::min<size_t>(dilT, size_t(smax) / dilT));
+    for (size_t c0 = 0, S_ = 0; c0 < sl.size(); c0 += S_) {
+      S_ = std::min(chunk_rows * dilT, (c0 / pass_len + 1) * pass_len - c0);
+      S_ = std::min(S_, sl.size() - c0);
+      const int S = int(S_);
       // time planning
       long long prod_total = 0;
       for (auto &f : fams) {
@@ -701,6 +740,7 @@ void Engine::contract(int single_block_pair, int kahan) {
             return d_prod.as<double2>();
           };
           int ntime = root ? S : int(np.time_index.size());
+          GridTile tile;
           // Conservative bounds check of every operand and output index the
           // launch can form (largest time base + largest qn/combo offset).
           auto slab_count = [&](int child) -> long long {
@@ -724,6 +764,11 @@ void Engine::contract(int single_block_pair, int kahan) {
             launch_prod(src(ns.left), np.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
                         np.d_Yt.as<long long>(), np.d_Yq.as<long long>(), d_prod.as<double2>(), np.base, np.stride,
                         np.d_combos.as<ComboProd>(), int(np.combos.size()), ntime, D_, stream_);
+          } else if (ns.kind == kTrace2 && grid_tile(np.h_Xt, np.h_Yt, tile)) {
+            upload(f.d_tile, std::vector<GridTile>{tile});
+            launch_trace_grid(src(ns.left), np.d_Xq.as<long long>(), src(ns.right), np.d_Yq.as<long long>(),
+                              f.d_gitems.as<GroupItem>(), int(f.gitems.size()), f.d_gpairs.as<int2>(),
+                              f.d_tile.as<GridTile>(), 1, d_gv.as<double2>(), f.gbase, f.gstride, D_, stream_);
           } else {
             launch_trace_groups(src(ns.left), np.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
                                 np.d_Yt.as<long long>(), np.d_Yq.as<long long>(), f.d_gitems.as<GroupItem>(),

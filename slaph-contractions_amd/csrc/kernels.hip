@@ -436,6 +436,75 @@ __global__ __launch_bounds__(64) void k_trace_groups(
 }
 
 // ---------------------------------------------------------------------------
+// Time-separable traces as small GEMMs over time on FP64 MFMA: for one
+// (trace id, mask group) and one tile of nL x nR slice pairs,
+//   out[a][b] = sum_combos sum_{x,y} L_a[x,y] R_b[y,x],
+// i.e. a 16 x 16 x (combos * D^2) complex GEMM whose K index runs over the
+// matrix elements.  Slabs of 128 elements of the 16 left and 16 right
+// matrices are staged in LDS (the right ones transposed on the way in), so
+// every loaded element feeds 16 outputs.  4 waves split the K steps and are
+// reduced through LDS at the end.
+// ---------------------------------------------------------------------------
+constexpr int TG_SE = 128;
+
+__global__ __launch_bounds__(256) void k_trace_grid(const double2 *__restrict__ sX, const long long *__restrict__ Xq,
+                                                    const double2 *__restrict__ sY, const long long *__restrict__ Yq,
+                                                    const GroupItem *__restrict__ items, const int2 *__restrict__ pairs,
+                                                    const GridTile *__restrict__ tiles, double2 *__restrict__ out,
+                                                    long long obase, long long ostride, int D) {
+  const GroupItem g = items[blockIdx.x];
+  const GridTile &T = tiles[blockIdx.y];
+  __shared__ double2 Ls[kGridT][TG_SE + 1];
+  __shared__ double2 Rs[kGridT][TG_SE + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int DD = D * D;
+  const int SE = DD < TG_SE ? DD : TG_SE;
+  const int nslab = DD / SE, rows = SE / D;
+  const long long xq = Xq[g.qn], yq = Yq[g.qn];
+  const int nL = T.nL, nR = T.nR;
+  d4 cr = d4{0.0, 0.0, 0.0, 0.0}, ci = d4{0.0, 0.0, 0.0, 0.0};
+  for (int c = 0; c < g.n; ++c) {
+    const int2 p = pairs[g.list_off + c];
+    for (int h = 0; h < nslab; ++h) {
+      for (int idx = tid; idx < kGridT * SE; idx += 256) {
+        const int a = idx / SE, e = idx % SE;
+        const int x = e % D, yy = e / D;
+        Ls[a][e] = a < nL ? sX[(T.Lt[a] + xq + p.x) * DD + (long long)h * SE + e] : make_double2(0.0, 0.0);
+        Rs[a][e] = a < nR ? sY[(T.Rt[a] + yq + p.y) * DD + (long long)x * D + h * rows + yy] : make_double2(0.0, 0.0);
+      }
+      __syncthreads();
+      for (int ks = wave; ks < SE / 4; ks += 4) {
+        const int e = ks * 4 + (lane >> 4);
+        const double2 u = Ls[lane & 15][e], v = Rs[lane & 15][e];
+        // (ux + i uy)(vx + i vy) = (ux vx - uy vy) + i (ux vy + uy vx)
+        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(u.x, v.x, cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-u.y, v.y, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(u.x, v.y, ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(u.y, v.x, ci, 0, 0, 0);
+      }
+      __syncthreads();
+    }
+  }
+  // reduce the 4 waves' partial 16 x 16 tiles (reusing the left staging area)
+  double2 *part = &Ls[0][0];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int a = (lane >> 4) + 4 * i, b = lane & 15;
+    part[wave * 256 + a * 16 + b] = make_double2(cr[i], ci[i]);
+  }
+  __syncthreads();
+  {
+    const int a = tid / 16, b = tid % 16;
+    double2 s = part[tid];
+    s = cadd(s, part[256 + tid]);
+    s = cadd(s, part[512 + tid]);
+    s = cadd(s, part[768 + tid]);
+    const int sp = (a < nL && b < nR) ? T.sidx[a][b] : -1;
+    if (sp >= 0) out[obase + sp * ostride + g.dst] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // resolve_request{1,2,3} (reference: Diagram.cpp:9-94, DilutedTrace.cpp:5-82)
 // on mask-group sums: mean, sum_g L_g sum_{h: m_g & m_h = 0} R_h, and the
 // 3-way form that rejects only masks shared by all three traces
@@ -662,6 +731,18 @@ void launch_trace_groups(const double2 *sX, const long long *Xt, const long long
   if (nitems <= 0 || ntime <= 0) return;
   if (ntime > 65535) throw std::runtime_error("launch_trace_groups: too many slice pairs");
   hipLaunchKernelGGL(k_trace_groups, dim3(nitems, ntime), dim3(64), 0, s, sX, Xt, Xq, sY, Yt, Yq, items, pairs, out,
+                     obase, ostride, D);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_trace_grid(const double2 *sX, const long long *Xq, const double2 *sY, const long long *Yq,
+                       const GroupItem *items, int nitems, const int2 *pairs, const GridTile *tiles, int ntiles,
+                       double2 *out, long long obase, long long ostride, int D, hipStream_t s) {
+  if (nitems <= 0 || ntiles <= 0) return;
+  if (D * D % (D * D < TG_SE ? D * D : TG_SE) != 0 || D > 32)
+    throw std::runtime_error("launch_trace_grid: unsupported factor dimension");
+  if (ntiles > 65535) throw std::runtime_error("launch_trace_grid: too many tiles");
+  hipLaunchKernelGGL(k_trace_grid, dim3(nitems, ntiles), dim3(256), 0, s, sX, Xq, sY, Yq, items, pairs, tiles, out,
                      obase, ostride, D);
   SLAPH_HIP(hipGetLastError());
 }

@@ -114,6 +114,20 @@ struct GroupItem {
 void launch_trace_groups(const double2 *slabX, const long long *Xt, const long long *Xq, const double2 *slabY,
                          const long long *Yt, const long long *Yq, const GroupItem *items, int nitems, const int2 *pairs,
                          double2 *out, long long obase, long long ostride, int ntime, int D, hipStream_t s);
+// Time-separable traces: within one tile of slice pairs the left operand
+// depends on one time index a < nL and the right one on b < nR, and every
+// (a, b) is a slice pair sidx[a][b] (or -1).  Output per (item, tile):
+// out[obase + sidx[a][b]*ostride + dst] = sum_combos tr(L_a,i R_b,j).
+constexpr int kGridT = 16;
+struct GridTile {
+  long long Lt[kGridT];
+  long long Rt[kGridT];
+  int sidx[kGridT][kGridT];
+  int nL, nR;
+};
+void launch_trace_grid(const double2 *slabX, const long long *Xq, const double2 *slabY, const long long *Yq,
+                       const GroupItem *items, int nitems, const int2 *pairs, const GridTile *tiles, int ntiles,
+                       double2 *out, long long obase, long long ostride, int D, hipStream_t s);
 void launch_resolve(const double2 *gvals, const uint16_t *gmasks, const CorrJob *jobs, int ncorr, int nsp,
                     double2 *cval, hipStream_t s);
 void launch_accumulate(const double2 *cval, const int *tdelta, int nsp, int ncorr, double2 *sum, double2 *comp,
