@@ -135,7 +135,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="pipi32", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--chunk-gib", type=float, default=24.0)
+    ap.add_argument("--chunk-gib", type=float, default=0.0, help="chunk memory budget (0: automatic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "

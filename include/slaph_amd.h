@@ -115,7 +115,9 @@ int slc_fetch_correlators(const slc_ctx *ctx, const char *diagram, double *out);
 int slc_stage_time(const slc_ctx *ctx, const char *what, double *ms);
 
 /* Upper bound on the device memory the contraction may use for per-chunk
- * products and traces (bytes); sets the number of slice pairs per launch. */
+ * products and traces (bytes); sets the number of slice pairs per launch.
+ * 0 (the default) = 80 % of the device memory free when contract starts;
+ * negative values are rejected. */
 int slc_set_chunk_budget(slc_ctx *ctx, long long bytes);
 
 /* C = A^H diag(phase[k/3]) B on the device (batch of `batch` products with

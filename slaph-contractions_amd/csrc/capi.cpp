@@ -195,7 +195,7 @@ int slc_stage_time(const slc_ctx *cctx, const char *what, double *ms) {
 
 int slc_set_chunk_budget(slc_ctx *ctx, long long bytes) {
   return guarded(ctx, [&] {
-    if (bytes <= 0) throw std::out_of_range("budget must be positive");
+    if (bytes < 0) throw std::out_of_range("budget must not be negative");
     ctx->eng->set_chunk_budget(size_t(bytes));
   });
 }

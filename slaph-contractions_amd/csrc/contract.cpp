@@ -16,6 +16,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
 #include <map>
 #include <set>
 #include <unordered_map>
@@ -213,6 +214,7 @@ struct NodePlan {
   std::vector<int> trl_qn;  // trl -> qn (in the owner's qn space)
   int owner = -1;           // node whose product table this node reads
   DevBuf d_combos, d_Xq, d_Yq;
+  std::vector<long long> h_Xq, h_Yq;
   // time part (per chunk)
   std::map<std::vector<long long>, int> time_index;
   std::vector<int> time_left, time_right;  // child time index (product children)
@@ -240,6 +242,15 @@ struct Family {
   long long gstride = 0;     // group sums per slice pair
   long long gbase = 0;       // offset of this family in the chunk's group-sum buffer
   DevBuf d_gitems, d_gpairs, d_tile;
+  // trl -> earlier trl of equal value (tr(X_a Y_b) = tr(X_b Y_a) when both
+  // operands read the same table at the same time), or -1
+  std::vector<int> mirror_of;
+  // left-stationary trace plan (time independent)
+  std::vector<LsBlock> ls_blocks;
+  std::vector<LsChunk> ls_chunks;
+  std::vector<LsTask> ls_tasks;
+  std::vector<long long> ls_slot_dst;
+  DevBuf d_lsb, d_lsc, d_lst, d_lsd;
 };
 
 struct CorrPlan {
@@ -279,6 +290,98 @@ bool grid_tile(std::vector<long long> const &Xt, std::vector<long long> const &Y
   for (size_t a = 0; a < L.size(); ++a) tile.Lt[a] = L[a];
   for (size_t b = 0; b < R.size(); ++b) tile.Rt[b] = R[b];
   return true;
+}
+
+// Left-stationary plan of a family's two-operand trace (time independent):
+// group sums (items) are blocked by their left operator entry, at most
+// kLsSlots per block; a block's left combinations are staged in chunks of
+// trace_lstat_chunk(D); inside a chunk each item's (i, j) pairs are sorted by
+// the right combination j, consecutive pairs with the same j share one task,
+// and the chunk's task list is cut into kLsWaves ranges of about equal length.
+// Each wave flushes its per-lane sum at the end of every run and of its range
+// into its own slot copy, so the order of every sum is fixed by the plan.
+void plan_lstat(Family &f, NodePlan const &root, int D) {
+  f.ls_blocks.clear();
+  f.ls_chunks.clear();
+  f.ls_tasks.clear();
+  f.ls_slot_dst.clear();
+  const int CL = trace_lstat_chunk(D);
+  if (f.shape.nodes.back().kind != kTrace2 || CL <= 0 || f.gitems.empty()) return;
+  std::map<long long, std::vector<int>> by_left;
+  for (size_t k = 0; k < f.gitems.size(); ++k) by_left[root.h_Xq.at(f.gitems[k].qn)].push_back(int(k));
+  for (auto const &kv : by_left)
+    for (size_t s0 = 0; s0 < kv.second.size(); s0 += kLsSlots) {
+      const size_t s1 = std::min(kv.second.size(), s0 + kLsSlots);
+      LsBlock b;
+      b.xq = kv.first;
+      b.chunk_off = int(f.ls_chunks.size());
+      b.slot_off = int(f.ls_slot_dst.size());
+      b.nslot = int(s1 - s0);
+      int imin = INT32_MAX, imax = -1;
+      for (size_t s = s0; s < s1; ++s) {
+        GroupItem const &it = f.gitems[kv.second[s]];
+        f.ls_slot_dst.push_back(it.dst);
+        for (int c = 0; c < it.n; ++c) {
+          imin = std::min(imin, f.gpairs[it.list_off + c].x);
+          imax = std::max(imax, f.gpairs[it.list_off + c].x);
+        }
+      }
+      for (int i0 = imin; i0 <= imax; i0 += CL) {
+        const int n = std::min(CL, imax + 1 - i0);
+        std::vector<LsTask> tasks;
+        for (size_t s = s0; s < s1; ++s) {
+          GroupItem const &it = f.gitems[kv.second[s]];
+          std::vector<std::pair<int, int>> pr;  // (j, i - i0)
+          for (int c = 0; c < it.n; ++c) {
+            int2 p = f.gpairs[it.list_off + c];
+            if (p.x >= i0 && p.x < i0 + n) pr.push_back({p.y, p.x - i0});
+          }
+          if (pr.empty()) continue;
+          std::sort(pr.begin(), pr.end());
+          const long long yq = root.h_Yq.at(it.qn);
+          for (size_t a = 0; a < pr.size();) {
+            LsTask t;
+            long long yrel = yq + pr[a].first;
+            if (yrel > INT32_MAX) throw std::runtime_error("trace plan: right operand index overflows");
+            t.yrel = int(yrel);
+            t.ia = short(pr[a].second);
+            t.ib = -1;
+            if (a + 1 < pr.size() && pr[a + 1].first == pr[a].first) {
+              t.ib = short(pr[a + 1].second);
+              a += 2;
+            } else {
+              a += 1;
+            }
+            t.slot = a == pr.size() ? int(s - s0) : -1;
+            tasks.push_back(t);
+          }
+        }
+        if (tasks.empty()) continue;
+        LsChunk ch;
+        ch.i0 = i0;
+        ch.n = n;
+        const int base = int(f.ls_tasks.size()), nt = int(tasks.size());
+        for (int w = 0; w <= kLsWaves; ++w) ch.wtask[w] = base + int((long long)nt * w / kLsWaves);
+        // a wave's range ends with a flush into the slot of its last task
+        for (int w = 0; w < kLsWaves; ++w) {
+          int last = ch.wtask[w + 1] - 1 - base;
+          if (last < ch.wtask[w] - base) continue;
+          if (tasks[last].slot < 0) {
+            int k = last;
+            while (tasks[k].slot < 0) ++k;  // the run's own slot, found at its end
+            tasks[last].slot = tasks[k].slot;
+          }
+        }
+        f.ls_tasks.insert(f.ls_tasks.end(), tasks.begin(), tasks.end());
+        f.ls_chunks.push_back(ch);
+      }
+      b.nchunk = int(f.ls_chunks.size()) - b.chunk_off;
+      f.ls_blocks.push_back(b);
+    }
+  upload(f.d_lsb, f.ls_blocks);
+  upload(f.d_lsc, f.ls_chunks);
+  upload(f.d_lst, f.ls_tasks);
+  upload(f.d_lsd, f.ls_slot_dst);
 }
 
 // Every (factor type, time key) a leaf of this family reads at slice pair
@@ -459,6 +562,8 @@ void Engine::contract(int single_block_pair, int kahan) {
       upload(np.d_Xq, Xq);
       upload(np.d_Yq, Yq);
       upload(np.d_combos, np.combos);
+      np.h_Xq = std::move(Xq);
+      np.h_Yq = std::move(Yq);
     }
   }
 
@@ -485,10 +590,90 @@ void Engine::contract(int single_block_pair, int kahan) {
                   std::vector<std::vector<int>>(need2.begin(), need2.end()));
     times_.factors_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
+  auto slab_of = [&](int q) -> const double2 * {
+    return q == 0 ? q0_.as<double2>() : (q == 1 ? q1_.as<double2>() : q2_.as<double2>());
+  };
+  auto leaf_base = [&](LeafSpec const &ls, std::vector<int> const &tt) -> long long {
+    if (ls.q == 0) return (long long)tt[ls.a] * q_total_[0];
+    if (ls.q == 1) return ((long long)tt[ls.a] * nb_ + B(tt[ls.b])) * q_total_[1];
+    int slot = q2slot_.at(((size_t)B(tt[ls.a]) * Lt_ + tt[ls.b]) * nb_ + B(tt[ls.c]));
+    if (slot < 0) throw std::logic_error("Q2 key was not built");
+    return (long long)slot * q_total_[2];
+  };
+  // ---- mirror traces: when the two operands of a trace read the same table
+  // at the same time for every slice pair, tr(X_a Y_b) = tr(X_b Y_a) with the
+  // combinations swapped and the same used-index masks, so (b, a) reuses the
+  // sums of (a, b) ----
+  for (auto &f : fams) {
+    size_t ntrl = f.tr_ids.size();
+    f.mirror_of.assign(ntrl, -1);
+    auto const &rs = f.shape.nodes.back();
+    if (rs.kind != kTrace2) continue;
+    auto const &ln = f.shape.nodes[rs.left], &rn = f.shape.nodes[rs.right];
+    bool same_table = ln.kind == kLeaf && rn.kind == kLeaf
+                          ? f.shape.leaves[ln.leaf].q == f.shape.leaves[rn.leaf].q
+                          : (ln.kind == kProd && rn.kind == kProd && f.nodes[rs.left].owner == f.nodes[rs.right].owner);
+    if (!same_table) continue;
+    std::function<void(int, std::vector<int> &)> leaves_of = [&](int n, std::vector<int> &out) {
+      auto const &ns = f.shape.nodes[n];
+      if (ns.kind == kLeaf) {
+        out.push_back(ns.leaf);
+        return;
+      }
+      leaves_of(ns.left, out);
+      if (ns.right >= 0) leaves_of(ns.right, out);
+    };
+    std::vector<int> lL, lR;
+    leaves_of(rs.left, lL);
+    leaves_of(rs.right, lR);
+    bool same_time = lL.size() == lR.size();
+    for (auto const &sl : bp_slices)
+      for (auto const &sp : sl) {
+        if (!same_time) break;
+        auto tt = trace_times(f, sp.source, sp.sink);
+        for (size_t k = 0; k < lL.size() && same_time; ++k)
+          same_time = leaf_base(f.shape.leaves[lL[k]], tt) == leaf_base(f.shape.leaves[lR[k]], tt);
+      }
+    if (!same_time) continue;
+    NodePlan const &root = f.nodes.back();
+    std::map<std::pair<int, int>, int> first;
+    for (size_t trl = 0; trl < ntrl; ++trl) {
+      int q = root.trl_qn[trl];
+      first.emplace(std::make_pair(root.qn_left[q], root.qn_right[q]), int(trl));
+    }
+    for (size_t trl = 0; trl < ntrl; ++trl) {
+      int q = root.trl_qn[trl];
+      auto it = first.find(std::make_pair(root.qn_right[q], root.qn_left[q]));
+      if (it == first.end() || it->second >= int(trl) || f.mirror_of[it->second] >= 0) continue;
+      int c = it->second;
+      if (f.ncomb[c] != f.ncomb[trl]) continue;
+      std::vector<uint16_t> ma(masks.begin() + f.mask_off[c], masks.begin() + f.mask_off[c] + f.ncomb[c]);
+      std::vector<uint16_t> mb(masks.begin() + f.mask_off[trl], masks.begin() + f.mask_off[trl] + f.ncomb[trl]);
+      std::sort(ma.begin(), ma.end());
+      std::sort(mb.begin(), mb.end());
+      if (ma == mb) f.mirror_of[trl] = c;
+    }
+  }
+  // A trace vector that only single-trace correlators read needs one sum over
+  // all its combinations, not one per mask (resolve_request1, Diagram.cpp:9-27).
+  std::vector<std::vector<char>> multi(fams.size());
+  for (size_t fi = 0; fi < fams.size(); ++fi) multi[fi].assign(fams[fi].tr_ids.size(), 0);
+  for (auto const &cp : corrs)
+    for (auto const &r : cp.reqs)
+      if (r.size() > 1)
+        for (auto const &ref : r) multi[ref.first][ref.second] = 1;
+  for (size_t fi = 0; fi < fams.size(); ++fi)
+    for (size_t trl = 0; trl < fams[fi].tr_ids.size(); ++trl)
+      if (fams[fi].mirror_of[trl] >= 0) {
+        char &a = multi[fi][trl], &b = multi[fi][fams[fi].mirror_of[trl]];
+        a = b = char(a | b);
+      }
+
   // ---- mask groups per trace vector and the exact normalisation of each
   // correlator (the num_summands of DilutedTrace.cpp:5-82, Diagram.cpp:9-27) ----
   std::vector<uint16_t> gmasks;
-  for (auto &f : fams) {
+  for (size_t fi = 0; fi < fams.size(); ++fi) {
+    Family &f = fams[fi];
     NodePlan const &root = f.nodes.back();
     size_t ntrl = f.tr_ids.size();
     f.ngroup.assign(ntrl, 0);
@@ -499,11 +684,19 @@ void Engine::contract(int single_block_pair, int kahan) {
     f.gpairs.clear();
     long long off = 0;
     for (size_t trl = 0; trl < ntrl; ++trl) {
+      if (f.mirror_of[trl] >= 0) {
+        int c = f.mirror_of[trl];
+        f.ngroup[trl] = f.ngroup[c];
+        f.goff[trl] = f.goff[c];
+        f.gmask_off[trl] = f.gmask_off[c];
+        f.gcount[trl] = f.gcount[c];
+        continue;
+      }
       std::vector<uint16_t> gm;
       std::vector<std::vector<int>> members;
       for (int c = 0; c < f.ncomb[trl]; ++c) {
         uint16_t m = masks[f.mask_off[trl] + c];
-        size_t g = std::find(gm.begin(), gm.end(), m) - gm.begin();
+        size_t g = multi[fi][trl] ? std::find(gm.begin(), gm.end(), m) - gm.begin() : 0;
         if (g == gm.size()) {
           gm.push_back(m);
           members.emplace_back();
@@ -529,6 +722,7 @@ void Engine::contract(int single_block_pair, int kahan) {
     f.gstride = off;
     upload(f.d_gitems, f.gitems);
     upload(f.d_gpairs, f.gpairs);
+    plan_lstat(f, root, D_);
   }
   for (auto &cp : corrs) {
     cp.inv_n.resize(cp.reqs.size());
@@ -560,18 +754,19 @@ void Engine::contract(int single_block_pair, int kahan) {
   DevBuf d_gmasks;
   upload(d_gmasks, gmasks);
 
-  // Slices per chunk from the memory budget (worst case: every product node
-  // has a distinct time key per slice pair).
+  // Memory budget of one chunk (product tables + group sums + correlator
+  // values).  0 = automatic: 80 % of the device memory free at this point.
   const long long DD = (long long)D_ * D_;
-  double per_sp = 0;
-  for (auto const &f : fams)
-    for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
-      auto const &ns = f.shape.nodes[n];
-      if (ns.kind == kProd) per_sp += double(f.nodes[n].stride) * DD * sizeof(double2);
-      if (n + 1 == f.shape.nodes.size()) per_sp += double(f.gstride) * sizeof(double2);
-    }
-  for (auto const &c : corrs) per_sp += double(c.reqs.size()) * sizeof(double2);
-  int smax = int(std::max(1.0, std::min(65535.0, double(chunk_budget_) / std::max(per_sp, 1.0))));
+  size_t budget = chunk_budget_;
+  if (budget == 0) {
+    size_t free_b = 0, total_b = 0;
+    SLAPH_HIP(hipMemGetInfo(&free_b, &total_b));
+    budget = free_b / 10 * 8;
+  }
+  long long gstride_all = 0;
+  for (auto const &f : fams) gstride_all += f.gstride;
+  size_t max_corr = 1;
+  for (auto const &c : corrs) max_corr = std::max(max_corr, c.reqs.size());
 
   // ---- accumulators ----
   std::vector<DevBuf> d_sum(corrs.size()), d_comp(corrs.size());
@@ -584,41 +779,31 @@ void Engine::contract(int single_block_pair, int kahan) {
   }
   DevBuf d_prod, d_cval, d_td, d_gv;
   std::vector<DevBuf> d_jobs_c(corrs.size());
-  size_t max_corr = 1;
-  for (auto const &c : corrs) max_corr = std::max(max_corr, c.reqs.size());
-  d_cval.alloc(sizeof(double2) * max_corr * (size_t)std::min<size_t>(smax, (size_t)Lt_ * Lt_));
 
-  auto slab_of = [&](int q) -> const double2 * {
-    return q == 0 ? q0_.as<double2>() : (q == 1 ? q1_.as<double2>() : q2_.as<double2>());
-  };
-  auto leaf_base = [&](LeafSpec const &ls, std::vector<int> const &tt) -> long long {
-    if (ls.q == 0) return (long long)tt[ls.a] * q_total_[0];
-    if (ls.q == 1) return ((long long)tt[ls.a] * nb_ + B(tt[ls.b])) * q_total_[1];
-    int slot = q2slot_.at(((size_t)B(tt[ls.a]) * Lt_ + tt[ls.b]) * nb_ + B(tt[ls.c]));
-    if (slot < 0) throw std::logic_error("Q2 key was not built");
-    return (long long)slot * q_total_[2];
-  };
 
   EventTimer ctimer(stream_);
   const bool progress = std::getenv("SLAPH_PROGRESS") != nullptr;
+  // SLAPH_TRACE_GROUPS=1 selects the one-wave-per-group trace kernel for every
+  // non-separable trace (a cross-check of the left-stationary kernel).
+  const bool lstat_off = std::getenv("SLAPH_TRACE_GROUPS") != nullptr;
   int bp_done = 0;
+  // Chunks are whole rows (one source time, all sink times of the block) of
+  // one direction of the block pair, so that time-separable traces see a
+  // rectangular tile of slice pairs.  A chunk starts as a whole pass
+  // (dilT rows); the row count halves while its plan exceeds the budget.
+  const size_t pass_len = (size_t)dilT * dilT;
+  size_t chunk_rows = (size_t)dilT;
   for (auto const &sl : bp_slices) {
     if (progress) {
       // like the reference's per-block-pair message (Correlators.cpp:92-97)
-      std::fprintf(stderr, "[slaph] block pair %d of %zu (%zu slice pairs, %d per launch)\n", bp_done++,
-                   bp_slices.size(), sl.size(), smax);
+      std::fprintf(stderr, "[slaph] block pair %d of %zu (%zu slice pairs, %zu rows of %d per launch)\n", bp_done++,
+                   bp_slices.size(), sl.size(), chunk_rows, dilT);
       std::fflush(stderr);
     }
-    // Chunks are whole rows (one source time, all sink times of the block)
-    // of one direction of the block pair, so that time-separable traces see
-    // a rectangular tile of slice pairs.
-    const size_t pass_len = (size_t)dilT * dilT;
-    const size_t chunk_rows = std::max<size_t>(1, std::min<size_t>(dilT, size_t(smax) / dilT));
     for (size_t c0 = 0, S_ = 0; c0 < sl.size(); c0 += S_) {
-      S_ = std::min(chunk_rows * dilT, (c0 / pass_len + 1) * pass_len - c0);
-      S_ = std::min(S_, sl.size() - c0);
-      const int S = int(S_);
-      // time planning
+      // time planning of slice pairs c0 .. c0+S-1; returns the number of
+      // product matrices the chunk stores
+      auto plan_chunk = [&](int S) -> long long {
       long long prod_total = 0;
       for (auto &f : fams) {
         std::vector<std::vector<int>> tts(S);
@@ -712,12 +897,31 @@ void Engine::contract(int single_block_pair, int kahan) {
               Yt[ti] = r.kind == kLeaf ? np.time_leafbase_r[ti] : ro.base + np.time_right[ti] * ro.stride;
             }
           }
-          upload(np.d_Xt, Xt);
-          upload(np.d_Yt, Yt);
           np.h_Xt = std::move(Xt);
           np.h_Yt = std::move(Yt);
         }
       }
+      return prod_total;
+      };
+      int S = 0;
+      long long prod_total = 0;
+      for (;;) {
+        S_ = std::min(chunk_rows * dilT, (c0 / pass_len + 1) * pass_len - c0);
+        S_ = std::min(S_, sl.size() - c0);
+        S = int(S_);
+        prod_total = plan_chunk(S);
+        double bytes = sizeof(double2) * (double(prod_total) * DD + double(S) * (gstride_all + max_corr));
+        if (bytes <= double(budget) || chunk_rows == 1) break;
+        chunk_rows = std::max<size_t>(1, chunk_rows / 2);
+      }
+      for (auto &f : fams)
+        for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+          NodePlan &np = f.nodes[n];
+          if (f.shape.nodes[n].kind == kLeaf || np.owner != int(n)) continue;
+          upload(np.d_Xt, np.h_Xt);
+          upload(np.d_Yt, np.h_Yt);
+        }
+      d_cval.ensure(sizeof(double2) * max_corr * (size_t)S);
       d_prod.ensure(sizeof(double2) * std::max<long long>(1, prod_total * DD));
       // trace values go straight into mask-group sums
       long long g_total = 0;
@@ -769,6 +973,11 @@ void Engine::contract(int single_block_pair, int kahan) {
             launch_trace_grid(src(ns.left), np.d_Xq.as<long long>(), src(ns.right), np.d_Yq.as<long long>(),
                               f.d_gitems.as<GroupItem>(), int(f.gitems.size()), f.d_gpairs.as<int2>(),
                               f.d_tile.as<GridTile>(), 1, d_gv.as<double2>(), f.gbase, f.gstride, D_, stream_);
+          } else if (ns.kind == kTrace2 && !f.ls_blocks.empty() && !lstat_off) {
+            launch_trace_lstat(src(ns.left), np.d_Xt.as<long long>(), src(ns.right), np.d_Yt.as<long long>(),
+                               f.d_lsb.as<LsBlock>(), int(f.ls_blocks.size()), f.d_lsc.as<LsChunk>(),
+                               f.d_lst.as<LsTask>(), f.d_lsd.as<long long>(), d_gv.as<double2>(), f.gbase, f.gstride,
+                               ntime, D_, stream_);
           } else {
             launch_trace_groups(src(ns.left), np.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
                                 np.d_Yt.as<long long>(), np.d_Yq.as<long long>(), f.d_gitems.as<GroupItem>(),

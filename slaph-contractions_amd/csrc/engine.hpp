@@ -148,7 +148,7 @@ class Engine {
   };
   std::vector<DiagramResult> results_;
 
-  size_t chunk_budget_ = size_t(24) << 30;
+  size_t chunk_budget_ = 0;  // 0: from the free device memory
   StageTimes times_;
 };
 

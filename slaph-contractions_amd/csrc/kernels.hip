@@ -357,36 +357,97 @@ __global__ __launch_bounds__(64) void k_prod(const double2 *__restrict__ sX, con
   }
 }
 
-// tr(X Y) = sum_{a,b} X[a,b] Y[b,a] (reference: factor_to_trace(L, R),
-// DilutedTrace.cpp:84-138, which forms the full product first), or tr(X)
-// when no right operand is given (DilutedTrace.cpp:140-165).
-__global__ __launch_bounds__(64) void k_trace(const double2 *__restrict__ sX, const long long *__restrict__ Xt,
-                                              const long long *__restrict__ Xq, const double2 *__restrict__ sY,
-                                              const long long *__restrict__ Yt, const long long *__restrict__ Yq,
-                                              double2 *__restrict__ out, long long obase, long long ostride,
-                                              const ComboProd *__restrict__ combos, int D) {
-  const ComboProd c = combos[blockIdx.x];
-  const int ti = blockIdx.y;
-  const long long DD = (long long)D * D;
-  const double2 *x = sX + (Xt[ti] + Xq[c.qn] + c.i) * DD;
-  double re = 0.0, im = 0.0;
-  if (sY) {
-    const double2 *y = sY + (Yt[ti] + Yq[c.qn] + c.j) * DD;
-    for (int e = threadIdx.x; e < D * D; e += 64) {
-      const int a = e % D, b = e / D;
-      const double2 p = cmul(x[e], y[a * D + b]);
-      re += p.x;
-      im += p.y;
+// ---------------------------------------------------------------------------
+// Non-separable traces tr(X Y) = sum_{a,b} X[a,b] Y[b,a] (reference:
+// factor_to_trace(L, R), DilutedTrace.cpp:84-138), left-stationary: one
+// workgroup per (time, block of slots sharing one left operator entry).  The
+// block's left matrices are staged chunk by chunk in LDS, transposed, so that
+// element e of a right matrix meets its partner at the same index.  Each
+// wavefront streams the right matrices of its tasks into registers once
+// (EPL = D*D/64 elements per lane; the next task's matrix is loaded while the
+// current one is used) and multiplies them with one or two staged left
+// matrices.  At the end of a run (or of the wave's task range) the wave sum
+// is added to the wave's own copy of the run's slot; the copies are summed in
+// wave order at the end, so the summation order is fixed by the plan.
+// ---------------------------------------------------------------------------
+constexpr int kLsStageBytes = 48 * 1024;
+
+template <int D>
+__global__ __launch_bounds__(64 * kLsWaves) void k_trace_lstat(
+    const double2 *__restrict__ sX, const long long *__restrict__ Xt, const double2 *__restrict__ sY,
+    const long long *__restrict__ Yt, const LsBlock *__restrict__ blocks, int nblock, const LsChunk *__restrict__ chunks,
+    const LsTask *__restrict__ tasks, const long long *__restrict__ slot_dst, double2 *__restrict__ out,
+    long long obase, long long ostride) {
+  constexpr int DD = D * D, EPL = DD / 64, CL = kLsStageBytes / (DD * 16);
+  __shared__ double2 Ls[CL * DD];
+  __shared__ double2 acc[kLsWaves][kLsSlots];
+  const int ti = blockIdx.x / nblock;
+  const LsBlock B = blocks[blockIdx.x % nblock];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int k = tid; k < kLsWaves * kLsSlots; k += 64 * kLsWaves) (&acc[0][0])[k] = make_double2(0.0, 0.0);
+  const double2 *xb = sX + (Xt[ti] + B.xq) * DD;
+  const double2 *yb = sY + Yt[ti] * DD;
+  for (int c = 0; c < B.nchunk; ++c) {
+    const LsChunk C = chunks[B.chunk_off + c];
+    __syncthreads();  // the previous chunk is no longer read
+    for (int idx = tid; idx < C.n * DD; idx += 64 * kLsWaves) {
+      const int m = idx / DD, f = idx % DD;
+      Ls[m * DD + f / D + (f % D) * D] = xb[(long long)(C.i0 + m) * DD + f];
     }
-  } else {
-    for (int a = threadIdx.x; a < D; a += 64) {
-      re += x[a * D + a].x;
-      im += x[a * D + a].y;
+    __syncthreads();
+    const int t0 = C.wtask[wave], t1 = C.wtask[wave + 1];
+    if (t0 >= t1) continue;
+    double re = 0.0, im = 0.0;
+    double2 r[EPL], rn[EPL];
+    LsTask T = tasks[t0];
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) r[k] = yb[(long long)T.yrel * DD + lane + 64 * k];
+    for (int t = t0; t < t1; ++t) {
+      LsTask Tn = T;
+      if (t + 1 < t1) {
+        Tn = tasks[t + 1];
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) rn[k] = yb[(long long)Tn.yrel * DD + lane + 64 * k];
+      }
+      const double2 *la = Ls + T.ia * DD + lane;
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        const double2 u = la[64 * k], v = r[k];
+        re = fma(u.x, v.x, re);
+        re = fma(-u.y, v.y, re);
+        im = fma(u.x, v.y, im);
+        im = fma(u.y, v.x, im);
+      }
+      if (T.ib >= 0) {
+        const double2 *lb = Ls + T.ib * DD + lane;
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+          const double2 u = lb[64 * k], v = r[k];
+          re = fma(u.x, v.x, re);
+          re = fma(-u.y, v.y, re);
+          im = fma(u.x, v.y, im);
+          im = fma(u.y, v.x, im);
+        }
+      }
+      if (T.slot >= 0) {
+        re = wave_sum(re);
+        im = wave_sum(im);
+        if (lane == 0) acc[wave][T.slot] = cadd(acc[wave][T.slot], make_double2(re, im));
+        re = 0.0;
+        im = 0.0;
+      }
+      T = Tn;
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) r[k] = rn[k];
     }
   }
-  re = wave_sum(re);
-  im = wave_sum(im);
-  if (threadIdx.x == 0) out[obase + ti * ostride + c.o] = make_double2(re, im);
+  __syncthreads();
+  for (int k = tid; k < B.nslot; k += 64 * kLsWaves) {
+    double2 v = acc[0][k];
+#pragma unroll
+    for (int w = 1; w < kLsWaves; ++w) v = cadd(v, acc[w][k]);
+    out[obase + ti * ostride + slot_dst[B.slot_off + k]] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -715,13 +776,40 @@ void launch_prod(const double2 *sX, const long long *Xt, const long long *Xq, co
   SLAPH_HIP(hipGetLastError());
 }
 
-void launch_trace(const double2 *sX, const long long *Xt, const long long *Xq, const double2 *sY, const long long *Yt,
-                  const long long *Yq, double2 *out, long long obase, long long ostride, const ComboProd *combos,
-                  int ncombo, int ntime, int D, hipStream_t s) {
-  if (ncombo <= 0 || ntime <= 0) return;
-  if (ntime > 65535) throw std::runtime_error("launch_trace: too many slice pairs");
-  hipLaunchKernelGGL(k_trace, dim3(ncombo, ntime), dim3(64), 0, s, sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos,
-                     D);
+bool trace_lstat_supported(int D) { return D == 8 || D == 16 || D == 24 || D == 32; }
+
+int trace_lstat_chunk(int D) {
+  if (!trace_lstat_supported(D)) return 0;
+  return kLsStageBytes / (D * D * int(sizeof(double2)));
+}
+
+void launch_trace_lstat(const double2 *sX, const long long *Xt, const double2 *sY, const long long *Yt,
+                        const LsBlock *blocks, int nblock, const LsChunk *chunks, const LsTask *tasks,
+                        const long long *slot_dst, double2 *out, long long obase, long long ostride, int ntime, int D,
+                        hipStream_t s) {
+  if (nblock <= 0 || ntime <= 0) return;
+  if ((long long)nblock * ntime > 0x7fffffffLL) throw std::runtime_error("launch_trace_lstat: grid too large");
+  const dim3 grid(unsigned(nblock * ntime)), block(64 * kLsWaves);
+  switch (D) {
+    case 8:
+      hipLaunchKernelGGL(k_trace_lstat<8>, grid, block, 0, s, sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst,
+                         out, obase, ostride);
+      break;
+    case 16:
+      hipLaunchKernelGGL(k_trace_lstat<16>, grid, block, 0, s, sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst,
+                         out, obase, ostride);
+      break;
+    case 24:
+      hipLaunchKernelGGL(k_trace_lstat<24>, grid, block, 0, s, sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst,
+                         out, obase, ostride);
+      break;
+    case 32:
+      hipLaunchKernelGGL(k_trace_lstat<32>, grid, block, 0, s, sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst,
+                         out, obase, ostride);
+      break;
+    default:
+      throw std::runtime_error("launch_trace_lstat: unsupported factor dimension");
+  }
   SLAPH_HIP(hipGetLastError());
 }
 

@@ -83,10 +83,6 @@ struct ComboProd {
 void launch_prod(const double2 *slabX, const long long *Xt, const long long *Xq, const double2 *slabY,
                  const long long *Yt, const long long *Yq, double2 *out, long long obase, long long ostride,
                  const ComboProd *combos, int ncombo, int ntime, int D, hipStream_t s);
-// out scalar index = obase + ti*ostride + o; right slab null -> trace of a single matrix.
-void launch_trace(const double2 *slabX, const long long *Xt, const long long *Xq, const double2 *slabY,
-                  const long long *Yt, const long long *Yq, double2 *out, long long obase, long long ostride,
-                  const ComboProd *combos, int ncombo, int ntime, int D, hipStream_t s);
 
 // Correlator resolution: mean / 2-way / 3-way masked inner products.
 // A trace vector seen through its used-index mask groups: the trace values of
@@ -128,6 +124,36 @@ struct GridTile {
 void launch_trace_grid(const double2 *slabX, const long long *Xq, const double2 *slabY, const long long *Yq,
                        const GroupItem *items, int nitems, const int2 *pairs, const GridTile *tiles, int ntiles,
                        double2 *out, long long obase, long long ostride, int D, hipStream_t s);
+// Non-separable traces, left-stationary.  A block is a set of (trace id,
+// mask group) slots that share one left operator entry (left matrices
+// X[Xt[ti] + xq + i]); its chunks stage left combinations [i0, i0 + n) and
+// list tasks per wavefront.  A task multiplies right matrix Y[Yt[ti] + yrel]
+// with staged left combinations ia (and ib >= 0); slot >= 0 marks the last
+// task of a run, whose sum is added to that slot.  Output:
+// out[obase + ti*ostride + slot_dst[slot_off + k]] = slot k.
+constexpr int kLsWaves = 4;
+constexpr int kLsSlots = 64;
+struct LsBlock {
+  long long xq;
+  int chunk_off, nchunk;
+  int slot_off, nslot;
+};
+struct LsChunk {
+  int i0, n;
+  int wtask[kLsWaves + 1];  // task range of each wavefront
+};
+struct LsTask {
+  int yrel;
+  short ia, ib;
+  int slot;
+};
+// Largest number of left matrices one chunk stages for matrix size D.
+int trace_lstat_chunk(int D);
+bool trace_lstat_supported(int D);
+void launch_trace_lstat(const double2 *slabX, const long long *Xt, const double2 *slabY, const long long *Yt,
+                        const LsBlock *blocks, int nblock, const LsChunk *chunks, const LsTask *tasks,
+                        const long long *slot_dst, double2 *out, long long obase, long long ostride, int ntime, int D,
+                        hipStream_t s);
 void launch_resolve(const double2 *gvals, const uint16_t *gmasks, const CorrJob *jobs, int ncorr, int nsp,
                     double2 *cval, hipStream_t s);
 void launch_accumulate(const double2 *cval, const int *tdelta, int nsp, int ncorr, double2 *sum, double2 *comp,

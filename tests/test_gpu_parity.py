@@ -123,7 +123,7 @@ def test_l4_single_block_pair_and_chunking(slaph, l4_ctx, l4_oracle):
     try:
         slaph.contract(ctx, single_time_slice_combination=1)
     finally:
-        ctx.set_chunk_budget(24 << 30)
+        ctx.set_chunk_budget(0)
     want = o.contract(cfg, lut, vdv, perams, rnds, single_block_pair=1)
     for d, lst in want.items():
         got = ctx.correlators(d)
@@ -150,11 +150,22 @@ correlator_list = C6cD:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0
 """
 
 
-def test_host_inputs_small_pipi_matches_oracle(slaph):
+WIDE = SMALL.replace("C4cB:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0\n", "C4cB:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1\n").replace(
+    "C4cC:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0\n", "C4cC:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1\n")
+
+
+@pytest.mark.parametrize("infile,trace_kernel", [(SMALL, "lstat"), (WIDE, "lstat"), (WIDE, "groups")])
+def test_host_inputs_small_pipi_matches_oracle(slaph, infile, trace_kernel, monkeypatch):
     """Config-1-like lattice (4^3 x 8, N=8, dilE 4, 4 random vectors) with
-    seeded host inputs: every correlator against the oracle at 1e-12 norm-wise."""
-    say("small pipi")
-    cfg = o.parse_infile(SMALL)
+    seeded host inputs: every correlator against the oracle at 1e-12
+    norm-wise.  The wide variant (C4cB / C4cC up to |P|^2 = 1) runs the
+    non-separable traces through both trace kernels."""
+    say("small pipi", "wide" if infile is WIDE else "", trace_kernel)
+    if trace_kernel == "groups":
+        monkeypatch.setenv("SLAPH_TRACE_GROUPS", "1")
+    else:
+        monkeypatch.delenv("SLAPH_TRACE_GROUPS", raising=False)
+    cfg = o.parse_infile(infile)
     lut = o.init_lookup_tables(cfg)
     rng = np.random.default_rng(11)
     Lt, N, K = cfg["Lt"], cfg["number_of_eigen_vec"], 3 * 64
@@ -165,7 +176,7 @@ def test_host_inputs_small_pipi_matches_oracle(slaph):
     rnds = [((rng.integers(0, 2, Lt * N * 4) * 2 - 1) + 1j * (rng.integers(0, 2, Lt * N * 4) * 2 - 1)) / np.sqrt(2)
             for _ in range(nrnd)]
     ctx = slaph.Context(0)
-    ctx.configure(SMALL)
+    ctx.configure(infile)
     for t in range(Lt):
         ctx.set_eigenvectors(t, ev[t])
     for r in range(nrnd):
@@ -173,13 +184,18 @@ def test_host_inputs_small_pipi_matches_oracle(slaph):
         ctx.set_random_vector(r, rnds[r])
     slaph.OperatorFactory(ctx).create_operators()
     slaph.contract(ctx)
-    vdv = o.build_vdaggerv(cfg, lut, ev)
-    want = o.contract(cfg, lut, vdv, perams, rnds)
+    if infile not in _SMALL_WANT:
+        vdv = o.build_vdaggerv(cfg, lut, ev)
+        _SMALL_WANT[infile] = o.contract(cfg, lut, vdv, perams, rnds)
+    want = _SMALL_WANT[infile]
     for d, lst in want.items():
         got = ctx.correlators(d)
         for name, vals in lst:
             scale = max(np.max(np.abs(vals)), 1e-300)
             assert np.max(np.abs(got[name] - vals)) / scale <= 1e-12, (d, name)
+
+
+_SMALL_WANT = {}  # oracle correlators per input file (same seeded inputs)
 
 
 def test_synthetic_inputs_match_oracle(slaph):
