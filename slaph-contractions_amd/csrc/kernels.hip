@@ -8,6 +8,8 @@
 // that the host planner builds.
 #include "kernels.hpp"
 
+#include <cstdlib>
+
 namespace slaph {
 
 // Defined in host.cpp: the sparse gamma tables as plain arrays.
@@ -354,6 +356,56 @@ __global__ __launch_bounds__(64) void k_prod(const double2 *__restrict__ sX, con
     double2 s = make_double2(0.0, 0.0);
     for (int k = 0; k < D; ++k) cfma(s, A[k * D + r], B[col * D + k]);
     o[e] = s;
+  }
+}
+
+// D = 16 factor products on FP64 MFMA, computed transposed: C^T = Y^T X^T is
+// one 16 x 16 x 16 MFMA tile (4 k-steps x 4 real products).  With the MFMA
+// operand layout A[m = lane%16][k = lane/16], B[k = lane/16][n = lane%16] and
+// result element i of lane l at (l/16 + 4i, l%16), lane l reads X at l + 64s
+// (k-step s, coalesced), Y at (4s + l/16) + 16 (l%16) (64-byte runs), and
+// writes C at l + 64i (coalesced).  A wave computes kProdPerWave consecutive
+// combinations of one time index; consecutive combinations that share the
+// left matrix keep it in registers.
+constexpr int kProdPerWave = 4;
+
+__global__ __launch_bounds__(256) void k_prod16(const double2 *__restrict__ sX, const long long *__restrict__ Xt,
+                                                const long long *__restrict__ Xq, const double2 *__restrict__ sY,
+                                                const long long *__restrict__ Yt, const long long *__restrict__ Yq,
+                                                double2 *__restrict__ out, long long obase, long long ostride,
+                                                const ComboProd *__restrict__ combos, int ncombo) {
+  const int ti = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c0 = (blockIdx.x * 4 + wave) * kProdPerWave;
+  const int c1 = min(c0 + kProdPerWave, ncombo);
+  const long long xt = Xt[ti], yt = Yt[ti];
+  const int yoff = (lane >> 4) + 16 * (lane & 15);
+  double2 xa[4];
+  long long xcur = -1;
+  for (int c = c0; c < c1; ++c) {
+    const ComboProd cb = combos[c];
+    const long long xi = xt + Xq[cb.qn] + cb.i;
+    if (xi != xcur) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) xa[s] = sX[xi * 256 + lane + 64 * s];
+      xcur = xi;
+    }
+    const double2 *y = sY + (yt + Yq[cb.qn] + cb.j) * 256 + yoff;
+    double2 yb[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) yb[s] = y[4 * s];
+    d4 dr = d4{0.0, 0.0, 0.0, 0.0}, di = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      // C[r][c] = sum_k X[r][k] Y[k][c]: re = Xr Yr - Xi Yi, im = Xr Yi + Xi Yr
+      dr = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].x, xa[s].x, dr, 0, 0, 0);
+      dr = __builtin_amdgcn_mfma_f64_16x16x4f64(-yb[s].y, xa[s].y, dr, 0, 0, 0);
+      di = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].y, xa[s].x, di, 0, 0, 0);
+      di = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].x, xa[s].y, di, 0, 0, 0);
+    }
+    double2 *o = out + (obase + ti * ostride + cb.o) * 256;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o[lane + 64 * i] = make_double2(dr[i], di[i]);
   }
 }
 
@@ -771,8 +823,14 @@ void launch_prod(const double2 *sX, const long long *Xt, const long long *Xq, co
   if (ncombo <= 0 || ntime <= 0) return;
   if (D > kMaxD) throw std::runtime_error("factor dimension above 32 is not supported");
   if (ntime > 65535) throw std::runtime_error("launch_prod: too many time keys");
-  hipLaunchKernelGGL(k_prod, dim3(ncombo, ntime), dim3(64), 0, s, sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos,
-                     D);
+  if (D == 16 && !std::getenv("SLAPH_PROD_GENERIC")) {
+    const int per_block = 4 * kProdPerWave;
+    hipLaunchKernelGGL(k_prod16, dim3((ncombo + per_block - 1) / per_block, ntime), dim3(256), 0, s, sX, Xt, Xq, sY,
+                       Yt, Yq, out, obase, ostride, combos, ncombo);
+  } else {
+    hipLaunchKernelGGL(k_prod, dim3(ncombo, ntime), dim3(64), 0, s, sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride,
+                       combos, D);
+  }
   SLAPH_HIP(hipGetLastError());
 }
 

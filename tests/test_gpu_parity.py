@@ -154,17 +154,20 @@ WIDE = SMALL.replace("C4cB:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0\n", "C4cB:Q0:Op0:Q0:Op
     "C4cC:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0\n", "C4cC:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1\n")
 
 
-@pytest.mark.parametrize("infile,trace_kernel", [(SMALL, "lstat"), (WIDE, "lstat"), (WIDE, "groups")])
-def test_host_inputs_small_pipi_matches_oracle(slaph, infile, trace_kernel, monkeypatch):
+@pytest.mark.parametrize("infile,kernels", [(SMALL, "lstat+mfma"), (WIDE, "lstat+mfma"), (WIDE, "groups+generic")],
+                         ids=["small", "wide", "wide-alt-kernels"])
+def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypatch):
     """Config-1-like lattice (4^3 x 8, N=8, dilE 4, 4 random vectors) with
     seeded host inputs: every correlator against the oracle at 1e-12
     norm-wise.  The wide variant (C4cB / C4cC up to |P|^2 = 1) runs the
-    non-separable traces through both trace kernels."""
-    say("small pipi", "wide" if infile is WIDE else "", trace_kernel)
-    if trace_kernel == "groups":
-        monkeypatch.setenv("SLAPH_TRACE_GROUPS", "1")
-    else:
-        monkeypatch.delenv("SLAPH_TRACE_GROUPS", raising=False)
+    non-separable traces and the D = 16 products through both kernels each
+    (left-stationary / one wave per group; MFMA / generic product)."""
+    say("small pipi", "wide" if infile is WIDE else "", kernels)
+    for var, on in (("SLAPH_TRACE_GROUPS", "groups" in kernels), ("SLAPH_PROD_GENERIC", "generic" in kernels)):
+        if on:
+            monkeypatch.setenv(var, "1")
+        else:
+            monkeypatch.delenv(var, raising=False)
     cfg = o.parse_infile(infile)
     lut = o.init_lookup_tables(cfg)
     rng = np.random.default_rng(11)
