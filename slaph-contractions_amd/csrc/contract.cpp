@@ -13,6 +13,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -213,6 +214,7 @@ struct NodePlan {
   long long cmaxX = 0, cmaxY = 0, cmaxO = 0;  // largest qn offset + combo index per operand
   std::vector<int> trl_qn;  // trl -> qn (in the owner's qn space)
   int owner = -1;           // node whose product table this node reads
+  bool transposed = false;  // product table stored as Y^T (a trace's right operand)
   DevBuf d_combos, d_Xq, d_Yq;
   std::vector<long long> h_Xq, h_Yq;
   // time part (per chunk)
@@ -245,6 +247,7 @@ struct Family {
   // trl -> earlier trl of equal value (tr(X_a Y_b) = tr(X_b Y_a) when both
   // operands read the same table at the same time), or -1
   std::vector<int> mirror_of;
+  bool coincide = false;  // both trace operands read the same factors at every slice pair
   // left-stationary trace plan (time independent)
   std::vector<LsBlock> ls_blocks;
   std::vector<LsChunk> ls_chunks;
@@ -439,6 +442,46 @@ void Engine::contract(int single_block_pair, int kahan) {
     corrs.push_back(std::move(cp));
   }
 
+  // ---- the slice pairs of every block pair (Correlators.cpp:84-127) ----
+  auto bps = block_pairs(Lt_, dilT);
+  std::vector<std::vector<SlicePair>> bp_slices;
+  for (size_t b = 0; b < bps.size(); ++b) {
+    if (single_block_pair >= 0 && int(b) != single_block_pair) continue;
+    bp_slices.push_back(slice_pairs(bps[b].first, bps[b].second, Lt_, dilT));
+  }
+  // (factor type, time key) of a leaf at trace times tt
+  auto leaf_key = [&](LeafSpec const &ls, std::vector<int> const &tt) {
+    if (ls.q == 0) return std::array<int, 4>{0, tt[ls.a], 0, 0};
+    if (ls.q == 1) return std::array<int, 4>{1, tt[ls.a], B(tt[ls.b]), 0};
+    return std::array<int, 4>{2, B(tt[ls.a]), tt[ls.b], B(tt[ls.c])};
+  };
+  // Do the left and right operands of the family's trace read the same
+  // factors at every slice pair?
+  auto operands_coincide = [&](Family const &f) {
+    auto const &rs = f.shape.nodes.back();
+    if (rs.kind != kTrace2) return false;
+    std::function<void(int, std::vector<int> &)> leaves_of = [&](int n, std::vector<int> &out) {
+      auto const &ns = f.shape.nodes[n];
+      if (ns.kind == kLeaf) {
+        out.push_back(ns.leaf);
+        return;
+      }
+      leaves_of(ns.left, out);
+      if (ns.right >= 0) leaves_of(ns.right, out);
+    };
+    std::vector<int> lL, lR;
+    leaves_of(rs.left, lL);
+    leaves_of(rs.right, lR);
+    if (lL.size() != lR.size()) return false;
+    for (auto const &sl : bp_slices)
+      for (auto const &sp : sl) {
+        auto tt = trace_times(f, sp.source, sp.sink);
+        for (size_t k = 0; k < lL.size(); ++k)
+          if (leaf_key(f.shape.leaves[lL[k]], tt) != leaf_key(f.shape.leaves[lR[k]], tt)) return false;
+      }
+    return true;
+  };
+
   // ---- quantum-number planning per family ----
   compute_q_offsets();
   static const char *qname[3] = {"Q0", "Q1", "Q2"};
@@ -447,22 +490,32 @@ void Engine::contract(int single_block_pair, int kahan) {
   for (auto &f : fams) {
     auto const &cl = gd_.trace_indices_map.at(f.name);
     f.nodes.resize(f.shape.nodes.size());
+    f.coincide = operands_coincide(f);
     // Product nodes with the same subtree shape (same factor types in the same
     // places) share one product table: an alias keeps its own time/qn index
     // maps into its owner's storage, so an instance (time key, operator
-    // entries) is computed once however many nodes read it.
+    // entries) is computed once however many nodes read it.  Exception: when
+    // the trace's operands do not coincide, a product that is the trace's
+    // right operand keeps a table of its own, stored transposed, so that
+    // tr(X Y) = sum_e X[e] Y^T[e] reads both operands in memory order.
     std::vector<std::string> sig(f.shape.nodes.size());
+    const int rroot = f.shape.nodes.back().kind == kTrace2 ? f.shape.nodes.back().right : -1;
     for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
       auto const &ns = f.shape.nodes[n];
       if (ns.kind == kLeaf) sig[n] = "L" + std::to_string(f.shape.leaves[ns.leaf].q);
       else sig[n] = "P(" + sig[ns.left] + "," + (ns.right >= 0 ? sig[ns.right] : "") + ")";
       f.nodes[n].owner = int(n);
-      if (ns.kind == kProd)
-        for (size_t m = 0; m < n; ++m)
-          if (f.shape.nodes[m].kind == kProd && sig[m] == sig[n]) {
-            f.nodes[n].owner = f.nodes[m].owner;
-            break;
-          }
+      f.nodes[n].transposed = false;
+      if (ns.kind != kProd) continue;
+      if (!f.coincide && int(n) == rroot) {
+        f.nodes[n].transposed = true;
+        continue;
+      }
+      for (size_t m = 0; m < n; ++m)
+        if (f.shape.nodes[m].kind == kProd && sig[m] == sig[n] && !f.nodes[m].transposed) {
+          f.nodes[n].owner = f.nodes[m].owner;
+          break;
+        }
     }
     for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
       auto const &ns = f.shape.nodes[n];
@@ -567,13 +620,7 @@ void Engine::contract(int single_block_pair, int kahan) {
     }
   }
 
-  // ---- chunks of slice pairs and the factor keys they need ----
-  auto bps = block_pairs(Lt_, dilT);
-  std::vector<std::vector<SlicePair>> bp_slices;
-  for (size_t b = 0; b < bps.size(); ++b) {
-    if (single_block_pair >= 0 && int(b) != single_block_pair) continue;
-    bp_slices.push_back(slice_pairs(bps[b].first, bps[b].second, Lt_, dilT));
-  }
+  // ---- factor keys the slice pairs need ----
   std::set<std::vector<int>> need1, need2;
   for (auto const &sl : bp_slices)
     for (auto const &sp : sl)
@@ -613,28 +660,7 @@ void Engine::contract(int single_block_pair, int kahan) {
     bool same_table = ln.kind == kLeaf && rn.kind == kLeaf
                           ? f.shape.leaves[ln.leaf].q == f.shape.leaves[rn.leaf].q
                           : (ln.kind == kProd && rn.kind == kProd && f.nodes[rs.left].owner == f.nodes[rs.right].owner);
-    if (!same_table) continue;
-    std::function<void(int, std::vector<int> &)> leaves_of = [&](int n, std::vector<int> &out) {
-      auto const &ns = f.shape.nodes[n];
-      if (ns.kind == kLeaf) {
-        out.push_back(ns.leaf);
-        return;
-      }
-      leaves_of(ns.left, out);
-      if (ns.right >= 0) leaves_of(ns.right, out);
-    };
-    std::vector<int> lL, lR;
-    leaves_of(rs.left, lL);
-    leaves_of(rs.right, lR);
-    bool same_time = lL.size() == lR.size();
-    for (auto const &sl : bp_slices)
-      for (auto const &sp : sl) {
-        if (!same_time) break;
-        auto tt = trace_times(f, sp.source, sp.sink);
-        for (size_t k = 0; k < lL.size() && same_time; ++k)
-          same_time = leaf_base(f.shape.leaves[lL[k]], tt) == leaf_base(f.shape.leaves[lR[k]], tt);
-      }
-    if (!same_time) continue;
+    if (!same_table || !f.coincide) continue;
     NodePlan const &root = f.nodes.back();
     std::map<std::pair<int, int>, int> first;
     for (size_t trl = 0; trl < ntrl; ++trl) {
@@ -784,7 +810,7 @@ void Engine::contract(int single_block_pair, int kahan) {
   EventTimer ctimer(stream_);
   const bool progress = std::getenv("SLAPH_PROGRESS") != nullptr;
   // SLAPH_TRACE_GROUPS=1 selects the one-wave-per-group trace kernel for every
-  // non-separable trace (a cross-check of the left-stationary kernel).
+  // trace (a cross-check of the tiled and left-stationary kernels).
   const bool lstat_off = std::getenv("SLAPH_TRACE_GROUPS") != nullptr;
   int bp_done = 0;
   // Chunks are whole rows (one source time, all sink times of the block) of
@@ -964,25 +990,32 @@ void Engine::contract(int single_block_pair, int kahan) {
               (root ? f.gbase + (long long)(ntime - 1) * f.gstride + f.gstride > g_total
                     : np.base + (long long)(ntime - 1) * np.stride + np.cmaxO >= prod_total))
             throw std::logic_error("contraction plan indexes outside its buffers (" + f.name + ")");
+          // operands stored transposed: only a trace's right operand may be
+          auto is_t = [&](int child) {
+            return child >= 0 && f.shape.nodes[child].kind == kProd && f.nodes[f.nodes[child].owner].transposed;
+          };
+          const int rt = is_t(ns.right) ? 1 : 0;
+          if (is_t(ns.left) || (!root && rt)) throw std::logic_error("transposed product table read as a factor");
           if (!root) {
             launch_prod(src(ns.left), np.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
                         np.d_Yt.as<long long>(), np.d_Yq.as<long long>(), d_prod.as<double2>(), np.base, np.stride,
-                        np.d_combos.as<ComboProd>(), int(np.combos.size()), ntime, D_, stream_);
-          } else if (ns.kind == kTrace2 && grid_tile(np.h_Xt, np.h_Yt, tile)) {
+                        np.d_combos.as<ComboProd>(), int(np.combos.size()), ntime, D_, np.transposed ? 1 : 0,
+                        stream_);
+          } else if (ns.kind == kTrace2 && !lstat_off && grid_tile(np.h_Xt, np.h_Yt, tile)) {
             upload(f.d_tile, std::vector<GridTile>{tile});
             launch_trace_grid(src(ns.left), np.d_Xq.as<long long>(), src(ns.right), np.d_Yq.as<long long>(),
                               f.d_gitems.as<GroupItem>(), int(f.gitems.size()), f.d_gpairs.as<int2>(),
-                              f.d_tile.as<GridTile>(), 1, d_gv.as<double2>(), f.gbase, f.gstride, D_, stream_);
+                              f.d_tile.as<GridTile>(), 1, d_gv.as<double2>(), f.gbase, f.gstride, D_, rt, stream_);
           } else if (ns.kind == kTrace2 && !f.ls_blocks.empty() && !lstat_off) {
             launch_trace_lstat(src(ns.left), np.d_Xt.as<long long>(), src(ns.right), np.d_Yt.as<long long>(),
                                f.d_lsb.as<LsBlock>(), int(f.ls_blocks.size()), f.d_lsc.as<LsChunk>(),
                                f.d_lst.as<LsTask>(), f.d_lsd.as<long long>(), d_gv.as<double2>(), f.gbase, f.gstride,
-                               ntime, D_, stream_);
+                               ntime, D_, rt, stream_);
           } else {
             launch_trace_groups(src(ns.left), np.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
                                 np.d_Yt.as<long long>(), np.d_Yq.as<long long>(), f.d_gitems.as<GroupItem>(),
                                 int(f.gitems.size()), f.d_gpairs.as<int2>(), d_gv.as<double2>(), f.gbase, f.gstride,
-                                ntime, D_, stream_);
+                                ntime, D_, rt, stream_);
           }
         }
       // resolve and accumulate per diagram

@@ -338,7 +338,7 @@ __global__ __launch_bounds__(64) void k_prod(const double2 *__restrict__ sX, con
                                              const long long *__restrict__ Xq, const double2 *__restrict__ sY,
                                              const long long *__restrict__ Yt, const long long *__restrict__ Yq,
                                              double2 *__restrict__ out, long long obase, long long ostride,
-                                             const ComboProd *__restrict__ combos, int D) {
+                                             const ComboProd *__restrict__ combos, int D, int transposed) {
   __shared__ double2 A[kMaxD * kMaxD], B[kMaxD * kMaxD];
   const ComboProd c = combos[blockIdx.x];
   const int ti = blockIdx.y;
@@ -355,20 +355,23 @@ __global__ __launch_bounds__(64) void k_prod(const double2 *__restrict__ sX, con
     const int r = e % D, col = e / D;
     double2 s = make_double2(0.0, 0.0);
     for (int k = 0; k < D; ++k) cfma(s, A[k * D + r], B[col * D + k]);
-    o[e] = s;
+    o[transposed ? col + r * D : e] = s;
   }
 }
 
-// D = 16 factor products on FP64 MFMA, computed transposed: C^T = Y^T X^T is
-// one 16 x 16 x 16 MFMA tile (4 k-steps x 4 real products).  With the MFMA
-// operand layout A[m = lane%16][k = lane/16], B[k = lane/16][n = lane%16] and
-// result element i of lane l at (l/16 + 4i, l%16), lane l reads X at l + 64s
-// (k-step s, coalesced), Y at (4s + l/16) + 16 (l%16) (64-byte runs), and
-// writes C at l + 64i (coalesced).  A wave computes kProdPerWave consecutive
-// combinations of one time index; consecutive combinations that share the
-// left matrix keep it in registers.
+// D = 16 factor products on FP64 MFMA: one 16 x 16 x 16 MFMA tile (4 k-steps x
+// 4 real products).  With the MFMA operand layout A[m = lane%16][k = lane/16],
+// B[k = lane/16][n = lane%16] and result element i of lane l at
+// (l/16 + 4i, l%16), lane l reads X at l + 64s (k-step s, coalesced) and Y at
+// (4s + l/16) + 16 (l%16) (64-byte runs), and writes element l + 64i of the
+// output (coalesced).  Computing C^T = Y^T X^T (Y as the A operand) makes that
+// store C in column-major order; computing C = X Y (X as the A operand) makes
+// it store C^T.  A wave computes kProdPerWave consecutive combinations of one
+// time index; consecutive combinations that share the left matrix keep it in
+// registers.
 constexpr int kProdPerWave = 4;
 
+template <bool kTransposed>
 __global__ __launch_bounds__(256) void k_prod16(const double2 *__restrict__ sX, const long long *__restrict__ Xt,
                                                 const long long *__restrict__ Xq, const double2 *__restrict__ sY,
                                                 const long long *__restrict__ Yt, const long long *__restrict__ Yq,
@@ -398,10 +401,17 @@ __global__ __launch_bounds__(256) void k_prod16(const double2 *__restrict__ sX, 
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       // C[r][c] = sum_k X[r][k] Y[k][c]: re = Xr Yr - Xi Yi, im = Xr Yi + Xi Yr
-      dr = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].x, xa[s].x, dr, 0, 0, 0);
-      dr = __builtin_amdgcn_mfma_f64_16x16x4f64(-yb[s].y, xa[s].y, dr, 0, 0, 0);
-      di = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].y, xa[s].x, di, 0, 0, 0);
-      di = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].x, xa[s].y, di, 0, 0, 0);
+      if (kTransposed) {
+        dr = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s].x, yb[s].x, dr, 0, 0, 0);
+        dr = __builtin_amdgcn_mfma_f64_16x16x4f64(-xa[s].y, yb[s].y, dr, 0, 0, 0);
+        di = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s].x, yb[s].y, di, 0, 0, 0);
+        di = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s].y, yb[s].x, di, 0, 0, 0);
+      } else {
+        dr = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].x, xa[s].x, dr, 0, 0, 0);
+        dr = __builtin_amdgcn_mfma_f64_16x16x4f64(-yb[s].y, xa[s].y, dr, 0, 0, 0);
+        di = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].y, xa[s].x, di, 0, 0, 0);
+        di = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].x, xa[s].y, di, 0, 0, 0);
+      }
     }
     double2 *o = out + (obase + ti * ostride + cb.o) * 256;
 #pragma unroll
@@ -429,7 +439,7 @@ __global__ __launch_bounds__(64 * kLsWaves) void k_trace_lstat(
     const double2 *__restrict__ sX, const long long *__restrict__ Xt, const double2 *__restrict__ sY,
     const long long *__restrict__ Yt, const LsBlock *__restrict__ blocks, int nblock, const LsChunk *__restrict__ chunks,
     const LsTask *__restrict__ tasks, const long long *__restrict__ slot_dst, double2 *__restrict__ out,
-    long long obase, long long ostride) {
+    long long obase, long long ostride, int rt) {
   constexpr int DD = D * D, EPL = DD / 64, CL = kLsStageBytes / (DD * 16);
   __shared__ double2 Ls[CL * DD];
   __shared__ double2 acc[kLsWaves][kLsSlots];
@@ -444,7 +454,7 @@ __global__ __launch_bounds__(64 * kLsWaves) void k_trace_lstat(
     __syncthreads();  // the previous chunk is no longer read
     for (int idx = tid; idx < C.n * DD; idx += 64 * kLsWaves) {
       const int m = idx / DD, f = idx % DD;
-      Ls[m * DD + f / D + (f % D) * D] = xb[(long long)(C.i0 + m) * DD + f];
+      Ls[m * DD + (rt ? f : f / D + (f % D) * D)] = xb[(long long)(C.i0 + m) * DD + f];
     }
     __syncthreads();
     const int t0 = C.wtask[wave], t1 = C.wtask[wave + 1];
@@ -513,7 +523,7 @@ __global__ __launch_bounds__(64) void k_trace_groups(
     const double2 *__restrict__ sX, const long long *__restrict__ Xt, const long long *__restrict__ Xq,
     const double2 *__restrict__ sY, const long long *__restrict__ Yt, const long long *__restrict__ Yq,
     const GroupItem *__restrict__ items, const int2 *__restrict__ pairs, double2 *__restrict__ out, long long obase,
-    long long ostride, int D) {
+    long long ostride, int D, int rt) {
   const GroupItem g = items[blockIdx.x];
   const int ti = blockIdx.y;
   const long long DD = (long long)D * D;
@@ -527,7 +537,7 @@ __global__ __launch_bounds__(64) void k_trace_groups(
       const double2 *y = yb + p.y * DD;
       for (int e = threadIdx.x; e < D * D; e += 64) {
         const int a = e % D, b = e / D;
-        const double2 u = x[e], v = y[a * D + b];
+        const double2 u = x[e], v = y[rt ? e : a * D + b];
         re = fma(u.x, v.x, re);
         re = fma(-u.y, v.y, re);
         im = fma(u.x, v.y, im);
@@ -551,55 +561,69 @@ __global__ __launch_bounds__(64) void k_trace_groups(
 // ---------------------------------------------------------------------------
 // Time-separable traces as small GEMMs over time on FP64 MFMA: for one
 // (trace id, mask group) and one tile of nL x nR slice pairs,
-//   out[a][b] = sum_combos sum_{x,y} L_a[x,y] R_b[y,x],
-// i.e. a 16 x 16 x (combos * D^2) complex GEMM whose K index runs over the
-// matrix elements.  Slabs of 128 elements of the 16 left and 16 right
-// matrices are staged in LDS (the right ones transposed on the way in), so
-// every loaded element feeds 16 outputs.  4 waves split the K steps and are
-// reduced through LDS at the end.
+//   out[a][b] = sum_(i,j) sum_{x,y} L_a,i[x,y] R_b,j[y,x],
+// a 16 x 16 x (pairs * D^2) complex GEMM whose K index runs over the matrix
+// elements.  The MFMA operands come straight from memory: for k-step s lane l
+// supplies element e = 4s + l/16 of L_{a = l%16} and the partner element of
+// R_{b = l%16} (index e itself when the right matrices are stored transposed,
+// so four lanes read 64 contiguous bytes of each matrix).  Pairs that share a
+// left matrix are merged first, sum_j tr(L_i R_j) = tr(L_i sum_j R_j): the
+// right elements of a run of (up to two) equal i are added in registers and
+// the run costs one pass of MFMAs.  Wave w takes the runs r with r % 4 == w;
+// the 4 partial tiles are reduced through LDS.
 // ---------------------------------------------------------------------------
-constexpr int TG_SE = 128;
+__device__ __forceinline__ void mfma_c(d4 &cr, d4 &ci, double2 u, double2 v) {
+  // (ux + i uy)(vx + i vy) = (ux vx - uy vy) + i (ux vy + uy vx)
+  cr = __builtin_amdgcn_mfma_f64_16x16x4f64(u.x, v.x, cr, 0, 0, 0);
+  cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-u.y, v.y, cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f64_16x16x4f64(u.x, v.y, ci, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f64_16x16x4f64(u.y, v.x, ci, 0, 0, 0);
+}
 
 __global__ __launch_bounds__(256) void k_trace_grid(const double2 *__restrict__ sX, const long long *__restrict__ Xq,
                                                     const double2 *__restrict__ sY, const long long *__restrict__ Yq,
                                                     const GroupItem *__restrict__ items, const int2 *__restrict__ pairs,
                                                     const GridTile *__restrict__ tiles, double2 *__restrict__ out,
-                                                    long long obase, long long ostride, int D) {
+                                                    long long obase, long long ostride, int D, int rt) {
   const GroupItem g = items[blockIdx.x];
   const GridTile &T = tiles[blockIdx.y];
-  __shared__ double2 Ls[kGridT][TG_SE + 1];
-  __shared__ double2 Rs[kGridT][TG_SE + 1];
+  __shared__ double2 part[4 * 256];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int DD = D * D;
-  const int SE = DD < TG_SE ? DD : TG_SE;
-  const int nslab = DD / SE, rows = SE / D;
-  const long long xq = Xq[g.qn], yq = Yq[g.qn];
+  const int DD = D * D, q = lane >> 4;
   const int nL = T.nL, nR = T.nR;
+  const int la = min(lane & 15, nL - 1), lb = min(lane & 15, nR - 1);  // rows past the tile repeat its last
+  const double2 *xrow = sX + (T.Lt[la] + Xq[g.qn]) * DD;
+  const double2 *yrow = sY + (T.Rt[lb] + Yq[g.qn]) * DD;
+  const int2 *pl = pairs + g.list_off;
   d4 cr = d4{0.0, 0.0, 0.0, 0.0}, ci = d4{0.0, 0.0, 0.0, 0.0};
-  for (int c = 0; c < g.n; ++c) {
-    const int2 p = pairs[g.list_off + c];
-    for (int h = 0; h < nslab; ++h) {
-      for (int idx = tid; idx < kGridT * SE; idx += 256) {
-        const int a = idx / SE, e = idx % SE;
-        const int x = e % D, yy = e / D;
-        Ls[a][e] = a < nL ? sX[(T.Lt[a] + xq + p.x) * DD + (long long)h * SE + e] : make_double2(0.0, 0.0);
-        Rs[a][e] = a < nR ? sY[(T.Rt[a] + yq + p.y) * DD + (long long)x * D + h * rows + yy] : make_double2(0.0, 0.0);
+  int run = 0;
+  for (int c = 0; c < g.n; ++run) {
+    const int i = pl[c].x;
+    const bool two = c + 1 < g.n && pl[c + 1].x == i;
+    if (run % 4 == wave) {
+      const double2 *x = xrow + (long long)i * DD + q;
+      const double2 *y0 = yrow + (long long)pl[c].y * DD;
+      const double2 *y1 = yrow + (long long)pl[two ? c + 1 : c].y * DD;
+      if (rt) {
+        y0 += q;
+        y1 += q;
+        if (two) {
+#pragma unroll 4
+          for (int s = 0; s < DD / 4; ++s) mfma_c(cr, ci, x[4 * s], cadd(y0[4 * s], y1[4 * s]));
+        } else {
+#pragma unroll 4
+          for (int s = 0; s < DD / 4; ++s) mfma_c(cr, ci, x[4 * s], y0[4 * s]);
+        }
+      } else {
+        for (int s = 0; s < DD / 4; ++s) {
+          const int e = 4 * s + q, er = (e % D) * D + e / D;
+          mfma_c(cr, ci, x[4 * s], two ? cadd(y0[er], y1[er]) : y0[er]);
+        }
       }
-      __syncthreads();
-      for (int ks = wave; ks < SE / 4; ks += 4) {
-        const int e = ks * 4 + (lane >> 4);
-        const double2 u = Ls[lane & 15][e], v = Rs[lane & 15][e];
-        // (ux + i uy)(vx + i vy) = (ux vx - uy vy) + i (ux vy + uy vx)
-        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(u.x, v.x, cr, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-u.y, v.y, cr, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(u.x, v.y, ci, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(u.y, v.x, ci, 0, 0, 0);
-      }
-      __syncthreads();
     }
+    c += two ? 2 : 1;
   }
-  // reduce the 4 waves' partial 16 x 16 tiles (reusing the left staging area)
-  double2 *part = &Ls[0][0];
+  // reduce the 4 waves' partial 16 x 16 tiles
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int a = (lane >> 4) + 4 * i, b = lane & 15;
@@ -819,17 +843,22 @@ void launch_identity(double2 *out, int N, int count, hipStream_t s) {
 
 void launch_prod(const double2 *sX, const long long *Xt, const long long *Xq, const double2 *sY, const long long *Yt,
                  const long long *Yq, double2 *out, long long obase, long long ostride, const ComboProd *combos,
-                 int ncombo, int ntime, int D, hipStream_t s) {
+                 int ncombo, int ntime, int D, int transposed, hipStream_t s) {
   if (ncombo <= 0 || ntime <= 0) return;
   if (D > kMaxD) throw std::runtime_error("factor dimension above 32 is not supported");
   if (ntime > 65535) throw std::runtime_error("launch_prod: too many time keys");
   if (D == 16 && !std::getenv("SLAPH_PROD_GENERIC")) {
     const int per_block = 4 * kProdPerWave;
-    hipLaunchKernelGGL(k_prod16, dim3((ncombo + per_block - 1) / per_block, ntime), dim3(256), 0, s, sX, Xt, Xq, sY,
-                       Yt, Yq, out, obase, ostride, combos, ncombo);
+    const dim3 grid((ncombo + per_block - 1) / per_block, ntime);
+    if (transposed)
+      hipLaunchKernelGGL(k_prod16<true>, grid, dim3(256), 0, s, sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos,
+                         ncombo);
+    else
+      hipLaunchKernelGGL(k_prod16<false>, grid, dim3(256), 0, s, sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos,
+                         ncombo);
   } else {
     hipLaunchKernelGGL(k_prod, dim3(ncombo, ntime), dim3(64), 0, s, sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride,
-                       combos, D);
+                       combos, D, transposed);
   }
   SLAPH_HIP(hipGetLastError());
 }
@@ -844,26 +873,26 @@ int trace_lstat_chunk(int D) {
 void launch_trace_lstat(const double2 *sX, const long long *Xt, const double2 *sY, const long long *Yt,
                         const LsBlock *blocks, int nblock, const LsChunk *chunks, const LsTask *tasks,
                         const long long *slot_dst, double2 *out, long long obase, long long ostride, int ntime, int D,
-                        hipStream_t s) {
+                        int rt, hipStream_t s) {
   if (nblock <= 0 || ntime <= 0) return;
   if ((long long)nblock * ntime > 0x7fffffffLL) throw std::runtime_error("launch_trace_lstat: grid too large");
   const dim3 grid(unsigned(nblock * ntime)), block(64 * kLsWaves);
   switch (D) {
     case 8:
       hipLaunchKernelGGL(k_trace_lstat<8>, grid, block, 0, s, sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst,
-                         out, obase, ostride);
+                         out, obase, ostride, rt);
       break;
     case 16:
       hipLaunchKernelGGL(k_trace_lstat<16>, grid, block, 0, s, sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst,
-                         out, obase, ostride);
+                         out, obase, ostride, rt);
       break;
     case 24:
       hipLaunchKernelGGL(k_trace_lstat<24>, grid, block, 0, s, sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst,
-                         out, obase, ostride);
+                         out, obase, ostride, rt);
       break;
     case 32:
       hipLaunchKernelGGL(k_trace_lstat<32>, grid, block, 0, s, sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst,
-                         out, obase, ostride);
+                         out, obase, ostride, rt);
       break;
     default:
       throw std::runtime_error("launch_trace_lstat: unsupported factor dimension");
@@ -873,23 +902,23 @@ void launch_trace_lstat(const double2 *sX, const long long *Xt, const double2 *s
 
 void launch_trace_groups(const double2 *sX, const long long *Xt, const long long *Xq, const double2 *sY,
                          const long long *Yt, const long long *Yq, const GroupItem *items, int nitems, const int2 *pairs,
-                         double2 *out, long long obase, long long ostride, int ntime, int D, hipStream_t s) {
+                         double2 *out, long long obase, long long ostride, int ntime, int D, int rt,
+                         hipStream_t s) {
   if (nitems <= 0 || ntime <= 0) return;
   if (ntime > 65535) throw std::runtime_error("launch_trace_groups: too many slice pairs");
   hipLaunchKernelGGL(k_trace_groups, dim3(nitems, ntime), dim3(64), 0, s, sX, Xt, Xq, sY, Yt, Yq, items, pairs, out,
-                     obase, ostride, D);
+                     obase, ostride, D, rt);
   SLAPH_HIP(hipGetLastError());
 }
 
 void launch_trace_grid(const double2 *sX, const long long *Xq, const double2 *sY, const long long *Yq,
                        const GroupItem *items, int nitems, const int2 *pairs, const GridTile *tiles, int ntiles,
-                       double2 *out, long long obase, long long ostride, int D, hipStream_t s) {
+                       double2 *out, long long obase, long long ostride, int D, int rt, hipStream_t s) {
   if (nitems <= 0 || ntiles <= 0) return;
-  if (D * D % (D * D < TG_SE ? D * D : TG_SE) != 0 || D > 32)
-    throw std::runtime_error("launch_trace_grid: unsupported factor dimension");
+  if (D % 4 != 0 || D > kMaxD) throw std::runtime_error("launch_trace_grid: unsupported factor dimension");
   if (ntiles > 65535) throw std::runtime_error("launch_trace_grid: too many tiles");
   hipLaunchKernelGGL(k_trace_grid, dim3(nitems, ntiles), dim3(256), 0, s, sX, Xq, sY, Yq, items, pairs, tiles, out,
-                     obase, ostride, D);
+                     obase, ostride, D, rt);
   SLAPH_HIP(hipGetLastError());
 }
 

@@ -76,13 +76,16 @@ void launch_identity(double2 *out, int N, int count, hipStream_t s);
 
 // Table-driven batched factor product / trace kernels.
 // Operand matrix index = Xt[ti] + Xq[qn] + i; output index = obase + ti*ostride + o.
+// transposed != 0 stores each product as (X Y)^T.
 struct ComboProd {
   int qn, i, j;
   long long o;
 };
 void launch_prod(const double2 *slabX, const long long *Xt, const long long *Xq, const double2 *slabY,
                  const long long *Yt, const long long *Yq, double2 *out, long long obase, long long ostride,
-                 const ComboProd *combos, int ncombo, int ntime, int D, hipStream_t s);
+                 const ComboProd *combos, int ncombo, int ntime, int D, int transposed, hipStream_t s);
+// In the trace launchers below, rt != 0 means the right matrices are stored
+// transposed (Y^T), so that tr(X Y) = sum_e X[e] Y^T[e].
 
 // Correlator resolution: mean / 2-way / 3-way masked inner products.
 // A trace vector seen through its used-index mask groups: the trace values of
@@ -109,11 +112,13 @@ struct GroupItem {
 };
 void launch_trace_groups(const double2 *slabX, const long long *Xt, const long long *Xq, const double2 *slabY,
                          const long long *Yt, const long long *Yq, const GroupItem *items, int nitems, const int2 *pairs,
-                         double2 *out, long long obase, long long ostride, int ntime, int D, hipStream_t s);
+                         double2 *out, long long obase, long long ostride, int ntime, int D, int rt,
+                         hipStream_t s);
 // Time-separable traces: within one tile of slice pairs the left operand
 // depends on one time index a < nL and the right one on b < nR, and every
 // (a, b) is a slice pair sidx[a][b] (or -1).  Output per (item, tile):
 // out[obase + sidx[a][b]*ostride + dst] = sum_combos tr(L_a,i R_b,j).
+// An item's pairs must be sorted by their left index i.
 constexpr int kGridT = 16;
 struct GridTile {
   long long Lt[kGridT];
@@ -123,7 +128,7 @@ struct GridTile {
 };
 void launch_trace_grid(const double2 *slabX, const long long *Xq, const double2 *slabY, const long long *Yq,
                        const GroupItem *items, int nitems, const int2 *pairs, const GridTile *tiles, int ntiles,
-                       double2 *out, long long obase, long long ostride, int D, hipStream_t s);
+                       double2 *out, long long obase, long long ostride, int D, int rt, hipStream_t s);
 // Non-separable traces, left-stationary.  A block is a set of (trace id,
 // mask group) slots that share one left operator entry (left matrices
 // X[Xt[ti] + xq + i]); its chunks stage left combinations [i0, i0 + n) and
@@ -153,7 +158,7 @@ bool trace_lstat_supported(int D);
 void launch_trace_lstat(const double2 *slabX, const long long *Xt, const double2 *slabY, const long long *Yt,
                         const LsBlock *blocks, int nblock, const LsChunk *chunks, const LsTask *tasks,
                         const long long *slot_dst, double2 *out, long long obase, long long ostride, int ntime, int D,
-                        hipStream_t s);
+                        int rt, hipStream_t s);
 void launch_resolve(const double2 *gvals, const uint16_t *gmasks, const CorrJob *jobs, int ncorr, int nsp,
                     double2 *cval, hipStream_t s);
 void launch_accumulate(const double2 *cval, const int *tdelta, int nsp, int ncorr, double2 *sum, double2 *comp,
