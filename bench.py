@@ -56,6 +56,19 @@ correlator_list = C6cD:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0
 """.format(**w)
 
 
+def vdv_traffic(workload):
+    """HBM bytes of one VdaggerV launch, measured with two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE, copy-calibrated) and kept under profiles/; None
+    for a workload without such a measurement."""
+    path = os.path.join(ROOT, "profiles", "k_vdv_%s_traffic.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("bytes_per_launch"):
+            return float(line.split()[1])
+    return None
+
+
 def cpu_baseline(w, text, max_seconds=30.0):
     """Numpy oracle on a bounded sample of the same workload, extrapolated to
     one configuration: one VdaggerV product, Q0 of one timeslice, Q2 of one
@@ -193,7 +206,8 @@ def main():
                       "contract_incl_factors": stage["contract"]},
         "roofline": {"bound": "mfma", "kernel": "k_vdv (VdaggerV)", "achieved": achieved,
                      "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA,
-                     "traffic": None, "algorithmic_flops_per_launch": flops, "avg_launch_ms": kern_ms},
+                     "traffic": vdv_traffic(args.workload), "traffic_unit": "bytes per launch (HBM, rocprofv3 --pmc)",
+                     "algorithmic_flops_per_launch": flops, "avg_launch_ms": kern_ms},
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, text)
