@@ -395,7 +395,41 @@ std::vector<int> trace_times(Family const &f, int tso, int tsi) {
   return tt;
 }
 
+// Time part of one chunk of slice pairs, per product/trace node that owns a
+// table: where its table starts in the chunk's product buffer, its time
+// count, and its operands' time bases (uploaded once).
+struct NodeChunk {
+  long long base = 0;
+  int ntime = 0;
+  std::vector<long long> h_Xt, h_Yt;
+  DevBuf d_Xt, d_Yt;
+  bool grid = false;  // the trace runs as one slice-pair tile
+  DevBuf d_tile;
+};
+struct ChunkPlan {
+  size_t bp = 0, c0 = 0;  // block pair, first slice pair
+  int S = 0;
+  long long prod_total = 0, g_total = 0;
+  std::vector<long long> gbase;               // per family: offset of its group sums
+  std::vector<std::vector<NodeChunk>> nodes;  // per family, per node
+  DevBuf d_td;                                // time delta per slice pair
+  std::vector<DevBuf> d_jobs;                 // per diagram: resolve jobs
+};
+
 }  // namespace
+
+struct ContractCache {
+  int single_block_pair = -2;
+  std::vector<Family> fams;
+  std::vector<CorrPlan> corrs;
+  std::vector<std::vector<SlicePair>> bp_slices;
+  std::vector<uint16_t> masks, gmasks;
+  std::vector<std::vector<int>> need1, need2;
+  bool post = false;  // mask groups, mirrors and normalisations are planned
+  DevBuf d_gmasks;
+  std::vector<ChunkPlan> chunks;
+  DevBuf d_prod, d_gv, d_cval;
+};
 
 void Engine::contract(int single_block_pair, int kahan) {
   require_inputs();
@@ -405,236 +439,257 @@ void Engine::contract(int single_block_pair, int kahan) {
   const int dilT = gd_.dilT();
   auto B = [dilT](int t) { return t / dilT; };  // DilutionScheme::time_to_block, block type
 
-  // ---- families and correlators (time independent) ----
-  std::vector<Family> fams;
-  std::map<std::pair<std::string, std::vector<Loc>>, int> fam_of;
-  std::vector<CorrPlan> corrs;
-  for (auto const &kv : gd_.correlator_requests_map) {
-    if (kv.second.empty()) continue;
-    CorrPlan cp;
-    cp.name = kv.first;
-    for (auto const &req : kv.second) {
-      if (req.traces.empty() || req.traces.size() > 3) throw std::runtime_error("This many traces are not implemented yet.");
-      std::vector<std::pair<int, int>> refs;
-      for (auto const &tr : req.traces) {
-        auto key = std::make_pair(tr.name, tr.locs);
-        auto it = fam_of.find(key);
-        if (it == fam_of.end()) {
-          Family f;
-          f.name = tr.name;
-          f.locs = tr.locs;
-          f.shape = make_shape(tr.name);
-          if (int(f.locs.size()) != f.shape.ntimes) throw std::logic_error("trace time count mismatch");
-          fams.push_back(std::move(f));
-          it = fam_of.emplace(key, int(fams.size()) - 1).first;
-        }
-        Family &f = fams[it->second];
-        auto jt = f.trl_of.find(tr.tr_id);
-        if (jt == f.trl_of.end()) {
-          f.tr_ids.push_back(tr.tr_id);
-          jt = f.trl_of.emplace(tr.tr_id, int(f.tr_ids.size()) - 1).first;
-        }
-        refs.push_back({it->second, jt->second});
-      }
-      cp.reqs.push_back(refs);
-      cp.datasets.push_back(req.dataset);
+  // The plan depends on the input file only; it is built on the first call
+  // (or after configure / a budget change) and reused for every later gauge
+  // configuration.
+  const bool fresh = !plan_ || plan_->single_block_pair != single_block_pair;
+  if (fresh) {
+    plan_ = std::make_shared<ContractCache>();
+    plan_->single_block_pair = single_block_pair;
+  }
+  // a call that throws leaves no half-built plan behind
+  struct PlanGuard {
+    std::shared_ptr<ContractCache> &p;
+    bool done = false;
+    ~PlanGuard() {
+      if (!done) p.reset();
     }
-    corrs.push_back(std::move(cp));
-  }
-
-  // ---- the slice pairs of every block pair (Correlators.cpp:84-127) ----
-  auto bps = block_pairs(Lt_, dilT);
-  std::vector<std::vector<SlicePair>> bp_slices;
-  for (size_t b = 0; b < bps.size(); ++b) {
-    if (single_block_pair >= 0 && int(b) != single_block_pair) continue;
-    bp_slices.push_back(slice_pairs(bps[b].first, bps[b].second, Lt_, dilT));
-  }
-  // (factor type, time key) of a leaf at trace times tt
-  auto leaf_key = [&](LeafSpec const &ls, std::vector<int> const &tt) {
-    if (ls.q == 0) return std::array<int, 4>{0, tt[ls.a], 0, 0};
-    if (ls.q == 1) return std::array<int, 4>{1, tt[ls.a], B(tt[ls.b]), 0};
-    return std::array<int, 4>{2, B(tt[ls.a]), tt[ls.b], B(tt[ls.c])};
-  };
-  // Do the left and right operands of the family's trace read the same
-  // factors at every slice pair?
-  auto operands_coincide = [&](Family const &f) {
-    auto const &rs = f.shape.nodes.back();
-    if (rs.kind != kTrace2) return false;
-    std::function<void(int, std::vector<int> &)> leaves_of = [&](int n, std::vector<int> &out) {
-      auto const &ns = f.shape.nodes[n];
-      if (ns.kind == kLeaf) {
-        out.push_back(ns.leaf);
-        return;
-      }
-      leaves_of(ns.left, out);
-      if (ns.right >= 0) leaves_of(ns.right, out);
-    };
-    std::vector<int> lL, lR;
-    leaves_of(rs.left, lL);
-    leaves_of(rs.right, lR);
-    if (lL.size() != lR.size()) return false;
-    for (auto const &sl : bp_slices)
-      for (auto const &sp : sl) {
-        auto tt = trace_times(f, sp.source, sp.sink);
-        for (size_t k = 0; k < lL.size(); ++k)
-          if (leaf_key(f.shape.leaves[lL[k]], tt) != leaf_key(f.shape.leaves[lR[k]], tt)) return false;
-      }
-    return true;
-  };
-
-  // ---- quantum-number planning per family ----
+  } guard{plan_};
+  ContractCache &C = *plan_;
+  auto &fams = C.fams;
+  auto &corrs = C.corrs;
+  auto &bp_slices = C.bp_slices;
+  auto &masks = C.masks;
+  auto &gmasks = C.gmasks;
   compute_q_offsets();
-  static const char *qname[3] = {"Q0", "Q1", "Q2"};
-  Templates tm;
-  std::vector<uint16_t> masks;
-  for (auto &f : fams) {
-    auto const &cl = gd_.trace_indices_map.at(f.name);
-    f.nodes.resize(f.shape.nodes.size());
-    f.coincide = operands_coincide(f);
-    // Product nodes with the same subtree shape (same factor types in the same
-    // places) share one product table: an alias keeps its own time/qn index
-    // maps into its owner's storage, so an instance (time key, operator
-    // entries) is computed once however many nodes read it.  Exception: when
-    // the trace's operands do not coincide, a product that is the trace's
-    // right operand keeps a table of its own, stored transposed, so that
-    // tr(X Y) = sum_e X[e] Y^T[e] reads both operands in memory order.
-    std::vector<std::string> sig(f.shape.nodes.size());
-    const int rroot = f.shape.nodes.back().kind == kTrace2 ? f.shape.nodes.back().right : -1;
-    for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
-      auto const &ns = f.shape.nodes[n];
-      if (ns.kind == kLeaf) sig[n] = "L" + std::to_string(f.shape.leaves[ns.leaf].q);
-      else sig[n] = "P(" + sig[ns.left] + "," + (ns.right >= 0 ? sig[ns.right] : "") + ")";
-      f.nodes[n].owner = int(n);
-      f.nodes[n].transposed = false;
-      if (ns.kind != kProd) continue;
-      if (!f.coincide && int(n) == rroot) {
-        f.nodes[n].transposed = true;
-        continue;
-      }
-      for (size_t m = 0; m < n; ++m)
-        if (f.shape.nodes[m].kind == kProd && sig[m] == sig[n] && !f.nodes[m].transposed) {
-          f.nodes[n].owner = f.nodes[m].owner;
-          break;
+  if (fresh) {
+    // ---- families and correlators (time independent) ----
+    std::map<std::pair<std::string, std::vector<Loc>>, int> fam_of;
+    for (auto const &kv : gd_.correlator_requests_map) {
+      if (kv.second.empty()) continue;
+      CorrPlan cp;
+      cp.name = kv.first;
+      for (auto const &req : kv.second) {
+        if (req.traces.empty() || req.traces.size() > 3) throw std::runtime_error("This many traces are not implemented yet.");
+        std::vector<std::pair<int, int>> refs;
+        for (auto const &tr : req.traces) {
+          auto key = std::make_pair(tr.name, tr.locs);
+          auto it = fam_of.find(key);
+          if (it == fam_of.end()) {
+            Family f;
+            f.name = tr.name;
+            f.locs = tr.locs;
+            f.shape = make_shape(tr.name);
+            if (int(f.locs.size()) != f.shape.ntimes) throw std::logic_error("trace time count mismatch");
+            fams.push_back(std::move(f));
+            it = fam_of.emplace(key, int(fams.size()) - 1).first;
+          }
+          Family &f = fams[it->second];
+          auto jt = f.trl_of.find(tr.tr_id);
+          if (jt == f.trl_of.end()) {
+            f.tr_ids.push_back(tr.tr_id);
+            jt = f.trl_of.emplace(tr.tr_id, int(f.tr_ids.size()) - 1).first;
+          }
+          refs.push_back({it->second, jt->second});
         }
+        cp.reqs.push_back(refs);
+        cp.datasets.push_back(req.dataset);
+      }
+      corrs.push_back(std::move(cp));
     }
-    for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
-      auto const &ns = f.shape.nodes[n];
-      NodePlan &self = f.nodes[n];
-      NodePlan &np = f.nodes[self.owner];
-      bool root = n + 1 == f.shape.nodes.size();
-      self.trl_qn.resize(f.tr_ids.size());
-      size_t nqn_before = np.qn_key.size();
-      for (size_t trl = 0; trl < f.tr_ids.size(); ++trl) {
-        auto const &c = cl.at(f.tr_ids[trl]);
+
+    // ---- the slice pairs of every block pair (Correlators.cpp:84-127) ----
+    auto bps = block_pairs(Lt_, dilT);
+    for (size_t b = 0; b < bps.size(); ++b) {
+      if (single_block_pair >= 0 && int(b) != single_block_pair) continue;
+      bp_slices.push_back(slice_pairs(bps[b].first, bps[b].second, Lt_, dilT));
+    }
+    // (factor type, time key) of a leaf at trace times tt
+    auto leaf_key = [&](LeafSpec const &ls, std::vector<int> const &tt) {
+      if (ls.q == 0) return std::array<int, 4>{0, tt[ls.a], 0, 0};
+      if (ls.q == 1) return std::array<int, 4>{1, tt[ls.a], B(tt[ls.b]), 0};
+      return std::array<int, 4>{2, B(tt[ls.a]), tt[ls.b], B(tt[ls.c])};
+    };
+    // Do the left and right operands of the family's trace read the same
+    // factors at every slice pair?
+    auto operands_coincide = [&](Family const &f) {
+      auto const &rs = f.shape.nodes.back();
+      if (rs.kind != kTrace2) return false;
+      std::function<void(int, std::vector<int> &)> leaves_of = [&](int n, std::vector<int> &out) {
+        auto const &ns = f.shape.nodes[n];
         if (ns.kind == kLeaf) {
-          auto const &ls = f.shape.leaves[ns.leaf];
-          self.trl_qn[trl] = c.at(ls.cidx);  // leaf qn = operator entry
+          out.push_back(ns.leaf);
+          return;
+        }
+        leaves_of(ns.left, out);
+        if (ns.right >= 0) leaves_of(ns.right, out);
+      };
+      std::vector<int> lL, lR;
+      leaves_of(rs.left, lL);
+      leaves_of(rs.right, lR);
+      if (lL.size() != lR.size()) return false;
+      for (auto const &sl : bp_slices)
+        for (auto const &sp : sl) {
+          auto tt = trace_times(f, sp.source, sp.sink);
+          for (size_t k = 0; k < lL.size(); ++k)
+            if (leaf_key(f.shape.leaves[lL[k]], tt) != leaf_key(f.shape.leaves[lR[k]], tt)) return false;
+        }
+      return true;
+    };
+
+    // ---- quantum-number planning per family ----
+    static const char *qname[3] = {"Q0", "Q1", "Q2"};
+    Templates tm;
+    for (auto &f : fams) {
+      auto const &cl = gd_.trace_indices_map.at(f.name);
+      f.nodes.resize(f.shape.nodes.size());
+      f.coincide = operands_coincide(f);
+      // Product nodes with the same subtree shape (same factor types in the same
+      // places) share one product table: an alias keeps its own time/qn index
+      // maps into its owner's storage, so an instance (time key, operator
+      // entries) is computed once however many nodes read it.  Exception: when
+      // the trace's operands do not coincide, a product that is the trace's
+      // right operand keeps a table of its own, stored transposed, so that
+      // tr(X Y) = sum_e X[e] Y^T[e] reads both operands in memory order.
+      std::vector<std::string> sig(f.shape.nodes.size());
+      const int rroot = f.shape.nodes.back().kind == kTrace2 ? f.shape.nodes.back().right : -1;
+      for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+        auto const &ns = f.shape.nodes[n];
+        if (ns.kind == kLeaf) sig[n] = "L" + std::to_string(f.shape.leaves[ns.leaf].q);
+        else sig[n] = "P(" + sig[ns.left] + "," + (ns.right >= 0 ? sig[ns.right] : "") + ")";
+        f.nodes[n].owner = int(n);
+        f.nodes[n].transposed = false;
+        if (ns.kind != kProd) continue;
+        if (!f.coincide && int(n) == rroot) {
+          f.nodes[n].transposed = true;
           continue;
         }
-        std::vector<int> key;
-        auto append = [&](int child) {
-          auto const &cs = f.shape.nodes[child];
-          if (cs.kind == kLeaf) key.push_back(c.at(f.shape.leaves[cs.leaf].cidx));
-          else {
-            NodePlan const &co = f.nodes[f.nodes[child].owner];
-            auto const &k2 = co.qn_key[f.nodes[child].trl_qn[trl]];
-            key.insert(key.end(), k2.begin(), k2.end());
+        for (size_t m = 0; m < n; ++m)
+          if (f.shape.nodes[m].kind == kProd && sig[m] == sig[n] && !f.nodes[m].transposed) {
+            f.nodes[n].owner = f.nodes[m].owner;
+            break;
           }
+      }
+      for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+        auto const &ns = f.shape.nodes[n];
+        NodePlan &self = f.nodes[n];
+        NodePlan &np = f.nodes[self.owner];
+        bool root = n + 1 == f.shape.nodes.size();
+        self.trl_qn.resize(f.tr_ids.size());
+        size_t nqn_before = np.qn_key.size();
+        for (size_t trl = 0; trl < f.tr_ids.size(); ++trl) {
+          auto const &c = cl.at(f.tr_ids[trl]);
+          if (ns.kind == kLeaf) {
+            auto const &ls = f.shape.leaves[ns.leaf];
+            self.trl_qn[trl] = c.at(ls.cidx);  // leaf qn = operator entry
+            continue;
+          }
+          std::vector<int> key;
+          auto append = [&](int child) {
+            auto const &cs = f.shape.nodes[child];
+            if (cs.kind == kLeaf) key.push_back(c.at(f.shape.leaves[cs.leaf].cidx));
+            else {
+              NodePlan const &co = f.nodes[f.nodes[child].owner];
+              auto const &k2 = co.qn_key[f.nodes[child].trl_qn[trl]];
+              key.insert(key.end(), k2.begin(), k2.end());
+            }
+          };
+          append(ns.left);
+          if (ns.right >= 0) append(ns.right);
+          if (root) key.push_back(int(trl));  // every trl is its own trace output
+          auto it = np.qn_index.find(key);
+          if (it == np.qn_index.end()) {
+            it = np.qn_index.emplace(key, int(np.qn_key.size())).first;
+            np.qn_key.push_back(key);
+            np.qn_left.push_back(f.nodes[ns.left].trl_qn[trl]);
+            np.qn_right.push_back(ns.right >= 0 ? f.nodes[ns.right].trl_qn[trl] : 0);
+          }
+          self.trl_qn[trl] = it->second;
+        }
+        if (ns.kind == kLeaf) continue;
+        if (self.owner != int(n) && np.qn_key.size() == nqn_before) continue;  // alias, nothing new
+        // templates and combos (of the owner, over all its qn entries)
+        auto const &os = f.shape.nodes[self.owner];
+        auto tmpl_of = [&](int child, int cqn) {
+          auto const &cs = f.shape.nodes[child];
+          if (cs.kind == kLeaf) {
+            auto const &ls = f.shape.leaves[cs.leaf];
+            return tm.leaf(gd_.quarkline_lookuptable.at(qname[ls.q]).at(cqn).rnd);
+          }
+          return f.nodes[f.nodes[child].owner].qn_tmpl[cqn];
         };
-        append(ns.left);
-        if (ns.right >= 0) append(ns.right);
-        if (root) key.push_back(int(trl));  // every trl is its own trace output
-        auto it = np.qn_index.find(key);
-        if (it == np.qn_index.end()) {
-          it = np.qn_index.emplace(key, int(np.qn_key.size())).first;
-          np.qn_key.push_back(key);
-          np.qn_left.push_back(f.nodes[ns.left].trl_qn[trl]);
-          np.qn_right.push_back(ns.right >= 0 ? f.nodes[ns.right].trl_qn[trl] : 0);
-        }
-        self.trl_qn[trl] = it->second;
-      }
-      if (ns.kind == kLeaf) continue;
-      if (self.owner != int(n) && np.qn_key.size() == nqn_before) continue;  // alias, nothing new
-      // templates and combos (of the owner, over all its qn entries)
-      auto const &os = f.shape.nodes[self.owner];
-      auto tmpl_of = [&](int child, int cqn) {
-        auto const &cs = f.shape.nodes[child];
-        if (cs.kind == kLeaf) {
-          auto const &ls = f.shape.leaves[cs.leaf];
-          return tm.leaf(gd_.quarkline_lookuptable.at(qname[ls.q]).at(cqn).rnd);
-        }
-        return f.nodes[f.nodes[child].owner].qn_tmpl[cqn];
-      };
-      np.qn_off.assign(np.qn_key.size() + 1, 0);
-      np.qn_tmpl.assign(np.qn_key.size(), -1);
-      np.combos.clear();
-      np.cmaxX = np.cmaxY = np.cmaxO = 0;
-      if (root) {
-        f.mask_off.assign(f.tr_ids.size(), 0);
-        f.ncomb.assign(f.tr_ids.size(), 0);
-      }
-      for (size_t q = 0; q < np.qn_key.size(); ++q) {
-        int lt = tmpl_of(os.left, np.qn_left[q]);
-        PairTmpl const *pt;
-        if (ns.kind == kProd) {
-          pt = &tm.product(lt, tmpl_of(os.right, np.qn_right[q]));
-          np.qn_tmpl[q] = pt->out;
-        } else if (ns.kind == kTrace2) {
-          pt = &tm.trace2(lt, tmpl_of(os.right, np.qn_right[q]));
-        } else {
-          pt = &tm.trace1(lt);
-        }
-        for (size_t k = 0; k < pt->combos.size(); ++k)
-          np.combos.push_back({int(q), pt->combos[k][0], pt->combos[k][1], np.qn_off[q] + (long long)k});
-        np.qn_off[q + 1] = np.qn_off[q] + (long long)pt->combos.size();
+        np.qn_off.assign(np.qn_key.size() + 1, 0);
+        np.qn_tmpl.assign(np.qn_key.size(), -1);
+        np.combos.clear();
+        np.cmaxX = np.cmaxY = np.cmaxO = 0;
         if (root) {
-          int trl = np.qn_key[q].back();
-          f.mask_off[trl] = int(masks.size());
-          f.ncomb[trl] = int(pt->combos.size());
-          masks.insert(masks.end(), pt->masks.begin(), pt->masks.end());
+          f.mask_off.assign(f.tr_ids.size(), 0);
+          f.ncomb.assign(f.tr_ids.size(), 0);
         }
+        for (size_t q = 0; q < np.qn_key.size(); ++q) {
+          int lt = tmpl_of(os.left, np.qn_left[q]);
+          PairTmpl const *pt;
+          if (ns.kind == kProd) {
+            pt = &tm.product(lt, tmpl_of(os.right, np.qn_right[q]));
+            np.qn_tmpl[q] = pt->out;
+          } else if (ns.kind == kTrace2) {
+            pt = &tm.trace2(lt, tmpl_of(os.right, np.qn_right[q]));
+          } else {
+            pt = &tm.trace1(lt);
+          }
+          for (size_t k = 0; k < pt->combos.size(); ++k)
+            np.combos.push_back({int(q), pt->combos[k][0], pt->combos[k][1], np.qn_off[q] + (long long)k});
+          np.qn_off[q + 1] = np.qn_off[q] + (long long)pt->combos.size();
+          if (root) {
+            int trl = np.qn_key[q].back();
+            f.mask_off[trl] = int(masks.size());
+            f.ncomb[trl] = int(pt->combos.size());
+            masks.insert(masks.end(), pt->masks.begin(), pt->masks.end());
+          }
+        }
+        np.stride = np.qn_off.back();
+        // operand qn offsets: leaf child -> first pair of the operator entry;
+        // product child -> its qn offset
+        std::vector<long long> Xq(np.qn_key.size()), Yq(np.qn_key.size(), 0);
+        auto off_of = [&](int child, int cqn) -> long long {
+          auto const &cs = f.shape.nodes[child];
+          if (cs.kind == kLeaf) return q_off_[f.shape.leaves[cs.leaf].q].at(cqn);
+          return f.nodes[f.nodes[child].owner].qn_off[cqn];
+        };
+        for (size_t q = 0; q < np.qn_key.size(); ++q) {
+          Xq[q] = off_of(os.left, np.qn_left[q]);
+          if (os.right >= 0) Yq[q] = off_of(os.right, np.qn_right[q]);
+        }
+        for (auto const &cb : np.combos) {
+          np.cmaxX = std::max(np.cmaxX, Xq[cb.qn] + cb.i);
+          np.cmaxY = std::max(np.cmaxY, Yq[cb.qn] + cb.j);
+          np.cmaxO = std::max(np.cmaxO, cb.o);
+        }
+        upload(np.d_Xq, Xq);
+        upload(np.d_Yq, Yq);
+        upload(np.d_combos, np.combos);
+        np.h_Xq = std::move(Xq);
+        np.h_Yq = std::move(Yq);
       }
-      np.stride = np.qn_off.back();
-      // operand qn offsets: leaf child -> first pair of the operator entry;
-      // product child -> its qn offset
-      std::vector<long long> Xq(np.qn_key.size()), Yq(np.qn_key.size(), 0);
-      auto off_of = [&](int child, int cqn) -> long long {
-        auto const &cs = f.shape.nodes[child];
-        if (cs.kind == kLeaf) return q_off_[f.shape.leaves[cs.leaf].q].at(cqn);
-        return f.nodes[f.nodes[child].owner].qn_off[cqn];
-      };
-      for (size_t q = 0; q < np.qn_key.size(); ++q) {
-        Xq[q] = off_of(os.left, np.qn_left[q]);
-        if (os.right >= 0) Yq[q] = off_of(os.right, np.qn_right[q]);
-      }
-      for (auto const &cb : np.combos) {
-        np.cmaxX = std::max(np.cmaxX, Xq[cb.qn] + cb.i);
-        np.cmaxY = std::max(np.cmaxY, Yq[cb.qn] + cb.j);
-        np.cmaxO = std::max(np.cmaxO, cb.o);
-      }
-      upload(np.d_Xq, Xq);
-      upload(np.d_Yq, Yq);
-      upload(np.d_combos, np.combos);
-      np.h_Xq = std::move(Xq);
-      np.h_Yq = std::move(Yq);
     }
-  }
 
-  // ---- factor keys the slice pairs need ----
-  std::set<std::vector<int>> need1, need2;
-  for (auto const &sl : bp_slices)
-    for (auto const &sp : sl)
-      for (auto const &f : fams) {
-        auto tt = trace_times(f, sp.source, sp.sink);
-        for (auto const &ls : f.shape.leaves) {
-          if (ls.q == 1) need1.insert({tt[ls.a], B(tt[ls.b])});
-          if (ls.q == 2) need2.insert({B(tt[ls.a]), tt[ls.b], B(tt[ls.c])});
+    // ---- factor keys the slice pairs need ----
+    std::set<std::vector<int>> need1, need2;
+    for (auto const &sl : bp_slices)
+      for (auto const &sp : sl)
+        for (auto const &f : fams) {
+          auto tt = trace_times(f, sp.source, sp.sink);
+          for (auto const &ls : f.shape.leaves) {
+            if (ls.q == 1) need1.insert({tt[ls.a], B(tt[ls.b])});
+            if (ls.q == 2) need2.insert({B(tt[ls.a]), tt[ls.b], B(tt[ls.c])});
+          }
         }
-      }
+    C.need1.assign(need1.begin(), need1.end());
+    C.need2.assign(need2.begin(), need2.end());
+  }
   {
     auto t0 = std::chrono::steady_clock::now();
-    build_factors(std::vector<std::vector<int>>(need1.begin(), need1.end()),
-                  std::vector<std::vector<int>>(need2.begin(), need2.end()));
+    build_factors(C.need1, C.need2);
     times_.factors_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   auto slab_of = [&](int q) -> const double2 * {
@@ -647,154 +702,363 @@ void Engine::contract(int single_block_pair, int kahan) {
     if (slot < 0) throw std::logic_error("Q2 key was not built");
     return (long long)slot * q_total_[2];
   };
-  // ---- mirror traces: when the two operands of a trace read the same table
-  // at the same time for every slice pair, tr(X_a Y_b) = tr(X_b Y_a) with the
-  // combinations swapped and the same used-index masks, so (b, a) reuses the
-  // sums of (a, b) ----
-  for (auto &f : fams) {
-    size_t ntrl = f.tr_ids.size();
-    f.mirror_of.assign(ntrl, -1);
-    auto const &rs = f.shape.nodes.back();
-    if (rs.kind != kTrace2) continue;
-    auto const &ln = f.shape.nodes[rs.left], &rn = f.shape.nodes[rs.right];
-    bool same_table = ln.kind == kLeaf && rn.kind == kLeaf
-                          ? f.shape.leaves[ln.leaf].q == f.shape.leaves[rn.leaf].q
-                          : (ln.kind == kProd && rn.kind == kProd && f.nodes[rs.left].owner == f.nodes[rs.right].owner);
-    if (!same_table || !f.coincide) continue;
-    NodePlan const &root = f.nodes.back();
-    std::map<std::pair<int, int>, int> first;
-    for (size_t trl = 0; trl < ntrl; ++trl) {
-      int q = root.trl_qn[trl];
-      first.emplace(std::make_pair(root.qn_left[q], root.qn_right[q]), int(trl));
-    }
-    for (size_t trl = 0; trl < ntrl; ++trl) {
-      int q = root.trl_qn[trl];
-      auto it = first.find(std::make_pair(root.qn_right[q], root.qn_left[q]));
-      if (it == first.end() || it->second >= int(trl) || f.mirror_of[it->second] >= 0) continue;
-      int c = it->second;
-      if (f.ncomb[c] != f.ncomb[trl]) continue;
-      std::vector<uint16_t> ma(masks.begin() + f.mask_off[c], masks.begin() + f.mask_off[c] + f.ncomb[c]);
-      std::vector<uint16_t> mb(masks.begin() + f.mask_off[trl], masks.begin() + f.mask_off[trl] + f.ncomb[trl]);
-      std::sort(ma.begin(), ma.end());
-      std::sort(mb.begin(), mb.end());
-      if (ma == mb) f.mirror_of[trl] = c;
-    }
-  }
-  // A trace vector that only single-trace correlators read needs one sum over
-  // all its combinations, not one per mask (resolve_request1, Diagram.cpp:9-27).
-  std::vector<std::vector<char>> multi(fams.size());
-  for (size_t fi = 0; fi < fams.size(); ++fi) multi[fi].assign(fams[fi].tr_ids.size(), 0);
-  for (auto const &cp : corrs)
-    for (auto const &r : cp.reqs)
-      if (r.size() > 1)
-        for (auto const &ref : r) multi[ref.first][ref.second] = 1;
-  for (size_t fi = 0; fi < fams.size(); ++fi)
-    for (size_t trl = 0; trl < fams[fi].tr_ids.size(); ++trl)
-      if (fams[fi].mirror_of[trl] >= 0) {
-        char &a = multi[fi][trl], &b = multi[fi][fams[fi].mirror_of[trl]];
-        a = b = char(a | b);
+  if (!C.post) {
+    // ---- mirror traces: when the two operands of a trace read the same table
+    // at the same time for every slice pair, tr(X_a Y_b) = tr(X_b Y_a) with the
+    // combinations swapped and the same used-index masks, so (b, a) reuses the
+    // sums of (a, b) ----
+    for (auto &f : fams) {
+      size_t ntrl = f.tr_ids.size();
+      f.mirror_of.assign(ntrl, -1);
+      auto const &rs = f.shape.nodes.back();
+      if (rs.kind != kTrace2) continue;
+      auto const &ln = f.shape.nodes[rs.left], &rn = f.shape.nodes[rs.right];
+      bool same_table = ln.kind == kLeaf && rn.kind == kLeaf
+                            ? f.shape.leaves[ln.leaf].q == f.shape.leaves[rn.leaf].q
+                            : (ln.kind == kProd && rn.kind == kProd && f.nodes[rs.left].owner == f.nodes[rs.right].owner);
+      if (!same_table || !f.coincide) continue;
+      NodePlan const &root = f.nodes.back();
+      std::map<std::pair<int, int>, int> first;
+      for (size_t trl = 0; trl < ntrl; ++trl) {
+        int q = root.trl_qn[trl];
+        first.emplace(std::make_pair(root.qn_left[q], root.qn_right[q]), int(trl));
       }
-
-  // ---- mask groups per trace vector and the exact normalisation of each
-  // correlator (the num_summands of DilutedTrace.cpp:5-82, Diagram.cpp:9-27) ----
-  std::vector<uint16_t> gmasks;
-  for (size_t fi = 0; fi < fams.size(); ++fi) {
-    Family &f = fams[fi];
-    NodePlan const &root = f.nodes.back();
-    size_t ntrl = f.tr_ids.size();
-    f.ngroup.assign(ntrl, 0);
-    f.goff.assign(ntrl, 0);
-    f.gmask_off.assign(ntrl, 0);
-    f.gcount.assign(ntrl, {});
-    f.gitems.clear();
-    f.gpairs.clear();
-    long long off = 0;
-    for (size_t trl = 0; trl < ntrl; ++trl) {
-      if (f.mirror_of[trl] >= 0) {
-        int c = f.mirror_of[trl];
-        f.ngroup[trl] = f.ngroup[c];
-        f.goff[trl] = f.goff[c];
-        f.gmask_off[trl] = f.gmask_off[c];
-        f.gcount[trl] = f.gcount[c];
-        continue;
+      for (size_t trl = 0; trl < ntrl; ++trl) {
+        int q = root.trl_qn[trl];
+        auto it = first.find(std::make_pair(root.qn_right[q], root.qn_left[q]));
+        if (it == first.end() || it->second >= int(trl) || f.mirror_of[it->second] >= 0) continue;
+        int c = it->second;
+        if (f.ncomb[c] != f.ncomb[trl]) continue;
+        std::vector<uint16_t> ma(masks.begin() + f.mask_off[c], masks.begin() + f.mask_off[c] + f.ncomb[c]);
+        std::vector<uint16_t> mb(masks.begin() + f.mask_off[trl], masks.begin() + f.mask_off[trl] + f.ncomb[trl]);
+        std::sort(ma.begin(), ma.end());
+        std::sort(mb.begin(), mb.end());
+        if (ma == mb) f.mirror_of[trl] = c;
       }
-      std::vector<uint16_t> gm;
-      std::vector<std::vector<int>> members;
-      for (int c = 0; c < f.ncomb[trl]; ++c) {
-        uint16_t m = masks[f.mask_off[trl] + c];
-        size_t g = multi[fi][trl] ? std::find(gm.begin(), gm.end(), m) - gm.begin() : 0;
-        if (g == gm.size()) {
-          gm.push_back(m);
-          members.emplace_back();
+    }
+    // A trace vector that only single-trace correlators read needs one sum over
+    // all its combinations, not one per mask (resolve_request1, Diagram.cpp:9-27).
+    std::vector<std::vector<char>> multi(fams.size());
+    for (size_t fi = 0; fi < fams.size(); ++fi) multi[fi].assign(fams[fi].tr_ids.size(), 0);
+    for (auto const &cp : corrs)
+      for (auto const &r : cp.reqs)
+        if (r.size() > 1)
+          for (auto const &ref : r) multi[ref.first][ref.second] = 1;
+    for (size_t fi = 0; fi < fams.size(); ++fi)
+      for (size_t trl = 0; trl < fams[fi].tr_ids.size(); ++trl)
+        if (fams[fi].mirror_of[trl] >= 0) {
+          char &a = multi[fi][trl], &b = multi[fi][fams[fi].mirror_of[trl]];
+          a = b = char(a | b);
         }
-        members[g].push_back(c);
-      }
-      f.ngroup[trl] = int(gm.size());
-      f.goff[trl] = int(off);
-      f.gmask_off[trl] = int(gmasks.size());
-      gmasks.insert(gmasks.end(), gm.begin(), gm.end());
-      int q = root.trl_qn[trl];
-      for (size_t g = 0; g < gm.size(); ++g) {
-        f.gitems.push_back({q, int(f.gpairs.size()), int(members[g].size()), off + (long long)g});
-        for (int c : members[g]) {
-          auto const &cb = root.combos.at(root.qn_off[q] + c);  // combos of qn q are stored at qn_off[q]..
-          if (cb.qn != q || cb.o != root.qn_off[q] + c) throw std::logic_error("trace combo table out of order");
-          f.gpairs.push_back(make_int2(cb.i, cb.j));
-        }
-        f.gcount[trl].push_back(int(members[g].size()));
-      }
-      off += (long long)gm.size();
-    }
-    f.gstride = off;
-    upload(f.d_gitems, f.gitems);
-    upload(f.d_gpairs, f.gpairs);
-    plan_lstat(f, root, D_);
-  }
-  for (auto &cp : corrs) {
-    cp.inv_n.resize(cp.reqs.size());
-    for (size_t i = 0; i < cp.reqs.size(); ++i) {
-      auto const &r = cp.reqs[i];
-      auto gmk = [&](int k, int g) { return gmasks[fams[r[k].first].gmask_off[r[k].second] + g]; };
-      auto cnt = [&](int k, int g) { return (long long)fams[r[k].first].gcount[r[k].second][g]; };
-      auto ng = [&](int k) { return fams[r[k].first].ngroup[r[k].second]; };
-      long long n = 0;
-      if (r.size() == 1) {
-        n = fams[r[0].first].ncomb[r[0].second];
-      } else if (r.size() == 2) {
-        for (int g = 0; g < ng(0); ++g)
-          for (int h = 0; h < ng(1); ++h)
-            if (!(gmk(0, g) & gmk(1, h))) n += cnt(0, g) * cnt(1, h);
-      } else {
-        for (int g = 0; g < ng(0); ++g)
-          for (int h = 0; h < ng(1); ++h)
-            for (int k = 0; k < ng(2); ++k)
-              if (!(gmk(0, g) & gmk(1, h) & gmk(2, k))) n += cnt(0, g) * cnt(1, h) * cnt(2, k);
-      }
-      if (n == 0)
-        throw std::runtime_error(
-            "compcomp_t inner_product(vector<DilutedTrace>, vector<DilutedTrace>) has an empty result. Not enough "
-            "random vectors?");
-      cp.inv_n[i] = 1.0 / double(n);
-    }
-  }
-  DevBuf d_gmasks;
-  upload(d_gmasks, gmasks);
 
-  // Memory budget of one chunk (product tables + group sums + correlator
-  // values).  0 = automatic: 80 % of the device memory free at this point.
+    // ---- mask groups per trace vector and the exact normalisation of each
+    // correlator (the num_summands of DilutedTrace.cpp:5-82, Diagram.cpp:9-27) ----
+    gmasks.clear();
+    for (size_t fi = 0; fi < fams.size(); ++fi) {
+      Family &f = fams[fi];
+      NodePlan const &root = f.nodes.back();
+      size_t ntrl = f.tr_ids.size();
+      f.ngroup.assign(ntrl, 0);
+      f.goff.assign(ntrl, 0);
+      f.gmask_off.assign(ntrl, 0);
+      f.gcount.assign(ntrl, {});
+      f.gitems.clear();
+      f.gpairs.clear();
+      long long off = 0;
+      for (size_t trl = 0; trl < ntrl; ++trl) {
+        if (f.mirror_of[trl] >= 0) {
+          int c = f.mirror_of[trl];
+          f.ngroup[trl] = f.ngroup[c];
+          f.goff[trl] = f.goff[c];
+          f.gmask_off[trl] = f.gmask_off[c];
+          f.gcount[trl] = f.gcount[c];
+          continue;
+        }
+        std::vector<uint16_t> gm;
+        std::vector<std::vector<int>> members;
+        for (int c = 0; c < f.ncomb[trl]; ++c) {
+          uint16_t m = masks[f.mask_off[trl] + c];
+          size_t g = multi[fi][trl] ? std::find(gm.begin(), gm.end(), m) - gm.begin() : 0;
+          if (g == gm.size()) {
+            gm.push_back(m);
+            members.emplace_back();
+          }
+          members[g].push_back(c);
+        }
+        f.ngroup[trl] = int(gm.size());
+        f.goff[trl] = int(off);
+        f.gmask_off[trl] = int(gmasks.size());
+        gmasks.insert(gmasks.end(), gm.begin(), gm.end());
+        int q = root.trl_qn[trl];
+        for (size_t g = 0; g < gm.size(); ++g) {
+          f.gitems.push_back({q, int(f.gpairs.size()), int(members[g].size()), off + (long long)g});
+          for (int c : members[g]) {
+            auto const &cb = root.combos.at(root.qn_off[q] + c);  // combos of qn q are stored at qn_off[q]..
+            if (cb.qn != q || cb.o != root.qn_off[q] + c) throw std::logic_error("trace combo table out of order");
+            f.gpairs.push_back(make_int2(cb.i, cb.j));
+          }
+          f.gcount[trl].push_back(int(members[g].size()));
+        }
+        off += (long long)gm.size();
+      }
+      f.gstride = off;
+      upload(f.d_gitems, f.gitems);
+      upload(f.d_gpairs, f.gpairs);
+      plan_lstat(f, root, D_);
+    }
+    for (auto &cp : corrs) {
+      cp.inv_n.resize(cp.reqs.size());
+      for (size_t i = 0; i < cp.reqs.size(); ++i) {
+        auto const &r = cp.reqs[i];
+        auto gmk = [&](int k, int g) { return gmasks[fams[r[k].first].gmask_off[r[k].second] + g]; };
+        auto cnt = [&](int k, int g) { return (long long)fams[r[k].first].gcount[r[k].second][g]; };
+        auto ng = [&](int k) { return fams[r[k].first].ngroup[r[k].second]; };
+        long long n = 0;
+        if (r.size() == 1) {
+          n = fams[r[0].first].ncomb[r[0].second];
+        } else if (r.size() == 2) {
+          for (int g = 0; g < ng(0); ++g)
+            for (int h = 0; h < ng(1); ++h)
+              if (!(gmk(0, g) & gmk(1, h))) n += cnt(0, g) * cnt(1, h);
+        } else {
+          for (int g = 0; g < ng(0); ++g)
+            for (int h = 0; h < ng(1); ++h)
+              for (int k = 0; k < ng(2); ++k)
+                if (!(gmk(0, g) & gmk(1, h) & gmk(2, k))) n += cnt(0, g) * cnt(1, h) * cnt(2, k);
+        }
+        if (n == 0)
+          throw std::runtime_error(
+              "compcomp_t inner_product(vector<DilutedTrace>, vector<DilutedTrace>) has an empty result. Not enough "
+              "random vectors?");
+        cp.inv_n[i] = 1.0 / double(n);
+      }
+    }
+    upload(C.d_gmasks, gmasks);
+    C.post = true;
+  }
+
   const long long DD = (long long)D_ * D_;
-  size_t budget = chunk_budget_;
-  if (budget == 0) {
-    size_t free_b = 0, total_b = 0;
-    SLAPH_HIP(hipMemGetInfo(&free_b, &total_b));
-    budget = free_b / 10 * 8;
-  }
   long long gstride_all = 0;
   for (auto const &f : fams) gstride_all += f.gstride;
   size_t max_corr = 1;
   for (auto const &c : corrs) max_corr = std::max(max_corr, c.reqs.size());
+  const bool progress = std::getenv("SLAPH_PROGRESS") != nullptr;
+  // SLAPH_TRACE_GROUPS=1 selects the one-wave-per-group trace kernel for every
+  // trace (a cross-check of the tiled and left-stationary kernels).
+  const bool lstat_off = std::getenv("SLAPH_TRACE_GROUPS") != nullptr;
+  auto slab_count = [&](Family const &f, int child, long long prod_total) -> long long {
+    auto const &cs = f.shape.nodes[child];
+    if (cs.kind != kLeaf) return prod_total;
+    int q = f.shape.leaves[cs.leaf].q;
+    DevBuf const &b = q == 0 ? q0_ : (q == 1 ? q1_ : q2_);
+    return (long long)(b.bytes() / (sizeof(double2) * DD));
+  };
 
-  // ---- accumulators ----
+  // ---- per-chunk time plans (built once) ----
+  // Chunks are whole rows (one source time, all sink times of the block) of
+  // one direction of the block pair, so that time-separable traces see a
+  // rectangular tile of slice pairs.  A chunk starts as a whole pass (dilT
+  // rows); the row count halves while its plan exceeds the memory budget
+  // (product tables + group sums + correlator values; 0 = automatic: 80 % of
+  // the device memory free at this point).
+  if (C.chunks.empty()) {
+    size_t budget = chunk_budget_;
+    if (budget == 0) {
+      size_t free_b = 0, total_b = 0;
+      SLAPH_HIP(hipMemGetInfo(&free_b, &total_b));
+      budget = free_b / 10 * 8;
+    }
+    const size_t pass_len = (size_t)dilT * dilT;
+    size_t chunk_rows = (size_t)dilT;
+    for (size_t bp = 0; bp < bp_slices.size(); ++bp) {
+      auto const &sl = bp_slices[bp];
+      for (size_t c0 = 0, S_ = 0; c0 < sl.size(); c0 += S_) {
+        auto plan_chunk = [&](int S) -> long long {
+        long long prod_total = 0;
+        for (auto &f : fams) {
+          std::vector<std::vector<int>> tts(S);
+          for (int s = 0; s < S; ++s) tts[s] = trace_times(f, sl[c0 + s].source, sl[c0 + s].sink);
+          // subtree leaf bases per node and slice pair
+          std::vector<std::vector<std::vector<long long>>> keys(f.shape.nodes.size(),
+                                                                std::vector<std::vector<long long>>(S));
+          for (auto &np : f.nodes) {
+            np.time_index.clear();
+            np.time_left.clear();
+            np.time_right.clear();
+            np.time_leafbase_l.clear();
+            np.time_leafbase_r.clear();
+            np.sp_time.assign(S, 0);
+          }
+          // (a) time keys; aliases index into their owner's time table
+          for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+            auto const &ns = f.shape.nodes[n];
+            NodePlan &self = f.nodes[n];
+            NodePlan &np = f.nodes[self.owner];
+            bool root = n + 1 == f.shape.nodes.size();
+            for (int s = 0; s < S; ++s) {
+              if (ns.kind == kLeaf) {
+                keys[n][s] = {leaf_base(f.shape.leaves[ns.leaf], tts[s])};
+                continue;
+              }
+              std::vector<long long> key = keys[ns.left][s];
+              if (ns.right >= 0) key.insert(key.end(), keys[ns.right][s].begin(), keys[ns.right][s].end());
+              keys[n][s] = key;
+              int ti;
+              if (root) {
+                ti = s;
+              } else {
+                auto it = np.time_index.find(key);
+                if (it != np.time_index.end()) {
+                  self.sp_time[s] = it->second;
+                  continue;
+                }
+                ti = int(np.time_index.size());
+                np.time_index.emplace(key, ti);
+              }
+              self.sp_time[s] = ti;
+              auto child_t = [&](int child, std::vector<int> &ct, std::vector<long long> &lb) {
+                if (child < 0) {
+                  ct.push_back(0);
+                  lb.push_back(0);
+                  return;
+                }
+                if (f.shape.nodes[child].kind == kLeaf) {
+                  ct.push_back(0);
+                  lb.push_back(keys[child][s][0]);
+                } else {
+                  ct.push_back(f.nodes[child].sp_time[s]);
+                  lb.push_back(0);
+                }
+              };
+              child_t(ns.left, np.time_left, np.time_leafbase_l);
+              child_t(ns.right, np.time_right, np.time_leafbase_r);
+            }
+          }
+          // (b) storage of the owners' product tables
+          for (size_t n = 0; n + 1 < f.shape.nodes.size(); ++n) {
+            NodePlan &np = f.nodes[n];
+            if (f.shape.nodes[n].kind != kProd || np.owner != int(n)) continue;
+            np.base = prod_total;
+            prod_total += (long long)np.time_index.size() * np.stride;
+          }
+          for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+            NodePlan &np = f.nodes[n];
+            if (np.owner != int(n)) {
+              np.base = f.nodes[np.owner].base;
+              np.stride = f.nodes[np.owner].stride;
+            }
+          }
+          // (c) operand time tables of the owners: leaf -> slab base; product ->
+          // base + ti * stride of the child's owner
+          for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+            auto const &ns = f.shape.nodes[n];
+            NodePlan &np = f.nodes[n];
+            if (ns.kind == kLeaf || np.owner != int(n)) continue;
+            bool root = n + 1 == f.shape.nodes.size();
+            int ntime = root ? S : int(np.time_index.size());
+            std::vector<long long> Xt(ntime), Yt(ntime, 0);
+            for (int ti = 0; ti < ntime; ++ti) {
+              auto const &l = f.shape.nodes[ns.left];
+              NodePlan const &lo = f.nodes[f.nodes[ns.left].owner];
+              Xt[ti] = l.kind == kLeaf ? np.time_leafbase_l[ti] : lo.base + np.time_left[ti] * lo.stride;
+              if (ns.right >= 0) {
+                auto const &r = f.shape.nodes[ns.right];
+                NodePlan const &ro = f.nodes[f.nodes[ns.right].owner];
+                Yt[ti] = r.kind == kLeaf ? np.time_leafbase_r[ti] : ro.base + np.time_right[ti] * ro.stride;
+              }
+            }
+            np.h_Xt = std::move(Xt);
+            np.h_Yt = std::move(Yt);
+          }
+        }
+        return prod_total;
+        };
+        int S = 0;
+        long long prod_total = 0;
+        for (;;) {
+          S_ = std::min(chunk_rows * dilT, (c0 / pass_len + 1) * pass_len - c0);
+          S_ = std::min(S_, sl.size() - c0);
+          S = int(S_);
+          prod_total = plan_chunk(S);
+          double bytes = sizeof(double2) * (double(prod_total) * DD + double(S) * (gstride_all + max_corr));
+          if (bytes <= double(budget) || chunk_rows == 1) break;
+          chunk_rows = std::max<size_t>(1, chunk_rows / 2);
+        }
+        ChunkPlan ch;
+        ch.bp = bp;
+        ch.c0 = c0;
+        ch.S = S;
+        ch.prod_total = prod_total;
+        ch.gbase.resize(fams.size());
+        for (size_t fi = 0; fi < fams.size(); ++fi) {
+          ch.gbase[fi] = ch.g_total;  // trace values go straight into mask-group sums
+          ch.g_total += (long long)S * fams[fi].gstride;
+        }
+        ch.nodes.resize(fams.size());
+        for (size_t fi = 0; fi < fams.size(); ++fi) {
+          Family &f = fams[fi];
+          ch.nodes[fi].resize(f.shape.nodes.size());
+          for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+            auto const &ns = f.shape.nodes[n];
+            NodePlan &np = f.nodes[n];
+            NodeChunk &nc = ch.nodes[fi][n];
+            nc.base = np.base;
+            if (ns.kind == kLeaf || np.owner != int(n)) continue;
+            const bool root = n + 1 == f.shape.nodes.size();
+            nc.ntime = root ? S : int(np.time_index.size());
+            nc.h_Xt = np.h_Xt;
+            nc.h_Yt = np.h_Yt;
+            upload(nc.d_Xt, nc.h_Xt);
+            upload(nc.d_Yt, nc.h_Yt);
+            // Conservative bounds check of every operand and output index the
+            // launches can form (largest time base + largest qn/combo offset).
+            auto max_of = [](std::vector<long long> const &v) {
+              long long m = 0;
+              for (auto x : v) m = std::max(m, x);
+              return m;
+            };
+            if (max_of(nc.h_Xt) + np.cmaxX >= slab_count(f, ns.left, prod_total) ||
+                (ns.right >= 0 && max_of(nc.h_Yt) + np.cmaxY >= slab_count(f, ns.right, prod_total)) ||
+                (root ? ch.gbase[fi] + (long long)(nc.ntime - 1) * f.gstride + f.gstride > ch.g_total
+                      : np.base + (long long)(nc.ntime - 1) * np.stride + np.cmaxO >= prod_total))
+              throw std::logic_error("contraction plan indexes outside its buffers (" + f.name + ")");
+            GridTile tile;
+            if (root && ns.kind == kTrace2 && !lstat_off && grid_tile(nc.h_Xt, nc.h_Yt, tile)) {
+              nc.grid = true;
+              upload(nc.d_tile, std::vector<GridTile>{tile});
+            }
+          }
+        }
+        std::vector<int> td(S);
+        for (int s = 0; s < S; ++s) td[s] = time_delta(sl[c0 + s].source, sl[c0 + s].sink, Lt_);
+        upload(ch.d_td, td);
+        ch.d_jobs.resize(corrs.size());
+        for (size_t c = 0; c < corrs.size(); ++c) {
+          auto const &cp = corrs[c];
+          std::vector<CorrJob> jobs(cp.reqs.size());
+          for (size_t i = 0; i < cp.reqs.size(); ++i) {
+            jobs[i].ntr = int(cp.reqs[i].size());
+            jobs[i].inv_n = cp.inv_n[i];
+            for (size_t k = 0; k < cp.reqs[i].size(); ++k) {
+              auto const &f = fams[cp.reqs[i][k].first];
+              int trl = cp.reqs[i][k].second;
+              jobs[i].tr[k].base = ch.gbase[cp.reqs[i][k].first] + f.goff[trl];
+              jobs[i].tr[k].stride = f.gstride;
+              jobs[i].tr[k].ngroup = f.ngroup[trl];
+              jobs[i].tr[k].gmask_off = f.gmask_off[trl];
+              if (jobs[i].tr[k].base + (long long)(S - 1) * f.gstride + f.ngroup[trl] > ch.g_total ||
+                  f.gmask_off[trl] + f.ngroup[trl] > (long long)gmasks.size())
+                throw std::logic_error("correlator request indexes outside the group-sum buffer");
+            }
+          }
+          upload(ch.d_jobs[c], jobs);
+        }
+        C.chunks.push_back(std::move(ch));
+      }
+    }
+  }
+
+  // ---- accumulators and chunk buffers ----
   std::vector<DevBuf> d_sum(corrs.size()), d_comp(corrs.size());
   for (size_t c = 0; c < corrs.size(); ++c) {
     size_t n = (size_t)Lt_ * corrs[c].reqs.size();
@@ -803,251 +1067,80 @@ void Engine::contract(int single_block_pair, int kahan) {
     SLAPH_HIP(hipMemsetAsync(d_sum[c].as<void>(), 0, sizeof(double2) * n, stream_));
     SLAPH_HIP(hipMemsetAsync(d_comp[c].as<void>(), 0, sizeof(double2) * n, stream_));
   }
-  DevBuf d_prod, d_cval, d_td, d_gv;
-  std::vector<DevBuf> d_jobs_c(corrs.size());
+  {
+    long long np_max = 1, ng_max = 1, nc_max = 1;
+    for (auto const &ch : C.chunks) {
+      np_max = std::max(np_max, ch.prod_total * DD);
+      ng_max = std::max(ng_max, ch.g_total);
+      nc_max = std::max(nc_max, (long long)max_corr * ch.S);
+    }
+    C.d_prod.ensure(sizeof(double2) * np_max);
+    C.d_gv.ensure(sizeof(double2) * ng_max);
+    C.d_cval.ensure(sizeof(double2) * nc_max);
+  }
 
-
+  // ---- launches: per chunk, products in topological order, then traces,
+  // then resolve + accumulate per diagram; no host work between chunks ----
   EventTimer ctimer(stream_);
-  const bool progress = std::getenv("SLAPH_PROGRESS") != nullptr;
-  // SLAPH_TRACE_GROUPS=1 selects the one-wave-per-group trace kernel for every
-  // trace (a cross-check of the tiled and left-stationary kernels).
-  const bool lstat_off = std::getenv("SLAPH_TRACE_GROUPS") != nullptr;
-  int bp_done = 0;
-  // Chunks are whole rows (one source time, all sink times of the block) of
-  // one direction of the block pair, so that time-separable traces see a
-  // rectangular tile of slice pairs.  A chunk starts as a whole pass
-  // (dilT rows); the row count halves while its plan exceeds the budget.
-  const size_t pass_len = (size_t)dilT * dilT;
-  size_t chunk_rows = (size_t)dilT;
-  for (auto const &sl : bp_slices) {
-    if (progress) {
+  size_t last_bp = size_t(-1);
+  for (auto const &ch : C.chunks) {
+    if (progress && ch.bp != last_bp) {
       // like the reference's per-block-pair message (Correlators.cpp:92-97)
-      std::fprintf(stderr, "[slaph] block pair %d of %zu (%zu slice pairs, %zu rows of %d per launch)\n", bp_done++,
-                   bp_slices.size(), sl.size(), chunk_rows, dilT);
+      std::fprintf(stderr, "[slaph] block pair %zu of %zu (%zu slice pairs, %d per launch)\n", ch.bp,
+                   bp_slices.size(), bp_slices[ch.bp].size(), ch.S);
       std::fflush(stderr);
     }
-    for (size_t c0 = 0, S_ = 0; c0 < sl.size(); c0 += S_) {
-      // time planning of slice pairs c0 .. c0+S-1; returns the number of
-      // product matrices the chunk stores
-      auto plan_chunk = [&](int S) -> long long {
-      long long prod_total = 0;
-      for (auto &f : fams) {
-        std::vector<std::vector<int>> tts(S);
-        for (int s = 0; s < S; ++s) tts[s] = trace_times(f, sl[c0 + s].source, sl[c0 + s].sink);
-        // subtree leaf bases per node and slice pair
-        std::vector<std::vector<std::vector<long long>>> keys(f.shape.nodes.size(),
-                                                              std::vector<std::vector<long long>>(S));
-        for (auto &np : f.nodes) {
-          np.time_index.clear();
-          np.time_left.clear();
-          np.time_right.clear();
-          np.time_leafbase_l.clear();
-          np.time_leafbase_r.clear();
-          np.sp_time.assign(S, 0);
-        }
-        // (a) time keys; aliases index into their owner's time table
-        for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
-          auto const &ns = f.shape.nodes[n];
-          NodePlan &self = f.nodes[n];
-          NodePlan &np = f.nodes[self.owner];
-          bool root = n + 1 == f.shape.nodes.size();
-          for (int s = 0; s < S; ++s) {
-            if (ns.kind == kLeaf) {
-              keys[n][s] = {leaf_base(f.shape.leaves[ns.leaf], tts[s])};
-              continue;
-            }
-            std::vector<long long> key = keys[ns.left][s];
-            if (ns.right >= 0) key.insert(key.end(), keys[ns.right][s].begin(), keys[ns.right][s].end());
-            keys[n][s] = key;
-            int ti;
-            if (root) {
-              ti = s;
-            } else {
-              auto it = np.time_index.find(key);
-              if (it != np.time_index.end()) {
-                self.sp_time[s] = it->second;
-                continue;
-              }
-              ti = int(np.time_index.size());
-              np.time_index.emplace(key, ti);
-            }
-            self.sp_time[s] = ti;
-            auto child_t = [&](int child, std::vector<int> &ct, std::vector<long long> &lb) {
-              if (child < 0) {
-                ct.push_back(0);
-                lb.push_back(0);
-                return;
-              }
-              if (f.shape.nodes[child].kind == kLeaf) {
-                ct.push_back(0);
-                lb.push_back(keys[child][s][0]);
-              } else {
-                ct.push_back(f.nodes[child].sp_time[s]);
-                lb.push_back(0);
-              }
-            };
-            child_t(ns.left, np.time_left, np.time_leafbase_l);
-            child_t(ns.right, np.time_right, np.time_leafbase_r);
-          }
-        }
-        // (b) storage of the owners' product tables
-        for (size_t n = 0; n + 1 < f.shape.nodes.size(); ++n) {
-          NodePlan &np = f.nodes[n];
-          if (f.shape.nodes[n].kind != kProd || np.owner != int(n)) continue;
-          np.base = prod_total;
-          prod_total += (long long)np.time_index.size() * np.stride;
-        }
-        for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
-          NodePlan &np = f.nodes[n];
-          if (np.owner != int(n)) {
-            np.base = f.nodes[np.owner].base;
-            np.stride = f.nodes[np.owner].stride;
-          }
-        }
-        // (c) operand time tables of the owners: leaf -> slab base; product ->
-        // base + ti * stride of the child's owner
-        for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
-          auto const &ns = f.shape.nodes[n];
-          NodePlan &np = f.nodes[n];
-          if (ns.kind == kLeaf || np.owner != int(n)) continue;
-          bool root = n + 1 == f.shape.nodes.size();
-          int ntime = root ? S : int(np.time_index.size());
-          std::vector<long long> Xt(ntime), Yt(ntime, 0);
-          for (int ti = 0; ti < ntime; ++ti) {
-            auto const &l = f.shape.nodes[ns.left];
-            NodePlan const &lo = f.nodes[f.nodes[ns.left].owner];
-            Xt[ti] = l.kind == kLeaf ? np.time_leafbase_l[ti] : lo.base + np.time_left[ti] * lo.stride;
-            if (ns.right >= 0) {
-              auto const &r = f.shape.nodes[ns.right];
-              NodePlan const &ro = f.nodes[f.nodes[ns.right].owner];
-              Yt[ti] = r.kind == kLeaf ? np.time_leafbase_r[ti] : ro.base + np.time_right[ti] * ro.stride;
-            }
-          }
-          np.h_Xt = std::move(Xt);
-          np.h_Yt = std::move(Yt);
+    last_bp = ch.bp;
+    const int S = ch.S;
+    for (size_t fi = 0; fi < fams.size(); ++fi) {
+      Family &f = fams[fi];
+      for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+        auto const &ns = f.shape.nodes[n];
+        NodePlan &np = f.nodes[n];
+        NodeChunk const &nc = ch.nodes[fi][n];
+        if (ns.kind == kLeaf || np.owner != int(n)) continue;
+        const bool root = n + 1 == f.shape.nodes.size();
+        auto src = [&](int child) -> const double2 * {
+          if (child < 0) return nullptr;
+          auto const &cs = f.shape.nodes[child];
+          if (cs.kind == kLeaf) return slab_of(f.shape.leaves[cs.leaf].q);
+          return C.d_prod.as<double2>();
+        };
+        // operands stored transposed: only a trace's right operand may be
+        auto is_t = [&](int child) {
+          return child >= 0 && f.shape.nodes[child].kind == kProd && f.nodes[f.nodes[child].owner].transposed;
+        };
+        const int rt = is_t(ns.right) ? 1 : 0;
+        if (is_t(ns.left) || (!root && rt)) throw std::logic_error("transposed product table read as a factor");
+        const long long gb = ch.gbase[fi];
+        if (!root) {
+          launch_prod(src(ns.left), nc.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
+                      nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), C.d_prod.as<double2>(), nc.base, np.stride,
+                      np.d_combos.as<ComboProd>(), int(np.combos.size()), nc.ntime, D_, np.transposed ? 1 : 0,
+                      stream_);
+        } else if (nc.grid) {
+          launch_trace_grid(src(ns.left), np.d_Xq.as<long long>(), src(ns.right), np.d_Yq.as<long long>(),
+                            f.d_gitems.as<GroupItem>(), int(f.gitems.size()), f.d_gpairs.as<int2>(),
+                            nc.d_tile.as<GridTile>(), 1, C.d_gv.as<double2>(), gb, f.gstride, D_, rt, stream_);
+        } else if (ns.kind == kTrace2 && !f.ls_blocks.empty() && !lstat_off) {
+          launch_trace_lstat(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
+                             f.d_lsb.as<LsBlock>(), int(f.ls_blocks.size()), f.d_lsc.as<LsChunk>(),
+                             f.d_lst.as<LsTask>(), f.d_lsd.as<long long>(), C.d_gv.as<double2>(), gb, f.gstride,
+                             nc.ntime, D_, rt, stream_);
+        } else {
+          launch_trace_groups(src(ns.left), nc.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
+                              nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), f.d_gitems.as<GroupItem>(),
+                              int(f.gitems.size()), f.d_gpairs.as<int2>(), C.d_gv.as<double2>(), gb, f.gstride,
+                              nc.ntime, D_, rt, stream_);
         }
       }
-      return prod_total;
-      };
-      int S = 0;
-      long long prod_total = 0;
-      for (;;) {
-        S_ = std::min(chunk_rows * dilT, (c0 / pass_len + 1) * pass_len - c0);
-        S_ = std::min(S_, sl.size() - c0);
-        S = int(S_);
-        prod_total = plan_chunk(S);
-        double bytes = sizeof(double2) * (double(prod_total) * DD + double(S) * (gstride_all + max_corr));
-        if (bytes <= double(budget) || chunk_rows == 1) break;
-        chunk_rows = std::max<size_t>(1, chunk_rows / 2);
-      }
-      for (auto &f : fams)
-        for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
-          NodePlan &np = f.nodes[n];
-          if (f.shape.nodes[n].kind == kLeaf || np.owner != int(n)) continue;
-          upload(np.d_Xt, np.h_Xt);
-          upload(np.d_Yt, np.h_Yt);
-        }
-      d_cval.ensure(sizeof(double2) * max_corr * (size_t)S);
-      d_prod.ensure(sizeof(double2) * std::max<long long>(1, prod_total * DD));
-      // trace values go straight into mask-group sums
-      long long g_total = 0;
-      for (auto &f : fams) {
-        f.gbase = g_total;
-        g_total += (long long)S * f.gstride;
-      }
-      d_gv.ensure(sizeof(double2) * std::max<long long>(1, g_total));
-      // launches: products in topological order, then traces
-      for (auto &f : fams)
-        for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
-          auto const &ns = f.shape.nodes[n];
-          NodePlan &np = f.nodes[n];
-          if (ns.kind == kLeaf || np.owner != int(n)) continue;
-          bool root = n + 1 == f.shape.nodes.size();
-          auto src = [&](int child) -> const double2 * {
-            if (child < 0) return nullptr;
-            auto const &cs = f.shape.nodes[child];
-            if (cs.kind == kLeaf) return slab_of(f.shape.leaves[cs.leaf].q);
-            return d_prod.as<double2>();
-          };
-          int ntime = root ? S : int(np.time_index.size());
-          GridTile tile;
-          // Conservative bounds check of every operand and output index the
-          // launch can form (largest time base + largest qn/combo offset).
-          auto slab_count = [&](int child) -> long long {
-            auto const &cs = f.shape.nodes[child];
-            if (cs.kind != kLeaf) return prod_total;
-            int q = f.shape.leaves[cs.leaf].q;
-            DevBuf const &b = q == 0 ? q0_ : (q == 1 ? q1_ : q2_);
-            return (long long)(b.bytes() / (sizeof(double2) * DD));
-          };
-          auto max_of = [](DevBuf const &, std::vector<long long> const &v) {
-            long long m = 0;
-            for (auto x : v) m = std::max(m, x);
-            return m;
-          };
-          if (max_of(np.d_Xt, np.h_Xt) + np.cmaxX >= slab_count(ns.left) ||
-              (ns.right >= 0 && max_of(np.d_Yt, np.h_Yt) + np.cmaxY >= slab_count(ns.right)) ||
-              (root ? f.gbase + (long long)(ntime - 1) * f.gstride + f.gstride > g_total
-                    : np.base + (long long)(ntime - 1) * np.stride + np.cmaxO >= prod_total))
-            throw std::logic_error("contraction plan indexes outside its buffers (" + f.name + ")");
-          // operands stored transposed: only a trace's right operand may be
-          auto is_t = [&](int child) {
-            return child >= 0 && f.shape.nodes[child].kind == kProd && f.nodes[f.nodes[child].owner].transposed;
-          };
-          const int rt = is_t(ns.right) ? 1 : 0;
-          if (is_t(ns.left) || (!root && rt)) throw std::logic_error("transposed product table read as a factor");
-          if (!root) {
-            launch_prod(src(ns.left), np.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
-                        np.d_Yt.as<long long>(), np.d_Yq.as<long long>(), d_prod.as<double2>(), np.base, np.stride,
-                        np.d_combos.as<ComboProd>(), int(np.combos.size()), ntime, D_, np.transposed ? 1 : 0,
-                        stream_);
-          } else if (ns.kind == kTrace2 && !lstat_off && grid_tile(np.h_Xt, np.h_Yt, tile)) {
-            upload(f.d_tile, std::vector<GridTile>{tile});
-            launch_trace_grid(src(ns.left), np.d_Xq.as<long long>(), src(ns.right), np.d_Yq.as<long long>(),
-                              f.d_gitems.as<GroupItem>(), int(f.gitems.size()), f.d_gpairs.as<int2>(),
-                              f.d_tile.as<GridTile>(), 1, d_gv.as<double2>(), f.gbase, f.gstride, D_, rt, stream_);
-          } else if (ns.kind == kTrace2 && !f.ls_blocks.empty() && !lstat_off) {
-            launch_trace_lstat(src(ns.left), np.d_Xt.as<long long>(), src(ns.right), np.d_Yt.as<long long>(),
-                               f.d_lsb.as<LsBlock>(), int(f.ls_blocks.size()), f.d_lsc.as<LsChunk>(),
-                               f.d_lst.as<LsTask>(), f.d_lsd.as<long long>(), d_gv.as<double2>(), f.gbase, f.gstride,
-                               ntime, D_, rt, stream_);
-          } else {
-            launch_trace_groups(src(ns.left), np.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
-                                np.d_Yt.as<long long>(), np.d_Yq.as<long long>(), f.d_gitems.as<GroupItem>(),
-                                int(f.gitems.size()), f.d_gpairs.as<int2>(), d_gv.as<double2>(), f.gbase, f.gstride,
-                                ntime, D_, rt, stream_);
-          }
-        }
-      // resolve and accumulate per diagram
-      std::vector<int> td(S);
-      for (int s = 0; s < S; ++s) td[s] = time_delta(sl[c0 + s].source, sl[c0 + s].sink, Lt_);
-      upload(d_td, td);
-      for (size_t c = 0; c < corrs.size(); ++c) {
-        auto const &cp = corrs[c];
-        std::vector<CorrJob> jobs(cp.reqs.size());
-        for (size_t i = 0; i < cp.reqs.size(); ++i) {
-          jobs[i].ntr = int(cp.reqs[i].size());
-          jobs[i].inv_n = cp.inv_n[i];
-          for (size_t k = 0; k < cp.reqs[i].size(); ++k) {
-            auto const &f = fams[cp.reqs[i][k].first];
-            int trl = cp.reqs[i][k].second;
-            jobs[i].tr[k].base = f.gbase + f.goff[trl];
-            jobs[i].tr[k].stride = f.gstride;
-            jobs[i].tr[k].ngroup = f.ngroup[trl];
-            jobs[i].tr[k].gmask_off = f.gmask_off[trl];
-            if (jobs[i].tr[k].base + (long long)(S - 1) * f.gstride + f.ngroup[trl] > g_total ||
-                f.gmask_off[trl] + f.ngroup[trl] > (long long)gmasks.size())
-              throw std::logic_error("correlator request indexes outside the group-sum buffer");
-          }
-        }
-        upload(d_jobs_c[c], jobs);
-        launch_resolve(d_gv.as<double2>(), d_gmasks.as<uint16_t>(), d_jobs_c[c].as<CorrJob>(), int(cp.reqs.size()), S,
-                       d_cval.as<double2>(), stream_);
-        launch_accumulate(d_cval.as<double2>(), d_td.as<int>(), S, int(cp.reqs.size()), d_sum[c].as<double2>(),
-                          d_comp[c].as<double2>(), kahan, stream_);
-      }
-      // host tables are re-filled for the next chunk: wait for this one
-      SLAPH_HIP(hipStreamSynchronize(stream_));
+    }
+    for (size_t c = 0; c < corrs.size(); ++c) {
+      int ncorr = int(corrs[c].reqs.size());
+      launch_resolve(C.d_gv.as<double2>(), C.d_gmasks.as<uint16_t>(), ch.d_jobs[c].as<CorrJob>(), ncorr, S,
+                     C.d_cval.as<double2>(), stream_);
+      launch_accumulate(C.d_cval.as<double2>(), ch.d_td.as<int>(), S, ncorr, d_sum[c].as<double2>(),
+                        d_comp[c].as<double2>(), kahan, stream_);
     }
   }
   double kms = ctimer.stop();
@@ -1070,6 +1163,7 @@ void Engine::contract(int single_block_pair, int kahan) {
     results_.push_back(std::move(r));
   }
   times_.contract_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  guard.done = true;
 }
 
 }  // namespace slaph

@@ -88,6 +88,7 @@ void Engine::configure(std::string const &text) {
   if (D_ > 32) throw std::runtime_error("4*dilE above 32 is not supported");
   have_ev_ = have_peram_ = have_rnd_ = have_ops_ = false;
   results_.clear();
+  plan_.reset();
   SLAPH_HIP(hipSetDevice(device_));
   alloc_inputs();
 }

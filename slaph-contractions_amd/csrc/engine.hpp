@@ -63,6 +63,8 @@ class EventTimer {
   hipEvent_t a_, b_;
 };
 
+struct ContractCache;  // contract.cpp
+
 struct StageTimes {
   double vdaggerv_ms = 0, factors_ms = 0, contract_ms = 0;
   double vdaggerv_kernel_ms = 0;  // k_ahb only
@@ -107,7 +109,10 @@ class Engine {
   std::string correlator_name(std::string const &diagram, int i) const;
   void fetch_correlators(std::string const &diagram, cplx *out) const;  // n_corr x Lt
 
-  void set_chunk_budget(size_t bytes) { chunk_budget_ = bytes; }
+  void set_chunk_budget(size_t bytes) {
+    chunk_budget_ = bytes;
+    plan_.reset();  // chunk tables depend on the budget
+  }
   StageTimes const &times() const { return times_; }
   hipStream_t stream() const { return stream_; }
 
@@ -150,6 +155,10 @@ class Engine {
 
   size_t chunk_budget_ = 0;  // 0: from the free device memory
   StageTimes times_;
+  // contract()'s plan: everything derived from the input file alone (quantum
+  // numbers, trace groups, per-chunk index tables); rebuilt after configure()
+  // or a budget change, reused by every later gauge configuration.
+  std::shared_ptr<ContractCache> plan_;
 };
 
 }  // namespace slaph

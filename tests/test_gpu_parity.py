@@ -201,6 +201,33 @@ def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypat
 _SMALL_WANT = {}  # oracle correlators per input file (same seeded inputs)
 
 
+def test_plan_reuse_is_deterministic(slaph):
+    """The contraction plan is built on the first call and reused: a second
+    call on new operators gives bit-identical correlators, a budget change
+    rebuilds the plan (one-row chunks) and agrees to rounding."""
+    say("plan reuse")
+    ctx = slaph.Context(0)
+    ctx.configure(WIDE)
+    ctx.synthetic_inputs(77, True)
+    ops = slaph.OperatorFactory(ctx)
+    ops.create_operators()
+    slaph.contract(ctx)
+    first = {d: ctx.correlators(d) for d in ctx.diagrams()}
+    ops.create_operators()
+    slaph.contract(ctx)
+    for d, vals in first.items():
+        again = ctx.correlators(d)
+        for name, v in vals.items():
+            assert np.array_equal(again[name], v), (d, name)
+    ctx.set_chunk_budget(1)
+    slaph.contract(ctx)
+    for d, vals in first.items():
+        small = ctx.correlators(d)
+        for name, v in vals.items():
+            scale = max(np.max(np.abs(v)), 1e-300)
+            assert np.max(np.abs(small[name] - v)) / scale <= 1e-13, (d, name)
+
+
 def test_synthetic_inputs_match_oracle(slaph):
     """Device-generated inputs: eigenvectors orthonormal (V^H V = 1), random
     vectors in {(+-1 +-i)/sqrt 2}, and every correlator of the small pipi
