@@ -303,6 +303,10 @@ bool grid_tile(std::vector<long long> const &Xt, std::vector<long long> const &Y
 // and the chunk's task list is cut into kLsWaves ranges of about equal length.
 // Each wave flushes its per-lane sum at the end of every run and of its range
 // into its own slot copy, so the order of every sum is fixed by the plan.
+// Blocks are ordered by connected component of the (left entry, right entry)
+// graph of the traces, so that blocks reading the same right matrices are
+// neighbours in launch order (and, through the XCD-aware block mapping of the
+// kernel, run against the same L2).
 void plan_lstat(Family &f, NodePlan const &root, int D) {
   f.ls_blocks.clear();
   f.ls_chunks.clear();
@@ -312,8 +316,27 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
   if (f.shape.nodes.back().kind != kTrace2 || CL <= 0 || f.gitems.empty()) return;
   std::map<long long, std::vector<int>> by_left;
   for (size_t k = 0; k < f.gitems.size(); ++k) by_left[root.h_Xq.at(f.gitems[k].qn)].push_back(int(k));
-  for (auto const &kv : by_left)
-    for (size_t s0 = 0; s0 < kv.second.size(); s0 += kLsSlots) {
+  // components: left and right entries are nodes (one node space when both
+  // operands read the same table), every trace an edge
+  std::map<std::pair<int, long long>, int> node_id;
+  std::vector<int> parent;
+  auto node = [&](int side, long long off) {
+    auto it = node_id.emplace(std::make_pair(side, off), int(parent.size())).first;
+    if (it->second == int(parent.size())) parent.push_back(it->second);
+    return it->second;
+  };
+  std::function<int(int)> find = [&](int x) { return parent[x] == x ? x : parent[x] = find(parent[x]); };
+  for (auto const &it : f.gitems) {
+    int a = find(node(0, root.h_Xq.at(it.qn))), b = find(node(f.coincide ? 0 : 1, root.h_Yq.at(it.qn)));
+    if (a != b) parent[std::max(a, b)] = std::min(a, b);
+  }
+  std::vector<std::pair<int, long long>> order;  // (component, left entry)
+  for (auto const &kv : by_left) order.push_back({find(node(0, kv.first)), kv.first});
+  std::stable_sort(order.begin(), order.end(),
+                   [](std::pair<int, long long> const &x, std::pair<int, long long> const &y) { return x.first < y.first; });
+  for (auto const &ord : order)
+    for (size_t s0 = 0, nk = by_left[ord.second].size(); s0 < nk; s0 += kLsSlots) {
+      auto const &kv = *by_left.find(ord.second);
       const size_t s1 = std::min(kv.second.size(), s0 + kLsSlots);
       LsBlock b;
       b.xq = kv.first;

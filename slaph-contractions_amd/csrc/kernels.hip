@@ -127,11 +127,16 @@ __global__ __launch_bounds__(256) void k_ahb(const AhBJob *__restrict__ jobs, in
 // through double-buffered LDS (the A and B panels are columns of the same
 // V(t)); the momentum phases of the chunk's sites are staged next to them and
 // applied to the B fragment in registers, so every V element loaded feeds
-// 4 * np MFMAs.
+// 3 * np MFMAs.  The complex product uses three real products (Gauss):
+// conj(a) b = (ax bx + ay by) + i ((ax - ay)(bx + by) - ax bx + ay by), so the
+// real and imaginary parts come from P1 = ax bx, P2 = ay by and
+// P3 = (ax - ay)(bx + by) accumulated separately: 3 MFMAs instead of 4.
 // ---------------------------------------------------------------------------
 constexpr int VDV_T = 32, VDV_KC = 32, VDV_SITES = VDV_KC / 3 + 2;
 
-__global__ __launch_bounds__(256) void k_vdv(const VdvJob *__restrict__ jobs, int N, long long K, int tiles_n) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_vdv(const VdvJob *__restrict__ jobs,
+                                                                                      int N, long long K,
+                                                                                      int tiles_n) {
   const VdvJob &J = jobs[blockIdx.y];
   const int np = J.np;
   const int m0 = (blockIdx.x / tiles_n) * VDV_T, n0 = (blockIdx.x % tiles_n) * VDV_T;
@@ -141,11 +146,12 @@ __global__ __launch_bounds__(256) void k_vdv(const VdvJob *__restrict__ jobs, in
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const double2 *__restrict__ V = J.V;
-  d4 cr[kVdvGroup], ci[kVdvGroup];
+  d4 p1[kVdvGroup], p2[kVdvGroup], p3[kVdvGroup];
 #pragma unroll
   for (int p = 0; p < kVdvGroup; ++p) {
-    cr[p] = d4{0.0, 0.0, 0.0, 0.0};
-    ci[p] = d4{0.0, 0.0, 0.0, 0.0};
+    p1[p] = d4{0.0, 0.0, 0.0, 0.0};
+    p2[p] = d4{0.0, 0.0, 0.0, 0.0};
+    p3[p] = d4{0.0, 0.0, 0.0, 0.0};
   }
   double2 ra[4], rb[4], rp = make_double2(1.0, 0.0);
   const int nk = int((K + VDV_KC - 1) / VDV_KC);
@@ -187,16 +193,15 @@ __global__ __launch_bounds__(256) void k_vdv(const VdvJob *__restrict__ jobs, in
       const int kr = kq + (lane >> 4);
       const double2 a = As[buf][wm * 16 + (lane & 15)][kr];
       const double2 v = Bs[buf][wn * 16 + (lane & 15)][kr];
+      const double ad = a.x - a.y;
       const int sl = int((k0 + kr) / 3 - s0);
 #pragma unroll
       for (int p = 0; p < kVdvGroup; ++p) {
         if (p < np) {
           const double2 b = cmul(Ph[buf][p][sl], v);
-          // conj(a) b = (ax bx + ay by) + i (ax by - ay bx)
-          cr[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, cr[p], 0, 0, 0);
-          cr[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, cr[p], 0, 0, 0);
-          ci[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, ci[p], 0, 0, 0);
-          ci[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.x, ci[p], 0, 0, 0);
+          p1[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, p1[p], 0, 0, 0);
+          p2[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, p2[p], 0, 0, 0);
+          p3[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, b.x + b.y, p3[p], 0, 0, 0);
         }
       }
     }
@@ -211,7 +216,8 @@ __global__ __launch_bounds__(256) void k_vdv(const VdvJob *__restrict__ jobs, in
     for (int i = 0; i < 4; ++i) {
       const int row = m0 + wm * 16 + (lane >> 4) + 4 * i;
       const int col = n0 + wn * 16 + (lane & 15);
-      if (row < N && col < N) C[(long long)col * N + row] = make_double2(cr[p][i], ci[p][i]);
+      if (row < N && col < N)
+        C[(long long)col * N + row] = make_double2(p1[p][i] + p2[p][i], p3[p][i] - p1[p][i] + p2[p][i]);
     }
   }
 }
@@ -443,8 +449,13 @@ __global__ __launch_bounds__(64 * kLsWaves) void k_trace_lstat(
   constexpr int DD = D * D, EPL = DD / 64, CL = kLsStageBytes / (DD * 16);
   __shared__ double2 Ls[CL * DD];
   __shared__ double2 acc[kLsWaves][kLsSlots];
-  const int ti = blockIdx.x / nblock;
-  const LsBlock B = blocks[blockIdx.x % nblock];
+  // Workgroups are dealt round-robin to the 8 XCDs, each with its own L2:
+  // map them so that XCD x runs the x-th contiguous eighth of the (time,
+  // block) sequence, where neighbouring blocks share right matrices.
+  const int G = gridDim.x, per = G / 8, rem = G % 8, x = blockIdx.x % 8, j = blockIdx.x / 8;
+  const int logical = x < rem ? x * (per + 1) + j : rem * (per + 1) + (x - rem) * per + j;
+  const int ti = logical / nblock;
+  const LsBlock B = blocks[logical % nblock];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int k = tid; k < kLsWaves * kLsSlots; k += 64 * kLsWaves) (&acc[0][0])[k] = make_double2(0.0, 0.0);
   const double2 *xb = sX + (Xt[ti] + B.xq) * DD;

@@ -33,7 +33,7 @@ void launch_ahb(const AhBJob *d_jobs, int njobs, int M, int N, int K, hipStream_
 // VdaggerV of one timeslice for a group of up to kVdvGroup momenta:
 // C[p] = V^H diag(phase[p][k/3]) V, V: K x N column-major (ld = K), C: N x N.
 // phase[p] may be null (unit phase, e.g. for a Gram matrix).
-constexpr int kVdvGroup = 8;
+constexpr int kVdvGroup = 6;
 struct VdvJob {
   const double2 *V;
   const double2 *phase[kVdvGroup];
