@@ -248,6 +248,11 @@ struct Family {
   // operands read the same table at the same time), or -1
   std::vector<int> mirror_of;
   bool coincide = false;  // both trace operands read the same factors at every slice pair
+  // subset sums of the trace vectors that multi-trace correlators read
+  std::vector<int> zslot;  // trl -> slot, -1 if none
+  std::vector<ZItem> zitems;
+  long long zstride = 0;  // per slice pair
+  DevBuf d_zitems;
   // left-stationary trace plan (time independent)
   std::vector<LsBlock> ls_blocks;
   std::vector<LsChunk> ls_chunks;
@@ -434,6 +439,8 @@ struct ChunkPlan {
   int S = 0;
   long long prod_total = 0, g_total = 0;
   std::vector<long long> gbase;               // per family: offset of its group sums
+  std::vector<long long> zbase;               // per family: offset of its subset sums
+  long long z_total = 0;
   std::vector<std::vector<NodeChunk>> nodes;  // per family, per node
   DevBuf d_td;                                // time delta per slice pair
   std::vector<DevBuf> d_jobs;                 // per diagram: resolve jobs
@@ -450,8 +457,10 @@ struct ContractCache {
   std::vector<std::vector<int>> need1, need2;
   bool post = false;  // mask groups, mirrors and normalisations are planned
   DevBuf d_gmasks;
+  std::vector<uint16_t> zmask;  // subset list of the Moebius form (empty: not used)
+  DevBuf d_zmask;
   std::vector<ChunkPlan> chunks;
-  DevBuf d_prod, d_gv, d_cval;
+  DevBuf d_prod, d_gv, d_cval, d_zv;
 };
 
 void Engine::contract(int single_block_pair, int kahan) {
@@ -856,6 +865,44 @@ void Engine::contract(int single_block_pair, int kahan) {
         cp.inv_n[i] = 1.0 / double(n);
       }
     }
+    // ---- subset sums for the Moebius form of the masked inner products:
+    // the subset list is every subset of every group mask of a trace vector
+    // that a multi-trace correlator reads (SLAPH_RESOLVE_GROUPS=1 keeps the
+    // direct group loops, a cross-check) ----
+    {
+      std::set<uint16_t> zs;
+      for (size_t fi = 0; fi < fams.size(); ++fi)
+        for (size_t trl = 0; trl < fams[fi].tr_ids.size(); ++trl) {
+          Family const &f = fams[fi];
+          if (!multi[fi][trl] || f.mirror_of[trl] >= 0) continue;
+          for (int g = 0; g < f.ngroup[trl]; ++g) {
+            const uint16_t m = gmasks[f.gmask_off[trl] + g];
+            for (uint16_t sub = m;; sub = uint16_t((sub - 1) & m)) {
+              zs.insert(sub);
+              if (sub == 0) break;
+            }
+          }
+        }
+      C.zmask.clear();
+      if (zs.size() > 1 && zs.size() <= 4096 && !std::getenv("SLAPH_RESOLVE_GROUPS")) C.zmask.assign(zs.begin(), zs.end());
+      const long long nz = (long long)C.zmask.size();
+      for (size_t fi = 0; fi < fams.size(); ++fi) {
+        Family &f = fams[fi];
+        f.zslot.assign(f.tr_ids.size(), -1);
+        f.zitems.clear();
+        if (nz > 0)
+          for (size_t trl = 0; trl < f.tr_ids.size(); ++trl) {
+            if (!multi[fi][trl] || f.mirror_of[trl] >= 0) continue;
+            f.zslot[trl] = int(f.zitems.size());
+            f.zitems.push_back({(long long)f.goff[trl], f.ngroup[trl], f.gmask_off[trl], (long long)f.zitems.size() * nz});
+          }
+        for (size_t trl = 0; trl < f.tr_ids.size(); ++trl)
+          if (f.mirror_of[trl] >= 0) f.zslot[trl] = f.zslot[f.mirror_of[trl]];
+        f.zstride = (long long)f.zitems.size() * nz;
+        upload(f.d_zitems, f.zitems);
+      }
+      upload(C.d_zmask, C.zmask);
+    }
     upload(C.d_gmasks, gmasks);
     C.post = true;
   }
@@ -1017,6 +1064,11 @@ void Engine::contract(int single_block_pair, int kahan) {
           ch.gbase[fi] = ch.g_total;  // trace values go straight into mask-group sums
           ch.g_total += (long long)S * fams[fi].gstride;
         }
+        ch.zbase.resize(fams.size());
+        for (size_t fi = 0; fi < fams.size(); ++fi) {
+          ch.zbase[fi] = ch.z_total;
+          ch.z_total += (long long)S * fams[fi].zstride;
+        }
         ch.nodes.resize(fams.size());
         for (size_t fi = 0; fi < fams.size(); ++fi) {
           Family &f = fams[fi];
@@ -1073,6 +1125,22 @@ void Engine::contract(int single_block_pair, int kahan) {
                   f.gmask_off[trl] + f.ngroup[trl] > (long long)gmasks.size())
                 throw std::logic_error("correlator request indexes outside the group-sum buffer");
             }
+            // Moebius form when every trace of the request has subset sums
+            bool z_ok = cp.reqs[i].size() > 1;
+            for (size_t k = 0; k < cp.reqs[i].size(); ++k)
+              z_ok = z_ok && fams[cp.reqs[i][k].first].zslot[cp.reqs[i][k].second] >= 0;
+            for (size_t k = 0; k < 3; ++k) {
+              jobs[i].tr[k].zbase = -1;
+              jobs[i].tr[k].zstride = 0;
+            }
+            for (size_t k = 0; z_ok && k < cp.reqs[i].size(); ++k) {
+              auto const &f = fams[cp.reqs[i][k].first];
+              const long long nz = (long long)C.zmask.size();
+              jobs[i].tr[k].zbase = ch.zbase[cp.reqs[i][k].first] + f.zslot[cp.reqs[i][k].second] * nz;
+              jobs[i].tr[k].zstride = f.zstride;
+              if (jobs[i].tr[k].zbase + (long long)(S - 1) * f.zstride + nz > ch.z_total)
+                throw std::logic_error("correlator request indexes outside the subset-sum buffer");
+            }
           }
           upload(ch.d_jobs[c], jobs);
         }
@@ -1100,6 +1168,9 @@ void Engine::contract(int single_block_pair, int kahan) {
     C.d_prod.ensure(sizeof(double2) * np_max);
     C.d_gv.ensure(sizeof(double2) * ng_max);
     C.d_cval.ensure(sizeof(double2) * nc_max);
+    long long nz_max = 1;
+    for (auto const &ch : C.chunks) nz_max = std::max(nz_max, ch.z_total);
+    C.d_zv.ensure(sizeof(double2) * nz_max);
   }
 
   // ---- launches: per chunk, products in topological order, then traces,
@@ -1157,11 +1228,16 @@ void Engine::contract(int single_block_pair, int kahan) {
                               nc.ntime, D_, rt, stream_);
         }
       }
+      // subset sums of this family's trace vectors for the Moebius form
+      launch_zeta(C.d_gv.as<double2>(), ch.gbase[fi], f.gstride, C.d_gmasks.as<uint16_t>(), f.d_zitems.as<ZItem>(),
+                  int(f.zitems.size()), C.d_zmask.as<uint16_t>(), int(C.zmask.size()), C.d_zv.as<double2>(),
+                  ch.zbase[fi], f.zstride, S, stream_);
     }
     for (size_t c = 0; c < corrs.size(); ++c) {
       int ncorr = int(corrs[c].reqs.size());
       launch_resolve(C.d_gv.as<double2>(), C.d_gmasks.as<uint16_t>(), ch.d_jobs[c].as<CorrJob>(), ncorr, S,
-                     C.d_cval.as<double2>(), stream_);
+                     C.d_zv.as<double2>(), C.d_zmask.as<uint16_t>(), int(C.zmask.size()), C.d_cval.as<double2>(),
+                     stream_);
       launch_accumulate(C.d_cval.as<double2>(), ch.d_td.as<int>(), S, ncorr, d_sum[c].as<double2>(),
                         d_comp[c].as<double2>(), kahan, stream_);
     }

@@ -656,10 +656,31 @@ __global__ __launch_bounds__(256) void k_trace_grid(const double2 *__restrict__ 
 // resolve_request{1,2,3} (reference: Diagram.cpp:9-94, DilutedTrace.cpp:5-82)
 // on mask-group sums: mean, sum_g L_g sum_{h: m_g & m_h = 0} R_h, and the
 // 3-way form that rejects only masks shared by all three traces
-// (DilutedTrace.cpp:63).  One thread per (slice pair, correlator).
+// (DilutedTrace.cpp:63).  One thread per (slice pair, correlator).  When the
+// traces carry subset sums (zbase >= 0), the 2- and 3-way forms are evaluated
+// by Moebius inversion: sum_S (-1)^|S| Z0(S) Z1(S) (Z2(S)) over the subset
+// list, the same sum re-associated (nz terms instead of ngroup^ntr).
 // ---------------------------------------------------------------------------
+__global__ void k_zeta(const double2 *__restrict__ gv, long long gbase, long long gstride,
+                       const uint16_t *__restrict__ gm, const ZItem *__restrict__ items, int nitems,
+                       const uint16_t *__restrict__ zmask, int nz, double2 *__restrict__ zv, long long zbase,
+                       long long zstride, int nsp) {
+  const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (id >= (long long)nsp * nitems * nz) return;
+  const int q = int(id % nz), it = int((id / nz) % nitems), s = int(id / ((long long)nz * nitems));
+  const ZItem I = items[it];
+  const double2 *v = gv + gbase + s * gstride + I.goff;
+  const uint16_t *m = gm + I.gmask_off;
+  const uint16_t S = zmask[q];
+  double2 z = make_double2(0.0, 0.0);
+  for (int g = 0; g < I.ngroup; ++g)
+    if ((m[g] & S) == S) z = cadd(z, v[g]);
+  zv[zbase + s * zstride + I.zoff + q] = z;
+}
+
 __global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__restrict__ gm,
-                          const CorrJob *__restrict__ jobs, int ncorr, int nsp, double2 *__restrict__ cval) {
+                          const CorrJob *__restrict__ jobs, int ncorr, int nsp, const double2 *__restrict__ zv,
+                          const uint16_t *__restrict__ zmask, int nz, double2 *__restrict__ cval) {
   const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (id >= (long long)ncorr * nsp) return;
   const int corr = int(id % ncorr), s = int(id / ncorr);
@@ -668,6 +689,20 @@ __global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__rest
   double2 res = make_double2(0.0, 0.0);
   if (J.ntr == 1) {
     for (int g = 0; g < J.tr[0].ngroup; ++g) res = cadd(res, v0[g]);
+  } else if (J.tr[0].zbase >= 0) {
+    const double2 *z0 = zv + J.tr[0].zbase + s * J.tr[0].zstride;
+    const double2 *z1 = zv + J.tr[1].zbase + s * J.tr[1].zstride;
+    const double2 *z2 = J.ntr == 3 ? zv + J.tr[2].zbase + s * J.tr[2].zstride : nullptr;
+    for (int q = 0; q < nz; ++q) {
+      double2 t = cmul(z0[q], z1[q]);
+      if (z2) t = cmul(t, z2[q]);
+      if (__popc(zmask[q]) & 1) {
+        res.x -= t.x;
+        res.y -= t.y;
+      } else {
+        res = cadd(res, t);
+      }
+    }
   } else if (J.ntr == 2) {
     const double2 *v1 = gv + J.tr[1].base + s * J.tr[1].stride;
     const uint16_t *m0 = gm + J.tr[0].gmask_off, *m1 = gm + J.tr[1].gmask_off;
@@ -933,11 +968,22 @@ void launch_trace_grid(const double2 *sX, const long long *Xq, const double2 *sY
   SLAPH_HIP(hipGetLastError());
 }
 
-void launch_resolve(const double2 *gv, const uint16_t *gm, const CorrJob *jobs, int ncorr, int nsp, double2 *cval,
-                    hipStream_t s) {
+void launch_zeta(const double2 *gv, long long gbase, long long gstride, const uint16_t *gm, const ZItem *items,
+                 int nitems, const uint16_t *zmask, int nz, double2 *zv, long long zbase, long long zstride, int nsp,
+                 hipStream_t s) {
+  long long n = (long long)nsp * nitems * nz;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zeta, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, gv, gbase, gstride, gm, items, nitems,
+                     zmask, nz, zv, zbase, zstride, nsp);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_resolve(const double2 *gv, const uint16_t *gm, const CorrJob *jobs, int ncorr, int nsp,
+                    const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, hipStream_t s) {
   long long n = (long long)ncorr * nsp;
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_resolve, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, gv, gm, jobs, ncorr, nsp, cval);
+  hipLaunchKernelGGL(k_resolve, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, gv, gm, jobs, ncorr, nsp, zv, zmask,
+                     nz, cval);
   SLAPH_HIP(hipGetLastError());
 }
 

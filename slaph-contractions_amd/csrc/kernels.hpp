@@ -96,7 +96,22 @@ struct TraceRef {
   long long stride;  // group-sum stride per slice pair
   int ngroup;        // number of mask groups
   int gmask_off;     // into group masks[]
+  long long zbase;   // offset of the subset sums Z(S) at slice pair 0, or -1
+  long long zstride;
 };
+// Subset sums of one trace vector: Z(S) = sum of the group sums whose mask
+// contains S, for every S of the subset list zmask[0..nz).  The masked inner
+// products then follow from Moebius inversion on the subset lattice,
+// [m0 & m1 (& m2) == 0] = sum_{S subset of m0 & m1 (& m2)} (-1)^|S|, i.e.
+// sum_S (-1)^|S| Z0(S) Z1(S) (Z2(S)).
+struct ZItem {
+  long long goff;  // the trace's group sums within one slice pair
+  int ngroup, gmask_off;
+  long long zoff;  // its subset sums within one slice pair
+};
+void launch_zeta(const double2 *gvals, long long gbase, long long gstride, const uint16_t *gmasks, const ZItem *items,
+                 int nitems, const uint16_t *zmask, int nz, double2 *zv, long long zbase, long long zstride, int nsp,
+                 hipStream_t s);
 struct CorrJob {
   int ntr;
   double inv_n;  // 1 / number of admitted combinations (host-computed, exact)
@@ -160,7 +175,7 @@ void launch_trace_lstat(const double2 *slabX, const long long *Xt, const double2
                         const long long *slot_dst, double2 *out, long long obase, long long ostride, int ntime, int D,
                         int rt, hipStream_t s);
 void launch_resolve(const double2 *gvals, const uint16_t *gmasks, const CorrJob *jobs, int ncorr, int nsp,
-                    double2 *cval, hipStream_t s);
+                    const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, hipStream_t s);
 void launch_accumulate(const double2 *cval, const int *tdelta, int nsp, int ncorr, double2 *sum, double2 *comp,
                        int kahan, hipStream_t s);
 void launch_scale(double2 *x, long long n, double f, hipStream_t s);

@@ -161,9 +161,11 @@ def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypat
     seeded host inputs: every correlator against the oracle at 1e-12
     norm-wise.  The wide variant (C4cB / C4cC up to |P|^2 = 1) runs the
     non-separable traces and the D = 16 products through both kernels each
-    (left-stationary / one wave per group; MFMA / generic product)."""
+    (left-stationary / one wave per group; MFMA / generic product), and the
+    masked inner products both by Moebius inversion and by direct group loops."""
     say("small pipi", "wide" if infile is WIDE else "", kernels)
-    for var, on in (("SLAPH_TRACE_GROUPS", "groups" in kernels), ("SLAPH_PROD_GENERIC", "generic" in kernels)):
+    for var, on in (("SLAPH_TRACE_GROUPS", "groups" in kernels), ("SLAPH_PROD_GENERIC", "generic" in kernels),
+                    ("SLAPH_RESOLVE_GROUPS", "groups" in kernels)):
         if on:
             monkeypatch.setenv(var, "1")
         else:
