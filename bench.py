@@ -148,6 +148,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="pipi32", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-precision-forms", action="store_true",
+                    help="skip timing the FP32 precision forms against the FP64 path")
     ap.add_argument("--chunk-gib", type=float, default=0.0, help="chunk memory budget (0: automatic)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -213,10 +215,49 @@ def main():
                      "issued_fp64_mfma_flops_per_launch": 0.75 * flops,
                      "issued_frac": 0.75 * achieved / PEAK_FP64_MFMA},
     }
+    if not args.no_precision_forms:
+        pf = precision_forms(ctx, ops, ms / 1000.0)
+        line["precision_forms"] = pf
+        ok = [f for f in pf if f["within_1e-10"]]
+        # `value` is the plain FP64 path; a mixed form would replace it only if
+        # every correlator stayed within 1e-10 of FP64
+        line["fastest_mixed_form_within_1e-10"] = min(ok, key=lambda f: f["s_per_config"]) if ok else None
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, text)
     print(json.dumps(line))
     return 0
+
+
+def precision_forms(ctx, ops, fp64_s):
+    """Time each precision form (one warm-up and one timed configuration on the
+    inputs of the FP64 run) and state its per-correlator relative error against
+    the FP64 path: err = max_t |C(t) - C_fp64(t)| / max_t |C_fp64(t)|."""
+    import numpy as np
+    import slaph
+
+    def correlators():
+        return {d: ctx.correlators(d) for d in ctx.diagrams()}
+
+    ref = correlators()
+    out = []
+    for vdv, fac in (("fp32", "fp64"), ("fp64", "fp32"), ("fp32", "fp32")):
+        ctx.set_precision(vdv, fac)
+        ops.create_operators()
+        slaph.contract(ctx)
+        t0 = time.perf_counter()
+        ops.create_operators()
+        slaph.contract(ctx)
+        s = time.perf_counter() - t0
+        got = correlators()
+        err = np.array([np.max(np.abs(got[d][n] - v)) / max(np.max(np.abs(v)), 1e-300)
+                        for d, vals in ref.items() for n, v in vals.items()])
+        out.append({"vdaggerv": vdv, "factors": fac, "s_per_config": s, "speedup_vs_fp64": fp64_s / s,
+                    "max_rel_err": float(err.max()), "median_rel_err": float(np.median(err)),
+                    "n_correlators": int(err.size), "within_1e-10": bool(err.max() <= 1e-10)})
+        print("# precision form vdaggerv=%s factors=%s: %.3f s, max rel err %.2e" % (vdv, fac, s, err.max()),
+              file=sys.stderr, flush=True)
+    ctx.set_precision("fp64", "fp64")
+    return out
 
 
 if __name__ == "__main__":

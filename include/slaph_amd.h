@@ -120,6 +120,16 @@ int slc_stage_time(const slc_ctx *ctx, const char *what, double *ms);
  * negative values are rejected. */
 int slc_set_chunk_budget(slc_ctx *ctx, long long bytes);
 
+/* Precision forms beside the plain FP64 path (which the reference's
+ * complex<double> arithmetic, OperatorFactory.cpp:294-300 and
+ * DilutedFactorFactory.cpp:33-253, defines; no reference counterpart):
+ *   vdaggerv: 0 = FP64 MFMA (default), 1 = single-pass FP32 MFMA (V and the
+ *             phases rounded to FP32, FP32 accumulation);
+ *   factors:  0 = FP64 (default), 1 = diluted factors and product tables
+ *             stored in FP32 (built, multiplied and traced in FP64).
+ * Correlator accumulation is FP64 (optionally Kahan) in every form. */
+int slc_set_precision(slc_ctx *ctx, int vdaggerv, int factors);
+
 /* C = A^H diag(phase[k/3]) B on the device (batch of `batch` products with
  * element strides sA, sB, sC): the kernel behind VdaggerV
  * (OperatorFactory.cpp:294-300).  A, B, C, phase are device pointers; phase

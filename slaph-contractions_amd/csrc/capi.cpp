@@ -200,6 +200,10 @@ int slc_set_chunk_budget(slc_ctx *ctx, long long bytes) {
   });
 }
 
+int slc_set_precision(slc_ctx *ctx, int vdaggerv, int factors) {
+  return guarded(ctx, [&] { ctx->eng->set_precision(vdaggerv, factors); });
+}
+
 int slc_ahb_device(slc_ctx *ctx, const double *A, const double *B, double *C, const double *phase, int M, int N,
                    int K, long long lda, long long ldb, long long ldc, int batch, long long sA, long long sB,
                    long long sC) {

@@ -719,13 +719,27 @@ void Engine::contract(int single_block_pair, int kahan) {
     C.need1.assign(need1.begin(), need1.end());
     C.need2.assign(need2.begin(), need2.end());
   }
+  // FP32-storage precision form: the factors are built in FP64 and stored
+  // rounded to FP32 for the products and traces (arithmetic stays FP64)
+  const int f32 = prec_factors_;
+  const size_t elem = f32 ? sizeof(float2) : sizeof(double2);
   {
     auto t0 = std::chrono::steady_clock::now();
     build_factors(C.need1, C.need2);
+    if (f32) {
+      DevBuf *src[3] = {&q0_, &q1_, &q2_}, *dst[3] = {&q0f_, &q1f_, &q2f_};
+      for (int q = 0; q < 3; ++q) {
+        long long n = (long long)(src[q]->bytes() / sizeof(double2));
+        dst[q]->ensure(std::max<size_t>(16, sizeof(float2) * n));
+        launch_to_f32(src[q]->as<double2>(), dst[q]->as<void>(), n, stream_);
+      }
+      SLAPH_HIP(hipStreamSynchronize(stream_));
+    }
     times_.factors_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
-  auto slab_of = [&](int q) -> const double2 * {
-    return q == 0 ? q0_.as<double2>() : (q == 1 ? q1_.as<double2>() : q2_.as<double2>());
+  auto slab_of = [&](int q) -> const void * {
+    if (f32) return q == 0 ? q0f_.as<void>() : (q == 1 ? q1f_.as<void>() : q2f_.as<void>());
+    return q == 0 ? q0_.as<void>() : (q == 1 ? q1_.as<void>() : q2_.as<void>());
   };
   auto leaf_base = [&](LeafSpec const &ls, std::vector<int> const &tt) -> long long {
     if (ls.q == 0) return (long long)tt[ls.a] * q_total_[0];
@@ -1165,7 +1179,7 @@ void Engine::contract(int single_block_pair, int kahan) {
       ng_max = std::max(ng_max, ch.g_total);
       nc_max = std::max(nc_max, (long long)max_corr * ch.S);
     }
-    C.d_prod.ensure(sizeof(double2) * np_max);
+    C.d_prod.ensure(elem * np_max);
     C.d_gv.ensure(sizeof(double2) * ng_max);
     C.d_cval.ensure(sizeof(double2) * nc_max);
     long long nz_max = 1;
@@ -1194,11 +1208,11 @@ void Engine::contract(int single_block_pair, int kahan) {
         NodeChunk const &nc = ch.nodes[fi][n];
         if (ns.kind == kLeaf || np.owner != int(n)) continue;
         const bool root = n + 1 == f.shape.nodes.size();
-        auto src = [&](int child) -> const double2 * {
+        auto src = [&](int child) -> const void * {
           if (child < 0) return nullptr;
           auto const &cs = f.shape.nodes[child];
           if (cs.kind == kLeaf) return slab_of(f.shape.leaves[cs.leaf].q);
-          return C.d_prod.as<double2>();
+          return C.d_prod.as<void>();
         };
         // operands stored transposed: only a trace's right operand may be
         auto is_t = [&](int child) {
@@ -1209,23 +1223,23 @@ void Engine::contract(int single_block_pair, int kahan) {
         const long long gb = ch.gbase[fi];
         if (!root) {
           launch_prod(src(ns.left), nc.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
-                      nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), C.d_prod.as<double2>(), nc.base, np.stride,
-                      np.d_combos.as<ComboProd>(), int(np.combos.size()), nc.ntime, D_, np.transposed ? 1 : 0,
+                      nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), C.d_prod.as<void>(), nc.base, np.stride,
+                      np.d_combos.as<ComboProd>(), int(np.combos.size()), nc.ntime, D_, np.transposed ? 1 : 0, f32,
                       stream_);
         } else if (nc.grid) {
           launch_trace_grid(src(ns.left), np.d_Xq.as<long long>(), src(ns.right), np.d_Yq.as<long long>(),
                             f.d_gitems.as<GroupItem>(), int(f.gitems.size()), f.d_gpairs.as<int2>(),
-                            nc.d_tile.as<GridTile>(), 1, C.d_gv.as<double2>(), gb, f.gstride, D_, rt, stream_);
+                            nc.d_tile.as<GridTile>(), 1, C.d_gv.as<double2>(), gb, f.gstride, D_, rt, f32, stream_);
         } else if (ns.kind == kTrace2 && !f.ls_blocks.empty() && !lstat_off) {
           launch_trace_lstat(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
                              f.d_lsb.as<LsBlock>(), int(f.ls_blocks.size()), f.d_lsc.as<LsChunk>(),
                              f.d_lst.as<LsTask>(), f.d_lsd.as<long long>(), C.d_gv.as<double2>(), gb, f.gstride,
-                             nc.ntime, D_, rt, stream_);
+                             nc.ntime, D_, rt, f32, stream_);
         } else {
           launch_trace_groups(src(ns.left), nc.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
                               nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), f.d_gitems.as<GroupItem>(),
                               int(f.gitems.size()), f.d_gpairs.as<int2>(), C.d_gv.as<double2>(), gb, f.gstride,
-                              nc.ntime, D_, rt, stream_);
+                              nc.ntime, D_, rt, f32, stream_);
         }
       }
       // subset sums of this family's trace vectors for the Moebius form

@@ -237,7 +237,7 @@ void Engine::synthetic_inputs(unsigned long long seed, bool orthonormalise) {
     upload(dj, jobs);
     std::vector<cplx> hG((size_t)Lt_ * N_ * N_);
     for (int pass = 0; pass < 2; ++pass) {
-      launch_vdv(dj.as<VdvJob>(), Lt_, N_, K_, stream_);
+      launch_vdv(dj.as<VdvJob>(), Lt_, N_, K_, 0, stream_);
       SLAPH_HIP(hipMemcpyAsync(hG.data(), G.as<void>(), sizeof(cplx) * hG.size(), hipMemcpyDeviceToHost, stream_));
       SLAPH_HIP(hipStreamSynchronize(stream_));
       for (int t = 0; t < Lt_; ++t) {
@@ -340,7 +340,7 @@ void Engine::create_operators() {
   upload(dj, jobs);
   EventTimer tm(stream_);
   for (size_t i = 0; i < jobs.size(); i += 65535)
-    launch_vdv(dj.as<VdvJob>() + i, int(std::min<size_t>(65535, jobs.size() - i)), N_, K_, stream_);
+    launch_vdv(dj.as<VdvJob>() + i, int(std::min<size_t>(65535, jobs.size() - i)), N_, K_, prec_vdv_, stream_);
   times_.vdaggerv_kernel_ms = tm.stop();
   times_.vdaggerv_launches = int((jobs.size() + 65534) / 65535);
   times_.vdaggerv_ms = times_.vdaggerv_kernel_ms;

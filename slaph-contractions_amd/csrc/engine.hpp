@@ -113,6 +113,16 @@ class Engine {
     chunk_budget_ = bytes;
     plan_.reset();  // chunk tables depend on the budget
   }
+  // Precision forms: vdaggerv 0 = FP64, 1 = single-pass FP32 MFMA; factors
+  // 0 = FP64, 1 = factor slabs and product tables stored in FP32 (arithmetic
+  // and every accumulation in FP64).
+  void set_precision(int vdaggerv, int factors) {
+    if (vdaggerv < 0 || vdaggerv > 1 || factors < 0 || factors > 1)
+      throw std::out_of_range("precision: vdaggerv and factors must be 0 or 1");
+    prec_vdv_ = vdaggerv;
+    if (factors != prec_factors_) plan_.reset();
+    prec_factors_ = factors;
+  }
   StageTimes const &times() const { return times_; }
   hipStream_t stream() const { return stream_; }
 
@@ -154,6 +164,8 @@ class Engine {
   std::vector<DiagramResult> results_;
 
   size_t chunk_budget_ = 0;  // 0: from the free device memory
+  int prec_vdv_ = 0, prec_factors_ = 0;
+  DevBuf q0f_, q1f_, q2f_;  // FP32 copies of the factor slabs (factors precision 1)
   StageTimes times_;
   // contract()'s plan: everything derived from the input file alone (quantum
   // numbers, trace groups, per-chunk index tables); rebuilt after configure()

@@ -26,6 +26,16 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // conj(a) * b
   return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+// Factor slabs and product tables are stored as double2 (plain path) or
+// float2 (FP32-storage precision form); arithmetic is FP64 either way.
+__device__ __forceinline__ double2 ld2(const double2 *p) { return *p; }
+__device__ __forceinline__ double2 ld2(const float2 *p) {
+  const float2 v = *p;
+  return make_double2(double(v.x), double(v.y));
+}
+__device__ __forceinline__ void st2(double2 *p, double2 v) { *p = v; }
+__device__ __forceinline__ void st2(float2 *p, double2 v) { *p = make_float2(float(v.x), float(v.y)); }
 __device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b) {
   acc.x = fma(a.x, b.x, acc.x);
   acc.x = fma(-a.y, b.y, acc.x);
@@ -134,24 +144,55 @@ __global__ __launch_bounds__(256) void k_ahb(const AhBJob *__restrict__ jobs, in
 // ---------------------------------------------------------------------------
 constexpr int VDV_T = 32, VDV_KC = 32, VDV_SITES = VDV_KC / 3 + 2;
 
+// Operand type, MFMA and output layout per precision: FP64 (the plain path)
+// or single-pass FP32 (a precision form: V and the phases rounded to FP32,
+// FP32 accumulation).  16x16x4 f64 result element i of lane l sits in row
+// l/16 + 4i, the f32 one in row 4(l/16) + i.
+template <bool kF32>
+struct VdvPrec;
+template <>
+struct VdvPrec<false> {
+  typedef double R;
+  typedef double2 S;
+  typedef d4 Acc;
+  static __device__ Acc mfma(R a, R b, Acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  static __device__ S cvt(double2 x) { return x; }
+  static __device__ int row(int lane, int i) { return (lane >> 4) + 4 * i; }
+};
+typedef float f4 __attribute__((ext_vector_type(4)));
+template <>
+struct VdvPrec<true> {
+  typedef float R;
+  typedef float2 S;
+  typedef f4 Acc;
+  static __device__ Acc mfma(R a, R b, Acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static __device__ S cvt(double2 x) { return make_float2(float(x.x), float(x.y)); }
+  static __device__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
+};
+
+template <bool kF32>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_vdv(const VdvJob *__restrict__ jobs,
                                                                                       int N, long long K,
                                                                                       int tiles_n) {
+  typedef VdvPrec<kF32> P;
+  typedef typename P::R R;
+  typedef typename P::S S;
+  typedef typename P::Acc Acc;
   const VdvJob &J = jobs[blockIdx.y];
   const int np = J.np;
   const int m0 = (blockIdx.x / tiles_n) * VDV_T, n0 = (blockIdx.x % tiles_n) * VDV_T;
-  __shared__ double2 As[2][VDV_T][VDV_KC + 1];
-  __shared__ double2 Bs[2][VDV_T][VDV_KC + 1];
-  __shared__ double2 Ph[2][kVdvGroup][VDV_SITES];
+  __shared__ S As[2][VDV_T][VDV_KC + 1];
+  __shared__ S Bs[2][VDV_T][VDV_KC + 1];
+  __shared__ S Ph[2][kVdvGroup][VDV_SITES];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const double2 *__restrict__ V = J.V;
-  d4 p1[kVdvGroup], p2[kVdvGroup], p3[kVdvGroup];
+  Acc p1[kVdvGroup], p2[kVdvGroup], p3[kVdvGroup];
 #pragma unroll
   for (int p = 0; p < kVdvGroup; ++p) {
-    p1[p] = d4{0.0, 0.0, 0.0, 0.0};
-    p2[p] = d4{0.0, 0.0, 0.0, 0.0};
-    p3[p] = d4{0.0, 0.0, 0.0, 0.0};
+    p1[p] = Acc{0, 0, 0, 0};
+    p2[p] = Acc{0, 0, 0, 0};
+    p3[p] = Acc{0, 0, 0, 0};
   }
   double2 ra[4], rb[4], rp = make_double2(1.0, 0.0);
   const int nk = int((K + VDV_KC - 1) / VDV_KC);
@@ -176,10 +217,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int idx = tid + r * 256;
-      As[buf][idx / VDV_KC][idx % VDV_KC] = ra[r];
-      Bs[buf][idx / VDV_KC][idx % VDV_KC] = rb[r];
+      As[buf][idx / VDV_KC][idx % VDV_KC] = P::cvt(ra[r]);
+      Bs[buf][idx / VDV_KC][idx % VDV_KC] = P::cvt(rb[r]);
     }
-    if (tid < kVdvGroup * VDV_SITES) Ph[buf][tid / VDV_SITES][tid % VDV_SITES] = rp;
+    if (tid < kVdvGroup * VDV_SITES) Ph[buf][tid / VDV_SITES][tid % VDV_SITES] = P::cvt(rp);
   };
   load(0);
   store(0);
@@ -191,17 +232,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
     for (int kq = 0; kq < VDV_KC; kq += 4) {
       const int kr = kq + (lane >> 4);
-      const double2 a = As[buf][wm * 16 + (lane & 15)][kr];
-      const double2 v = Bs[buf][wn * 16 + (lane & 15)][kr];
-      const double ad = a.x - a.y;
+      const S a = As[buf][wm * 16 + (lane & 15)][kr];
+      const S v = Bs[buf][wn * 16 + (lane & 15)][kr];
+      const R ad = a.x - a.y;
       const int sl = int((k0 + kr) / 3 - s0);
 #pragma unroll
       for (int p = 0; p < kVdvGroup; ++p) {
         if (p < np) {
-          const double2 b = cmul(Ph[buf][p][sl], v);
-          p1[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, p1[p], 0, 0, 0);
-          p2[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, p2[p], 0, 0, 0);
-          p3[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, b.x + b.y, p3[p], 0, 0, 0);
+          const S ph = Ph[buf][p][sl];
+          const R bx = ph.x * v.x - ph.y * v.y, by = ph.x * v.y + ph.y * v.x;
+          p1[p] = P::mfma(a.x, bx, p1[p]);
+          p2[p] = P::mfma(a.y, by, p2[p]);
+          p3[p] = P::mfma(ad, bx + by, p3[p]);
         }
       }
     }
@@ -214,10 +256,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     double2 *C = J.C[p];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int row = m0 + wm * 16 + (lane >> 4) + 4 * i;
+      const int row = m0 + wm * 16 + P::row(lane, i);
       const int col = n0 + wn * 16 + (lane & 15);
       if (row < N && col < N)
-        C[(long long)col * N + row] = make_double2(p1[p][i] + p2[p][i], p3[p][i] - p1[p][i] + p2[p][i]);
+        C[(long long)col * N + row] =
+            make_double2(double(p1[p][i]) + double(p2[p][i]), double(p3[p][i]) - double(p1[p][i]) + double(p2[p][i]));
     }
   }
 }
@@ -340,28 +383,29 @@ __global__ void k_identity(double2 *out, int N, int count) {
 // ---------------------------------------------------------------------------
 constexpr int kMaxD = 32;
 
-__global__ __launch_bounds__(64) void k_prod(const double2 *__restrict__ sX, const long long *__restrict__ Xt,
-                                             const long long *__restrict__ Xq, const double2 *__restrict__ sY,
+template <typename T>
+__global__ __launch_bounds__(64) void k_prod(const T *__restrict__ sX, const long long *__restrict__ Xt,
+                                             const long long *__restrict__ Xq, const T *__restrict__ sY,
                                              const long long *__restrict__ Yt, const long long *__restrict__ Yq,
-                                             double2 *__restrict__ out, long long obase, long long ostride,
+                                             T *__restrict__ out, long long obase, long long ostride,
                                              const ComboProd *__restrict__ combos, int D, int transposed) {
   __shared__ double2 A[kMaxD * kMaxD], B[kMaxD * kMaxD];
   const ComboProd c = combos[blockIdx.x];
   const int ti = blockIdx.y;
   const long long DD = (long long)D * D;
-  const double2 *x = sX + (Xt[ti] + Xq[c.qn] + c.i) * DD;
-  const double2 *y = sY + (Yt[ti] + Yq[c.qn] + c.j) * DD;
+  const T *x = sX + (Xt[ti] + Xq[c.qn] + c.i) * DD;
+  const T *y = sY + (Yt[ti] + Yq[c.qn] + c.j) * DD;
   for (int e = threadIdx.x; e < D * D; e += 64) {
-    A[e] = x[e];
-    B[e] = y[e];
+    A[e] = ld2(x + e);
+    B[e] = ld2(y + e);
   }
   __syncthreads();
-  double2 *o = out + (obase + ti * ostride + c.o) * DD;
+  T *o = out + (obase + ti * ostride + c.o) * DD;
   for (int e = threadIdx.x; e < D * D; e += 64) {
     const int r = e % D, col = e / D;
     double2 s = make_double2(0.0, 0.0);
     for (int k = 0; k < D; ++k) cfma(s, A[k * D + r], B[col * D + k]);
-    o[transposed ? col + r * D : e] = s;
+    st2(o + (transposed ? col + r * D : e), s);
   }
 }
 
@@ -377,11 +421,11 @@ __global__ __launch_bounds__(64) void k_prod(const double2 *__restrict__ sX, con
 // registers.
 constexpr int kProdPerWave = 4;
 
-template <bool kTransposed>
-__global__ __launch_bounds__(256) void k_prod16(const double2 *__restrict__ sX, const long long *__restrict__ Xt,
-                                                const long long *__restrict__ Xq, const double2 *__restrict__ sY,
+template <typename T, bool kTransposed>
+__global__ __launch_bounds__(256) void k_prod16(const T *__restrict__ sX, const long long *__restrict__ Xt,
+                                                const long long *__restrict__ Xq, const T *__restrict__ sY,
                                                 const long long *__restrict__ Yt, const long long *__restrict__ Yq,
-                                                double2 *__restrict__ out, long long obase, long long ostride,
+                                                T *__restrict__ out, long long obase, long long ostride,
                                                 const ComboProd *__restrict__ combos, int ncombo) {
   const int ti = blockIdx.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -396,13 +440,13 @@ __global__ __launch_bounds__(256) void k_prod16(const double2 *__restrict__ sX, 
     const long long xi = xt + Xq[cb.qn] + cb.i;
     if (xi != xcur) {
 #pragma unroll
-      for (int s = 0; s < 4; ++s) xa[s] = sX[xi * 256 + lane + 64 * s];
+      for (int s = 0; s < 4; ++s) xa[s] = ld2(sX + xi * 256 + lane + 64 * s);
       xcur = xi;
     }
-    const double2 *y = sY + (yt + Yq[cb.qn] + cb.j) * 256 + yoff;
+    const T *y = sY + (yt + Yq[cb.qn] + cb.j) * 256 + yoff;
     double2 yb[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) yb[s] = y[4 * s];
+    for (int s = 0; s < 4; ++s) yb[s] = ld2(y + 4 * s);
     d4 dr = d4{0.0, 0.0, 0.0, 0.0}, di = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -419,9 +463,9 @@ __global__ __launch_bounds__(256) void k_prod16(const double2 *__restrict__ sX, 
         di = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].x, xa[s].y, di, 0, 0, 0);
       }
     }
-    double2 *o = out + (obase + ti * ostride + cb.o) * 256;
+    T *o = out + (obase + ti * ostride + cb.o) * 256;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) o[lane + 64 * i] = make_double2(dr[i], di[i]);
+    for (int i = 0; i < 4; ++i) st2(o + lane + 64 * i, make_double2(dr[i], di[i]));
   }
 }
 
@@ -440,9 +484,9 @@ __global__ __launch_bounds__(256) void k_prod16(const double2 *__restrict__ sX, 
 // ---------------------------------------------------------------------------
 constexpr int kLsStageBytes = 48 * 1024;
 
-template <int D>
+template <int D, typename TS>
 __global__ __launch_bounds__(64 * kLsWaves) void k_trace_lstat(
-    const double2 *__restrict__ sX, const long long *__restrict__ Xt, const double2 *__restrict__ sY,
+    const TS *__restrict__ sX, const long long *__restrict__ Xt, const TS *__restrict__ sY,
     const long long *__restrict__ Yt, const LsBlock *__restrict__ blocks, int nblock, const LsChunk *__restrict__ chunks,
     const LsTask *__restrict__ tasks, const long long *__restrict__ slot_dst, double2 *__restrict__ out,
     long long obase, long long ostride, int rt) {
@@ -458,14 +502,14 @@ __global__ __launch_bounds__(64 * kLsWaves) void k_trace_lstat(
   const LsBlock B = blocks[logical % nblock];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int k = tid; k < kLsWaves * kLsSlots; k += 64 * kLsWaves) (&acc[0][0])[k] = make_double2(0.0, 0.0);
-  const double2 *xb = sX + (Xt[ti] + B.xq) * DD;
-  const double2 *yb = sY + Yt[ti] * DD;
+  const TS *xb = sX + (Xt[ti] + B.xq) * DD;
+  const TS *yb = sY + Yt[ti] * DD;
   for (int c = 0; c < B.nchunk; ++c) {
     const LsChunk C = chunks[B.chunk_off + c];
     __syncthreads();  // the previous chunk is no longer read
     for (int idx = tid; idx < C.n * DD; idx += 64 * kLsWaves) {
       const int m = idx / DD, f = idx % DD;
-      Ls[m * DD + (rt ? f : f / D + (f % D) * D)] = xb[(long long)(C.i0 + m) * DD + f];
+      Ls[m * DD + (rt ? f : f / D + (f % D) * D)] = ld2(xb + (long long)(C.i0 + m) * DD + f);
     }
     __syncthreads();
     const int t0 = C.wtask[wave], t1 = C.wtask[wave + 1];
@@ -474,13 +518,13 @@ __global__ __launch_bounds__(64 * kLsWaves) void k_trace_lstat(
     double2 r[EPL], rn[EPL];
     LsTask T = tasks[t0];
 #pragma unroll
-    for (int k = 0; k < EPL; ++k) r[k] = yb[(long long)T.yrel * DD + lane + 64 * k];
+    for (int k = 0; k < EPL; ++k) r[k] = ld2(yb + (long long)T.yrel * DD + lane + 64 * k);
     for (int t = t0; t < t1; ++t) {
       LsTask Tn = T;
       if (t + 1 < t1) {
         Tn = tasks[t + 1];
 #pragma unroll
-        for (int k = 0; k < EPL; ++k) rn[k] = yb[(long long)Tn.yrel * DD + lane + 64 * k];
+        for (int k = 0; k < EPL; ++k) rn[k] = ld2(yb + (long long)Tn.yrel * DD + lane + 64 * k);
       }
       const double2 *la = Ls + T.ia * DD + lane;
 #pragma unroll
@@ -530,25 +574,26 @@ __global__ __launch_bounds__(64 * kLsWaves) void k_trace_lstat(
 // accumulates its share of sum_combos sum_{a,b} X[a,b] Y[b,a] over all
 // combinations of the group, then one wave reduction.
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(64) void k_trace_groups(
-    const double2 *__restrict__ sX, const long long *__restrict__ Xt, const long long *__restrict__ Xq,
-    const double2 *__restrict__ sY, const long long *__restrict__ Yt, const long long *__restrict__ Yq,
+    const T *__restrict__ sX, const long long *__restrict__ Xt, const long long *__restrict__ Xq,
+    const T *__restrict__ sY, const long long *__restrict__ Yt, const long long *__restrict__ Yq,
     const GroupItem *__restrict__ items, const int2 *__restrict__ pairs, double2 *__restrict__ out, long long obase,
     long long ostride, int D, int rt) {
   const GroupItem g = items[blockIdx.x];
   const int ti = blockIdx.y;
   const long long DD = (long long)D * D;
-  const double2 *xb = sX + (Xt[ti] + Xq[g.qn]) * DD;
+  const T *xb = sX + (Xt[ti] + Xq[g.qn]) * DD;
   double re = 0.0, im = 0.0;
   if (sY) {
-    const double2 *yb = sY + (Yt[ti] + Yq[g.qn]) * DD;
+    const T *yb = sY + (Yt[ti] + Yq[g.qn]) * DD;
     for (int c = 0; c < g.n; ++c) {
       const int2 p = pairs[g.list_off + c];
-      const double2 *x = xb + p.x * DD;
-      const double2 *y = yb + p.y * DD;
+      const T *x = xb + p.x * DD;
+      const T *y = yb + p.y * DD;
       for (int e = threadIdx.x; e < D * D; e += 64) {
         const int a = e % D, b = e / D;
-        const double2 u = x[e], v = y[rt ? e : a * D + b];
+        const double2 u = ld2(x + e), v = ld2(y + (rt ? e : a * D + b));
         re = fma(u.x, v.x, re);
         re = fma(-u.y, v.y, re);
         im = fma(u.x, v.y, im);
@@ -557,10 +602,11 @@ __global__ __launch_bounds__(64) void k_trace_groups(
     }
   } else {
     for (int c = 0; c < g.n; ++c) {
-      const double2 *x = xb + pairs[g.list_off + c].x * DD;
+      const T *x = xb + pairs[g.list_off + c].x * DD;
       for (int a = threadIdx.x; a < D; a += 64) {
-        re += x[a * D + a].x;
-        im += x[a * D + a].y;
+        const double2 u = ld2(x + a * D + a);
+        re += u.x;
+        im += u.y;
       }
     }
   }
@@ -591,20 +637,21 @@ __device__ __forceinline__ void mfma_c(d4 &cr, d4 &ci, double2 u, double2 v) {
   ci = __builtin_amdgcn_mfma_f64_16x16x4f64(u.y, v.x, ci, 0, 0, 0);
 }
 
-__global__ __launch_bounds__(256) void k_trace_grid(const double2 *__restrict__ sX, const long long *__restrict__ Xq,
-                                                    const double2 *__restrict__ sY, const long long *__restrict__ Yq,
+template <typename T>
+__global__ __launch_bounds__(256) void k_trace_grid(const T *__restrict__ sX, const long long *__restrict__ Xq,
+                                                    const T *__restrict__ sY, const long long *__restrict__ Yq,
                                                     const GroupItem *__restrict__ items, const int2 *__restrict__ pairs,
                                                     const GridTile *__restrict__ tiles, double2 *__restrict__ out,
                                                     long long obase, long long ostride, int D, int rt) {
   const GroupItem g = items[blockIdx.x];
-  const GridTile &T = tiles[blockIdx.y];
+  const GridTile &Tl = tiles[blockIdx.y];
   __shared__ double2 part[4 * 256];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int DD = D * D, q = lane >> 4;
-  const int nL = T.nL, nR = T.nR;
+  const int nL = Tl.nL, nR = Tl.nR;
   const int la = min(lane & 15, nL - 1), lb = min(lane & 15, nR - 1);  // rows past the tile repeat its last
-  const double2 *xrow = sX + (T.Lt[la] + Xq[g.qn]) * DD;
-  const double2 *yrow = sY + (T.Rt[lb] + Yq[g.qn]) * DD;
+  const T *xrow = sX + (Tl.Lt[la] + Xq[g.qn]) * DD;
+  const T *yrow = sY + (Tl.Rt[lb] + Yq[g.qn]) * DD;
   const int2 *pl = pairs + g.list_off;
   d4 cr = d4{0.0, 0.0, 0.0, 0.0}, ci = d4{0.0, 0.0, 0.0, 0.0};
   int run = 0;
@@ -612,23 +659,23 @@ __global__ __launch_bounds__(256) void k_trace_grid(const double2 *__restrict__ 
     const int i = pl[c].x;
     const bool two = c + 1 < g.n && pl[c + 1].x == i;
     if (run % 4 == wave) {
-      const double2 *x = xrow + (long long)i * DD + q;
-      const double2 *y0 = yrow + (long long)pl[c].y * DD;
-      const double2 *y1 = yrow + (long long)pl[two ? c + 1 : c].y * DD;
+      const T *x = xrow + (long long)i * DD + q;
+      const T *y0 = yrow + (long long)pl[c].y * DD;
+      const T *y1 = yrow + (long long)pl[two ? c + 1 : c].y * DD;
       if (rt) {
         y0 += q;
         y1 += q;
         if (two) {
 #pragma unroll 4
-          for (int s = 0; s < DD / 4; ++s) mfma_c(cr, ci, x[4 * s], cadd(y0[4 * s], y1[4 * s]));
+          for (int s = 0; s < DD / 4; ++s) mfma_c(cr, ci, ld2(x + 4 * s), cadd(ld2(y0 + 4 * s), ld2(y1 + 4 * s)));
         } else {
 #pragma unroll 4
-          for (int s = 0; s < DD / 4; ++s) mfma_c(cr, ci, x[4 * s], y0[4 * s]);
+          for (int s = 0; s < DD / 4; ++s) mfma_c(cr, ci, ld2(x + 4 * s), ld2(y0 + 4 * s));
         }
       } else {
         for (int s = 0; s < DD / 4; ++s) {
           const int e = 4 * s + q, er = (e % D) * D + e / D;
-          mfma_c(cr, ci, x[4 * s], two ? cadd(y0[er], y1[er]) : y0[er]);
+          mfma_c(cr, ci, ld2(x + 4 * s), two ? cadd(ld2(y0 + er), ld2(y1 + er)) : ld2(y0 + er));
         }
       }
     }
@@ -647,7 +694,7 @@ __global__ __launch_bounds__(256) void k_trace_grid(const double2 *__restrict__ 
     s = cadd(s, part[256 + tid]);
     s = cadd(s, part[512 + tid]);
     s = cadd(s, part[768 + tid]);
-    const int sp = (a < nL && b < nR) ? T.sidx[a][b] : -1;
+    const int sp = (a < nL && b < nR) ? Tl.sidx[a][b] : -1;
     if (sp >= 0) out[obase + sp * ostride + g.dst] = s;
   }
 }
@@ -759,6 +806,11 @@ __global__ void k_accumulate(const double2 *__restrict__ cval, const int *__rest
   }
 }
 
+__global__ void k_to_f32(const double2 *__restrict__ in, float2 *__restrict__ out, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = make_float2(float(in[i].x), float(in[i].y));
+}
+
 __global__ void k_scale(double2 *x, long long n, double f) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     x[i].x *= f;
@@ -842,11 +894,14 @@ void launch_ahb(const AhBJob *d_jobs, int njobs, int M, int N, int K, hipStream_
   SLAPH_HIP(hipGetLastError());
 }
 
-void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, hipStream_t s) {
+void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, int f32, hipStream_t s) {
   if (njobs <= 0) return;
   if (njobs > 65535) throw std::runtime_error("launch_vdv: too many jobs in one launch");
   const int tiles = (N + VDV_T - 1) / VDV_T;
-  hipLaunchKernelGGL(k_vdv, dim3(tiles * tiles, njobs), dim3(256), 0, s, d_jobs, N, K, tiles);
+  if (f32)
+    hipLaunchKernelGGL(k_vdv<true>, dim3(tiles * tiles, njobs), dim3(256), 0, s, d_jobs, N, K, tiles);
+  else
+    hipLaunchKernelGGL(k_vdv<false>, dim3(tiles * tiles, njobs), dim3(256), 0, s, d_jobs, N, K, tiles);
   SLAPH_HIP(hipGetLastError());
 }
 
@@ -887,25 +942,68 @@ void launch_identity(double2 *out, int N, int count, hipStream_t s) {
   SLAPH_HIP(hipGetLastError());
 }
 
-void launch_prod(const double2 *sX, const long long *Xt, const long long *Xq, const double2 *sY, const long long *Yt,
-                 const long long *Yq, double2 *out, long long obase, long long ostride, const ComboProd *combos,
-                 int ncombo, int ntime, int D, int transposed, hipStream_t s) {
-  if (ncombo <= 0 || ntime <= 0) return;
-  if (D > kMaxD) throw std::runtime_error("factor dimension above 32 is not supported");
-  if (ntime > 65535) throw std::runtime_error("launch_prod: too many time keys");
+namespace {
+template <typename T>
+void launch_prod_t(const void *sX, const long long *Xt, const long long *Xq, const void *sY, const long long *Yt,
+                   const long long *Yq, void *out, long long obase, long long ostride, const ComboProd *combos,
+                   int ncombo, int ntime, int D, int transposed, hipStream_t s) {
+  const T *x = static_cast<const T *>(sX), *y = static_cast<const T *>(sY);
+  T *o = static_cast<T *>(out);
   if (D == 16 && !std::getenv("SLAPH_PROD_GENERIC")) {
     const int per_block = 4 * kProdPerWave;
     const dim3 grid((ncombo + per_block - 1) / per_block, ntime);
     if (transposed)
-      hipLaunchKernelGGL(k_prod16<true>, grid, dim3(256), 0, s, sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos,
+      hipLaunchKernelGGL((k_prod16<T, true>), grid, dim3(256), 0, s, x, Xt, Xq, y, Yt, Yq, o, obase, ostride, combos,
                          ncombo);
     else
-      hipLaunchKernelGGL(k_prod16<false>, grid, dim3(256), 0, s, sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos,
+      hipLaunchKernelGGL((k_prod16<T, false>), grid, dim3(256), 0, s, x, Xt, Xq, y, Yt, Yq, o, obase, ostride, combos,
                          ncombo);
   } else {
-    hipLaunchKernelGGL(k_prod, dim3(ncombo, ntime), dim3(64), 0, s, sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride,
-                       combos, D, transposed);
+    hipLaunchKernelGGL(k_prod<T>, dim3(ncombo, ntime), dim3(64), 0, s, x, Xt, Xq, y, Yt, Yq, o, obase, ostride, combos,
+                       D, transposed);
   }
+}
+
+template <typename T>
+void launch_trace_lstat_t(const void *sX, const long long *Xt, const void *sY, const long long *Yt,
+                          const LsBlock *blocks, int nblock, const LsChunk *chunks, const LsTask *tasks,
+                          const long long *slot_dst, double2 *out, long long obase, long long ostride, int ntime, int D,
+                          int rt, hipStream_t s) {
+  const T *x = static_cast<const T *>(sX), *y = static_cast<const T *>(sY);
+  const dim3 grid(unsigned(nblock * ntime)), block(64 * kLsWaves);
+  switch (D) {
+    case 8:
+      hipLaunchKernelGGL((k_trace_lstat<8, T>), grid, block, 0, s, x, Xt, y, Yt, blocks, nblock, chunks, tasks,
+                         slot_dst, out, obase, ostride, rt);
+      break;
+    case 16:
+      hipLaunchKernelGGL((k_trace_lstat<16, T>), grid, block, 0, s, x, Xt, y, Yt, blocks, nblock, chunks, tasks,
+                         slot_dst, out, obase, ostride, rt);
+      break;
+    case 24:
+      hipLaunchKernelGGL((k_trace_lstat<24, T>), grid, block, 0, s, x, Xt, y, Yt, blocks, nblock, chunks, tasks,
+                         slot_dst, out, obase, ostride, rt);
+      break;
+    case 32:
+      hipLaunchKernelGGL((k_trace_lstat<32, T>), grid, block, 0, s, x, Xt, y, Yt, blocks, nblock, chunks, tasks,
+                         slot_dst, out, obase, ostride, rt);
+      break;
+    default:
+      throw std::runtime_error("launch_trace_lstat: unsupported factor dimension");
+  }
+}
+}  // namespace
+
+void launch_prod(const void *sX, const long long *Xt, const long long *Xq, const void *sY, const long long *Yt,
+                 const long long *Yq, void *out, long long obase, long long ostride, const ComboProd *combos,
+                 int ncombo, int ntime, int D, int transposed, int f32, hipStream_t s) {
+  if (ncombo <= 0 || ntime <= 0) return;
+  if (D > kMaxD) throw std::runtime_error("factor dimension above 32 is not supported");
+  if (ntime > 65535) throw std::runtime_error("launch_prod: too many time keys");
+  if (f32)
+    launch_prod_t<float2>(sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos, ncombo, ntime, D, transposed, s);
+  else
+    launch_prod_t<double2>(sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos, ncombo, ntime, D, transposed, s);
   SLAPH_HIP(hipGetLastError());
 }
 
@@ -916,55 +1014,54 @@ int trace_lstat_chunk(int D) {
   return kLsStageBytes / (D * D * int(sizeof(double2)));
 }
 
-void launch_trace_lstat(const double2 *sX, const long long *Xt, const double2 *sY, const long long *Yt,
+void launch_trace_lstat(const void *sX, const long long *Xt, const void *sY, const long long *Yt,
                         const LsBlock *blocks, int nblock, const LsChunk *chunks, const LsTask *tasks,
                         const long long *slot_dst, double2 *out, long long obase, long long ostride, int ntime, int D,
-                        int rt, hipStream_t s) {
+                        int rt, int f32, hipStream_t s) {
   if (nblock <= 0 || ntime <= 0) return;
   if ((long long)nblock * ntime > 0x7fffffffLL) throw std::runtime_error("launch_trace_lstat: grid too large");
-  const dim3 grid(unsigned(nblock * ntime)), block(64 * kLsWaves);
-  switch (D) {
-    case 8:
-      hipLaunchKernelGGL(k_trace_lstat<8>, grid, block, 0, s, sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst,
-                         out, obase, ostride, rt);
-      break;
-    case 16:
-      hipLaunchKernelGGL(k_trace_lstat<16>, grid, block, 0, s, sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst,
-                         out, obase, ostride, rt);
-      break;
-    case 24:
-      hipLaunchKernelGGL(k_trace_lstat<24>, grid, block, 0, s, sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst,
-                         out, obase, ostride, rt);
-      break;
-    case 32:
-      hipLaunchKernelGGL(k_trace_lstat<32>, grid, block, 0, s, sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst,
-                         out, obase, ostride, rt);
-      break;
-    default:
-      throw std::runtime_error("launch_trace_lstat: unsupported factor dimension");
-  }
+  if (f32)
+    launch_trace_lstat_t<float2>(sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst, out, obase, ostride, ntime,
+                                 D, rt, s);
+  else
+    launch_trace_lstat_t<double2>(sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst, out, obase, ostride, ntime,
+                                  D, rt, s);
   SLAPH_HIP(hipGetLastError());
 }
 
-void launch_trace_groups(const double2 *sX, const long long *Xt, const long long *Xq, const double2 *sY,
+void launch_trace_groups(const void *sX, const long long *Xt, const long long *Xq, const void *sY,
                          const long long *Yt, const long long *Yq, const GroupItem *items, int nitems, const int2 *pairs,
-                         double2 *out, long long obase, long long ostride, int ntime, int D, int rt,
+                         double2 *out, long long obase, long long ostride, int ntime, int D, int rt, int f32,
                          hipStream_t s) {
   if (nitems <= 0 || ntime <= 0) return;
   if (ntime > 65535) throw std::runtime_error("launch_trace_groups: too many slice pairs");
-  hipLaunchKernelGGL(k_trace_groups, dim3(nitems, ntime), dim3(64), 0, s, sX, Xt, Xq, sY, Yt, Yq, items, pairs, out,
-                     obase, ostride, D, rt);
+  if (f32)
+    hipLaunchKernelGGL(k_trace_groups<float2>, dim3(nitems, ntime), dim3(64), 0, s, static_cast<const float2 *>(sX),
+                       Xt, Xq, static_cast<const float2 *>(sY), Yt, Yq, items, pairs, out, obase, ostride, D, rt);
+  else
+    hipLaunchKernelGGL(k_trace_groups<double2>, dim3(nitems, ntime), dim3(64), 0, s, static_cast<const double2 *>(sX),
+                       Xt, Xq, static_cast<const double2 *>(sY), Yt, Yq, items, pairs, out, obase, ostride, D, rt);
   SLAPH_HIP(hipGetLastError());
 }
 
-void launch_trace_grid(const double2 *sX, const long long *Xq, const double2 *sY, const long long *Yq,
+void launch_trace_grid(const void *sX, const long long *Xq, const void *sY, const long long *Yq,
                        const GroupItem *items, int nitems, const int2 *pairs, const GridTile *tiles, int ntiles,
-                       double2 *out, long long obase, long long ostride, int D, int rt, hipStream_t s) {
+                       double2 *out, long long obase, long long ostride, int D, int rt, int f32, hipStream_t s) {
   if (nitems <= 0 || ntiles <= 0) return;
   if (D % 4 != 0 || D > kMaxD) throw std::runtime_error("launch_trace_grid: unsupported factor dimension");
   if (ntiles > 65535) throw std::runtime_error("launch_trace_grid: too many tiles");
-  hipLaunchKernelGGL(k_trace_grid, dim3(nitems, ntiles), dim3(256), 0, s, sX, Xq, sY, Yq, items, pairs, tiles, out,
-                     obase, ostride, D, rt);
+  if (f32)
+    hipLaunchKernelGGL(k_trace_grid<float2>, dim3(nitems, ntiles), dim3(256), 0, s, static_cast<const float2 *>(sX),
+                       Xq, static_cast<const float2 *>(sY), Yq, items, pairs, tiles, out, obase, ostride, D, rt);
+  else
+    hipLaunchKernelGGL(k_trace_grid<double2>, dim3(nitems, ntiles), dim3(256), 0, s, static_cast<const double2 *>(sX),
+                       Xq, static_cast<const double2 *>(sY), Yq, items, pairs, tiles, out, obase, ostride, D, rt);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_to_f32(const double2 *in, void *out, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_to_f32, grid1d(n, 256), dim3(256), 0, s, in, static_cast<float2 *>(out), n);
   SLAPH_HIP(hipGetLastError());
 }
 

@@ -40,7 +40,9 @@ struct VdvJob {
   double2 *C[kVdvGroup];
   int np;
 };
-void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, hipStream_t s);
+// f32 != 0: single-pass FP32 MFMA (V and phases rounded to FP32, FP32
+// accumulation), results widened to FP64.
+void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, int f32, hipStream_t s);
 
 // Q0 of one (t, operator, random pair): D x D.
 struct Q0Job {
@@ -76,16 +78,20 @@ void launch_identity(double2 *out, int N, int count, hipStream_t s);
 
 // Table-driven batched factor product / trace kernels.
 // Operand matrix index = Xt[ti] + Xq[qn] + i; output index = obase + ti*ostride + o.
-// transposed != 0 stores each product as (X Y)^T.
+// transposed != 0 stores each product as (X Y)^T.  The operand slabs and the
+// output are double2 arrays, or float2 arrays when f32 != 0 (FP32-storage
+// precision form; the arithmetic is FP64 in both cases).
 struct ComboProd {
   int qn, i, j;
   long long o;
 };
-void launch_prod(const double2 *slabX, const long long *Xt, const long long *Xq, const double2 *slabY,
-                 const long long *Yt, const long long *Yq, double2 *out, long long obase, long long ostride,
-                 const ComboProd *combos, int ncombo, int ntime, int D, int transposed, hipStream_t s);
+void launch_prod(const void *slabX, const long long *Xt, const long long *Xq, const void *slabY, const long long *Yt,
+                 const long long *Yq, void *out, long long obase, long long ostride, const ComboProd *combos,
+                 int ncombo, int ntime, int D, int transposed, int f32, hipStream_t s);
 // In the trace launchers below, rt != 0 means the right matrices are stored
-// transposed (Y^T), so that tr(X Y) = sum_e X[e] Y^T[e].
+// transposed (Y^T), so that tr(X Y) = sum_e X[e] Y^T[e]; the operand slabs
+// are double2 or (f32 != 0) float2 arrays, the group sums always double2.
+void launch_to_f32(const double2 *in, void *out, long long n, hipStream_t s);
 
 // Correlator resolution: mean / 2-way / 3-way masked inner products.
 // A trace vector seen through its used-index mask groups: the trace values of
@@ -125,9 +131,9 @@ struct GroupItem {
   int list_off, n;
   long long dst;
 };
-void launch_trace_groups(const double2 *slabX, const long long *Xt, const long long *Xq, const double2 *slabY,
+void launch_trace_groups(const void *slabX, const long long *Xt, const long long *Xq, const void *slabY,
                          const long long *Yt, const long long *Yq, const GroupItem *items, int nitems, const int2 *pairs,
-                         double2 *out, long long obase, long long ostride, int ntime, int D, int rt,
+                         double2 *out, long long obase, long long ostride, int ntime, int D, int rt, int f32,
                          hipStream_t s);
 // Time-separable traces: within one tile of slice pairs the left operand
 // depends on one time index a < nL and the right one on b < nR, and every
@@ -141,9 +147,9 @@ struct GridTile {
   int sidx[kGridT][kGridT];
   int nL, nR;
 };
-void launch_trace_grid(const double2 *slabX, const long long *Xq, const double2 *slabY, const long long *Yq,
+void launch_trace_grid(const void *slabX, const long long *Xq, const void *slabY, const long long *Yq,
                        const GroupItem *items, int nitems, const int2 *pairs, const GridTile *tiles, int ntiles,
-                       double2 *out, long long obase, long long ostride, int D, int rt, hipStream_t s);
+                       double2 *out, long long obase, long long ostride, int D, int rt, int f32, hipStream_t s);
 // Non-separable traces, left-stationary.  A block is a set of (trace id,
 // mask group) slots that share one left operator entry (left matrices
 // X[Xt[ti] + xq + i]); its chunks stage left combinations [i0, i0 + n) and
@@ -170,10 +176,10 @@ struct LsTask {
 // Largest number of left matrices one chunk stages for matrix size D.
 int trace_lstat_chunk(int D);
 bool trace_lstat_supported(int D);
-void launch_trace_lstat(const double2 *slabX, const long long *Xt, const double2 *slabY, const long long *Yt,
+void launch_trace_lstat(const void *slabX, const long long *Xt, const void *slabY, const long long *Yt,
                         const LsBlock *blocks, int nblock, const LsChunk *chunks, const LsTask *tasks,
                         const long long *slot_dst, double2 *out, long long obase, long long ostride, int ntime, int D,
-                        int rt, hipStream_t s);
+                        int rt, int f32, hipStream_t s);
 void launch_resolve(const double2 *gvals, const uint16_t *gmasks, const CorrJob *jobs, int ncorr, int nsp,
                     const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, hipStream_t s);
 void launch_accumulate(const double2 *cval, const int *tdelta, int nsp, int ncorr, double2 *sum, double2 *comp,

@@ -58,6 +58,7 @@ _SIGS = {
     "slc_fetch_correlators": (_c.c_int, [_P, _c.c_char_p, _D]),
     "slc_stage_time": (_c.c_int, [_P, _c.c_char_p, _D]),
     "slc_set_chunk_budget": (_c.c_int, [_P, _c.c_longlong]),
+    "slc_set_precision": (_c.c_int, [_P, _c.c_int, _c.c_int]),
     "slc_ahb_device": (_c.c_int, [_P, _P, _P, _P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_longlong,
                                   _c.c_longlong, _c.c_longlong, _c.c_int, _c.c_longlong, _c.c_longlong,
                                   _c.c_longlong]),
@@ -207,6 +208,15 @@ class Context:
 
     def set_chunk_budget(self, nbytes: int):
         self._check(_lib.slc_set_chunk_budget(self._h, int(nbytes)))
+
+    def set_precision(self, vdaggerv: str = "fp64", factors: str = "fp64"):
+        """Precision forms: vdaggerv "fp64" | "fp32" (single-pass FP32 MFMA);
+        factors "fp64" | "fp32" (factor slabs and product tables stored in
+        FP32, arithmetic and accumulation FP64)."""
+        modes = {"fp64": 0, "fp32": 1}
+        if vdaggerv not in modes or factors not in modes:
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self._check(_lib.slc_set_precision(self._h, modes[vdaggerv], modes[factors]))
 
     def ahb_host(self, A: np.ndarray, B: np.ndarray, phase=None) -> np.ndarray:
         """A^H diag(phase[k/3]) B on the device for host matrices A (K x M), B (K x N)."""

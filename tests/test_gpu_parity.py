@@ -203,6 +203,43 @@ def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypat
 _SMALL_WANT = {}  # oracle correlators per input file (same seeded inputs)
 
 
+def _rel_errors(got, want):
+    """Per-correlator norm-wise relative error max_t |C - C_ref| / max_t |C_ref|."""
+    out = []
+    for d, vals in want.items():
+        for name, v in vals.items():
+            out.append(np.max(np.abs(got[d][name] - v)) / max(np.max(np.abs(v)), 1e-300))
+    return np.array(out)
+
+
+def test_precision_forms_against_fp64(slaph):
+    """The precision forms run the same contraction with FP32 pieces: the
+    single-pass FP32-MFMA VdaggerV and FP32 storage of factors and products.
+    Every correlator stays within FP32 rounding of the FP64 path (the bound
+    here is 1e-4; the FP32 unit roundoff is 6e-8), and switching back to FP64
+    reproduces the FP64 result bit for bit."""
+    say("precision forms")
+    ctx = slaph.Context(0)
+    ctx.configure(WIDE)
+    ctx.synthetic_inputs(99, True)
+    ops = slaph.OperatorFactory(ctx)
+
+    def run(vdv, fac):
+        ctx.set_precision(vdv, fac)
+        ops.create_operators()
+        slaph.contract(ctx)
+        return {d: ctx.correlators(d) for d in ctx.diagrams()}
+
+    ref = run("fp64", "fp64")
+    for vdv, fac in (("fp32", "fp64"), ("fp64", "fp32"), ("fp32", "fp32")):
+        err = _rel_errors(run(vdv, fac), ref)
+        say("  vdaggerv %s factors %s: max %.2e median %.2e" % (vdv, fac, err.max(), np.median(err)))
+        assert err.max() < 1e-4, (vdv, fac, err.max())
+        assert err.max() > 0.0, (vdv, fac)  # the FP32 pieces did run
+    again = run("fp64", "fp64")
+    assert np.all(_rel_errors(again, ref) == 0.0)
+
+
 def test_plan_reuse_is_deterministic(slaph):
     """The contraction plan is built on the first call and reused: a second
     call on new operators gives bit-identical correlators, a budget change
