@@ -476,9 +476,9 @@ __global__ __launch_bounds__(256) void k_prod16(const T *__restrict__ sX, const 
 // block's left matrices are staged chunk by chunk in LDS, transposed, so that
 // element e of a right matrix meets its partner at the same index.  Each
 // wavefront streams the right matrices of its tasks into registers once
-// (EPL = D*D/64 elements per lane; the next task's matrix is loaded while the
-// current one is used) and multiplies them with one or two staged left
-// matrices.  At the end of a run (or of the wave's task range) the wave sum
+// (EPL = D*D/64 elements per lane; up to PF tasks ahead are in flight, since
+// the kernel is bound by the latency of these loads) and multiplies them with
+// one or two staged left matrices.  At the end of a run (or of the wave's task range) the wave sum
 // is added to the wave's own copy of the run's slot; the copies are summed in
 // wave order at the end, so the summation order is fixed by the plan.
 // ---------------------------------------------------------------------------
@@ -491,6 +491,7 @@ __global__ __launch_bounds__(64 * kLsWaves) void k_trace_lstat(
     const LsTask *__restrict__ tasks, const long long *__restrict__ slot_dst, double2 *__restrict__ out,
     long long obase, long long ostride, int rt) {
   constexpr int DD = D * D, EPL = DD / 64, CL = kLsStageBytes / (DD * 16);
+  constexpr int PF = EPL <= 4 ? 4 : (EPL <= 8 ? 2 : 1);  // right matrices in flight per wave
   __shared__ double2 Ls[CL * DD];
   __shared__ double2 acc[kLsWaves][kLsSlots];
   // Workgroups are dealt round-robin to the 8 XCDs, each with its own L2:
@@ -515,47 +516,57 @@ __global__ __launch_bounds__(64 * kLsWaves) void k_trace_lstat(
     const int t0 = C.wtask[wave], t1 = C.wtask[wave + 1];
     if (t0 >= t1) continue;
     double re = 0.0, im = 0.0;
-    double2 r[EPL], rn[EPL];
-    LsTask T = tasks[t0];
+    // ring of PF tasks in flight (slot u always holds task t + u of the
+    // current round; the loop over u is unrolled, so the ring indexes are
+    // compile-time constants)
+    double2 r[PF][EPL];
+    LsTask tk[PF];
 #pragma unroll
-    for (int k = 0; k < EPL; ++k) r[k] = ld2(yb + (long long)T.yrel * DD + lane + 64 * k);
-    for (int t = t0; t < t1; ++t) {
-      LsTask Tn = T;
-      if (t + 1 < t1) {
-        Tn = tasks[t + 1];
+    for (int u = 0; u < PF; ++u)
+      if (t0 + u < t1) {
+        tk[u] = tasks[t0 + u];
 #pragma unroll
-        for (int k = 0; k < EPL; ++k) rn[k] = ld2(yb + (long long)Tn.yrel * DD + lane + 64 * k);
+        for (int k = 0; k < EPL; ++k) r[u][k] = ld2(yb + (long long)tk[u].yrel * DD + lane + 64 * k);
       }
-      const double2 *la = Ls + T.ia * DD + lane;
+    for (int t = t0; t < t1; t += PF) {
 #pragma unroll
-      for (int k = 0; k < EPL; ++k) {
-        const double2 u = la[64 * k], v = r[k];
-        re = fma(u.x, v.x, re);
-        re = fma(-u.y, v.y, re);
-        im = fma(u.x, v.y, im);
-        im = fma(u.y, v.x, im);
-      }
-      if (T.ib >= 0) {
-        const double2 *lb = Ls + T.ib * DD + lane;
+      for (int u = 0; u < PF; ++u) {
+        if (t + u >= t1) break;
+        const LsTask T = tk[u];
+        const double2 *la = Ls + T.ia * DD + lane;
 #pragma unroll
         for (int k = 0; k < EPL; ++k) {
-          const double2 u = lb[64 * k], v = r[k];
-          re = fma(u.x, v.x, re);
-          re = fma(-u.y, v.y, re);
-          im = fma(u.x, v.y, im);
-          im = fma(u.y, v.x, im);
+          const double2 a = la[64 * k], v = r[u][k];
+          re = fma(a.x, v.x, re);
+          re = fma(-a.y, v.y, re);
+          im = fma(a.x, v.y, im);
+          im = fma(a.y, v.x, im);
+        }
+        if (T.ib >= 0) {
+          const double2 *lb = Ls + T.ib * DD + lane;
+#pragma unroll
+          for (int k = 0; k < EPL; ++k) {
+            const double2 a = lb[64 * k], v = r[u][k];
+            re = fma(a.x, v.x, re);
+            re = fma(-a.y, v.y, re);
+            im = fma(a.x, v.y, im);
+            im = fma(a.y, v.x, im);
+          }
+        }
+        if (T.slot >= 0) {
+          re = wave_sum(re);
+          im = wave_sum(im);
+          if (lane == 0) acc[wave][T.slot] = cadd(acc[wave][T.slot], make_double2(re, im));
+          re = 0.0;
+          im = 0.0;
+        }
+        // refill this ring slot with the task PF positions ahead
+        if (t + u + PF < t1) {
+          tk[u] = tasks[t + u + PF];
+#pragma unroll
+          for (int k = 0; k < EPL; ++k) r[u][k] = ld2(yb + (long long)tk[u].yrel * DD + lane + 64 * k);
         }
       }
-      if (T.slot >= 0) {
-        re = wave_sum(re);
-        im = wave_sum(im);
-        if (lane == 0) acc[wave][T.slot] = cadd(acc[wave][T.slot], make_double2(re, im));
-        re = 0.0;
-        im = 0.0;
-      }
-      T = Tn;
-#pragma unroll
-      for (int k = 0; k < EPL; ++k) r[k] = rn[k];
     }
   }
   __syncthreads();
