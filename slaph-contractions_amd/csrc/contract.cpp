@@ -300,6 +300,36 @@ bool grid_tile(std::vector<long long> const &Xt, std::vector<long long> const &Y
   return true;
 }
 
+// Orders a family's group-sum items by connected component of the (left
+// entry, right entry) graph of its traces, then by left entry, so that items
+// launched next to each other (and, through the XCD-aware block mapping of the
+// trace kernels, on the same L2) read the same operand matrices.  Item
+// destinations are stored in the items, so the order is free.
+void order_items_by_component(Family &f, NodePlan const &root) {
+  if (f.shape.nodes.back().kind != kTrace2 || f.gitems.empty()) return;
+  std::map<std::pair<int, long long>, int> node_id;
+  std::vector<int> parent;
+  auto node = [&](int side, long long off) {
+    auto it = node_id.emplace(std::make_pair(side, off), int(parent.size())).first;
+    if (it->second == int(parent.size())) parent.push_back(it->second);
+    return it->second;
+  };
+  std::function<int(int)> find = [&](int x) { return parent[x] == x ? x : parent[x] = find(parent[x]); };
+  for (auto const &it : f.gitems) {
+    int a = find(node(0, root.h_Xq.at(it.qn))), b = find(node(f.coincide ? 0 : 1, root.h_Yq.at(it.qn)));
+    if (a != b) parent[std::max(a, b)] = std::min(a, b);
+  }
+  std::vector<std::pair<std::pair<int, long long>, int>> key(f.gitems.size());
+  for (size_t k = 0; k < f.gitems.size(); ++k) {
+    long long xq = root.h_Xq.at(f.gitems[k].qn);
+    key[k] = {{find(node(0, xq)), xq}, int(k)};
+  }
+  std::stable_sort(key.begin(), key.end(), [](auto const &x, auto const &y) { return x.first < y.first; });
+  std::vector<GroupItem> sorted(f.gitems.size());
+  for (size_t k = 0; k < key.size(); ++k) sorted[k] = f.gitems[key[k].second];
+  f.gitems = std::move(sorted);
+}
+
 // Left-stationary plan of a family's two-operand trace (time independent):
 // group sums (items) are blocked by their left operator entry, at most
 // kLsSlots per block; a block's left combinations are staged in chunks of
@@ -848,6 +878,7 @@ void Engine::contract(int single_block_pair, int kahan) {
         off += (long long)gm.size();
       }
       f.gstride = off;
+      order_items_by_component(f, root);
       upload(f.d_gitems, f.gitems);
       upload(f.d_gpairs, f.gpairs);
       plan_lstat(f, root, D_);
