@@ -490,7 +490,8 @@ struct ContractCache {
   std::vector<uint16_t> zmask;  // subset list of the Moebius form (empty: not used)
   DevBuf d_zmask;
   std::vector<ChunkPlan> chunks;
-  DevBuf d_prod, d_gv, d_cval, d_zv;
+  DevBuf d_prod, d_gv, d_cval, d_zv, d_res;
+  std::vector<DevBuf> d_sum, d_comp;  // per diagram: Lt x n_corr accumulators
 };
 
 void Engine::contract(int single_block_pair, int kahan) {
@@ -1194,8 +1195,10 @@ void Engine::contract(int single_block_pair, int kahan) {
     }
   }
 
-  // ---- accumulators and chunk buffers ----
-  std::vector<DevBuf> d_sum(corrs.size()), d_comp(corrs.size());
+  // ---- accumulators and chunk buffers (kept with the plan) ----
+  auto &d_sum = C.d_sum, &d_comp = C.d_comp;
+  d_sum.resize(corrs.size());
+  d_comp.resize(corrs.size());
   for (size_t c = 0; c < corrs.size(); ++c) {
     size_t n = (size_t)Lt_ * corrs[c].reqs.size();
     d_sum[c].alloc(sizeof(double2) * n);
@@ -1287,24 +1290,26 @@ void Engine::contract(int single_block_pair, int kahan) {
                         d_comp[c].as<double2>(), kahan, stream_);
     }
   }
+  // ---- results: correlator_[t][i].value() / Lt (Diagram.cpp:132-147),
+  // formed on the device as [correlator][t] and copied once ----
+  results_.resize(corrs.size());
+  size_t res_total = 0;
+  for (auto const &cp : corrs) res_total += (size_t)Lt_ * cp.reqs.size();
+  C.d_res.ensure(sizeof(double2) * std::max<size_t>(1, res_total));
+  for (size_t c = 0, off = 0; c < corrs.size(); off += (size_t)Lt_ * corrs[c].reqs.size(), ++c)
+    launch_finalize(d_sum[c].as<double2>(), d_comp[c].as<double2>(), int(corrs[c].reqs.size()), Lt_,
+                    C.d_res.as<double2>() + off, stream_);
   double kms = ctimer.stop();
   (void)kms;
-
-  // ---- results: correlator_[t][i].value() / Lt (Diagram.cpp:132-147) ----
-  results_.clear();
-  for (size_t c = 0; c < corrs.size(); ++c) {
-    size_t n = (size_t)Lt_ * corrs[c].reqs.size();
-    std::vector<cplx> sum(n), comp(n);
-    SLAPH_HIP(hipMemcpy(sum.data(), d_sum[c].as<void>(), sizeof(cplx) * n, hipMemcpyDeviceToHost));
-    SLAPH_HIP(hipMemcpy(comp.data(), d_comp[c].as<void>(), sizeof(cplx) * n, hipMemcpyDeviceToHost));
-    DiagramResult r;
-    r.name = corrs[c].name;
-    r.names = corrs[c].datasets;
-    size_t nc = corrs[c].reqs.size();
-    r.values.resize(n);
-    for (size_t i = 0; i < nc; ++i)
-      for (int t = 0; t < Lt_; ++t) r.values[i * Lt_ + t] = (sum[t * nc + i] + comp[t * nc + i]) / double(Lt_);
-    results_.push_back(std::move(r));
+  for (size_t c = 0, off = 0; c < corrs.size(); off += (size_t)Lt_ * corrs[c].reqs.size(), ++c) {
+    DiagramResult &r = results_[c];
+    if (r.name != corrs[c].name || r.names.size() != corrs[c].datasets.size()) {
+      r.name = corrs[c].name;
+      r.names = corrs[c].datasets;
+    }
+    r.values.resize((size_t)Lt_ * corrs[c].reqs.size());
+    SLAPH_HIP(hipMemcpy(r.values.data(), C.d_res.as<double2>() + off, sizeof(cplx) * r.values.size(),
+                        hipMemcpyDeviceToHost));
   }
   times_.contract_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   guard.done = true;

@@ -22,6 +22,7 @@ void DevBuf::reset() {
 }
 
 void DevBuf::alloc(size_t bytes) {
+  if (bytes == bytes_ && p_) return;  // same size: keep the allocation (and its address)
   reset();
   if (bytes == 0) return;
   SLAPH_HIP(hipMalloc(&p_, bytes));
@@ -89,6 +90,8 @@ void Engine::configure(std::string const &text) {
   have_ev_ = have_peram_ = have_rnd_ = have_ops_ = false;
   results_.clear();
   plan_.reset();
+  fjobs_.reset();
+  phases_ready_ = false;
   SLAPH_HIP(hipSetDevice(device_));
   alloc_inputs();
 }
@@ -296,20 +299,24 @@ void Engine::create_operators() {
   int nvdv = int(gd_.vdaggerv_lookup.size());
   int Vs = gd_.Lx * gd_.Ly * gd_.Lz;
   vdv_.alloc(sizeof(double2) * std::max(1, nvdv) * Lt_ * N_ * N_);
-  phase_.alloc(sizeof(double2) * std::max(1, nvdv) * Vs);
-  // Momentum phases exp(-i 2 pi p.x / L), site = (x Ly + y) Lz + z
-  // (create_momenta, OperatorFactory.cpp:64-95).
-  std::vector<cplx> ph((size_t)std::max(1, nvdv) * Vs);
-  for (auto const &op : gd_.vdaggerv_lookup) {
-    double ipx = op.mom[0] * 2.0 * M_PI / gd_.Lx, ipy = op.mom[1] * 2.0 * M_PI / gd_.Ly,
-           ipz = op.mom[2] * 2.0 * M_PI / gd_.Lz;
-    for (int x = 0; x < gd_.Lx; ++x)
-      for (int y = 0; y < gd_.Ly; ++y)
-        for (int z = 0; z < gd_.Lz; ++z)
-          ph[(size_t)op.id * Vs + (x * gd_.Ly + y) * gd_.Lz + z] =
-              std::exp(cplx(0, -1) * (ipx * x + ipy * y + ipz * z));
+  if (!phases_ready_) {
+    // Momentum phases exp(-i 2 pi p.x / L), site = (x Ly + y) Lz + z
+    // (create_momenta, OperatorFactory.cpp:64-95); they depend on the input
+    // file only and are uploaded once per configure.
+    phase_.alloc(sizeof(double2) * std::max(1, nvdv) * Vs);
+    std::vector<cplx> ph((size_t)std::max(1, nvdv) * Vs);
+    for (auto const &op : gd_.vdaggerv_lookup) {
+      double ipx = op.mom[0] * 2.0 * M_PI / gd_.Lx, ipy = op.mom[1] * 2.0 * M_PI / gd_.Ly,
+             ipz = op.mom[2] * 2.0 * M_PI / gd_.Lz;
+      for (int x = 0; x < gd_.Lx; ++x)
+        for (int y = 0; y < gd_.Ly; ++y)
+          for (int z = 0; z < gd_.Lz; ++z)
+            ph[(size_t)op.id * Vs + (x * gd_.Ly + y) * gd_.Lz + z] =
+                std::exp(cplx(0, -1) * (ipx * x + ipy * y + ipz * z));
+    }
+    SLAPH_HIP(hipMemcpy(phase_.as<void>(), ph.data(), sizeof(cplx) * ph.size(), hipMemcpyHostToDevice));
+    phases_ready_ = true;
   }
-  SLAPH_HIP(hipMemcpyAsync(phase_.as<void>(), ph.data(), sizeof(cplx) * ph.size(), hipMemcpyHostToDevice, stream_));
   // Non-trivial momenta in balanced groups of at most kVdvGroup: each
   // workgroup of the kernel serves a whole group from one V(t) panel load.
   std::vector<int> ids;
@@ -386,6 +393,22 @@ void Engine::compute_q_offsets() {
   }
 }
 
+// The job tables of build_factors (device pointers into the slabs, the
+// operators, the perambulators and the random vectors).  They depend on the
+// factor keys and on those buffers' addresses only, so a repeated call with
+// the same keys (every gauge configuration of a run) launches them as they are.
+struct Engine::FactorJobs {
+  std::vector<std::vector<int>> q1keys, q2keys;
+  std::vector<const void *> sig;
+  DevBuf q0j, q1j, drs;
+  int nq0 = 0, nq1 = 0;
+  struct PerT {
+    DevBuf aj, bj;
+    int na = 0, nb = 0;
+  };
+  std::vector<PerT> q2t;
+};
+
 void Engine::build_factors(std::vector<std::vector<int>> const &q1keys, std::vector<std::vector<int>> const &q2keys) {
   compute_q_offsets();
   const long long DD = (long long)D_ * D_;
@@ -393,61 +416,67 @@ void Engine::build_factors(std::vector<std::vector<int>> const &q1keys, std::vec
   const size_t rsz = (size_t)Lt_ * N_ * 4, psz = gd_.peram_rows() * gd_.peram_cols();
   const double2 *rho = rho_.as<double2>();
   const double2 *peram = peram_.as<double2>();
-  DevBuf dj;
-  // Q0 for every timeslice.
-  {
-    auto const &lst = gd_.quarkline_lookuptable.at("Q0");
-    q0_.alloc(sizeof(double2) * std::max<long long>(1, Lt_ * q_total_[0] * DD));
-    std::vector<Q0Job> jobs;
-    for (int t = 0; t < Lt_; ++t)
-      for (size_t e = 0; e < lst.size(); ++e)
-        for (size_t j = 0; j < lst[e].rnd.size(); ++j) {
-          auto const &p = lst[e].rnd[j];
-          jobs.push_back({vdv_ptr(lst[e].id_vdaggerv, lst[e].dagger, t), rho + p.first * rsz + (size_t)t * N_ * 4,
-                          rho + p.second * rsz + (size_t)t * N_ * 4,
-                          q0_.as<double2>() + ((long long)t * q_total_[0] + q_off_[0][e] + (long long)j) * DD,
-                          lst[e].gamma.at(0)});
-        }
-    upload(dj, jobs);
-    launch_q0(dj.as<Q0Job>(), int(jobs.size()), N_, gd_.dilE(), stream_);
-    SLAPH_HIP(hipStreamSynchronize(stream_));
+  auto const &lst0 = gd_.quarkline_lookuptable.at("Q0");
+  auto const &lst1 = gd_.quarkline_lookuptable.at("Q1");
+  auto const &lst2 = gd_.quarkline_lookuptable.at("Q2");
+  // slabs: Q0 for every timeslice, Q1 for the needed (t, b), Q2 for the
+  // needed (b1, t, b2) (an unchanged size keeps the allocation)
+  q0_.alloc(sizeof(double2) * std::max<long long>(1, Lt_ * q_total_[0] * DD));
+  q1need_.assign((size_t)Lt_ * nb_, 0);
+  for (auto const &k : q1keys) q1need_[k[0] * nb_ + k[1]] = 1;
+  q1_.alloc(sizeof(double2) * std::max<long long>(1, (long long)Lt_ * nb_ * q_total_[1] * DD));
+  q2slot_.assign((size_t)nb_ * Lt_ * nb_, -1);
+  int nslot = 0;
+  for (auto const &k : q2keys) {
+    int &s = q2slot_[((size_t)k[0] * Lt_ + k[1]) * nb_ + k[2]];
+    if (s < 0) s = nslot++;
   }
-  // Q1 for the needed (t, b).
-  {
-    auto const &lst = gd_.quarkline_lookuptable.at("Q1");
-    q1need_.assign((size_t)Lt_ * nb_, 0);
-    for (auto const &k : q1keys) q1need_[k[0] * nb_ + k[1]] = 1;
-    q1_.alloc(sizeof(double2) * std::max<long long>(1, (long long)Lt_ * nb_ * q_total_[1] * DD));
-    std::vector<Q1Job> jobs;
-    for (int t = 0; t < Lt_; ++t)
-      for (int b = 0; b < nb_; ++b) {
-        if (!q1need_[t * nb_ + b]) continue;
-        for (size_t e = 0; e < lst.size(); ++e)
-          for (size_t j = 0; j < lst[e].rnd.size(); ++j) {
-            auto const &p = lst[e].rnd[j];
-            jobs.push_back({vdv_ptr(lst[e].id_vdaggerv, lst[e].dagger, t), rho + p.first * rsz + (size_t)t * N_ * 4,
-                            peram + p.second * psz,
-                            q1_.as<double2>() + (((long long)t * nb_ + b) * q_total_[1] + q_off_[1][e] + (long long)j) * DD,
-                            t, b, lst[e].gamma.at(0)});
+  q2_.alloc(sizeof(double2) * std::max<long long>(1, nslot * q_total_[2] * DD));
+  const long long msz = (long long)D_ * 4 * N_;
+  auto signature = [&] {
+    return std::vector<const void *>{vdv_.as<void>(), vdvdag_.as<void>(), rho_.as<void>(), peram_.as<void>(),
+                                     q0_.as<void>(),  q1_.as<void>(),     q2_.as<void>(),  mbuf_.as<void>()};
+  };
+  if (!fjobs_ || fjobs_->q1keys != q1keys || fjobs_->q2keys != q2keys || fjobs_->sig != signature()) {
+    auto fj = std::make_shared<FactorJobs>();
+    fj->q1keys = q1keys;
+    fj->q2keys = q2keys;
+    {
+      std::vector<Q0Job> jobs;
+      for (int t = 0; t < Lt_; ++t)
+        for (size_t e = 0; e < lst0.size(); ++e)
+          for (size_t j = 0; j < lst0[e].rnd.size(); ++j) {
+            auto const &p = lst0[e].rnd[j];
+            jobs.push_back({vdv_ptr(lst0[e].id_vdaggerv, lst0[e].dagger, t), rho + p.first * rsz + (size_t)t * N_ * 4,
+                            rho + p.second * rsz + (size_t)t * N_ * 4,
+                            q0_.as<double2>() + ((long long)t * q_total_[0] + q_off_[0][e] + (long long)j) * DD,
+                            lst0[e].gamma.at(0)});
           }
-      }
-    upload(dj, jobs);
-    launch_q1(dj.as<Q1Job>(), int(jobs.size()), N_, gd_.dilE(), prow, stream_);
-    SLAPH_HIP(hipStreamSynchronize(stream_));
-  }
-  // Q2 for the needed (b1, t, b2): M' = gval(bd) s(bd,c) P1_bd^H VdV per
-  // (b1, t, operator, r1), then Q2 = M' * stacked P2 blocks.
-  {
-    auto const &lst = gd_.quarkline_lookuptable.at("Q2");
-    q2slot_.assign((size_t)nb_ * Lt_ * nb_, -1);
-    int nslot = 0;
-    for (auto const &k : q2keys) {
-      int &s = q2slot_[((size_t)k[0] * Lt_ + k[1]) * nb_ + k[2]];
-      if (s < 0) s = nslot++;
+      upload(fj->q0j, jobs);
+      fj->nq0 = int(jobs.size());
     }
-    q2_.alloc(sizeof(double2) * std::max<long long>(1, nslot * q_total_[2] * DD));
-    // rowscale[g][bd][row] = gval_g(bd) * s(bd, row / dilE); s = -1 iff
-    // exactly one of bd, c lies in {2,3} (DilutedFactorFactory.cpp:215-216).
+    {
+      std::vector<Q1Job> jobs;
+      for (int t = 0; t < Lt_; ++t)
+        for (int b = 0; b < nb_; ++b) {
+          if (!q1need_[t * nb_ + b]) continue;
+          for (size_t e = 0; e < lst1.size(); ++e)
+            for (size_t j = 0; j < lst1[e].rnd.size(); ++j) {
+              auto const &p = lst1[e].rnd[j];
+              jobs.push_back(
+                  {vdv_ptr(lst1[e].id_vdaggerv, lst1[e].dagger, t), rho + p.first * rsz + (size_t)t * N_ * 4,
+                   peram + p.second * psz,
+                   q1_.as<double2>() + (((long long)t * nb_ + b) * q_total_[1] + q_off_[1][e] + (long long)j) * DD, t,
+                   b, lst1[e].gamma.at(0)});
+            }
+        }
+      upload(fj->q1j, jobs);
+      fj->nq1 = int(jobs.size());
+    }
+    // Q2: M' = gval(bd) s(bd,c) P1_bd^H VdV per (b1, t, operator, r1), then
+    // Q2 = M' * stacked P2 blocks.  rowscale[g][bd][row] = gval_g(bd) *
+    // s(bd, row / dilE); s = -1 iff exactly one of bd, c lies in {2,3}
+    // (DilutedFactorFactory.cpp:215-216).
     auto const &gt = gamma_table();
     std::vector<cplx> rs(16 * 4 * D_);
     for (int g = 0; g < 16; ++g)
@@ -457,57 +486,71 @@ void Engine::build_factors(std::vector<std::vector<int>> const &q1keys, std::vec
           double sgn = ((bd + c) == 3 || std::abs(bd - c) > 1) ? -1.0 : 1.0;
           rs[(g * 4 + bd) * D_ + row] = gt[g].value[bd] * sgn;
         }
-    DevBuf drs, Mbuf, djb;
-    upload(drs, rs);
+    upload(fj->drs, rs);
+    // M' jobs per timeslice keyed by (b1, entry, r1); the M' buffer is shared
+    // by the timeslices (they run one after another on the stream)
+    std::vector<std::map<std::array<int, 3>, int>> mjob(Lt_);
+    std::vector<std::vector<Q2BJob>> bjobs(Lt_);
+    std::vector<std::vector<int>> bjob_m(Lt_);
+    size_t mmax = 1;
     for (int t = 0; t < Lt_; ++t) {
-      // M' jobs for this timeslice, keyed by (b1, entry, r1).
-      std::map<std::array<int, 3>, int> mjob;
-      std::vector<AhBJob> ajobs;
-      std::vector<Q2BJob> bjobs;
-      std::vector<int> bjob_m;
       for (int b1 = 0; b1 < nb_; ++b1)
         for (int b2 = 0; b2 < nb_; ++b2) {
           int slot = q2slot_[((size_t)b1 * Lt_ + t) * nb_ + b2];
           if (slot < 0) continue;
-          for (size_t e = 0; e < lst.size(); ++e)
-            for (size_t j = 0; j < lst[e].rnd.size(); ++j) {
-              auto const &p = lst[e].rnd[j];
+          for (size_t e = 0; e < lst2.size(); ++e)
+            for (size_t j = 0; j < lst2[e].rnd.size(); ++j) {
+              auto const &p = lst2[e].rnd[j];
               std::array<int, 3> key{b1, int(e), p.first};
-              auto it = mjob.find(key);
-              if (it == mjob.end()) it = mjob.emplace(key, int(mjob.size())).first;
-              bjobs.push_back({nullptr, peram + p.second * psz,
-                               q2_.as<double2>() + ((long long)slot * q_total_[2] + q_off_[2][e] + (long long)j) * DD, t,
-                               b2, lst[e].gamma.at(0)});
-              bjob_m.push_back(it->second);
+              auto it = mjob[t].find(key);
+              if (it == mjob[t].end()) it = mjob[t].emplace(key, int(mjob[t].size())).first;
+              bjobs[t].push_back({nullptr, peram + p.second * psz,
+                                  q2_.as<double2>() + ((long long)slot * q_total_[2] + q_off_[2][e] + (long long)j) * DD,
+                                  t, b2, lst2[e].gamma.at(0)});
+              bjob_m[t].push_back(it->second);
             }
         }
-      if (mjob.empty()) continue;
-      const long long msz = (long long)D_ * 4 * N_;
-      Mbuf.ensure(sizeof(double2) * msz * mjob.size());
-      for (auto const &kv : mjob) {
+      mmax = std::max(mmax, mjob[t].size());
+    }
+    mbuf_.ensure(sizeof(double2) * msz * mmax);
+    fj->q2t.resize(Lt_);
+    for (int t = 0; t < Lt_; ++t) {
+      if (mjob[t].empty()) continue;
+      std::vector<AhBJob> ajobs;
+      for (auto const &kv : mjob[t]) {
         int b1 = kv.first[0], e = kv.first[1], r1 = kv.first[2];
         for (int bd = 0; bd < 4; ++bd) {
           AhBJob a;
           a.A = peram + r1 * psz + (long long)(b1 * D_) * prow + (long long)(t * 4 + bd) * N_;
           a.lda = prow;
-          a.B = vdv_ptr(lst[e].id_vdaggerv, lst[e].dagger, t);
+          a.B = vdv_ptr(lst2[e].id_vdaggerv, lst2[e].dagger, t);
           a.ldb = N_;
-          a.C = Mbuf.as<double2>() + kv.second * msz + (long long)bd * N_ * D_;
+          a.C = mbuf_.as<double2>() + kv.second * msz + (long long)bd * N_ * D_;
           a.ldc = D_;
           a.phase = nullptr;
-          a.rowscale = drs.as<double2>() + (lst[e].gamma.at(0) * 4 + bd) * D_;
+          a.rowscale = fj->drs.as<double2>() + (lst2[e].gamma.at(0) * 4 + bd) * D_;
           ajobs.push_back(a);
         }
       }
-      for (size_t i = 0; i < bjobs.size(); ++i) bjobs[i].Mp = Mbuf.as<double2>() + (long long)bjob_m[i] * msz;
-      upload(dj, ajobs);
-      for (size_t i = 0; i < ajobs.size(); i += 65535)
-        launch_ahb(dj.as<AhBJob>() + i, int(std::min<size_t>(65535, ajobs.size() - i)), D_, N_, N_, stream_);
-      upload(djb, bjobs);
-      launch_q2b(djb.as<Q2BJob>(), int(bjobs.size()), N_, gd_.dilE(), prow, stream_);
-      SLAPH_HIP(hipStreamSynchronize(stream_));
+      for (size_t i = 0; i < bjobs[t].size(); ++i) bjobs[t][i].Mp = mbuf_.as<double2>() + (long long)bjob_m[t][i] * msz;
+      upload(fj->q2t[t].aj, ajobs);
+      upload(fj->q2t[t].bj, bjobs[t]);
+      fj->q2t[t].na = int(ajobs.size());
+      fj->q2t[t].nb = int(bjobs[t].size());
     }
+    fj->sig = signature();
+    fjobs_ = fj;
   }
+  FactorJobs const &fj = *fjobs_;
+  launch_q0(fj.q0j.as<Q0Job>(), fj.nq0, N_, gd_.dilE(), stream_);
+  launch_q1(fj.q1j.as<Q1Job>(), fj.nq1, N_, gd_.dilE(), prow, stream_);
+  for (auto const &pt : fj.q2t) {
+    if (pt.na == 0) continue;
+    for (int i = 0; i < pt.na; i += 65535)
+      launch_ahb(pt.aj.as<AhBJob>() + i, std::min(65535, pt.na - i), D_, N_, N_, stream_);
+    launch_q2b(pt.bj.as<Q2BJob>(), pt.nb, N_, gd_.dilE(), prow, stream_);
+  }
+  SLAPH_HIP(hipStreamSynchronize(stream_));
 }
 
 int Engine::diluted_factor(int qtype, const int *key, int op, cplx *out, int *ric, int cap) {

@@ -166,6 +166,10 @@ class Engine {
   size_t chunk_budget_ = 0;  // 0: from the free device memory
   int prec_vdv_ = 0, prec_factors_ = 0;
   DevBuf q0f_, q1f_, q2f_;  // FP32 copies of the factor slabs (factors precision 1)
+  struct FactorJobs;        // build_factors' job tables (engine.cpp)
+  std::shared_ptr<FactorJobs> fjobs_;
+  DevBuf mbuf_;  // Q2 stage-1 products M' of one timeslice
+  bool phases_ready_ = false;  // momentum phases uploaded for the current configuration
   StageTimes times_;
   // contract()'s plan: everything derived from the input file alone (quantum
   // numbers, trace groups, per-chunk index tables); rebuilt after configure()

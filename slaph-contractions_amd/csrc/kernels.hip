@@ -820,6 +820,15 @@ __global__ void k_accumulate(const double2 *__restrict__ cval, const int *__rest
   }
 }
 
+__global__ void k_finalize(const double2 *__restrict__ sum, const double2 *__restrict__ comp, int ncorr, int Lt,
+                           double2 *__restrict__ out) {
+  const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (id >= (long long)ncorr * Lt) return;
+  const int t = int(id % Lt), i = int(id / Lt);
+  const long long at = (long long)t * ncorr + i;
+  out[id] = make_double2((sum[at].x + comp[at].x) / double(Lt), (sum[at].y + comp[at].y) / double(Lt));
+}
+
 __global__ void k_to_f32(const double2 *__restrict__ in, float2 *__restrict__ out, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     out[i] = make_float2(float(in[i].x), float(in[i].y));
@@ -1076,6 +1085,13 @@ void launch_trace_grid(const void *sX, const long long *Xq, const void *sY, cons
 void launch_to_f32(const double2 *in, void *out, long long n, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_to_f32, grid1d(n, 256), dim3(256), 0, s, in, static_cast<float2 *>(out), n);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_finalize(const double2 *sum, const double2 *comp, int ncorr, int Lt, double2 *out, hipStream_t s) {
+  const long long n = (long long)ncorr * Lt;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_finalize, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, sum, comp, ncorr, Lt, out);
   SLAPH_HIP(hipGetLastError());
 }
 

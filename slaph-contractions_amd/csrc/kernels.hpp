@@ -185,6 +185,8 @@ void launch_resolve(const double2 *gvals, const uint16_t *gmasks, const CorrJob 
 void launch_accumulate(const double2 *cval, const int *tdelta, int nsp, int ncorr, double2 *sum, double2 *comp,
                        int kahan, hipStream_t s);
 void launch_scale(double2 *x, long long n, double f, hipStream_t s);
+// Correlator values (Diagram.cpp:132-147): out[i*Lt + t] = (sum + comp)[t*ncorr + i] / Lt.
+void launch_finalize(const double2 *sum, const double2 *comp, int ncorr, int Lt, double2 *out, hipStream_t s);
 
 // Synthetic inputs: seeded counter-based Gaussian / Z2xZ2 fills.
 void launch_fill_gauss(double2 *x, long long n, unsigned long long seed, hipStream_t s);
