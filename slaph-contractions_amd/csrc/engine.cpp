@@ -403,8 +403,8 @@ struct Engine::FactorJobs {
   DevBuf q0j, q1j, drs;
   int nq0 = 0, nq1 = 0;
   struct PerT {
-    DevBuf aj, bj;
-    int na = 0, nb = 0;
+    DevBuf ag, ar, bj;  // stage-1 groups and rows, stage-2 jobs
+    int ng = 0, maxj = 0, nb = 0;
   };
   std::vector<PerT> q2t;
 };
@@ -516,26 +516,30 @@ void Engine::build_factors(std::vector<std::vector<int>> const &q1keys, std::vec
     fj->q2t.resize(Lt_);
     for (int t = 0; t < Lt_; ++t) {
       if (mjob[t].empty()) continue;
-      std::vector<AhBJob> ajobs;
+      // one group per operator entry: its jobs share the right matrix VdV(e, t)
+      std::map<int, std::vector<AhBRow>> by_entry;
       for (auto const &kv : mjob[t]) {
         int b1 = kv.first[0], e = kv.first[1], r1 = kv.first[2];
-        for (int bd = 0; bd < 4; ++bd) {
-          AhBJob a;
-          a.A = peram + r1 * psz + (long long)(b1 * D_) * prow + (long long)(t * 4 + bd) * N_;
-          a.lda = prow;
-          a.B = vdv_ptr(lst2[e].id_vdaggerv, lst2[e].dagger, t);
-          a.ldb = N_;
-          a.C = mbuf_.as<double2>() + kv.second * msz + (long long)bd * N_ * D_;
-          a.ldc = D_;
-          a.phase = nullptr;
-          a.rowscale = fj->drs.as<double2>() + (lst2[e].gamma.at(0) * 4 + bd) * D_;
-          ajobs.push_back(a);
-        }
+        for (int bd = 0; bd < 4; ++bd)
+          by_entry[e].push_back({peram + r1 * psz + (long long)(b1 * D_) * prow + (long long)(t * 4 + bd) * N_,
+                                 fj->drs.as<double2>() + (lst2[e].gamma.at(0) * 4 + bd) * D_,
+                                 mbuf_.as<double2>() + kv.second * msz + (long long)bd * N_ * D_});
+      }
+      std::vector<AhBGroup> groups;
+      std::vector<AhBRow> rows;
+      int maxj = 0;
+      for (auto const &kv : by_entry) {
+        groups.push_back({vdv_ptr(lst2[kv.first].id_vdaggerv, lst2[kv.first].dagger, t), int(rows.size()),
+                          int(kv.second.size())});
+        rows.insert(rows.end(), kv.second.begin(), kv.second.end());
+        maxj = std::max(maxj, int(kv.second.size()));
       }
       for (size_t i = 0; i < bjobs[t].size(); ++i) bjobs[t][i].Mp = mbuf_.as<double2>() + (long long)bjob_m[t][i] * msz;
-      upload(fj->q2t[t].aj, ajobs);
+      upload(fj->q2t[t].ag, groups);
+      upload(fj->q2t[t].ar, rows);
       upload(fj->q2t[t].bj, bjobs[t]);
-      fj->q2t[t].na = int(ajobs.size());
+      fj->q2t[t].ng = int(groups.size());
+      fj->q2t[t].maxj = maxj;
       fj->q2t[t].nb = int(bjobs[t].size());
     }
     fj->sig = signature();
@@ -545,9 +549,8 @@ void Engine::build_factors(std::vector<std::vector<int>> const &q1keys, std::vec
   launch_q0(fj.q0j.as<Q0Job>(), fj.nq0, N_, gd_.dilE(), stream_);
   launch_q1(fj.q1j.as<Q1Job>(), fj.nq1, N_, gd_.dilE(), prow, stream_);
   for (auto const &pt : fj.q2t) {
-    if (pt.na == 0) continue;
-    for (int i = 0; i < pt.na; i += 65535)
-      launch_ahb(pt.aj.as<AhBJob>() + i, std::min(65535, pt.na - i), D_, N_, N_, stream_);
+    if (pt.ng == 0) continue;
+    launch_ahb_grouped(pt.ag.as<AhBGroup>(), pt.ng, pt.maxj, pt.ar.as<AhBRow>(), D_, N_, N_, prow, N_, D_, stream_);
     launch_q2b(pt.bj.as<Q2BJob>(), pt.nb, N_, gd_.dilE(), prow, stream_);
   }
   SLAPH_HIP(hipStreamSynchronize(stream_));

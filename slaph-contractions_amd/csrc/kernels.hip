@@ -130,6 +130,74 @@ __global__ __launch_bounds__(256) void k_ahb(const AhBJob *__restrict__ jobs, in
       }
 }
 
+// Grouped form of the same product for the first Q2 stage: the jobs of rpj
+// rows (rpj = D) that share one right matrix B are stacked into one tall
+// A^H B, so a 64 x 64 tile holds 64 / D jobs; rows of job j come from
+// rows[job0 + j].A and go to rows[job0 + j].C scaled by its rowscale.
+__global__ __launch_bounds__(256) void k_ahb_grouped(const AhBGroup *__restrict__ groups,
+                                                     const AhBRow *__restrict__ rows, int rpj, int N, int K,
+                                                     long long lda, long long ldb, long long ldc) {
+  const AhBGroup G = groups[blockIdx.z];
+  const int m0 = blockIdx.x * AHB_TM, n0 = blockIdx.y * AHB_TN;
+  const int M = G.njob * rpj;
+  if (m0 >= M) return;
+  __shared__ double2 As[AHB_TM][AHB_KC + 1];
+  __shared__ double2 Bs[AHB_TN][AHB_KC + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  d4 cr[2][2], ci[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      cr[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+      ci[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+  for (int k0 = 0; k0 < K; k0 += AHB_KC) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int idx = tid + r * 256;
+      const int kk = idx % AHB_KC, mm = idx / AHB_KC;
+      const int k = k0 + kk, m = m0 + mm, n = n0 + mm;
+      double2 av = make_double2(0.0, 0.0), bv = make_double2(0.0, 0.0);
+      if (k < K && m < M) av = rows[G.job0 + m / rpj].A[(long long)(m % rpj) * lda + k];
+      if (k < K && n < N) bv = G.B[(long long)n * ldb + k];
+      As[mm][kk] = av;
+      Bs[mm][kk] = bv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kq = 0; kq < AHB_KC; kq += 4) {
+      const int kr = kq + (lane >> 4);
+      const double2 av[2] = {As[wm * 32 + (lane & 15)][kr], As[wm * 32 + 16 + (lane & 15)][kr]};
+      const double2 bv[2] = {Bs[wn * 32 + (lane & 15)][kr], Bs[wn * 32 + 16 + (lane & 15)][kr]};
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) {
+          // conj(a) b = (ax bx + ay by) + i (ax by - ay bx)
+          cr[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi].x, bv[ni].x, cr[mi][ni], 0, 0, 0);
+          cr[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi].y, bv[ni].y, cr[mi][ni], 0, 0, 0);
+          ci[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi].x, bv[ni].y, ci[mi][ni], 0, 0, 0);
+          ci[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[mi].y, bv[ni].x, ci[mi][ni], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = m0 + wm * 32 + mi * 16 + (lane >> 4) + 4 * i;
+        const int col = n0 + wn * 32 + ni * 16 + (lane & 15);
+        if (row < M && col < N) {
+          const AhBRow R = rows[G.job0 + row / rpj];
+          const double2 v = cmul(R.rowscale[row % rpj], make_double2(cr[mi][ni][i], ci[mi][ni][i]));
+          R.C[(long long)col * ldc + row % rpj] = v;
+        }
+      }
+}
+
 // ---------------------------------------------------------------------------
 // VdaggerV for a group of momenta (reference: OperatorFactory.cpp:294-300).
 // One workgroup = one 32 x 32 complex output tile of every momentum of the
@@ -357,6 +425,34 @@ __global__ __launch_bounds__(256) void k_q2b(const Q2BJob *__restrict__ jobs, in
   }
 }
 
+// D = 16: Q2 = M' (16 x 4N) * stacked P2 rows (4N x 16) as one MFMA tile per
+// job (one wavefront): K index k = bd*N + w; lane l supplies M'[l%16][4s + l/16]
+// (address k*16 + row: 1 KB coalesced per k-step) and P2[4s + l/16][l%16]
+// (4 consecutive w of one perambulator column).  4 jobs per workgroup.
+__global__ __launch_bounds__(256) void k_q2b16(const Q2BJob *__restrict__ jobs, int njobs, int N, long long prow) {
+  const int job = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (job >= njobs) return;
+  const Q2BJob J = jobs[job];
+  const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
+  d4 cr = d4{0.0, 0.0, 0.0, 0.0}, ci = cr;
+  for (int bd = 0; bd < 4; ++bd) {
+    const int g = c_gamma_row[J.gamma][bd];
+    const double2 *pcol = J.peram2 + (long long)(J.b2 * 16 + col) * prow + (long long)(J.t * 4 + g) * N + q;
+    const double2 *mrow = J.Mp + (long long)bd * N * 16 + q * 16 + col;
+#pragma unroll 4
+    for (int w0 = 0; w0 < N; w0 += 4) {
+      const double2 a = mrow[(long long)w0 * 16], b = pcol[w0];
+      // (ax + i ay)(bx + i by) = (ax bx - ay by) + i (ax by + ay bx)
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, cr, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, ci, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, ci, 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) J.out[(lane >> 4) + 4 * i + 16 * col] = make_double2(cr[i], ci[i]);
+}
+
 __global__ void k_dagger(const double2 *__restrict__ in, double2 *__restrict__ out, int N, int count) {
   const long long total = (long long)N * N * count;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
@@ -410,7 +506,7 @@ __global__ __launch_bounds__(64) void k_prod(const T *__restrict__ sX, const lon
 }
 
 // D = 16 factor products on FP64 MFMA: one 16 x 16 x 16 MFMA tile (4 k-steps x
-// 4 real products).  With the MFMA operand layout A[m = lane%16][k = lane/16],
+// 3 real products, the Gauss form).  With the MFMA operand layout A[m = lane%16][k = lane/16],
 // B[k = lane/16][n = lane%16] and result element i of lane l at
 // (l/16 + 4i, l%16), lane l reads X at l + 64s (k-step s, coalesced) and Y at
 // (4s + l/16) + 16 (l%16) (64-byte runs), and writes element l + 64i of the
@@ -447,25 +543,25 @@ __global__ __launch_bounds__(256) void k_prod16(const T *__restrict__ sX, const 
     double2 yb[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) yb[s] = ld2(y + 4 * s);
-    d4 dr = d4{0.0, 0.0, 0.0, 0.0}, di = d4{0.0, 0.0, 0.0, 0.0};
+    // C[r][c] = sum_k X[r][k] Y[k][c] from three real products:
+    // P1 = Xr Yr, P2 = Xi Yi, P3 = (Xr + Xi)(Yr + Yi); re = P1 - P2, im = P3 - P1 - P2
+    d4 p1 = d4{0.0, 0.0, 0.0, 0.0}, p2 = p1, p3 = p1;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      // C[r][c] = sum_k X[r][k] Y[k][c]: re = Xr Yr - Xi Yi, im = Xr Yi + Xi Yr
+      const double xs = xa[s].x + xa[s].y, ys = yb[s].x + yb[s].y;
       if (kTransposed) {
-        dr = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s].x, yb[s].x, dr, 0, 0, 0);
-        dr = __builtin_amdgcn_mfma_f64_16x16x4f64(-xa[s].y, yb[s].y, dr, 0, 0, 0);
-        di = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s].x, yb[s].y, di, 0, 0, 0);
-        di = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s].y, yb[s].x, di, 0, 0, 0);
+        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s].x, yb[s].x, p1, 0, 0, 0);
+        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s].y, yb[s].y, p2, 0, 0, 0);
+        p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(xs, ys, p3, 0, 0, 0);
       } else {
-        dr = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].x, xa[s].x, dr, 0, 0, 0);
-        dr = __builtin_amdgcn_mfma_f64_16x16x4f64(-yb[s].y, xa[s].y, dr, 0, 0, 0);
-        di = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].y, xa[s].x, di, 0, 0, 0);
-        di = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].x, xa[s].y, di, 0, 0, 0);
+        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].x, xa[s].x, p1, 0, 0, 0);
+        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(yb[s].y, xa[s].y, p2, 0, 0, 0);
+        p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ys, xs, p3, 0, 0, 0);
       }
     }
     T *o = out + (obase + ti * ostride + cb.o) * 256;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) st2(o + lane + 64 * i, make_double2(dr[i], di[i]));
+    for (int i = 0; i < 4; ++i) st2(o + lane + 64 * i, make_double2(p1[i] - p2[i], p3[i] - p1[i] - p2[i]));
   }
 }
 
@@ -917,6 +1013,15 @@ void launch_ahb(const AhBJob *d_jobs, int njobs, int M, int N, int K, hipStream_
   SLAPH_HIP(hipGetLastError());
 }
 
+void launch_ahb_grouped(const AhBGroup *d_groups, int ngroups, int max_jobs, const AhBRow *d_rows, int rpj, int N,
+                        int K, long long lda, long long ldb, long long ldc, hipStream_t s) {
+  if (ngroups <= 0 || max_jobs <= 0) return;
+  if (ngroups > 65535) throw std::runtime_error("launch_ahb_grouped: too many groups in one launch");
+  dim3 grid((max_jobs * rpj + AHB_TM - 1) / AHB_TM, (N + AHB_TN - 1) / AHB_TN, ngroups);
+  hipLaunchKernelGGL(k_ahb_grouped, grid, dim3(256), 0, s, d_groups, d_rows, rpj, N, K, lda, ldb, ldc);
+  SLAPH_HIP(hipGetLastError());
+}
+
 void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, int f32, hipStream_t s) {
   if (njobs <= 0) return;
   if (njobs > 65535) throw std::runtime_error("launch_vdv: too many jobs in one launch");
@@ -949,7 +1054,10 @@ void launch_q1(const Q1Job *d_jobs, int njobs, int N, int dilE, long long prow, 
 void launch_q2b(const Q2BJob *d_jobs, int njobs, int N, int dilE, long long prow, hipStream_t s) {
   if (njobs <= 0) return;
   upload_gamma_constants();
-  hipLaunchKernelGGL(k_q2b, dim3(njobs), dim3(256), 0, s, d_jobs, N, dilE, prow);
+  if (dilE == 4 && N % 4 == 0)
+    hipLaunchKernelGGL(k_q2b16, dim3((njobs + 3) / 4), dim3(256), 0, s, d_jobs, njobs, N, prow);
+  else
+    hipLaunchKernelGGL(k_q2b, dim3(njobs), dim3(256), 0, s, d_jobs, N, dilE, prow);
   SLAPH_HIP(hipGetLastError());
 }
 

@@ -29,6 +29,20 @@ struct AhBJob {
   long long lda, ldb, ldc;
 };
 void launch_ahb(const AhBJob *d_jobs, int njobs, int M, int N, int K, hipStream_t s);
+// Grouped products of rpj-row jobs: group g stacks rows[job0 .. job0+njob)
+// (row r of job j at rows[j].A + r*lda) against one right matrix B (K x N,
+// ldb); job j's rows go to rows[j].C (rpj x N, ldc) scaled by rows[j].rowscale.
+struct AhBRow {
+  const double2 *A;
+  const double2 *rowscale;
+  double2 *C;
+};
+struct AhBGroup {
+  const double2 *B;
+  int job0, njob;
+};
+void launch_ahb_grouped(const AhBGroup *d_groups, int ngroups, int max_jobs, const AhBRow *d_rows, int rpj, int N,
+                        int K, long long lda, long long ldb, long long ldc, hipStream_t s);
 
 // VdaggerV of one timeslice for a group of up to kVdvGroup momenta:
 // C[p] = V^H diag(phase[p][k/3]) V, V: K x N column-major (ld = K), C: N x N.
