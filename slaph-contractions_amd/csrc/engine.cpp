@@ -4,10 +4,12 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <numeric>
 #include <set>
+#include <tuple>
 #include <unordered_map>
 
 namespace slaph {
@@ -91,6 +93,7 @@ void Engine::configure(std::string const &text) {
   results_.clear();
   plan_.reset();
   fjobs_.reset();
+  ft_.reset();
   phases_ready_ = false;
   SLAPH_HIP(hipSetDevice(device_));
   alloc_inputs();
@@ -326,6 +329,32 @@ void Engine::create_operators() {
     else
       ids.push_back(op.id);
   }
+  const char *direct = std::getenv("SLAPH_VDV_DIRECT");
+  if (prec_vdv_ == 0 && !ids.empty() && !(direct && *direct && *direct != '0')) {
+    vdaggerv_fourier(ids);
+  } else {
+    vdaggerv_direct(ids);
+  }
+  // Daggered copies of the operators that some factor uses with -p
+  // (build_VdaggerV_lookup's dagger flag, init_lookup_tables.cpp:257-259).
+  dag_slot_.assign(std::max(1, nvdv), -1);
+  int ndag = 0;
+  for (auto const &kv : gd_.quarkline_lookuptable)
+    for (auto const &e : kv.second)
+      if (e.dagger && dag_slot_[e.id_vdaggerv] < 0) dag_slot_[e.id_vdaggerv] = ndag++;
+  vdvdag_.alloc(sizeof(double2) * std::max(1, ndag) * Lt_ * N_ * N_);
+  for (int id = 0; id < nvdv; ++id)
+    if (dag_slot_[id] >= 0)
+      launch_dagger(vdv_.as<double2>() + (size_t)id * Lt_ * N_ * N_,
+                    vdvdag_.as<double2>() + (size_t)dag_slot_[id] * Lt_ * N_ * N_, N_, Lt_, stream_);
+  SLAPH_HIP(hipStreamSynchronize(stream_));
+  have_ops_ = true;
+}
+
+// Direct form: one V(t) panel pass per group of up to kVdvGroup momenta
+// (FP64 or the single-pass FP32 precision form).
+void Engine::vdaggerv_direct(std::vector<int> const &ids) {
+  int Vs = gd_.Lx * gd_.Ly * gd_.Lz;
   int ngroups = int((ids.size() + kVdvGroup - 1) / kVdvGroup);
   std::vector<VdvJob> jobs;
   for (int t = 0; t < Lt_; ++t) {
@@ -351,20 +380,140 @@ void Engine::create_operators() {
   times_.vdaggerv_kernel_ms = tm.stop();
   times_.vdaggerv_launches = int((jobs.size() + 65534) / 65535);
   times_.vdaggerv_ms = times_.vdaggerv_kernel_ms;
-  // Daggered copies of the operators that some factor uses with -p
-  // (build_VdaggerV_lookup's dagger flag, init_lookup_tables.cpp:257-259).
-  dag_slot_.assign(std::max(1, nvdv), -1);
-  int ndag = 0;
-  for (auto const &kv : gd_.quarkline_lookuptable)
-    for (auto const &e : kv.second)
-      if (e.dagger && dag_slot_[e.id_vdaggerv] < 0) dag_slot_[e.id_vdaggerv] = ndag++;
-  vdvdag_.alloc(sizeof(double2) * std::max(1, ndag) * Lt_ * N_ * N_);
-  for (int id = 0; id < nvdv; ++id)
-    if (dag_slot_[id] >= 0)
-      launch_dagger(vdv_.as<double2>() + (size_t)id * Lt_ * N_ * N_,
-                    vdvdag_.as<double2>() + (size_t)dag_slot_[id] * Lt_ * N_ * N_, N_, Lt_, stream_);
-  SLAPH_HIP(hipStreamSynchronize(stream_));
-  have_ops_ = true;
+}
+
+// Sitewise Fourier form (FP64): W_site = V_site^H V_site per 16 x 16 tile
+// pair by MFMA, then the z, y and x transforms of the separable phase
+// exp(-i 2 pi (px x / Lx + py y / Ly + pz z / Lz)) (create_momenta,
+// OperatorFactory.cpp:64-95).  Everything below the phase tables depends on
+// the input file only and is kept until the next configure().
+struct Engine::FtPlan {
+  int NT = 0, Npad = 0, ntp = 0, TC = 0;
+  DevBuf tiles;  // int2 (I, J), I <= J
+  struct Target {
+    int id, c, lower;
+  };
+  struct Pass {
+    FtPass p{};
+    std::vector<Target> targets;
+    DevBuf ey, zcs, ex, tg;
+  };
+  std::vector<Pass> passes;
+  int max_nc = 0;
+};
+
+void Engine::vdaggerv_fourier(std::vector<int> const &ids) {
+  const int Lx = gd_.Lx, Ly = gd_.Ly, Lz = gd_.Lz;
+  if (!ft_) {
+    auto F = std::make_shared<FtPlan>();
+    F->NT = (N_ + 15) / 16;
+    F->Npad = 16 * F->NT;
+    std::vector<int2> tl;
+    for (int I = 0; I < F->NT; ++I)
+      for (int J = I; J < F->NT; ++J) tl.push_back(make_int2(I, J));
+    F->ntp = int(tl.size());
+    upload(F->tiles, tl);
+    // Outputs: C(p) on every tile pair from the (py, pz) sums of +p, and the
+    // lower triangle conj(C(-p)[i,j]) from those of -p.
+    struct Want {
+      int id, px, py, pz, lower;
+    };
+    std::vector<Want> want;
+    std::map<std::pair<int, int>, int> combo_ix;
+    for (int id : ids) {
+      auto const &m = gd_.vdaggerv_lookup.at(id).mom;
+      want.push_back({id, m[0], m[1], m[2], 0});
+      if (F->NT > 1) want.push_back({id, -m[0], -m[1], -m[2], 1});
+    }
+    std::vector<std::pair<int, int>> combos;  // (py, pz)
+    for (auto const &w : want)
+      if (!combo_ix.count({w.py, w.pz})) {
+        combo_ix[{w.py, w.pz}] = 0;
+        combos.push_back({w.py, w.pz});
+      }
+    std::sort(combos.begin(), combos.end(), [](std::pair<int, int> a, std::pair<int, int> b) {
+      return std::make_tuple(std::abs(a.second), a.second, a.first) <
+             std::make_tuple(std::abs(b.second), b.second, b.first);
+    });
+    // Passes: at most kFtMaxC combinations and kFtMaxK nonzero |pz| each.
+    std::vector<std::vector<std::pair<int, int>>> pc;
+    std::vector<std::vector<int>> pk;
+    std::map<std::pair<int, int>, std::pair<int, int>> where;  // combo -> (pass, index)
+    for (auto const &cb : combos) {
+      int k = std::abs(cb.second);
+      bool fits = !pc.empty() && int(pc.back().size()) < kFtMaxC &&
+                  (k == 0 || std::count(pk.back().begin(), pk.back().end(), k) ||
+                   int(pk.back().size()) < kFtMaxK);
+      if (!fits) {
+        pc.emplace_back();
+        pk.emplace_back();
+      }
+      if (k != 0 && !std::count(pk.back().begin(), pk.back().end(), k)) pk.back().push_back(k);
+      where[cb] = {int(pc.size()) - 1, int(pc.back().size())};
+      pc.back().push_back(cb);
+    }
+    const double tpi = 2.0 * M_PI;
+    F->passes.resize(pc.size());
+    for (size_t ip = 0; ip < pc.size(); ++ip) {
+      FtPlan::Pass &P = F->passes[ip];
+      P.p.nc = int(pc[ip].size());
+      P.p.nk = int(pk[ip].size());
+      F->max_nc = std::max(F->max_nc, P.p.nc);
+      std::vector<cplx> ey((size_t)P.p.nc * Ly), zcs((size_t)kFtMaxK * Lz);  // rows past nk stay zero
+      for (int c = 0; c < P.p.nc; ++c) {
+        int py = pc[ip][c].first, pz = pc[ip][c].second;
+        int s = pz == 0 ? -1 : int(std::find(pk[ip].begin(), pk[ip].end(), std::abs(pz)) - pk[ip].begin());
+        P.p.tsel[c] = pz == 0 ? 0 : (pz > 0 ? 1 + 2 * s : 2 + 2 * s);
+        for (int y = 0; y < Ly; ++y) ey[(size_t)c * Ly + y] = std::exp(cplx(0, -tpi * py * y / Ly));
+      }
+      for (int s = 0; s < P.p.nk; ++s)
+        for (int z = 0; z < Lz; ++z) {
+          double a = tpi * pk[ip][s] * z / Lz;
+          zcs[(size_t)s * Lz + z] = cplx(std::cos(a), std::sin(a));
+        }
+      std::vector<cplx> ex;
+      for (auto const &w : want) {
+        auto at = where.at({w.py, w.pz});
+        if (at.first != int(ip)) continue;
+        P.targets.push_back({w.id, at.second, w.lower});
+        for (int x = 0; x < Lx; ++x) ex.push_back(std::exp(cplx(0, -tpi * w.px * x / Lx)));
+      }
+      upload(P.ey, ey);
+      upload(P.zcs, zcs);
+      upload(P.ex, ex);
+    }
+    // Timeslice chunk: the row-major copy and the per-x sums within ~4 GiB.
+    const double per_t = 16.0 * K_ * F->Npad + 16.0 * Lx * F->max_nc * F->ntp * 256;
+    F->TC = std::max(1, std::min(Lt_, int((4.0 * (1ull << 30)) / per_t)));
+    ft_ = F;
+  }
+  FtPlan &F = *ft_;
+  // Output pointers of the targets (vdv_ may have moved since the plan).
+  for (auto &P : F.passes) {
+    std::vector<FtTarget> tg;
+    for (auto const &t : P.targets)
+      tg.push_back(FtTarget{vdv_.as<double2>() + (size_t)t.id * Lt_ * N_ * N_, t.c, t.lower});
+    upload(P.tg, tg);
+  }
+  vr_.ensure(sizeof(double2) * (size_t)F.TC * K_ * F.Npad);
+  ftu_.ensure(sizeof(double2) * (size_t)F.TC * Lx * F.max_nc * F.ntp * 256);
+  EventTimer tm(stream_);
+  int launches = 0;
+  for (int t0 = 0; t0 < Lt_; t0 += F.TC) {
+    int nt = std::min(F.TC, Lt_ - t0);
+    launch_vrow(V_.as<double2>() + (size_t)t0 * K_ * N_, vr_.as<double2>(), K_, N_, F.Npad, nt, stream_);
+    ++launches;
+    for (auto &P : F.passes) {
+      launch_vdv_ft(vr_.as<double2>(), K_ * F.Npad, F.Npad, Lx, Ly, Lz, F.tiles.as<int2>(), F.ntp, nt, P.p,
+                    P.ey.as<double2>(), P.zcs.as<double2>(), ftu_.as<double2>(), stream_);
+      launch_vdv_xsum(ftu_.as<double2>(), P.p.nc, F.ntp, nt, Lx, F.tiles.as<int2>(), P.tg.as<FtTarget>(),
+                      int(P.targets.size()), P.ex.as<double2>(), N_, t0, (long long)N_ * N_, stream_);
+      launches += 2;
+    }
+  }
+  times_.vdaggerv_kernel_ms = tm.stop();
+  times_.vdaggerv_launches = launches;
+  times_.vdaggerv_ms = times_.vdaggerv_kernel_ms;
 }
 
 const double2 *Engine::vdv_ptr(int id, bool dagger, int t) const {

@@ -137,6 +137,8 @@ class Engine {
   const double2 *vdv_ptr(int id, bool dagger, int t) const;
   void compute_q_offsets();
   void build_factors(std::vector<std::vector<int>> const &q1keys, std::vector<std::vector<int>> const &q2keys);
+  void vdaggerv_fourier(std::vector<int> const &ids);
+  void vdaggerv_direct(std::vector<int> const &ids);
 
   int device_;
   hipStream_t stream_ = nullptr;
@@ -170,6 +172,9 @@ class Engine {
   std::shared_ptr<FactorJobs> fjobs_;
   DevBuf mbuf_;  // Q2 stage-1 products M' of one timeslice
   bool phases_ready_ = false;  // momentum phases uploaded for the current configuration
+  struct FtPlan;  // tile pairs, passes and phase tables of the Fourier VdaggerV (engine.cpp)
+  std::shared_ptr<FtPlan> ft_;
+  DevBuf vr_, ftu_;  // row-major eigenvectors and per-x sums of one timeslice chunk
   StageTimes times_;
   // contract()'s plan: everything derived from the input file alone (quantum
   // numbers, trace groups, per-chunk index tables); rebuilt after configure()

@@ -334,6 +334,217 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 }
 
 // ---------------------------------------------------------------------------
+// VdaggerV by the sitewise Fourier factorisation (see kernels.hpp).  One
+// workgroup = one (timeslice, x, tile pair I <= J); its two waves run the same
+// per-site W MFMAs (lane l: colour l/16, column l%16; colour 3 is zero
+// padding) and each transforms two of the four accumulator rows, so that the
+// z sums (cos / sin per |pz|) and the y sums per (py, pz) stay in registers.
+// Per site: 3 MFMAs (Gauss: W = (P1 + P2) + i (P3 - P1 + P2)) and
+// 2 (3 + 2 + 4 nk) + 4 FP64 vector operations per lane.  The operands of the
+// next four sites are loaded while the current four are processed.
+// ---------------------------------------------------------------------------
+template <int NC, int NK, int H>
+__device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, int Npad, int I, int J, int Ly, int Lz,
+                                            long long site0, FtPass const &P, const double2 *__restrict__ ey,
+                                            const double2 *__restrict__ zcs, double2 *__restrict__ Uo,
+                                            long long ustride) {
+  const int lane = threadIdx.x & 63, c = lane >> 4, m = lane & 15;
+  // Colour-3 lanes (the fourth K slot) read colour 2 and zero their A
+  // operand, so that every load is unconditional.
+  const double amask = c < 3 ? 1.0 : 0.0;
+  const int cl = c < 3 ? c : 2;
+  const long long sstep = 3LL * Npad;
+  const double2 *pa = Vr + (site0 * 3 + cl) * Npad + I * 16 + m;
+  const double2 *pb = Vr + (site0 * 3 + cl) * Npad + J * 16 + m;
+  const double2 zero2 = make_double2(0.0, 0.0);
+  double2 U[NC][2];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) U[k][0] = U[k][1] = zero2;
+  double a0r[2], a0i[2], cr[NK][2], ci[NK][2], sr[NK][2], si[NK][2];
+  const int GZ = (Lz + 3) >> 2, ng = Ly * GZ;
+  double2 na[4], nb[4];
+  auto load = [&](int g) {
+    const int y = g / GZ, z0 = (g - y * GZ) * 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long off = (long long)(y * Lz + min(z0 + u, Lz - 1)) * sstep;
+      na[u] = pa[off];
+      nb[u] = pb[off];
+    }
+  };
+  // W of site u of the current group: three MFMAs (Gauss).
+  auto mf = [&](const double2 &a, const double2 &b, d4 &p1, d4 &p2, d4 &p3) {
+    const d4 dz = {0.0, 0.0, 0.0, 0.0};
+    const double ax = a.x * amask, ay = a.y * amask;
+    p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ax, b.x, dz, 0, 0, 0);
+    p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ay, b.y, dz, 0, 0, 0);
+    p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ax - ay, b.x + b.y, dz, 0, 0, 0);
+  };
+  // z transform of the two rows of this wave.
+  auto zacc = [&](int z, const d4 &p1, const d4 &p2, const d4 &p3) {
+    double2 cs[NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) cs[k] = zcs[k * Lz + z];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int r = 2 * H + q;
+      const double wr = p1[r] + p2[r], wi = (p3[r] - p1[r]) + p2[r];
+      a0r[q] += wr;
+      a0i[q] += wi;
+#pragma unroll
+      for (int k = 0; k < NK; ++k) {
+        cr[k][q] = fma(cs[k].x, wr, cr[k][q]);
+        ci[k][q] = fma(cs[k].x, wi, ci[k][q]);
+        sr[k][q] = fma(cs[k].y, wr, sr[k][q]);
+        si[k][q] = fma(cs[k].y, wi, si[k][q]);
+      }
+    }
+  };
+  load(0);
+  for (int g = 0; g < ng; ++g) {
+    double2 ca[4], cb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ca[u] = na[u];
+      cb[u] = nb[u];
+    }
+    if (g + 1 < ng) load(g + 1);
+    const int y = g / GZ, z0 = (g - y * GZ) * 4;
+    if (z0 == 0) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        a0r[q] = a0i[q] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NK; ++k) cr[k][q] = ci[k][q] = sr[k][q] = si[k][q] = 0.0;
+      }
+    }
+    // Software pipeline: the MFMAs of site u + 1 are issued before the vector
+    // work of site u.
+    d4 e1, e2, e3, o1, o2, o3;
+    mf(ca[0], cb[0], e1, e2, e3);
+    if (z0 + 1 < Lz) mf(ca[1], cb[1], o1, o2, o3);
+    zacc(z0, e1, e2, e3);
+    if (z0 + 2 < Lz) mf(ca[2], cb[2], e1, e2, e3);
+    if (z0 + 1 < Lz) zacc(z0 + 1, o1, o2, o3);
+    if (z0 + 3 < Lz) mf(ca[3], cb[3], o1, o2, o3);
+    if (z0 + 2 < Lz) zacc(z0 + 2, e1, e2, e3);
+    if (z0 + 3 < Lz) zacc(z0 + 3, o1, o2, o3);
+    if (z0 + 4 >= Lz) {
+      // End of a z line: T(+-k) = (CR +- SI) + i (CI -+ SR), then
+      // U[combination] += exp(-i 2 pi py y / Ly) T(pz).
+#pragma unroll
+      for (int k2 = 0; k2 < NC; ++k2) {
+        if (k2 < P.nc) {
+          const int sel = P.tsel[k2];
+          const double2 e = ey[k2 * Ly + y];
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            double tr = a0r[q], ti = a0i[q];
+#pragma unroll
+            for (int k = 0; k < NK; ++k) {
+              if (sel == 1 + 2 * k) {
+                tr = cr[k][q] + si[k][q];
+                ti = ci[k][q] - sr[k][q];
+              } else if (sel == 2 + 2 * k) {
+                tr = cr[k][q] - si[k][q];
+                ti = ci[k][q] + sr[k][q];
+              }
+            }
+            U[k2][q].x = fma(e.x, tr, fma(-e.y, ti, U[k2][q].x));
+            U[k2][q].y = fma(e.x, ti, fma(e.y, tr, U[k2][q].y));
+          }
+        }
+      }
+    }
+  }
+  // Accumulator row r of lane l is tile row l/16 + 4r, column l%16: pair
+  // index (l/16 + 4r) * 16 + l%16 = l + 64 r.
+#pragma unroll
+  for (int k2 = 0; k2 < NC; ++k2) {
+    if (k2 < P.nc) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) Uo[k2 * ustride + lane + 64 * (2 * H + q)] = U[k2][q];
+    }
+  }
+}
+
+template <int NC, int NK>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_vdv_ft(
+    const double2 *__restrict__ Vr, long long vstride, int Npad, int Lx, int Ly, int Lz,
+    const int2 *__restrict__ tiles, int ntp, FtPass P, const double2 *__restrict__ ey,
+    const double2 *__restrict__ zcs, double2 *__restrict__ U) {
+  // XCD x runs the x-th contiguous eighth of the (t, x, tile pair) blocks, so
+  // the tile pairs of one x slab share that XCD's L2.
+  const int G = gridDim.x, per = G / 8, rem = G % 8, xcd = blockIdx.x % 8, jx = blockIdx.x / 8;
+  const int lb = xcd < rem ? xcd * (per + 1) + jx : rem * (per + 1) + (xcd - rem) * per + jx;
+  const int tp = lb % ntp, x = (lb / ntp) % Lx, t = lb / (ntp * Lx);
+  const int2 ij = tiles[tp];
+  const long long site0 = (long long)x * Ly * Lz;
+  const long long ustride = (long long)ntp * 256;
+  double2 *Uo = U + (((long long)t * Lx + x) * P.nc * ntp + tp) * 256;
+  if (threadIdx.x < 64)
+    vdv_ft_wave<NC, NK, 0>(Vr + t * vstride, Npad, ij.x, ij.y, Ly, Lz, site0, P, ey, zcs, Uo, ustride);
+  else
+    vdv_ft_wave<NC, NK, 1>(Vr + t * vstride, Npad, ij.x, ij.y, Ly, Lz, site0, P, ey, zcs, Uo, ustride);
+}
+
+// C(p) from the per-x sums: thread = (row, column) of one tile pair.
+__global__ __launch_bounds__(256) void k_vdv_xsum(const double2 *__restrict__ U, int nc, int ntp, int Lx,
+                                                  const int2 *__restrict__ tiles, const FtTarget *__restrict__ tg,
+                                                  int ntg, const double2 *__restrict__ ex, int N, int t0,
+                                                  long long tstride) {
+  const int tp = blockIdx.x, t = blockIdx.y, q = threadIdx.x;
+  const int row = q & 15, col = q >> 4;
+  const int2 ij = tiles[tp];
+  const int i = ij.x * 16 + row, j = ij.y * 16 + col;
+  const bool ok = i < N && j < N;
+  const long long xstride = (long long)nc * ntp * 256;
+  const double2 *u0 = U + (long long)t * Lx * xstride + (long long)tp * 256 + row * 16 + col;
+  for (int k = 0; k < ntg; ++k) {
+    const FtTarget T = tg[k];
+    if (T.lower && ij.x == ij.y) continue;
+    const double2 *u = u0 + (long long)T.c * ntp * 256;
+    const double2 *e = ex + (long long)k * Lx;
+    double sr = 0.0, si = 0.0;
+    for (int x = 0; x < Lx; ++x) {
+      const double2 v = u[x * xstride], f = e[x];
+      sr = fma(f.x, v.x, fma(-f.y, v.y, sr));
+      si = fma(f.x, v.y, fma(f.y, v.x, si));
+    }
+    if (!ok) continue;
+    double2 *C = T.C + (long long)(t0 + t) * tstride;
+    if (!T.lower)
+      C[(long long)j * N + i] = make_double2(sr, si);
+    else
+      C[(long long)i * N + j] = make_double2(sr, -si);
+  }
+}
+
+// Column-major K x N -> row-major K x Npad through a 64 x 16 LDS tile.
+__global__ __launch_bounds__(256) void k_vrow(const double2 *__restrict__ V, double2 *__restrict__ Vr, long long K,
+                                              int N, int Npad) {
+  __shared__ double2 tile[16][65];
+  const long long k0 = (long long)blockIdx.x * 64;
+  const int n0 = blockIdx.y * 16, t = blockIdx.z;
+  const double2 *src = V + (long long)t * K * N;
+  double2 *dst = Vr + (long long)t * K * Npad;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int kk = tid & 63, cc = (tid >> 6) + 4 * r;
+    const long long k = k0 + kk;
+    tile[cc][kk] = (k < K && n0 + cc < N) ? src[(long long)(n0 + cc) * K + k] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int cc = tid & 15, kk = (tid >> 4) + 16 * r;
+    const long long k = k0 + kk;
+    if (k < K) dst[k * Npad + n0 + cc] = tile[cc][kk];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Q0[(e1 + dilE g(d)), (e2 + dilE d)] =
 //   gval(d) sum_{v=e1 mod dilE} conj rho1(v, g(d)) sum_{w=e2 mod dilE} VdV[v,w] rho2(w,d)
 // (reference: DilutedFactorFactory<Q0>::build, DilutedFactorFactory.cpp:33-101)
@@ -1030,6 +1241,42 @@ void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, int f32, hi
     hipLaunchKernelGGL(k_vdv<true>, dim3(tiles * tiles, njobs), dim3(256), 0, s, d_jobs, N, K, tiles);
   else
     hipLaunchKernelGGL(k_vdv<false>, dim3(tiles * tiles, njobs), dim3(256), 0, s, d_jobs, N, K, tiles);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_vdv_ft(const double2 *Vr, long long vstride, int Npad, int Lx, int Ly, int Lz, const int2 *tiles,
+                   int ntp, int nt, FtPass const &pass, const double2 *ey, const double2 *zcs, double2 *U,
+                   hipStream_t s) {
+  if (nt <= 0 || ntp <= 0 || pass.nc <= 0) return;
+  if (pass.nc > kFtMaxC || pass.nk < 0 || pass.nk > kFtMaxK)
+    throw std::runtime_error("launch_vdv_ft: pass exceeds the combination limits");
+  if (Npad % 16 != 0) throw std::runtime_error("launch_vdv_ft: Npad must be a multiple of 16");
+  const long long blocks = (long long)nt * Lx * ntp;
+  if (blocks > 0x7fffffffLL) throw std::runtime_error("launch_vdv_ft: grid too large");
+  if (pass.nc <= 9 && pass.nk <= 1)
+    hipLaunchKernelGGL((k_vdv_ft<9, 1>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride, Npad, Lx, Ly, Lz,
+                       tiles, ntp, pass, ey, zcs, U);
+  else
+    hipLaunchKernelGGL((k_vdv_ft<kFtMaxC, kFtMaxK>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride, Npad, Lx,
+                       Ly, Lz, tiles, ntp, pass, ey, zcs, U);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_vdv_xsum(const double2 *U, int nc, int ntp, int nt, int Lx, const int2 *tiles, const FtTarget *targets,
+                     int ntarget, const double2 *ex, int N, int t0, long long tstride, hipStream_t s) {
+  if (nt <= 0 || ntp <= 0 || ntarget <= 0) return;
+  if (nt > 65535) throw std::runtime_error("launch_vdv_xsum: too many timeslices in one launch");
+  hipLaunchKernelGGL(k_vdv_xsum, dim3(ntp, nt), dim3(256), 0, s, U, nc, ntp, Lx, tiles, targets, ntarget, ex, N, t0,
+                     tstride);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_vrow(const double2 *V, double2 *Vr, long long K, int N, int Npad, int nt, hipStream_t s) {
+  if (nt <= 0) return;
+  if (Npad % 16 != 0 || Npad < N) throw std::runtime_error("launch_vrow: bad Npad");
+  const long long kb = (K + 63) / 64;
+  if (kb > 0x7fffffffLL || nt > 65535) throw std::runtime_error("launch_vrow: grid too large");
+  hipLaunchKernelGGL(k_vrow, dim3(unsigned(kb), Npad / 16, nt), dim3(256), 0, s, V, Vr, K, N, Npad);
   SLAPH_HIP(hipGetLastError());
 }
 

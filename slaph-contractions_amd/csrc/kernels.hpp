@@ -58,6 +58,45 @@ struct VdvJob {
 // accumulation), results widened to FP64.
 void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, int f32, hipStream_t s);
 
+// VdaggerV by a sitewise Fourier factorisation.  The momentum phase of site
+// (x, y, z) factorises, exp(-i 2 pi (px x / Lx + py y / Ly + pz z / Lz)), so
+//   C(p)[i,j] = sum_x ex(x) sum_y ey(y) sum_z ez(z) W_xyz[i,j],
+//   W_xyz[i,j] = sum_c conj(V[3 site + c, i]) V[3 site + c, j].
+// k_vdv_ft forms W of one site for a 16 x 16 (i, j) tile with three FP64
+// MFMAs, transforms it over z on the vector ALUs (cos / sin sums, which give
+// +pz and -pz together) and over y into per-(py, pz) sums U(x), stored per x.
+// k_vdv_xsum sums U over x with the px phases into the N x N matrices.  Only
+// tile pairs I <= J are formed: C(p)[j,i] = conj(C(-p)[i,j]) supplies the
+// lower triangle.
+// V is read in a row-major copy Vr (K x Npad, Npad = 16 * ceil(N / 16),
+// padding columns zero) made by launch_vrow from the column-major input.
+constexpr int kFtMaxC = 13;  // (py, pz) combinations of one pass
+constexpr int kFtMaxK = 2;   // nonzero |pz| values of one pass
+struct FtPass {
+  int nc, nk;
+  // Per combination, which z transform it reads: 0 = pz 0, 1 + 2s = +k_s,
+  // 2 + 2s = -k_s, with k_s the s-th nonzero |pz| of the pass.
+  int tsel[kFtMaxC];
+};
+// ey: nc x Ly phases exp(-i 2 pi py y / Ly); zcs: nk x Lz pairs
+// (cos, sin)(2 pi k_s z / Lz).  U: [t][x][combination][tile pair][256].
+void launch_vdv_ft(const double2 *Vr, long long vstride, int Npad, int Lx, int Ly, int Lz, const int2 *tiles,
+                   int ntp, int nt, FtPass const &pass, const double2 *ey, const double2 *zcs, double2 *U,
+                   hipStream_t s);
+// One output of the x sum: combination c of the pass, phases ex[k * Lx + x]
+// = exp(-i 2 pi q x / Lx); lower == 0 stores C[t][j][i] (column-major
+// element (i, j)) for every tile pair, lower == 1 stores conj at (j, i) for
+// tile pairs I < J.  C points at timeslice 0; U holds timeslices t0 .. t0 + nt;
+// tstride = N * N.
+struct FtTarget {
+  double2 *C;
+  int c, lower;
+};
+void launch_vdv_xsum(const double2 *U, int nc, int ntp, int nt, int Lx, const int2 *tiles, const FtTarget *targets,
+                     int ntarget, const double2 *ex, int N, int t0, long long tstride, hipStream_t s);
+// Vr[t][k * Npad + n] = V[t][n * K + k] (zero for n >= N), nt timeslices.
+void launch_vrow(const double2 *V, double2 *Vr, long long K, int N, int Npad, int nt, hipStream_t s);
+
 // Q0 of one (t, operator, random pair): D x D.
 struct Q0Job {
   const double2 *vdv;   // N x N (daggered if needed)

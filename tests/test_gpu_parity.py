@@ -161,11 +161,12 @@ def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypat
     seeded host inputs: every correlator against the oracle at 1e-12
     norm-wise.  The wide variant (C4cB / C4cC up to |P|^2 = 1) runs the
     non-separable traces and the D = 16 products through both kernels each
-    (left-stationary / one wave per group; MFMA / generic product), and the
-    masked inner products both by Moebius inversion and by direct group loops."""
+    (left-stationary / one wave per group; MFMA / generic product), the
+    masked inner products both by Moebius inversion and by direct group loops,
+    and VdaggerV both by the sitewise Fourier form and directly."""
     say("small pipi", "wide" if infile is WIDE else "", kernels)
     for var, on in (("SLAPH_TRACE_GROUPS", "groups" in kernels), ("SLAPH_PROD_GENERIC", "generic" in kernels),
-                    ("SLAPH_RESOLVE_GROUPS", "groups" in kernels)):
+                    ("SLAPH_RESOLVE_GROUPS", "groups" in kernels), ("SLAPH_VDV_DIRECT", "groups" in kernels)):
         if on:
             monkeypatch.setenv(var, "1")
         else:
@@ -210,6 +211,52 @@ def _rel_errors(got, want):
         for name, v in vals.items():
             out.append(np.max(np.abs(got[d][name] - v)) / max(np.max(np.abs(v)), 1e-300))
     return np.array(out)
+
+
+VDV_ODD = """
+Lt = 2
+Lx = 4
+Ly = 6
+Lz = 5
+number_of_eigen_vec = 37
+momentum_cutoff_0 = 12
+[quarks]
+quark = u:4:TB:2:EI:4:DF:4:synthetic
+[operator_lists]
+operator_list = g5.d0.p0,1,2,3,4,5,6
+[correlator_lists]
+correlator_list = C2c:Q0:Op0:Q0:Op0:P0,1,2,3,4
+correlator_list = C4cD:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0
+"""
+
+
+@pytest.mark.parametrize("direct", [False, True], ids=["fourier", "direct"])
+def test_vdaggerv_odd_lattice_matches_oracle(slaph, direct, monkeypatch):
+    """VdaggerV on a 4 x 6 x 5 lattice with N = 37 (three 16-column tiles, the
+    last one ragged) and all momenta up to |p|^2 = 6 (components up to 2, so
+    the Fourier form needs several passes over (py, pz) combinations with
+    two nonzero |pz|): every operator at every timeslice against the oracle
+    at 1e-13 relative to the largest entry."""
+    say("vdaggerv odd lattice", "direct" if direct else "fourier")
+    if direct:
+        monkeypatch.setenv("SLAPH_VDV_DIRECT", "1")
+    else:
+        monkeypatch.delenv("SLAPH_VDV_DIRECT", raising=False)
+    cfg = o.parse_infile(VDV_ODD)
+    lut = o.init_lookup_tables(cfg)
+    ctx = slaph.Context(0)
+    ctx.configure(VDV_ODD)
+    ctx.synthetic_inputs(21, False)
+    ev = [ctx.get_input("eigenvectors", t) for t in range(cfg["Lt"])]
+    ops = slaph.OperatorFactory(ctx)
+    ops.create_operators()
+    want = o.build_vdaggerv(cfg, lut, ev)
+    assert len(lut["vdv_lookup"]) > 20
+    for vid, mom, _ in lut["vdv_lookup"]:
+        for t in range(cfg["Lt"]):
+            w = want[(vid, t)]
+            got = ops.return_vdaggerv(vid, t)
+            assert np.max(np.abs(got - w)) <= 1e-13 * np.max(np.abs(w)), (vid, tuple(mom), t)
 
 
 def test_precision_forms_against_fp64(slaph):
