@@ -153,8 +153,8 @@ def main():
     ap.add_argument("--chunk-gib", type=float, default=0.0, help="chunk memory budget (0: automatic)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the work would shard)", file=sys.stderr)
+        print("bench.py runs on exactly one GPU: there is no multi-GPU path "
+              "(block pairs of slice pairs would be split across GPUs)", file=sys.stderr)
         return 2
     import slaph
     w = WORKLOADS[args.workload]

@@ -257,8 +257,12 @@ struct Family {
   std::vector<LsBlock> ls_blocks;
   std::vector<LsChunk> ls_chunks;
   std::vector<LsTask> ls_tasks;
-  std::vector<long long> ls_slot_dst;
-  DevBuf d_lsb, d_lsc, d_lst, d_lsd;
+  std::vector<long long> ls_slot_dst;  // slot -> partial index
+  std::vector<int> ls_left;            // staged left matrices (offset from the time base)
+  std::vector<GroupItem> ls_red;       // per item: its partial sums in ls_ridx
+  std::vector<int> ls_ridx;
+  long long ls_npart = 0;  // partial sums per slice pair
+  DevBuf d_lsb, d_lsc, d_lst, d_lsd, d_lsl, d_lsr, d_lsri;
 };
 
 struct CorrPlan {
@@ -330,29 +334,77 @@ void order_items_by_component(Family &f, NodePlan const &root) {
   f.gitems = std::move(sorted);
 }
 
-// Left-stationary plan of a family's two-operand trace (time independent):
-// group sums (items) are blocked by their left operator entry, at most
-// kLsSlots per block; a block's left combinations are staged in chunks of
-// trace_lstat_chunk(D); inside a chunk each item's (i, j) pairs are sorted by
-// the right combination j, consecutive pairs with the same j share one task,
-// and the chunk's task list is cut into kLsWaves ranges of about equal length.
-// Each wave flushes its per-lane sum at the end of every run and of its range
-// into its own slot copy, so the order of every sum is fixed by the plan.
-// Blocks are ordered by connected component of the (left entry, right entry)
-// graph of the traces, so that blocks reading the same right matrices are
-// neighbours in launch order (and, through the XCD-aware block mapping of the
-// kernel, run against the same L2).
+// Left-stationary plan of a family's two-operand trace (time independent).
+// Left combinations are cut into pieces of at most trace_lstat_chunk(D); a
+// unit is (piece, left operator entry, up to kLsSlots of its items) and owns
+// one partial sum per item.  Units are packed into blocks of up to
+// kLsBlockChunks chunks (one LDS staging each); left entries run in order of
+// the connected components of the (left entry, right entry) trace graph.
+// Entry-major order (default): a block holds the pieces of one entry.
+// Class-major order (SLAPH_LSTAT_CLASSES=1): the (left combination, right
+// combination) pairs of all traces split into classes, the connected
+// components of the pair graph (one node space when both operands read the
+// same table; for a four-quark trace with four distinct random vectors a
+// class is one unordered pair of end vectors), and all entries' work in one
+// class runs before the next class.  Inside a chunk each slot's (i, j) pairs
+// are sorted by j, consecutive pairs with the same j share one task, and the
+// task list is cut into kLsWaves ranges of about equal length; each wave
+// flushes its per-lane sum at the end of every run and of its range into its
+// own slot copy.  ls_red lists each item's partial sums in piece order, so
+// the order of every sum is fixed by the plan.
+constexpr int kLsBlockChunks = 6;  // chunks (LDS stagings) per block
+
 void plan_lstat(Family &f, NodePlan const &root, int D) {
   f.ls_blocks.clear();
   f.ls_chunks.clear();
   f.ls_tasks.clear();
   f.ls_slot_dst.clear();
+  f.ls_left.clear();
+  f.ls_red.clear();
+  f.ls_ridx.clear();
+  f.ls_npart = 0;
   const int CL = trace_lstat_chunk(D);
   if (f.shape.nodes.back().kind != kTrace2 || CL <= 0 || f.gitems.empty()) return;
+  std::function<int(std::vector<int> &, int)> find = [&](std::vector<int> &par, int x) {
+    return par[x] == x ? x : par[x] = find(par, par[x]);
+  };
+  // (1) classes of combinations
+  std::map<std::pair<int, int>, int> cnode_id;
+  std::vector<int> cpar;
+  auto cnode = [&](int side, int c) {
+    auto it = cnode_id.emplace(std::make_pair(side, c), int(cpar.size())).first;
+    if (it->second == int(cpar.size())) cpar.push_back(it->second);
+    return it->second;
+  };
+  for (auto const &it : f.gitems)
+    for (int c = 0; c < it.n; ++c) {
+      int2 p = f.gpairs[it.list_off + c];
+      int a = find(cpar, cnode(0, p.x)), b = find(cpar, cnode(f.coincide ? 0 : 1, p.y));
+      if (a != b) cpar[std::max(a, b)] = std::min(a, b);
+    }
+  std::map<int, std::vector<int>> members;  // class root -> left combinations (ascending)
+  // One class of all combinations (contiguous pieces, entry-major blocks)
+  // unless SLAPH_LSTAT_CLASSES=1 selects the class-major plan: it reads about
+  // half the bytes from HBM but runs shorter chunks, and is slower here.
+  const bool noclass = std::getenv("SLAPH_LSTAT_CLASSES") == nullptr;
+  for (auto const &kv : cnode_id)
+    if (kv.first.first == 0) members[noclass ? 0 : find(cpar, kv.second)].push_back(kv.first.second);
+  std::vector<std::vector<int>> pieces;  // at most CL left combinations of one class
+  for (auto &kv : members) std::sort(kv.second.begin(), kv.second.end());
+  {
+    std::vector<std::vector<int>> cls;
+    for (auto const &kv : members) cls.push_back(kv.second);
+    std::sort(cls.begin(), cls.end());
+    for (auto const &c : cls)
+      for (size_t a = 0; a < c.size(); a += CL)
+        pieces.emplace_back(c.begin() + a, c.begin() + std::min(c.size(), a + CL));
+  }
+  std::map<int, int> piece_of;
+  for (size_t pc = 0; pc < pieces.size(); ++pc)
+    for (int i : pieces[pc]) piece_of[i] = int(pc);
+  // (2) left entries in trace-graph component order
   std::map<long long, std::vector<int>> by_left;
   for (size_t k = 0; k < f.gitems.size(); ++k) by_left[root.h_Xq.at(f.gitems[k].qn)].push_back(int(k));
-  // components: left and right entries are nodes (one node space when both
-  // operands read the same table), every trace an edge
   std::map<std::pair<int, long long>, int> node_id;
   std::vector<int> parent;
   auto node = [&](int side, long long off) {
@@ -360,89 +412,153 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
     if (it->second == int(parent.size())) parent.push_back(it->second);
     return it->second;
   };
-  std::function<int(int)> find = [&](int x) { return parent[x] == x ? x : parent[x] = find(parent[x]); };
   for (auto const &it : f.gitems) {
-    int a = find(node(0, root.h_Xq.at(it.qn))), b = find(node(f.coincide ? 0 : 1, root.h_Yq.at(it.qn)));
+    int a = find(parent, node(0, root.h_Xq.at(it.qn))), b = find(parent, node(f.coincide ? 0 : 1, root.h_Yq.at(it.qn)));
     if (a != b) parent[std::max(a, b)] = std::min(a, b);
   }
   std::vector<std::pair<int, long long>> order;  // (component, left entry)
-  for (auto const &kv : by_left) order.push_back({find(node(0, kv.first)), kv.first});
+  for (auto const &kv : by_left) order.push_back({find(parent, node(0, kv.first)), kv.first});
   std::stable_sort(order.begin(), order.end(),
                    [](std::pair<int, long long> const &x, std::pair<int, long long> const &y) { return x.first < y.first; });
-  for (auto const &ord : order)
-    for (size_t s0 = 0, nk = by_left[ord.second].size(); s0 < nk; s0 += kLsSlots) {
-      auto const &kv = *by_left.find(ord.second);
-      const size_t s1 = std::min(kv.second.size(), s0 + kLsSlots);
-      LsBlock b;
-      b.xq = kv.first;
-      b.chunk_off = int(f.ls_chunks.size());
-      b.slot_off = int(f.ls_slot_dst.size());
-      b.nslot = int(s1 - s0);
-      int imin = INT32_MAX, imax = -1;
-      for (size_t s = s0; s < s1; ++s) {
-        GroupItem const &it = f.gitems[kv.second[s]];
-        f.ls_slot_dst.push_back(it.dst);
-        for (int c = 0; c < it.n; ++c) {
-          imin = std::min(imin, f.gpairs[it.list_off + c].x);
-          imax = std::max(imax, f.gpairs[it.list_off + c].x);
-        }
-      }
-      for (int i0 = imin; i0 <= imax; i0 += CL) {
-        const int n = std::min(CL, imax + 1 - i0);
-        std::vector<LsTask> tasks;
-        for (size_t s = s0; s < s1; ++s) {
-          GroupItem const &it = f.gitems[kv.second[s]];
-          std::vector<std::pair<int, int>> pr;  // (j, i - i0)
-          for (int c = 0; c < it.n; ++c) {
-            int2 p = f.gpairs[it.list_off + c];
-            if (p.x >= i0 && p.x < i0 + n) pr.push_back({p.y, p.x - i0});
-          }
-          if (pr.empty()) continue;
-          std::sort(pr.begin(), pr.end());
-          const long long yq = root.h_Yq.at(it.qn);
-          for (size_t a = 0; a < pr.size();) {
-            LsTask t;
-            long long yrel = yq + pr[a].first;
-            if (yrel > INT32_MAX) throw std::runtime_error("trace plan: right operand index overflows");
-            t.yrel = int(yrel);
-            t.ia = short(pr[a].second);
-            t.ib = -1;
-            if (a + 1 < pr.size() && pr[a + 1].first == pr[a].first) {
-              t.ib = short(pr[a + 1].second);
-              a += 2;
-            } else {
-              a += 1;
-            }
-            t.slot = a == pr.size() ? int(s - s0) : -1;
-            tasks.push_back(t);
-          }
-        }
-        if (tasks.empty()) continue;
-        LsChunk ch;
-        ch.i0 = i0;
-        ch.n = n;
-        const int base = int(f.ls_tasks.size()), nt = int(tasks.size());
-        for (int w = 0; w <= kLsWaves; ++w) ch.wtask[w] = base + int((long long)nt * w / kLsWaves);
-        // a wave's range ends with a flush into the slot of its last task
-        for (int w = 0; w < kLsWaves; ++w) {
-          int last = ch.wtask[w + 1] - 1 - base;
-          if (last < ch.wtask[w] - base) continue;
-          if (tasks[last].slot < 0) {
-            int k = last;
-            while (tasks[k].slot < 0) ++k;  // the run's own slot, found at its end
-            tasks[last].slot = tasks[k].slot;
-          }
-        }
-        f.ls_tasks.insert(f.ls_tasks.end(), tasks.begin(), tasks.end());
-        f.ls_chunks.push_back(ch);
-      }
-      b.nchunk = int(f.ls_chunks.size()) - b.chunk_off;
-      f.ls_blocks.push_back(b);
+  // (3) blocks: per piece, consecutive left entries packed into one chunk
+  std::vector<std::vector<int>> item_parts(f.gitems.size());
+  struct CurChunk {
+    std::vector<int> left;
+    std::vector<LsTask> tasks;
+  };
+  struct Cur {
+    std::vector<CurChunk> ch;
+    int nslot = 0;
+  } cur;
+  auto flush = [&]() {
+    if (cur.ch.empty()) {
+      cur = Cur();
+      return;
     }
+    LsBlock b;
+    b.chunk_off = int(f.ls_chunks.size());
+    b.nchunk = int(cur.ch.size());
+    b.slot_off = int(f.ls_slot_dst.size()) - cur.nslot;
+    b.nslot = cur.nslot;
+    for (auto &cc : cur.ch) {
+      LsChunk ch;
+      ch.loff = int(f.ls_left.size());
+      ch.n = int(cc.left.size());
+      f.ls_left.insert(f.ls_left.end(), cc.left.begin(), cc.left.end());
+      const int base = int(f.ls_tasks.size()), nt = int(cc.tasks.size());
+      for (int w = 0; w <= kLsWaves; ++w) ch.wtask[w] = base + int((long long)nt * w / kLsWaves);
+      // a wave's range ends with a flush into the slot of its last task
+      for (int w = 0; w < kLsWaves; ++w) {
+        int last = ch.wtask[w + 1] - 1 - base;
+        if (last < ch.wtask[w] - base) continue;
+        if (cc.tasks[last].slot < 0) {
+          int k = last;
+          while (cc.tasks[k].slot < 0) ++k;  // the run's own slot, found at its end
+          cc.tasks[last].slot = cc.tasks[k].slot;
+        }
+      }
+      f.ls_tasks.insert(f.ls_tasks.end(), cc.tasks.begin(), cc.tasks.end());
+      f.ls_chunks.push_back(ch);
+    }
+    f.ls_blocks.push_back(b);
+    cur = Cur();
+  };
+  struct Unit {
+    int pc;
+    long long xq;
+    size_t s0;
+  };
+  std::vector<Unit> units;
+  const bool entry_major = noclass;
+  if (entry_major) {
+    for (auto const &ord : order)
+      for (size_t pc = 0; pc < pieces.size(); ++pc)
+        for (size_t s0 = 0, nk = by_left[ord.second].size(); s0 < nk; s0 += kLsSlots)
+          units.push_back({int(pc), ord.second, s0});
+  } else {
+    for (size_t pc = 0; pc < pieces.size(); ++pc)
+      for (auto const &ord : order)
+        for (size_t s0 = 0, nk = by_left[ord.second].size(); s0 < nk; s0 += kLsSlots)
+          units.push_back({int(pc), ord.second, s0});
+  }
+  for (size_t u = 0; u < units.size(); ++u) {
+    const int pc = units[u].pc;
+    if (u > 0 && (entry_major ? units[u - 1].xq != units[u].xq : units[u - 1].pc != pc)) flush();
+    const size_t s0 = units[u].s0, nk = by_left[units[u].xq].size();
+    const long long xq = units[u].xq;
+    auto const &all = by_left[xq];
+    const std::vector<int> items(all.begin() + s0, all.begin() + std::min(nk, s0 + kLsSlots));
+    // this entry's pairs in the piece, per item
+    std::vector<std::vector<std::pair<int, int>>> prs(items.size());  // (j, i)
+    std::vector<int> used;
+    int nslot = 0;
+    for (size_t s = 0; s < items.size(); ++s) {
+      GroupItem const &it = f.gitems[items[s]];
+      for (int c = 0; c < it.n; ++c) {
+        int2 p = f.gpairs[it.list_off + c];
+        if (piece_of.at(p.x) != int(pc)) continue;
+        prs[s].push_back({p.y, p.x});
+        used.push_back(p.x);
+      }
+      if (!prs[s].empty()) ++nslot;
+    }
+    if (nslot == 0) continue;
+    std::sort(used.begin(), used.end());
+    used.erase(std::unique(used.begin(), used.end()), used.end());
+    if (cur.nslot + nslot > kLsSlots) flush();
+    if (cur.ch.empty() || int(cur.ch.back().left.size() + used.size()) > CL) {
+      if (int(cur.ch.size()) == kLsBlockChunks) flush();
+      cur.ch.emplace_back();
+    }
+    CurChunk &cc = cur.ch.back();
+    std::map<int, int> pos;
+    for (int i : used) {
+      long long off = xq + i;
+      if (off > INT32_MAX) throw std::runtime_error("trace plan: left operand index overflows");
+      pos[i] = int(cc.left.size());
+      cc.left.push_back(int(off));
+    }
+    for (size_t s = 0; s < items.size(); ++s) {
+      auto &pr = prs[s];
+      if (pr.empty()) continue;
+      std::sort(pr.begin(), pr.end());
+      const int slot = cur.nslot++;
+      item_parts[items[s]].push_back(int(f.ls_npart));
+      f.ls_slot_dst.push_back(f.ls_npart++);
+      const long long yq = root.h_Yq.at(f.gitems[items[s]].qn);
+      for (size_t a = 0; a < pr.size();) {
+        LsTask t;
+        long long yrel = yq + pr[a].first;
+        if (yrel > INT32_MAX) throw std::runtime_error("trace plan: right operand index overflows");
+        t.yrel = int(yrel);
+        t.ia = short(pos.at(pr[a].second));
+        t.ib = -1;
+        if (a + 1 < pr.size() && pr[a + 1].first == pr[a].first) {
+          t.ib = short(pos.at(pr[a + 1].second));
+          a += 2;
+        } else {
+          a += 1;
+        }
+        t.slot = a == pr.size() ? slot : -1;
+        cc.tasks.push_back(t);
+      }
+    }
+  }
+  flush();
+  // (4) per item: its partial sums, in piece order
+  for (size_t k = 0; k < f.gitems.size(); ++k) {
+    GroupItem r{0, int(f.ls_ridx.size()), int(item_parts[k].size()), f.gitems[k].dst};
+    f.ls_ridx.insert(f.ls_ridx.end(), item_parts[k].begin(), item_parts[k].end());
+    f.ls_red.push_back(r);
+  }
+  if (f.ls_npart > INT32_MAX) throw std::runtime_error("trace plan: too many partial sums");
   upload(f.d_lsb, f.ls_blocks);
   upload(f.d_lsc, f.ls_chunks);
   upload(f.d_lst, f.ls_tasks);
   upload(f.d_lsd, f.ls_slot_dst);
+  upload(f.d_lsl, f.ls_left);
+  upload(f.d_lsr, f.ls_red);
+  upload(f.d_lsri, f.ls_ridx);
 }
 
 // Every (factor type, time key) a leaf of this family reads at slice pair
@@ -471,6 +587,8 @@ struct ChunkPlan {
   std::vector<long long> gbase;               // per family: offset of its group sums
   std::vector<long long> zbase;               // per family: offset of its subset sums
   long long z_total = 0;
+  std::vector<long long> pbase;               // per family: offset of its partial trace sums
+  long long p_total = 0;
   std::vector<std::vector<NodeChunk>> nodes;  // per family, per node
   DevBuf d_td;                                // time delta per slice pair
   std::vector<DevBuf> d_jobs;                 // per diagram: resolve jobs
@@ -490,7 +608,7 @@ struct ContractCache {
   std::vector<uint16_t> zmask;  // subset list of the Moebius form (empty: not used)
   DevBuf d_zmask;
   std::vector<ChunkPlan> chunks;
-  DevBuf d_prod, d_gv, d_cval, d_zv, d_res;
+  DevBuf d_prod, d_gv, d_cval, d_zv, d_res, d_pv;
   std::vector<DevBuf> d_sum, d_comp;  // per diagram: Lt x n_corr accumulators
 };
 
@@ -954,14 +1072,16 @@ void Engine::contract(int single_block_pair, int kahan) {
   }
 
   const long long DD = (long long)D_ * D_;
-  long long gstride_all = 0;
-  for (auto const &f : fams) gstride_all += f.gstride;
-  size_t max_corr = 1;
-  for (auto const &c : corrs) max_corr = std::max(max_corr, c.reqs.size());
-  const bool progress = std::getenv("SLAPH_PROGRESS") != nullptr;
   // SLAPH_TRACE_GROUPS=1 selects the one-wave-per-group trace kernel for every
   // trace (a cross-check of the tiled and left-stationary kernels).
   const bool lstat_off = std::getenv("SLAPH_TRACE_GROUPS") != nullptr;
+  long long gstride_all = 0, npart_all = 0;
+  for (auto const &f : fams) gstride_all += f.gstride;
+  if (!lstat_off)
+    for (auto const &f : fams) npart_all += f.ls_npart;
+  size_t max_corr = 1;
+  for (auto const &c : corrs) max_corr = std::max(max_corr, c.reqs.size());
+  const bool progress = std::getenv("SLAPH_PROGRESS") != nullptr;
   auto slab_count = [&](Family const &f, int child, long long prod_total) -> long long {
     auto const &cs = f.shape.nodes[child];
     if (cs.kind != kLeaf) return prod_total;
@@ -1096,7 +1216,7 @@ void Engine::contract(int single_block_pair, int kahan) {
           S_ = std::min(S_, sl.size() - c0);
           S = int(S_);
           prod_total = plan_chunk(S);
-          double bytes = sizeof(double2) * (double(prod_total) * DD + double(S) * (gstride_all + max_corr));
+          double bytes = sizeof(double2) * (double(prod_total) * DD + double(S) * (gstride_all + npart_all + max_corr));
           if (bytes <= double(budget) || chunk_rows == 1) break;
           chunk_rows = std::max<size_t>(1, chunk_rows / 2);
         }
@@ -1114,6 +1234,11 @@ void Engine::contract(int single_block_pair, int kahan) {
         for (size_t fi = 0; fi < fams.size(); ++fi) {
           ch.zbase[fi] = ch.z_total;
           ch.z_total += (long long)S * fams[fi].zstride;
+        }
+        ch.pbase.resize(fams.size());
+        for (size_t fi = 0; fi < fams.size(); ++fi) {
+          ch.pbase[fi] = ch.p_total;
+          if (!lstat_off) ch.p_total += (long long)S * fams[fi].ls_npart;
         }
         ch.nodes.resize(fams.size());
         for (size_t fi = 0; fi < fams.size(); ++fi) {
@@ -1216,9 +1341,11 @@ void Engine::contract(int single_block_pair, int kahan) {
     C.d_prod.ensure(elem * np_max);
     C.d_gv.ensure(sizeof(double2) * ng_max);
     C.d_cval.ensure(sizeof(double2) * nc_max);
-    long long nz_max = 1;
+    long long nz_max = 1, npv_max = 1;
     for (auto const &ch : C.chunks) nz_max = std::max(nz_max, ch.z_total);
+    for (auto const &ch : C.chunks) npv_max = std::max(npv_max, ch.p_total);
     C.d_zv.ensure(sizeof(double2) * nz_max);
+    C.d_pv.ensure(sizeof(double2) * npv_max);
   }
 
   // ---- launches: per chunk, products in topological order, then traces,
@@ -1265,10 +1392,15 @@ void Engine::contract(int single_block_pair, int kahan) {
                             f.d_gitems.as<GroupItem>(), int(f.gitems.size()), f.d_gpairs.as<int2>(),
                             nc.d_tile.as<GridTile>(), 1, C.d_gv.as<double2>(), gb, f.gstride, D_, rt, f32, stream_);
         } else if (ns.kind == kTrace2 && !f.ls_blocks.empty() && !lstat_off) {
+          if (ch.pbase[fi] + (long long)nc.ntime * f.ls_npart > ch.p_total)
+            throw std::logic_error("trace plan indexes outside the partial-sum buffer");
           launch_trace_lstat(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
                              f.d_lsb.as<LsBlock>(), int(f.ls_blocks.size()), f.d_lsc.as<LsChunk>(),
-                             f.d_lst.as<LsTask>(), f.d_lsd.as<long long>(), C.d_gv.as<double2>(), gb, f.gstride,
-                             nc.ntime, D_, rt, f32, stream_);
+                             f.d_lsl.as<int>(), f.d_lst.as<LsTask>(), f.d_lsd.as<long long>(), C.d_pv.as<double2>(),
+                             ch.pbase[fi], f.ls_npart, nc.ntime, D_, rt, f32, stream_);
+          launch_ls_reduce(C.d_pv.as<double2>() + ch.pbase[fi], f.ls_npart, f.d_lsr.as<GroupItem>(),
+                           int(f.ls_red.size()), f.d_lsri.as<int>(), C.d_gv.as<double2>(), gb, f.gstride, nc.ntime,
+                           stream_);
         } else {
           launch_trace_groups(src(ns.left), nc.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
                               nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), f.d_gitems.as<GroupItem>(),

@@ -48,9 +48,29 @@ __device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b) {
 __constant__ int c_gamma_row[16][4];
 __constant__ double2 c_gamma_val[16][4];
 
+// Lane-permuted copy of a double through DPP (two 32-bit moves); lanes that
+// the row mask leaves out read 0.
+template <int kCtrl, int kRowMask = 0xf>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, kRowMask, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, kRowMask, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes, returned in every lane: pairs, quads (quad_perm),
+// 8 and 16 lanes (row half-mirror, row mirror), then rows 0+1 and 2+3
+// (row_bcast:15) and all four rows (row_bcast:31) into lane 63.  DPP moves
+// are vector-ALU operations, so the chain has no LDS round trips.
 __device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_d<0xB1>(v);        // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E>(v);        // quad_perm [2,3,0,1]
+  v += dpp_d<0x141>(v);       // row_half_mirror
+  v += dpp_d<0x140>(v);       // row_mirror
+  v += dpp_d<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_d<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
 }
 
 // ---------------------------------------------------------------------------
@@ -779,24 +799,29 @@ __global__ __launch_bounds__(256) void k_prod16(const T *__restrict__ sX, const 
 // ---------------------------------------------------------------------------
 // Non-separable traces tr(X Y) = sum_{a,b} X[a,b] Y[b,a] (reference:
 // factor_to_trace(L, R), DilutedTrace.cpp:84-138), left-stationary: one
-// workgroup per (time, block of slots sharing one left operator entry).  The
-// block's left matrices are staged chunk by chunk in LDS, transposed, so that
-// element e of a right matrix meets its partner at the same index.  Each
-// wavefront streams the right matrices of its tasks into registers once
-// (EPL = D*D/64 elements per lane; up to PF tasks ahead are in flight, since
-// the kernel is bound by the latency of these loads) and multiplies them with
-// one or two staged left matrices.  At the end of a run (or of the wave's task range) the wave sum
-// is added to the wave's own copy of the run's slot; the copies are summed in
-// wave order at the end, so the summation order is fixed by the plan.
+// workgroup per (time, block of slots).  The block's left matrices (any list
+// of operator entries and combinations) are staged chunk by chunk in LDS,
+// transposed, so that element e of a right matrix meets its partner at the
+// same index.  Each wavefront streams the right matrices of its tasks into
+// registers once (EPL = D*D/64 elements per lane; up to PF tasks ahead are in
+// flight, since the kernel is bound by the latency of these loads) and
+// multiplies them with one or two staged left matrices.  At the end of a run
+// (or of the wave's task range) the wave sum is added to the wave's own copy
+// of the run's slot; the copies are summed in wave order at the end, so the
+// summation order is fixed by the plan.
 // ---------------------------------------------------------------------------
 constexpr int kLsStageBytes = 48 * 1024;
+// LDS slot of element e of a staged matrix: the low four bits of the 16-byte
+// slot index are XORed with the next four, so that both the lane-contiguous
+// reads and the transposing writes (stride D slots) hit distinct banks.
+__device__ __forceinline__ int ls_swz(int e) { return e ^ ((e >> 4) & 15); }
 
 template <int D, typename TS>
-__global__ __launch_bounds__(64 * kLsWaves) void k_trace_lstat(
+__global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3))) void k_trace_lstat(
     const TS *__restrict__ sX, const long long *__restrict__ Xt, const TS *__restrict__ sY,
     const long long *__restrict__ Yt, const LsBlock *__restrict__ blocks, int nblock, const LsChunk *__restrict__ chunks,
-    const LsTask *__restrict__ tasks, const long long *__restrict__ slot_dst, double2 *__restrict__ out,
-    long long obase, long long ostride, int rt) {
+    const int *__restrict__ left, const LsTask *__restrict__ tasks, const long long *__restrict__ slot_dst,
+    double2 *__restrict__ out, long long obase, long long ostride, int rt) {
   constexpr int DD = D * D, EPL = DD / 64, CL = kLsStageBytes / (DD * 16);
   constexpr int PF = EPL <= 4 ? 4 : (EPL <= 8 ? 2 : 1);  // right matrices in flight per wave
   __shared__ double2 Ls[CL * DD];
@@ -810,68 +835,115 @@ __global__ __launch_bounds__(64 * kLsWaves) void k_trace_lstat(
   const LsBlock B = blocks[logical % nblock];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int k = tid; k < kLsWaves * kLsSlots; k += 64 * kLsWaves) (&acc[0][0])[k] = make_double2(0.0, 0.0);
-  const TS *xb = sX + (Xt[ti] + B.xq) * DD;
+  const TS *xb = sX + Xt[ti] * DD;
   const TS *yb = sY + Yt[ti] * DD;
-  for (int c = 0; c < B.nchunk; ++c) {
+  // The next chunk's left matrices are loaded into registers while the
+  // current chunk's tasks run, and written to LDS between the two.
+  constexpr int SPT = (CL * DD + 64 * kLsWaves - 1) / (64 * kLsWaves);
+  TS pre[SPT];
+  auto fetch = [&](int c) {
     const LsChunk C = chunks[B.chunk_off + c];
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+      const int idx = tid + 64 * kLsWaves * k;
+      // D*D is a multiple of 64, so the staged matrix m is uniform in a wave
+      // and its offset comes through the scalar cache.
+      const int m = __builtin_amdgcn_readfirstlane(idx / DD);
+      if (m < C.n) pre[k] = xb[(long long)left[C.loff + m] * DD + idx % DD];
+    }
+  };
+  auto commit = [&](int c) {
+    const int n = chunks[B.chunk_off + c].n;
     __syncthreads();  // the previous chunk is no longer read
-    for (int idx = tid; idx < C.n * DD; idx += 64 * kLsWaves) {
-      const int m = idx / DD, f = idx % DD;
-      Ls[m * DD + (rt ? f : f / D + (f % D) * D)] = ld2(xb + (long long)(C.i0 + m) * DD + f);
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) {
+      const int idx = tid + 64 * kLsWaves * k, m = idx / DD, f = idx % DD;
+      if (idx < n * DD) Ls[m * DD + ls_swz(rt ? f : f / D + (f % D) * D)] = ld2(&pre[k]);
     }
     __syncthreads();
+  };
+  int so[EPL];  // swizzled LDS slots of this lane's elements lane + 64 k
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) so[k] = ls_swz(lane + 64 * k);
+  if (B.nchunk > 0) fetch(0);
+  for (int c = 0; c < B.nchunk; ++c) {
+    const LsChunk C = chunks[B.chunk_off + c];
+    commit(c);
     const int t0 = C.wtask[wave], t1 = C.wtask[wave + 1];
-    if (t0 >= t1) continue;
     double re = 0.0, im = 0.0;
-    // ring of PF tasks in flight (slot u always holds task t + u of the
-    // current round; the loop over u is unrolled, so the ring indexes are
-    // compile-time constants)
+    // The wave's task descriptors, up to 64 at a time, one per lane; the task
+    // loop reads them with readlane, so no memory round trip sits between a
+    // task and the loads of the task PF positions ahead.  Ring of PF tasks in
+    // flight: slot u holds task t + u of the current round (the loop over u
+    // is unrolled, so the ring indexes are compile-time constants).  The
+    // first batch's descriptors and ring loads are issued before the next
+    // chunk's left matrices, so that its first tasks do not wait for them.
+    int qy = 0, qab = 0, qs = -1;
     double2 r[PF][EPL];
-    LsTask tk[PF];
-#pragma unroll
-    for (int u = 0; u < PF; ++u)
-      if (t0 + u < t1) {
-        tk[u] = tasks[t0 + u];
-#pragma unroll
-        for (int k = 0; k < EPL; ++k) r[u][k] = ld2(yb + (long long)tk[u].yrel * DD + lane + 64 * k);
+    auto batch = [&](int tb) {
+      const int nt = min(64, t1 - tb);
+      qy = 0;
+      qab = 0;
+      qs = -1;
+      if (lane < nt) {
+        const LsTask q = tasks[tb + lane];
+        qy = q.yrel;
+        qab = int((unsigned short)q.ia) | (int(q.ib) << 16);
+        qs = q.slot;
       }
-    for (int t = t0; t < t1; t += PF) {
+      // ring loads past the last task repeat the last task's matrix, so that
+      // every load is unconditional and the waits are exact
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
-        if (t + u >= t1) break;
-        const LsTask T = tk[u];
-        const double2 *la = Ls + T.ia * DD + lane;
+        const long long y = __builtin_amdgcn_readlane(qy, min(u, nt - 1));
 #pragma unroll
-        for (int k = 0; k < EPL; ++k) {
-          const double2 a = la[64 * k], v = r[u][k];
-          re = fma(a.x, v.x, re);
-          re = fma(-a.y, v.y, re);
-          im = fma(a.x, v.y, im);
-          im = fma(a.y, v.x, im);
-        }
-        if (T.ib >= 0) {
-          const double2 *lb = Ls + T.ib * DD + lane;
+        for (int k = 0; k < EPL; ++k) r[u][k] = ld2(yb + y * DD + lane + 64 * k);
+      }
+    };
+    if (t0 < t1) batch(t0);
+    if (c + 1 < B.nchunk) fetch(c + 1);
+    for (int tb = t0; tb < t1; tb += 64) {
+      if (tb != t0) batch(tb);
+      const int nt = min(64, t1 - tb);
+      for (int t = 0; t < nt; t += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+          if (t + u < nt) {
+          const int ab = __builtin_amdgcn_readlane(qab, t + u);
+          const int ia = ab & 0xffff, ib = ab >> 16;
+          const int slot = __builtin_amdgcn_readlane(qs, t + u);
+          const double2 *la = Ls + ia * DD;
 #pragma unroll
           for (int k = 0; k < EPL; ++k) {
-            const double2 a = lb[64 * k], v = r[u][k];
+            const double2 a = la[so[k]], v = r[u][k];
             re = fma(a.x, v.x, re);
             re = fma(-a.y, v.y, re);
             im = fma(a.x, v.y, im);
             im = fma(a.y, v.x, im);
           }
-        }
-        if (T.slot >= 0) {
-          re = wave_sum(re);
-          im = wave_sum(im);
-          if (lane == 0) acc[wave][T.slot] = cadd(acc[wave][T.slot], make_double2(re, im));
-          re = 0.0;
-          im = 0.0;
-        }
-        // refill this ring slot with the task PF positions ahead
-        if (t + u + PF < t1) {
-          tk[u] = tasks[t + u + PF];
+          if (ib >= 0) {
+            const double2 *lb = Ls + ib * DD;
 #pragma unroll
-          for (int k = 0; k < EPL; ++k) r[u][k] = ld2(yb + (long long)tk[u].yrel * DD + lane + 64 * k);
+            for (int k = 0; k < EPL; ++k) {
+              const double2 a = lb[so[k]], v = r[u][k];
+              re = fma(a.x, v.x, re);
+              re = fma(-a.y, v.y, re);
+              im = fma(a.x, v.y, im);
+              im = fma(a.y, v.x, im);
+            }
+          }
+          if (slot >= 0) {
+            re = wave_sum(re);
+            im = wave_sum(im);
+            if (lane == 0) acc[wave][slot] = cadd(acc[wave][slot], make_double2(re, im));
+            re = 0.0;
+            im = 0.0;
+          }
+          }
+          // refill this ring slot with the task PF positions ahead
+          const long long y = __builtin_amdgcn_readlane(qy, min(t + u + PF, nt - 1));
+#pragma unroll
+          for (int k = 0; k < EPL; ++k) r[u][k] = ld2(yb + y * DD + lane + 64 * k);
         }
       }
     }
@@ -1344,27 +1416,27 @@ void launch_prod_t(const void *sX, const long long *Xt, const long long *Xq, con
 
 template <typename T>
 void launch_trace_lstat_t(const void *sX, const long long *Xt, const void *sY, const long long *Yt,
-                          const LsBlock *blocks, int nblock, const LsChunk *chunks, const LsTask *tasks,
-                          const long long *slot_dst, double2 *out, long long obase, long long ostride, int ntime, int D,
-                          int rt, hipStream_t s) {
+                          const LsBlock *blocks, int nblock, const LsChunk *chunks, const int *left,
+                          const LsTask *tasks, const long long *slot_dst, double2 *out, long long obase,
+                          long long ostride, int ntime, int D, int rt, hipStream_t s) {
   const T *x = static_cast<const T *>(sX), *y = static_cast<const T *>(sY);
   const dim3 grid(unsigned(nblock * ntime)), block(64 * kLsWaves);
   switch (D) {
     case 8:
-      hipLaunchKernelGGL((k_trace_lstat<8, T>), grid, block, 0, s, x, Xt, y, Yt, blocks, nblock, chunks, tasks,
-                         slot_dst, out, obase, ostride, rt);
+      hipLaunchKernelGGL((k_trace_lstat<8, T>), grid, block, 0, s, x, Xt, y, Yt, blocks, nblock, chunks, left,
+                         tasks, slot_dst, out, obase, ostride, rt);
       break;
     case 16:
-      hipLaunchKernelGGL((k_trace_lstat<16, T>), grid, block, 0, s, x, Xt, y, Yt, blocks, nblock, chunks, tasks,
-                         slot_dst, out, obase, ostride, rt);
+      hipLaunchKernelGGL((k_trace_lstat<16, T>), grid, block, 0, s, x, Xt, y, Yt, blocks, nblock, chunks, left,
+                         tasks, slot_dst, out, obase, ostride, rt);
       break;
     case 24:
-      hipLaunchKernelGGL((k_trace_lstat<24, T>), grid, block, 0, s, x, Xt, y, Yt, blocks, nblock, chunks, tasks,
-                         slot_dst, out, obase, ostride, rt);
+      hipLaunchKernelGGL((k_trace_lstat<24, T>), grid, block, 0, s, x, Xt, y, Yt, blocks, nblock, chunks, left,
+                         tasks, slot_dst, out, obase, ostride, rt);
       break;
     case 32:
-      hipLaunchKernelGGL((k_trace_lstat<32, T>), grid, block, 0, s, x, Xt, y, Yt, blocks, nblock, chunks, tasks,
-                         slot_dst, out, obase, ostride, rt);
+      hipLaunchKernelGGL((k_trace_lstat<32, T>), grid, block, 0, s, x, Xt, y, Yt, blocks, nblock, chunks, left,
+                         tasks, slot_dst, out, obase, ostride, rt);
       break;
     default:
       throw std::runtime_error("launch_trace_lstat: unsupported factor dimension");
@@ -1393,17 +1465,39 @@ int trace_lstat_chunk(int D) {
 }
 
 void launch_trace_lstat(const void *sX, const long long *Xt, const void *sY, const long long *Yt,
-                        const LsBlock *blocks, int nblock, const LsChunk *chunks, const LsTask *tasks,
-                        const long long *slot_dst, double2 *out, long long obase, long long ostride, int ntime, int D,
-                        int rt, int f32, hipStream_t s) {
+                        const LsBlock *blocks, int nblock, const LsChunk *chunks, const int *left,
+                        const LsTask *tasks, const long long *slot_dst, double2 *out, long long obase,
+                        long long ostride, int ntime, int D, int rt, int f32, hipStream_t s) {
   if (nblock <= 0 || ntime <= 0) return;
   if ((long long)nblock * ntime > 0x7fffffffLL) throw std::runtime_error("launch_trace_lstat: grid too large");
   if (f32)
-    launch_trace_lstat_t<float2>(sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst, out, obase, ostride, ntime,
-                                 D, rt, s);
+    launch_trace_lstat_t<float2>(sX, Xt, sY, Yt, blocks, nblock, chunks, left, tasks, slot_dst, out, obase, ostride,
+                                 ntime, D, rt, s);
   else
-    launch_trace_lstat_t<double2>(sX, Xt, sY, Yt, blocks, nblock, chunks, tasks, slot_dst, out, obase, ostride, ntime,
-                                  D, rt, s);
+    launch_trace_lstat_t<double2>(sX, Xt, sY, Yt, blocks, nblock, chunks, left, tasks, slot_dst, out, obase, ostride,
+                                  ntime, D, rt, s);
+  SLAPH_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void k_ls_reduce(const double2 *__restrict__ part, long long pstride,
+                                                   const GroupItem *__restrict__ items, int nitems,
+                                                   const int *__restrict__ pidx, double2 *__restrict__ out,
+                                                   long long obase, long long ostride) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x, ti = blockIdx.y;
+  if (k >= nitems) return;
+  const GroupItem it = items[k];
+  const double2 *p = part + (long long)ti * pstride;
+  double2 v = make_double2(0.0, 0.0);
+  for (int c = 0; c < it.n; ++c) v = cadd(v, p[pidx[it.list_off + c]]);
+  out[obase + (long long)ti * ostride + it.dst] = v;
+}
+
+void launch_ls_reduce(const double2 *part, long long pstride, const GroupItem *items, int nitems, const int *pidx,
+                      double2 *out, long long obase, long long ostride, int ntime, hipStream_t s) {
+  if (nitems <= 0 || ntime <= 0) return;
+  if (ntime > 65535) throw std::runtime_error("launch_ls_reduce: too many slice pairs in one launch");
+  hipLaunchKernelGGL(k_ls_reduce, dim3((nitems + 255) / 256, ntime), dim3(256), 0, s, part, pstride, items, nitems,
+                     pidx, out, obase, ostride);
   SLAPH_HIP(hipGetLastError());
 }
 

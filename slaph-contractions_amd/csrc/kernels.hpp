@@ -204,21 +204,19 @@ void launch_trace_grid(const void *slabX, const long long *Xq, const void *slabY
                        const GroupItem *items, int nitems, const int2 *pairs, const GridTile *tiles, int ntiles,
                        double2 *out, long long obase, long long ostride, int D, int rt, int f32, hipStream_t s);
 // Non-separable traces, left-stationary.  A block is a set of (trace id,
-// mask group) slots that share one left operator entry (left matrices
-// X[Xt[ti] + xq + i]); its chunks stage left combinations [i0, i0 + n) and
-// list tasks per wavefront.  A task multiplies right matrix Y[Yt[ti] + yrel]
-// with staged left combinations ia (and ib >= 0); slot >= 0 marks the last
-// task of a run, whose sum is added to that slot.  Output:
-// out[obase + ti*ostride + slot_dst[slot_off + k]] = slot k.
+// mask group) slots; its chunks stage left matrices X[Xt[ti] + left[loff + m]]
+// (m < n, any entries and combinations) and list tasks per wavefront.  A task
+// multiplies right matrix Y[Yt[ti] + yrel] with staged left matrices ia (and
+// ib >= 0); slot >= 0 marks the last task of a run, whose sum is added to
+// that slot.  Output: out[obase + ti*ostride + slot_dst[slot_off + k]] = slot k.
 constexpr int kLsWaves = 4;
 constexpr int kLsSlots = 64;
 struct LsBlock {
-  long long xq;
   int chunk_off, nchunk;
   int slot_off, nslot;
 };
 struct LsChunk {
-  int i0, n;
+  int loff, n;
   int wtask[kLsWaves + 1];  // task range of each wavefront
 };
 struct LsTask {
@@ -230,9 +228,13 @@ struct LsTask {
 int trace_lstat_chunk(int D);
 bool trace_lstat_supported(int D);
 void launch_trace_lstat(const void *slabX, const long long *Xt, const void *slabY, const long long *Yt,
-                        const LsBlock *blocks, int nblock, const LsChunk *chunks, const LsTask *tasks,
-                        const long long *slot_dst, double2 *out, long long obase, long long ostride, int ntime, int D,
-                        int rt, int f32, hipStream_t s);
+                        const LsBlock *blocks, int nblock, const LsChunk *chunks, const int *left,
+                        const LsTask *tasks, const long long *slot_dst, double2 *out, long long obase,
+                        long long ostride, int ntime, int D, int rt, int f32, hipStream_t s);
+// Sums of partial slots: out[obase + ti*ostride + items[k].dst] =
+// sum_{p in pidx[items[k].list_off .. + n)} part[ti*pstride + p], in list order.
+void launch_ls_reduce(const double2 *part, long long pstride, const GroupItem *items, int nitems, const int *pidx,
+                      double2 *out, long long obase, long long ostride, int ntime, hipStream_t s);
 void launch_resolve(const double2 *gvals, const uint16_t *gmasks, const CorrJob *jobs, int ncorr, int nsp,
                     const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, hipStream_t s);
 void launch_accumulate(const double2 *cval, const int *tdelta, int nsp, int ncorr, double2 *sum, double2 *comp,
