@@ -9,9 +9,12 @@ dilution TB16 / EI4 / DF4.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pipi32|pipi24|small]
 
-Prints one JSON line.  The roofline object is the VdaggerV kernel (FP64
-matrix cores), timed with HIP events on the stream it runs on; cpu_baseline is
-the numpy oracle timed on a bounded sample on this host and extrapolated.
+Prints one JSON line.  The roofline object is the largest kernel of the step,
+the left-stationary trace kernel k_trace_lstat (C4cC traces; HBM-streaming),
+timed with HIP events around each of its launches on the stream it runs on;
+its algorithmic bytes per launch are the distinct product matrices the launch
+reads plus the partial sums it writes.  cpu_baseline is the numpy oracle timed
+on a bounded sample on this host and extrapolated.
 """
 import argparse
 import json
@@ -32,7 +35,6 @@ WORKLOADS = {
     "small": dict(L=4, T=8, N=8, nrnd=4, dilT=2, dilE=4, pmax="0,1"),
 }
 
-PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0      # GB/s
 
 
@@ -56,11 +58,11 @@ correlator_list = C6cD:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0
 """.format(**w)
 
 
-def vdv_traffic(workload):
-    """HBM bytes of one VdaggerV launch, measured with two rocprofv3 --pmc runs
-    (FETCH_SIZE, WRITE_SIZE, copy-calibrated) and kept under profiles/; None
-    for a workload without such a measurement."""
-    path = os.path.join(ROOT, "profiles", "k_vdv_%s_traffic.txt" % workload)
+def kernel_traffic(kernel, workload):
+    """HBM bytes of one launch of `kernel`, measured with two rocprofv3 --pmc
+    runs (FETCH_SIZE, WRITE_SIZE, copy-calibrated) and kept under profiles/;
+    None for a workload without such a measurement."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_traffic.txt" % (kernel, workload))
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -170,23 +172,24 @@ def main():
         ops.create_operators()
         slaph.contract(ctx)
         print("# warmup %d done" % i, file=sys.stderr, flush=True)
-    vdv_ms, stage = [], {"vdaggerv": 0.0, "factors": 0.0, "contract": 0.0}
+    stage = {"vdaggerv": 0.0, "factors": 0.0, "contract": 0.0}
+    ls = {"ms": 0.0, "launches": 0.0, "bytes": 0.0}
     t0 = time.perf_counter()
     for i in range(args.steps):
         ops.create_operators()
-        vdv_ms.append(ctx.stage_time("vdaggerv_kernel"))
         slaph.contract(ctx)
         for k in stage:
             stage[k] += ctx.stage_time(k) / args.steps
+        for k in ls:
+            ls[k] += ctx.kernel_stat("k_trace_lstat", k)
         print("# step %d done" % i, file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     ms = elapsed / args.steps * 1000.0
     L, T, N = w["L"], w["T"], w["N"]
-    K = 3 * L ** 3
-    n_vdv = ctx.query("n_vdaggerv") - (1 if ctx.query("index_of_unity") >= 0 else 0)
-    flops = 8.0 * N * N * K * n_vdv * T
-    kern_ms = sum(vdv_ms) / len(vdv_ms)
-    achieved = flops / (kern_ms / 1e3) / 1e12
+    launches = max(ls["launches"], 1.0)
+    kern_ms = ls["ms"] / launches
+    bytes_per_launch = ls["bytes"] / launches
+    achieved = bytes_per_launch / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
     line = {
         "metric": "s per gauge config (VdV+all diagrams) L=32 T=64 Nev=220: mixed vs FP64 @1e-10",
         "value": ms / 1000.0,
@@ -206,14 +209,12 @@ def main():
                    "precision": "fp64"},
         "stages_ms": {"vdaggerv": stage["vdaggerv"], "factors": stage["factors"],
                       "contract_incl_factors": stage["contract"]},
-        "roofline": {"bound": "mfma", "kernel": "k_vdv (VdaggerV)", "achieved": achieved,
-                     "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA,
-                     "traffic": vdv_traffic(args.workload), "traffic_unit": "bytes per launch (HBM, rocprofv3 --pmc)",
-                     "algorithmic_flops_per_launch": flops, "avg_launch_ms": kern_ms,
-                     # the kernel forms each complex product from 3 real ones: the MFMA
-                     # pipes execute 3/4 of the algorithmic count
-                     "issued_fp64_mfma_flops_per_launch": 0.75 * flops,
-                     "issued_frac": 0.75 * achieved / PEAK_FP64_MFMA},
+        "roofline": {"bound": "hbm", "kernel": "k_trace_lstat (C4cC traces)", "achieved": achieved,
+                     "peak": PEAK_HBM, "unit": "GB/s", "frac": achieved / PEAK_HBM,
+                     "traffic": kernel_traffic("k_trace_lstat", args.workload),
+                     "traffic_unit": "bytes per launch (HBM, rocprofv3 --pmc)",
+                     "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": kern_ms,
+                     "launches_per_step": ls["launches"] / args.steps},
     }
     if not args.no_precision_forms:
         pf = precision_forms(ctx, ops, ms / 1000.0)

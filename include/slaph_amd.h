@@ -114,6 +114,13 @@ int slc_fetch_correlators(const slc_ctx *ctx, const char *diagram, double *out);
  * events): what = "vdaggerv", "vdaggerv_kernel", "factors", "contract". */
 int slc_stage_time(const slc_ctx *ctx, const char *what, double *ms);
 
+/* Launch statistics of one kernel in the last contract (no reference
+ * counterpart; for roofline measurements): kernel = "k_trace_lstat";
+ * what = "ms" (device time summed over its launches, HIP events on the
+ * engine stream), "launches", or "bytes" (algorithmic bytes summed over the
+ * launches: each operand matrix once, every partial sum written once). */
+int slc_kernel_stat(const slc_ctx *ctx, const char *kernel, const char *what, double *value);
+
 /* Upper bound on the device memory the contraction may use for per-chunk
  * products and traces (bytes); sets the number of slice pairs per launch.
  * 0 (the default) = 80 % of the device memory free when contract starts;

@@ -193,6 +193,18 @@ int slc_stage_time(const slc_ctx *cctx, const char *what, double *ms) {
   });
 }
 
+int slc_kernel_stat(const slc_ctx *cctx, const char *kernel, const char *what, double *value) {
+  slc_ctx *ctx = const_cast<slc_ctx *>(cctx);
+  return guarded(ctx, [&] {
+    std::string w = what ? what : "";
+    auto const ks = ctx->eng->kernel_stat(kernel ? kernel : "");
+    if (w == "ms") *value = ks.ms;
+    else if (w == "launches") *value = ks.launches;
+    else if (w == "bytes") *value = ks.bytes;
+    else throw std::out_of_range("unknown kernel statistic " + w);
+  });
+}
+
 int slc_set_chunk_budget(slc_ctx *ctx, long long bytes) {
   return guarded(ctx, [&] {
     if (bytes < 0) throw std::out_of_range("budget must not be negative");

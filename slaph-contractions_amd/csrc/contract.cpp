@@ -262,6 +262,7 @@ struct Family {
   std::vector<GroupItem> ls_red;       // per item: its partial sums in ls_ridx
   std::vector<int> ls_ridx;
   long long ls_npart = 0;  // partial sums per slice pair
+  double ls_bytes = 0;     // algorithmic bytes per slice pair: distinct operand matrices + partial sums
   DevBuf d_lsb, d_lsc, d_lst, d_lsd, d_lsl, d_lsr, d_lsri;
 };
 
@@ -552,6 +553,21 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
     f.ls_red.push_back(r);
   }
   if (f.ls_npart > INT32_MAX) throw std::runtime_error("trace plan: too many partial sums");
+  {
+    std::set<long long> lset, rset;
+    for (auto const &it : f.gitems)
+      for (int c = 0; c < it.n; ++c) {
+        int2 p = f.gpairs[it.list_off + c];
+        lset.insert(root.h_Xq.at(it.qn) + p.x);
+        rset.insert(root.h_Yq.at(it.qn) + p.y);
+      }
+    size_t nmat = lset.size() + rset.size();
+    if (f.coincide) {  // both operands read one table at one time base
+      lset.insert(rset.begin(), rset.end());
+      nmat = lset.size();
+    }
+    f.ls_bytes = double(sizeof(double2)) * (double(nmat) * D * D + double(f.ls_npart));
+  }
   upload(f.d_lsb, f.ls_blocks);
   upload(f.d_lsc, f.ls_chunks);
   upload(f.d_lst, f.ls_tasks);
@@ -609,6 +625,10 @@ struct ContractCache {
   DevBuf d_zmask;
   std::vector<ChunkPlan> chunks;
   DevBuf d_prod, d_gv, d_cval, d_zv, d_res, d_pv;
+  std::vector<hipEvent_t> ev;  // event pairs around the left-stationary trace launches
+  ~ContractCache() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
   std::vector<DevBuf> d_sum, d_comp;  // per diagram: Lt x n_corr accumulators
 };
 
@@ -1351,6 +1371,8 @@ void Engine::contract(int single_block_pair, int kahan) {
   // ---- launches: per chunk, products in topological order, then traces,
   // then resolve + accumulate per diagram; no host work between chunks ----
   EventTimer ctimer(stream_);
+  size_t ls_ev = 0;  // events used around left-stationary trace launches
+  double ls_bytes = 0;
   size_t last_bp = size_t(-1);
   for (auto const &ch : C.chunks) {
     if (progress && ch.bp != last_bp) {
@@ -1394,10 +1416,19 @@ void Engine::contract(int single_block_pair, int kahan) {
         } else if (ns.kind == kTrace2 && !f.ls_blocks.empty() && !lstat_off) {
           if (ch.pbase[fi] + (long long)nc.ntime * f.ls_npart > ch.p_total)
             throw std::logic_error("trace plan indexes outside the partial-sum buffer");
+          if (ls_ev + 2 > C.ev.size()) {
+            C.ev.resize(ls_ev + 2, nullptr);
+            SLAPH_HIP(hipEventCreate(&C.ev[ls_ev]));
+            SLAPH_HIP(hipEventCreate(&C.ev[ls_ev + 1]));
+          }
+          SLAPH_HIP(hipEventRecord(C.ev[ls_ev], stream_));
+          ls_bytes += f.ls_bytes * nc.ntime;
           launch_trace_lstat(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
                              f.d_lsb.as<LsBlock>(), int(f.ls_blocks.size()), f.d_lsc.as<LsChunk>(),
                              f.d_lsl.as<int>(), f.d_lst.as<LsTask>(), f.d_lsd.as<long long>(), C.d_pv.as<double2>(),
                              ch.pbase[fi], f.ls_npart, nc.ntime, D_, rt, f32, stream_);
+          SLAPH_HIP(hipEventRecord(C.ev[ls_ev + 1], stream_));
+          ls_ev += 2;
           launch_ls_reduce(C.d_pv.as<double2>() + ch.pbase[fi], f.ls_npart, f.d_lsr.as<GroupItem>(),
                            int(f.ls_red.size()), f.d_lsri.as<int>(), C.d_gv.as<double2>(), gb, f.gstride, nc.ntime,
                            stream_);
@@ -1433,6 +1464,17 @@ void Engine::contract(int single_block_pair, int kahan) {
                     C.d_res.as<double2>() + off, stream_);
   double kms = ctimer.stop();
   (void)kms;
+  {
+    KernelStat ks;
+    for (size_t e = 0; e < ls_ev; e += 2) {
+      float ms = 0;
+      SLAPH_HIP(hipEventElapsedTime(&ms, C.ev[e], C.ev[e + 1]));
+      ks.ms += ms;
+    }
+    ks.launches = int(ls_ev / 2);
+    ks.bytes = ls_bytes;
+    kstats_["k_trace_lstat"] = ks;
+  }
   for (size_t c = 0, off = 0; c < corrs.size(); off += (size_t)Lt_ * corrs[c].reqs.size(), ++c) {
     DiagramResult &r = results_[c];
     if (r.name != corrs[c].name || r.names.size() != corrs[c].datasets.size()) {

@@ -10,6 +10,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -67,8 +68,17 @@ struct ContractCache;  // contract.cpp
 
 struct StageTimes {
   double vdaggerv_ms = 0, factors_ms = 0, contract_ms = 0;
-  double vdaggerv_kernel_ms = 0;  // k_ahb only
+  double vdaggerv_kernel_ms = 0;  // the VdaggerV kernels only
   int vdaggerv_launches = 0;
+};
+
+// Launches of one kernel in the last create_operators / contract: summed
+// device time (HIP events around each launch on the engine stream) and the
+// summed algorithmic bytes the launches must move (each operand matrix read
+// once, every output written once).
+struct KernelStat {
+  double ms = 0, bytes = 0;
+  int launches = 0;
 };
 
 // Blocking upload of a host vector into a grow-only device buffer.
@@ -124,6 +134,11 @@ class Engine {
     prec_factors_ = factors;
   }
   StageTimes const &times() const { return times_; }
+  KernelStat kernel_stat(std::string const &name) const {
+    auto it = kstats_.find(name);
+    if (it == kstats_.end()) throw std::out_of_range("no statistics for kernel " + name);
+    return it->second;
+  }
   hipStream_t stream() const { return stream_; }
 
   // Raw entry: C = A^H diag(phase) B for device buffers (tests and tools).
@@ -176,6 +191,7 @@ class Engine {
   std::shared_ptr<FtPlan> ft_;
   DevBuf vr_, ftu_;  // row-major eigenvectors and per-x sums of one timeslice chunk
   StageTimes times_;
+  std::map<std::string, KernelStat> kstats_;
   // contract()'s plan: everything derived from the input file alone (quantum
   // numbers, trace groups, per-chunk index tables); rebuilt after configure()
   // or a budget change, reused by every later gauge configuration.

@@ -57,6 +57,7 @@ _SIGS = {
     "slc_correlator_name": (_c.c_int, [_P, _c.c_char_p, _c.c_int, _c.c_char_p, _c.c_int]),
     "slc_fetch_correlators": (_c.c_int, [_P, _c.c_char_p, _D]),
     "slc_stage_time": (_c.c_int, [_P, _c.c_char_p, _D]),
+    "slc_kernel_stat": (_c.c_int, [_P, _c.c_char_p, _c.c_char_p, _D]),
     "slc_set_chunk_budget": (_c.c_int, [_P, _c.c_longlong]),
     "slc_set_precision": (_c.c_int, [_P, _c.c_int, _c.c_int]),
     "slc_ahb_device": (_c.c_int, [_P, _P, _P, _P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_longlong,
@@ -204,6 +205,13 @@ class Context:
     def stage_time(self, what: str) -> float:
         v = _c.c_double()
         self._check(_lib.slc_stage_time(self._h, what.encode(), _c.byref(v)))
+        return v.value
+
+    def kernel_stat(self, kernel: str, what: str) -> float:
+        """Launch statistics of one kernel in the last contract: what = "ms"
+        (summed device time), "launches" or "bytes" (summed algorithmic bytes)."""
+        v = _c.c_double()
+        self._check(_lib.slc_kernel_stat(self._h, kernel.encode(), what.encode(), _c.byref(v)))
         return v.value
 
     def set_chunk_budget(self, nbytes: int):
