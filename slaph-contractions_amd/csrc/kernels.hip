@@ -833,7 +833,9 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3
   const int logical = x < rem ? x * (per + 1) + j : rem * (per + 1) + (x - rem) * per + j;
   const int ti = logical / nblock;
   const LsBlock B = blocks[logical % nblock];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the wave index through readfirstlane: uniform, so that per-wave table
+  // entries are read as scalars
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   for (int k = tid; k < kLsWaves * kLsSlots; k += 64 * kLsWaves) (&acc[0][0])[k] = make_double2(0.0, 0.0);
   const TS *xb = sX + Xt[ti] * DD;
   const TS *yb = sY + Yt[ti] * DD;
@@ -912,25 +914,28 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3
           const int ab = __builtin_amdgcn_readlane(qab, t + u);
           const int ia = ab & 0xffff, ib = ab >> 16;
           const int slot = __builtin_amdgcn_readlane(qs, t + u);
+          // the partners are read together and added first:
+          // tr(L_a R) + tr(L_b R) = sum_e (L_a + L_b)[e] R[e]
           const double2 *la = Ls + ia * DD;
+          double2 w[EPL];
 #pragma unroll
-          for (int k = 0; k < EPL; ++k) {
-            const double2 a = la[so[k]], v = r[u][k];
-            re = fma(a.x, v.x, re);
-            re = fma(-a.y, v.y, re);
-            im = fma(a.x, v.y, im);
-            im = fma(a.y, v.x, im);
-          }
+          for (int k = 0; k < EPL; ++k) w[k] = la[so[k]];
           if (ib >= 0) {
             const double2 *lb = Ls + ib * DD;
 #pragma unroll
             for (int k = 0; k < EPL; ++k) {
-              const double2 a = lb[so[k]], v = r[u][k];
-              re = fma(a.x, v.x, re);
-              re = fma(-a.y, v.y, re);
-              im = fma(a.x, v.y, im);
-              im = fma(a.y, v.x, im);
+              const double2 b = lb[so[k]];
+              w[k].x += b.x;
+              w[k].y += b.y;
             }
+          }
+#pragma unroll
+          for (int k = 0; k < EPL; ++k) {
+            const double2 v = r[u][k];
+            re = fma(w[k].x, v.x, re);
+            re = fma(-w[k].y, v.y, re);
+            im = fma(w[k].x, v.y, im);
+            im = fma(w[k].y, v.x, im);
           }
           if (slot >= 0) {
             re = wave_sum(re);
