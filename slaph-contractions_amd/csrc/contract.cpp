@@ -364,7 +364,7 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
   f.ls_red.clear();
   f.ls_ridx.clear();
   f.ls_npart = 0;
-  const int CL = trace_lstat_chunk(D);
+  const int CL = trace_lstat_chunk(D), PF = trace_lstat_ring(D);
   if (f.shape.nodes.back().kind != kTrace2 || CL <= 0 || f.gitems.empty()) return;
   std::function<int(std::vector<int> &, int)> find = [&](std::vector<int> &par, int x) {
     return par[x] == x ? x : par[x] = find(par, par[x]);
@@ -441,26 +441,45 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
     b.nchunk = int(cur.ch.size());
     b.slot_off = int(f.ls_slot_dst.size()) - cur.nslot;
     b.nslot = cur.nslot;
-    for (auto &cc : cur.ch) {
-      LsChunk ch;
-      ch.loff = int(f.ls_left.size());
-      ch.n = int(cc.left.size());
-      f.ls_left.insert(f.ls_left.end(), cc.left.begin(), cc.left.end());
-      const int base = int(f.ls_tasks.size()), nt = int(cc.tasks.size());
-      for (int w = 0; w <= kLsWaves; ++w) ch.wtask[w] = base + int((long long)nt * w / kLsWaves);
-      // a wave's range ends with a flush into the slot of its last task
+    // each chunk's tasks cut into kLsWaves ranges of about equal length; a
+    // wave's range ends with a flush into the slot of its last task
+    std::vector<std::array<int, kLsWaves + 1>> cut(cur.ch.size());
+    for (size_t c = 0; c < cur.ch.size(); ++c) {
+      auto &cc = cur.ch[c];
+      const int nt = int(cc.tasks.size());
+      for (int w = 0; w <= kLsWaves; ++w) cut[c][w] = int((long long)nt * w / kLsWaves);
       for (int w = 0; w < kLsWaves; ++w) {
-        int last = ch.wtask[w + 1] - 1 - base;
-        if (last < ch.wtask[w] - base) continue;
+        int last = cut[c][w + 1] - 1;
+        if (last < cut[c][w]) continue;
         if (cc.tasks[last].slot < 0) {
           int k = last;
           while (cc.tasks[k].slot < 0) ++k;  // the run's own slot, found at its end
           cc.tasks[last].slot = cc.tasks[k].slot;
         }
       }
-      f.ls_tasks.insert(f.ls_tasks.end(), cc.tasks.begin(), cc.tasks.end());
-      f.ls_chunks.push_back(ch);
     }
+    // tasks wave-major: wave w's ranges of all chunks are contiguous, each
+    // padded to a multiple of the ring depth with null tasks
+    std::vector<LsChunk> chs(cur.ch.size());
+    for (size_t c = 0; c < cur.ch.size(); ++c) {
+      chs[c].loff = int(f.ls_left.size());
+      chs[c].n = int(cur.ch[c].left.size());
+      f.ls_left.insert(f.ls_left.end(), cur.ch[c].left.begin(), cur.ch[c].left.end());
+    }
+    for (int w = 0; w < kLsWaves; ++w)
+      for (size_t c = 0; c < cur.ch.size(); ++c) {
+        auto const &cc = cur.ch[c];
+        chs[c].wbeg[w] = int(f.ls_tasks.size());
+        f.ls_tasks.insert(f.ls_tasks.end(), cc.tasks.begin() + cut[c][w], cc.tasks.begin() + cut[c][w + 1]);
+        while ((f.ls_tasks.size() - chs[c].wbeg[w]) % PF != 0) {
+          LsTask t = f.ls_tasks.back();
+          t.ia = t.ib = -1;
+          t.slot = -1;
+          f.ls_tasks.push_back(t);
+        }
+        chs[c].wcnt[w] = int(f.ls_tasks.size()) - chs[c].wbeg[w];
+      }
+    f.ls_chunks.insert(f.ls_chunks.end(), chs.begin(), chs.end());
     f.ls_blocks.push_back(b);
     cur = Cur();
   };

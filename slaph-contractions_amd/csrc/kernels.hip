@@ -867,53 +867,55 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3
   int so[EPL];  // swizzled LDS slots of this lane's elements lane + 64 k
 #pragma unroll
   for (int k = 0; k < EPL; ++k) so[k] = ls_swz(lane + 64 * k);
-  if (B.nchunk > 0) fetch(0);
+  // Each wave's tasks of the block form one contiguous stream, chunk by
+  // chunk, every chunk's share padded to a multiple of PF (null tasks have
+  // ia < 0).  The ring of PF right matrices in flight runs through the whole
+  // stream: the first tasks of chunk c + 1 are loaded while chunk c finishes,
+  // so only the LDS staging waits at a chunk boundary.  The descriptors sit
+  // one per lane, 64 stream positions at a time; a new batch starts before a
+  // round would read past it (a round reads up to 2 PF - 1 positions ahead).
+  const int s0 = chunks[B.chunk_off].wbeg[wave];
+  const int ns = chunks[B.chunk_off + B.nchunk - 1].wbeg[wave] + chunks[B.chunk_off + B.nchunk - 1].wcnt[wave] - s0;
+  int bb = 0;  // stream position of the descriptor batch
+  int qy = 0, qab = -1, qs = -1;
+  auto batch = [&]() {
+    qy = 0;
+    qab = -1;
+    qs = -1;
+    if (bb + lane < ns) {
+      const LsTask q = tasks[s0 + bb + lane];
+      qy = q.yrel;
+      qab = int((unsigned short)q.ia) | (int(q.ib) << 16);
+      qs = q.slot;
+    }
+  };
+  fetch(0);
+  double2 r[PF][EPL];
+  if (ns > 0) {
+    batch();
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const long long y = __builtin_amdgcn_readlane(qy, min(u, ns - 1));
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) r[u][k] = ld2(yb + y * DD + lane + 64 * k);
+    }
+  }
+  double re = 0.0, im = 0.0;
+  int pos = 0;  // stream position of the next task
   for (int c = 0; c < B.nchunk; ++c) {
-    const LsChunk C = chunks[B.chunk_off + c];
+    const int cnt = chunks[B.chunk_off + c].wcnt[wave];
     commit(c);
-    const int t0 = C.wtask[wave], t1 = C.wtask[wave + 1];
-    double re = 0.0, im = 0.0;
-    // The wave's task descriptors, up to 64 at a time, one per lane; the task
-    // loop reads them with readlane, so no memory round trip sits between a
-    // task and the loads of the task PF positions ahead.  Ring of PF tasks in
-    // flight: slot u holds task t + u of the current round (the loop over u
-    // is unrolled, so the ring indexes are compile-time constants).  The
-    // first batch's descriptors and ring loads are issued before the next
-    // chunk's left matrices, so that its first tasks do not wait for them.
-    int qy = 0, qab = 0, qs = -1;
-    double2 r[PF][EPL];
-    auto batch = [&](int tb) {
-      const int nt = min(64, t1 - tb);
-      qy = 0;
-      qab = 0;
-      qs = -1;
-      if (lane < nt) {
-        const LsTask q = tasks[tb + lane];
-        qy = q.yrel;
-        qab = int((unsigned short)q.ia) | (int(q.ib) << 16);
-        qs = q.slot;
+    if (c + 1 < B.nchunk) fetch(c + 1);
+    for (int e = pos + cnt; pos < e; pos += PF) {
+      if (pos - bb > 64 - 2 * PF) {  // a round reads up to pos - bb + 2 PF - 1
+        bb = pos;
+        batch();
       }
-      // ring loads past the last task repeat the last task's matrix, so that
-      // every load is unconditional and the waits are exact
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
-        const long long y = __builtin_amdgcn_readlane(qy, min(u, nt - 1));
-#pragma unroll
-        for (int k = 0; k < EPL; ++k) r[u][k] = ld2(yb + y * DD + lane + 64 * k);
-      }
-    };
-    if (t0 < t1) batch(t0);
-    if (c + 1 < B.nchunk) fetch(c + 1);
-    for (int tb = t0; tb < t1; tb += 64) {
-      if (tb != t0) batch(tb);
-      const int nt = min(64, t1 - tb);
-      for (int t = 0; t < nt; t += PF) {
-#pragma unroll
-        for (int u = 0; u < PF; ++u) {
-          if (t + u < nt) {
-          const int ab = __builtin_amdgcn_readlane(qab, t + u);
-          const int ia = ab & 0xffff, ib = ab >> 16;
-          const int slot = __builtin_amdgcn_readlane(qs, t + u);
+        const int ab = __builtin_amdgcn_readlane(qab, pos + u - bb);
+        const int ia = (short)(ab & 0xffff), ib = ab >> 16;
+        if (ia >= 0) {
           // the partners are read together and added first:
           // tr(L_a R) + tr(L_b R) = sum_e (L_a + L_b)[e] R[e]
           const double2 *la = Ls + ia * DD;
@@ -937,6 +939,7 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3
             im = fma(w[k].x, v.y, im);
             im = fma(w[k].y, v.x, im);
           }
+          const int slot = __builtin_amdgcn_readlane(qs, pos + u - bb);
           if (slot >= 0) {
             re = wave_sum(re);
             im = wave_sum(im);
@@ -944,12 +947,12 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3
             re = 0.0;
             im = 0.0;
           }
-          }
-          // refill this ring slot with the task PF positions ahead
-          const long long y = __builtin_amdgcn_readlane(qy, min(t + u + PF, nt - 1));
-#pragma unroll
-          for (int k = 0; k < EPL; ++k) r[u][k] = ld2(yb + y * DD + lane + 64 * k);
         }
+        // refill this ring slot with the task PF positions ahead (past the
+        // stream's end: its last task again, so that every load is issued)
+        const long long y = __builtin_amdgcn_readlane(qy, min(pos + u + PF, ns - 1) - bb);
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) r[u][k] = ld2(yb + y * DD + lane + 64 * k);
       }
     }
   }
@@ -1467,6 +1470,11 @@ bool trace_lstat_supported(int D) { return D == 8 || D == 16 || D == 24 || D == 
 int trace_lstat_chunk(int D) {
   if (!trace_lstat_supported(D)) return 0;
   return kLsStageBytes / (D * D * int(sizeof(double2)));
+}
+
+int trace_lstat_ring(int D) {
+  const int epl = D * D / 64;
+  return epl <= 4 ? 4 : (epl <= 8 ? 2 : 1);
 }
 
 void launch_trace_lstat(const void *sX, const long long *Xt, const void *sY, const long long *Yt,

@@ -217,15 +217,18 @@ struct LsBlock {
 };
 struct LsChunk {
   int loff, n;
-  int wtask[kLsWaves + 1];  // task range of each wavefront
+  int wbeg[kLsWaves], wcnt[kLsWaves];  // each wavefront's tasks [wbeg, wbeg + wcnt)
 };
 struct LsTask {
   int yrel;
   short ia, ib;
   int slot;
 };
-// Largest number of left matrices one chunk stages for matrix size D.
+// Largest number of left matrices one chunk stages for matrix size D; each
+// wavefront's task count per chunk must be a multiple of trace_lstat_ring(D),
+// and a wavefront's tasks of one block are contiguous (chunk after chunk).
 int trace_lstat_chunk(int D);
+int trace_lstat_ring(int D);
 bool trace_lstat_supported(int D);
 void launch_trace_lstat(const void *slabX, const long long *Xt, const void *slabY, const long long *Yt,
                         const LsBlock *blocks, int nblock, const LsChunk *chunks, const int *left,
