@@ -330,7 +330,7 @@ void Engine::create_operators() {
       ids.push_back(op.id);
   }
   const char *direct = std::getenv("SLAPH_VDV_DIRECT");
-  if (prec_vdv_ == 0 && !ids.empty() && !(direct && *direct && *direct != '0')) {
+  if (prec_vdv_ == 0 && !ids.empty() && gd_.Lz <= kFtMaxLz && !(direct && *direct && *direct != '0')) {
     vdaggerv_fourier(ids);
   } else {
     vdaggerv_direct(ids);

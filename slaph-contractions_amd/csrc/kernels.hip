@@ -363,11 +363,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // 2 (3 + 2 + 4 nk) + 4 FP64 vector operations per lane.  The operands of the
 // next four sites are loaded while the current four are processed.
 // ---------------------------------------------------------------------------
-template <int NC, int NK, int H>
+template <int NC, int NK, int H, bool kFast>
 __device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, int Npad, int I, int J, int Ly, int Lz,
                                             long long site0, FtPass const &P, const double2 *__restrict__ ey,
-                                            const double2 *__restrict__ zcs, double2 *__restrict__ Uo,
-                                            long long ustride) {
+                                            const double2 *zl, double2 *__restrict__ Uo, long long ustride) {
   const int lane = threadIdx.x & 63, c = lane >> 4, m = lane & 15;
   // Colour-3 lanes (the fourth K slot) read colour 2 and zero their A
   // operand, so that every load is unconditional.
@@ -381,18 +380,7 @@ __device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, int 
 #pragma unroll
   for (int k = 0; k < NC; ++k) U[k][0] = U[k][1] = zero2;
   double a0r[2], a0i[2], cr[NK][2], ci[NK][2], sr[NK][2], si[NK][2];
-  const int GZ = (Lz + 3) >> 2, ng = Ly * GZ;
-  double2 na[4], nb[4];
-  auto load = [&](int g) {
-    const int y = g / GZ, z0 = (g - y * GZ) * 4;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const long long off = (long long)(y * Lz + min(z0 + u, Lz - 1)) * sstep;
-      na[u] = pa[off];
-      nb[u] = pb[off];
-    }
-  };
-  // W of site u of the current group: three MFMAs (Gauss).
+  // W of one site: three MFMAs (Gauss).
   auto mf = [&](const double2 &a, const double2 &b, d4 &p1, d4 &p2, d4 &p3) {
     const d4 dz = {0.0, 0.0, 0.0, 0.0};
     const double ax = a.x * amask, ay = a.y * amask;
@@ -400,11 +388,8 @@ __device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, int 
     p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ay, b.y, dz, 0, 0, 0);
     p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ax - ay, b.x + b.y, dz, 0, 0, 0);
   };
-  // z transform of the two rows of this wave.
-  auto zacc = [&](int z, const d4 &p1, const d4 &p2, const d4 &p3) {
-    double2 cs[NK];
-#pragma unroll
-    for (int k = 0; k < NK; ++k) cs[k] = zcs[k * Lz + z];
+  // z transform of the two rows of this wave, site phases cs (cos, sin).
+  auto zacc = [&](const double2 *cs, const d4 &p1, const d4 &p2, const d4 &p3) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int r = 2 * H + q;
@@ -420,61 +405,129 @@ __device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, int 
       }
     }
   };
-  load(0);
-  for (int g = 0; g < ng; ++g) {
-    double2 ca[4], cb[4];
+  auto zero_z = [&]() {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      ca[u] = na[u];
-      cb[u] = nb[u];
+    for (int q = 0; q < 2; ++q) {
+      a0r[q] = a0i[q] = 0.0;
+#pragma unroll
+      for (int k = 0; k < NK; ++k) cr[k][q] = ci[k][q] = sr[k][q] = si[k][q] = 0.0;
     }
-    if (g + 1 < ng) load(g + 1);
-    const int y = g / GZ, z0 = (g - y * GZ) * 4;
-    if (z0 == 0) {
+  };
+  // End of a z line: T(+-k) = (CR +- SI) + i (CI -+ SR), then
+  // U[combination] += exp(-i 2 pi py y / Ly) T(pz).
+  auto yline = [&](int y) {
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        a0r[q] = a0i[q] = 0.0;
+    for (int k2 = 0; k2 < NC; ++k2) {
+      if (k2 < P.nc) {
+        const int sel = P.tsel[k2];
+        const double2 e = ey[k2 * Ly + y];
 #pragma unroll
-        for (int k = 0; k < NK; ++k) cr[k][q] = ci[k][q] = sr[k][q] = si[k][q] = 0.0;
-      }
-    }
-    // Software pipeline: the MFMAs of site u + 1 are issued before the vector
-    // work of site u.
-    d4 e1, e2, e3, o1, o2, o3;
-    mf(ca[0], cb[0], e1, e2, e3);
-    if (z0 + 1 < Lz) mf(ca[1], cb[1], o1, o2, o3);
-    zacc(z0, e1, e2, e3);
-    if (z0 + 2 < Lz) mf(ca[2], cb[2], e1, e2, e3);
-    if (z0 + 1 < Lz) zacc(z0 + 1, o1, o2, o3);
-    if (z0 + 3 < Lz) mf(ca[3], cb[3], o1, o2, o3);
-    if (z0 + 2 < Lz) zacc(z0 + 2, e1, e2, e3);
-    if (z0 + 3 < Lz) zacc(z0 + 3, o1, o2, o3);
-    if (z0 + 4 >= Lz) {
-      // End of a z line: T(+-k) = (CR +- SI) + i (CI -+ SR), then
-      // U[combination] += exp(-i 2 pi py y / Ly) T(pz).
+        for (int q = 0; q < 2; ++q) {
+          double tr = a0r[q], ti = a0i[q];
 #pragma unroll
-      for (int k2 = 0; k2 < NC; ++k2) {
-        if (k2 < P.nc) {
-          const int sel = P.tsel[k2];
-          const double2 e = ey[k2 * Ly + y];
-#pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            double tr = a0r[q], ti = a0i[q];
-#pragma unroll
-            for (int k = 0; k < NK; ++k) {
-              if (sel == 1 + 2 * k) {
-                tr = cr[k][q] + si[k][q];
-                ti = ci[k][q] - sr[k][q];
-              } else if (sel == 2 + 2 * k) {
-                tr = cr[k][q] - si[k][q];
-                ti = ci[k][q] + sr[k][q];
-              }
+          for (int k = 0; k < NK; ++k) {
+            if (sel == 1 + 2 * k) {
+              tr = cr[k][q] + si[k][q];
+              ti = ci[k][q] - sr[k][q];
+            } else if (sel == 2 + 2 * k) {
+              tr = cr[k][q] - si[k][q];
+              ti = ci[k][q] + sr[k][q];
             }
-            U[k2][q].x = fma(e.x, tr, fma(-e.y, ti, U[k2][q].x));
-            U[k2][q].y = fma(e.x, ti, fma(e.y, tr, U[k2][q].y));
           }
+          U[k2][q].x = fma(e.x, tr, fma(-e.y, ti, U[k2][q].x));
+          U[k2][q].y = fma(e.x, ti, fma(e.y, tr, U[k2][q].y));
         }
       }
+    }
+  };
+  double2 na[4], nb[4];
+  if constexpr (kFast) {
+    double2 nc_[4][NK];
+    // Lz % 4 == 0: a group is four consecutive sites of one z line; the
+    // operand pointers and the z position advance incrementally.
+    const int ng = Ly * (Lz >> 2);
+    const long long gstep = 4 * sstep;
+    const double2 *qa = pa, *qb = pb;
+    auto load = [&](int z0) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        na[u] = qa[u * sstep];
+        nb[u] = qb[u * sstep];
+#pragma unroll
+        for (int k = 0; k < NK; ++k) nc_[u][k] = zl[k * Lz + z0 + u];
+      }
+    };
+    load(0);
+    int y = 0, z0 = 0;
+    for (int g = 0; g < ng; ++g) {
+      double2 ca[4], cb[4], cs[4][NK];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        ca[u] = na[u];
+        cb[u] = nb[u];
+#pragma unroll
+        for (int k = 0; k < NK; ++k) cs[u][k] = nc_[u][k];
+      }
+      const int zn = z0 + 4 == Lz ? 0 : z0 + 4;
+      if (g + 1 < ng) {
+        qa += gstep;
+        qb += gstep;
+        load(zn);
+      }
+      if (z0 == 0) zero_z();
+      // Software pipeline: the MFMAs of site u + 1 are issued before the
+      // vector work of site u.
+      d4 e1, e2, e3, o1, o2, o3;
+      mf(ca[0], cb[0], e1, e2, e3);
+      mf(ca[1], cb[1], o1, o2, o3);
+      zacc(cs[0], e1, e2, e3);
+      mf(ca[2], cb[2], e1, e2, e3);
+      zacc(cs[1], o1, o2, o3);
+      mf(ca[3], cb[3], o1, o2, o3);
+      zacc(cs[2], e1, e2, e3);
+      zacc(cs[3], o1, o2, o3);
+      if (zn == 0) {
+        yline(y);
+        ++y;
+      }
+      z0 = zn;
+    }
+  } else {
+    const int GZ = (Lz + 3) >> 2, ng = Ly * GZ;
+    auto load = [&](int g) {
+      const int y = g / GZ, z0 = (g - y * GZ) * 4;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const long long off = (long long)(y * Lz + min(z0 + u, Lz - 1)) * sstep;
+        na[u] = pa[off];
+        nb[u] = pb[off];
+      }
+    };
+    load(0);
+    for (int g = 0; g < ng; ++g) {
+      double2 ca[4], cb[4], cs[4][NK];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        ca[u] = na[u];
+        cb[u] = nb[u];
+      }
+      if (g + 1 < ng) load(g + 1);
+      const int y = g / GZ, z0 = (g - y * GZ) * 4;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int k = 0; k < NK; ++k) cs[u][k] = zl[k * Lz + min(z0 + u, Lz - 1)];
+      if (z0 == 0) zero_z();
+      d4 e1, e2, e3, o1, o2, o3;
+      mf(ca[0], cb[0], e1, e2, e3);
+      if (z0 + 1 < Lz) mf(ca[1], cb[1], o1, o2, o3);
+      zacc(cs[0], e1, e2, e3);
+      if (z0 + 2 < Lz) mf(ca[2], cb[2], e1, e2, e3);
+      if (z0 + 1 < Lz) zacc(cs[1], o1, o2, o3);
+      if (z0 + 3 < Lz) mf(ca[3], cb[3], o1, o2, o3);
+      if (z0 + 2 < Lz) zacc(cs[2], e1, e2, e3);
+      if (z0 + 3 < Lz) zacc(cs[3], o1, o2, o3);
+      if (z0 + 4 >= Lz) yline(y);
     }
   }
   // Accumulator row r of lane l is tile row l/16 + 4r, column l%16: pair
@@ -488,8 +541,10 @@ __device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, int 
   }
 }
 
-template <int NC, int NK>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_vdv_ft(
+// The wide instantiation (13 combinations, two |pz|) needs more than 256
+// registers per lane and runs at one wave per SIMD.
+template <int NC, int NK, bool kFast>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NC > 9 ? 1 : 2, 2))) void k_vdv_ft(
     const double2 *__restrict__ Vr, long long vstride, int Npad, int Lx, int Ly, int Lz,
     const int2 *__restrict__ tiles, int ntp, FtPass P, const double2 *__restrict__ ey,
     const double2 *__restrict__ zcs, double2 *__restrict__ U) {
@@ -502,10 +557,15 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   const long long site0 = (long long)x * Ly * Lz;
   const long long ustride = (long long)ntp * 256;
   double2 *Uo = U + (((long long)t * Lx + x) * P.nc * ntp + tp) * 256;
+  // z phases of the pass in LDS (read with one broadcast per site and |pz|)
+  __shared__ double2 zl[NK * kFtMaxLz];
+  for (int i = threadIdx.x; i < NK * Lz; i += blockDim.x) zl[(i / Lz) * Lz + i % Lz] = zcs[(i / Lz) * Lz + i % Lz];
+  __syncthreads();
+  const double2 *V = Vr + t * vstride;
   if (threadIdx.x < 64)
-    vdv_ft_wave<NC, NK, 0>(Vr + t * vstride, Npad, ij.x, ij.y, Ly, Lz, site0, P, ey, zcs, Uo, ustride);
+    vdv_ft_wave<NC, NK, 0, kFast>(V, Npad, ij.x, ij.y, Ly, Lz, site0, P, ey, zl, Uo, ustride);
   else
-    vdv_ft_wave<NC, NK, 1>(Vr + t * vstride, Npad, ij.x, ij.y, Ly, Lz, site0, P, ey, zcs, Uo, ustride);
+    vdv_ft_wave<NC, NK, 1, kFast>(V, Npad, ij.x, ij.y, Ly, Lz, site0, P, ey, zl, Uo, ustride);
 }
 
 // C(p) from the per-x sums: thread = (row, column) of one tile pair.
@@ -1332,14 +1392,26 @@ void launch_vdv_ft(const double2 *Vr, long long vstride, int Npad, int Lx, int L
   if (pass.nc > kFtMaxC || pass.nk < 0 || pass.nk > kFtMaxK)
     throw std::runtime_error("launch_vdv_ft: pass exceeds the combination limits");
   if (Npad % 16 != 0) throw std::runtime_error("launch_vdv_ft: Npad must be a multiple of 16");
+  if (Lz > kFtMaxLz) throw std::runtime_error("launch_vdv_ft: Lz above kFtMaxLz");
   const long long blocks = (long long)nt * Lx * ntp;
   if (blocks > 0x7fffffffLL) throw std::runtime_error("launch_vdv_ft: grid too large");
-  if (pass.nc <= 9 && pass.nk <= 1)
-    hipLaunchKernelGGL((k_vdv_ft<9, 1>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride, Npad, Lx, Ly, Lz,
-                       tiles, ntp, pass, ey, zcs, U);
-  else
-    hipLaunchKernelGGL((k_vdv_ft<kFtMaxC, kFtMaxK>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride, Npad, Lx,
-                       Ly, Lz, tiles, ntp, pass, ey, zcs, U);
+  // Lz % 4 == 0: the incremental-address form; otherwise the general one
+  const bool fast = (Lz & 3) == 0;
+  if (pass.nc <= 9 && pass.nk <= 1) {
+    if (fast)
+      hipLaunchKernelGGL((k_vdv_ft<9, 1, true>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride, Npad, Lx, Ly,
+                         Lz, tiles, ntp, pass, ey, zcs, U);
+    else
+      hipLaunchKernelGGL((k_vdv_ft<9, 1, false>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride, Npad, Lx, Ly,
+                         Lz, tiles, ntp, pass, ey, zcs, U);
+  } else {
+    if (fast)
+      hipLaunchKernelGGL((k_vdv_ft<kFtMaxC, kFtMaxK, true>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride,
+                         Npad, Lx, Ly, Lz, tiles, ntp, pass, ey, zcs, U);
+    else
+      hipLaunchKernelGGL((k_vdv_ft<kFtMaxC, kFtMaxK, false>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride,
+                         Npad, Lx, Ly, Lz, tiles, ntp, pass, ey, zcs, U);
+  }
   SLAPH_HIP(hipGetLastError());
 }
 

@@ -72,6 +72,7 @@ void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, int f32, hi
 // padding columns zero) made by launch_vrow from the column-major input.
 constexpr int kFtMaxC = 13;  // (py, pz) combinations of one pass
 constexpr int kFtMaxK = 2;   // nonzero |pz| values of one pass
+constexpr int kFtMaxLz = 128;  // largest Lz (z phases staged in LDS)
 struct FtPass {
   int nc, nk;
   // Per combination, which z transform it reads: 0 = pz 0, 1 + 2s = +k_s,
