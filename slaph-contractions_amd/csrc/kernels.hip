@@ -36,6 +36,18 @@ __device__ __forceinline__ double2 ld2(const float2 *p) {
 }
 __device__ __forceinline__ void st2(double2 *p, double2 v) { *p = v; }
 __device__ __forceinline__ void st2(float2 *p, double2 v) { *p = make_float2(float(v.x), float(v.y)); }
+// Streaming (non-temporal) stores for tables that are written once and read
+// by a later launch.
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st2_nt(double2 *p, double2 v) {
+  d2v w = {v.x, v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<d2v *>(p));
+}
+__device__ __forceinline__ void st2_nt(float2 *p, double2 v) {
+  f2v w = {float(v.x), float(v.y)};
+  __builtin_nontemporal_store(w, reinterpret_cast<f2v *>(p));
+}
 __device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b) {
   acc.x = fma(a.x, b.x, acc.x);
   acc.x = fma(-a.y, b.y, acc.x);
@@ -815,25 +827,42 @@ __global__ __launch_bounds__(256) void k_prod16(const T *__restrict__ sX, const 
                                                 T *__restrict__ out, long long obase, long long ostride,
                                                 const ComboProd *__restrict__ combos, int ncombo) {
   const int ti = blockIdx.y;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the wave index through readfirstlane: uniform, so that the combination
+  // tables come through the scalar cache
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int c0 = (blockIdx.x * 4 + wave) * kProdPerWave;
   const int c1 = min(c0 + kProdPerWave, ncombo);
+  if (c0 >= c1) return;
   const long long xt = Xt[ti], yt = Yt[ti];
   const int yoff = (lane >> 4) + 16 * (lane & 15);
   double2 xa[4];
   long long xcur = -1;
+  // the right operand of the next combination is loaded while the current
+  // one is multiplied and stored
+  ComboProd cn = combos[c0];
+  double2 yn[4];
+  {
+    const T *y = sY + (yt + Yq[cn.qn] + cn.j) * 256 + yoff;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) yn[s] = ld2(y + 4 * s);
+  }
   for (int c = c0; c < c1; ++c) {
-    const ComboProd cb = combos[c];
+    const ComboProd cb = cn;
+    double2 yb[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) yb[s] = yn[s];
     const long long xi = xt + Xq[cb.qn] + cb.i;
     if (xi != xcur) {
 #pragma unroll
       for (int s = 0; s < 4; ++s) xa[s] = ld2(sX + xi * 256 + lane + 64 * s);
       xcur = xi;
     }
-    const T *y = sY + (yt + Yq[cb.qn] + cb.j) * 256 + yoff;
-    double2 yb[4];
+    if (c + 1 < c1) {
+      cn = combos[c + 1];
+      const T *y = sY + (yt + Yq[cn.qn] + cn.j) * 256 + yoff;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) yb[s] = ld2(y + 4 * s);
+      for (int s = 0; s < 4; ++s) yn[s] = ld2(y + 4 * s);
+    }
     // C[r][c] = sum_k X[r][k] Y[k][c] from three real products:
     // P1 = Xr Yr, P2 = Xi Yi, P3 = (Xr + Xi)(Yr + Yi); re = P1 - P2, im = P3 - P1 - P2
     d4 p1 = d4{0.0, 0.0, 0.0, 0.0}, p2 = p1, p3 = p1;
@@ -852,7 +881,7 @@ __global__ __launch_bounds__(256) void k_prod16(const T *__restrict__ sX, const 
     }
     T *o = out + (obase + ti * ostride + cb.o) * 256;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) st2(o + lane + 64 * i, make_double2(p1[i] - p2[i], p3[i] - p1[i] - p2[i]));
+    for (int i = 0; i < 4; ++i) st2_nt(o + lane + 64 * i, make_double2(p1[i] - p2[i], p3[i] - p1[i] - p2[i]));
   }
 }
 
