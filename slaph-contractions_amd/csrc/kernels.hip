@@ -1137,7 +1137,7 @@ __global__ __launch_bounds__(256) void k_trace_grid(const T *__restrict__ sX, co
   const GroupItem g = items[blockIdx.x];
   const GridTile &Tl = tiles[blockIdx.y];
   __shared__ double2 part[4 * 256];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int DD = D * D, q = lane >> 4;
   const int nL = Tl.nL, nR = Tl.nR;
   const int la = min(lane & 15, nL - 1), lb = min(lane & 15, nR - 1);  // rows past the tile repeat its last
