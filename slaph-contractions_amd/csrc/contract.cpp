@@ -263,7 +263,10 @@ struct Family {
   std::vector<int> ls_ridx;
   long long ls_npart = 0;  // partial sums per slice pair
   double ls_bytes = 0;     // algorithmic bytes per slice pair: distinct operand matrices + partial sums
-  DevBuf d_lsb, d_lsc, d_lst, d_lsd, d_lsl, d_lsr, d_lsri;
+  // queued form: one record per block (one chunk each), see launch_trace_lstat_q
+  bool ls_queue = false;
+  std::vector<int> ls_units;
+  DevBuf d_lsb, d_lsc, d_lst, d_lsd, d_lsl, d_lsr, d_lsri, d_lsu, d_lsq;
 };
 
 struct CorrPlan {
@@ -363,8 +366,13 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
   f.ls_left.clear();
   f.ls_red.clear();
   f.ls_ridx.clear();
+  f.ls_units.clear();
   f.ls_npart = 0;
-  const int CL = trace_lstat_chunk(D), PF = trace_lstat_ring(D);
+  // The queued form (default; SLAPH_LSTAT_BLOCKS=1 selects blocks of up to
+  // kLsBlockChunks chunks in entry-major order) runs one chunk per unit in
+  // component -> piece -> entry order.
+  f.ls_queue = std::getenv("SLAPH_LSTAT_BLOCKS") == nullptr;
+  const int CL = f.ls_queue ? trace_lstat_qchunk(D) : trace_lstat_chunk(D), PF = trace_lstat_ring(D);
   if (f.shape.nodes.back().kind != kTrace2 || CL <= 0 || f.gitems.empty()) return;
   std::function<int(std::vector<int> &, int)> find = [&](std::vector<int> &par, int x) {
     return par[x] == x ? x : par[x] = find(par, par[x]);
@@ -490,7 +498,19 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
   };
   std::vector<Unit> units;
   const bool entry_major = noclass;
-  if (entry_major) {
+  if (f.ls_queue) {
+    // per component: every piece of its entries before the next piece, so
+    // that the units running together read one component-piece's matrices
+    for (size_t g = 0; g < order.size();) {
+      size_t h = g;
+      while (h < order.size() && order[h].first == order[g].first) ++h;
+      for (size_t pc = 0; pc < pieces.size(); ++pc)
+        for (size_t k = g; k < h; ++k)
+          for (size_t s0 = 0, nk = by_left[order[k].second].size(); s0 < nk; s0 += kLsSlots)
+            units.push_back({int(pc), order[k].second, s0});
+      g = h;
+    }
+  } else if (entry_major) {
     for (auto const &ord : order)
       for (size_t pc = 0; pc < pieces.size(); ++pc)
         for (size_t s0 = 0, nk = by_left[ord.second].size(); s0 < nk; s0 += kLsSlots)
@@ -503,7 +523,7 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
   }
   for (size_t u = 0; u < units.size(); ++u) {
     const int pc = units[u].pc;
-    if (u > 0 && (entry_major ? units[u - 1].xq != units[u].xq : units[u - 1].pc != pc)) flush();
+    if (u > 0 && (f.ls_queue || (entry_major ? units[u - 1].xq != units[u].xq : units[u - 1].pc != pc))) flush();
     const size_t s0 = units[u].s0, nk = by_left[units[u].xq].size();
     const long long xq = units[u].xq;
     auto const &all = by_left[xq];
@@ -565,6 +585,33 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
     }
   }
   flush();
+  if (f.ls_queue) {
+    // unit records; the task array ends with a null task, so that an empty
+    // range's first entry is readable
+    LsTask nt{0, -1, -1, -1};
+    f.ls_tasks.push_back(nt);
+    const int yr = kLsUnitHdr + kLsWaves * PF, ur = yr + CL;
+    f.ls_units.assign(f.ls_blocks.size() * ur, 0);
+    for (size_t b = 0; b < f.ls_blocks.size(); ++b) {
+      LsBlock const &bl = f.ls_blocks[b];
+      if (bl.nchunk != 1) throw std::logic_error("queued trace plan: a block must hold one chunk");
+      LsChunk const &c = f.ls_chunks[bl.chunk_off];
+      if (c.n < 1 || c.n > CL) throw std::logic_error("queued trace plan: chunk size out of range");
+      int *r = &f.ls_units[b * ur];
+      r[0] = c.n;
+      r[1] = bl.slot_off;
+      r[2] = bl.nslot;
+      for (int w = 0; w < kLsWaves; ++w) {
+        r[3 + w] = c.wbeg[w];
+        r[3 + kLsWaves + w] = c.wcnt[w];
+        for (int u = 0; u < PF && u < c.wcnt[w]; ++u) r[kLsUnitHdr + w * PF + u] = f.ls_tasks[c.wbeg[w] + u].yrel;
+      }
+      for (int m = 0; m < CL; ++m) r[yr + m] = f.ls_left[c.loff + std::min(m, c.n - 1)];
+      for (int k = 0; k < bl.nslot; ++k)
+        if (f.ls_slot_dst[bl.slot_off + k] != bl.slot_off + k)
+          throw std::logic_error("queued trace plan: partial sums must follow slot order");
+    }
+  }
   // (4) per item: its partial sums, in piece order
   for (size_t k = 0; k < f.gitems.size(); ++k) {
     GroupItem r{0, int(f.ls_ridx.size()), int(item_parts[k].size()), f.gitems[k].dst};
@@ -587,6 +634,16 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
     }
     f.ls_bytes = double(sizeof(double2)) * (double(nmat) * D * D + double(f.ls_npart));
   }
+  if (std::getenv("SLAPH_PROGRESS")) {
+    size_t null_tasks = 0, paired = 0;
+    for (auto const &t : f.ls_tasks) {
+      null_tasks += t.ia < 0;
+      paired += t.ib >= 0;
+    }
+    std::fprintf(stderr, "left-stationary trace plan (D = %d): %zu blocks, %zu chunks, %zu tasks (%zu null, %zu with two left matrices), %zu staged left matrices, %lld partial sums\n",
+                 D, f.ls_blocks.size(), f.ls_chunks.size(), f.ls_tasks.size(), null_tasks, paired, f.ls_left.size(),
+                 (long long)f.ls_npart);
+  }
   upload(f.d_lsb, f.ls_blocks);
   upload(f.d_lsc, f.ls_chunks);
   upload(f.d_lst, f.ls_tasks);
@@ -594,6 +651,10 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
   upload(f.d_lsl, f.ls_left);
   upload(f.d_lsr, f.ls_red);
   upload(f.d_lsri, f.ls_ridx);
+  if (f.ls_queue) {
+    upload(f.d_lsu, f.ls_units);
+    f.d_lsq.ensure(8 * sizeof(int));
+  }
 }
 
 // Every (factor type, time key) a leaf of this family reads at slice pair
@@ -1442,10 +1503,16 @@ void Engine::contract(int single_block_pair, int kahan) {
           }
           SLAPH_HIP(hipEventRecord(C.ev[ls_ev], stream_));
           ls_bytes += f.ls_bytes * nc.ntime;
-          launch_trace_lstat(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
-                             f.d_lsb.as<LsBlock>(), int(f.ls_blocks.size()), f.d_lsc.as<LsChunk>(),
-                             f.d_lsl.as<int>(), f.d_lst.as<LsTask>(), f.d_lsd.as<long long>(), C.d_pv.as<double2>(),
-                             ch.pbase[fi], f.ls_npart, nc.ntime, D_, rt, f32, stream_);
+          if (f.ls_queue)
+            launch_trace_lstat_q(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
+                                 f.d_lsu.as<int>(), int(f.ls_blocks.size()), f.d_lst.as<LsTask>(),
+                                 f.d_lsd.as<long long>(), C.d_pv.as<double2>(), ch.pbase[fi], f.ls_npart, nc.ntime, D_,
+                                 rt, f32, f.d_lsq.as<int>(), stream_);
+          else
+            launch_trace_lstat(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
+                               f.d_lsb.as<LsBlock>(), int(f.ls_blocks.size()), f.d_lsc.as<LsChunk>(),
+                               f.d_lsl.as<int>(), f.d_lst.as<LsTask>(), f.d_lsd.as<long long>(), C.d_pv.as<double2>(),
+                               ch.pbase[fi], f.ls_npart, nc.ntime, D_, rt, f32, stream_);
           SLAPH_HIP(hipEventRecord(C.ev[ls_ev + 1], stream_));
           ls_ev += 2;
           launch_ls_reduce(C.d_pv.as<double2>() + ch.pbase[fi], f.ls_npart, f.d_lsr.as<GroupItem>(),

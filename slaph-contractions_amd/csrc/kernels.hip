@@ -1054,6 +1054,268 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3
   }
 }
 
+// Queued form of the same traces: one workgroup per resident slot.  The
+// units (time, block of one chunk) of XCD x's contiguous eighth of the unit
+// sequence are claimed one at a time from counter x (then from the other
+// counters once x's range is spent), so that the units running together on
+// one XCD are neighbours in plan order and find each other's right matrices
+// in that XCD's L2.  Claims run two units ahead: the ring of right matrices
+// runs on from unit g into unit g+1, and at the end of unit g the workgroup
+// loads unit g+1's left matrices into registers, its first task batch and
+// unit g+2's record.  A unit record is kLsUnitHdr ints (n, slot_off, nslot,
+// wbeg[kLsWaves], wcnt[kLsWaves], 0), each wave's first PF right matrices
+// (0 past the wave's range), then the unit's left matrix offsets, so that no
+// descriptor load waits on another.  The task array ends with one
+// null task, so that an empty range still has a readable first entry.
+// Results do not depend on which workgroup runs a unit: each unit writes its
+// own partial sums.
+template <int D, typename TS>
+__global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3))) void k_trace_lstat_q(
+    const TS *__restrict__ sX, const long long *__restrict__ Xt, const TS *__restrict__ sY,
+    const long long *__restrict__ Yt, const int *__restrict__ units, int nunit, const LsTask *__restrict__ tasks,
+    const long long *__restrict__ slot_dst, double2 *__restrict__ out, long long obase, long long ostride, int ntime,
+    int rt, int *__restrict__ qcnt) {
+  constexpr int DD = D * D, EPL = DD / 64;
+  constexpr int PF = EPL <= 4 ? 4 : (EPL <= 8 ? 2 : 1);
+  constexpr int YR = kLsUnitHdr + kLsWaves * PF;  // offset of the staged left matrices in a record
+  constexpr int CL = kLsStageBytes / (DD * 16) < 64 - YR ? kLsStageBytes / (DD * 16) : 64 - YR;
+  constexpr int UR = YR + CL;  // ints per unit record, one per lane
+  static_assert(UR <= 64, "a unit record must fit one int per lane");
+  static_assert(kLsSlots <= 64 * kLsWaves, "one thread per slot");
+  __shared__ double2 Ls[CL * DD];
+  __shared__ double2 acc[kLsWaves][kLsSlots];
+  __shared__ int qslot[2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int total = nunit * ntime, per = total / 8, rem = total % 8, home = blockIdx.x % 8;
+  // Thread 0's claims: a fetch-and-add on counter (home + tried) % 8, whose
+  // value is a position in that counter's range; a spent range moves the
+  // claim on to the next counter (synchronously: at most eight times).
+  int tried = 0;
+  auto resolve = [&](int v) -> int {
+    for (;;) {
+      const int x = (home + tried) & 7;
+      if (v < per + (x < rem ? 1 : 0)) return x * per + min(x, rem) + v;
+      if (++tried == 8) return -1;
+      v = atomicAdd(&qcnt[(home + tried) & 7], 1);
+    }
+  };
+  for (int k = tid; k < kLsWaves * kLsSlots; k += 64 * kLsWaves) (&acc[0][0])[k] = make_double2(0.0, 0.0);
+  int qv = 0;
+  if (tid == 0) {
+    const int g0 = resolve(atomicAdd(&qcnt[home], 1));
+    const int g1 = g0 >= 0 ? resolve(atomicAdd(&qcnt[(home + tried) & 7], 1)) : -1;
+    if (g1 >= 0 && tried < 8) qv = atomicAdd(&qcnt[(home + tried) & 7], 1);
+    qslot[0] = g0;
+    qslot[1] = g1;
+  }
+  __syncthreads();
+  int g = __builtin_amdgcn_readfirstlane(qslot[0]), gn = __builtin_amdgcn_readfirstlane(qslot[1]);
+  if (g < 0) return;
+  // unit record (one int per lane), and the unit's time bases
+  auto rec = [&](int u) { return units[(long long)(u % nunit) * UR + min(lane, UR - 1)]; };
+  auto tbase = [&](const long long *tb, int u) { return tb[u / nunit]; };
+  int ud = rec(g), udn = rec(max(gn, 0));
+  long long xt = tbase(Xt, g), yt = tbase(Yt, g), xtn = tbase(Xt, max(gn, 0)), ytn = tbase(Yt, max(gn, 0));
+  constexpr int SPT = (CL * DD + 64 * kLsWaves - 1) / (64 * kLsWaves);
+  TS pre[SPT];
+  // a wave-uniform 64-bit value through the scalar unit
+  auto uni = [](long long v) {
+    return (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(int(v >> 32)) << 32) |
+                       (unsigned)__builtin_amdgcn_readfirstlane(int(v)));
+  };
+  auto fetch = [&](int r, long long xbase) {
+    const TS *xb = sX + uni(xbase) * DD;
+    const int n = __builtin_amdgcn_readlane(r, 0);
+    if constexpr (DD == 64 * kLsWaves) {  // element tid of matrix k
+#pragma unroll
+      for (int k = 0; k < SPT; ++k)
+        if (k < n) pre[k] = xb[(long long)__builtin_amdgcn_readlane(r, YR + k) * DD + tid];
+    } else {
+#pragma unroll
+      for (int k = 0; k < SPT; ++k) {
+        const int idx = tid + 64 * kLsWaves * k;
+        const int m = __builtin_amdgcn_readfirstlane(idx / DD);
+        if (m < n) pre[k] = xb[(long long)__builtin_amdgcn_readlane(r, YR + m) * DD + idx % DD];
+      }
+    }
+  };
+  auto commit = [&](int n) {
+    if constexpr (DD == 64 * kLsWaves) {
+      // element tid of matrix k: one LDS address, the matrices at constant offsets
+      double2 *dst = Ls + ls_swz(rt ? tid : tid / D + (tid % D) * D);
+#pragma unroll
+      for (int k = 0; k < SPT; ++k)
+        if (k < n) dst[k * DD] = ld2(&pre[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < SPT; ++k) {
+        const int idx = tid + 64 * kLsWaves * k, m = idx / DD, f = idx % DD;
+        if (idx < n * DD) Ls[m * DD + ls_swz(rt ? f : f / D + (f % D) * D)] = ld2(&pre[k]);
+      }
+    }
+  };
+  // one task batch: positions b0 + lane of this wave's range of a unit
+  // (lanes past the range repeat its last task; those positions are never read)
+  auto batch = [&](int r, int b0, int &qy, int &qab, int &qs) {
+    const int s0 = __builtin_amdgcn_readlane(r, 3 + wave), ns = __builtin_amdgcn_readlane(r, 3 + kLsWaves + wave);
+    const int *t = reinterpret_cast<const int *>(tasks + s0 + min(b0 + lane, max(ns - 1, 0)));
+    qy = t[0];
+    qab = t[1];
+    qs = t[2];
+  };
+  auto ybase = [&](long long yoff) { return sY + uni(yoff) * DD; };
+  // an empty use: the loads that produced these registers are waited for here
+  auto keep3 = [](int &a, int &b, int &c) { asm volatile("" : "+v"(a), "+v"(b), "+v"(c)); };
+  int so[EPL];  // swizzled LDS slots of this lane's elements lane + 64 k
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) so[k] = ls_swz(lane + 64 * k);
+  fetch(ud, xt);
+  int qy, qab, qs;
+  batch(ud, 0, qy, qab, qs);
+  double2 r[PF][EPL];
+  {
+    const TS *yb = ybase(yt);
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const long long y = __builtin_amdgcn_readlane(qy, u);
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) r[u][k] = ld2(yb + y * DD + lane + 64 * k);
+    }
+  }
+  int pti = 0, pso = 0, pns = 0;  // the previous unit's slots, summed at the next unit's start
+  for (;;) {
+    __syncthreads();  // every wave is done with the previous unit
+    if (tid < pns) {
+      double2 v = acc[0][tid];
+      acc[0][tid] = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int w = 1; w < kLsWaves; ++w) {
+        v = cadd(v, acc[w][tid]);
+        acc[w][tid] = make_double2(0.0, 0.0);
+      }
+      out[obase + pti * ostride + pso + tid] = v;
+    }
+    commit(__builtin_amdgcn_readlane(ud, 0));
+    // this unit's first batch and the next unit's record were loaded before
+    // this unit's left matrices, on which commit has just waited
+    keep3(qy, qab, qs);
+    keep3(ud, udn, udn);
+    if (tid == 0) qslot[0] = gn >= 0 && tried < 8 ? resolve(qv) : -1;
+    __syncthreads();
+    const int g2 = __builtin_amdgcn_readfirstlane(qslot[0]);
+    const int nsn = gn >= 0 ? __builtin_amdgcn_readlane(udn, 3 + kLsWaves + wave) : 0;
+    const TS *yb = ybase(yt), *ybn = ybase(ytn);
+    const int ns = __builtin_amdgcn_readlane(ud, 3 + kLsWaves + wave);
+    double re = 0.0, im = 0.0;
+    // Windows of 64 - PF stream positions share one descriptor batch (a
+    // round reads up to pos - bb + 2 PF - 1); inside a window the only loads
+    // are the ring refills, so that their waits stay exact.
+    for (int bb = 0; bb < ns; bb += 64 - PF) {
+      if (bb > 0) {
+        batch(ud, bb, qy, qab, qs);
+        keep3(qy, qab, qs);
+      }
+      const int wend = min(ns, bb + 64 - PF);
+      for (int pos = bb; pos < wend; pos += PF) {
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const int ab = __builtin_amdgcn_readlane(qab, pos + u - bb);
+        const int ia = (short)(ab & 0xffff), ib = ab >> 16;
+        if (ia >= 0) {
+          // tr(L_a R) + tr(L_b R) = sum_e (L_a + L_b)[e] R[e]
+          const double2 *la = Ls + ia * DD;
+          double2 w[EPL];
+#pragma unroll
+          for (int k = 0; k < EPL; ++k) w[k] = la[so[k]];
+          if (ib >= 0) {
+            const double2 *lb = Ls + ib * DD;
+#pragma unroll
+            for (int k = 0; k < EPL; ++k) {
+              const double2 b = lb[so[k]];
+              w[k].x += b.x;
+              w[k].y += b.y;
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < EPL; ++k) {
+            const double2 v = r[u][k];
+            re = fma(w[k].x, v.x, re);
+            re = fma(-w[k].y, v.y, re);
+            im = fma(w[k].x, v.y, im);
+            im = fma(w[k].y, v.x, im);
+          }
+          const int slot = __builtin_amdgcn_readlane(qs, pos + u - bb);
+          if (slot >= 0) {
+            re = wave_sum(re);
+            im = wave_sum(im);
+            if (lane == 0) acc[wave][slot] = cadd(acc[wave][slot], make_double2(re, im));
+            re = 0.0;
+            im = 0.0;
+          }
+        }
+        // refill this ring slot with the stream position PF ahead: this
+        // unit's task, else the next unit's, else (past the end) matrix 0
+        const int p = pos + u + PF;
+        const TS *base = p < ns ? yb : (p - ns < nsn ? ybn : yb);
+        const long long y =
+            p < ns ? __builtin_amdgcn_readlane(qy, p - bb)
+                   : (p - ns < nsn ? __builtin_amdgcn_readlane(udn, kLsUnitHdr + wave * PF + min(p - ns, PF - 1)) : 0);
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) r[u][k] = ld2(base + y * DD + lane + 64 * k);
+      }
+      }
+    }
+    if (ns == 0) {  // no task of this wave in the unit: the ring starts the next unit
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const TS *base = u < nsn ? ybn : yb;
+        const long long y = u < nsn ? __builtin_amdgcn_readlane(udn, kLsUnitHdr + wave * PF + u) : 0;
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) r[u][k] = ld2(base + y * DD + lane + 64 * k);
+      }
+    }
+    // Loads for later units are issued only now, behind this unit's last
+    // ring refills: vector loads complete in issue order, so a long-latency
+    // left matrix load issued earlier would hold up every ring wait after
+    // it.  The claim after next, the record and time bases of the unit after
+    // next, the next unit's first task batch and its left matrices:
+    if (tid == 0 && g2 >= 0 && tried < 8) qv = atomicAdd(&qcnt[(home + tried) & 7], 1);
+    const int ud2 = rec(max(g2, 0));
+    const long long xt2 = tbase(Xt, max(g2, 0)), yt2 = tbase(Yt, max(g2, 0));
+    int qyN = 0, qabN = -1, qsN = -1;
+    if (gn >= 0) {
+      batch(udn, 0, qyN, qabN, qsN);
+      fetch(udn, xtn);
+    }
+    pti = g / nunit;
+    pso = __builtin_amdgcn_readlane(ud, 1);
+    pns = __builtin_amdgcn_readlane(ud, 2);
+    if (gn < 0) break;
+    g = gn;
+    ud = udn;
+    xt = xtn;
+    yt = ytn;
+    qy = qyN;
+    qab = qabN;
+    qs = qsN;
+    gn = g2;
+    udn = ud2;
+    xtn = xt2;
+    ytn = yt2;
+  }
+  __syncthreads();
+  if (tid < pns) {
+    double2 v = acc[0][tid];
+#pragma unroll
+    for (int w = 1; w < kLsWaves; ++w) v = cadd(v, acc[w][tid]);
+    out[obase + pti * ostride + pso + tid] = v;
+  }
+}
+
+__global__ void k_ls_qreset(int *q) {
+  if (threadIdx.x < 8) q[threadIdx.x] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // Traces fused with their mask-group sums (reference: factor_to_trace,
 // DilutedTrace.cpp:84-165, followed by the sums inside inner_product /
@@ -1552,7 +1814,63 @@ void launch_trace_lstat_t(const void *sX, const long long *Xt, const void *sY, c
       throw std::runtime_error("launch_trace_lstat: unsupported factor dimension");
   }
 }
+template <int D, typename T>
+void launch_lstat_q_d(const T *x, const long long *Xt, const T *y, const long long *Yt, const int *units, int nunit,
+                      const LsTask *tasks, const long long *slot_dst, double2 *out, long long obase, long long ostride,
+                      int ntime, int rt, int *qcount, hipStream_t s) {
+  // one workgroup per resident slot (occupancy x compute units), at most one per unit
+  static int slots = 0;
+  if (slots == 0) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    SLAPH_HIP(hipGetDevice(&dev));
+    SLAPH_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    SLAPH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_lstat_q<D, T>, 64 * kLsWaves, 0));
+    slots = std::max(1, ncu * std::max(1, per_cu));
+  }
+  const long long total = (long long)nunit * ntime;
+  const unsigned grid = unsigned(std::min<long long>(total, slots));
+  hipLaunchKernelGGL(k_ls_qreset, dim3(1), dim3(64), 0, s, qcount);
+  hipLaunchKernelGGL((k_trace_lstat_q<D, T>), dim3(grid), dim3(64 * kLsWaves), 0, s, x, Xt, y, Yt, units, nunit, tasks,
+                     slot_dst, out, obase, ostride, ntime, rt, qcount);
+}
+
+template <typename T>
+void launch_trace_lstat_q_t(const void *sX, const long long *Xt, const void *sY, const long long *Yt, const int *units,
+                            int nunit, const LsTask *tasks, const long long *slot_dst, double2 *out, long long obase,
+                            long long ostride, int ntime, int D, int rt, int *qcount, hipStream_t s) {
+  const T *x = static_cast<const T *>(sX), *y = static_cast<const T *>(sY);
+  switch (D) {
+    case 8:
+      launch_lstat_q_d<8>(x, Xt, y, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, rt, qcount, s);
+      break;
+    case 16:
+      launch_lstat_q_d<16>(x, Xt, y, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, rt, qcount, s);
+      break;
+    case 24:
+      launch_lstat_q_d<24>(x, Xt, y, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, rt, qcount, s);
+      break;
+    case 32:
+      launch_lstat_q_d<32>(x, Xt, y, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, rt, qcount, s);
+      break;
+    default:
+      throw std::runtime_error("launch_trace_lstat_q: unsupported factor dimension");
+  }
+}
 }  // namespace
+
+void launch_trace_lstat_q(const void *sX, const long long *Xt, const void *sY, const long long *Yt, const int *units,
+                          int nunit, const LsTask *tasks, const long long *slot_dst, double2 *out, long long obase,
+                          long long ostride, int ntime, int D, int rt, int f32, int *qcount, hipStream_t s) {
+  if (nunit <= 0 || ntime <= 0) return;
+  if ((long long)nunit * ntime > 0x7fffffffLL) throw std::runtime_error("launch_trace_lstat_q: too many units");
+  if (f32)
+    launch_trace_lstat_q_t<float2>(sX, Xt, sY, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, D, rt,
+                                   qcount, s);
+  else
+    launch_trace_lstat_q_t<double2>(sX, Xt, sY, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, D, rt,
+                                    qcount, s);
+  SLAPH_HIP(hipGetLastError());
+}
 
 void launch_prod(const void *sX, const long long *Xt, const long long *Xq, const void *sY, const long long *Yt,
                  const long long *Yq, void *out, long long obase, long long ostride, const ComboProd *combos,
@@ -1577,6 +1895,11 @@ int trace_lstat_chunk(int D) {
 int trace_lstat_ring(int D) {
   const int epl = D * D / 64;
   return epl <= 4 ? 4 : (epl <= 8 ? 2 : 1);
+}
+
+int trace_lstat_qchunk(int D) {
+  if (!trace_lstat_supported(D)) return 0;
+  return std::min(trace_lstat_chunk(D), 64 - kLsUnitHdr - kLsWaves * trace_lstat_ring(D));
 }
 
 void launch_trace_lstat(const void *sX, const long long *Xt, const void *sY, const long long *Yt,
