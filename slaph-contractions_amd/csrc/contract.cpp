@@ -267,6 +267,10 @@ struct Family {
   bool ls_queue = false;
   std::vector<int> ls_units;
   DevBuf d_lsb, d_lsc, d_lst, d_lsd, d_lsl, d_lsr, d_lsri, d_lsu, d_lsq;
+  // right-stationary slice-pair tile plan (time independent), see launch_trace_grid_r
+  std::vector<GridRGroup> gr_groups;
+  std::vector<int> gr_runs;
+  DevBuf d_grg, d_grr;
 };
 
 struct CorrPlan {
@@ -655,6 +659,65 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
     upload(f.d_lsu, f.ls_units);
     f.d_lsq.ensure(8 * sizeof(int));
   }
+}
+
+// Right-stationary plan of a family's two-operand trace for the slice-pair
+// tile kernel: items grouped by their right operator entry, kGridIPW to a
+// group, in item order.  An item's runs are its pairs with one left
+// combination i (one or two right combinations, consecutive in the item's
+// list, as the item-wise kernel takes them); a group's run records are keyed
+// by the right combination(s), and each item fills its slot of the first
+// record with its key that it has not used yet.
+void plan_gridr(Family &f, NodePlan const &root) {
+  f.gr_groups.clear();
+  f.gr_runs.clear();
+  if (f.shape.nodes.back().kind != kTrace2 || f.gitems.empty() || std::getenv("SLAPH_GRID_ITEMS")) return;
+  std::map<long long, std::vector<int>> by_right;
+  for (size_t k = 0; k < f.gitems.size(); ++k) by_right[root.h_Yq.at(f.gitems[k].qn)].push_back(int(k));
+  for (auto const &kv : by_right) {
+    auto const &its = kv.second;
+    for (size_t a = 0; a < its.size(); a += kGridIPW) {
+      GridRGroup g{};
+      g.yq = kv.first;
+      g.nitem = int(std::min<size_t>(kGridIPW, its.size() - a));
+      g.run_off = int(f.gr_runs.size() / kGridRun);
+      std::map<std::pair<int, int>, std::vector<int>> recs;  // key -> record indices (relative)
+      int nrec = 0;
+      for (int w = 0; w < g.nitem; ++w) {
+        GroupItem const &it = f.gitems[its[a + w]];
+        g.dst[w] = it.dst;
+        const long long xq = root.h_Xq.at(it.qn);
+        int2 const *p = &f.gpairs[it.list_off];
+        for (int c = 0; c < it.n;) {
+          const bool two = c + 1 < it.n && p[c + 1].x == p[c].x;
+          const int j1 = p[c].y, j2 = two ? p[c + 1].y : -1;
+          const long long xo = xq + p[c].x;
+          if (xo > INT32_MAX) throw std::runtime_error("grid plan: left operand index overflows");
+          auto &lst = recs[{j1, j2}];
+          int rec = -1;
+          for (int r : lst)
+            if (f.gr_runs[(size_t)(g.run_off + r) * kGridRun + 2 + w] < 0) {
+              rec = r;
+              break;
+            }
+          if (rec < 0) {
+            rec = nrec++;
+            lst.push_back(rec);
+            f.gr_runs.insert(f.gr_runs.end(), kGridRun, -1);
+            f.gr_runs[(size_t)(g.run_off + rec) * kGridRun] = j1;
+            f.gr_runs[(size_t)(g.run_off + rec) * kGridRun + 1] = j2;
+          }
+          f.gr_runs[(size_t)(g.run_off + rec) * kGridRun + 2 + w] = int(xo);
+          c += two ? 2 : 1;
+        }
+      }
+      g.nrun = nrec;
+      f.gr_groups.push_back(g);
+    }
+  }
+  if (f.gr_runs.size() / kGridRun > INT32_MAX) throw std::runtime_error("grid plan: too many runs");
+  upload(f.d_grg, f.gr_groups);
+  upload(f.d_grr, f.gr_runs);
 }
 
 // Every (factor type, time key) a leaf of this family reads at slice pair
@@ -1101,6 +1164,7 @@ void Engine::contract(int single_block_pair, int kahan) {
       upload(f.d_gitems, f.gitems);
       upload(f.d_gpairs, f.gpairs);
       plan_lstat(f, root, D_);
+      plan_gridr(f, root);
     }
     for (auto &cp : corrs) {
       cp.inv_n.resize(cp.reqs.size());
@@ -1489,6 +1553,10 @@ void Engine::contract(int single_block_pair, int kahan) {
                       nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), C.d_prod.as<void>(), nc.base, np.stride,
                       np.d_combos.as<ComboProd>(), int(np.combos.size()), nc.ntime, D_, np.transposed ? 1 : 0, f32,
                       stream_);
+        } else if (nc.grid && !f.gr_groups.empty()) {
+          launch_trace_grid_r(src(ns.left), src(ns.right), f.d_grg.as<GridRGroup>(), int(f.gr_groups.size()),
+                              f.d_grr.as<int>(), nc.d_tile.as<GridTile>(), 1, C.d_gv.as<double2>(), gb, f.gstride,
+                              D_, rt, f32, stream_);
         } else if (nc.grid) {
           launch_trace_grid(src(ns.left), np.d_Xq.as<long long>(), src(ns.right), np.d_Yq.as<long long>(),
                             f.d_gitems.as<GroupItem>(), int(f.gitems.size()), f.d_gpairs.as<int2>(),

@@ -1452,6 +1452,83 @@ __global__ __launch_bounds__(256) void k_trace_grid(const T *__restrict__ sX, co
   }
 }
 
+// Right-stationary grid traces: one workgroup per (group of items that read
+// the same right entry, slice-pair tile).  For each run the right element
+// (or the sum of the two right elements of a partner pair) is loaded once and
+// multiplied into the tile of every item of the group that has the run, with
+// that item's left element: per k-step one or two right loads and one left
+// load per item feed four MFMAs per item.  Wave w takes the runs r with
+// r % 4 == w; the 4 partial tiles of each item are reduced through LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trace_grid_r(const T *__restrict__ sX, const T *__restrict__ sY,
+                                                      const GridRGroup *__restrict__ groups,
+                                                      const int *__restrict__ runs, const GridTile *__restrict__ tiles,
+                                                      double2 *__restrict__ out, long long obase, long long ostride,
+                                                      int D, int rt) {
+  const GridRGroup &G = groups[blockIdx.x];
+  const GridTile &Tl = tiles[blockIdx.y];
+  __shared__ double2 part[4 * 256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int DD = D * D, q = lane >> 4;
+  const int nL = Tl.nL, nR = Tl.nR, nitem = G.nitem, nrun = G.nrun;
+  const int la = min(lane & 15, nL - 1), lb = min(lane & 15, nR - 1);  // rows past the tile repeat its last
+  const T *xrow = sX + Tl.Lt[la] * DD;
+  const T *yrow = sY + (Tl.Rt[lb] + G.yq) * DD;
+  const int *rr0 = runs + (long long)G.run_off * kGridRun;
+  d4 cr[kGridIPW], ci[kGridIPW];
+#pragma unroll
+  for (int w = 0; w < kGridIPW; ++w) cr[w] = ci[w] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int r = wave; r < nrun; r += 4) {
+    const int *rr = rr0 + r * kGridRun;
+    const int j1 = rr[0], j2 = rr[1];
+    int xo[kGridIPW];
+#pragma unroll
+    for (int w = 0; w < kGridIPW; ++w) xo[w] = rr[2 + w];
+    const T *y0 = yrow + (long long)j1 * DD, *y1 = yrow + (long long)(j2 >= 0 ? j2 : j1) * DD;
+    const T *xw[kGridIPW];
+#pragma unroll
+    for (int w = 0; w < kGridIPW; ++w) xw[w] = xrow + (long long)max(xo[w], 0) * DD + q;
+    if (rt) {
+      y0 += q;
+      y1 += q;
+#pragma unroll 2
+      for (int s = 0; s < DD / 4; ++s) {
+        const double2 v = j2 >= 0 ? cadd(ld2(y0 + 4 * s), ld2(y1 + 4 * s)) : ld2(y0 + 4 * s);
+#pragma unroll
+        for (int w = 0; w < kGridIPW; ++w)
+          if (xo[w] >= 0) mfma_c(cr[w], ci[w], ld2(xw[w] + 4 * s), v);
+      }
+    } else {
+      for (int s = 0; s < DD / 4; ++s) {
+        const int e = 4 * s + q, er = (e % D) * D + e / D;
+        const double2 v = j2 >= 0 ? cadd(ld2(y0 + er), ld2(y1 + er)) : ld2(y0 + er);
+#pragma unroll
+        for (int w = 0; w < kGridIPW; ++w)
+          if (xo[w] >= 0) mfma_c(cr[w], ci[w], ld2(xw[w] + 4 * s), v);
+      }
+    }
+  }
+  // reduce the 4 waves' partial 16 x 16 tiles, item by item
+  const int a = tid / 16, b = tid % 16;
+  const int sp = (a < nL && b < nR) ? Tl.sidx[a][b] : -1;
+#pragma unroll
+  for (int w = 0; w < kGridIPW; ++w) {
+    if (w >= nitem) break;
+    if (w > 0) __syncthreads();  // the previous item's tile has been read
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int ra = (lane >> 4) + 4 * i, rb = lane & 15;
+      part[wave * 256 + ra * 16 + rb] = make_double2(cr[w][i], ci[w][i]);
+    }
+    __syncthreads();
+    double2 v = part[tid];
+    v = cadd(v, part[256 + tid]);
+    v = cadd(v, part[512 + tid]);
+    v = cadd(v, part[768 + tid]);
+    if (sp >= 0) out[obase + sp * ostride + G.dst[w]] = v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // resolve_request{1,2,3} (reference: Diagram.cpp:9-94, DilutedTrace.cpp:5-82)
 // on mask-group sums: mean, sum_g L_g sum_{h: m_g & m_h = 0} R_h, and the
@@ -1955,6 +2032,23 @@ void launch_trace_groups(const void *sX, const long long *Xt, const long long *X
   else
     hipLaunchKernelGGL(k_trace_groups<double2>, dim3(nitems, ntime), dim3(64), 0, s, static_cast<const double2 *>(sX),
                        Xt, Xq, static_cast<const double2 *>(sY), Yt, Yq, items, pairs, out, obase, ostride, D, rt);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_trace_grid_r(const void *sX, const void *sY, const GridRGroup *groups, int ngroups, const int *runs,
+                         const GridTile *tiles, int ntiles, double2 *out, long long obase, long long ostride, int D,
+                         int rt, int f32, hipStream_t s) {
+  if (ngroups <= 0 || ntiles <= 0) return;
+  if (D % 4 != 0 || D > kMaxD) throw std::runtime_error("launch_trace_grid_r: unsupported factor dimension");
+  if (ntiles > 65535) throw std::runtime_error("launch_trace_grid_r: too many tiles");
+  if (f32)
+    hipLaunchKernelGGL(k_trace_grid_r<float2>, dim3(ngroups, ntiles), dim3(256), 0, s,
+                       static_cast<const float2 *>(sX), static_cast<const float2 *>(sY), groups, runs, tiles, out,
+                       obase, ostride, D, rt);
+  else
+    hipLaunchKernelGGL(k_trace_grid_r<double2>, dim3(ngroups, ntiles), dim3(256), 0, s,
+                       static_cast<const double2 *>(sX), static_cast<const double2 *>(sY), groups, runs, tiles, out,
+                       obase, ostride, D, rt);
   SLAPH_HIP(hipGetLastError());
 }
 

@@ -204,6 +204,22 @@ struct GridTile {
 void launch_trace_grid(const void *slabX, const long long *Xq, const void *slabY, const long long *Yq,
                        const GroupItem *items, int nitems, const int2 *pairs, const GridTile *tiles, int ntiles,
                        double2 *out, long long obase, long long ostride, int D, int rt, int f32, hipStream_t s);
+// Right-stationary form of the same traces: a group is up to kGridIPW items
+// that read one right operator entry (offset yq).  Its runs are records of
+// kGridRun ints: the right combination(s) j1, j2 (j2 = -1: one matrix), then
+// per item the left matrix offset (left entry + combination) whose trace pairs
+// with R_j1 (+ R_j2), or -1.  Item w's tile goes to out[obase + sidx*ostride +
+// dst[w]].  Every right pair is read once per group instead of once per item.
+constexpr int kGridIPW = 4;
+constexpr int kGridRun = 2 + kGridIPW;
+struct GridRGroup {
+  long long yq;
+  int nitem, nrun, run_off, pad;
+  long long dst[kGridIPW];
+};
+void launch_trace_grid_r(const void *slabX, const void *slabY, const GridRGroup *groups, int ngroups, const int *runs,
+                         const GridTile *tiles, int ntiles, double2 *out, long long obase, long long ostride, int D,
+                         int rt, int f32, hipStream_t s);
 // Non-separable traces, left-stationary.  A block is a set of (trace id,
 // mask group) slots; its chunks stage left matrices X[Xt[ti] + left[loff + m]]
 // (m < n, any entries and combinations) and list tasks per wavefront.  A task

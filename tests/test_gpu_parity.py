@@ -154,8 +154,9 @@ WIDE = SMALL.replace("C4cB:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0\n", "C4cB:Q0:Op0:Q0:Op
     "C4cC:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0\n", "C4cC:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1\n")
 
 
-@pytest.mark.parametrize("infile,kernels", [(SMALL, "lstat+mfma"), (WIDE, "lstat+mfma"), (WIDE, "groups+generic")],
-                         ids=["small", "wide", "wide-alt-kernels"])
+@pytest.mark.parametrize("infile,kernels", [(SMALL, "lstat+mfma"), (WIDE, "lstat+mfma"), (WIDE, "groups+generic"),
+                                            (WIDE, "blocks+items")],
+                         ids=["small", "wide", "wide-alt-kernels", "wide-block-traces"])
 def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypatch):
     """Config-1-like lattice (4^3 x 8, N=8, dilE 4, 4 random vectors) with
     seeded host inputs: every correlator against the oracle at 1e-12
@@ -163,10 +164,14 @@ def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypat
     non-separable traces and the D = 16 products through both kernels each
     (left-stationary / one wave per group; MFMA / generic product), the
     masked inner products both by Moebius inversion and by direct group loops,
-    and VdaggerV both by the sitewise Fourier form and directly."""
+    and VdaggerV both by the sitewise Fourier form and directly.  The
+    block-traces variant runs the left-stationary traces in blocks of chunks
+    instead of the queued form, and the slice-pair tiles item by item instead
+    of grouped by right operand."""
     say("small pipi", "wide" if infile is WIDE else "", kernels)
     for var, on in (("SLAPH_TRACE_GROUPS", "groups" in kernels), ("SLAPH_PROD_GENERIC", "generic" in kernels),
-                    ("SLAPH_RESOLVE_GROUPS", "groups" in kernels), ("SLAPH_VDV_DIRECT", "groups" in kernels)):
+                    ("SLAPH_RESOLVE_GROUPS", "groups" in kernels), ("SLAPH_VDV_DIRECT", "groups" in kernels),
+                    ("SLAPH_LSTAT_BLOCKS", "blocks" in kernels), ("SLAPH_GRID_ITEMS", "items" in kernels)):
         if on:
             monkeypatch.setenv(var, "1")
         else:
