@@ -10,7 +10,7 @@ dilution TB16 / EI4 / DF4.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pipi32|pipi24|small]
 
 Prints one JSON line.  The roofline object is the largest kernel of the step,
-the left-stationary trace kernel k_trace_lstat (C4cC traces; HBM-streaming),
+the queued left-stationary trace kernel k_trace_lstat_q (C4cC traces),
 timed with HIP events around each of its launches on the stream it runs on;
 its algorithmic bytes per launch are the distinct product matrices the launch
 reads plus the partial sums it writes.  cpu_baseline is the numpy oracle timed
@@ -56,6 +56,10 @@ correlator_list = C4cB:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1,2,3,4
 correlator_list = C4cC:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1,2,3,4
 correlator_list = C6cD:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0
 """.format(**w)
+
+
+# the kernel of the roofline object (the block form runs under SLAPH_LSTAT_BLOCKS=1)
+LSTAT_KERNEL = "k_trace_lstat" if os.environ.get("SLAPH_LSTAT_BLOCKS") else "k_trace_lstat_q"
 
 
 def kernel_traffic(kernel, workload):
@@ -181,7 +185,7 @@ def main():
         for k in stage:
             stage[k] += ctx.stage_time(k) / args.steps
         for k in ls:
-            ls[k] += ctx.kernel_stat("k_trace_lstat", k)
+            ls[k] += ctx.kernel_stat(LSTAT_KERNEL, k)
         print("# step %d done" % i, file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     ms = elapsed / args.steps * 1000.0
@@ -209,9 +213,9 @@ def main():
                    "precision": "fp64"},
         "stages_ms": {"vdaggerv": stage["vdaggerv"], "factors": stage["factors"],
                       "contract_incl_factors": stage["contract"]},
-        "roofline": {"bound": "hbm", "kernel": "k_trace_lstat (C4cC traces)", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": LSTAT_KERNEL + " (C4cC traces)", "achieved": achieved,
                      "peak": PEAK_HBM, "unit": "GB/s", "frac": achieved / PEAK_HBM,
-                     "traffic": kernel_traffic("k_trace_lstat", args.workload),
+                     "traffic": kernel_traffic(LSTAT_KERNEL, args.workload),
                      "traffic_unit": "bytes per launch (HBM, rocprofv3 --pmc)",
                      "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": kern_ms,
                      "launches_per_step": ls["launches"] / args.steps},

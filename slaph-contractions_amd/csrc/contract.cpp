@@ -1517,6 +1517,7 @@ void Engine::contract(int single_block_pair, int kahan) {
   EventTimer ctimer(stream_);
   size_t ls_ev = 0;  // events used around left-stationary trace launches
   double ls_bytes = 0;
+  bool ls_queued = false;
   size_t last_bp = size_t(-1);
   for (auto const &ch : C.chunks) {
     if (progress && ch.bp != last_bp) {
@@ -1571,6 +1572,7 @@ void Engine::contract(int single_block_pair, int kahan) {
           }
           SLAPH_HIP(hipEventRecord(C.ev[ls_ev], stream_));
           ls_bytes += f.ls_bytes * nc.ntime;
+          ls_queued = f.ls_queue;
           if (f.ls_queue)
             launch_trace_lstat_q(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
                                  f.d_lsu.as<int>(), int(f.ls_blocks.size()), f.d_lst.as<LsTask>(),
@@ -1627,7 +1629,9 @@ void Engine::contract(int single_block_pair, int kahan) {
     }
     ks.launches = int(ls_ev / 2);
     ks.bytes = ls_bytes;
-    kstats_["k_trace_lstat"] = ks;
+    // keyed by the kernel the left-stationary traces ran on (the queued form
+    // unless SLAPH_LSTAT_BLOCKS selected blocks)
+    kstats_[ls_queued ? "k_trace_lstat_q" : "k_trace_lstat"] = ks;
   }
   for (size_t c = 0, off = 0; c < corrs.size(); off += (size_t)Lt_ * corrs[c].reqs.size(), ++c) {
     DiagramResult &r = results_[c];

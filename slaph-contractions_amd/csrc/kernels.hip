@@ -1459,8 +1459,11 @@ __global__ __launch_bounds__(256) void k_trace_grid(const T *__restrict__ sX, co
 // that item's left element: per k-step one or two right loads and one left
 // load per item feed four MFMAs per item.  Wave w takes the runs r with
 // r % 4 == w; the 4 partial tiles of each item are reduced through LDS.
+// Three waves per SIMD (164 registers): measured 113 ms per 32^3 x 64
+// configuration, 142 ms when the compiler is left to use more registers for
+// two waves per SIMD.
 template <typename T>
-__global__ __launch_bounds__(256) void k_trace_grid_r(const T *__restrict__ sX, const T *__restrict__ sY,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_trace_grid_r(const T *__restrict__ sX, const T *__restrict__ sY,
                                                       const GridRGroup *__restrict__ groups,
                                                       const int *__restrict__ runs, const GridTile *__restrict__ tiles,
                                                       double2 *__restrict__ out, long long obase, long long ostride,
