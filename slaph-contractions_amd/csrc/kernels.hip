@@ -1060,9 +1060,9 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3
 // counters once x's range is spent), so that the units running together on
 // one XCD are neighbours in plan order and find each other's right matrices
 // in that XCD's L2.  Claims run two units ahead: the ring of right matrices
-// runs on from unit g into unit g+1, and at the end of unit g the workgroup
-// loads unit g+1's left matrices into registers, its first task batch and
-// unit g+2's record.  A unit record is kLsUnitHdr ints (n, slot_off, nslot,
+// runs on from unit g into unit g+1; at the start of unit g the workgroup
+// loads unit g+1's first task batch and unit g+2's record, and at its end
+// unit g+1's left matrices (into registers).  A unit record is kLsUnitHdr ints (n, slot_off, nslot,
 // wbeg[kLsWaves], wcnt[kLsWaves], 0), each wave's first PF right matrices
 // (0 past the wave's range), then the unit's left matrix offsets, so that no
 // descriptor load waits on another.  The task array ends with one
@@ -1205,6 +1205,15 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3
     const int g2 = __builtin_amdgcn_readfirstlane(qslot[0]);
     const int nsn = gn >= 0 ? __builtin_amdgcn_readlane(udn, 3 + kLsWaves + wave) : 0;
     const TS *yb = ybase(yt), *ybn = ybase(ytn);
+    // The record and time bases of the unit after next and the next unit's
+    // first task batch (small loads that hit L2) go out now; its left
+    // matrices (long-latency loads) only behind this unit's last ring refill,
+    // since vector loads complete in issue order and each ring wait would
+    // otherwise wait for them too.
+    const int ud2 = rec(max(g2, 0));
+    const long long xt2 = tbase(Xt, max(g2, 0)), yt2 = tbase(Yt, max(g2, 0));
+    int qyN = 0, qabN = -1, qsN = -1;
+    if (gn >= 0) batch(udn, 0, qyN, qabN, qsN);
     const int ns = __builtin_amdgcn_readlane(ud, 3 + kLsWaves + wave);
     double re = 0.0, im = 0.0;
     // Windows of 64 - PF stream positions share one descriptor batch (a
@@ -1274,19 +1283,9 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3
         for (int k = 0; k < EPL; ++k) r[u][k] = ld2(base + y * DD + lane + 64 * k);
       }
     }
-    // Loads for later units are issued only now, behind this unit's last
-    // ring refills: vector loads complete in issue order, so a long-latency
-    // left matrix load issued earlier would hold up every ring wait after
-    // it.  The claim after next, the record and time bases of the unit after
-    // next, the next unit's first task batch and its left matrices:
+    // the claim after next, then the next unit's left matrices
     if (tid == 0 && g2 >= 0 && tried < 8) qv = atomicAdd(&qcnt[(home + tried) & 7], 1);
-    const int ud2 = rec(max(g2, 0));
-    const long long xt2 = tbase(Xt, max(g2, 0)), yt2 = tbase(Yt, max(g2, 0));
-    int qyN = 0, qabN = -1, qsN = -1;
-    if (gn >= 0) {
-      batch(udn, 0, qyN, qabN, qsN);
-      fetch(udn, xtn);
-    }
+    if (gn >= 0) fetch(udn, xtn);
     pti = g / nunit;
     pso = __builtin_amdgcn_readlane(ud, 1);
     pns = __builtin_amdgcn_readlane(ud, 2);
