@@ -662,57 +662,103 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
 }
 
 // Right-stationary plan of a family's two-operand trace for the slice-pair
-// tile kernel: items grouped by their right operator entry, kGridIPW to a
-// group, in item order.  An item's runs are its pairs with one left
-// combination i (one or two right combinations, consecutive in the item's
-// list, as the item-wise kernel takes them); a group's run records are keyed
-// by the right combination(s), and each item fills its slot of the first
-// record with its key that it has not used yet.
+// tile kernel.  Items are grouped per connected component of the (left
+// entry, right entry) trace graph: the component's left entries are cut into
+// blocks of kGridIPW, and for each block and each right entry one group holds
+// the items of that right entry with a left entry in the block (more than
+// kGridIPW such items spill into further groups).  Groups run block by block,
+// so the groups in flight on one XCD read the same few left entries.  An
+// item's runs are its pairs with one left combination i (one or two right
+// combinations, consecutive in the item's list, as the item-wise kernel takes
+// them); a group's run records are keyed by the right combination(s), and each
+// item fills its slot of the first record with its key that it has not used.
 void plan_gridr(Family &f, NodePlan const &root) {
   f.gr_groups.clear();
   f.gr_runs.clear();
   if (f.shape.nodes.back().kind != kTrace2 || f.gitems.empty() || std::getenv("SLAPH_GRID_ITEMS")) return;
-  std::map<long long, std::vector<int>> by_right;
-  for (size_t k = 0; k < f.gitems.size(); ++k) by_right[root.h_Yq.at(f.gitems[k].qn)].push_back(int(k));
-  for (auto const &kv : by_right) {
-    auto const &its = kv.second;
-    for (size_t a = 0; a < its.size(); a += kGridIPW) {
-      GridRGroup g{};
-      g.yq = kv.first;
-      g.nitem = int(std::min<size_t>(kGridIPW, its.size() - a));
-      g.run_off = int(f.gr_runs.size() / kGridRun);
-      std::map<std::pair<int, int>, std::vector<int>> recs;  // key -> record indices (relative)
-      int nrec = 0;
-      for (int w = 0; w < g.nitem; ++w) {
-        GroupItem const &it = f.gitems[its[a + w]];
-        g.dst[w] = it.dst;
-        const long long xq = root.h_Xq.at(it.qn);
-        int2 const *p = &f.gpairs[it.list_off];
-        for (int c = 0; c < it.n;) {
-          const bool two = c + 1 < it.n && p[c + 1].x == p[c].x;
-          const int j1 = p[c].y, j2 = two ? p[c + 1].y : -1;
-          const long long xo = xq + p[c].x;
-          if (xo > INT32_MAX) throw std::runtime_error("grid plan: left operand index overflows");
-          auto &lst = recs[{j1, j2}];
-          int rec = -1;
-          for (int r : lst)
-            if (f.gr_runs[(size_t)(g.run_off + r) * kGridRun + 2 + w] < 0) {
-              rec = r;
-              break;
-            }
-          if (rec < 0) {
-            rec = nrec++;
-            lst.push_back(rec);
-            f.gr_runs.insert(f.gr_runs.end(), kGridRun, -1);
-            f.gr_runs[(size_t)(g.run_off + rec) * kGridRun] = j1;
-            f.gr_runs[(size_t)(g.run_off + rec) * kGridRun + 1] = j2;
+  std::function<int(std::vector<int> &, int)> find = [&](std::vector<int> &par, int x) {
+    return par[x] == x ? x : par[x] = find(par, par[x]);
+  };
+  std::map<std::pair<int, long long>, int> node_id;
+  std::vector<int> parent;
+  auto node = [&](int side, long long off) {
+    auto it = node_id.emplace(std::make_pair(side, off), int(parent.size())).first;
+    if (it->second == int(parent.size())) parent.push_back(it->second);
+    return it->second;
+  };
+  for (auto const &it : f.gitems) {
+    int a = find(parent, node(0, root.h_Xq.at(it.qn))), b = find(parent, node(f.coincide ? 0 : 1, root.h_Yq.at(it.qn)));
+    if (a != b) parent[std::max(a, b)] = std::min(a, b);
+  }
+  // components in order of first appearance; per component its left entries
+  // (in order of first appearance) and its items
+  std::vector<int> comp_order;
+  std::map<int, std::vector<long long>> comp_left;
+  std::map<int, std::vector<int>> comp_items;
+  for (size_t k = 0; k < f.gitems.size(); ++k) {
+    const long long xq = root.h_Xq.at(f.gitems[k].qn);
+    const int c = find(parent, node(0, xq));
+    if (comp_items.find(c) == comp_items.end()) comp_order.push_back(c);
+    comp_items[c].push_back(int(k));
+    auto &cl = comp_left[c];
+    if (std::find(cl.begin(), cl.end(), xq) == cl.end()) cl.push_back(xq);
+  }
+  auto add_group = [&](long long yq, std::vector<int> const &its) {
+    GridRGroup g{};
+    g.yq = yq;
+    g.nitem = int(its.size());
+    g.run_off = int(f.gr_runs.size() / kGridRun);
+    std::map<std::pair<int, int>, std::vector<int>> recs;  // key -> record indices (relative)
+    int nrec = 0;
+    for (int w = 0; w < g.nitem; ++w) {
+      GroupItem const &it = f.gitems[its[w]];
+      g.dst[w] = it.dst;
+      const long long xq = root.h_Xq.at(it.qn);
+      int2 const *p = &f.gpairs[it.list_off];
+      for (int c = 0; c < it.n;) {
+        const bool two = c + 1 < it.n && p[c + 1].x == p[c].x;
+        const int j1 = p[c].y, j2 = two ? p[c + 1].y : -1;
+        const long long xo = xq + p[c].x;
+        if (xo > INT32_MAX) throw std::runtime_error("grid plan: left operand index overflows");
+        auto &lst = recs[{j1, j2}];
+        int rec = -1;
+        for (int r : lst)
+          if (f.gr_runs[(size_t)(g.run_off + r) * kGridRun + 2 + w] < 0) {
+            rec = r;
+            break;
           }
-          f.gr_runs[(size_t)(g.run_off + rec) * kGridRun + 2 + w] = int(xo);
-          c += two ? 2 : 1;
+        if (rec < 0) {
+          rec = nrec++;
+          lst.push_back(rec);
+          f.gr_runs.insert(f.gr_runs.end(), kGridRun, -1);
+          f.gr_runs[(size_t)(g.run_off + rec) * kGridRun] = j1;
+          f.gr_runs[(size_t)(g.run_off + rec) * kGridRun + 1] = j2;
         }
+        f.gr_runs[(size_t)(g.run_off + rec) * kGridRun + 2 + w] = int(xo);
+        c += two ? 2 : 1;
       }
-      g.nrun = nrec;
-      f.gr_groups.push_back(g);
+    }
+    g.nrun = nrec;
+    f.gr_groups.push_back(g);
+  };
+  for (int c : comp_order) {
+    auto const &cl = comp_left[c];
+    for (size_t b0 = 0; b0 < cl.size(); b0 += kGridIPW) {
+      std::set<long long> block(cl.begin() + b0, cl.begin() + std::min(cl.size(), b0 + kGridIPW));
+      std::map<long long, std::vector<int>> by_right;  // right entry -> items with a left entry in the block
+      std::vector<long long> right_order;
+      for (int k : comp_items[c]) {
+        GroupItem const &it = f.gitems[k];
+        if (!block.count(root.h_Xq.at(it.qn))) continue;
+        const long long yq = root.h_Yq.at(it.qn);
+        if (by_right.find(yq) == by_right.end()) right_order.push_back(yq);
+        by_right[yq].push_back(k);
+      }
+      for (long long yq : right_order) {
+        auto const &its = by_right[yq];
+        for (size_t a = 0; a < its.size(); a += kGridIPW)
+          add_group(yq, std::vector<int>(its.begin() + a, its.begin() + std::min(its.size(), a + kGridIPW)));
+      }
     }
   }
   if (f.gr_runs.size() / kGridRun > INT32_MAX) throw std::runtime_error("grid plan: too many runs");

@@ -1452,7 +1452,7 @@ __global__ __launch_bounds__(256) void k_trace_grid(const T *__restrict__ sX, co
 }
 
 // Right-stationary grid traces: one workgroup per (group of items that read
-// the same right entry, slice-pair tile).  For each run the right element
+// the same right entry, slice-pair tile), groups in plan order per XCD.  For each run the right element
 // (or the sum of the two right elements of a partner pair) is loaded once and
 // multiplied into the tile of every item of the group that has the run, with
 // that item's left element: per k-step one or two right loads and one left
@@ -1467,7 +1467,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                       const int *__restrict__ runs, const GridTile *__restrict__ tiles,
                                                       double2 *__restrict__ out, long long obase, long long ostride,
                                                       int D, int rt) {
-  const GridRGroup &G = groups[blockIdx.x];
+  // Workgroups are dealt round-robin to the 8 XCDs, each with its own L2:
+  // XCD x runs the x-th contiguous eighth of the groups, in which
+  // neighbouring groups read the same left entries.
+  const int NG = gridDim.x, per = NG / 8, rem = NG % 8, x = blockIdx.x % 8, jx = blockIdx.x / 8;
+  const int gi = x < rem ? x * (per + 1) + jx : rem * (per + 1) + (x - rem) * per + jx;
+  const GridRGroup &G = groups[gi];
   const GridTile &Tl = tiles[blockIdx.y];
   __shared__ double2 part[4 * 256];
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
