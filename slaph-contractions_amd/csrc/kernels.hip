@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void k_ahb(const AhBJob *__restrict__ jobs, in
 // rows (rpj = D) that share one right matrix B are stacked into one tall
 // A^H B, so a 64 x 64 tile holds 64 / D jobs; rows of job j come from
 // rows[job0 + j].A and go to rows[job0 + j].C scaled by its rowscale.
-__global__ __launch_bounds__(256) void k_ahb_grouped(const AhBGroup *__restrict__ groups,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_ahb_grouped(const AhBGroup *__restrict__ groups,
                                                      const AhBRow *__restrict__ rows, int rpj, int N, int K,
                                                      long long lda, long long ldb, long long ldc) {
   const AhBGroup G = groups[blockIdx.z];
@@ -177,12 +177,11 @@ __global__ __launch_bounds__(256) void k_ahb_grouped(const AhBGroup *__restrict_
   __shared__ double2 Bs[AHB_TN][AHB_KC + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  d4 cr[2][2], ci[2][2];
+  // conj(a) b by three real products (Gauss): P1 = ax bx, P2 = ay by,
+  // P3 = (ax - ay)(bx + by); re = P1 + P2, im = P3 - P1 + P2
+  d4 p1[2][2], p2[2][2], p3[2][2];
   for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) {
-      cr[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-      ci[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-    }
+    for (int b = 0; b < 2; ++b) p1[a][b] = p2[a][b] = p3[a][b] = d4{0.0, 0.0, 0.0, 0.0};
   for (int k0 = 0; k0 < K; k0 += AHB_KC) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -201,15 +200,15 @@ __global__ __launch_bounds__(256) void k_ahb_grouped(const AhBGroup *__restrict_
       const int kr = kq + (lane >> 4);
       const double2 av[2] = {As[wm * 32 + (lane & 15)][kr], As[wm * 32 + 16 + (lane & 15)][kr]};
       const double2 bv[2] = {Bs[wn * 32 + (lane & 15)][kr], Bs[wn * 32 + 16 + (lane & 15)][kr]};
+      const double am[2] = {av[0].x - av[0].y, av[1].x - av[1].y};
+      const double bp[2] = {bv[0].x + bv[0].y, bv[1].x + bv[1].y};
 #pragma unroll
       for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
         for (int ni = 0; ni < 2; ++ni) {
-          // conj(a) b = (ax bx + ay by) + i (ax by - ay bx)
-          cr[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi].x, bv[ni].x, cr[mi][ni], 0, 0, 0);
-          cr[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi].y, bv[ni].y, cr[mi][ni], 0, 0, 0);
-          ci[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi].x, bv[ni].y, ci[mi][ni], 0, 0, 0);
-          ci[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[mi].y, bv[ni].x, ci[mi][ni], 0, 0, 0);
+          p1[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi].x, bv[ni].x, p1[mi][ni], 0, 0, 0);
+          p2[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi].y, bv[ni].y, p2[mi][ni], 0, 0, 0);
+          p3[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[mi], bp[ni], p3[mi][ni], 0, 0, 0);
         }
     }
     __syncthreads();
@@ -224,7 +223,8 @@ __global__ __launch_bounds__(256) void k_ahb_grouped(const AhBGroup *__restrict_
         const int col = n0 + wn * 32 + ni * 16 + (lane & 15);
         if (row < M && col < N) {
           const AhBRow R = rows[G.job0 + row / rpj];
-          const double2 v = cmul(R.rowscale[row % rpj], make_double2(cr[mi][ni][i], ci[mi][ni][i]));
+          const double q1 = p1[mi][ni][i], q2 = p2[mi][ni][i];
+          const double2 v = cmul(R.rowscale[row % rpj], make_double2(q1 + q2, p3[mi][ni][i] - q1 + q2));
           R.C[(long long)col * ldc + row % rpj] = v;
         }
       }
