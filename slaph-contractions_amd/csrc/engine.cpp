@@ -549,7 +549,7 @@ void Engine::compute_q_offsets() {
 struct Engine::FactorJobs {
   std::vector<std::vector<int>> q1keys, q2keys;
   std::vector<const void *> sig;
-  DevBuf q0j, q1j, drs;
+  DevBuf q0g, q0o, q1j, drs;
   int nq0 = 0, nq1 = 0;
   struct PerT {
     DevBuf ag, ar, bj;  // stage-1 groups and rows, stage-2 jobs
@@ -591,18 +591,24 @@ void Engine::build_factors(std::vector<std::vector<int>> const &q1keys, std::vec
     fj->q1keys = q1keys;
     fj->q2keys = q2keys;
     {
-      std::vector<Q0Job> jobs;
+      // one group per (t, entry, r2): its jobs share X = VdV rho2
+      std::vector<Q0Group> groups;
+      std::vector<Q0Out> outs;
       for (int t = 0; t < Lt_; ++t)
-        for (size_t e = 0; e < lst0.size(); ++e)
-          for (size_t j = 0; j < lst0[e].rnd.size(); ++j) {
-            auto const &p = lst0[e].rnd[j];
-            jobs.push_back({vdv_ptr(lst0[e].id_vdaggerv, lst0[e].dagger, t), rho + p.first * rsz + (size_t)t * N_ * 4,
-                            rho + p.second * rsz + (size_t)t * N_ * 4,
-                            q0_.as<double2>() + ((long long)t * q_total_[0] + q_off_[0][e] + (long long)j) * DD,
-                            lst0[e].gamma.at(0)});
+        for (size_t e = 0; e < lst0.size(); ++e) {
+          std::map<int, std::vector<size_t>> by_r2;
+          for (size_t j = 0; j < lst0[e].rnd.size(); ++j) by_r2[lst0[e].rnd[j].second].push_back(j);
+          for (auto const &kv : by_r2) {
+            groups.push_back({vdv_ptr(lst0[e].id_vdaggerv, lst0[e].dagger, t), rho + kv.first * rsz + (size_t)t * N_ * 4,
+                              lst0[e].gamma.at(0), int(outs.size()), int(kv.second.size())});
+            for (size_t j : kv.second)
+              outs.push_back({rho + lst0[e].rnd[j].first * rsz + (size_t)t * N_ * 4,
+                              q0_.as<double2>() + ((long long)t * q_total_[0] + q_off_[0][e] + (long long)j) * DD});
           }
-      upload(fj->q0j, jobs);
-      fj->nq0 = int(jobs.size());
+        }
+      upload(fj->q0g, groups);
+      upload(fj->q0o, outs);
+      fj->nq0 = int(groups.size());
     }
     {
       std::vector<Q1Job> jobs;
@@ -695,7 +701,7 @@ void Engine::build_factors(std::vector<std::vector<int>> const &q1keys, std::vec
     fjobs_ = fj;
   }
   FactorJobs const &fj = *fjobs_;
-  launch_q0(fj.q0j.as<Q0Job>(), fj.nq0, N_, gd_.dilE(), stream_);
+  launch_q0(fj.q0g.as<Q0Group>(), fj.nq0, fj.q0o.as<Q0Out>(), N_, gd_.dilE(), stream_);
   launch_q1(fj.q1j.as<Q1Job>(), fj.nq1, N_, gd_.dilE(), prow, stream_);
   for (auto const &pt : fj.q2t) {
     if (pt.ng == 0) continue;

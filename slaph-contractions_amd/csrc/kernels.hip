@@ -641,34 +641,40 @@ __global__ __launch_bounds__(256) void k_vrow(const double2 *__restrict__ V, dou
 //   gval(d) sum_{v=e1 mod dilE} conj rho1(v, g(d)) sum_{w=e2 mod dilE} VdV[v,w] rho2(w,d)
 // (reference: DilutedFactorFactory<Q0>::build, DilutedFactorFactory.cpp:33-101)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_q0(const Q0Job *__restrict__ jobs, int N, int dilE) {
+// One workgroup per group of jobs that share VdV, rho2 and gamma: X is formed
+// once in LDS, then every job of the group reads it with its own rho1.
+__global__ __launch_bounds__(256) void k_q0(const Q0Group *__restrict__ groups, const Q0Out *__restrict__ outs,
+                                            int N, int dilE) {
   extern __shared__ double2 X[];  // [v][d][e2] : N * 4 * dilE
-  const Q0Job J = jobs[blockIdx.x];
+  const Q0Group G = groups[blockIdx.x];
   const int D = 4 * dilE;
   for (int idx = threadIdx.x; idx < N * dilE; idx += blockDim.x) {
     const int v = idx % N, e2 = idx / N;
     double2 acc[4];
     for (int d = 0; d < 4; ++d) acc[d] = make_double2(0.0, 0.0);
     for (int w = e2; w < N; w += dilE) {
-      const double2 x = J.vdv[(long long)w * N + v];
+      const double2 x = G.vdv[(long long)w * N + v];
 #pragma unroll
-      for (int d = 0; d < 4; ++d) cfma(acc[d], x, J.rho2[4 * w + d]);
+      for (int d = 0; d < 4; ++d) cfma(acc[d], x, G.rho2[4 * w + d]);
     }
     for (int d = 0; d < 4; ++d) X[(v * 4 + d) * dilE + e2] = acc[d];
   }
   __syncthreads();
-  for (int o = threadIdx.x; o < D * D; o += blockDim.x) {
-    const int row = o % D, col = o / D;
-    const int d = col / dilE, e2 = col % dilE;
-    const int g = c_gamma_row[J.gamma][d];
-    double2 s = make_double2(0.0, 0.0);
-    if (row / dilE == g) {
-      const int e1 = row % dilE;
-      for (int v = e1; v < N; v += dilE) cfma(s, make_double2(J.rho1[4 * v + g].x, -J.rho1[4 * v + g].y),
-                                              X[(v * 4 + d) * dilE + e2]);
-      s = cmul(c_gamma_val[J.gamma][d], s);
+  for (int jj = 0; jj < G.njob; ++jj) {
+    const Q0Out O = outs[G.job_off + jj];
+    for (int o = threadIdx.x; o < D * D; o += blockDim.x) {
+      const int row = o % D, col = o / D;
+      const int d = col / dilE, e2 = col % dilE;
+      const int g = c_gamma_row[G.gamma][d];
+      double2 s = make_double2(0.0, 0.0);
+      if (row / dilE == g) {
+        const int e1 = row % dilE;
+        for (int v = e1; v < N; v += dilE)
+          cfma(s, make_double2(O.rho1[4 * v + g].x, -O.rho1[4 * v + g].y), X[(v * 4 + d) * dilE + e2]);
+        s = cmul(c_gamma_val[G.gamma][d], s);
+      }
+      O.out[o] = s;
     }
-    J.out[o] = s;
   }
 }
 
@@ -737,7 +743,9 @@ __global__ __launch_bounds__(256) void k_q2b16(const Q2BJob *__restrict__ jobs, 
   if (job >= njobs) return;
   const Q2BJob J = jobs[job];
   const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
-  d4 cr = d4{0.0, 0.0, 0.0, 0.0}, ci = cr;
+  // (ax + i ay)(bx + i by) by three real products (Gauss): P1 = ax bx,
+  // P2 = ay by, P3 = (ax + ay)(bx + by); re = P1 - P2, im = P3 - P1 - P2
+  d4 p1 = d4{0.0, 0.0, 0.0, 0.0}, p2 = p1, p3 = p1;
   for (int bd = 0; bd < 4; ++bd) {
     const int g = c_gamma_row[J.gamma][bd];
     const double2 *pcol = J.peram2 + (long long)(J.b2 * 16 + col) * prow + (long long)(J.t * 4 + g) * N + q;
@@ -745,15 +753,14 @@ __global__ __launch_bounds__(256) void k_q2b16(const Q2BJob *__restrict__ jobs, 
 #pragma unroll 4
     for (int w0 = 0; w0 < N; w0 += 4) {
       const double2 a = mrow[(long long)w0 * 16], b = pcol[w0];
-      // (ax + i ay)(bx + i by) = (ax bx - ay by) + i (ax by + ay bx)
-      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, cr, 0, 0, 0);
-      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, cr, 0, 0, 0);
-      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, ci, 0, 0, 0);
-      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, ci, 0, 0, 0);
+      p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, p1, 0, 0, 0);
+      p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, p2, 0, 0, 0);
+      p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x + a.y, b.x + b.y, p3, 0, 0, 0);
     }
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) J.out[(lane >> 4) + 4 * i + 16 * col] = make_double2(cr[i], ci[i]);
+  for (int i = 0; i < 4; ++i)
+    J.out[(lane >> 4) + 4 * i + 16 * col] = make_double2(p1[i] - p2[i], p3[i] - p1[i] - p2[i]);
 }
 
 __global__ void k_dagger(const double2 *__restrict__ in, double2 *__restrict__ out, int N, int count) {
@@ -1808,12 +1815,12 @@ void launch_vrow(const double2 *V, double2 *Vr, long long K, int N, int Npad, in
   SLAPH_HIP(hipGetLastError());
 }
 
-void launch_q0(const Q0Job *d_jobs, int njobs, int N, int dilE, hipStream_t s) {
-  if (njobs <= 0) return;
+void launch_q0(const Q0Group *d_groups, int ngroups, const Q0Out *d_outs, int N, int dilE, hipStream_t s) {
+  if (ngroups <= 0) return;
   upload_gamma_constants();
   size_t lds = sizeof(double2) * N * 4 * dilE;
   if (lds > 160 * 1024) throw std::runtime_error("Q0: N*4*dilE too large for LDS");
-  hipLaunchKernelGGL(k_q0, dim3(njobs), dim3(256), lds, s, d_jobs, N, dilE);
+  hipLaunchKernelGGL(k_q0, dim3(ngroups), dim3(256), lds, s, d_groups, d_outs, N, dilE);
   SLAPH_HIP(hipGetLastError());
 }
 

@@ -99,14 +99,19 @@ void launch_vdv_xsum(const double2 *U, int nc, int ntp, int nt, int Lx, const in
 void launch_vrow(const double2 *V, double2 *Vr, long long K, int N, int Npad, int nt, hipStream_t s);
 
 // Q0 of one (t, operator, random pair): D x D.
-struct Q0Job {
+// Q0 in groups of jobs that share VdV, rho2 and gamma: X = VdV rho2 (diluted)
+// is formed once per group, then each job's D x D block from its rho1.
+struct Q0Group {
   const double2 *vdv;   // N x N (daggered if needed)
-  const double2 *rho1;  // random vector r1 at timeslice t: N x 4 (index 4v+d)
-  const double2 *rho2;
-  double2 *out;
+  const double2 *rho2;  // random vector r2 at timeslice t: N x 4 (index 4v+d)
   int gamma;
+  int job_off, njob;
 };
-void launch_q0(const Q0Job *d_jobs, int njobs, int N, int dilE, hipStream_t s);
+struct Q0Out {
+  const double2 *rho1;  // random vector r1 at timeslice t
+  double2 *out;
+};
+void launch_q0(const Q0Group *d_groups, int ngroups, const Q0Out *d_outs, int N, int dilE, hipStream_t s);
 
 // Q1 of one (t, b, operator, random pair): D x D.
 struct Q1Job {
