@@ -795,7 +795,8 @@ struct ChunkPlan {
   std::vector<long long> pbase;               // per family: offset of its partial trace sums
   long long p_total = 0;
   std::vector<std::vector<NodeChunk>> nodes;  // per family, per node
-  DevBuf d_td;                                // time delta per slice pair
+  DevBuf d_tds, d_tdg;                        // slice pairs grouped by time separation (ascending in a group)
+  int ntdg = 0;
   std::vector<DevBuf> d_jobs;                 // per diagram: resolve jobs
 };
 
@@ -1485,9 +1486,19 @@ void Engine::contract(int single_block_pair, int kahan) {
             }
           }
         }
-        std::vector<int> td(S);
-        for (int s = 0; s < S; ++s) td[s] = time_delta(sl[c0 + s].source, sl[c0 + s].sink, Lt_);
-        upload(ch.d_td, td);
+        {
+          std::map<int, std::vector<int>> by_t;
+          for (int s = 0; s < S; ++s) by_t[time_delta(sl[c0 + s].source, sl[c0 + s].sink, Lt_)].push_back(s);
+          std::vector<int> tds;
+          std::vector<int4> tdg;
+          for (auto const &kv : by_t) {
+            tdg.push_back(make_int4(kv.first, int(tds.size()), int(kv.second.size()), 0));
+            tds.insert(tds.end(), kv.second.begin(), kv.second.end());
+          }
+          upload(ch.d_tds, tds);
+          upload(ch.d_tdg, tdg);
+          ch.ntdg = int(tdg.size());
+        }
         ch.d_jobs.resize(corrs.size());
         for (size_t c = 0; c < corrs.size(); ++c) {
           auto const &cp = corrs[c];
@@ -1651,8 +1662,8 @@ void Engine::contract(int single_block_pair, int kahan) {
       launch_resolve(C.d_gv.as<double2>(), C.d_gmasks.as<uint16_t>(), ch.d_jobs[c].as<CorrJob>(), ncorr, S,
                      C.d_zv.as<double2>(), C.d_zmask.as<uint16_t>(), int(C.zmask.size()), C.d_cval.as<double2>(),
                      stream_);
-      launch_accumulate(C.d_cval.as<double2>(), ch.d_td.as<int>(), S, ncorr, d_sum[c].as<double2>(),
-                        d_comp[c].as<double2>(), kahan, stream_);
+      launch_accumulate(C.d_cval.as<double2>(), ch.d_tds.as<int>(), ch.d_tdg.as<int4>(), ch.ntdg, ncorr,
+                        d_sum[c].as<double2>(), d_comp[c].as<double2>(), kahan, stream_);
     }
   }
   // ---- results: correlator_[t][i].value() / Lt (Diagram.cpp:132-147),

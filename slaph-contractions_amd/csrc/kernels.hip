@@ -1623,20 +1623,26 @@ __global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__rest
 
 // correlator_[t][i] += c (reference: Diagram::assemble, Diagram.cpp:96-114);
 // Neumaier compensation (KahanAccumulator.hpp:23-32) when kahan != 0.
-__global__ void k_accumulate(const double2 *__restrict__ cval, const int *__restrict__ tdelta, int nsp, int ncorr,
-                             double2 *__restrict__ sum, double2 *__restrict__ comp, int kahan) {
-  const int corr = blockIdx.x * blockDim.x + threadIdx.x;
-  if (corr >= ncorr) return;
-  for (int s = 0; s < nsp; ++s) {
-    const long long at = (long long)tdelta[s] * ncorr + corr;
-    const double2 x = cval[(long long)s * ncorr + corr];
-    double2 acc = sum[at];
-    if (!kahan) {
-      sum[at] = cadd(acc, x);
-    } else {
+// One thread per (time separation, correlator): the slice pairs of that
+// separation are added in ascending order into a register copy of the sum,
+// the same sequence of additions as one pass over all slice pairs in order.
+__global__ void k_accumulate(const double2 *__restrict__ cval, const int *__restrict__ sidx,
+                             const int4 *__restrict__ tgroups, int ngroups, int ncorr, double2 *__restrict__ sum,
+                             double2 *__restrict__ comp, int kahan) {
+  const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (id >= (long long)ngroups * ncorr) return;
+  const int corr = int(id % ncorr);
+  const int4 G = tgroups[id / ncorr];
+  const long long at = (long long)G.x * ncorr + corr;
+  double2 acc = sum[at];
+  if (!kahan) {
+    for (int k = 0; k < G.z; ++k) acc = cadd(acc, cval[(long long)sidx[G.y + k] * ncorr + corr]);
+  } else {
+    double2 c = comp[at];
+    for (int k = 0; k < G.z; ++k) {
+      const double2 x = cval[(long long)sidx[G.y + k] * ncorr + corr];
       const double2 t = cadd(acc, x);
       const double na = hypot(acc.x, acc.y), nx = hypot(x.x, x.y);
-      double2 c = comp[at];
       if (na >= nx) {
         c.x += (acc.x - t.x) + x.x;
         c.y += (acc.y - t.y) + x.y;
@@ -1644,10 +1650,11 @@ __global__ void k_accumulate(const double2 *__restrict__ cval, const int *__rest
         c.x += (x.x - t.x) + acc.x;
         c.y += (x.y - t.y) + acc.y;
       }
-      comp[at] = c;
-      sum[at] = t;
+      acc = t;
     }
+    comp[at] = c;
   }
+  sum[at] = acc;
 }
 
 __global__ void k_finalize(const double2 *__restrict__ sum, const double2 *__restrict__ comp, int ncorr, int Lt,
@@ -2113,11 +2120,11 @@ void launch_resolve(const double2 *gv, const uint16_t *gm, const CorrJob *jobs, 
   SLAPH_HIP(hipGetLastError());
 }
 
-void launch_accumulate(const double2 *cval, const int *tdelta, int nsp, int ncorr, double2 *sum, double2 *comp,
-                       int kahan, hipStream_t s) {
-  if (ncorr <= 0 || nsp <= 0) return;
-  hipLaunchKernelGGL(k_accumulate, dim3((ncorr + 255) / 256), dim3(256), 0, s, cval, tdelta, nsp, ncorr, sum, comp,
-                     kahan);
+void launch_accumulate(const double2 *cval, const int *sidx, const int4 *tgroups, int ngroups, int ncorr,
+                       double2 *sum, double2 *comp, int kahan, hipStream_t s) {
+  if (ncorr <= 0 || ngroups <= 0) return;
+  hipLaunchKernelGGL(k_accumulate, grid1d((long long)ngroups * ncorr, 256), dim3(256), 0, s, cval, sidx, tgroups,
+                     ngroups, ncorr, sum, comp, kahan);
   SLAPH_HIP(hipGetLastError());
 }
 

@@ -276,8 +276,11 @@ void launch_ls_reduce(const double2 *part, long long pstride, const GroupItem *i
                       double2 *out, long long obase, long long ostride, int ntime, hipStream_t s);
 void launch_resolve(const double2 *gvals, const uint16_t *gmasks, const CorrJob *jobs, int ncorr, int nsp,
                     const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, hipStream_t s);
-void launch_accumulate(const double2 *cval, const int *tdelta, int nsp, int ncorr, double2 *sum, double2 *comp,
-                       int kahan, hipStream_t s);
+// sum[t*ncorr + c] (+ Kahan-Neumaier compensation comp) += cval[s*ncorr + c]
+// for the slice pairs s of each time separation t, in the order listed:
+// tgroups[g] = (t, offset into sidx, count), sidx ascending within a group.
+void launch_accumulate(const double2 *cval, const int *sidx, const int4 *tgroups, int ngroups, int ncorr,
+                       double2 *sum, double2 *comp, int kahan, hipStream_t s);
 void launch_scale(double2 *x, long long n, double f, hipStream_t s);
 // Correlator values (Diagram.cpp:132-147): out[i*Lt + t] = (sum + comp)[t*ncorr + i] / Lt.
 void launch_finalize(const double2 *sum, const double2 *comp, int ncorr, int Lt, double2 *out, hipStream_t s);
