@@ -376,7 +376,8 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
   // kLsBlockChunks chunks in entry-major order) runs one chunk per unit in
   // component -> piece -> entry order.
   f.ls_queue = std::getenv("SLAPH_LSTAT_BLOCKS") == nullptr;
-  const int CL = f.ls_queue ? trace_lstat_qchunk(D) : trace_lstat_chunk(D), PF = trace_lstat_ring(D);
+  const int CL = f.ls_queue ? trace_lstat_qchunk(D) : trace_lstat_chunk(D);
+  const int PF = f.ls_queue ? trace_lstat_qring(D) : trace_lstat_ring(D);
   if (f.shape.nodes.back().kind != kTrace2 || CL <= 0 || f.gitems.empty()) return;
   std::function<int(std::vector<int> &, int)> find = [&](std::vector<int> &par, int x) {
     return par[x] == x ? x : par[x] = find(par, par[x]);

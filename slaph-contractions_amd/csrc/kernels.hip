@@ -1077,13 +1077,13 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3
 // Results do not depend on which workgroup runs a unit: each unit writes its
 // own partial sums.
 template <int D, typename TS>
-__global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3))) void k_trace_lstat_q(
+__global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(2))) void k_trace_lstat_q(
     const TS *__restrict__ sX, const long long *__restrict__ Xt, const TS *__restrict__ sY,
     const long long *__restrict__ Yt, const int *__restrict__ units, int nunit, const LsTask *__restrict__ tasks,
     const long long *__restrict__ slot_dst, double2 *__restrict__ out, long long obase, long long ostride, int ntime,
     int rt, int *__restrict__ qcnt) {
   constexpr int DD = D * D, EPL = DD / 64;
-  constexpr int PF = EPL <= 4 ? 4 : (EPL <= 8 ? 2 : 1);
+  constexpr int PF = EPL <= 4 ? 6 : (EPL <= 8 ? 2 : 1);  // trace_lstat_qring(D)
   constexpr int YR = kLsUnitHdr + kLsWaves * PF;  // offset of the staged left matrices in a record
   constexpr int CL = kLsStageBytes / (DD * 16) < 64 - YR ? kLsStageBytes / (DD * 16) : 64 - YR;
   constexpr int UR = YR + CL;  // ints per unit record, one per lane
@@ -1223,15 +1223,17 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3
     if (gn >= 0) batch(udn, 0, qyN, qabN, qsN);
     const int ns = __builtin_amdgcn_readlane(ud, 3 + kLsWaves + wave);
     double re = 0.0, im = 0.0;
-    // Windows of 64 - PF stream positions share one descriptor batch (a
-    // round reads up to pos - bb + 2 PF - 1); inside a window the only loads
-    // are the ring refills, so that their waits stay exact.
-    for (int bb = 0; bb < ns; bb += 64 - PF) {
+    // Windows of WIN stream positions share one descriptor batch: a round
+    // reads up to pos - bb + 2 PF - 1 <= 63, and a window is a whole number
+    // of rounds.  Inside a window the only loads are the ring refills, so
+    // that their waits stay exact.
+    constexpr int WIN = PF * ((64 - PF) / PF);
+    for (int bb = 0; bb < ns; bb += WIN) {
       if (bb > 0) {
         batch(ud, bb, qy, qab, qs);
         keep3(qy, qab, qs);
       }
-      const int wend = min(ns, bb + 64 - PF);
+      const int wend = min(ns, bb + WIN);
       for (int pos = bb; pos < wend; pos += PF) {
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
@@ -1917,10 +1919,11 @@ void launch_lstat_q_d(const T *x, const long long *Xt, const T *y, const long lo
                       const LsTask *tasks, const long long *slot_dst, double2 *out, long long obase, long long ostride,
                       int ntime, int rt, int *qcount, hipStream_t s) {
   // Two workgroups per compute unit (fewer if fewer fit), at most one per
-  // unit of work.  Three fit, but with two the units in flight on one XCD
-  // span fewer component-pieces and more of their right matrices stay in
-  // its L2: on the 32^3 x 64 workload 512 workgroups run the launch in
-  // 14.5 ms, 768 in 16.6 ms, 384 in 16.5 ms.
+  // unit of work.  With two the units in flight on one XCD span few
+  // component-pieces and more of their right matrices stay in its L2: on the
+  // 32^3 x 64 workload (ring of 4 at three waves per SIMD) 512 workgroups ran
+  // the launch in 14.5 ms, 768 in 16.6 ms, 384 in 16.5 ms.  At two waves per
+  // SIMD the registers allow a ring of 6: 14.2 ms (8: 14.8 ms).
   static int slots = 0;
   if (slots == 0) {
     int dev = 0, ncu = 0, per_cu = 0;
@@ -1999,9 +2002,14 @@ int trace_lstat_ring(int D) {
   return epl <= 4 ? 4 : (epl <= 8 ? 2 : 1);
 }
 
+int trace_lstat_qring(int D) {
+  const int epl = D * D / 64;
+  return epl <= 4 ? 6 : (epl <= 8 ? 2 : 1);
+}
+
 int trace_lstat_qchunk(int D) {
   if (!trace_lstat_supported(D)) return 0;
-  return std::min(trace_lstat_chunk(D), 64 - kLsUnitHdr - kLsWaves * trace_lstat_ring(D));
+  return std::min(trace_lstat_chunk(D), 64 - kLsUnitHdr - kLsWaves * trace_lstat_qring(D));
 }
 
 void launch_trace_lstat(const void *sX, const long long *Xt, const void *sY, const long long *Yt,

@@ -257,15 +257,17 @@ void launch_trace_lstat(const void *slabX, const long long *Xt, const void *slab
                         const LsTask *tasks, const long long *slot_dst, double2 *out, long long obase,
                         long long ostride, int ntime, int D, int rt, int f32, hipStream_t s);
 // Queued form: every block has one chunk (at most trace_lstat_qchunk(D) left
-// matrices), given as a unit record of kLsUnitHdr ints (n, slot_off, nslot,
-// wbeg[kLsWaves], wcnt[kLsWaves], 0), then kLsWaves * trace_lstat_ring(D)
-// ints (each wave's first right matrices, 0 past its range), then
+// matrices), each wave's task count a multiple of trace_lstat_qring(D), given
+// as a unit record of kLsUnitHdr ints (n, slot_off, nslot, wbeg[kLsWaves],
+// wcnt[kLsWaves], 0), then kLsWaves * trace_lstat_qring(D) ints (each wave's
+// first right matrices, 0 past its range), then
 // trace_lstat_qchunk(D) left matrix offsets (the first n used).  Partial sum
 // k of a unit goes to out[obase + ti*ostride + slot_off + k].  The task array
 // ends with one null task.  qcount: 8 ints of device memory (one claim
 // counter per XCD), cleared by the launch.
 constexpr int kLsUnitHdr = 12;
 int trace_lstat_qchunk(int D);
+int trace_lstat_qring(int D);
 void launch_trace_lstat_q(const void *slabX, const long long *Xt, const void *slabY, const long long *Yt,
                           const int *units, int nunit, const LsTask *tasks, const long long *slot_dst, double2 *out,
                           long long obase, long long ostride, int ntime, int D, int rt, int f32, int *qcount,
