@@ -1081,7 +1081,7 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(2
     const TS *__restrict__ sX, const long long *__restrict__ Xt, const TS *__restrict__ sY,
     const long long *__restrict__ Yt, const int *__restrict__ units, int nunit, const LsTask *__restrict__ tasks,
     const long long *__restrict__ slot_dst, double2 *__restrict__ out, long long obase, long long ostride, int ntime,
-    int rt, int *__restrict__ qcnt) {
+    int rt, int *__restrict__ qcnt, int win) {
   constexpr int DD = D * D, EPL = DD / 64;
   constexpr int PF = EPL <= 4 ? 6 : (EPL <= 8 ? 2 : 1);  // trace_lstat_qring(D)
   constexpr int YR = kLsUnitHdr + kLsWaves * PF;  // offset of the staged left matrices in a record
@@ -1223,17 +1223,17 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(2
     if (gn >= 0) batch(udn, 0, qyN, qabN, qsN);
     const int ns = __builtin_amdgcn_readlane(ud, 3 + kLsWaves + wave);
     double re = 0.0, im = 0.0;
-    // Windows of WIN stream positions share one descriptor batch: a round
+    // Windows of win stream positions share one descriptor batch: a round
     // reads up to pos - bb + 2 PF - 1 <= 63, and a window is a whole number
-    // of rounds.  Inside a window the only loads are the ring refills, so
-    // that their waits stay exact.
-    constexpr int WIN = PF * ((64 - PF) / PF);
-    for (int bb = 0; bb < ns; bb += WIN) {
+    // of rounds (win a multiple of PF, at most 64 - PF; the launcher checks).
+    // Inside a window the only loads are the ring refills, so that their
+    // waits stay exact.
+    for (int bb = 0; bb < ns; bb += win) {
       if (bb > 0) {
         batch(ud, bb, qy, qab, qs);
         keep3(qy, qab, qs);
       }
-      const int wend = min(ns, bb + WIN);
+      const int wend = min(ns, bb + win);
       for (int pos = bb; pos < wend; pos += PF) {
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
@@ -1934,9 +1934,16 @@ void launch_lstat_q_d(const T *x, const long long *Xt, const T *y, const long lo
   }
   const long long total = (long long)nunit * ntime;
   const unsigned grid = unsigned(std::min<long long>(total, slots));
+  // descriptor window: the largest whole number of rounds that keeps a
+  // round's reads inside the 64-lane batch; SLAPH_LSTAT_WINDOW=k selects k
+  // rounds (tests use 1 to run the batch refresh at every round)
+  const int pf = trace_lstat_qring(D);
+  int win = pf * ((64 - pf) / pf);
+  if (const char *e = std::getenv("SLAPH_LSTAT_WINDOW")) win = pf * std::max(1, std::min(std::atoi(e), (64 - pf) / pf));
+  if (win % pf != 0 || win < pf || win > 64 - pf) throw std::logic_error("launch_trace_lstat_q: bad descriptor window");
   hipLaunchKernelGGL(k_ls_qreset, dim3(1), dim3(64), 0, s, qcount);
   hipLaunchKernelGGL((k_trace_lstat_q<D, T>), dim3(grid), dim3(64 * kLsWaves), 0, s, x, Xt, y, Yt, units, nunit, tasks,
-                     slot_dst, out, obase, ostride, ntime, rt, qcount);
+                     slot_dst, out, obase, ostride, ntime, rt, qcount, win);
 }
 
 template <typename T>

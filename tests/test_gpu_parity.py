@@ -155,8 +155,8 @@ WIDE = SMALL.replace("C4cB:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0\n", "C4cB:Q0:Op0:Q0:Op
 
 
 @pytest.mark.parametrize("infile,kernels", [(SMALL, "lstat+mfma"), (WIDE, "lstat+mfma"), (WIDE, "groups+generic"),
-                                            (WIDE, "blocks+items")],
-                         ids=["small", "wide", "wide-alt-kernels", "wide-block-traces"])
+                                            (WIDE, "blocks+items"), (WIDE, "lstat+mfma+window1")],
+                         ids=["small", "wide", "wide-alt-kernels", "wide-block-traces", "wide-one-round-windows"])
 def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypatch):
     """Config-1-like lattice (4^3 x 8, N=8, dilE 4, 4 random vectors) with
     seeded host inputs: every correlator against the oracle at 1e-12
@@ -167,11 +167,15 @@ def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypat
     and VdaggerV both by the sitewise Fourier form and directly.  The
     block-traces variant runs the left-stationary traces in blocks of chunks
     instead of the queued form, and the slice-pair tiles item by item instead
-    of grouped by right operand."""
+    of grouped by right operand.  The one-round-windows variant reloads the
+    queued form's task descriptors at every round of its ring (the largest
+    units of the 32^3 x 64 workload fit one window, so only this variant
+    runs that reload)."""
     say("small pipi", "wide" if infile is WIDE else "", kernels)
     for var, on in (("SLAPH_TRACE_GROUPS", "groups" in kernels), ("SLAPH_PROD_GENERIC", "generic" in kernels),
                     ("SLAPH_RESOLVE_GROUPS", "groups" in kernels), ("SLAPH_VDV_DIRECT", "groups" in kernels),
-                    ("SLAPH_LSTAT_BLOCKS", "blocks" in kernels), ("SLAPH_GRID_ITEMS", "items" in kernels)):
+                    ("SLAPH_LSTAT_BLOCKS", "blocks" in kernels), ("SLAPH_GRID_ITEMS", "items" in kernels),
+                    ("SLAPH_LSTAT_WINDOW", "window1" in kernels)):
         if on:
             monkeypatch.setenv(var, "1")
         else:
