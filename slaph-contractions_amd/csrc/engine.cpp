@@ -504,7 +504,8 @@ void Engine::vdaggerv_fourier(std::vector<int> const &ids) {
     launch_vrow(V_.as<double2>() + (size_t)t0 * K_ * N_, vr_.as<double2>(), K_, N_, F.Npad, nt, stream_);
     ++launches;
     for (auto &P : F.passes) {
-      launch_vdv_ft(vr_.as<double2>(), K_ * F.Npad, F.Npad, Lx, Ly, Lz, F.tiles.as<int2>(), F.ntp, nt, P.p,
+      launch_vdv_ft(vr_.as<double2>(), K_ * F.Npad, FtLayout{F.Npad, 1, F.Npad}, Lx, Ly, Lz, F.tiles.as<int2>(), F.ntp,
+                    nt, P.p,
                     P.ey.as<double2>(), P.zcs.as<double2>(), ftu_.as<double2>(), stream_);
       launch_vdv_xsum(ftu_.as<double2>(), P.p.nc, F.ntp, nt, Lx, F.tiles.as<int2>(), P.tg.as<FtTarget>(),
                       int(P.targets.size()), P.ex.as<double2>(), N_, t0, (long long)N_ * N_, stream_);

@@ -376,7 +376,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // next four sites are loaded while the current four are processed.
 // ---------------------------------------------------------------------------
 template <int NC, int NK, int H, bool kFast>
-__device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, int Npad, int I, int J, int Ly, int Lz,
+__device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, FtLayout lay, int I, int J, int Ly, int Lz,
                                             long long site0, FtPass const &P, const double2 *__restrict__ ey,
                                             const double2 *zl, double2 *__restrict__ Uo, long long ustride) {
   const int lane = threadIdx.x & 63, c = lane >> 4, m = lane & 15;
@@ -384,9 +384,9 @@ __device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, int 
   // operand, so that every load is unconditional.
   const double amask = c < 3 ? 1.0 : 0.0;
   const int cl = c < 3 ? c : 2;
-  const long long sstep = 3LL * Npad;
-  const double2 *pa = Vr + (site0 * 3 + cl) * Npad + I * 16 + m;
-  const double2 *pb = Vr + (site0 * 3 + cl) * Npad + J * 16 + m;
+  const long long sstep = 3LL * lay.kstride;
+  const double2 *pa = Vr + (site0 * 3 + cl) * lay.kstride + (long long)min(I * 16 + m, lay.ncol - 1) * lay.nstride;
+  const double2 *pb = Vr + (site0 * 3 + cl) * lay.kstride + (long long)min(J * 16 + m, lay.ncol - 1) * lay.nstride;
   const double2 zero2 = make_double2(0.0, 0.0);
   double2 U[NC][2];
 #pragma unroll
@@ -557,7 +557,7 @@ __device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, int 
 // registers per lane and runs at one wave per SIMD.
 template <int NC, int NK, bool kFast>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NC > 9 ? 1 : 2, 2))) void k_vdv_ft(
-    const double2 *__restrict__ Vr, long long vstride, int Npad, int Lx, int Ly, int Lz,
+    const double2 *__restrict__ Vr, long long vstride, FtLayout lay, int Lx, int Ly, int Lz,
     const int2 *__restrict__ tiles, int ntp, FtPass P, const double2 *__restrict__ ey,
     const double2 *__restrict__ zcs, double2 *__restrict__ U) {
   // XCD x runs the x-th contiguous eighth of the (t, x, tile pair) blocks, so
@@ -575,9 +575,9 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NC > 9 ? 1 
   __syncthreads();
   const double2 *V = Vr + t * vstride;
   if (threadIdx.x < 64)
-    vdv_ft_wave<NC, NK, 0, kFast>(V, Npad, ij.x, ij.y, Ly, Lz, site0, P, ey, zl, Uo, ustride);
+    vdv_ft_wave<NC, NK, 0, kFast>(V, lay, ij.x, ij.y, Ly, Lz, site0, P, ey, zl, Uo, ustride);
   else
-    vdv_ft_wave<NC, NK, 1, kFast>(V, Npad, ij.x, ij.y, Ly, Lz, site0, P, ey, zl, Uo, ustride);
+    vdv_ft_wave<NC, NK, 1, kFast>(V, lay, ij.x, ij.y, Ly, Lz, site0, P, ey, zl, Uo, ustride);
 }
 
 // C(p) from the per-x sums: thread = (row, column) of one tile pair.
@@ -1776,13 +1776,13 @@ void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, int f32, hi
   SLAPH_HIP(hipGetLastError());
 }
 
-void launch_vdv_ft(const double2 *Vr, long long vstride, int Npad, int Lx, int Ly, int Lz, const int2 *tiles,
+void launch_vdv_ft(const double2 *Vr, long long vstride, FtLayout lay, int Lx, int Ly, int Lz, const int2 *tiles,
                    int ntp, int nt, FtPass const &pass, const double2 *ey, const double2 *zcs, double2 *U,
                    hipStream_t s) {
   if (nt <= 0 || ntp <= 0 || pass.nc <= 0) return;
   if (pass.nc > kFtMaxC || pass.nk < 0 || pass.nk > kFtMaxK)
     throw std::runtime_error("launch_vdv_ft: pass exceeds the combination limits");
-  if (Npad % 16 != 0) throw std::runtime_error("launch_vdv_ft: Npad must be a multiple of 16");
+  if (lay.ncol < 1) throw std::runtime_error("launch_vdv_ft: no columns");
   if (Lz > kFtMaxLz) throw std::runtime_error("launch_vdv_ft: Lz above kFtMaxLz");
   const long long blocks = (long long)nt * Lx * ntp;
   if (blocks > 0x7fffffffLL) throw std::runtime_error("launch_vdv_ft: grid too large");
@@ -1790,18 +1790,18 @@ void launch_vdv_ft(const double2 *Vr, long long vstride, int Npad, int Lx, int L
   const bool fast = (Lz & 3) == 0;
   if (pass.nc <= 9 && pass.nk <= 1) {
     if (fast)
-      hipLaunchKernelGGL((k_vdv_ft<9, 1, true>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride, Npad, Lx, Ly,
+      hipLaunchKernelGGL((k_vdv_ft<9, 1, true>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride, lay, Lx, Ly,
                          Lz, tiles, ntp, pass, ey, zcs, U);
     else
-      hipLaunchKernelGGL((k_vdv_ft<9, 1, false>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride, Npad, Lx, Ly,
+      hipLaunchKernelGGL((k_vdv_ft<9, 1, false>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride, lay, Lx, Ly,
                          Lz, tiles, ntp, pass, ey, zcs, U);
   } else {
     if (fast)
       hipLaunchKernelGGL((k_vdv_ft<kFtMaxC, kFtMaxK, true>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride,
-                         Npad, Lx, Ly, Lz, tiles, ntp, pass, ey, zcs, U);
+                         lay, Lx, Ly, Lz, tiles, ntp, pass, ey, zcs, U);
     else
       hipLaunchKernelGGL((k_vdv_ft<kFtMaxC, kFtMaxK, false>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride,
-                         Npad, Lx, Ly, Lz, tiles, ntp, pass, ey, zcs, U);
+                         lay, Lx, Ly, Lz, tiles, ntp, pass, ey, zcs, U);
   }
   SLAPH_HIP(hipGetLastError());
 }

@@ -68,11 +68,20 @@ void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, int f32, hi
 // k_vdv_xsum sums U over x with the px phases into the N x N matrices.  Only
 // tile pairs I <= J are formed: C(p)[j,i] = conj(C(-p)[i,j]) supplies the
 // lower triangle.
-// V is read in a row-major copy Vr (K x Npad, Npad = 16 * ceil(N / 16),
-// padding columns zero) made by launch_vrow from the column-major input.
+// V(t) = V + t * vstride is read through FtLayout: element (k, n) (k = 3
+// site + colour, eigenvector n) at k * kstride + n * nstride; columns n >=
+// ncol of the last 16-column tile read column ncol - 1 (their outputs are
+// never stored).  The engine reads a row-major copy made by launch_vrow,
+// {Npad, 1, Npad}: reading the column-major input in place ({1, K, N}) costs
+// 16 cache lines per load instead of 8 and was 11 ms slower per 32^3 x 64
+// configuration than the copy (8 ms).
 constexpr int kFtMaxC = 13;  // (py, pz) combinations of one pass
 constexpr int kFtMaxK = 2;   // nonzero |pz| values of one pass
 constexpr int kFtMaxLz = 128;  // largest Lz (z phases staged in LDS)
+struct FtLayout {
+  long long kstride, nstride;
+  int ncol;
+};
 struct FtPass {
   int nc, nk;
   // Per combination, which z transform it reads: 0 = pz 0, 1 + 2s = +k_s,
@@ -81,7 +90,7 @@ struct FtPass {
 };
 // ey: nc x Ly phases exp(-i 2 pi py y / Ly); zcs: nk x Lz pairs
 // (cos, sin)(2 pi k_s z / Lz).  U: [t][x][combination][tile pair][256].
-void launch_vdv_ft(const double2 *Vr, long long vstride, int Npad, int Lx, int Ly, int Lz, const int2 *tiles,
+void launch_vdv_ft(const double2 *V, long long vstride, FtLayout lay, int Lx, int Ly, int Lz, const int2 *tiles,
                    int ntp, int nt, FtPass const &pass, const double2 *ey, const double2 *zcs, double2 *U,
                    hipStream_t s);
 // One output of the x sum: combination c of the pass, phases ex[k * Lx + x]
