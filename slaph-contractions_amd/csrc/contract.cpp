@@ -271,6 +271,10 @@ struct Family {
   std::vector<GridRGroup> gr_groups;
   std::vector<int> gr_runs;
   DevBuf d_grg, d_grr;
+  // GEMM form of the slice-pair tiles (every item with the same runs), see launch_trace_gemm
+  std::vector<GridGTile> gg_tiles;
+  std::vector<int> gg_runs;
+  DevBuf d_ggt, d_ggr;
 };
 
 struct CorrPlan {
@@ -767,6 +771,94 @@ void plan_gridr(Family &f, NodePlan const &root) {
   upload(f.d_grr, f.gr_runs);
 }
 
+// GEMM plan of a family's two-operand trace for the slice-pair tile kernel,
+// when every item has the same runs (an item's pairs with one left
+// combination i, one or two right combinations, consecutive in its list) and
+// no two items share (left entry, right entry): per connected component of
+// the trace graph, its left and right entries in blocks of kGemmT, one tile
+// per pair of blocks that holds an item.  Otherwise the plan stays empty and
+// the right-stationary form runs.
+void plan_gridg(Family &f, NodePlan const &root) {
+  f.gg_tiles.clear();
+  f.gg_runs.clear();
+  if (f.shape.nodes.back().kind != kTrace2 || f.gitems.empty() || std::getenv("SLAPH_GRID_ITEMS") ||
+      std::getenv("SLAPH_GRID_GROUPS"))
+    return;
+  auto runs_of = [&](GroupItem const &it) {
+    std::vector<int> r;
+    int2 const *p = &f.gpairs[it.list_off];
+    for (int c = 0; c < it.n;) {
+      const bool two = c + 1 < it.n && p[c + 1].x == p[c].x;
+      r.push_back(p[c].x);
+      r.push_back(p[c].y);
+      r.push_back(two ? p[c + 1].y : -1);
+      c += two ? 2 : 1;
+    }
+    return r;
+  };
+  const std::vector<int> runs = runs_of(f.gitems[0]);
+  std::map<std::pair<long long, long long>, long long> item_of;  // (xq, yq) -> dst
+  for (auto const &it : f.gitems) {
+    if (runs_of(it) != runs) return;
+    auto key = std::make_pair(root.h_Xq.at(it.qn), root.h_Yq.at(it.qn));
+    if (!item_of.emplace(key, it.dst).second) return;
+  }
+  std::function<int(std::vector<int> &, int)> find = [&](std::vector<int> &par, int x) {
+    return par[x] == x ? x : par[x] = find(par, par[x]);
+  };
+  std::map<std::pair<int, long long>, int> node_id;
+  std::vector<int> parent;
+  auto node = [&](int side, long long off) {
+    auto it = node_id.emplace(std::make_pair(side, off), int(parent.size())).first;
+    if (it->second == int(parent.size())) parent.push_back(it->second);
+    return it->second;
+  };
+  for (auto const &it : f.gitems) {
+    int a = find(parent, node(0, root.h_Xq.at(it.qn))), b = find(parent, node(f.coincide ? 0 : 1, root.h_Yq.at(it.qn)));
+    if (a != b) parent[std::max(a, b)] = std::min(a, b);
+  }
+  std::vector<int> comp_order;
+  std::map<int, std::vector<long long>> comp_left, comp_right;
+  for (auto const &it : f.gitems) {
+    const long long xq = root.h_Xq.at(it.qn), yq = root.h_Yq.at(it.qn);
+    const int c = find(parent, node(0, xq));
+    if (comp_left.find(c) == comp_left.end()) comp_order.push_back(c);
+    auto &cl = comp_left[c];
+    if (std::find(cl.begin(), cl.end(), xq) == cl.end()) cl.push_back(xq);
+    auto &cr = comp_right[c];
+    if (std::find(cr.begin(), cr.end(), yq) == cr.end()) cr.push_back(yq);
+  }
+  for (int c : comp_order) {
+    auto const &cl = comp_left[c], &cr = comp_right[c];
+    for (size_t e0 = 0; e0 < cl.size(); e0 += kGemmT)
+      for (size_t f0 = 0; f0 < cr.size(); f0 += kGemmT) {
+        GridGTile t{};
+        t.nE = int(std::min<size_t>(kGemmT, cl.size() - e0));
+        t.nF = int(std::min<size_t>(kGemmT, cr.size() - f0));
+        bool any = false;
+        for (int r = 0; r < kGemmT; ++r) {
+          t.xq[r] = cl[e0 + std::min(r, t.nE - 1)];
+          t.yq[r] = cr[f0 + std::min(r, t.nF - 1)];
+        }
+        for (int r = 0; r < kGemmT; ++r)
+          for (int q = 0; q < kGemmT; ++q) {
+            t.dst[r][q] = -1;
+            if (r < t.nE && q < t.nF) {
+              auto it = item_of.find({t.xq[r], t.yq[q]});
+              if (it != item_of.end()) {
+                t.dst[r][q] = it->second;
+                any = true;
+              }
+            }
+          }
+        if (any) f.gg_tiles.push_back(t);
+      }
+  }
+  f.gg_runs = runs;
+  upload(f.d_ggt, f.gg_tiles);
+  upload(f.d_ggr, f.gg_runs);
+}
+
 // Every (factor type, time key) a leaf of this family reads at slice pair
 // (t_so, t_si); the slab index is returned through `base`.
 std::vector<int> trace_times(Family const &f, int tso, int tsi) {
@@ -1213,6 +1305,7 @@ void Engine::contract(int single_block_pair, int kahan) {
       upload(f.d_gpairs, f.gpairs);
       plan_lstat(f, root, D_);
       plan_gridr(f, root);
+      plan_gridg(f, root);
     }
     for (auto &cp : corrs) {
       cp.inv_n.resize(cp.reqs.size());
@@ -1612,6 +1705,10 @@ void Engine::contract(int single_block_pair, int kahan) {
                       nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), C.d_prod.as<void>(), nc.base, np.stride,
                       np.d_combos.as<ComboProd>(), int(np.combos.size()), nc.ntime, D_, np.transposed ? 1 : 0, f32,
                       stream_);
+        } else if (nc.grid && !f.gg_tiles.empty()) {
+          launch_trace_gemm(src(ns.left), src(ns.right), f.d_ggt.as<GridGTile>(), int(f.gg_tiles.size()),
+                            f.d_ggr.as<int>(), int(f.gg_runs.size() / 3), nc.d_tile.as<GridTile>(), C.d_gv.as<double2>(),
+                            gb, f.gstride, D_, rt, f32, stream_);
         } else if (nc.grid && !f.gr_groups.empty()) {
           launch_trace_grid_r(src(ns.left), src(ns.right), f.d_grg.as<GridRGroup>(), int(f.gr_groups.size()),
                               f.d_grr.as<int>(), nc.d_tile.as<GridTile>(), 1, C.d_gv.as<double2>(), gb, f.gstride,

@@ -1545,6 +1545,102 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
 }
 
+// GEMM form of the slice-pair tile traces: one workgroup per tile of
+// kGemmT x kGemmT (left entry, right entry) pairs, i.e. a 64 x 64 output
+// block, rows (entry r, left time a), columns (entry c, right time b).  K runs
+// over (run, matrix element) in chunks of 16 elements of one run: each thread
+// loads 4 elements of one row of L and of one column of R (the two right
+// matrices of the run added), the chunk goes through LDS, and the four waves
+// (2 x 2, 32 x 32 each) multiply it with FP64 MFMA by three real products
+// (Gauss).  The next chunk's loads are issued before the current chunk's
+// MFMAs.  Every left element feeds 64 columns and every right element 64 rows.
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_trace_gemm(
+    const T *__restrict__ sX, const T *__restrict__ sY, const GridGTile *__restrict__ tiles,
+    const int *__restrict__ runs, int nrun, const GridTile *__restrict__ times, double2 *__restrict__ out,
+    long long obase, long long ostride, int D, int rt) {
+  constexpr int KC = 16;
+  const GridGTile &G = tiles[blockIdx.x];
+  const GridTile &Tm = times[0];
+  __shared__ double2 As[64][KC + 1];
+  __shared__ double2 Bs[64][KC + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave >> 1, wn = wave & 1;
+  const int DD = D * D, nL = Tm.nL, nR = Tm.nR;
+  // this thread's row of L / column of R and its 4 elements of a chunk
+  const int lr = tid >> 2, lk = (tid & 3) * 4;
+  const int er = min(lr >> 4, G.nE - 1), fc = min(lr >> 4, G.nF - 1);
+  const int ta = min(lr & 15, nL - 1), tb = min(lr & 15, nR - 1);  // rows past the time tile repeat its last
+  const T *xrow = sX + (Tm.Lt[ta] + G.xq[er]) * DD;
+  const T *yrow = sY + (Tm.Rt[tb] + G.yq[fc]) * DD;
+  const int cpr = DD / KC;  // chunks per run
+  const int nchunk = nrun * cpr;
+  double2 av[4], bv[4];
+  auto load = [&](int c) {
+    const int sidx = c / cpr, e0 = (c - sidx * cpr) * KC + lk;
+    const int i = runs[3 * sidx], j1 = runs[3 * sidx + 1], j2 = runs[3 * sidx + 2];
+    const T *xa = xrow + (long long)i * DD;
+    const T *y1 = yrow + (long long)j1 * DD, *y2 = yrow + (long long)(j2 >= 0 ? j2 : j1) * DD;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = e0 + u, er2 = rt ? e : (e % D) * D + e / D;
+      av[u] = ld2(xa + e);
+      bv[u] = j2 >= 0 ? cadd(ld2(y1 + er2), ld2(y2 + er2)) : ld2(y1 + er2);
+    }
+  };
+  // (ax + i ay)(bx + i by): P1 = ax bx, P2 = ay by, P3 = (ax + ay)(bx + by);
+  // re = P1 - P2, im = P3 - P1 - P2
+  d4 p1[2][2], p2[2][2], p3[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) p1[a][b] = p2[a][b] = p3[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  if (nchunk > 0) load(0);
+  for (int c = 0; c < nchunk; ++c) {
+    __syncthreads();  // the previous chunk has been read
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      As[lr][lk + u] = av[u];
+      Bs[lr][lk + u] = bv[u];
+    }
+    __syncthreads();
+    if (c + 1 < nchunk) load(c + 1);
+#pragma unroll
+    for (int kq = 0; kq < KC; kq += 4) {
+      const int kr = kq + (lane >> 4);
+      const double2 a2[2] = {As[wm * 32 + (lane & 15)][kr], As[wm * 32 + 16 + (lane & 15)][kr]};
+      const double2 b2[2] = {Bs[wn * 32 + (lane & 15)][kr], Bs[wn * 32 + 16 + (lane & 15)][kr]};
+      const double as[2] = {a2[0].x + a2[0].y, a2[1].x + a2[1].y};
+      const double bs[2] = {b2[0].x + b2[0].y, b2[1].x + b2[1].y};
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) {
+          p1[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[mi].x, b2[ni].x, p1[mi][ni], 0, 0, 0);
+          p2[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[mi].y, b2[ni].y, p2[mi][ni], 0, 0, 0);
+          p3[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[mi], bs[ni], p3[mi][ni], 0, 0, 0);
+        }
+    }
+  }
+  // rows (lane >> 4) + 4 i of each 16 x 16 block, column lane & 15
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = wm * 32 + mi * 16 + (lane >> 4) + 4 * i, col = wn * 32 + ni * 16 + (lane & 15);
+        const int r = row >> 4, a = row & 15, q = col >> 4, b = col & 15;
+        if (r >= G.nE || q >= G.nF || a >= nL || b >= nR) continue;
+        const long long dst = G.dst[r][q];
+        if (dst < 0) continue;
+        const int sp = Tm.sidx[a][b];
+        if (sp < 0) continue;
+        const double q1 = p1[mi][ni][i], q2 = p2[mi][ni][i];
+        out[obase + sp * ostride + dst] = make_double2(q1 - q2, p3[mi][ni][i] - q1 - q2);
+      }
+}
+
 // ---------------------------------------------------------------------------
 // resolve_request{1,2,3} (reference: Diagram.cpp:9-94, DilutedTrace.cpp:5-82)
 // on mask-group sums: mean, sum_g L_g sum_{h: m_g & m_h = 0} R_h, and the
@@ -2068,6 +2164,21 @@ void launch_trace_groups(const void *sX, const long long *Xt, const long long *X
   else
     hipLaunchKernelGGL(k_trace_groups<double2>, dim3(nitems, ntime), dim3(64), 0, s, static_cast<const double2 *>(sX),
                        Xt, Xq, static_cast<const double2 *>(sY), Yt, Yq, items, pairs, out, obase, ostride, D, rt);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_trace_gemm(const void *sX, const void *sY, const GridGTile *tiles, int ntiles, const int *runs, int nrun,
+                       const GridTile *times, double2 *out, long long obase, long long ostride, int D, int rt, int f32,
+                       hipStream_t s) {
+  if (ntiles <= 0) return;
+  if (D * D % 16 != 0 || D > kMaxD) throw std::runtime_error("launch_trace_gemm: unsupported factor dimension");
+  static_assert(kGemmT * 16 == 64, "the GEMM tile is 64 x 64");
+  if (f32)
+    hipLaunchKernelGGL(k_trace_gemm<float2>, dim3(ntiles), dim3(256), 0, s, static_cast<const float2 *>(sX),
+                       static_cast<const float2 *>(sY), tiles, runs, nrun, times, out, obase, ostride, D, rt);
+  else
+    hipLaunchKernelGGL(k_trace_gemm<double2>, dim3(ntiles), dim3(256), 0, s, static_cast<const double2 *>(sX),
+                       static_cast<const double2 *>(sY), tiles, runs, nrun, times, out, obase, ostride, D, rt);
   SLAPH_HIP(hipGetLastError());
 }
 

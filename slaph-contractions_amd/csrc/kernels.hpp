@@ -234,6 +234,22 @@ struct GridRGroup {
 void launch_trace_grid_r(const void *slabX, const void *slabY, const GridRGroup *groups, int ngroups, const int *runs,
                          const GridTile *tiles, int ntiles, double2 *out, long long obase, long long ostride, int D,
                          int rt, int f32, hipStream_t s);
+// GEMM form of the same traces when every item has the same runs (i_s, j1_s,
+// j2_s): the tiles of all (left entry, right entry) pairs of a block of
+// kGemmT x kGemmT entries are one complex GEMM, rows (left entry, left time),
+// columns (right entry, right time), K = (run, matrix element):
+//   C[(E, a)][(F, b)] = sum_s sum_e L(E, i_s, a)[e] (R(F, j1_s, b) + R(F, j2_s, b))[e'].
+// dst[r][c] is the item of (xq[r], yq[c]) or -1; rows/columns past nE/nF and
+// pairs without an item are computed and discarded.  runs: 3 ints per run.
+constexpr int kGemmT = 4;
+struct GridGTile {
+  long long xq[kGemmT], yq[kGemmT];
+  long long dst[kGemmT][kGemmT];
+  int nE, nF;
+};
+void launch_trace_gemm(const void *slabX, const void *slabY, const GridGTile *tiles, int ntiles, const int *runs,
+                       int nrun, const GridTile *times, double2 *out, long long obase, long long ostride, int D, int rt,
+                       int f32, hipStream_t s);
 // Non-separable traces, left-stationary.  A block is a set of (trace id,
 // mask group) slots; its chunks stage left matrices X[Xt[ti] + left[loff + m]]
 // (m < n, any entries and combinations) and list tasks per wavefront.  A task

@@ -170,12 +170,13 @@ def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypat
     of grouped by right operand.  The one-round-windows variant reloads the
     queued form's task descriptors at every round of its ring (the largest
     units of the 32^3 x 64 workload fit one window, so only this variant
-    runs that reload)."""
+    runs that reload) and runs the slice-pair tiles grouped by right operand
+    instead of as GEMMs."""
     say("small pipi", "wide" if infile is WIDE else "", kernels)
     for var, on in (("SLAPH_TRACE_GROUPS", "groups" in kernels), ("SLAPH_PROD_GENERIC", "generic" in kernels),
                     ("SLAPH_RESOLVE_GROUPS", "groups" in kernels), ("SLAPH_VDV_DIRECT", "groups" in kernels),
                     ("SLAPH_LSTAT_BLOCKS", "blocks" in kernels), ("SLAPH_GRID_ITEMS", "items" in kernels),
-                    ("SLAPH_LSTAT_WINDOW", "window1" in kernels)):
+                    ("SLAPH_LSTAT_WINDOW", "window1" in kernels), ("SLAPH_GRID_GROUPS", "window1" in kernels)):
         if on:
             monkeypatch.setenv(var, "1")
         else:
