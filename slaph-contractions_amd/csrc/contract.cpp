@@ -274,7 +274,7 @@ struct Family {
   // GEMM form of the slice-pair tiles (every item with the same runs), see launch_trace_gemm
   std::vector<GridGTile> gg_tiles;
   std::vector<int> gg_runs;
-  DevBuf d_ggt, d_ggr;
+  DevBuf d_ggt, d_ggr, d_ggp;  // d_ggp: partial tiles of the K split
 };
 
 struct CorrPlan {
@@ -1706,9 +1706,12 @@ void Engine::contract(int single_block_pair, int kahan) {
                       np.d_combos.as<ComboProd>(), int(np.combos.size()), nc.ntime, D_, np.transposed ? 1 : 0, f32,
                       stream_);
         } else if (nc.grid && !f.gg_tiles.empty()) {
-          launch_trace_gemm(src(ns.left), src(ns.right), f.d_ggt.as<GridGTile>(), int(f.gg_tiles.size()),
-                            f.d_ggr.as<int>(), int(f.gg_runs.size() / 3), nc.d_tile.as<GridTile>(), C.d_gv.as<double2>(),
-                            gb, f.gstride, D_, rt, f32, stream_);
+          const int ntile = int(f.gg_tiles.size()), nrun = int(f.gg_runs.size() / 3);
+          const int kp = trace_gemm_parts(ntile, nrun);
+          if (kp > 1) f.d_ggp.ensure(sizeof(double2) * (size_t)ntile * kp * 4096);
+          launch_trace_gemm(src(ns.left), src(ns.right), f.d_ggt.as<GridGTile>(), ntile, f.d_ggr.as<int>(), nrun,
+                            nc.d_tile.as<GridTile>(), C.d_gv.as<double2>(), gb, f.gstride, D_, rt, f32, kp,
+                            kp > 1 ? f.d_ggp.as<double2>() : nullptr, stream_);
         } else if (nc.grid && !f.gr_groups.empty()) {
           launch_trace_grid_r(src(ns.left), src(ns.right), f.d_grg.as<GridRGroup>(), int(f.gr_groups.size()),
                               f.d_grr.as<int>(), nc.d_tile.as<GridTile>(), 1, C.d_gv.as<double2>(), gb, f.gstride,

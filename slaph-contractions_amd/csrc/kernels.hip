@@ -1558,9 +1558,13 @@ template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_trace_gemm(
     const T *__restrict__ sX, const T *__restrict__ sY, const GridGTile *__restrict__ tiles,
     const int *__restrict__ runs, int nrun, const GridTile *__restrict__ times, double2 *__restrict__ out,
-    long long obase, long long ostride, int D, int rt) {
+    long long obase, long long ostride, int D, int rt, double2 *__restrict__ part) {
   constexpr int KC = 16;
   const GridGTile &G = tiles[blockIdx.x];
+  // K split over gridDim.y parts (runs r0 .. r1); with more than one part
+  // the partial tiles go to part[] and k_gemm_reduce sums them in order
+  const int kp = blockIdx.y, KP = gridDim.y;
+  const int r0 = int((long long)nrun * kp / KP), r1 = int((long long)nrun * (kp + 1) / KP);
   const GridTile &Tm = times[0];
   __shared__ double2 As[64][KC + 1];
   __shared__ double2 Bs[64][KC + 1];
@@ -1574,10 +1578,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const T *xrow = sX + (Tm.Lt[ta] + G.xq[er]) * DD;
   const T *yrow = sY + (Tm.Rt[tb] + G.yq[fc]) * DD;
   const int cpr = DD / KC;  // chunks per run
-  const int nchunk = nrun * cpr;
+  const int nchunk = (r1 - r0) * cpr;
   double2 av[4], bv[4];
   auto load = [&](int c) {
-    const int sidx = c / cpr, e0 = (c - sidx * cpr) * KC + lk;
+    const int sidx = r0 + c / cpr, e0 = (c % cpr) * KC + lk;
     const int i = runs[3 * sidx], j1 = runs[3 * sidx + 1], j2 = runs[3 * sidx + 2];
     const T *xa = xrow + (long long)i * DD;
     const T *y1 = yrow + (long long)j1 * DD, *y2 = yrow + (long long)(j2 >= 0 ? j2 : j1) * DD;
@@ -1630,6 +1634,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int row = wm * 32 + mi * 16 + (lane >> 4) + 4 * i, col = wn * 32 + ni * 16 + (lane & 15);
+        if (KP > 1) {
+          const double q1 = p1[mi][ni][i], q2 = p2[mi][ni][i];
+          part[((long long)blockIdx.x * KP + kp) * 4096 + row * 64 + col] =
+              make_double2(q1 - q2, p3[mi][ni][i] - q1 - q2);
+          continue;
+        }
         const int r = row >> 4, a = row & 15, q = col >> 4, b = col & 15;
         if (r >= G.nE || q >= G.nF || a >= nL || b >= nR) continue;
         const long long dst = G.dst[r][q];
@@ -1639,6 +1649,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const double q1 = p1[mi][ni][i], q2 = p2[mi][ni][i];
         out[obase + sp * ostride + dst] = make_double2(q1 - q2, p3[mi][ni][i] - q1 - q2);
       }
+}
+
+// Sum of the K parts of each GEMM tile element in part order, stored into its
+// item like the one-part kernel does.
+__global__ void k_gemm_reduce(const double2 *__restrict__ part, int KP, const GridGTile *__restrict__ tiles,
+                              int ntiles, const GridTile *__restrict__ times, double2 *__restrict__ out, long long obase,
+                              long long ostride) {
+  const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (id >= (long long)ntiles * 4096) return;
+  const int t = int(id >> 12), el = int(id & 4095), row = el >> 6, col = el & 63;
+  const GridGTile &G = tiles[t];
+  const GridTile &Tm = times[0];
+  const int r = row >> 4, a = row & 15, q = col >> 4, b = col & 15;
+  if (r >= G.nE || q >= G.nF || a >= Tm.nL || b >= Tm.nR) return;
+  const long long dst = G.dst[r][q];
+  const int sp = Tm.sidx[a][b];
+  if (dst < 0 || sp < 0) return;
+  double2 v = part[(long long)t * KP * 4096 + el];
+  for (int k = 1; k < KP; ++k) v = cadd(v, part[((long long)t * KP + k) * 4096 + el]);
+  out[obase + sp * ostride + dst] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -2167,18 +2197,37 @@ void launch_trace_groups(const void *sX, const long long *Xt, const long long *X
   SLAPH_HIP(hipGetLastError());
 }
 
+int trace_gemm_parts(int ntiles, int nrun) {
+  // enough workgroups to even out the compute units: at least 6 per unit
+  static int ncu = 0;
+  if (ncu == 0) {
+    int dev = 0;
+    SLAPH_HIP(hipGetDevice(&dev));
+    SLAPH_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  if (ntiles <= 0 || nrun <= 0) return 1;
+  const int want = (6 * ncu + ntiles - 1) / ntiles;
+  return std::max(1, std::min({want, 8, nrun}));
+}
+
 void launch_trace_gemm(const void *sX, const void *sY, const GridGTile *tiles, int ntiles, const int *runs, int nrun,
                        const GridTile *times, double2 *out, long long obase, long long ostride, int D, int rt, int f32,
-                       hipStream_t s) {
+                       int kparts, double2 *part, hipStream_t s) {
   if (ntiles <= 0) return;
   if (D * D % 16 != 0 || D > kMaxD) throw std::runtime_error("launch_trace_gemm: unsupported factor dimension");
+  if (kparts < 1 || kparts > 65535 || (kparts > 1 && part == nullptr))
+    throw std::runtime_error("launch_trace_gemm: bad K split");
   static_assert(kGemmT * 16 == 64, "the GEMM tile is 64 x 64");
+  const dim3 grid(ntiles, kparts);
   if (f32)
-    hipLaunchKernelGGL(k_trace_gemm<float2>, dim3(ntiles), dim3(256), 0, s, static_cast<const float2 *>(sX),
-                       static_cast<const float2 *>(sY), tiles, runs, nrun, times, out, obase, ostride, D, rt);
+    hipLaunchKernelGGL(k_trace_gemm<float2>, grid, dim3(256), 0, s, static_cast<const float2 *>(sX),
+                       static_cast<const float2 *>(sY), tiles, runs, nrun, times, out, obase, ostride, D, rt, part);
   else
-    hipLaunchKernelGGL(k_trace_gemm<double2>, dim3(ntiles), dim3(256), 0, s, static_cast<const double2 *>(sX),
-                       static_cast<const double2 *>(sY), tiles, runs, nrun, times, out, obase, ostride, D, rt);
+    hipLaunchKernelGGL(k_trace_gemm<double2>, grid, dim3(256), 0, s, static_cast<const double2 *>(sX),
+                       static_cast<const double2 *>(sY), tiles, runs, nrun, times, out, obase, ostride, D, rt, part);
+  if (kparts > 1)
+    hipLaunchKernelGGL(k_gemm_reduce, grid1d((long long)ntiles * 4096, 256), dim3(256), 0, s, part, kparts, tiles,
+                       ntiles, times, out, obase, ostride);
   SLAPH_HIP(hipGetLastError());
 }
 

@@ -247,9 +247,13 @@ struct GridGTile {
   long long dst[kGemmT][kGemmT];
   int nE, nF;
 };
+// K (the runs) is split into kparts (trace_gemm_parts) parts; with more than
+// one, part holds ntiles * kparts * 4096 double2 partial tiles that a second
+// kernel sums in part order.
+int trace_gemm_parts(int ntiles, int nrun);
 void launch_trace_gemm(const void *slabX, const void *slabY, const GridGTile *tiles, int ntiles, const int *runs,
                        int nrun, const GridTile *times, double2 *out, long long obase, long long ostride, int D, int rt,
-                       int f32, hipStream_t s);
+                       int f32, int kparts, double2 *part, hipStream_t s);
 // Non-separable traces, left-stationary.  A block is a set of (trace id,
 // mask group) slots; its chunks stage left matrices X[Xt[ti] + left[loff + m]]
 // (m < n, any entries and combinations) and list tasks per wavefront.  A task
