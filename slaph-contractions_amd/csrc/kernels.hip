@@ -367,15 +367,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
 // ---------------------------------------------------------------------------
 // VdaggerV by the sitewise Fourier factorisation (see kernels.hpp).  One
-// workgroup = one (timeslice, x, tile pair I <= J); its two waves run the same
-// per-site W MFMAs (lane l: colour l/16, column l%16; colour 3 is zero
-// padding) and each transforms two of the four accumulator rows, so that the
-// z sums (cos / sin per |pz|) and the y sums per (py, pz) stay in registers.
-// Per site: 3 MFMAs (Gauss: W = (P1 + P2) + i (P3 - P1 + P2)) and
-// 2 (3 + 2 + 4 nk) + 4 FP64 vector operations per lane.  The operands of the
-// next four sites are loaded while the current four are processed.
+// single-wave workgroup = one (timeslice, x, tile pair I <= J).  Per site the
+// wave forms W = V_I^H V_J on the 16 x 16 tile by four accumulating MFMAs
+// (lane l: colour l/16, column l%16; colour 3 is zero padding), Re W =
+// ax bx + ay by and Im W = ax by - ay bx, and transforms all four accumulator
+// rows, so that the z sums (cos / sin per |pz|) and the y sums per (py, pz)
+// stay in registers.  Per site: 4 MFMAs and 4 (2 + 4 nk) + 3 FP64 vector
+// operations per lane.  The operands of the next four sites are loaded while
+// the current four are processed.
 // ---------------------------------------------------------------------------
-template <int NC, int NK, int H, bool kFast>
+template <int NC, int NK, bool kFast>
 __device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, FtLayout lay, int I, int J, int Ly, int Lz,
                                             long long site0, FtPass const &P, const double2 *__restrict__ ey,
                                             const double2 *zl, double2 *__restrict__ Uo, long long ustride) {
@@ -388,24 +389,29 @@ __device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, FtLa
   const double2 *pa = Vr + (site0 * 3 + cl) * lay.kstride + (long long)min(I * 16 + m, lay.ncol - 1) * lay.nstride;
   const double2 *pb = Vr + (site0 * 3 + cl) * lay.kstride + (long long)min(J * 16 + m, lay.ncol - 1) * lay.nstride;
   const double2 zero2 = make_double2(0.0, 0.0);
-  double2 U[NC][2];
+  double2 U[NC][4];
 #pragma unroll
-  for (int k = 0; k < NC; ++k) U[k][0] = U[k][1] = zero2;
-  double a0r[2], a0i[2], cr[NK][2], ci[NK][2], sr[NK][2], si[NK][2];
-  // W of one site: three MFMAs (Gauss).
-  auto mf = [&](const double2 &a, const double2 &b, d4 &p1, d4 &p2, d4 &p3) {
+  for (int k = 0; k < NC; ++k)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) U[k][q] = zero2;
+  double a0r[4], a0i[4], cr[NK][4], ci[NK][4], sr[NK][4], si[NK][4];
+  // W of one site: wr = Re W, wi = Im W.  The four-MFMA form leaves no
+  // vector work between the MFMAs and the z transform (the three-multiply
+  // form needs three additions per element, and the wave is bound by its
+  // vector instructions).
+  auto mf = [&](const double2 &a, const double2 &b, d4 &wr, d4 &wi) {
     const d4 dz = {0.0, 0.0, 0.0, 0.0};
-    const double ax = a.x * amask, ay = a.y * amask;
-    p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ax, b.x, dz, 0, 0, 0);
-    p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ay, b.y, dz, 0, 0, 0);
-    p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ax - ay, b.x + b.y, dz, 0, 0, 0);
+    const double ax = a.x * amask, ay = a.y * amask, nay = a.y * -amask;
+    wr = __builtin_amdgcn_mfma_f64_16x16x4f64(ax, b.x, dz, 0, 0, 0);
+    wi = __builtin_amdgcn_mfma_f64_16x16x4f64(ax, b.y, dz, 0, 0, 0);
+    wr = __builtin_amdgcn_mfma_f64_16x16x4f64(ay, b.y, wr, 0, 0, 0);
+    wi = __builtin_amdgcn_mfma_f64_16x16x4f64(nay, b.x, wi, 0, 0, 0);
   };
-  // z transform of the two rows of this wave, site phases cs (cos, sin).
-  auto zacc = [&](const double2 *cs, const d4 &p1, const d4 &p2, const d4 &p3) {
+  // z transform of the four rows, site phases cs (cos, sin).
+  auto zacc = [&](const double2 *cs, const d4 &w_r, const d4 &w_i) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int r = 2 * H + q;
-      const double wr = p1[r] + p2[r], wi = (p3[r] - p1[r]) + p2[r];
+    for (int q = 0; q < 4; ++q) {
+      const double wr = w_r[q], wi = w_i[q];
       a0r[q] += wr;
       a0i[q] += wi;
 #pragma unroll
@@ -419,7 +425,7 @@ __device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, FtLa
   };
   auto zero_z = [&]() {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < 4; ++q) {
       a0r[q] = a0i[q] = 0.0;
 #pragma unroll
       for (int k = 0; k < NK; ++k) cr[k][q] = ci[k][q] = sr[k][q] = si[k][q] = 0.0;
@@ -434,7 +440,7 @@ __device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, FtLa
         const int sel = P.tsel[k2];
         const double2 e = ey[k2 * Ly + y];
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < 4; ++q) {
           double tr = a0r[q], ti = a0i[q];
 #pragma unroll
           for (int k = 0; k < NK; ++k) {
@@ -489,15 +495,15 @@ __device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, FtLa
       if (z0 == 0) zero_z();
       // Software pipeline: the MFMAs of site u + 1 are issued before the
       // vector work of site u.
-      d4 e1, e2, e3, o1, o2, o3;
-      mf(ca[0], cb[0], e1, e2, e3);
-      mf(ca[1], cb[1], o1, o2, o3);
-      zacc(cs[0], e1, e2, e3);
-      mf(ca[2], cb[2], e1, e2, e3);
-      zacc(cs[1], o1, o2, o3);
-      mf(ca[3], cb[3], o1, o2, o3);
-      zacc(cs[2], e1, e2, e3);
-      zacc(cs[3], o1, o2, o3);
+      d4 e1, e2, o1, o2;
+      mf(ca[0], cb[0], e1, e2);
+      mf(ca[1], cb[1], o1, o2);
+      zacc(cs[0], e1, e2);
+      mf(ca[2], cb[2], e1, e2);
+      zacc(cs[1], o1, o2);
+      mf(ca[3], cb[3], o1, o2);
+      zacc(cs[2], e1, e2);
+      zacc(cs[3], o1, o2);
       if (zn == 0) {
         yline(y);
         ++y;
@@ -530,15 +536,15 @@ __device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, FtLa
 #pragma unroll
         for (int k = 0; k < NK; ++k) cs[u][k] = zl[k * Lz + min(z0 + u, Lz - 1)];
       if (z0 == 0) zero_z();
-      d4 e1, e2, e3, o1, o2, o3;
-      mf(ca[0], cb[0], e1, e2, e3);
-      if (z0 + 1 < Lz) mf(ca[1], cb[1], o1, o2, o3);
-      zacc(cs[0], e1, e2, e3);
-      if (z0 + 2 < Lz) mf(ca[2], cb[2], e1, e2, e3);
-      if (z0 + 1 < Lz) zacc(cs[1], o1, o2, o3);
-      if (z0 + 3 < Lz) mf(ca[3], cb[3], o1, o2, o3);
-      if (z0 + 2 < Lz) zacc(cs[2], e1, e2, e3);
-      if (z0 + 3 < Lz) zacc(cs[3], o1, o2, o3);
+      d4 e1, e2, o1, o2;
+      mf(ca[0], cb[0], e1, e2);
+      if (z0 + 1 < Lz) mf(ca[1], cb[1], o1, o2);
+      zacc(cs[0], e1, e2);
+      if (z0 + 2 < Lz) mf(ca[2], cb[2], e1, e2);
+      if (z0 + 1 < Lz) zacc(cs[1], o1, o2);
+      if (z0 + 3 < Lz) mf(ca[3], cb[3], o1, o2);
+      if (z0 + 2 < Lz) zacc(cs[2], e1, e2);
+      if (z0 + 3 < Lz) zacc(cs[3], o1, o2);
       if (z0 + 4 >= Lz) yline(y);
     }
   }
@@ -548,15 +554,15 @@ __device__ __forceinline__ void vdv_ft_wave(const double2 *__restrict__ Vr, FtLa
   for (int k2 = 0; k2 < NC; ++k2) {
     if (k2 < P.nc) {
 #pragma unroll
-      for (int q = 0; q < 2; ++q) Uo[k2 * ustride + lane + 64 * (2 * H + q)] = U[k2][q];
+      for (int q = 0; q < 4; ++q) Uo[k2 * ustride + lane + 64 * q] = U[k2][q];
     }
   }
 }
 
-// The wide instantiation (13 combinations, two |pz|) needs more than 256
-// registers per lane and runs at one wave per SIMD.
+// The accumulators of four rows need more than 256 registers per lane: one
+// wave per SIMD.
 template <int NC, int NK, bool kFast>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NC > 9 ? 1 : 2, 2))) void k_vdv_ft(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_vdv_ft(
     const double2 *__restrict__ Vr, long long vstride, FtLayout lay, int Lx, int Ly, int Lz,
     const int2 *__restrict__ tiles, int ntp, FtPass P, const double2 *__restrict__ ey,
     const double2 *__restrict__ zcs, double2 *__restrict__ U) {
@@ -574,10 +580,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NC > 9 ? 1 
   for (int i = threadIdx.x; i < NK * Lz; i += blockDim.x) zl[(i / Lz) * Lz + i % Lz] = zcs[(i / Lz) * Lz + i % Lz];
   __syncthreads();
   const double2 *V = Vr + t * vstride;
-  if (threadIdx.x < 64)
-    vdv_ft_wave<NC, NK, 0, kFast>(V, lay, ij.x, ij.y, Ly, Lz, site0, P, ey, zl, Uo, ustride);
-  else
-    vdv_ft_wave<NC, NK, 1, kFast>(V, lay, ij.x, ij.y, Ly, Lz, site0, P, ey, zl, Uo, ustride);
+  vdv_ft_wave<NC, NK, kFast>(V, lay, ij.x, ij.y, Ly, Lz, site0, P, ey, zl, Uo, ustride);
 }
 
 // C(p) from the per-x sums: thread = (row, column) of one tile pair.
@@ -1916,17 +1919,17 @@ void launch_vdv_ft(const double2 *Vr, long long vstride, FtLayout lay, int Lx, i
   const bool fast = (Lz & 3) == 0;
   if (pass.nc <= 9 && pass.nk <= 1) {
     if (fast)
-      hipLaunchKernelGGL((k_vdv_ft<9, 1, true>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride, lay, Lx, Ly,
+      hipLaunchKernelGGL((k_vdv_ft<9, 1, true>), dim3(unsigned(blocks)), dim3(64), 0, s, Vr, vstride, lay, Lx, Ly,
                          Lz, tiles, ntp, pass, ey, zcs, U);
     else
-      hipLaunchKernelGGL((k_vdv_ft<9, 1, false>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride, lay, Lx, Ly,
+      hipLaunchKernelGGL((k_vdv_ft<9, 1, false>), dim3(unsigned(blocks)), dim3(64), 0, s, Vr, vstride, lay, Lx, Ly,
                          Lz, tiles, ntp, pass, ey, zcs, U);
   } else {
     if (fast)
-      hipLaunchKernelGGL((k_vdv_ft<kFtMaxC, kFtMaxK, true>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride,
+      hipLaunchKernelGGL((k_vdv_ft<kFtMaxC, kFtMaxK, true>), dim3(unsigned(blocks)), dim3(64), 0, s, Vr, vstride,
                          lay, Lx, Ly, Lz, tiles, ntp, pass, ey, zcs, U);
     else
-      hipLaunchKernelGGL((k_vdv_ft<kFtMaxC, kFtMaxK, false>), dim3(unsigned(blocks)), dim3(128), 0, s, Vr, vstride,
+      hipLaunchKernelGGL((k_vdv_ft<kFtMaxC, kFtMaxK, false>), dim3(unsigned(blocks)), dim3(64), 0, s, Vr, vstride,
                          lay, Lx, Ly, Lz, tiles, ntp, pass, ey, zcs, U);
   }
   SLAPH_HIP(hipGetLastError());
