@@ -1563,10 +1563,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int *__restrict__ runs, int nrun, const GridTile *__restrict__ times, double2 *__restrict__ out,
     long long obase, long long ostride, int D, int rt, double2 *__restrict__ part) {
   constexpr int KC = 16;
-  const GridGTile &G = tiles[blockIdx.x];
   // K split over gridDim.y parts (runs r0 .. r1); with more than one part
-  // the partial tiles go to part[] and k_gemm_reduce sums them in order
-  const int kp = blockIdx.y, KP = gridDim.y;
+  // the partial tiles go to part[] and k_gemm_reduce sums them in order.
+  // XCD x (workgroups are dealt to the XCDs round-robin in dispatch order)
+  // runs the x-th contiguous eighth of the (part, tile) sequence, so that the
+  // tiles of one component, which share left rows and right columns, meet in
+  // the same L2 at the same K part.
+  const int NB = gridDim.x * gridDim.y, lin = blockIdx.y * gridDim.x + blockIdx.x;
+  const int per = NB / 8, rem = NB % 8, xcd = lin % 8, jx = lin / 8;
+  const int lb = xcd < rem ? xcd * (per + 1) + jx : rem * (per + 1) + (xcd - rem) * per + jx;
+  const int tile = lb % gridDim.x, kp = lb / gridDim.x, KP = gridDim.y;
+  const GridGTile &G = tiles[tile];
   const int r0 = int((long long)nrun * kp / KP), r1 = int((long long)nrun * (kp + 1) / KP);
   const GridTile &Tm = times[0];
   __shared__ double2 As[64][KC + 1];
@@ -1639,7 +1646,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const int row = wm * 32 + mi * 16 + (lane >> 4) + 4 * i, col = wn * 32 + ni * 16 + (lane & 15);
         if (KP > 1) {
           const double q1 = p1[mi][ni][i], q2 = p2[mi][ni][i];
-          part[((long long)blockIdx.x * KP + kp) * 4096 + row * 64 + col] =
+          part[((long long)tile * KP + kp) * 4096 + row * 64 + col] =
               make_double2(q1 - q2, p3[mi][ni][i] - q1 - q2);
           continue;
         }
