@@ -1710,7 +1710,12 @@ __global__ void k_zeta(const double2 *__restrict__ gv, long long gbase, long lon
 __global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__restrict__ gm,
                           const CorrJob *__restrict__ jobs, int ncorr, int nsp, const double2 *__restrict__ zv,
                           const uint16_t *__restrict__ zmask, int nz, double2 *__restrict__ cval) {
-  const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  // XCD x (workgroups are dealt to the XCDs round-robin) runs the x-th
+  // contiguous eighth of the (slice pair, correlator) sequence, so each XCD's
+  // L2 holds the trace values of its own slice pairs only.
+  const int NB = gridDim.x, per = NB / 8, rem = NB % 8, xcd = blockIdx.x % 8, jx = blockIdx.x / 8;
+  const int lb = xcd < rem ? xcd * (per + 1) + jx : rem * (per + 1) + (xcd - rem) * per + jx;
+  const long long id = lb * (long long)blockDim.x + threadIdx.x;
   if (id >= (long long)ncorr * nsp) return;
   const int corr = int(id % ncorr), s = int(id / ncorr);
   const CorrJob J = jobs[corr];
@@ -2300,6 +2305,7 @@ void launch_resolve(const double2 *gv, const uint16_t *gm, const CorrJob *jobs, 
                     const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, hipStream_t s) {
   long long n = (long long)ncorr * nsp;
   if (n <= 0) return;
+  if ((n + 255) / 256 > 0x7fffffffLL) throw std::runtime_error("launch_resolve: grid too large");
   hipLaunchKernelGGL(k_resolve, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, gv, gm, jobs, ncorr, nsp, zv, zmask,
                      nz, cval);
   SLAPH_HIP(hipGetLastError());
