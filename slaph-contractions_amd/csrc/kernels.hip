@@ -836,11 +836,18 @@ __global__ __launch_bounds__(256) void k_prod16(const T *__restrict__ sX, const 
                                                 const long long *__restrict__ Yt, const long long *__restrict__ Yq,
                                                 T *__restrict__ out, long long obase, long long ostride,
                                                 const ComboProd *__restrict__ combos, int ncombo) {
-  const int ti = blockIdx.y;
+  // XCD x (workgroups are dealt to the XCDs round-robin in dispatch order)
+  // runs the x-th contiguous eighth of the (time, combination block)
+  // sequence, so that the combinations that share operand matrices read them
+  // through the same L2.
+  const int NB = gridDim.x * gridDim.y, lin = blockIdx.y * gridDim.x + blockIdx.x;
+  const int per = NB / 8, rem = NB % 8, xcd = lin % 8, jx = lin / 8;
+  const int lb = xcd < rem ? xcd * (per + 1) + jx : rem * (per + 1) + (xcd - rem) * per + jx;
+  const int ti = lb / gridDim.x, bx = lb % gridDim.x;
   // the wave index through readfirstlane: uniform, so that the combination
   // tables come through the scalar cache
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int c0 = (blockIdx.x * 4 + wave) * kProdPerWave;
+  const int c0 = (bx * 4 + wave) * kProdPerWave;
   const int c1 = min(c0 + kProdPerWave, ncombo);
   if (c0 >= c1) return;
   const long long xt = Xt[ti], yt = Yt[ti];
