@@ -2022,6 +2022,7 @@ void launch_prod_t(const void *sX, const long long *Xt, const long long *Xq, con
   if (D == 16 && !std::getenv("SLAPH_PROD_GENERIC")) {
     const int per_block = 4 * kProdPerWave;
     const dim3 grid((ncombo + per_block - 1) / per_block, ntime);
+    if ((long long)grid.x * grid.y > 0x7fffffffLL) throw std::runtime_error("launch_prod: grid too large");
     if (transposed)
       hipLaunchKernelGGL((k_prod16<T, true>), grid, dim3(256), 0, s, x, Xt, Xq, y, Yt, Yq, o, obase, ostride, combos,
                          ncombo);
@@ -2240,6 +2241,7 @@ void launch_trace_gemm(const void *sX, const void *sY, const GridGTile *tiles, i
   if (kparts < 1 || kparts > 65535 || (kparts > 1 && part == nullptr))
     throw std::runtime_error("launch_trace_gemm: bad K split");
   static_assert(kGemmT * 16 == 64, "the GEMM tile is 64 x 64");
+  if ((long long)ntiles * kparts > 0x7fffffffLL) throw std::runtime_error("launch_trace_gemm: grid too large");
   const dim3 grid(ntiles, kparts);
   if (f32)
     hipLaunchKernelGGL(k_trace_gemm<float2>, grid, dim3(256), 0, s, static_cast<const float2 *>(sX),
