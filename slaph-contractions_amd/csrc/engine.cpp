@@ -471,12 +471,19 @@ void Engine::vdaggerv_fourier(std::vector<int> const &ids) {
           double a = tpi * pk[ip][s] * z / Lz;
           zcs[(size_t)s * Lz + z] = cplx(std::cos(a), std::sin(a));
         }
-      std::vector<cplx> ex;
+      // targets sorted by combination (k_vdv_xsum reads each combination's
+      // per-x sums once for a run of targets), x phases in the same order
+      std::vector<std::pair<FtPlan::Target, int>> tl;  // (target, px)
       for (auto const &w : want) {
         auto at = where.at({w.py, w.pz});
         if (at.first != int(ip)) continue;
-        P.targets.push_back({w.id, at.second, w.lower});
-        for (int x = 0; x < Lx; ++x) ex.push_back(std::exp(cplx(0, -tpi * w.px * x / Lx)));
+        tl.push_back({FtPlan::Target{w.id, at.second, w.lower}, w.px});
+      }
+      std::stable_sort(tl.begin(), tl.end(), [](auto const &a, auto const &b) { return a.first.c < b.first.c; });
+      std::vector<cplx> ex;
+      for (auto const &e : tl) {
+        P.targets.push_back(e.first);
+        for (int x = 0; x < Lx; ++x) ex.push_back(std::exp(cplx(0, -tpi * e.second * x / Lx)));
       }
       upload(P.ey, ey);
       upload(P.zcs, zcs);

@@ -583,7 +583,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   vdv_ft_wave<NC, NK, kFast>(V, lay, ij.x, ij.y, Ly, Lz, site0, P, ey, zl, Uo, ustride);
 }
 
-// C(p) from the per-x sums: thread = (row, column) of one tile pair.
+// C(p) from the per-x sums: thread = (row, column) of one tile pair.  The
+// targets come sorted by combination: each run of up to kXsumGroup targets
+// that read the same combination's sums loads every per-x value once and
+// feeds all of them (each target's sum over x in ascending x, as alone).
+constexpr int kXsumGroup = 8;
 __global__ __launch_bounds__(256) void k_vdv_xsum(const double2 *__restrict__ U, int nc, int ntp, int Lx,
                                                   const int2 *__restrict__ tiles, const FtTarget *__restrict__ tg,
                                                   int ntg, const double2 *__restrict__ ex, int N, int t0,
@@ -595,23 +599,38 @@ __global__ __launch_bounds__(256) void k_vdv_xsum(const double2 *__restrict__ U,
   const bool ok = i < N && j < N;
   const long long xstride = (long long)nc * ntp * 256;
   const double2 *u0 = U + (long long)t * Lx * xstride + (long long)tp * 256 + row * 16 + col;
-  for (int k = 0; k < ntg; ++k) {
-    const FtTarget T = tg[k];
-    if (T.lower && ij.x == ij.y) continue;
-    const double2 *u = u0 + (long long)T.c * ntp * 256;
-    const double2 *e = ex + (long long)k * Lx;
-    double sr = 0.0, si = 0.0;
+  for (int k0 = 0; k0 < ntg;) {
+    const int c = tg[k0].c;
+    int k1 = k0 + 1;
+    while (k1 < ntg && k1 - k0 < kXsumGroup && tg[k1].c == c) ++k1;
+    const int ng = k1 - k0;
+    const double2 *u = u0 + (long long)c * ntp * 256;
+    double sr[kXsumGroup], si[kXsumGroup];
+#pragma unroll
+    for (int g = 0; g < kXsumGroup; ++g) sr[g] = si[g] = 0.0;
     for (int x = 0; x < Lx; ++x) {
-      const double2 v = u[x * xstride], f = e[x];
-      sr = fma(f.x, v.x, fma(-f.y, v.y, sr));
-      si = fma(f.x, v.y, fma(f.y, v.x, si));
+      const double2 v = u[x * xstride];
+#pragma unroll
+      for (int g = 0; g < kXsumGroup; ++g) {
+        if (g < ng) {
+          const double2 f = ex[(long long)(k0 + g) * Lx + x];
+          sr[g] = fma(f.x, v.x, fma(-f.y, v.y, sr[g]));
+          si[g] = fma(f.x, v.y, fma(f.y, v.x, si[g]));
+        }
+      }
     }
-    if (!ok) continue;
-    double2 *C = T.C + (long long)(t0 + t) * tstride;
-    if (!T.lower)
-      C[(long long)j * N + i] = make_double2(sr, si);
-    else
-      C[(long long)i * N + j] = make_double2(sr, -si);
+#pragma unroll
+    for (int g = 0; g < kXsumGroup; ++g) {
+      if (g >= ng || !ok) continue;
+      const FtTarget T = tg[k0 + g];
+      if (T.lower && ij.x == ij.y) continue;
+      double2 *C = T.C + (long long)(t0 + t) * tstride;
+      if (!T.lower)
+        C[(long long)j * N + i] = make_double2(sr[g], si[g]);
+      else
+        C[(long long)i * N + j] = make_double2(sr[g], -si[g]);
+    }
+    k0 = k1;
   }
 }
 
