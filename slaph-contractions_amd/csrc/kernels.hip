@@ -2240,7 +2240,9 @@ void launch_trace_groups(const void *sX, const long long *Xt, const long long *X
 }
 
 int trace_gemm_parts(int ntiles, int nrun) {
-  // enough workgroups to even out the compute units: at least 6 per unit
+  // At least 16 workgroups per compute unit, raised to a part count that
+  // divides the runs evenly when one is near (parts of unequal length leave
+  // workgroups waiting on the longest), at most 32 parts.
   static int ncu = 0;
   if (ncu == 0) {
     int dev = 0;
@@ -2248,8 +2250,10 @@ int trace_gemm_parts(int ntiles, int nrun) {
     SLAPH_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   }
   if (ntiles <= 0 || nrun <= 0) return 1;
-  const int want = (6 * ncu + ntiles - 1) / ntiles;
-  return std::max(1, std::min({want, 8, nrun}));
+  const int want = std::max(1, std::min({(16 * ncu + ntiles - 1) / ntiles, 32, nrun}));
+  for (int kp = want; kp <= std::min({2 * want, 32, nrun}); ++kp)
+    if (nrun % kp == 0) return kp;
+  return want;
 }
 
 void launch_trace_gemm(const void *sX, const void *sY, const GridGTile *tiles, int ntiles, const int *runs, int nrun,
