@@ -26,19 +26,37 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "slaph-contractions_amd"))
 sys.path.insert(0, ROOT)
 
+PIPI = ("C2c", "C4cD", "C4cB", "C4cC", "C6cD")
 WORKLOADS = {
-    # headline: 32^3 x 64, N_ev = 220
-    "pipi32": dict(L=32, T=64, N=220, nrnd=5, dilT=16, dilE=4, pmax="0,1,2,3"),
-    # 24^3 x 48, N_ev = 120 (production size of the reference's pipi_I2.in lattice)
-    "pipi24": dict(L=24, T=48, N=120, nrnd=5, dilT=4, dilE=8, pmax="0,1,2,3"),
+    # headline, the pipi I=2 workload: 32^3 x 64, N_ev = 220, 27 momenta
+    # |p|^2 <= 3, TB16 / EI4 / DF4, C2c C4cD C4cB C4cC (|P|^2 <= 4) + C6cD (P = 0)
+    "pipi32": dict(L=32, T=64, N=220, nrnd=5, dilT=16, dilE=4, pmax="0,1,2,3", diagrams=PIPI),
+    # plain FP64 at production size: 24^3 x 48, N_ev = 120, 33 momenta
+    # |p|^2 <= 4, TB4 / EI8 / DF4 (D = 32), C2c C3c C4cD C4cB
+    "c2": dict(L=24, T=48, N=120, nrnd=5, dilT=4, dilE=8, pmax="0,1,2,3,4", diagrams=("C2c", "C3c", "C4cD", "C4cB")),
     # tiny CPU-checkable case
-    "small": dict(L=4, T=8, N=8, nrnd=4, dilT=2, dilE=4, pmax="0,1"),
+    "small": dict(L=4, T=8, N=8, nrnd=4, dilT=2, dilE=4, pmax="0,1", diagrams=PIPI),
 }
 
 PEAK_HBM = 8000.0      # GB/s
+PEAK_FP64 = 78.6       # TFLOP/s, vector and MFMA
+
+_CORR = {
+    "C2c": "C2c:Q0:Op0:Q0:Op0:P0,1,2,3,4",
+    "C3c": "C3c:Q0:Op0:Q0:Op0:Q0:Op0:P0,1,2,3,4",
+    "C4cD": "C4cD:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1,2,3,4",
+    "C4cB": "C4cB:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1,2,3,4",
+    "C4cC": "C4cC:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1,2,3,4",
+    "C6cD": "C6cD:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0",
+}
 
 
-def infile(w):
+def infile(w, pmax=None, diagrams=None, total_momenta=None):
+    """Input file of a workload; pmax / diagrams / total_momenta override the
+    operator momenta, the diagram list and the total momenta of every list."""
+    lines = ["correlator_list = " + _CORR[d] for d in (diagrams or w["diagrams"])]
+    if total_momenta is not None:
+        lines = [ln.rsplit(":P", 1)[0] + ":P" + total_momenta for ln in lines]
     return """
 Lt = {T}
 Lx = {L}
@@ -48,14 +66,10 @@ number_of_eigen_vec = {N}
 [quarks]
 quark = u:{nrnd}:TB:{dilT}:EI:{dilE}:DF:4:synthetic
 [operator_lists]
-operator_list = g5.d0.p{pmax}
+operator_list = g5.d0.p{pm}
 [correlator_lists]
-correlator_list = C2c:Q0:Op0:Q0:Op0:P0,1,2,3,4
-correlator_list = C4cD:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1,2,3,4
-correlator_list = C4cB:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1,2,3,4
-correlator_list = C4cC:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1,2,3,4
-correlator_list = C6cD:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0
-""".format(**w)
+{corr}
+""".format(pm=pmax or w["pmax"], corr="\n".join(lines), **w)
 
 
 # the kernel of the roofline object (the block form runs under SLAPH_LSTAT_BLOCKS=1)

@@ -97,9 +97,26 @@ int slc_diluted_factor(slc_ctx *ctx, int qtype, const int *key, int op, double *
 
 /* contract() (src/Correlators.cpp:40-127): every diagram of the infile over
  * all dilution block pairs (or only block pair `single_time_slice_combination`
- * when >= 0, global_data.cpp:84-87).  kahan != 0 accumulates the slice pairs
- * with Neumaier compensation (include/KahanAccumulator.hpp:14-48). */
+ * when >= 0, global_data.cpp:84-87).  kahan != 0 sums like the reference's
+ * SLAPH_KAHAN build (include/KahanAccumulator.hpp:14-78): Neumaier
+ * compensation inside each slice pair (the mean of resolve_request1 and
+ * right_sum / result of inner_product, src/Diagram.cpp:9-27,
+ * src/DilutedTrace.cpp:5-82, in the reference's element order), plain sums
+ * across slice pairs (the per-slice-pair accumulators merge sum and zero
+ * compensation, Diagram.cpp:96-114).  That form evaluates every trace value
+ * on its own and is much slower than the default. */
 int slc_contract(slc_ctx *ctx, int single_time_slice_combination, int kahan);
+
+/* inner_product(DilutedTraces, DilutedTraces[, DilutedTraces]) of
+ * src/DilutedTrace.cpp:5-44 / :46-82 (ntraces = 2 / 3), or the mean of one
+ * trace vector as in resolve_request1 (src/Diagram.cpp:9-27, ntraces = 1),
+ * evaluated by the device resolve kernel of slc_contract.  values[k] holds
+ * n[k] complex trace values, used[k] their used random-vector masks
+ * (DilutedTrace::used_rnd_ids, uint16).  kahan != 0 sums like the SLAPH_KAHAN
+ * build.  out: one complex value.  No admitted combination: the reference's
+ * "has an empty result" runtime_error. */
+int slc_inner_product(slc_ctx *ctx, int ntraces, const double *const *values, const uint16_t *const *used,
+                      const int *n, int kahan, double *out);
 
 /* Results of the last slc_contract, per diagram (Diagram::write,
  * src/Diagram.cpp:132-147): names, counts, and n_corr x Lt complex values in

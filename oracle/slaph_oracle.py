@@ -692,40 +692,68 @@ def factor_to_trace1(vec):
     return out
 
 
-def inner_product2(left, right):
-    """inner_product(DT, DT) (DilutedTrace.cpp:5-44)."""
+class NativeAccumulator:
+    """NativeAccumulator (KahanAccumulator.hpp:50-71): plain sum."""
+
+    def __init__(self, zero=0.0):
+        self.s = zero
+
+    def add(self, x):
+        self.s += x
+        return self
+
+    def value(self):
+        return self.s
+
+
+def cdiv(z, n):
+    """std::complex<double> / double: each component divided by n (numpy's
+    complex division multiplies by a reciprocal and can differ in the last bit)."""
+    z = complex(z)
+    return complex(z.real / n, z.imag / n)
+
+
+def _accumulator(kahan):
+    """Accumulator<Complex>: Kahan with -DSLAPH_KAHAN, native otherwise
+    (KahanAccumulator.hpp:73-78)."""
+    return KahanAccumulator(0j) if kahan else NativeAccumulator(0j)
+
+
+def inner_product2(left, right, kahan=False):
+    """inner_product(DT, DT) (DilutedTrace.cpp:5-44); right_sum and result are
+    Accumulator<Complex> (:13, :16)."""
     n = 0
-    res = 0j
+    res = _accumulator(kahan)
     for lv, lu in left:
-        rs = 0j
+        rs = _accumulator(kahan)
         for rv, ru in right:
             if lu & ru:
                 continue
-            rs += rv
+            rs.add(rv)
             n += 1
-        res += lv * rs
+        res.add(lv * rs.value())
     if n == 0:
         raise RuntimeError("inner_product has an empty result. Not enough random vectors?")
-    return res / n
+    return cdiv(res.value(), n)
 
 
-def inner_product3(left, middle, right):
+def inner_product3(left, middle, right, kahan=False):
     """inner_product(DT, DT, DT) (DilutedTrace.cpp:46-82): rejects only masks
-    shared by all three (:63)."""
+    shared by all three (:63); right_sum and result are Accumulator<Complex>."""
     n = 0
-    res = 0j
+    res = _accumulator(kahan)
     for lv, lu in left:
         for mv, mu in middle:
-            rs = 0j
+            rs = _accumulator(kahan)
             for rv, ru in right:
                 if lu & mu & ru:
                     continue
-                rs += rv
+                rs.add(rv)
                 n += 1
-            res += lv * mv * rs
+            res.add(lv * mv * rs.value())
     if n == 0:
         raise RuntimeError("inner_product has an empty result. Not enough random vectors?")
-    return res / n
+    return cdiv(res.value(), n)
 
 
 class Traces:
@@ -823,35 +851,46 @@ class Traces:
         return out
 
 
-def resolve_request(trs, traces, t_so, t_si):
-    """resolve_request{1,2,3} (Diagram.cpp:9-94)."""
+def trace_mean(vec, kahan=False):
+    """resolve_request1's mean of one trace vector: std::accumulate into an
+    Accumulator<Complex>, divided by the vector's size (Diagram.cpp:24-25)."""
+    s = _accumulator(kahan)
+    for v, _ in vec:
+        s.add(v)
+    return cdiv(s.value(), len(vec))
+
+
+def resolve_request(trs, traces, t_so, t_si, kahan=False):
+    """resolve_request{1,2,3} (Diagram.cpp:9-94).  The single-trace mean is
+    std::accumulate into an Accumulator<Complex> (:24-25)."""
     vecs = []
     for name, tr_id, locs in trs:
         key = [t_so if l == "source" else t_si for l in locs]
         vecs.append(traces.get(name, key, tr_id))
     if len(vecs) == 1:
-        x = vecs[0]
-        s = 0j
-        for v, _ in x:
-            s += v
-        return s / len(x)
+        return trace_mean(vecs[0], kahan)
     if len(vecs) == 2:
-        return inner_product2(vecs[0], vecs[1])
+        return inner_product2(vecs[0], vecs[1], kahan)
     if len(vecs) == 3:
-        return inner_product3(vecs[0], vecs[1], vecs[2])
+        return inner_product3(vecs[0], vecs[1], vecs[2], kahan)
     raise RuntimeError("This many traces are not implemented yet.")
 
 
 def contract(cfg, lut, vdv, perams, rnds, single_block_pair=-1, kahan=False):
     """contract (Correlators.cpp:40-127) + Diagram::assemble/write
     (Diagram.cpp:96-147).  Returns {diagram: [(dataset_name, complex[Lt])]}.
-    Block pairs run sequentially; caches are cleared per block pair (:116)."""
+    Block pairs run sequentially; caches are cleared per block pair (:116).
+    kahan=True is the SLAPH_KAHAN build: compensated sums inside
+    resolve_request / inner_product only.  Across slice pairs the reference
+    adds a fresh per-slice-pair accumulator holding one value (compensation
+    zero) with KahanAccumulator::operator+=(KahanAccumulator) (:96-114,
+    KahanAccumulator.hpp:40-44), so that sum is plain in both builds."""
     Lt = cfg["Lt"]
     q0 = cfg["quarks"][0]
     dilT, dilE = q0["dilT"], q0["dilE"]
     fac = Factories(cfg, lut, vdv, perams, rnds, dilT, dilE)
     tr = Traces(fac, lut, dilT)
-    acc = {name: [[KahanAccumulator(0j) if kahan else 0j for _ in reqs] for _ in range(Lt)]
+    acc = {name: [[0j for _ in reqs] for _ in range(Lt)]
            for name, reqs in lut["requests"].items() if reqs}
     nb = Lt // dilT
     for b, (bso, bsi) in enumerate(block_pairs(Lt, dilT)):
@@ -863,11 +902,7 @@ def contract(cfg, lut, vdv, perams, rnds, single_block_pair=-1, kahan=False):
             for t_so, t_si in slice_pairs(bso, bsi, Lt, nb):
                 t = time_delta(t_so, t_si, Lt)
                 for i, (trs, _) in enumerate(reqs):
-                    v = resolve_request(trs, tr, t_so, t_si)
-                    if kahan:
-                        acc[name][t][i].add(v)
-                    else:
-                        acc[name][t][i] += v
+                    acc[name][t][i] += resolve_request(trs, tr, t_so, t_si, kahan)
         fac.clear()
         tr.clear()
     out = {}
@@ -876,7 +911,7 @@ def contract(cfg, lut, vdv, perams, rnds, single_block_pair=-1, kahan=False):
             continue
         lst = []
         for i, (_, dsn) in enumerate(reqs):
-            vals = np.array([(acc[name][t][i].value() if kahan else acc[name][t][i]) / Lt for t in range(Lt)])
+            vals = np.array([acc[name][t][i] / Lt for t in range(Lt)])
             lst.append((dsn, vals))
         out[name] = lst
     return out

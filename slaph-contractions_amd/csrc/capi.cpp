@@ -147,6 +147,19 @@ int slc_diluted_factor(slc_ctx *ctx, int qtype, const int *key, int op, double *
   });
 }
 
+int slc_inner_product(slc_ctx *ctx, int ntraces, const double *const *values, const uint16_t *const *used,
+                      const int *n, int kahan, double *out) {
+  return guarded(ctx, [&] {
+    if (!values || !used || !n || !out || ntraces < 1 || ntraces > 3)
+      throw std::out_of_range("slc_inner_product: bad arguments");
+    const slaph::cplx *v[3] = {nullptr, nullptr, nullptr};
+    for (int k = 0; k < ntraces; ++k) v[k] = reinterpret_cast<const slaph::cplx *>(values[k]);
+    slaph::cplx r = ctx->eng->inner_product(ntraces, v, used, n, kahan);
+    out[0] = r.real();
+    out[1] = r.imag();
+  });
+}
+
 int slc_contract(slc_ctx *ctx, int single, int kahan) {
   return guarded(ctx, [&] { ctx->eng->contract(single, kahan); });
 }

@@ -281,7 +281,7 @@ struct CorrPlan {
   std::string name;
   std::vector<std::vector<std::pair<int, int>>> reqs;  // correlator -> [(family, trl)]
   std::vector<std::string> datasets;
-  std::vector<double> inv_n;  // 1 / admitted combinations per correlator
+  std::vector<double> n_adm;  // admitted combinations per correlator
 };
 
 // Is the chunk a rectangular grid in the operands' time bases?  True when the
@@ -897,6 +897,9 @@ struct ChunkPlan {
 
 struct ContractCache {
   int single_block_pair = -2;
+  // Kahan form (reference build with SLAPH_KAHAN): every trace value is its
+  // own group, summed in the reference's order by the resolve kernel
+  bool per_element = false;
   std::vector<Family> fams;
   std::vector<CorrPlan> corrs;
   std::vector<std::vector<SlicePair>> bp_slices;
@@ -912,7 +915,7 @@ struct ContractCache {
   ~ContractCache() {
     for (auto e : ev) (void)hipEventDestroy(e);
   }
-  std::vector<DevBuf> d_sum, d_comp;  // per diagram: Lt x n_corr accumulators
+  std::vector<DevBuf> d_sum;  // per diagram: Lt x n_corr accumulators
 };
 
 void Engine::contract(int single_block_pair, int kahan) {
@@ -926,10 +929,12 @@ void Engine::contract(int single_block_pair, int kahan) {
   // The plan depends on the input file only; it is built on the first call
   // (or after configure / a budget change) and reused for every later gauge
   // configuration.
-  const bool fresh = !plan_ || plan_->single_block_pair != single_block_pair;
+  const bool per_element = kahan != 0;
+  const bool fresh = !plan_ || plan_->single_block_pair != single_block_pair || plan_->per_element != per_element;
   if (fresh) {
     plan_ = std::make_shared<ContractCache>();
     plan_->single_block_pair = single_block_pair;
+    plan_->per_element = per_element;
   }
   // a call that throws leaves no half-built plan behind
   struct PlanGuard {
@@ -1214,7 +1219,7 @@ void Engine::contract(int single_block_pair, int kahan) {
       bool same_table = ln.kind == kLeaf && rn.kind == kLeaf
                             ? f.shape.leaves[ln.leaf].q == f.shape.leaves[rn.leaf].q
                             : (ln.kind == kProd && rn.kind == kProd && f.nodes[rs.left].owner == f.nodes[rs.right].owner);
-      if (!same_table || !f.coincide) continue;
+      if (!same_table || !f.coincide || per_element) continue;
       NodePlan const &root = f.nodes.back();
       std::map<std::pair<int, int>, int> first;
       for (size_t trl = 0; trl < ntrl; ++trl) {
@@ -1276,7 +1281,9 @@ void Engine::contract(int single_block_pair, int kahan) {
         std::vector<std::vector<int>> members;
         for (int c = 0; c < f.ncomb[trl]; ++c) {
           uint16_t m = masks[f.mask_off[trl] + c];
-          size_t g = multi[fi][trl] ? std::find(gm.begin(), gm.end(), m) - gm.begin() : 0;
+          // per-element form: one group per trace value, in element order
+          size_t g = per_element ? gm.size()
+                                 : (multi[fi][trl] ? std::find(gm.begin(), gm.end(), m) - gm.begin() : 0);
           if (g == gm.size()) {
             gm.push_back(m);
             members.emplace_back();
@@ -1303,12 +1310,14 @@ void Engine::contract(int single_block_pair, int kahan) {
       order_items_by_component(f, root);
       upload(f.d_gitems, f.gitems);
       upload(f.d_gpairs, f.gpairs);
-      plan_lstat(f, root, D_);
-      plan_gridr(f, root);
-      plan_gridg(f, root);
+      if (!per_element) {  // the per-element form runs every trace through the group kernel
+        plan_lstat(f, root, D_);
+        plan_gridr(f, root);
+        plan_gridg(f, root);
+      }
     }
     for (auto &cp : corrs) {
-      cp.inv_n.resize(cp.reqs.size());
+      cp.n_adm.resize(cp.reqs.size());
       for (size_t i = 0; i < cp.reqs.size(); ++i) {
         auto const &r = cp.reqs[i];
         auto gmk = [&](int k, int g) { return gmasks[fams[r[k].first].gmask_off[r[k].second] + g]; };
@@ -1331,7 +1340,7 @@ void Engine::contract(int single_block_pair, int kahan) {
           throw std::runtime_error(
               "compcomp_t inner_product(vector<DilutedTrace>, vector<DilutedTrace>) has an empty result. Not enough "
               "random vectors?");
-        cp.inv_n[i] = 1.0 / double(n);
+        cp.n_adm[i] = double(n);
       }
     }
     // ---- subset sums for the Moebius form of the masked inner products:
@@ -1353,7 +1362,8 @@ void Engine::contract(int single_block_pair, int kahan) {
           }
         }
       C.zmask.clear();
-      if (zs.size() > 1 && zs.size() <= 4096 && !std::getenv("SLAPH_RESOLVE_GROUPS")) C.zmask.assign(zs.begin(), zs.end());
+      if (zs.size() > 1 && zs.size() <= 4096 && !std::getenv("SLAPH_RESOLVE_GROUPS") && !per_element)
+        C.zmask.assign(zs.begin(), zs.end());
       const long long nz = (long long)C.zmask.size();
       for (size_t fi = 0; fi < fams.size(); ++fi) {
         Family &f = fams[fi];
@@ -1379,7 +1389,7 @@ void Engine::contract(int single_block_pair, int kahan) {
   const long long DD = (long long)D_ * D_;
   // SLAPH_TRACE_GROUPS=1 selects the one-wave-per-group trace kernel for every
   // trace (a cross-check of the tiled and left-stationary kernels).
-  const bool lstat_off = std::getenv("SLAPH_TRACE_GROUPS") != nullptr;
+  const bool lstat_off = std::getenv("SLAPH_TRACE_GROUPS") != nullptr || per_element;
   long long gstride_all = 0, npart_all = 0;
   for (auto const &f : fams) gstride_all += f.gstride;
   if (!lstat_off)
@@ -1599,7 +1609,7 @@ void Engine::contract(int single_block_pair, int kahan) {
           std::vector<CorrJob> jobs(cp.reqs.size());
           for (size_t i = 0; i < cp.reqs.size(); ++i) {
             jobs[i].ntr = int(cp.reqs[i].size());
-            jobs[i].inv_n = cp.inv_n[i];
+            jobs[i].n = cp.n_adm[i];
             for (size_t k = 0; k < cp.reqs[i].size(); ++k) {
               auto const &f = fams[cp.reqs[i][k].first];
               int trl = cp.reqs[i][k].second;
@@ -1636,15 +1646,12 @@ void Engine::contract(int single_block_pair, int kahan) {
   }
 
   // ---- accumulators and chunk buffers (kept with the plan) ----
-  auto &d_sum = C.d_sum, &d_comp = C.d_comp;
+  auto &d_sum = C.d_sum;
   d_sum.resize(corrs.size());
-  d_comp.resize(corrs.size());
   for (size_t c = 0; c < corrs.size(); ++c) {
     size_t n = (size_t)Lt_ * corrs[c].reqs.size();
     d_sum[c].alloc(sizeof(double2) * n);
-    d_comp[c].alloc(sizeof(double2) * n);
     SLAPH_HIP(hipMemsetAsync(d_sum[c].as<void>(), 0, sizeof(double2) * n, stream_));
-    SLAPH_HIP(hipMemsetAsync(d_comp[c].as<void>(), 0, sizeof(double2) * n, stream_));
   }
   {
     long long np_max = 1, ng_max = 1, nc_max = 1;
@@ -1762,9 +1769,9 @@ void Engine::contract(int single_block_pair, int kahan) {
       int ncorr = int(corrs[c].reqs.size());
       launch_resolve(C.d_gv.as<double2>(), C.d_gmasks.as<uint16_t>(), ch.d_jobs[c].as<CorrJob>(), ncorr, S,
                      C.d_zv.as<double2>(), C.d_zmask.as<uint16_t>(), int(C.zmask.size()), C.d_cval.as<double2>(),
-                     stream_);
+                     per_element ? 1 : 0, stream_);
       launch_accumulate(C.d_cval.as<double2>(), ch.d_tds.as<int>(), ch.d_tdg.as<int4>(), ch.ntdg, ncorr,
-                        d_sum[c].as<double2>(), d_comp[c].as<double2>(), kahan, stream_);
+                        d_sum[c].as<double2>(), stream_);
     }
   }
   // ---- results: correlator_[t][i].value() / Lt (Diagram.cpp:132-147),
@@ -1774,8 +1781,7 @@ void Engine::contract(int single_block_pair, int kahan) {
   for (auto const &cp : corrs) res_total += (size_t)Lt_ * cp.reqs.size();
   C.d_res.ensure(sizeof(double2) * std::max<size_t>(1, res_total));
   for (size_t c = 0, off = 0; c < corrs.size(); off += (size_t)Lt_ * corrs[c].reqs.size(), ++c)
-    launch_finalize(d_sum[c].as<double2>(), d_comp[c].as<double2>(), int(corrs[c].reqs.size()), Lt_,
-                    C.d_res.as<double2>() + off, stream_);
+    launch_finalize(d_sum[c].as<double2>(), int(corrs[c].reqs.size()), Lt_, C.d_res.as<double2>() + off, stream_);
   double kms = ctimer.stop();
   (void)kms;
   {
@@ -1803,6 +1809,55 @@ void Engine::contract(int single_block_pair, int kahan) {
   }
   times_.contract_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   guard.done = true;
+}
+
+cplx Engine::inner_product(int ntr, const cplx *const *vals, const uint16_t *const *used, const int *n, int kahan) {
+  if (ntr < 1 || ntr > 3) throw std::runtime_error("This many traces are not implemented yet.");
+  if (!vals || !used || !n) throw std::out_of_range("inner_product: null argument");
+  std::vector<cplx> hv;
+  std::vector<uint16_t> hm;
+  CorrJob job{};
+  job.ntr = ntr;
+  for (int k = 0; k < ntr; ++k) {
+    if (n[k] < 1 || !vals[k] || !used[k]) throw std::out_of_range("inner_product: empty trace vector");
+    job.tr[k].base = (long long)hv.size();
+    job.tr[k].stride = 0;
+    job.tr[k].ngroup = n[k];
+    job.tr[k].gmask_off = int(hm.size());
+    job.tr[k].zbase = -1;
+    job.tr[k].zstride = 0;
+    hv.insert(hv.end(), vals[k], vals[k] + n[k]);
+    hm.insert(hm.end(), used[k], used[k] + n[k]);
+  }
+  // number of admitted combinations (num_summands, DilutedTrace.cpp:15-40, 56-73)
+  long long adm = 0;
+  if (ntr == 1) {
+    adm = n[0];
+  } else if (ntr == 2) {
+    for (int g = 0; g < n[0]; ++g)
+      for (int h = 0; h < n[1]; ++h) adm += !(used[0][g] & used[1][h]);
+  } else {
+    for (int g = 0; g < n[0]; ++g)
+      for (int h = 0; h < n[1]; ++h)
+        for (int k = 0; k < n[2]; ++k) adm += !(used[0][g] & used[1][h] & used[2][k]);
+  }
+  if (adm == 0)
+    throw std::runtime_error(
+        "compcomp_t inner_product(vector<DilutedTrace>, vector<DilutedTrace>) has an empty result. Not enough random "
+        "vectors?");
+  job.n = double(adm);
+  SLAPH_HIP(hipSetDevice(device_));
+  DevBuf dv, dm, dj, dout;
+  upload(dv, hv);
+  upload(dm, hm);
+  upload(dj, std::vector<CorrJob>{job});
+  dout.alloc(sizeof(double2));
+  launch_resolve(dv.as<double2>(), dm.as<uint16_t>(), dj.as<CorrJob>(), 1, 1, nullptr, nullptr, 0, dout.as<double2>(),
+                 kahan ? 1 : 0, stream_);
+  cplx r;
+  SLAPH_HIP(hipMemcpyAsync(&r, dout.as<void>(), sizeof(cplx), hipMemcpyDeviceToHost, stream_));
+  SLAPH_HIP(hipStreamSynchronize(stream_));
+  return r;
 }
 
 }  // namespace slaph

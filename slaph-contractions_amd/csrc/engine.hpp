@@ -114,6 +114,10 @@ class Engine {
 
   // contract (Correlators.cpp:40-127); results kept on the host.
   void contract(int single_block_pair, int kahan);
+  // inner_product(DilutedTraces...) (DilutedTrace.cpp:5-82) of 2 or 3 trace
+  // vectors, or the mean of one (resolve_request1, Diagram.cpp:9-27), by the
+  // device resolve kernel; kahan != 0 sums like the SLAPH_KAHAN build.
+  cplx inner_product(int ntr, const cplx *const *vals, const uint16_t *const *used, const int *n, int kahan);
   std::vector<std::string> diagrams() const;
   int correlator_count(std::string const &diagram) const;
   std::string correlator_name(std::string const &diagram, int i) const;

@@ -1733,9 +1733,37 @@ __global__ void k_zeta(const double2 *__restrict__ gv, long long gbase, long lon
   zv[zbase + s * zstride + I.zoff + q] = z;
 }
 
+// Neumaier-Kahan accumulator of KahanAccumulator<Complex>::operator+=
+// (include/KahanAccumulator.hpp:23-32): the branch compares complex
+// magnitudes, value() = sum + compensation.
+struct KahanC {
+  double2 s, c;
+  __device__ __forceinline__ KahanC() : s(make_double2(0.0, 0.0)), c(make_double2(0.0, 0.0)) {}
+  __device__ __forceinline__ void add(double2 x) {
+    const double2 t = cadd(s, x);
+    if (hypot(s.x, s.y) >= hypot(x.x, x.y)) {
+      c.x += (s.x - t.x) + x.x;
+      c.y += (s.y - t.y) + x.y;
+    } else {
+      c.x += (x.x - t.x) + s.x;
+      c.y += (x.y - t.y) + s.y;
+    }
+    s = t;
+  }
+  __device__ __forceinline__ double2 value() const { return cadd(s, c); }
+};
+
+// One (slice pair, correlator): resolve_request{1,2,3} (Diagram.cpp:9-94).
+// Plain sums in the default form; with kahan != 0 every group is one trace
+// value (the planner's per-element form) and the sums run as in the
+// reference's SLAPH_KAHAN build: the mean of resolve_request1 through
+// std::accumulate into a Kahan accumulator, and right_sum and result of
+// inner_product (DilutedTrace.cpp:5-82) as Kahan accumulators, in the
+// reference's element order.  The result is divided by the number of
+// admitted combinations, as the reference does.
 __global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__restrict__ gm,
                           const CorrJob *__restrict__ jobs, int ncorr, int nsp, const double2 *__restrict__ zv,
-                          const uint16_t *__restrict__ zmask, int nz, double2 *__restrict__ cval) {
+                          const uint16_t *__restrict__ zmask, int nz, double2 *__restrict__ cval, int kahan) {
   // XCD x (workgroups are dealt to the XCDs round-robin) runs the x-th
   // contiguous eighth of the (slice pair, correlator) sequence, so each XCD's
   // L2 holds the trace values of its own slice pairs only.
@@ -1747,7 +1775,34 @@ __global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__rest
   const CorrJob J = jobs[corr];
   const double2 *v0 = gv + J.tr[0].base + s * J.tr[0].stride;
   double2 res = make_double2(0.0, 0.0);
-  if (J.ntr == 1) {
+  if (kahan) {
+    KahanC acc;
+    if (J.ntr == 1) {
+      for (int g = 0; g < J.tr[0].ngroup; ++g) acc.add(v0[g]);
+    } else if (J.ntr == 2) {
+      const double2 *v1 = gv + J.tr[1].base + s * J.tr[1].stride;
+      const uint16_t *m0 = gm + J.tr[0].gmask_off, *m1 = gm + J.tr[1].gmask_off;
+      for (int g = 0; g < J.tr[0].ngroup; ++g) {
+        KahanC rs;
+        for (int h = 0; h < J.tr[1].ngroup; ++h)
+          if (!(m0[g] & m1[h])) rs.add(v1[h]);
+        acc.add(cmul(v0[g], rs.value()));
+      }
+    } else {
+      const double2 *v1 = gv + J.tr[1].base + s * J.tr[1].stride;
+      const double2 *v2 = gv + J.tr[2].base + s * J.tr[2].stride;
+      const uint16_t *m0 = gm + J.tr[0].gmask_off, *m1 = gm + J.tr[1].gmask_off, *m2 = gm + J.tr[2].gmask_off;
+      for (int g = 0; g < J.tr[0].ngroup; ++g)
+        for (int h = 0; h < J.tr[1].ngroup; ++h) {
+          const uint16_t mgh = m0[g] & m1[h];
+          KahanC rs;
+          for (int k = 0; k < J.tr[2].ngroup; ++k)
+            if (!(mgh & m2[k])) rs.add(v2[k]);
+          acc.add(cmul(cmul(v0[g], v1[h]), rs.value()));
+        }
+    }
+    res = acc.value();
+  } else if (J.ntr == 1) {
     for (int g = 0; g < J.tr[0].ngroup; ++g) res = cadd(res, v0[g]);
   } else if (J.tr[0].zbase >= 0) {
     const double2 *z0 = zv + J.tr[0].zbase + s * J.tr[0].zstride;
@@ -1785,54 +1840,35 @@ __global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__rest
         res = cadd(res, cmul(cmul(v0[g], v1[h]), rs));
       }
   }
-  res.x *= J.inv_n;
-  res.y *= J.inv_n;
-  cval[(long long)s * ncorr + corr] = res;
+  cval[(long long)s * ncorr + corr] = make_double2(res.x / J.n, res.y / J.n);
 }
 
-// correlator_[t][i] += c (reference: Diagram::assemble, Diagram.cpp:96-114);
-// Neumaier compensation (KahanAccumulator.hpp:23-32) when kahan != 0.
-// One thread per (time separation, correlator): the slice pairs of that
-// separation are added in ascending order into a register copy of the sum,
-// the same sequence of additions as one pass over all slice pairs in order.
+// correlator_[t][i] += c (reference: Diagram::assemble, Diagram.cpp:96-114).
+// The reference gives every slice pair a fresh accumulator holding its one
+// value and merges it with KahanAccumulator::operator+=(KahanAccumulator)
+// (KahanAccumulator.hpp:40-44), which adds sums and compensations separately:
+// the compensation stays zero, so across slice pairs the sum is plain in both
+// builds.  One thread per (time separation, correlator): the slice pairs of
+// that separation are added in ascending order into a register copy of the
+// sum, the same sequence of additions as one pass over all slice pairs.
 __global__ void k_accumulate(const double2 *__restrict__ cval, const int *__restrict__ sidx,
-                             const int4 *__restrict__ tgroups, int ngroups, int ncorr, double2 *__restrict__ sum,
-                             double2 *__restrict__ comp, int kahan) {
+                             const int4 *__restrict__ tgroups, int ngroups, int ncorr, double2 *__restrict__ sum) {
   const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (id >= (long long)ngroups * ncorr) return;
   const int corr = int(id % ncorr);
   const int4 G = tgroups[id / ncorr];
   const long long at = (long long)G.x * ncorr + corr;
   double2 acc = sum[at];
-  if (!kahan) {
-    for (int k = 0; k < G.z; ++k) acc = cadd(acc, cval[(long long)sidx[G.y + k] * ncorr + corr]);
-  } else {
-    double2 c = comp[at];
-    for (int k = 0; k < G.z; ++k) {
-      const double2 x = cval[(long long)sidx[G.y + k] * ncorr + corr];
-      const double2 t = cadd(acc, x);
-      const double na = hypot(acc.x, acc.y), nx = hypot(x.x, x.y);
-      if (na >= nx) {
-        c.x += (acc.x - t.x) + x.x;
-        c.y += (acc.y - t.y) + x.y;
-      } else {
-        c.x += (x.x - t.x) + acc.x;
-        c.y += (x.y - t.y) + acc.y;
-      }
-      acc = t;
-    }
-    comp[at] = c;
-  }
+  for (int k = 0; k < G.z; ++k) acc = cadd(acc, cval[(long long)sidx[G.y + k] * ncorr + corr]);
   sum[at] = acc;
 }
 
-__global__ void k_finalize(const double2 *__restrict__ sum, const double2 *__restrict__ comp, int ncorr, int Lt,
-                           double2 *__restrict__ out) {
+__global__ void k_finalize(const double2 *__restrict__ sum, int ncorr, int Lt, double2 *__restrict__ out) {
   const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (id >= (long long)ncorr * Lt) return;
   const int t = int(id % Lt), i = int(id / Lt);
   const long long at = (long long)t * ncorr + i;
-  out[id] = make_double2((sum[at].x + comp[at].x) / double(Lt), (sum[at].y + comp[at].y) / double(Lt));
+  out[id] = make_double2(sum[at].x / double(Lt), sum[at].y / double(Lt));
 }
 
 __global__ void k_to_f32(const double2 *__restrict__ in, float2 *__restrict__ out, long long n) {
@@ -2316,10 +2352,10 @@ void launch_to_f32(const double2 *in, void *out, long long n, hipStream_t s) {
   SLAPH_HIP(hipGetLastError());
 }
 
-void launch_finalize(const double2 *sum, const double2 *comp, int ncorr, int Lt, double2 *out, hipStream_t s) {
+void launch_finalize(const double2 *sum, int ncorr, int Lt, double2 *out, hipStream_t s) {
   const long long n = (long long)ncorr * Lt;
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_finalize, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, sum, comp, ncorr, Lt, out);
+  hipLaunchKernelGGL(k_finalize, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, sum, ncorr, Lt, out);
   SLAPH_HIP(hipGetLastError());
 }
 
@@ -2334,20 +2370,20 @@ void launch_zeta(const double2 *gv, long long gbase, long long gstride, const ui
 }
 
 void launch_resolve(const double2 *gv, const uint16_t *gm, const CorrJob *jobs, int ncorr, int nsp,
-                    const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, hipStream_t s) {
+                    const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, int kahan, hipStream_t s) {
   long long n = (long long)ncorr * nsp;
   if (n <= 0) return;
   if ((n + 255) / 256 > 0x7fffffffLL) throw std::runtime_error("launch_resolve: grid too large");
   hipLaunchKernelGGL(k_resolve, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, gv, gm, jobs, ncorr, nsp, zv, zmask,
-                     nz, cval);
+                     nz, cval, kahan);
   SLAPH_HIP(hipGetLastError());
 }
 
 void launch_accumulate(const double2 *cval, const int *sidx, const int4 *tgroups, int ngroups, int ncorr,
-                       double2 *sum, double2 *comp, int kahan, hipStream_t s) {
+                       double2 *sum, hipStream_t s) {
   if (ncorr <= 0 || ngroups <= 0) return;
   hipLaunchKernelGGL(k_accumulate, grid1d((long long)ngroups * ncorr, 256), dim3(256), 0, s, cval, sidx, tgroups,
-                     ngroups, ncorr, sum, comp, kahan);
+                     ngroups, ncorr, sum);
   SLAPH_HIP(hipGetLastError());
 }
 

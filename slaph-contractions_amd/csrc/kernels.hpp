@@ -188,7 +188,7 @@ void launch_zeta(const double2 *gvals, long long gbase, long long gstride, const
                  hipStream_t s);
 struct CorrJob {
   int ntr;
-  double inv_n;  // 1 / number of admitted combinations (host-computed, exact)
+  double n;  // number of admitted combinations (host-computed, exact); the value is divided by it
   TraceRef tr[3];
 };
 // Trace group sums: out[obase + ti*ostride + dst] = sum over the item's
@@ -305,16 +305,18 @@ void launch_trace_lstat_q(const void *slabX, const long long *Xt, const void *sl
 // sum_{p in pidx[items[k].list_off .. + n)} part[ti*pstride + p], in list order.
 void launch_ls_reduce(const double2 *part, long long pstride, const GroupItem *items, int nitems, const int *pidx,
                       double2 *out, long long obase, long long ostride, int ntime, hipStream_t s);
+// kahan != 0: Kahan-Neumaier sums inside each (slice pair, correlator) in the
+// reference's element order (every group one trace value, no subset sums).
 void launch_resolve(const double2 *gvals, const uint16_t *gmasks, const CorrJob *jobs, int ncorr, int nsp,
-                    const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, hipStream_t s);
-// sum[t*ncorr + c] (+ Kahan-Neumaier compensation comp) += cval[s*ncorr + c]
-// for the slice pairs s of each time separation t, in the order listed:
-// tgroups[g] = (t, offset into sidx, count), sidx ascending within a group.
+                    const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, int kahan, hipStream_t s);
+// sum[t*ncorr + c] += cval[s*ncorr + c] for the slice pairs s of each time
+// separation t, in the order listed: tgroups[g] = (t, offset into sidx,
+// count), sidx ascending within a group.
 void launch_accumulate(const double2 *cval, const int *sidx, const int4 *tgroups, int ngroups, int ncorr,
-                       double2 *sum, double2 *comp, int kahan, hipStream_t s);
+                       double2 *sum, hipStream_t s);
 void launch_scale(double2 *x, long long n, double f, hipStream_t s);
-// Correlator values (Diagram.cpp:132-147): out[i*Lt + t] = (sum + comp)[t*ncorr + i] / Lt.
-void launch_finalize(const double2 *sum, const double2 *comp, int ncorr, int Lt, double2 *out, hipStream_t s);
+// Correlator values (Diagram.cpp:132-147): out[i*Lt + t] = sum[t*ncorr + i] / Lt.
+void launch_finalize(const double2 *sum, int ncorr, int Lt, double2 *out, hipStream_t s);
 
 // Synthetic inputs: seeded counter-based Gaussian / Z2xZ2 fills.
 void launch_fill_gauss(double2 *x, long long n, unsigned long long seed, hipStream_t s);

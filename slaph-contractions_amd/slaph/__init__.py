@@ -51,6 +51,8 @@ _SIGS = {
     "slc_return_vdaggerv": (_c.c_int, [_P, _c.c_int, _c.c_int, _D]),
     "slc_diluted_factor": (_c.c_int, [_P, _c.c_int, _I, _c.c_int, _D, _I, _c.c_int, _I]),
     "slc_contract": (_c.c_int, [_P, _c.c_int, _c.c_int]),
+    "slc_inner_product": (_c.c_int, [_P, _c.c_int, _c.POINTER(_D), _c.POINTER(_c.POINTER(_c.c_uint16)), _I,
+                                     _c.c_int, _D]),
     "slc_diagram_count": (_c.c_int, [_P, _I]),
     "slc_diagram_name": (_c.c_int, [_P, _c.c_int, _c.c_char_p, _c.c_int]),
     "slc_correlator_count": (_c.c_int, [_P, _c.c_char_p, _I]),
@@ -280,6 +282,23 @@ class DilutedFactorFactory:
                                                 ric.ctypes.data_as(_I), cap, _c.byref(n)))
         mats = out[: n.value * D * D].reshape(n.value, D, D).transpose(0, 2, 1)  # column-major
         return [(mats[i], (int(ric[2 * i]), int(ric[2 * i + 1]))) for i in range(n.value)]
+
+
+def inner_product(ctx: Context, *traces, kahan: bool = False) -> complex:
+    """inner_product(DilutedTraces, ...) (DilutedTrace.cpp:5-82) of 2 or 3
+    trace vectors, or the mean of one (resolve_request1, Diagram.cpp:9-27),
+    on the device.  Each trace vector is a sequence of (value, used_mask)."""
+    k = len(traces)
+    if not 1 <= k <= 3:
+        raise ValueError("1 to 3 trace vectors")
+    vals = [np.ascontiguousarray([complex(v) for v, _ in tr], dtype=np.complex128) for tr in traces]
+    used = [np.ascontiguousarray([int(m) for _, m in tr], dtype=np.uint16) for tr in traces]
+    vp = (_D * k)(*[_dptr(v) for v in vals])
+    up = (_c.POINTER(_c.c_uint16) * k)(*[u.ctypes.data_as(_c.POINTER(_c.c_uint16)) for u in used])
+    n = (_c.c_int * k)(*[len(tr) for tr in traces])
+    out = np.zeros(2)
+    ctx._check(_lib.slc_inner_product(ctx._h, k, vp, up, n, int(kahan), _dptr(out)))
+    return complex(out[0], out[1])
 
 
 def contract(ctx: Context, single_time_slice_combination: int = -1, kahan: bool = False):

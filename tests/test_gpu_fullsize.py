@@ -1,21 +1,40 @@
-"""Full-size consistency of the trace kernels on the 32^3 x 64 workload
-(N_ev = 220, 5 random vectors, TB16/EI4/DF4, C2c / C4cD / C4cB / C4cC at
-|P|^2 <= 4 and C6cD at P = 0): the correlators of the production kernels
-(queued left-stationary traces, slice-pair tiles grouped by right operand)
-against those of the block-form and item-wise kernels on the same
-device-generated inputs.  Both sets sum the same traces, in different orders,
-so every correlator must agree to rounding (1e-12 relative); the oracle cannot
-run at this size."""
+"""Full-size checks on the two production workloads, on device-generated inputs:
+
+* pipi32 -- 32^3 x 64, N_ev = 220, 27 momenta |p|^2 <= 3, 5 random vectors,
+  TB16 / EI4 / DF4, C2c / C4cD / C4cB / C4cC at |P|^2 <= 4 and C6cD at P = 0
+  (58,489 correlators);
+* c2 -- 24^3 x 48, N_ev = 120, 33 momenta |p|^2 <= 4, 5 random vectors,
+  TB4 / EI8 / DF4 (D = 32), C2c / C3c / C4cD / C4cB at |P|^2 <= 4.
+
+At these sizes the oracle checks what it can finish in seconds: VdaggerV of
+sampled (p, t) against numpy on the downloaded eigenvectors, sampled Q0 / Q1 /
+Q2 keys against the oracle's factories (fed the device VdaggerV), and one
+whole dilution block pair (the reference's single_time_slice_combination,
+Correlators.cpp:87-90) with a reduced operator list against the oracle's
+contract.  The full correlator sets of the production kernels are compared
+with those of the alternative kernels (block-form and item-wise traces;
+direct group loops for the masked inner products, generic products and the
+direct VdaggerV), which share no trace, product, resolve or VdaggerV kernel
+with them, at 1e-12 relative."""
 import sys
 
 import numpy as np
 import pytest
 
 import bench
+from oracle import slaph_oracle as o
 
 pytestmark = pytest.mark.gpu
 
 SEED = 20261203
+SWITCHES = ("SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS", "SLAPH_RESOLVE_GROUPS", "SLAPH_PROD_GENERIC",
+            "SLAPH_VDV_DIRECT", "SLAPH_TRACE_GROUPS", "SLAPH_LSTAT_WINDOW", "SLAPH_GRID_GROUPS")
+COUNTS = {"pipi32": 58489}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
 
 
 @pytest.fixture(scope="module")
@@ -24,17 +43,18 @@ def slaph():
     return s
 
 
-def _correlators(slaph, monkeypatch, alternative):
-    for var in ("SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS"):
-        if alternative:
+def _set_switches(monkeypatch, on):
+    for var in SWITCHES:
+        if var in on:
             monkeypatch.setenv(var, "1")
         else:
             monkeypatch.delenv(var, raising=False)
-    print("pipi32", "block-form / item-wise traces" if alternative else "production traces", flush=True)
-    sys.stdout.flush()
+
+
+def _correlators(slaph, text):
     ctx = slaph.Context(0)
     try:
-        ctx.configure(bench.infile(bench.WORKLOADS["pipi32"]))
+        ctx.configure(text)
         ctx.synthetic_inputs(SEED, True)
         slaph.OperatorFactory(ctx).create_operators()
         slaph.contract(ctx)
@@ -43,19 +63,143 @@ def _correlators(slaph, monkeypatch, alternative):
         ctx.close()
 
 
-def test_pipi32_trace_kernels_agree(slaph, monkeypatch):
-    prod = _correlators(slaph, monkeypatch, False)
-    alt = _correlators(slaph, monkeypatch, True)
-    assert prod.keys() == alt.keys()
+def _agree(a, b, tol=1e-12):
+    assert a.keys() == b.keys()
     worst, n = 0.0, 0
-    for d, vals in prod.items():
-        assert vals.keys() == alt[d].keys(), d
+    for d, vals in a.items():
+        assert vals.keys() == b[d].keys(), d
         for name, v in vals.items():
-            ref = alt[d][name]
-            scale = max(np.max(np.abs(ref)), 1e-300)
-            err = np.max(np.abs(v - ref)) / scale
+            ref = b[d][name]
+            err = np.max(np.abs(v - ref)) / max(np.max(np.abs(ref)), 1e-300)
+            assert err <= tol, (d, name, err)
+            worst = max(worst, err)
+            n += 1
+    return n, worst
+
+
+@pytest.mark.parametrize("workload", ["pipi32", "c2"])
+def test_production_and_alternative_kernels_agree(slaph, workload, monkeypatch):
+    """Every correlator of the workload from the production kernels against
+    (a) the block-form left-stationary and item-wise tile trace kernels and
+    (b) direct group loops for the masked inner products (no Moebius
+    inversion: at pipi32 this checks the 16-term subset sums of all 43,561
+    C6cD correlators), the generic product kernel and the direct VdaggerV."""
+    text = bench.infile(bench.WORKLOADS[workload])
+    _set_switches(monkeypatch, ())
+    say(workload, "production kernels")
+    prod = _correlators(slaph, text)
+    for alt in (("SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS"),
+                ("SLAPH_RESOLVE_GROUPS", "SLAPH_PROD_GENERIC", "SLAPH_VDV_DIRECT")):
+        _set_switches(monkeypatch, alt)
+        say(workload, "alternative kernels", "+".join(alt))
+        n, worst = _agree(prod, _correlators(slaph, text))
+        if workload in COUNTS:
+            assert n == COUNTS[workload]
+        say("  %d correlators, largest relative difference %.2e" % (n, worst))
+    _set_switches(monkeypatch, ())
+
+
+def _sampled_ops(lut):
+    """Up to four non-trivial VdaggerV lookup entries, one with every momentum
+    component nonzero when there is one."""
+    ids = [vid for vid, mom, _ in lut["vdv_lookup"] if vid != lut["index_of_unity"]]
+    full = [vid for vid, mom, _ in lut["vdv_lookup"] if all(c != 0 for c in mom)]
+    pick = sorted(set(ids[:2] + ids[-1:] + full[:1]))
+    return pick
+
+
+@pytest.mark.parametrize("workload", ["pipi32", "c2"])
+def test_vdaggerv_and_factors_sampled_against_oracle(slaph, workload, monkeypatch):
+    """VdaggerV of sampled (p, t) against numpy on the downloaded eigenvectors
+    (1e-12 relative to the largest entry), the p = 0 operator equal to the
+    identity, and sampled Q0 / Q1 / Q2 keys through slc_diluted_factor against
+    the oracle's DilutedFactorFactory restatement fed the device VdaggerV."""
+    _set_switches(monkeypatch, ())
+    w = bench.WORKLOADS[workload]
+    text = bench.infile(w)
+    cfg = o.parse_infile(text)
+    lut = o.init_lookup_tables(cfg)
+    ctx = slaph.Context(0)
+    ctx.configure(text)
+    ctx.synthetic_inputs(SEED, True)
+    ops = slaph.OperatorFactory(ctx)
+    ops.create_operators()
+    Lt, L, N = w["T"], w["L"], w["N"]
+    say(workload, "vdaggerv sampled")
+    eye = ops.return_vdaggerv(lut["index_of_unity"], 3)
+    assert np.array_equal(eye, np.eye(N))
+    moms = {vid: mom for vid, mom, _ in lut["vdv_lookup"]}
+    for t in (0, Lt // 2 + 1):
+        V = ctx.get_input("eigenvectors", t)
+        for vid in _sampled_ops(lut):
+            want = o.vdaggerv(V, o.create_momenta(L, L, L, moms[vid]))
+            got = ops.return_vdaggerv(vid, t)
+            err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+            assert err <= 1e-12, (t, moms[vid], err)
+        del V
+    say(workload, "factors sampled")
+    nrnd = w["nrnd"]
+    perams = [ctx.get_input("perambulator", r) for r in range(nrnd)]
+    rnds = [ctx.get_input("random_vector", r) for r in range(nrnd)]
+
+    class DeviceVdv(dict):
+        def __missing__(self, key):
+            m = ops.return_vdaggerv(key[0], key[1])
+            self[key] = m
+            return m
+
+    fac = o.Factories(cfg, lut, DeviceVdv(), perams, rnds, w["dilT"], w["dilE"])
+    nb = Lt // w["dilT"]
+    cases = [("Q0", (1,)), ("Q0", (Lt - 3,)), ("Q2", (0, 1, 0)), ("Q2", (nb - 1, Lt // 2, 1))]
+    if lut["ql_lookup"]["Q1"]:
+        cases += [("Q1", (2, 0)), ("Q1", (Lt - 1, nb - 1))]
+    for qt, key in cases:
+        dev = slaph.DilutedFactorFactory(ctx, qt)
+        want = fac.get(qt, key)
+        nop = len(lut["ql_lookup"][qt])
+        for op in sorted({0, nop // 2, nop - 1}):
+            got = dev.get(key, op)
+            assert len(got) == len(want[op])
+            for (gm, gric), (wm, wric, _) in zip(got, want[op]):
+                assert gric == wric
+                err = np.max(np.abs(gm - wm)) / np.max(np.abs(wm))
+                assert err <= 1e-12, (qt, key, op, err)
+    ctx.close()
+
+
+@pytest.mark.parametrize("workload", ["pipi32", "c2"])
+def test_single_block_pair_reduced_operators_against_oracle(slaph, workload, monkeypatch):
+    """One whole diagonal block pair (single_time_slice_combination = 0) of
+    the full-size lattice with the operator list cut to |p|^2 <= 1 and total
+    momentum 0: every correlator against the oracle's contract at 1e-12
+    norm-wise (C6cD left out: its 3-way inner products are too slow for the
+    oracle here; they are covered by the kernel agreement test)."""
+    _set_switches(monkeypatch, ())
+    w = bench.WORKLOADS[workload]
+    diagrams = [d for d in w["diagrams"] if d != "C6cD"]
+    text = bench.infile(w, pmax="0,1", diagrams=diagrams, total_momenta="0")
+    cfg = o.parse_infile(text)
+    lut = o.init_lookup_tables(cfg)
+    ctx = slaph.Context(0)
+    ctx.configure(text)
+    ctx.synthetic_inputs(SEED + 1, True)
+    ops = slaph.OperatorFactory(ctx)
+    ops.create_operators()
+    say(workload, "single block pair, |p|^2 <= 1")
+    slaph.contract(ctx, single_time_slice_combination=0)
+    Lt, nrnd = w["T"], w["nrnd"]
+    perams = [ctx.get_input("perambulator", r) for r in range(nrnd)]
+    rnds = [ctx.get_input("random_vector", r) for r in range(nrnd)]
+    vdv = {(vid, t): ops.return_vdaggerv(vid, t) for vid, _, _ in lut["vdv_lookup"] for t in range(Lt)}
+    want = o.contract(cfg, lut, vdv, perams, rnds, single_block_pair=0)
+    worst, n = 0.0, 0
+    for d, lst in want.items():
+        got = ctx.correlators(d)
+        for name, vals in lst:
+            err = np.max(np.abs(got[name] - vals)) / max(np.max(np.abs(vals)), 1e-300)
             assert err <= 1e-12, (d, name, err)
             worst = max(worst, err)
             n += 1
-    assert n == 58489
-    print("pipi32: %d correlators, largest relative difference %.2e" % (n, worst), flush=True)
+    assert n > 0
+    say("  %d correlators, worst relative error %.2e" % (n, worst))
+    ctx.close()

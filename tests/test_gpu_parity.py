@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import slaph_oracle as o
+from test_oracle import kahan_trace_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -90,6 +91,25 @@ def test_l4_diluted_factors_match_oracle(slaph, l4_ctx, l4_oracle):
             for (gm, gric), (wm, wric, _) in zip(got, want[op]):
                 assert gric == wric
                 assert np.max(np.abs(gm - wm)) < 1e-12, (qt, key, op)
+
+
+@pytest.mark.parametrize("case", kahan_trace_cases(), ids=lambda c: c[0])
+def test_device_inner_products_kahan_known_answers(slaph, case):
+    """The device resolve kernel on trace vectors built from the reference's
+    Kahan known answers (KahanAccumulatorTest.cpp:22-59): with kahan the
+    compensated value of the SLAPH_KAHAN build, without it the plain sum,
+    both bit-equal to the oracle."""
+    name, vecs, want_kahan, want_plain = case
+    say("inner product", name)
+    ctx = slaph.Context(0)
+    assert slaph.inner_product(ctx, *vecs, kahan=True) == want_kahan
+    assert slaph.inner_product(ctx, *vecs, kahan=False) == want_plain
+
+
+def test_device_inner_product_empty_result_raises(slaph):
+    ctx = slaph.Context(0)
+    with pytest.raises(slaph.SlaphError, match="has an empty result"):
+        slaph.inner_product(ctx, [(1.0, 3)], [(2.0, 1)])
 
 
 @pytest.mark.parametrize("kahan", [False, True])
@@ -212,6 +232,81 @@ def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypat
 
 
 _SMALL_WANT = {}  # oracle correlators per input file (same seeded inputs)
+
+
+def dilution_infile(dilE, N):
+    """4^3 x 8 lattice with eigenvector interlace dilE (D = 4 dilE), N a
+    ragged multiple of it, 4 random vectors, C2c / C3c / C4cD / C4cB / C4cC up
+    to |P|^2 = 1 (C3c reads Q1 through trQ1Q0Q2)."""
+    return """
+Lt = 8
+Lx = 4
+Ly = 4
+Lz = 4
+number_of_eigen_vec = {N}
+[quarks]
+quark = u:4:TB:2:EI:{dilE}:DF:4:synthetic
+[operator_lists]
+operator_list = g5.d0.p0,1
+[correlator_lists]
+correlator_list = C2c:Q0:Op0:Q0:Op0:P0,1
+correlator_list = C3c:Q0:Op0:Q0:Op0:Q0:Op0:P0,1
+correlator_list = C4cD:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1
+correlator_list = C4cB:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1
+correlator_list = C4cC:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1
+""".format(N=N, dilE=dilE)
+
+
+_DIL_WANT = {}  # (dilE, N) -> (oracle correlators, inputs)
+
+KERNEL_SWITCHES = ("SLAPH_TRACE_GROUPS", "SLAPH_PROD_GENERIC", "SLAPH_RESOLVE_GROUPS", "SLAPH_VDV_DIRECT",
+                   "SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS", "SLAPH_LSTAT_WINDOW", "SLAPH_GRID_GROUPS")
+
+
+@pytest.mark.parametrize("dilE,N", [(6, 14), (8, 20)], ids=["EI6-D24", "EI8-D32"])
+@pytest.mark.parametrize("kernels", ["production", "blocks+items", "window1+gridgroups", "groups+generic"])
+def test_dilution_d24_d32_matches_oracle(slaph, dilE, N, kernels, monkeypatch):
+    """D = 24 and D = 32 (the EI6 of the reference's infiles/pipi_I2.in and the
+    EI8 of the 24^3 x 48 production workload) with N not a multiple of dilE:
+    Q0 / Q1 / Q2 at dilE 6 and 8, the generic product kernel, and the
+    left-stationary trace kernels at D = 24 / 32 (queued and block form,
+    one-round descriptor windows) against the oracle at 1e-12 norm-wise, on
+    device-generated inputs downloaded for the oracle."""
+    say("dilution D=%d N=%d" % (4 * dilE, N), kernels)
+    on = {"production": (), "blocks+items": ("SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS"),
+          "window1+gridgroups": ("SLAPH_LSTAT_WINDOW", "SLAPH_GRID_GROUPS"),
+          "groups+generic": ("SLAPH_TRACE_GROUPS", "SLAPH_PROD_GENERIC", "SLAPH_RESOLVE_GROUPS",
+                             "SLAPH_VDV_DIRECT")}[kernels]
+    for var in KERNEL_SWITCHES:
+        if var in on:
+            monkeypatch.setenv(var, "1")
+        else:
+            monkeypatch.delenv(var, raising=False)
+    text = dilution_infile(dilE, N)
+    ctx = slaph.Context(0)
+    ctx.configure(text)
+    assert ctx.query("dilE") == dilE
+    ctx.synthetic_inputs(4000 + dilE, True)
+    slaph.OperatorFactory(ctx).create_operators()
+    slaph.contract(ctx)
+    if (dilE, N) not in _DIL_WANT:
+        cfg = o.parse_infile(text)
+        lut = o.init_lookup_tables(cfg)
+        ev = [ctx.get_input("eigenvectors", t) for t in range(cfg["Lt"])]
+        perams = [ctx.get_input("perambulator", r) for r in range(4)]
+        rnds = [ctx.get_input("random_vector", r) for r in range(4)]
+        _DIL_WANT[(dilE, N)] = o.contract(cfg, lut, o.build_vdaggerv(cfg, lut, ev), perams, rnds)
+    want = _DIL_WANT[(dilE, N)]
+    assert set(want) == set(ctx.diagrams()) == {"C2c", "C3c", "C4cD", "C4cB", "C4cC"}
+    worst = 0.0
+    for d, lst in want.items():
+        got = ctx.correlators(d)
+        assert len(got) == len(lst)
+        for name, vals in lst:
+            err = np.max(np.abs(got[name] - vals)) / max(np.max(np.abs(vals)), 1e-300)
+            worst = max(worst, err)
+            assert err <= 1e-12, (d, name, err)
+    say("  worst relative error %.2e" % worst)
 
 
 def _rel_errors(got, want):

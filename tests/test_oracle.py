@@ -63,6 +63,62 @@ def test_kahan_known_answers():
     assert acc.value() == summand
 
 
+def kahan_trace_cases():
+    """Trace vectors built from the gtest known answers
+    (KahanAccumulatorTest.cpp:22-59): (values, masks) whose compensated and
+    plain sums differ.  Returns (name, [trace vectors], kahan value, plain value)."""
+    eps = float(np.finfo(float).eps)
+    hard = [(1.0 + 0.5j, 0)] + [(complex(eps / 100, 0.25 * eps / 100), 0)] * 10000
+    exact_hard = complex(1.0 + 10000 * (eps / 100), 0.5 + 10000 * (0.25 * eps / 100))
+    tiny = eps * 1e-15
+    prec = [(1.0, 1), (tiny, 2), (-1.0, 4)]
+    one = [(1.0, 0)]
+    return [
+        # resolve_request1: mean of one trace vector
+        ("mean-hard", [hard], o.cdiv(exact_hard, len(hard)), o.cdiv(1.0 + 0.5j, len(hard))),
+        ("mean-precision", [prec], tiny / 3, 0.0),
+        # inner_product 2-way: right_sum carries the compensation
+        ("inner2-precision", [one, prec], tiny / 3, 0.0),
+        # inner_product 3-way: right_sum of the third vector
+        ("inner3-precision", [one, one, prec], tiny / 3, 0.0),
+        # result carries the compensation: left values differ, right sums are 1
+        ("inner2-result", [prec, one], tiny / 3, 0.0),
+    ]
+
+
+@pytest.mark.parametrize("case", kahan_trace_cases(), ids=lambda c: c[0])
+def test_kahan_inner_products_known_answers(case):
+    """SLAPH_KAHAN build: resolve_request1's mean and inner_product's
+    right_sum / result are Kahan accumulators (Diagram.cpp:24-25,
+    DilutedTrace.cpp:13-33, 51-76); the native build sums plainly."""
+    name, vecs, want_kahan, want_plain = case
+
+    def run(kahan):
+        if len(vecs) == 1:
+            return o.trace_mean(vecs[0], kahan)
+        if len(vecs) == 2:
+            return o.inner_product2(vecs[0], vecs[1], kahan)
+        return o.inner_product3(vecs[0], vecs[1], vecs[2], kahan)
+
+    assert run(True) == want_kahan
+    assert run(False) == want_plain
+    assert want_kahan != want_plain
+
+
+def test_kahan_contract_sums_slice_pairs_plainly():
+    """The reference's per-slice-pair accumulator holds one value with zero
+    compensation, and merging it (KahanAccumulator.hpp:40-44) adds sum and
+    compensation separately: across slice pairs the SLAPH_KAHAN build sums
+    plainly (Diagram.cpp:96-114)."""
+    acc = o.KahanAccumulator(0.0)
+    for x in (1.0, np.finfo(float).eps * 1e-15, -1.0):
+        one = o.KahanAccumulator(0.0)
+        one.add(x)
+        assert one.c == 0.0
+        acc.merge(one)
+    assert acc.value() == 0.0  # a compensated sum would keep the middle term
+
+
 # --- dilution iterator: tests/dilution-iterator/control.txt ----------------
 def test_dilution_iterator_control_text():
     txt = (o.dilution_iterator_text(6, 2, False) + o.dilution_iterator_text(6, 2, True)
