@@ -144,16 +144,26 @@ class Context:
         return v.value
 
     # inputs ---------------------------------------------------------------
+    def _expect_shape(self, what, a, shape):
+        # the C side copies a fixed element count; a wrong shape would be
+        # misread or read past its end
+        if tuple(a.shape) != tuple(shape):
+            raise SlaphError("%s must have shape %s, got %s" % (what, tuple(shape), tuple(a.shape)))
+
     def set_eigenvectors(self, t: int, V: np.ndarray):
         V = np.asfortranarray(V, dtype=np.complex128)
+        K = 3 * self.query("Lx") * self.query("Ly") * self.query("Lz")
+        self._expect_shape("eigenvectors (3 Lx Ly Lz, nev)", V, (K, self.query("nev")))
         self._check(_lib.slc_set_eigenvectors(self._h, t, _dptr(V.reshape(-1, order="F"))))
 
     def set_perambulator(self, r: int, P: np.ndarray):
         P = np.asfortranarray(P, dtype=np.complex128)
+        self._expect_shape("perambulator (rows, cols)", P, (self.query("peram_rows"), self.query("peram_cols")))
         self._check(_lib.slc_set_perambulator(self._h, r, _dptr(P.reshape(-1, order="F"))))
 
     def set_random_vector(self, r: int, rho: np.ndarray):
         rho = np.ascontiguousarray(rho, dtype=np.complex128)
+        self._expect_shape("random vector (Lt nev 4,)", rho, (self.query("Lt") * self.query("nev") * 4,))
         self._check(_lib.slc_set_random_vector(self._h, r, _dptr(rho)))
 
     def read_inputs(self, base_dir: str, config: int):
