@@ -162,6 +162,19 @@ int slc_ahb_device(slc_ctx *ctx, const double *A, const double *B, double *C, co
                    int K, long long lda, long long ldb, long long ldc, int batch, long long sA, long long sB,
                    long long sC);
 
+/* slc_ahb_device with no allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * result and kernel as slc_ahb_device, whose per-call job table (one entry
+ * per product of the batch) is allocated with hipMalloc; here it lives in a
+ * fixed device array of 1024 entries (64 KiB) that each GPU holds until the
+ * process ends.  batch above 1024 is rejected with SLC_ERR_ARG.  The array is
+ * shared by every context on a GPU and is overwritten by each call: calls
+ * that overlap on one GPU race.  The call returns when the product is done. */
+int slc_ahb_device_noalloc(slc_ctx *ctx, const double *A, const double *B, double *C, const double *phase, int M,
+                           int N, int K, long long lda, long long ldb, long long ldc, int batch, long long sA,
+                           long long sB, long long sC);
+
 /* The same product on host buffers (A: K x M, B: K x N, C: M x N, phase of
  * length ceil(K/3) or NULL); copies through the device.  For checks. */
 int slc_ahb_host(slc_ctx *ctx, const double *A, const double *B, double *C, const double *phase, int M, int N,

@@ -241,6 +241,18 @@ int slc_ahb_device(slc_ctx *ctx, const double *A, const double *B, double *C, co
   });
 }
 
+int slc_ahb_device_noalloc(slc_ctx *ctx, const double *A, const double *B, double *C, const double *phase, int M,
+                           int N, int K, long long lda, long long ldb, long long ldc, int batch, long long sA,
+                           long long sB, long long sC) {
+  return guarded(ctx, [&] {
+    if (M <= 0 || N <= 0 || K <= 0 || batch <= 0 || lda < K || ldb < K || ldc < M)
+      throw std::out_of_range("invalid matrix shape");
+    ctx->eng->ahb_noalloc(reinterpret_cast<const double2 *>(A), reinterpret_cast<const double2 *>(B),
+                          reinterpret_cast<double2 *>(C), reinterpret_cast<const double2 *>(phase), M, N, K, lda, ldb,
+                          ldc, batch, sA, sB, sC);
+  });
+}
+
 int slc_ahb_host(slc_ctx *ctx, const double *A, const double *B, double *C, const double *phase, int M, int N,
                  int K) {
   return guarded(ctx, [&] {

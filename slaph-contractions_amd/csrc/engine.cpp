@@ -63,6 +63,9 @@ Engine::Engine(int device) : device_(device) {
   SLAPH_HIP(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) throw std::runtime_error("invalid HIP device " + std::to_string(device));
   SLAPH_HIP(hipSetDevice(device));
+  // device address of the allocation-free entry's job table, looked up once
+  // per device with that device current
+  ahb_table_ = ahb_noalloc_table(device);
   SLAPH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 }
 
@@ -288,6 +291,23 @@ void Engine::ahb_raw(const double2 *A, const double2 *B, double2 *C, const doubl
   DevBuf dj;
   upload(dj, jobs);
   launch_ahb(dj.as<AhBJob>(), batch, M, N, K, stream_);
+  SLAPH_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::ahb_noalloc(const double2 *A, const double2 *B, double2 *C, const double2 *phase, int M, int N, int K,
+                         long long lda, long long ldb, long long ldc, int batch, long long sA, long long sB,
+                         long long sC) {
+  // No device allocation: the job table is built on the host stack and copied
+  // into the per-GPU fixed table, which the kernel reads; the call returns
+  // after the kernel has finished, so the table is free for the next call.
+  if (batch < 1 || batch > kAhbNoallocJobs)
+    throw std::out_of_range("slc_ahb_device_noalloc: batch must be between 1 and " +
+                            std::to_string(kAhbNoallocJobs) + " (size of the fixed job table)");
+  AhBJob jobs[kAhbNoallocJobs];
+  for (int i = 0; i < batch; ++i) jobs[i] = {A + i * sA, B + i * sB, C + i * sC, phase, nullptr, lda, ldb, ldc};
+  SLAPH_HIP(hipSetDevice(device_));
+  SLAPH_HIP(hipMemcpyAsync(ahb_table_, jobs, sizeof(AhBJob) * batch, hipMemcpyHostToDevice, stream_));
+  launch_ahb(ahb_table_, batch, M, N, K, stream_);
   SLAPH_HIP(hipStreamSynchronize(stream_));
 }
 

@@ -148,6 +148,10 @@ class Engine {
   // Raw entry: C = A^H diag(phase) B for device buffers (tests and tools).
   void ahb_raw(const double2 *A, const double2 *B, double2 *C, const double2 *phase, int M, int N, int K,
                long long lda, long long ldb, long long ldc, int batch, long long sA, long long sB, long long sC);
+  // The same product with no device allocation (job table in the per-GPU
+  // fixed table; at most kAhbNoallocJobs products per call).
+  void ahb_noalloc(const double2 *A, const double2 *B, double2 *C, const double2 *phase, int M, int N, int K,
+                   long long lda, long long ldb, long long ldc, int batch, long long sA, long long sB, long long sC);
 
  private:
   void require_configured() const;
@@ -161,6 +165,7 @@ class Engine {
 
   int device_;
   hipStream_t stream_ = nullptr;
+  AhBJob *ahb_table_ = nullptr;  // device address of this GPU's fixed A^H B job table
   GlobalData gd_;
   bool configured_ = false;
   bool have_ev_ = false, have_peram_ = false, have_rnd_ = false, have_ops_ = false;

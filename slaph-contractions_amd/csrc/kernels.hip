@@ -9,6 +9,7 @@
 #include "kernels.hpp"
 
 #include <cstdlib>
+#include <mutex>
 
 namespace slaph {
 
@@ -1950,6 +1951,31 @@ dim3 grid1d(long long n, int block) {
 }
 
 }  // namespace
+
+// Job table of the allocation-free A^H B entry (slc_ahb_device_noalloc): one
+// fixed array per GPU instead of a per-call allocation.  In host code the
+// name `scratch` is only a stand-in for the symbol; the device address comes
+// from hipGetSymbolAddress (ahb_noalloc_table) and is the only one passed on.
+static __device__ __align__(256) AhBJob scratch[kAhbNoallocJobs];
+
+AhBJob *ahb_noalloc_table(int device) {
+  static std::mutex mu;
+  static AhBJob *addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) throw std::out_of_range("device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    int cur = -1;
+    SLAPH_HIP(hipGetDevice(&cur));
+    if (cur != device) SLAPH_HIP(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) SLAPH_HIP(hipSetDevice(cur));
+    SLAPH_HIP(e);
+    if (!p) throw std::runtime_error("hipGetSymbolAddress returned no address for the A^H B job table");
+    addr[device] = static_cast<AhBJob *>(p);
+  }
+  return addr[device];
+}
 
 void launch_ahb(const AhBJob *d_jobs, int njobs, int M, int N, int K, hipStream_t s) {
   if (njobs <= 0) return;

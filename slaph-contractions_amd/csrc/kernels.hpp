@@ -29,6 +29,13 @@ struct AhBJob {
   long long lda, ldb, ldc;
 };
 void launch_ahb(const AhBJob *d_jobs, int njobs, int M, int N, int K, hipStream_t s);
+// Allocation-free form: the job table lives in a fixed device array of
+// kAhbNoallocJobs entries (64 KiB) per GPU.  ahb_noalloc_table(device) looks
+// its address up once per device (hipGetSymbolAddress, with that device
+// current) and returns the cached address afterwards.
+constexpr int kAhbNoallocJobs = 1024;
+constexpr int kMaxDevices = 64;
+AhBJob *ahb_noalloc_table(int device);
 // Grouped products of rpj-row jobs: group g stacks rows[job0 .. job0+njob)
 // (row r of job j at rows[j].A + r*lda) against one right matrix B (K x N,
 // ldb); job j's rows go to rows[j].C (rpj x N, ldc) scaled by rows[j].rowscale.

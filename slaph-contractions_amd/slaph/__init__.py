@@ -65,6 +65,9 @@ _SIGS = {
     "slc_ahb_device": (_c.c_int, [_P, _P, _P, _P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_longlong,
                                   _c.c_longlong, _c.c_longlong, _c.c_int, _c.c_longlong, _c.c_longlong,
                                   _c.c_longlong]),
+    "slc_ahb_device_noalloc": (_c.c_int, [_P, _P, _P, _P, _P, _c.c_int, _c.c_int, _c.c_int, _c.c_longlong,
+                                          _c.c_longlong, _c.c_longlong, _c.c_int, _c.c_longlong, _c.c_longlong,
+                                          _c.c_longlong]),
     "slc_ahb_host": (_c.c_int, [_P, _D, _D, _D, _D, _c.c_int, _c.c_int, _c.c_int]),
     "slc_lookup_dump": (_c.c_int, [_c.c_char_p, _c.c_char_p, _c.c_int]),
     "slc_dilution_text": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.c_char_p, _c.c_int]),
@@ -253,6 +256,13 @@ class Context:
         """C = A^H diag(phase[k/3]) B on device pointers (ints)."""
         self._check(_lib.slc_ahb_device(self._h, A_ptr, B_ptr, C_ptr, phase_ptr, M, N, K, lda, ldb, ldc,
                                         batch, sA, sB, sC))
+
+    def ahb_device_noalloc(self, A_ptr, B_ptr, C_ptr, phase_ptr, M, N, K, lda, ldb, ldc, batch=1, sA=0, sB=0,
+                           sC=0):
+        """ahb_device without any device allocation (job table in a fixed
+        per-GPU array; batch <= 1024; overlapping calls on one GPU race)."""
+        self._check(_lib.slc_ahb_device_noalloc(self._h, A_ptr, B_ptr, C_ptr, phase_ptr, M, N, K, lda, ldb, ldc,
+                                                batch, sA, sB, sC))
 
 
 class OperatorFactory:
