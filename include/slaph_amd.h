@@ -47,8 +47,16 @@ int slc_configure(slc_ctx *ctx, const char *infile_text);
 
 /* Sizes after slc_configure.  `what` is one of "Lt", "Lx", "Ly", "Lz", "nev",
  * "dilT", "dilE", "n_rnd", "peram_rows", "peram_cols", "n_vdaggerv",
- * "index_of_unity", "n_Q0", "n_Q1", "n_Q2". */
+ * "index_of_unity", "n_Q0", "n_Q1", "n_Q2", "start_config", "end_config",
+ * "delta_config". */
 int slc_query(const slc_ctx *ctx, const char *what, long long *out);
+
+/* String settings of the infile after slc_configure (GlobalData members,
+ * include/global_data.hpp:51-107): "output_path", "handling_vdaggerv",
+ * "path_vdaggerv", "path_config", "path_eigenvectors", "name_eigenvectors".
+ * Copies at most len bytes (NUL-terminated) and returns the bytes needed
+ * including the NUL, or a negative code. */
+int slc_query_string(const slc_ctx *ctx, const char *what, char *buf, int len);
 
 /* EigenVector::read_eigen_vector for one timeslice (reference:
  * src/EigenVector.cpp:35-76): V is (3 Lx Ly Lz) x nev, column-major. */
@@ -79,10 +87,27 @@ int slc_synthetic_inputs(slc_ctx *ctx, unsigned long long seed, int orthonormali
  * its inputs from files only); used to check device-generated inputs. */
 int slc_get_input(slc_ctx *ctx, int kind, int index, double *out);
 
-/* OperatorFactory::create_operators with handling_vdaggerv = build
- * (src/OperatorFactory.cpp:220-326, 507-520): VdaggerV(p, t) =
- * V(t)^+ diag(exp(-i p.x)) V(t) for every lookup entry, on the GPU. */
+/* OperatorFactory::create_operators(filename, rnd_vec, config)
+ * (src/OperatorFactory.cpp:220-326, 507-520) for the configuration of the
+ * last slc_read_inputs (0 before any): VdaggerV(p, t) =
+ * V(t)^+ diag(exp(-i p.x)) V(t) for every lookup entry, on the GPU, as the
+ * infile's handling_vdaggerv says: "build"; "write" (build, then store every
+ * non-trivial operator, :303-313); "read" (load them, :328-394); "liuming"
+ * (load Liuming's files, :396-493).  A missing file is an error (the
+ * reference constructs that runtime_error without throwing it). */
 int slc_create_operators(slc_ctx *ctx);
+/* The same for an explicit configuration number (names the VdaggerV files). */
+int slc_create_operators_config(slc_ctx *ctx, int config);
+
+/* Host only: the VdaggerV file names of the reference, kind 0 = written
+ * ("/<path>/cnfg%04d/operators.%04d.p_<pxpypz>.d_<disp|000>.t_%03d",
+ * OperatorFactory.cpp:224-226, 306-311), 1 = read
+ * ("/<path>/cnfg%04d/operators.%04d.p_<pxpypz>_.t_%03d", :331-333, 352-356),
+ * 2 = Liuming ("/<path>/VdaggerV.p<px>p<py>p<pz>.conf%04d", :399-427; the
+ * reader tries -p first).  displacement: pairs of characters ("<x>y") or "".
+ * Returns the bytes needed including the NUL (writes at most len). */
+int slc_vdaggerv_filename(int kind, const char *path, int config, const int *mom, const char *displacement, int t,
+                          char *buf, int len);
 
 /* OperatorFactory::return_vdaggerv (include/OperatorsForMesons.hpp:66-69):
  * copies the nev x nev matrix of lookup entry id at timeslice t to out. */
@@ -117,6 +142,21 @@ int slc_contract(slc_ctx *ctx, int single_time_slice_combination, int kahan);
  * "has an empty result" runtime_error. */
 int slc_inner_product(slc_ctx *ctx, int ntraces, const double *const *values, const uint16_t *const *used,
                       const int *n, int kahan, double *out);
+
+/* AbstractDilutedTraceFactory::get(slice_pair, locations) of the trace
+ * factory `trace` ("trQ1", "trQ1Q1", "trQ0Q2", "trQ1Q1Q1", "trQ1Q0Q2",
+ * "trQ1Q1Q1Q1", "trQ2Q0Q2Q0", "trQ2Q0Q2Q0Q2Q0"; include/DilutedTraceFactory.hpp:21-29,
+ * src/DilutedTraceFactory.cpp:13-238) at the slice pair (t_source, t_sink),
+ * each of its nloc trace times at the source (0) or the sink (1).  Fills the
+ * trace vector of every trace id in trace_indices_map order: counts[id]
+ * values, written one after another to values (complex) and used (their
+ * DilutedTrace::used_rnd_ids masks), in the reference's element order
+ * (factor_to_trace, src/DilutedTrace.cpp:84-165).  *n_ids and *total receive
+ * the number of trace ids and of values; when count_cap or cap is too small
+ * nothing is written and SLC_ERR_ARG is returned (call again with larger
+ * buffers).  Computed on the device, like slc_contract. */
+int slc_diluted_trace(slc_ctx *ctx, const char *trace, int t_source, int t_sink, const int *locations, int nloc,
+                      int *counts, int count_cap, double *values, uint16_t *used, int cap, int *n_ids, int *total);
 
 /* Results of the last slc_contract, per diagram (Diagram::write,
  * src/Diagram.cpp:132-147): names, counts, and n_corr x Lt complex values in

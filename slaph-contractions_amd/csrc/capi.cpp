@@ -101,8 +101,31 @@ int slc_query(const slc_ctx *cctx, const char *what, long long *out) {
     else if (w == "n_Q0") *out = (long long)gd.quarkline_lookuptable.at("Q0").size();
     else if (w == "n_Q1") *out = (long long)gd.quarkline_lookuptable.at("Q1").size();
     else if (w == "n_Q2") *out = (long long)gd.quarkline_lookuptable.at("Q2").size();
+    else if (w == "start_config") *out = gd.start_config;
+    else if (w == "end_config") *out = gd.end_config;
+    else if (w == "delta_config") *out = gd.delta_config;
     else throw std::out_of_range("unknown query " + w);
   });
+}
+
+int slc_query_string(const slc_ctx *cctx, const char *what, char *buf, int len) {
+  slc_ctx *ctx = const_cast<slc_ctx *>(cctx);
+  int need = 0;
+  const int rc = guarded(ctx, [&] {
+    if (!what) throw std::out_of_range("null argument");
+    auto const &gd = ctx->eng->gd();
+    if (gd.quarks.empty()) throw std::out_of_range("context is not configured");
+    std::string w = what, v;
+    if (w == "output_path") v = gd.path_output;
+    else if (w == "handling_vdaggerv") v = gd.handling_vdaggerv;
+    else if (w == "path_vdaggerv") v = gd.path_vdaggerv;
+    else if (w == "path_config") v = gd.path_config;
+    else if (w == "path_eigenvectors") v = gd.path_eigenvectors;
+    else if (w == "name_eigenvectors") v = gd.name_eigenvectors;
+    else throw std::out_of_range("unknown query " + w);
+    need = copy_string(v, buf, len);
+  });
+  return rc < 0 ? rc : need;
 }
 
 int slc_set_eigenvectors(slc_ctx *ctx, int t, const double *V) {
@@ -133,7 +156,36 @@ int slc_get_input(slc_ctx *ctx, int kind, int index, double *out) {
 }
 
 int slc_create_operators(slc_ctx *ctx) {
-  return guarded(ctx, [&] { ctx->eng->create_operators(); });
+  return guarded(ctx, [&] { ctx->eng->create_operators(ctx->eng->last_config()); });
+}
+
+int slc_create_operators_config(slc_ctx *ctx, int config) {
+  return guarded(ctx, [&] { ctx->eng->create_operators(config); });
+}
+
+int slc_vdaggerv_filename(int kind, const char *path, int config, const int *mom, const char *displacement, int t,
+                          char *buf, int len) {
+  try {
+    if (!path || !mom) return SLC_ERR_ARG;
+    slaph::Mom p{{mom[0], mom[1], mom[2]}};
+    std::string name;
+    if (kind == 0) {
+      slaph::Displacement d;
+      std::string ds = displacement ? displacement : "";
+      if (ds.size() % 2) return SLC_ERR_ARG;
+      for (size_t i = 0; i < ds.size(); i += 2) d.push_back({ds[i], ds[i + 1]});
+      name = slaph::vdaggerv_write_name(path, config, p, d, t);
+    } else if (kind == 1) {
+      name = slaph::vdaggerv_read_name(path, config, p, t);
+    } else if (kind == 2) {
+      name = slaph::vdaggerv_liuming_name(path, config, p);
+    } else {
+      return SLC_ERR_ARG;
+    }
+    return copy_string(name, buf, len);
+  } catch (std::exception const &) {
+    return SLC_ERR_RUNTIME;
+  }
 }
 
 int slc_return_vdaggerv(slc_ctx *ctx, int id, int t, double *out) {
@@ -157,6 +209,36 @@ int slc_inner_product(slc_ctx *ctx, int ntraces, const double *const *values, co
     slaph::cplx r = ctx->eng->inner_product(ntraces, v, used, n, kahan);
     out[0] = r.real();
     out[1] = r.imag();
+  });
+}
+
+int slc_diluted_trace(slc_ctx *ctx, const char *trace, int t_source, int t_sink, const int *locations, int nloc,
+                      int *counts, int count_cap, double *values, uint16_t *used, int cap, int *n_ids, int *total) {
+  return guarded(ctx, [&] {
+    if (!trace || (nloc > 0 && !locations) || !n_ids || !total) throw std::out_of_range("slc_diluted_trace: null argument");
+    std::vector<slaph::Loc> locs;
+    for (int i = 0; i < nloc; ++i) {
+      if (locations[i] != 0 && locations[i] != 1) throw std::out_of_range("location must be 0 (source) or 1 (sink)");
+      locs.push_back(locations[i] == 0 ? slaph::Loc::source : slaph::Loc::sink);
+    }
+    auto tv = ctx->eng->diluted_trace(trace, t_source, t_sink, locs);
+    long long need = 0;
+    for (auto const &v : tv) need += (long long)v.size();
+    *n_ids = int(tv.size());
+    *total = int(need);
+    if (int(tv.size()) > count_cap || need > cap || (need > 0 && (!counts || !values || !used)))
+      throw std::out_of_range("slc_diluted_trace: output capacity too small (" + std::to_string(tv.size()) +
+                              " trace ids, " + std::to_string(need) + " values)");
+    long long at = 0;
+    for (size_t id = 0; id < tv.size(); ++id) {
+      counts[id] = int(tv[id].size());
+      for (auto const &e : tv[id]) {
+        values[2 * at] = e.first.real();
+        values[2 * at + 1] = e.first.imag();
+        used[at] = e.second;
+        ++at;
+      }
+    }
   });
 }
 

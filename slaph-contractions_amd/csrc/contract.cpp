@@ -918,7 +918,18 @@ struct ContractCache {
   std::vector<DevBuf> d_sum;  // per diagram: Lt x n_corr accumulators
 };
 
-void Engine::contract(int single_block_pair, int kahan) {
+// A restricted contraction: the given correlator requests at one slice pair
+// (diluted_trace runs every trace id of one trace factory this way and reads
+// the per-element trace values back instead of correlators).
+struct Engine::ContractScope {
+  std::map<std::string, std::vector<CorrelatorRequest>> requests;
+  SlicePair slice{0, 0};
+};
+
+void Engine::contract(int single_block_pair, int kahan) { run_contract(plan_, single_block_pair, kahan, nullptr); }
+
+void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_block_pair, int kahan,
+                          ContractScope const *scope) {
   require_inputs();
   if (!have_ops_) throw std::runtime_error("operators have not been created");
   SLAPH_HIP(hipSetDevice(device_));
@@ -954,7 +965,7 @@ void Engine::contract(int single_block_pair, int kahan) {
   if (fresh) {
     // ---- families and correlators (time independent) ----
     std::map<std::pair<std::string, std::vector<Loc>>, int> fam_of;
-    for (auto const &kv : gd_.correlator_requests_map) {
+    for (auto const &kv : scope ? scope->requests : gd_.correlator_requests_map) {
       if (kv.second.empty()) continue;
       CorrPlan cp;
       cp.name = kv.first;
@@ -989,10 +1000,11 @@ void Engine::contract(int single_block_pair, int kahan) {
 
     // ---- the slice pairs of every block pair (Correlators.cpp:84-127) ----
     auto bps = block_pairs(Lt_, dilT);
-    for (size_t b = 0; b < bps.size(); ++b) {
+    for (size_t b = 0; b < bps.size() && !scope; ++b) {
       if (single_block_pair >= 0 && int(b) != single_block_pair) continue;
       bp_slices.push_back(slice_pairs(bps[b].first, bps[b].second, Lt_, dilT));
     }
+    if (scope) bp_slices.push_back({scope->slice});
     // (factor type, time key) of a leaf at trace times tt
     auto leaf_key = [&](LeafSpec const &ls, std::vector<int> const &tt) {
       if (ls.q == 0) return std::array<int, 4>{0, tt[ls.a], 0, 0};
@@ -1774,6 +1786,11 @@ void Engine::contract(int single_block_pair, int kahan) {
                         d_sum[c].as<double2>(), stream_);
     }
   }
+  if (scope) {  // the caller reads the trace values of the plan
+    SLAPH_HIP(hipStreamSynchronize(stream_));
+    guard.done = true;
+    return;
+  }
   // ---- results: correlator_[t][i].value() / Lt (Diagram.cpp:132-147),
   // formed on the device as [correlator][t] and copied once ----
   results_.resize(corrs.size());
@@ -1809,6 +1826,43 @@ void Engine::contract(int single_block_pair, int kahan) {
   }
   times_.contract_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   guard.done = true;
+}
+
+std::vector<std::vector<std::pair<cplx, uint16_t>>> Engine::diluted_trace(std::string const &name, int t_source,
+                                                                         int t_sink, std::vector<Loc> const &locs) {
+  // AbstractDilutedTraceFactory::get(slice_pair, locations)
+  // (DilutedTraceFactory.hpp:21-29, DilutedTraceFactory.cpp:13-238): the trace
+  // vectors of every trace id of this factory, one value per random-vector
+  // combination in the reference's order with its used-index mask.
+  require_inputs();
+  if (!have_ops_) throw std::runtime_error("operators have not been created");
+  auto it = gd_.trace_indices_map.find(name);
+  if (it == gd_.trace_indices_map.end()) throw std::out_of_range("unknown trace factory " + name);
+  if (int(locs.size()) != make_shape(name).ntimes)
+    throw std::out_of_range("trace factory " + name + " takes " + std::to_string(make_shape(name).ntimes) +
+                            " locations");
+  if (t_source < 0 || t_source >= Lt_ || t_sink < 0 || t_sink >= Lt_) throw std::out_of_range("timeslice out of range");
+  std::vector<std::vector<std::pair<cplx, uint16_t>>> out(it->second.size());
+  if (it->second.empty()) return out;
+  ContractScope sc;
+  for (size_t id = 0; id < it->second.size(); ++id) {
+    CorrelatorRequest r;
+    r.traces.push_back(TraceRequest{name, int(id), locs});
+    sc.requests["trace"].push_back(r);
+  }
+  sc.slice = SlicePair{t_source, t_sink};
+  std::shared_ptr<ContractCache> plan;
+  run_contract(plan, -1, 1, &sc);
+  ContractCache &C = *plan;
+  if (C.fams.size() != 1 || C.chunks.size() != 1) throw std::logic_error("diluted_trace: unexpected plan");
+  Family const &f = C.fams[0];
+  ChunkPlan const &ch = C.chunks[0];
+  std::vector<cplx> gv((size_t)ch.g_total);
+  SLAPH_HIP(hipMemcpy(gv.data(), C.d_gv.as<void>(), sizeof(cplx) * gv.size(), hipMemcpyDeviceToHost));
+  for (size_t trl = 0; trl < f.tr_ids.size(); ++trl)
+    for (int g = 0; g < f.ngroup[trl]; ++g)
+      out[f.tr_ids[trl]].push_back({gv[ch.gbase[0] + f.goff[trl] + g], C.gmasks[f.gmask_off[trl] + g]});
+  return out;
 }
 
 cplx Engine::inner_product(int ntr, const cplx *const *vals, const uint16_t *const *used, const int *n, int kahan) {

@@ -106,9 +106,12 @@ void Engine::require_configured() const {
   if (!configured_) throw std::runtime_error("context is not configured");
 }
 
+// Perambulators and random vectors: what the factors and the contraction
+// read.  Eigenvectors are needed only to build VdaggerV (create_operators
+// checks for them), not when the operators are read from files.
 void Engine::require_inputs() const {
   require_configured();
-  if (!have_ev_ || !have_peram_ || !have_rnd_) throw std::runtime_error("inputs are not set");
+  if (!have_peram_ || !have_rnd_) throw std::runtime_error("inputs are not set");
 }
 
 void Engine::alloc_inputs() {
@@ -163,6 +166,10 @@ void Engine::read_inputs(std::string const &base, int config) {
       set_random_vector(r, rnd.data());
     }
   }
+  config_ = config;
+  // the reference reads eigenvectors only to build VdaggerV (build_vdaggerv,
+  // OperatorFactory.cpp:263-277), not when the operators come from files
+  if (gd_.handling_vdaggerv == "read" || gd_.handling_vdaggerv == "liuming") return;
   GlobalData g2 = gd_;
   if (!g2.path_eigenvectors.empty() && g2.path_eigenvectors[0] != '/') g2.path_eigenvectors = base + "/" + g2.path_eigenvectors;
   std::string prefix = eigenvector_prefix(g2, config);
@@ -266,6 +273,7 @@ void Engine::synthetic_inputs(unsigned long long seed, bool orthonormalise) {
 
 void Engine::get_input(int kind, int index, cplx *out) const {
   require_inputs();
+  if (kind == 0 && !have_ev_) throw std::runtime_error("inputs are not set");
   const double2 *src = nullptr;
   size_t n = 0;
   if (kind == 0 && index >= 0 && index < Lt_) {
@@ -314,14 +322,22 @@ void Engine::ahb_noalloc(const double2 *A, const double2 *B, double2 *C, const d
 // ---------------------------------------------------------------------------
 // OperatorFactory::create_operators / build_vdaggerv (OperatorFactory.cpp:220-326)
 // ---------------------------------------------------------------------------
-void Engine::create_operators() {
-  require_inputs();
-  if (!gd_.handling_vdaggerv.empty() && gd_.handling_vdaggerv != "build")
-    throw std::runtime_error("handling_vdaggerv '" + gd_.handling_vdaggerv + "' is not supported (only build)");
+void Engine::create_operators(int config) {
+  // handling_vdaggerv (OperatorFactory::create_operators, OperatorFactory.cpp:507-520):
+  // "build" / "write" form VdaggerV from the eigenvectors ("write" also stores
+  // every non-trivial operator), "read" / "liuming" load stored operators.
+  require_configured();
+  const std::string &mode = gd_.handling_vdaggerv;
+  if (mode != "build" && mode != "write" && mode != "read" && mode != "liuming")
+    throw std::runtime_error("The flag handling_vdaggerv in input file is wrong!");
   SLAPH_HIP(hipSetDevice(device_));
   int nvdv = int(gd_.vdaggerv_lookup.size());
   int Vs = gd_.Lx * gd_.Ly * gd_.Lz;
   vdv_.alloc(sizeof(double2) * std::max(1, nvdv) * Lt_ * N_ * N_);
+  if (mode == "read" || mode == "liuming") {
+    read_vdaggerv_files(config, mode == "liuming");
+  } else {
+  if (!have_ev_) throw std::runtime_error("inputs are not set");
   if (!phases_ready_) {
     // Momentum phases exp(-i 2 pi p.x / L), site = (x Ly + y) Lz + z
     // (create_momenta, OperatorFactory.cpp:64-95); they depend on the input
@@ -354,6 +370,8 @@ void Engine::create_operators() {
     vdaggerv_fourier(ids);
   } else {
     vdaggerv_direct(ids);
+  }
+  if (mode == "write") write_vdaggerv_files(config);
   }
   // Daggered copies of the operators that some factor uses with -p
   // (build_VdaggerV_lookup's dagger flag, init_lookup_tables.cpp:257-259).
@@ -542,6 +560,63 @@ void Engine::vdaggerv_fourier(std::vector<int> const &ids) {
   times_.vdaggerv_kernel_ms = tm.stop();
   times_.vdaggerv_launches = launches;
   times_.vdaggerv_ms = times_.vdaggerv_kernel_ms;
+}
+
+// write_vdaggerv (OperatorFactory.cpp:144-168, 303-313): every non-trivial
+// operator at every timeslice, N x N column-major complex<double>, into
+// "/<path_vdaggerv>/cnfg%04d/" (created when missing).
+void Engine::write_vdaggerv_files(int config) {
+  SLAPH_HIP(hipStreamSynchronize(stream_));
+  char dir[2048];
+  std::snprintf(dir, sizeof dir, "/%s/cnfg%04d", gd_.path_vdaggerv.c_str(), config);
+  make_directories(dir);
+  std::vector<cplx> m((size_t)N_ * N_);
+  for (auto const &op : gd_.vdaggerv_lookup) {
+    if (op.id == gd_.index_of_unity) continue;
+    for (int t = 0; t < Lt_; ++t) {
+      SLAPH_HIP(hipMemcpy(m.data(), vdv_ptr(op.id, false, t), sizeof(cplx) * m.size(), hipMemcpyDeviceToHost));
+      write_binary(vdaggerv_write_name(gd_.path_vdaggerv, config, op.mom, op.disp, t), m.data(),
+                   sizeof(cplx) * m.size());
+    }
+  }
+}
+
+// read_vdaggerv (OperatorFactory.cpp:328-394): one file per (operator, t),
+// column-major.  read_vdaggerv_liuming (:396-493): one file per operator with
+// the Lt matrices one after another, row-major; the file of -p holds the
+// adjoint of VdaggerV(p) (adjointInPlace), that of +p is taken as it is.  The
+// identity operator is not read.  A file that cannot be read is an error here
+// (the reference constructs the runtime_error without throwing it).
+void Engine::read_vdaggerv_files(int config, bool liuming) {
+  std::vector<cplx> m((size_t)N_ * N_), all;
+  for (auto const &op : gd_.vdaggerv_lookup) {
+    double2 *dst = vdv_.as<double2>() + (size_t)op.id * Lt_ * N_ * N_;
+    if (op.id == gd_.index_of_unity) {
+      launch_identity(dst, N_, Lt_, stream_);
+      continue;
+    }
+    if (!liuming) {
+      for (int t = 0; t < Lt_; ++t) {
+        read_binary(vdaggerv_read_name(gd_.path_vdaggerv, config, op.mom, t), m.data(), sizeof(cplx) * m.size());
+        SLAPH_HIP(hipMemcpy(dst + (size_t)t * N_ * N_, m.data(), sizeof(cplx) * m.size(), hipMemcpyHostToDevice));
+      }
+      continue;
+    }
+    const Mom neg{{-op.mom[0], -op.mom[1], -op.mom[2]}};
+    const std::string f1 = vdaggerv_liuming_name(gd_.path_vdaggerv, config, neg),
+                      f2 = vdaggerv_liuming_name(gd_.path_vdaggerv, config, op.mom);
+    const bool first = file_exists(f1);
+    if (!first && !file_exists(f2)) throw std::runtime_error("can't open " + f1 + " NOR " + f2);
+    all.resize((size_t)Lt_ * N_ * N_);
+    read_binary(first ? f1 : f2, all.data(), sizeof(cplx) * all.size());
+    for (int t = 0; t < Lt_; ++t) {
+      const cplx *rm = all.data() + (size_t)t * N_ * N_;  // row-major: element (r, c) at r N + c
+      for (int c = 0; c < N_; ++c)
+        for (int r = 0; r < N_; ++r)
+          m[(size_t)c * N_ + r] = first ? std::conj(rm[(size_t)c * N_ + r]) : rm[(size_t)r * N_ + c];
+      SLAPH_HIP(hipMemcpy(dst + (size_t)t * N_ * N_, m.data(), sizeof(cplx) * m.size(), hipMemcpyHostToDevice));
+    }
+  }
 }
 
 const double2 *Engine::vdv_ptr(int id, bool dagger, int t) const {

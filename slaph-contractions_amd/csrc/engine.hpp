@@ -104,8 +104,10 @@ class Engine {
   void synthetic_inputs(unsigned long long seed, bool orthonormalise);
   void get_input(int kind, int index, cplx *out) const;
 
-  // OperatorFactory::create_operators (build) / return_vdaggerv.
-  void create_operators();
+  // OperatorFactory::create_operators / return_vdaggerv; config names the
+  // VdaggerV files of handling_vdaggerv = write / read / liuming.
+  void create_operators(int config);
+  int last_config() const { return config_; }
   void return_vdaggerv(int id, int t, cplx *out);
 
   // DilutedFactorFactory<Q>::operator[](key) for one operator index: copies
@@ -118,6 +120,12 @@ class Engine {
   // vectors, or the mean of one (resolve_request1, Diagram.cpp:9-27), by the
   // device resolve kernel; kahan != 0 sums like the SLAPH_KAHAN build.
   cplx inner_product(int ntr, const cplx *const *vals, const uint16_t *const *used, const int *n, int kahan);
+  // AbstractDilutedTraceFactory::get (DilutedTraceFactory.hpp:21-29): for
+  // trace factory `name` at slice pair (t_source, t_sink) with the trace's
+  // locations, the trace vector of every trace id: (value, used mask) per
+  // random-vector combination, computed on the device.
+  std::vector<std::vector<std::pair<cplx, uint16_t>>> diluted_trace(std::string const &name, int t_source,
+                                                                    int t_sink, std::vector<Loc> const &locs);
   std::vector<std::string> diagrams() const;
   int correlator_count(std::string const &diagram) const;
   std::string correlator_name(std::string const &diagram, int i) const;
@@ -162,6 +170,12 @@ class Engine {
   void build_factors(std::vector<std::vector<int>> const &q1keys, std::vector<std::vector<int>> const &q2keys);
   void vdaggerv_fourier(std::vector<int> const &ids);
   void vdaggerv_direct(std::vector<int> const &ids);
+  void write_vdaggerv_files(int config);
+  void read_vdaggerv_files(int config, bool liuming);
+  struct ContractScope;  // contract.cpp
+  // contract() on `plan` (the plan member, or a private plan of a scoped
+  // call); with a scope, only its requests at its slice pair, and no results.
+  void run_contract(std::shared_ptr<ContractCache> &plan, int single_block_pair, int kahan, ContractScope const *scope);
 
   int device_;
   hipStream_t stream_ = nullptr;
@@ -169,6 +183,7 @@ class Engine {
   GlobalData gd_;
   bool configured_ = false;
   bool have_ev_ = false, have_peram_ = false, have_rnd_ = false, have_ops_ = false;
+  int config_ = 0;  // gauge configuration of the last read_inputs
   long long K_ = 0;
   int N_ = 0, D_ = 0, Lt_ = 0, nb_ = 0, nrnd_ = 0;
 

@@ -8,6 +8,8 @@
 #include <fstream>
 #include <sstream>
 #include <unordered_map>
+#include <cerrno>
+#include <sys/stat.h>
 
 namespace slaph {
 
@@ -332,6 +334,16 @@ void parse_infile(GlobalData &gd, std::string const &text) {
       gd.path_output = v;
     } else if (k == "handling_vdaggerv") {
       gd.handling_vdaggerv = v;
+    } else if (k == "path_vdaggerv") {
+      gd.path_vdaggerv = v;
+    } else if (k == "path_config") {
+      gd.path_config = v;
+    } else if (k == "alpha1") {
+      gd.hyp_alpha1 = std::stod(v);
+    } else if (k == "alpha2") {
+      gd.hyp_alpha2 = std::stod(v);
+    } else if (k == "iterations") {
+      gd.hyp_iterations = to_int(v);
     } else if (k == "start_config") {
       gd.start_config = to_int(v);
     } else if (k == "end_config") {
@@ -641,6 +653,30 @@ std::string eigenvector_prefix(GlobalData const &gd, int config) {
   return buf;
 }
 
+std::string vdaggerv_write_name(std::string const &path, int config, Mom const &p, Displacement const &d, int t) {
+  // "/%s/cnfg%04d/" + "operators.%04d.p_%s.d_%s.t_%03d" (OperatorFactory.cpp:224-226, 306-311);
+  // the displacement string is "000" when there is none (global_data_typedefs.hpp:107-116)
+  char buf[2048];
+  std::snprintf(buf, sizeof buf, "/%s/cnfg%04d/operators.%04d.p_%s.d_%s.t_%03d", path.c_str(), config, config,
+                mom_string(p).c_str(), disp_string(d).c_str(), t);
+  return buf;
+}
+
+std::string vdaggerv_read_name(std::string const &path, int config, Mom const &p, int t) {
+  // "/%s/cnfg%04d/operators.%04d" + ".p_" + digits + "_.t_%03d" (OperatorFactory.cpp:331-333, 352-356)
+  char buf[2048];
+  std::snprintf(buf, sizeof buf, "/%s/cnfg%04d/operators.%04d.p_%s_.t_%03d", path.c_str(), config, config,
+                mom_string(p).c_str(), t);
+  return buf;
+}
+
+std::string vdaggerv_liuming_name(std::string const &path, int config, Mom const &p) {
+  // "/%s/VdaggerV." + "p" x "p" y "p" z ".conf" + "%04d" (OperatorFactory.cpp:399-427)
+  char buf[2048];
+  std::snprintf(buf, sizeof buf, "/%s/VdaggerV.p%dp%dp%d.conf%04d", path.c_str(), p[0], p[1], p[2], config);
+  return buf;
+}
+
 void resort_perambulator(const cplx *file, cplx *out, int Lt, int nev, int dilT, int dilE, int dilD) {
   int nb = Lt / dilT;
   long long rows = 4LL * nev * Lt, ninv = (long long)nb * dilE * dilD;
@@ -656,6 +692,32 @@ void resort_perambulator(const cplx *file, cplx *out, int Lt, int nev, int dilT,
               long long col = (long long)dilE * dilD * b + (long long)dilE * dd + e;
               out[col * rows + row] = file[frow * ninv + fcol];
             }
+}
+
+void write_binary(std::string const &path, const void *src, size_t bytes) {
+  FILE *fp = std::fopen(path.c_str(), "wb");
+  if (!fp) throw std::runtime_error("can't open " + path);
+  size_t put = std::fwrite(src, 1, bytes, fp);
+  const bool ok = std::fclose(fp) == 0 && put == bytes;
+  if (!ok) throw std::runtime_error("Problems while write to " + path);
+}
+
+bool file_exists(std::string const &path) {
+  struct stat st;
+  return stat(path.c_str(), &st) == 0 && S_ISREG(st.st_mode);
+}
+
+void make_directories(std::string const &path) {
+  std::string cur;
+  for (size_t at = 0; at <= path.size();) {
+    size_t next = path.find('/', at);
+    if (next == std::string::npos) next = path.size();
+    cur = path.substr(0, next);
+    struct stat st;
+    if (!cur.empty() && stat(cur.c_str(), &st) != 0 && mkdir(cur.c_str(), 0777) != 0 && errno != EEXIST)
+      throw std::runtime_error("cannot create directory " + cur);
+    at = next + 1;
+  }
 }
 
 void read_binary(std::string const &path, void *dst, size_t bytes) {

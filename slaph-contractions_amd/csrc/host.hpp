@@ -109,7 +109,11 @@ struct GlobalData {
   int Lx = 0, Ly = 0, Lz = 0, Lt = 0;
   int nev = 0;
   int start_config = -1, end_config = 0, delta_config = 1;
-  std::string path_eigenvectors, name_eigenvectors, path_output, handling_vdaggerv;
+  // defaults of read_parameters (reference: src/global_data.cpp:100-160)
+  std::string path_eigenvectors = ".", name_eigenvectors = "eigenvector", path_output = "../../contractions",
+              handling_vdaggerv = "build", path_vdaggerv, path_config;
+  double hyp_alpha1 = 0, hyp_alpha2 = 0;  // HypPars (reference: global_data_typedefs.hpp)
+  int hyp_iterations = 0;
   std::vector<Quark> quarks;
   std::vector<Operators> operator_list;
   std::vector<CorrelatorSpec> correlator_list;
@@ -158,8 +162,18 @@ int time_delta(int t_source, int t_sink, int Lt);  // Correlators.cpp:21-23
 std::string perambulator_filename(Quark const &q, int config, int r, int Lt, int Vs);
 std::string randomvector_filename(Quark const &q, int config, int r, int nev);
 std::string eigenvector_prefix(GlobalData const &gd, int config);
+// VdaggerV cache files (reference: OperatorFactory.cpp:224-226, 303-313,
+// 331-364, 399-427).  The reference prefixes path_vdaggerv with "/" in all
+// three names; written and read names differ (".d_<disp>.t_%03d" against
+// "_.t_%03d").
+std::string vdaggerv_write_name(std::string const &path, int config, Mom const &p, Displacement const &d, int t);
+std::string vdaggerv_read_name(std::string const &path, int config, Mom const &p, int t);
+std::string vdaggerv_liuming_name(std::string const &path, int config, Mom const &p);
 // File layout -> column-major (rows 4N*t+N*d+v, cols dilE*dilD*b+dilE*d'+e).
 void resort_perambulator(const cplx *file, cplx *out, int Lt, int nev, int dilT, int dilE, int dilD);
 void read_binary(std::string const &path, void *dst, size_t bytes);
+void write_binary(std::string const &path, const void *src, size_t bytes);
+bool file_exists(std::string const &path);
+void make_directories(std::string const &path);  // every missing directory of the path
 
 }  // namespace slaph

@@ -48,11 +48,17 @@ _SIGS = {
     "slc_synthetic_inputs": (_c.c_int, [_P, _c.c_ulonglong, _c.c_int]),
     "slc_get_input": (_c.c_int, [_P, _c.c_int, _c.c_int, _D]),
     "slc_create_operators": (_c.c_int, [_P]),
+    "slc_create_operators_config": (_c.c_int, [_P, _c.c_int]),
+    "slc_vdaggerv_filename": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_int, _I, _c.c_char_p, _c.c_int, _c.c_char_p,
+                                         _c.c_int]),
+    "slc_query_string": (_c.c_int, [_P, _c.c_char_p, _c.c_char_p, _c.c_int]),
     "slc_return_vdaggerv": (_c.c_int, [_P, _c.c_int, _c.c_int, _D]),
     "slc_diluted_factor": (_c.c_int, [_P, _c.c_int, _I, _c.c_int, _D, _I, _c.c_int, _I]),
     "slc_contract": (_c.c_int, [_P, _c.c_int, _c.c_int]),
     "slc_inner_product": (_c.c_int, [_P, _c.c_int, _c.POINTER(_D), _c.POINTER(_c.POINTER(_c.c_uint16)), _I,
                                      _c.c_int, _D]),
+    "slc_diluted_trace": (_c.c_int, [_P, _c.c_char_p, _c.c_int, _c.c_int, _I, _c.c_int, _I, _c.c_int, _D,
+                                     _c.POINTER(_c.c_uint16), _c.c_int, _I, _I]),
     "slc_diagram_count": (_c.c_int, [_P, _I]),
     "slc_diagram_name": (_c.c_int, [_P, _c.c_int, _c.c_char_p, _c.c_int]),
     "slc_correlator_count": (_c.c_int, [_P, _c.c_char_p, _I]),
@@ -97,6 +103,19 @@ def lookup_dump(infile_text: str) -> str:
     rc = _lib.slc_lookup_dump(infile_text.encode(), buf, len(buf))
     if rc < 0:
         raise SlaphError(buf.value.decode())
+    return buf.value.decode()
+
+
+def vdaggerv_filename(kind: str, path: str, config: int, mom, displacement: str = "", t: int = 0) -> str:
+    """The reference's VdaggerV file names; kind "write", "read" or "liuming"."""
+    k = {"write": 0, "read": 1, "liuming": 2}[kind]
+    m = (_c.c_int * 3)(*[int(x) for x in mom])
+    args = (k, path.encode(), int(config), m, displacement.encode(), int(t))
+    need = _lib.slc_vdaggerv_filename(*args, None, 0)
+    if need < 0:
+        raise SlaphError("invalid VdaggerV file name arguments")
+    buf = _c.create_string_buffer(need)
+    _lib.slc_vdaggerv_filename(*args, buf, need)
     return buf.value.decode()
 
 
@@ -145,6 +164,12 @@ class Context:
         v = _c.c_longlong()
         self._check(_lib.slc_query(self._h, what.encode(), _c.byref(v)))
         return v.value
+
+    def query_string(self, what: str) -> str:
+        need = self._check(_lib.slc_query_string(self._h, what.encode(), None, 0))
+        buf = _c.create_string_buffer(need)
+        self._check(_lib.slc_query_string(self._h, what.encode(), buf, need))
+        return buf.value.decode()
 
     # inputs ---------------------------------------------------------------
     def _expect_shape(self, what, a, shape):
@@ -271,8 +296,14 @@ class OperatorFactory:
     def __init__(self, ctx: Context):
         self.ctx = ctx
 
-    def create_operators(self):
-        self.ctx._check(_lib.slc_create_operators(self.ctx._h))
+    def create_operators(self, config=None):
+        """create_operators for the configuration of the last read_inputs, or
+        `config` (it names the VdaggerV files of handling_vdaggerv
+        write / read / liuming)."""
+        if config is None:
+            self.ctx._check(_lib.slc_create_operators(self.ctx._h))
+        else:
+            self.ctx._check(_lib.slc_create_operators_config(self.ctx._h, int(config)))
 
     def return_vdaggerv(self, index: int, t: int) -> np.ndarray:
         N = self.ctx.query("nev")
@@ -302,6 +333,42 @@ class DilutedFactorFactory:
                                                 ric.ctypes.data_as(_I), cap, _c.byref(n)))
         mats = out[: n.value * D * D].reshape(n.value, D, D).transpose(0, 2, 1)  # column-major
         return [(mats[i], (int(ric[2 * i]), int(ric[2 * i + 1]))) for i in range(n.value)]
+
+
+class DilutedTraceFactory:
+    """AbstractDilutedTraceFactory (include/DilutedTraceFactory.hpp:21-29) of one
+    trace type: get((t_source, t_sink), locations) -> {trace id: [(value, used mask)]},
+    the reference's DilutedTracesMap, computed on the device."""
+
+    def __init__(self, ctx: Context, trace: str):
+        self.ctx = ctx
+        self.trace = trace
+
+    def get(self, slice_pair, locations):
+        locs = [0 if l in (0, "source") else 1 for l in locations]
+        lp = (_c.c_int * max(1, len(locs)))(*locs)
+        n_ids, total = _c.c_int(), _c.c_int()
+        cap_ids, cap = 64, 4096
+        for _ in range(2):
+            counts = np.zeros(cap_ids, dtype=np.int32)
+            vals = np.zeros(2 * cap)
+            used = np.zeros(cap, dtype=np.uint16)
+            rc = _lib.slc_diluted_trace(self.ctx._h, self.trace.encode(), int(slice_pair[0]), int(slice_pair[1]), lp,
+                                        len(locs), counts.ctypes.data_as(_I), cap_ids, _dptr(vals),
+                                        used.ctypes.data_as(_c.POINTER(_c.c_uint16)), cap, _c.byref(n_ids),
+                                        _c.byref(total))
+            if rc == 0:
+                break
+            if n_ids.value <= cap_ids and total.value <= cap:
+                self.ctx._check(rc)
+            cap_ids, cap = max(cap_ids, n_ids.value), max(cap, total.value)
+        self.ctx._check(rc)
+        out, at = {}, 0
+        for i in range(n_ids.value):
+            c = int(counts[i])
+            out[i] = [(complex(vals[2 * k], vals[2 * k + 1]), int(used[k])) for k in range(at, at + c)]
+            at += c
+        return out
 
 
 def inner_product(ctx: Context, *traces, kahan: bool = False) -> complex:

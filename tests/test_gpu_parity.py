@@ -93,6 +93,37 @@ def test_l4_diluted_factors_match_oracle(slaph, l4_ctx, l4_oracle):
                 assert np.max(np.abs(gm - wm)) < 1e-12, (qt, key, op)
 
 
+def test_l4_diluted_trace_factories_match_oracle(slaph, l4_ctx, l4_oracle):
+    """DilutedTraceFactory::get through slc_diluted_trace for all eight trace
+    types, at three slice pairs (same block, different blocks both ways):
+    every trace id's vector of (value, used-index mask) against the oracle's
+    trace factories, masks exactly and values at 1e-12 of the vector's scale."""
+    ctx, ops = l4_ctx
+    cfg, lut, perams, rnds, vdv = l4_oracle
+    q = cfg["quarks"][0]
+    fac = o.Factories(cfg, lut, vdv, perams, rnds, q["dilT"], q["dilE"])
+    locs_of = {}
+    for reqs in lut["requests"].values():
+        for trs, _ in reqs:
+            for name, _, locs in trs:
+                locs_of.setdefault(name, locs)
+    assert len(locs_of) == 8
+    for name, locs in sorted(locs_of.items()):
+        dev = slaph.DilutedTraceFactory(ctx, name)
+        for t_so, t_si in ((0, 1), (1, 2), (3, 0)):
+            say("l4 trace factory", name, locs, t_so, t_si)
+            tr = o.Traces(fac, lut, q["dilT"])
+            key = [t_so if l == "source" else t_si for l in locs]
+            got = dev.get((t_so, t_si), locs)
+            assert len(got) == len(lut["trace_indices"][name])
+            for tid, vec in got.items():
+                want = tr.get(name, key, tid)
+                assert [m for _, m in vec] == [m for _, m in want], (name, tid)
+                scale = max(abs(v) for v, _ in want)
+                assert max(abs(g - w) for (g, _), (w, _) in zip(vec, want)) <= 1e-12 * scale, (name, tid)
+            fac.clear()
+
+
 @pytest.mark.parametrize("case", kahan_trace_cases(), ids=lambda c: c[0])
 def test_device_inner_products_kahan_known_answers(slaph, case):
     """The device resolve kernel on trace vectors built from the reference's

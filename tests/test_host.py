@@ -31,6 +31,31 @@ def test_library_exports_every_declared_symbol(slaph):
     assert declared == set(slaph.exported_symbols())
 
 
+def test_h5_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "slaph_h5.h")).read()
+    declared = set(re.findall(r"\b(slh5_[a-z_0-9]+)\s*\(", header))
+    assert declared == {"slh5_open", "slh5_write", "slh5_close", "slh5_filename_ending", "slh5_last_error"}
+    path = os.path.join(ROOT, "slaph-contractions_amd", "libslaph_h5.so")
+    if not os.path.exists(path):
+        pytest.skip("libslaph_h5.so is not built (no HDF5)")
+    import ctypes
+    lib = ctypes.CDLL(path)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_vdaggerv_file_names_follow_the_reference(slaph):
+    """The format strings of OperatorFactory.cpp: written operators
+    "/%s/cnfg%04d/" + "operators.%04d.p_%s.d_%s.t_%03d" (:224-226, 306-311),
+    read ones ".p_" + digits + "_.t_%03d" (:331-333, 352-356), Liuming's
+    "/%s/VdaggerV." + "p<x>p<y>p<z>.conf" + "%04d" (:399-427)."""
+    f = slaph.vdaggerv_filename
+    assert f("write", "vdv", 1000, (1, 0, -1), "", 3) == "/vdv/cnfg1000/operators.1000.p_10-1.d_000.t_003"
+    assert f("write", "/abs/v", 7, (0, 0, 2), ">x<y", 12) == "//abs/v/cnfg0007/operators.0007.p_002.d_>x<y.t_012"
+    assert f("read", "vdv", 1000, (1, 0, -1), "", 3) == "/vdv/cnfg1000/operators.1000.p_10-1_.t_003"
+    assert f("liuming", "vdv", 7, (0, -1, 2)) == "/vdv/VdaggerV.p0p-1p2.conf0007"
+
+
 def test_dilution_text_matches_control(slaph):
     txt = (slaph.dilution_text(6, 2, False) + slaph.dilution_text(6, 2, True)
            + slaph.dilution_text(48, 4, False) + slaph.dilution_text(48, 4, True)
