@@ -158,3 +158,54 @@ def test_vdaggerv_read_missing_file_raises(slaph, tmp_path):
     ctx.read_inputs(L4, 1000)
     with pytest.raises(slaph.SlaphError, match="failed to open file"):
         slaph.OperatorFactory(ctx).create_operators()
+
+
+@pytest.mark.skipif(not os.path.exists(CONTRACT), reason="HDF5 writer is not built")
+def test_class_layer_program_l4(tmp_path):
+    """The C++ caller of tests/cpp/class_layer_l4.cpp (reference class names
+    over the C ABI): its VdaggerV, Q0 / Q2 factors, trQ0Q2 trace vector and
+    inner products against the oracle, and its correlator files through
+    WriteHDF5Correlator against the golden files with h5diff."""
+    from test_host import _build_class_layer_program
+    import oracle.slaph_oracle as o
+    prog = str(tmp_path / "prog")
+    r = _build_class_layer_program(prog)
+    assert r.returncode == 0, r.stderr
+    out = tmp_path / "out"
+    out.mkdir()
+    say("class layer program")
+    r = subprocess.run([prog, L4, str(out)], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-2000:])
+    _compare(str(out / "correlators"))
+    cfg = o.parse_infile(open(os.path.join(L4, "test_all.ini")).read())
+    lut = o.init_lookup_tables(cfg)
+    perams, rnds, ev = o.read_inputs(cfg, L4, 1000)
+    vdv = o.build_vdaggerv(cfg, lut, ev)
+    got = np.fromfile(str(out / "vdv_1_2.bin"), dtype=np.complex128).reshape(16, 16, order="F")
+    assert np.max(np.abs(got - vdv[(1, 2)])) < 1e-13
+    q = cfg["quarks"][0]
+    fac = o.Factories(cfg, lut, vdv, perams, rnds, q["dilT"], q["dilE"])
+    rics = [ln.split() for ln in open(str(out / "ric.txt"))]
+    D = 4 * q["dilE"]
+    for qt, key, op, fn in (("Q0", (1,), 0, "q0_t1_op0.bin"), ("Q2", (1, 2, 0), 3, "q2_1_2_0_op3.bin")):
+        want = fac.get(qt, key)[op]
+        mats = np.fromfile(str(out / fn), dtype=np.complex128).reshape(len(want), D, D).transpose(0, 2, 1)
+        assert [tuple(map(int, r[1:])) for r in rics if r[0] == qt] == [w[1] for w in want]
+        for gm, (wm, _, _) in zip(mats, want):
+            assert np.max(np.abs(gm - wm)) < 1e-12
+    tr = o.Traces(fac, lut, q["dilT"])
+    locs = None
+    for reqs in lut["requests"].values():
+        for trs, _ in reqs:
+            for name, _, l in trs:
+                if name == "trQ0Q2" and locs is None:
+                    locs = l
+    key = [0 if l == "source" else 1 for l in locs]
+    w0, w1 = tr.get("trQ0Q2", key, 0), tr.get("trQ0Q2", key, 1)
+    t0 = np.fromfile(str(out / "trQ0Q2_0_1_id0.bin"), dtype=np.complex128)
+    assert [int(x) for x in open(str(out / "used.txt"))] == [m for _, m in w0]
+    assert np.max(np.abs(t0 - np.array([v for v, _ in w0]))) <= 1e-12 * max(abs(v) for v, _ in w0)
+    ip = np.fromfile(str(out / "inner_0_1.bin"), dtype=np.complex128)
+    for val, kahan in zip(ip, (False, True)):
+        want = o.inner_product2(w0, w1, kahan)
+        assert abs(val - want) <= 1e-12 * max(abs(want), 1e-300)

@@ -143,3 +143,22 @@ def test_not_enough_random_vectors_is_reported(slaph):
     bad = PIPI.replace("u:5:", "u:1:")
     with pytest.raises(slaph.SlaphError, match="not enough random vectors"):
         slaph.lookup_dump(bad)
+
+
+def _build_class_layer_program(out):
+    lib = os.path.join(ROOT, "slaph-contractions_amd")
+    src = os.path.join(HERE, "cpp", "class_layer_l4.cpp")
+    cmd = ["g++", "-std=c++17", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"), src, "-L" + lib,
+           "-lslaph_amd", "-lslaph_h5", "-Wl,-rpath," + lib, "-o", out]
+    import subprocess
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+def test_class_layer_header_compiles_and_links(tmp_path):
+    """include/slaph_amd.hpp + slaph_h5.hpp: a caller written against the
+    reference's class names compiles without warnings and links against the
+    two libraries (run on the GPU by test_gpu_dropin.py)."""
+    if not os.path.exists(os.path.join(ROOT, "slaph-contractions_amd", "libslaph_h5.so")):
+        pytest.skip("libslaph_h5.so is not built (no HDF5)")
+    r = _build_class_layer_program(str(tmp_path / "prog"))
+    assert r.returncode == 0, r.stderr
