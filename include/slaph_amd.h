@@ -190,7 +190,9 @@ int slc_set_chunk_budget(slc_ctx *ctx, long long bytes);
  *   vdaggerv: 0 = FP64 MFMA (default), 1 = single-pass FP32 MFMA (V and the
  *             phases rounded to FP32, FP32 accumulation);
  *   factors:  0 = FP64 (default), 1 = diluted factors and product tables
- *             stored in FP32 (built, multiplied and traced in FP64).
+ *             stored in FP32, 2 = stored in a 48-bit form (each FP64 part
+ *             rounded to its upper 48 bits: 36 mantissa bits, relative
+ *             rounding error <= 2^-37); built, multiplied and traced in FP64.
  * Correlator accumulation is FP64 (optionally Kahan) in every form. */
 int slc_set_precision(slc_ctx *ctx, int vdaggerv, int factors);
 

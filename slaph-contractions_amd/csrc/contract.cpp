@@ -1188,10 +1188,10 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
     C.need1.assign(need1.begin(), need1.end());
     C.need2.assign(need2.begin(), need2.end());
   }
-  // FP32-storage precision form: the factors are built in FP64 and stored
-  // rounded to FP32 for the products and traces (arithmetic stays FP64)
-  const int f32 = prec_factors_;
-  const size_t elem = f32 ? sizeof(float2) : sizeof(double2);
+  // Storage precision forms (FP32, 48-bit): the factors are built in FP64 and
+  // stored rounded for the products and traces (arithmetic stays FP64)
+  const int f32 = prec_factors_;  // storage form of factor slabs and product tables
+  const size_t elem = storage_bytes(f32);
   {
     auto t0 = std::chrono::steady_clock::now();
     build_factors(C.need1, C.need2);
@@ -1199,8 +1199,8 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
       DevBuf *src[3] = {&q0_, &q1_, &q2_}, *dst[3] = {&q0f_, &q1f_, &q2f_};
       for (int q = 0; q < 3; ++q) {
         long long n = (long long)(src[q]->bytes() / sizeof(double2));
-        dst[q]->ensure(std::max<size_t>(16, sizeof(float2) * n));
-        launch_to_f32(src[q]->as<double2>(), dst[q]->as<void>(), n, stream_);
+        dst[q]->ensure(std::max<size_t>(16, elem * n));
+        launch_to_storage(src[q]->as<double2>(), dst[q]->as<void>(), n, f32, stream_);
       }
       SLAPH_HIP(hipStreamSynchronize(stream_));
     }

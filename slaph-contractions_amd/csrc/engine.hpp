@@ -136,11 +136,11 @@ class Engine {
     plan_.reset();  // chunk tables depend on the budget
   }
   // Precision forms: vdaggerv 0 = FP64, 1 = single-pass FP32 MFMA; factors
-  // 0 = FP64, 1 = factor slabs and product tables stored in FP32 (arithmetic
-  // and every accumulation in FP64).
+  // 0 = FP64, 1 / 2 = factor slabs and product tables stored in FP32 / in the
+  // 48-bit form (arithmetic and every accumulation in FP64).
   void set_precision(int vdaggerv, int factors) {
-    if (vdaggerv < 0 || vdaggerv > 1 || factors < 0 || factors > 1)
-      throw std::out_of_range("precision: vdaggerv and factors must be 0 or 1");
+    if (vdaggerv < 0 || vdaggerv > 1 || factors < 0 || factors > 2)
+      throw std::out_of_range("precision: vdaggerv must be 0 or 1, factors 0, 1 or 2");
     prec_vdv_ = vdaggerv;
     if (factors != prec_factors_) plan_.reset();
     prec_factors_ = factors;

@@ -37,6 +37,25 @@ __device__ __forceinline__ double2 ld2(const float2 *p) {
 }
 __device__ __forceinline__ void st2(double2 *p, double2 v) { *p = v; }
 __device__ __forceinline__ void st2(float2 *p, double2 v) { *p = make_float2(float(v.x), float(v.y)); }
+// 48-bit form: the dropped 16 bits are rounded half up in magnitude (adding
+// 2^15 to the bit pattern carries into the exponent exactly when the kept
+// mantissa overflows); finite values only.
+__device__ __forceinline__ double2 ld2(const d48c *p) {
+  const unsigned a = p->w[0], b = p->w[1], c = p->w[2];
+  const unsigned long long re = ((unsigned long long)a << 32) | ((unsigned long long)(c & 0xffffu) << 16);
+  const unsigned long long im = ((unsigned long long)b << 32) | ((unsigned long long)(c >> 16) << 16);
+  return make_double2(__longlong_as_double((long long)re), __longlong_as_double((long long)im));
+}
+__device__ __forceinline__ d48c to_d48(double2 v) {
+  const unsigned long long re = (unsigned long long)__double_as_longlong(v.x) + 0x8000ull;
+  const unsigned long long im = (unsigned long long)__double_as_longlong(v.y) + 0x8000ull;
+  d48c o;
+  o.w[0] = unsigned(re >> 32);
+  o.w[1] = unsigned(im >> 32);
+  o.w[2] = unsigned((re >> 16) & 0xffffu) | (unsigned((im >> 16) & 0xffffu) << 16);
+  return o;
+}
+__device__ __forceinline__ void st2(d48c *p, double2 v) { *p = to_d48(v); }
 // Streaming (non-temporal) stores for tables that are written once and read
 // by a later launch.
 typedef double d2v __attribute__((ext_vector_type(2)));
@@ -48,6 +67,12 @@ __device__ __forceinline__ void st2_nt(double2 *p, double2 v) {
 __device__ __forceinline__ void st2_nt(float2 *p, double2 v) {
   f2v w = {float(v.x), float(v.y)};
   __builtin_nontemporal_store(w, reinterpret_cast<f2v *>(p));
+}
+__device__ __forceinline__ void st2_nt(d48c *p, double2 v) {
+  const d48c o = to_d48(v);
+  __builtin_nontemporal_store(o.w[0], &p->w[0]);
+  __builtin_nontemporal_store(o.w[1], &p->w[1]);
+  __builtin_nontemporal_store(o.w[2], &p->w[2]);
 }
 __device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b) {
   acc.x = fma(a.x, b.x, acc.x);
@@ -1872,9 +1897,10 @@ __global__ void k_finalize(const double2 *__restrict__ sum, int ncorr, int Lt, d
   out[id] = make_double2(sum[at].x / double(Lt), sum[at].y / double(Lt));
 }
 
-__global__ void k_to_f32(const double2 *__restrict__ in, float2 *__restrict__ out, long long n) {
+template <typename T>
+__global__ void k_to_storage(const double2 *__restrict__ in, T *__restrict__ out, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    out[i] = make_float2(float(in[i].x), float(in[i].y));
+    st2(out + i, in[i]);
 }
 
 __global__ void k_scale(double2 *x, long long n, double f) {
@@ -1951,6 +1977,25 @@ dim3 grid1d(long long n, int block) {
 }
 
 }  // namespace
+
+// Calls f with a value of the element type of storage form `form`
+// (0 = double2, 1 = float2, 2 = d48c).
+template <typename F>
+void with_storage(int form, F &&f) {
+  switch (form) {
+    case 0:
+      f(double2{});
+      break;
+    case 1:
+      f(float2{});
+      break;
+    case 2:
+      f(d48c{});
+      break;
+    default:
+      throw std::runtime_error("unknown storage form " + std::to_string(form));
+  }
+}
 
 // Job table of the allocation-free A^H B entry (slc_ahb_device_noalloc): one
 // fixed array per GPU instead of a per-call allocation.  In host code the
@@ -2205,12 +2250,10 @@ void launch_trace_lstat_q(const void *sX, const long long *Xt, const void *sY, c
                           long long ostride, int ntime, int D, int rt, int f32, int *qcount, hipStream_t s) {
   if (nunit <= 0 || ntime <= 0) return;
   if ((long long)nunit * ntime > 0x7fffffffLL) throw std::runtime_error("launch_trace_lstat_q: too many units");
-  if (f32)
-    launch_trace_lstat_q_t<float2>(sX, Xt, sY, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, D, rt,
-                                   qcount, s);
-  else
-    launch_trace_lstat_q_t<double2>(sX, Xt, sY, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, D, rt,
-                                    qcount, s);
+  with_storage(f32, [&](auto tag) {
+    launch_trace_lstat_q_t<decltype(tag)>(sX, Xt, sY, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime,
+                                          D, rt, qcount, s);
+  });
   SLAPH_HIP(hipGetLastError());
 }
 
@@ -2220,10 +2263,9 @@ void launch_prod(const void *sX, const long long *Xt, const long long *Xq, const
   if (ncombo <= 0 || ntime <= 0) return;
   if (D > kMaxD) throw std::runtime_error("factor dimension above 32 is not supported");
   if (ntime > 65535) throw std::runtime_error("launch_prod: too many time keys");
-  if (f32)
-    launch_prod_t<float2>(sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos, ncombo, ntime, D, transposed, s);
-  else
-    launch_prod_t<double2>(sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos, ncombo, ntime, D, transposed, s);
+  with_storage(f32, [&](auto tag) {
+    launch_prod_t<decltype(tag)>(sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos, ncombo, ntime, D, transposed, s);
+  });
   SLAPH_HIP(hipGetLastError());
 }
 
@@ -2255,12 +2297,10 @@ void launch_trace_lstat(const void *sX, const long long *Xt, const void *sY, con
                         long long ostride, int ntime, int D, int rt, int f32, hipStream_t s) {
   if (nblock <= 0 || ntime <= 0) return;
   if ((long long)nblock * ntime > 0x7fffffffLL) throw std::runtime_error("launch_trace_lstat: grid too large");
-  if (f32)
-    launch_trace_lstat_t<float2>(sX, Xt, sY, Yt, blocks, nblock, chunks, left, tasks, slot_dst, out, obase, ostride,
-                                 ntime, D, rt, s);
-  else
-    launch_trace_lstat_t<double2>(sX, Xt, sY, Yt, blocks, nblock, chunks, left, tasks, slot_dst, out, obase, ostride,
-                                  ntime, D, rt, s);
+  with_storage(f32, [&](auto tag) {
+    launch_trace_lstat_t<decltype(tag)>(sX, Xt, sY, Yt, blocks, nblock, chunks, left, tasks, slot_dst, out, obase,
+                                        ostride, ntime, D, rt, s);
+  });
   SLAPH_HIP(hipGetLastError());
 }
 
@@ -2292,12 +2332,11 @@ void launch_trace_groups(const void *sX, const long long *Xt, const long long *X
                          hipStream_t s) {
   if (nitems <= 0 || ntime <= 0) return;
   if (ntime > 65535) throw std::runtime_error("launch_trace_groups: too many slice pairs");
-  if (f32)
-    hipLaunchKernelGGL(k_trace_groups<float2>, dim3(nitems, ntime), dim3(64), 0, s, static_cast<const float2 *>(sX),
-                       Xt, Xq, static_cast<const float2 *>(sY), Yt, Yq, items, pairs, out, obase, ostride, D, rt);
-  else
-    hipLaunchKernelGGL(k_trace_groups<double2>, dim3(nitems, ntime), dim3(64), 0, s, static_cast<const double2 *>(sX),
-                       Xt, Xq, static_cast<const double2 *>(sY), Yt, Yq, items, pairs, out, obase, ostride, D, rt);
+  with_storage(f32, [&](auto tag) {
+    using T = decltype(tag);
+    hipLaunchKernelGGL(k_trace_groups<T>, dim3(nitems, ntime), dim3(64), 0, s, static_cast<const T *>(sX), Xt, Xq,
+                       static_cast<const T *>(sY), Yt, Yq, items, pairs, out, obase, ostride, D, rt);
+  });
   SLAPH_HIP(hipGetLastError());
 }
 
@@ -2328,12 +2367,11 @@ void launch_trace_gemm(const void *sX, const void *sY, const GridGTile *tiles, i
   static_assert(kGemmT * 16 == 64, "the GEMM tile is 64 x 64");
   if ((long long)ntiles * kparts > 0x7fffffffLL) throw std::runtime_error("launch_trace_gemm: grid too large");
   const dim3 grid(ntiles, kparts);
-  if (f32)
-    hipLaunchKernelGGL(k_trace_gemm<float2>, grid, dim3(256), 0, s, static_cast<const float2 *>(sX),
-                       static_cast<const float2 *>(sY), tiles, runs, nrun, times, out, obase, ostride, D, rt, part);
-  else
-    hipLaunchKernelGGL(k_trace_gemm<double2>, grid, dim3(256), 0, s, static_cast<const double2 *>(sX),
-                       static_cast<const double2 *>(sY), tiles, runs, nrun, times, out, obase, ostride, D, rt, part);
+  with_storage(f32, [&](auto tag) {
+    using T = decltype(tag);
+    hipLaunchKernelGGL(k_trace_gemm<T>, grid, dim3(256), 0, s, static_cast<const T *>(sX), static_cast<const T *>(sY),
+                       tiles, runs, nrun, times, out, obase, ostride, D, rt, part);
+  });
   if (kparts > 1)
     hipLaunchKernelGGL(k_gemm_reduce, grid1d((long long)ntiles * 4096, 256), dim3(256), 0, s, part, kparts, tiles,
                        ntiles, times, out, obase, ostride);
@@ -2346,14 +2384,11 @@ void launch_trace_grid_r(const void *sX, const void *sY, const GridRGroup *group
   if (ngroups <= 0 || ntiles <= 0) return;
   if (D % 4 != 0 || D > kMaxD) throw std::runtime_error("launch_trace_grid_r: unsupported factor dimension");
   if (ntiles > 65535) throw std::runtime_error("launch_trace_grid_r: too many tiles");
-  if (f32)
-    hipLaunchKernelGGL(k_trace_grid_r<float2>, dim3(ngroups, ntiles), dim3(256), 0, s,
-                       static_cast<const float2 *>(sX), static_cast<const float2 *>(sY), groups, runs, tiles, out,
-                       obase, ostride, D, rt);
-  else
-    hipLaunchKernelGGL(k_trace_grid_r<double2>, dim3(ngroups, ntiles), dim3(256), 0, s,
-                       static_cast<const double2 *>(sX), static_cast<const double2 *>(sY), groups, runs, tiles, out,
-                       obase, ostride, D, rt);
+  with_storage(f32, [&](auto tag) {
+    using T = decltype(tag);
+    hipLaunchKernelGGL(k_trace_grid_r<T>, dim3(ngroups, ntiles), dim3(256), 0, s, static_cast<const T *>(sX),
+                       static_cast<const T *>(sY), groups, runs, tiles, out, obase, ostride, D, rt);
+  });
   SLAPH_HIP(hipGetLastError());
 }
 
@@ -2363,18 +2398,21 @@ void launch_trace_grid(const void *sX, const long long *Xq, const void *sY, cons
   if (nitems <= 0 || ntiles <= 0) return;
   if (D % 4 != 0 || D > kMaxD) throw std::runtime_error("launch_trace_grid: unsupported factor dimension");
   if (ntiles > 65535) throw std::runtime_error("launch_trace_grid: too many tiles");
-  if (f32)
-    hipLaunchKernelGGL(k_trace_grid<float2>, dim3(nitems, ntiles), dim3(256), 0, s, static_cast<const float2 *>(sX),
-                       Xq, static_cast<const float2 *>(sY), Yq, items, pairs, tiles, out, obase, ostride, D, rt);
-  else
-    hipLaunchKernelGGL(k_trace_grid<double2>, dim3(nitems, ntiles), dim3(256), 0, s, static_cast<const double2 *>(sX),
-                       Xq, static_cast<const double2 *>(sY), Yq, items, pairs, tiles, out, obase, ostride, D, rt);
+  with_storage(f32, [&](auto tag) {
+    using T = decltype(tag);
+    hipLaunchKernelGGL(k_trace_grid<T>, dim3(nitems, ntiles), dim3(256), 0, s, static_cast<const T *>(sX), Xq,
+                       static_cast<const T *>(sY), Yq, items, pairs, tiles, out, obase, ostride, D, rt);
+  });
   SLAPH_HIP(hipGetLastError());
 }
 
-void launch_to_f32(const double2 *in, void *out, long long n, hipStream_t s) {
+void launch_to_storage(const double2 *in, void *out, long long n, int form, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_to_f32, grid1d(n, 256), dim3(256), 0, s, in, static_cast<float2 *>(out), n);
+  if (form == 0) throw std::runtime_error("launch_to_storage: the FP64 form needs no conversion");
+  with_storage(form, [&](auto tag) {
+    using T = decltype(tag);
+    hipLaunchKernelGGL(k_to_storage<T>, grid1d(n, 256), dim3(256), 0, s, in, static_cast<T *>(out), n);
+  });
   SLAPH_HIP(hipGetLastError());
 }
 

@@ -151,11 +151,24 @@ void launch_q2b(const Q2BJob *d_jobs, int njobs, int N, int dilE, long long pera
 void launch_dagger(const double2 *in, double2 *out, int N, int count, hipStream_t s);
 void launch_identity(double2 *out, int N, int count, hipStream_t s);
 
+// Storage forms of factor slabs and product tables (the arithmetic is FP64 in
+// every form): 0 = double2; 1 = float2 (FP32); 2 = d48c, each FP64 part cut to
+// its upper 48 bits (sign, exponent and 36 of the 52 mantissa bits, rounded to
+// nearest: relative rounding error at most 2^-37): w[0], w[1] = upper 32 bits
+// of the real and imaginary part, w[2] = their next 16 bits (real in the low
+// half).
+struct __attribute__((aligned(4))) d48c {
+  unsigned int w[3];
+};
+inline size_t storage_bytes(int form) {
+  return form == 1 ? sizeof(float2) : (form == 2 ? sizeof(d48c) : sizeof(double2));
+}
+
 // Table-driven batched factor product / trace kernels.
 // Operand matrix index = Xt[ti] + Xq[qn] + i; output index = obase + ti*ostride + o.
 // transposed != 0 stores each product as (X Y)^T.  The operand slabs and the
-// output are double2 arrays, or float2 arrays when f32 != 0 (FP32-storage
-// precision form; the arithmetic is FP64 in both cases).
+// output are arrays of the storage form `f32` (0, 1 or 2 above; the name is
+// kept from the two-form days).
 struct ComboProd {
   int qn, i, j;
   long long o;
@@ -166,7 +179,8 @@ void launch_prod(const void *slabX, const long long *Xt, const long long *Xq, co
 // In the trace launchers below, rt != 0 means the right matrices are stored
 // transposed (Y^T), so that tr(X Y) = sum_e X[e] Y^T[e]; the operand slabs
 // are double2 or (f32 != 0) float2 arrays, the group sums always double2.
-void launch_to_f32(const double2 *in, void *out, long long n, hipStream_t s);
+// FP64 values -> storage form `form` (1 or 2), n complex elements.
+void launch_to_storage(const double2 *in, void *out, long long n, int form, hipStream_t s);
 
 // Correlator resolution: mean / 2-way / 3-way masked inner products.
 // A trace vector seen through its used-index mask groups: the trace values of

@@ -259,12 +259,14 @@ class Context:
 
     def set_precision(self, vdaggerv: str = "fp64", factors: str = "fp64"):
         """Precision forms: vdaggerv "fp64" | "fp32" (single-pass FP32 MFMA);
-        factors "fp64" | "fp32" (factor slabs and product tables stored in
-        FP32, arithmetic and accumulation FP64)."""
-        modes = {"fp64": 0, "fp32": 1}
-        if vdaggerv not in modes or factors not in modes:
-            raise ValueError("precision must be 'fp64' or 'fp32'")
-        self._check(_lib.slc_set_precision(self._h, modes[vdaggerv], modes[factors]))
+        factors "fp64" | "fp32" | "fp48" (factor slabs and product tables
+        stored in FP32 or in 48-bit FP64 -- 36 mantissa bits --, arithmetic and
+        accumulation FP64)."""
+        vmodes = {"fp64": 0, "fp32": 1}
+        fmodes = {"fp64": 0, "fp32": 1, "fp48": 2}
+        if vdaggerv not in vmodes or factors not in fmodes:
+            raise ValueError("unknown precision form %r / %r" % (vdaggerv, factors))
+        self._check(_lib.slc_set_precision(self._h, vmodes[vdaggerv], fmodes[factors]))
 
     def ahb_host(self, A: np.ndarray, B: np.ndarray, phase=None) -> np.ndarray:
         """A^H diag(phase[k/3]) B on the device for host matrices A (K x M), B (K x N)."""

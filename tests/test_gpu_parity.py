@@ -423,6 +423,33 @@ def test_precision_forms_against_fp64(slaph):
     assert np.all(_rel_errors(again, ref) == 0.0)
 
 
+@pytest.mark.parametrize("infile", ["wide", "EI6-D24", "EI8-D32"])
+def test_fp48_storage_form_within_1e_10(slaph, infile):
+    """Factor slabs and product tables in the 48-bit form (36 mantissa bits,
+    relative rounding error <= 2^-37 per stored part; every product, trace and
+    sum in FP64): every correlator within 1e-10 of the FP64 path (norm-wise,
+    per correlator), on D = 16, 24 and 32."""
+    text = {"wide": WIDE, "EI6-D24": dilution_infile(6, 14), "EI8-D32": dilution_infile(8, 20)}[infile]
+    say("fp48 storage", infile)
+    ctx = slaph.Context(0)
+    ctx.configure(text)
+    ctx.synthetic_inputs(314, True)
+    ops = slaph.OperatorFactory(ctx)
+    ops.create_operators()
+
+    def run(fac):
+        ctx.set_precision("fp64", fac)
+        slaph.contract(ctx)
+        return {d: ctx.correlators(d) for d in ctx.diagrams()}
+
+    ref = run("fp64")
+    err = _rel_errors(run("fp48"), ref)
+    say("  max %.2e median %.2e over %d correlators" % (err.max(), np.median(err), err.size))
+    assert err.max() <= 1e-10
+    assert err.max() > 0.0  # the rounded storage did run
+    assert np.all(_rel_errors(run("fp64"), ref) == 0.0)
+
+
 def test_plan_reuse_is_deterministic(slaph):
     """The contraction plan is built on the first call and reused: a second
     call on new operators gives bit-identical correlators, a budget change
