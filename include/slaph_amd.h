@@ -81,6 +81,33 @@ int slc_read_inputs(slc_ctx *ctx, const char *base_dir, int config);
  * `mock` fills of the reference (EigenVector.cpp:73-75, Perambulator.cpp:65-68). */
 int slc_synthetic_inputs(slc_ctx *ctx, unsigned long long seed, int orthonormalise);
 
+/* Timeslice streaming (on != 0): only one timeslice of eigenvectors is kept
+ * on the device; slc_create_operators produces V(t) timeslice by timeslice
+ * from the eigenvector files of slc_read_inputs or from the seeded generator
+ * of slc_synthetic_inputs, as the reference reads them per timeslice while it
+ * builds VdaggerV (src/OperatorFactory.cpp:263-277).  Call after
+ * slc_configure and before the inputs; slc_set_eigenvectors is then refused.
+ * No reference counterpart as a switch (the reference always streams). */
+int slc_set_timeslice_streaming(slc_ctx *ctx, int on);
+
+/* Gauge links of displaced operators (GaugeField, src/GaugeField.cpp): one
+ * timeslice t of Lx Ly Lz sites (site = (x Ly + y) Lz + z) x 3 directions
+ * (x, y, z) x one 3 x 3 SU(3) matrix, column-major (the element order of
+ * Eigen::Matrix3cd; GaugeField::tslices[t][site][dir]).  Reading ILDG/lime
+ * configurations (GaugeField.cpp:781-928) is not part of this path: links
+ * are handed over or generated. */
+int slc_set_gauge_links(slc_ctx *ctx, int t, const double *U);
+int slc_get_gauge_links(slc_ctx *ctx, int t, double *U);
+/* Seeded SU(3) links exp(i eps H), H traceless Hermitian from complex
+ * Gaussian entries, on every timeslice (no reference counterpart: synthetic
+ * gauge configurations in place of the lime files). */
+int slc_synthetic_gauge(slc_ctx *ctx, unsigned long long seed, double eps);
+/* `iterations` steps of GaugeField::smearing_stout (GaugeField.cpp:228-274)
+ * on every timeslice: three-dimensional staples, Omega = (rho / 4) C U^+,
+ * U' = exp(-q) U with q the traceless anti-Hermitian part of Omega^+ - Omega
+ * over 2 (the reference's factor 1/4 on rho kept). */
+int slc_smear_stout(slc_ctx *ctx, double rho, int iterations);
+
 /* Copy one resident input back to the host in the layouts above:
  * kind 0 = eigenvectors of timeslice `index`, 1 = perambulator `index`,
  * 2 = random vector `index`.  No reference counterpart (the reference reads

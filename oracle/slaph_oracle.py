@@ -22,7 +22,12 @@ Pins (each checked in tests/test_oracle.py):
   * everything else  - end to end: the 18 golden HDF5 files of
                        tests/integration-L4 (2,307 datasets) at 1e-12 absolute,
                        the reference's own gate (tests/integration-L4/compare:16)
-Parity unpinned: displaced operators (gauge links) are not restated here.
+  * displacement and stout smearing (gauge links) - closed forms in
+                       tests/test_oracle_gauge.py (unit links -> lattice shift,
+                       forward then backward = identity, gauge invariance of
+                       displaced VdaggerV, gauge covariance / SU(3) / rho = 0 of
+                       the stout step, the diagonal-link phase update); the
+                       reference's tests hold no displaced data
 """
 from __future__ import annotations
 
@@ -523,19 +528,93 @@ def vdaggerv(V, phase):
     return V.conj().T @ (mom[:, None] * V)
 
 
-def build_vdaggerv(cfg, lut, eigvecs):
+def build_vdaggerv(cfg, lut, eigvecs, links=None):
+    """build_vdaggerv (OperatorFactory.cpp:220-326).  Displaced operators are
+    V^+ (D V) without momentum phase (:287-300); links[t][site, mu] are the
+    3 x 3 gauge links of timeslice t (see displace)."""
     Lx, Ly, Lz = cfg["Lx"], cfg["Ly"], cfg["Lz"]
     nev = cfg["number_of_eigen_vec"]
     out = {}
     for vid, mom, disp in lut["vdv_lookup"]:
-        if disp:
-            raise NotImplementedError("displaced operators are not restated in the oracle")
         ph = create_momenta(Lx, Ly, Lz, mom)
         for t, V in enumerate(eigvecs):
             if vid == lut["index_of_unity"]:
                 out[(vid, t)] = np.eye(nev, dtype=complex)
+            elif disp:
+                if links is None:
+                    raise RuntimeError("displaced operators need gauge links")
+                out[(vid, t)] = V.conj().T @ displace(V, links[t], disp, (Lx, Ly, Lz))
             else:
                 out[(vid, t)] = vdaggerv(V, ph)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Gauge links and displacement (src/GaugeField.cpp, OperatorFactory.cpp:33-55).
+# links[site, mu] is the 3 x 3 SU(3) link U_mu(site), site = (x Ly + y) Lz + z,
+# mu = 0, 1, 2 for x, y, z (GaugeField::init, GaugeField.cpp:39-97, cubic case).
+# The reference's tests hold no displaced data: these functions are pinned by
+# closed forms in tests/test_oracle_gauge.py.  Two deliberate deviations from
+# the reference: displace reads the unchanged input of each step (the
+# reference updates in place while reading), and the negative staple of the
+# stout step is U_nu(x - nu)^+ U_mu(x - nu) U_nu(x - nu + mu); the reference's
+# smearing_stout (GaugeField.cpp:255-256) multiplies U_mu(x - nu + mu) there,
+# which is not gauge covariant (its HYP staple, :358-359, has U_nu).
+# ---------------------------------------------------------------------------
+
+
+def neighbour(L, mu, step):
+    """Site index of x + step * mu_hat for every site x (periodic)."""
+    Lx, Ly, Lz = L
+    x, y, z = np.meshgrid(np.arange(Lx), np.arange(Ly), np.arange(Lz), indexing="ij")
+    c = [x, y, z]
+    c[mu] = (c[mu] + step) % L[mu]
+    return ((c[0] * Ly + c[1]) * Lz + c[2]).reshape(-1)
+
+
+def displace(V, links, disp, L):
+    """D V for the one-link displacements of `disp` in order: '>' mu is
+    U_mu(x) V(x + mu) (GaugeField::forward_uv, GaugeField.cpp:543-550), '<' mu
+    is U_mu(x - mu)^+ V(x - mu) (backward_uv, :552-560).  Each step reads the
+    unchanged result of the previous one (the reference's
+    displace_eigenvectors updates in place while reading)."""
+    W = np.array(V, dtype=complex)
+    N = W.shape[1]
+    for d, axis in disp:
+        mu = "xyz".index(axis)
+        Vs = W.reshape(-1, 3, N)
+        if d == ">":
+            nb = neighbour(L, mu, 1)
+            W = np.einsum("sab,sbn->san", links[:, mu], Vs[nb]).reshape(-1, N)
+        elif d == "<":
+            nb = neighbour(L, mu, -1)
+            W = np.einsum("sba,sbn->san", links[nb, mu].conj(), Vs[nb]).reshape(-1, N)
+        else:
+            raise RuntimeError("Something wrong with the displacement in the operator definition!")
+    return W
+
+
+def stout_step(links, rho, L):
+    """One step of GaugeField::smearing_stout (GaugeField.cpp:228-274):
+    staples over the other two spatial directions, Omega = (rho / 4) C U^+,
+    q = (Omega^+ - Omega) / 2 - tr(Omega^+ - Omega) / 6, U' = exp(-q) U."""
+    import scipy.linalg
+    out = np.empty_like(links)
+    adj = lambda m: m.conj().swapaxes(-1, -2)  # noqa: E731
+    for mu in range(3):
+        C = np.zeros((links.shape[0], 3, 3), dtype=complex)
+        xm = neighbour(L, mu, 1)
+        for nu in range(3):
+            if nu == mu:
+                continue
+            xp, xd = neighbour(L, nu, 1), neighbour(L, nu, -1)
+            xdm = xm[xd]  # x - nu + mu
+            C += links[:, nu] @ links[xp, mu] @ adj(links[xm, nu])
+            C += adj(links[xd, nu]) @ links[xd, mu] @ links[xdm, nu]
+        om = (rho / 4.0) * C @ adj(links[:, mu])
+        a = adj(om) - om
+        q = 0.5 * a - (np.trace(a, axis1=1, axis2=2) / 6.0)[:, None, None] * np.eye(3)
+        out[:, mu] = np.stack([scipy.linalg.expm(-qs) for qs in q]) @ links[:, mu]
     return out
 
 

@@ -26,7 +26,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-I" + 
 # Packed FP32 instructions stay off in device code: some of their forms have
 # given wrong results on this hardware without any error.
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["kernels.hip", "host.cpp", "engine.cpp", "contract.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "gauge.hip", "host.cpp", "engine.cpp", "contract.cpp", "capi.cpp"]
 
 
 def compile_one(src):

@@ -155,6 +155,32 @@ int slc_get_input(slc_ctx *ctx, int kind, int index, double *out) {
   });
 }
 
+int slc_set_timeslice_streaming(slc_ctx *ctx, int on) {
+  return guarded(ctx, [&] { ctx->eng->set_streaming(on != 0); });
+}
+
+int slc_set_gauge_links(slc_ctx *ctx, int t, const double *U) {
+  return guarded(ctx, [&] {
+    if (!U) throw std::out_of_range("null gauge links");
+    ctx->eng->set_gauge_links(t, reinterpret_cast<const slaph::cplx *>(U));
+  });
+}
+
+int slc_get_gauge_links(slc_ctx *ctx, int t, double *U) {
+  return guarded(ctx, [&] {
+    if (!U) throw std::out_of_range("null output");
+    ctx->eng->get_gauge_links(t, reinterpret_cast<slaph::cplx *>(U));
+  });
+}
+
+int slc_synthetic_gauge(slc_ctx *ctx, unsigned long long seed, double eps) {
+  return guarded(ctx, [&] { ctx->eng->synthetic_gauge(seed, eps); });
+}
+
+int slc_smear_stout(slc_ctx *ctx, double rho, int iterations) {
+  return guarded(ctx, [&] { ctx->eng->smear_stout(rho, iterations); });
+}
+
 int slc_create_operators(slc_ctx *ctx) {
   return guarded(ctx, [&] { ctx->eng->create_operators(ctx->eng->last_config()); });
 }

@@ -1,6 +1,8 @@
 // Device-resident contraction engine; see engine.hpp.
 #include "engine.hpp"
 
+#include "gauge.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -82,7 +84,8 @@ void Engine::configure(std::string const &text) {
     if (q.dilT != gd.quarks[0].dilT || q.dilE != gd.quarks[0].dilE || q.dilD != 4)
       throw std::runtime_error("all quarks must share the dilution scheme of quark 0 with full Dirac dilution");
   if (gd.Lt % gd.quarks[0].dilT != 0) throw std::runtime_error("Lt must be a multiple of the time dilution block");
-  if (gd.need_gaugefield) throw std::runtime_error("displaced operators are not implemented on this path");
+  if (gd.need_gaugefield && gd.hyp_iterations > 0)
+    throw std::runtime_error("HYP smearing of the gauge links is not implemented on this path (stout smearing is)");
   gd_ = std::move(gd);
   configured_ = true;
   K_ = 3LL * gd_.Lx * gd_.Ly * gd_.Lz;
@@ -92,7 +95,8 @@ void Engine::configure(std::string const &text) {
   nb_ = gd_.n_blocks();
   nrnd_ = gd_.n_rnd_total();
   if (D_ > 32) throw std::runtime_error("4*dilE above 32 is not supported");
-  have_ev_ = have_peram_ = have_rnd_ = have_ops_ = false;
+  have_ev_ = have_peram_ = have_rnd_ = have_ops_ = have_gauge_ = false;
+  ev_source_ = 0;
   results_.clear();
   plan_.reset();
   fjobs_.reset();
@@ -115,13 +119,76 @@ void Engine::require_inputs() const {
 }
 
 void Engine::alloc_inputs() {
-  V_.alloc(sizeof(double2) * K_ * N_ * Lt_);
+  // streaming: one timeslice of eigenvectors resident
+  V_.alloc(sizeof(double2) * K_ * N_ * (streaming_ ? 1 : Lt_));
   rho_.alloc(sizeof(double2) * (size_t)nrnd_ * Lt_ * N_ * 4);
   peram_.alloc(sizeof(double2) * (size_t)nrnd_ * gd_.peram_rows() * gd_.peram_cols());
+  if (gd_.need_gaugefield) U_.alloc(sizeof(double2) * 27 * (size_t)(K_ / 3) * Lt_);
+}
+
+void Engine::set_streaming(bool on) {
+  require_configured();
+  if (on == streaming_) return;
+  streaming_ = on;
+  have_ev_ = have_ops_ = false;
+  ev_source_ = 0;
+  ft_.reset();
+  SLAPH_HIP(hipSetDevice(device_));
+  V_.alloc(sizeof(double2) * K_ * N_ * (streaming_ ? 1 : Lt_));
+}
+
+void Engine::set_gauge_links(int t, const cplx *U) {
+  require_configured();
+  if (!gd_.need_gaugefield) throw std::runtime_error("no displaced operator needs gauge links");
+  if (t < 0 || t >= Lt_) throw std::runtime_error("timeslice out of range");
+  const size_t n = 27 * (size_t)(K_ / 3);
+  SLAPH_HIP(hipSetDevice(device_));
+  SLAPH_HIP(hipMemcpy(U_.as<double2>() + (size_t)t * n, U, sizeof(double2) * n, hipMemcpyHostToDevice));
+  have_gauge_ = true;
+}
+
+void Engine::get_gauge_links(int t, cplx *U) const {
+  require_configured();
+  if (!have_gauge_) throw std::runtime_error("gauge links are not set");
+  if (t < 0 || t >= Lt_) throw std::runtime_error("timeslice out of range");
+  const size_t n = 27 * (size_t)(K_ / 3);
+  SLAPH_HIP(hipSetDevice(device_));
+  SLAPH_HIP(hipMemcpy(U, U_.as<double2>() + (size_t)t * n, sizeof(double2) * n, hipMemcpyDeviceToHost));
+}
+
+void Engine::synthetic_gauge(unsigned long long seed, double eps) {
+  require_configured();
+  if (!gd_.need_gaugefield) throw std::runtime_error("no displaced operator needs gauge links");
+  SLAPH_HIP(hipSetDevice(device_));
+  launch_su3_random(U_.as<double2>(), 3LL * (K_ / 3) * Lt_, seed, eps, stream_);
+  SLAPH_HIP(hipStreamSynchronize(stream_));
+  have_gauge_ = true;
+  have_ops_ = false;
+}
+
+void Engine::smear_stout(double rho, int iterations) {
+  require_configured();
+  if (!have_gauge_) throw std::runtime_error("gauge links are not set");
+  if (iterations < 0) throw std::out_of_range("stout smearing: negative iteration count");
+  SLAPH_HIP(hipSetDevice(device_));
+  const size_t n = 27 * (size_t)(K_ / 3);
+  DevBuf tmp;
+  tmp.alloc(sizeof(double2) * n);
+  for (int t = 0; t < Lt_; ++t) {
+    double2 *u = U_.as<double2>() + (size_t)t * n;
+    for (int i = 0; i < iterations; ++i) {
+      launch_stout_step(u, tmp.as<double2>(), gd_.Lx, gd_.Ly, gd_.Lz, rho, stream_);
+      SLAPH_HIP(hipMemcpyAsync(u, tmp.as<void>(), sizeof(double2) * n, hipMemcpyDeviceToDevice, stream_));
+    }
+  }
+  SLAPH_HIP(hipStreamSynchronize(stream_));
+  have_ops_ = false;
 }
 
 void Engine::set_eigenvectors(int t, const cplx *V) {
   require_configured();
+  if (streaming_)
+    throw std::runtime_error("timeslice streaming: eigenvectors come from the files or the seeded generator");
   if (t < 0 || t >= Lt_) throw std::runtime_error("timeslice out of range");
   SLAPH_HIP(hipSetDevice(device_));
   SLAPH_HIP(hipMemcpy(V_.as<double2>() + (size_t)t * K_ * N_, V, sizeof(double2) * K_ * N_, hipMemcpyHostToDevice));
@@ -172,15 +239,27 @@ void Engine::read_inputs(std::string const &base, int config) {
   if (gd_.handling_vdaggerv == "read" || gd_.handling_vdaggerv == "liuming") return;
   GlobalData g2 = gd_;
   if (!g2.path_eigenvectors.empty() && g2.path_eigenvectors[0] != '/') g2.path_eigenvectors = base + "/" + g2.path_eigenvectors;
-  std::string prefix = eigenvector_prefix(g2, config);
-  std::vector<cplx> V(K_ * N_);
-  for (int t = 0; t < Lt_; ++t) {
-    char suffix[16];
-    std::snprintf(suffix, sizeof suffix, "%03d", t);
-    // file: N vectors of K complex numbers each = column-major K x N
-    read_binary(prefix + suffix, V.data(), sizeof(cplx) * V.size());
-    set_eigenvectors(t, V.data());
+  ev_prefix_ = eigenvector_prefix(g2, config);
+  ev_source_ = 1;
+  have_ev_ = true;
+  if (streaming_) return;  // read timeslice by timeslice in create_operators
+  for (int t = 0; t < Lt_; ++t) load_eigenvectors(t, V_.as<double2>() + (size_t)t * K_ * N_);
+  SLAPH_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::load_eigenvectors(int t, double2 *dst) {
+  if (ev_source_ == 2) {
+    synthetic_ev(t, dst);
+    return;
   }
+  if (ev_source_ != 1) throw std::runtime_error("inputs are not set");
+  // EigenVector::read_eigen_vector (EigenVector.cpp:35-76): the file holds N
+  // vectors of K complex numbers, i.e. the column-major K x N matrix
+  char suffix[16];
+  std::snprintf(suffix, sizeof suffix, "%03d", t);
+  std::vector<cplx> V((size_t)K_ * N_);
+  read_binary(ev_prefix_ + suffix, V.data(), sizeof(cplx) * V.size());
+  SLAPH_HIP(hipMemcpy(dst, V.data(), sizeof(double2) * V.size(), hipMemcpyHostToDevice));
 }
 
 namespace {
@@ -234,49 +313,56 @@ void Engine::synthetic_inputs(unsigned long long seed, bool orthonormalise) {
     launch_fill_z2z2(rho_.as<double2>() + (size_t)r * Lt_ * N_ * 4, (long long)Lt_ * N_ * 4, seed_of(seed, 2, r),
                      stream_);
   }
-  for (int t = 0; t < Lt_; ++t)
-    launch_fill_gauss(V_.as<double2>() + (size_t)t * K_ * N_, K_ * N_, seed_of(seed, 3, t), stream_);
-  if (orthonormalise) {
-    // Two passes of Cholesky-QR: Gram matrices of all timeslices in one
-    // batched launch, Cholesky on the host, V <- V R^{-1} on the device.
-    DevBuf tmp, G, R, dj;
-    tmp.alloc(sizeof(double2) * K_ * N_);
-    G.alloc(sizeof(double2) * (size_t)Lt_ * N_ * N_);
-    R.alloc(sizeof(double2) * N_ * N_);
-    std::vector<VdvJob> jobs(Lt_);
-    for (int t = 0; t < Lt_; ++t) {
-      jobs[t] = VdvJob{};
-      jobs[t].V = V_.as<double2>() + (size_t)t * K_ * N_;
-      jobs[t].np = 1;
-      jobs[t].C[0] = G.as<double2>() + (size_t)t * N_ * N_;
-    }
-    upload(dj, jobs);
-    std::vector<cplx> hG((size_t)Lt_ * N_ * N_);
-    for (int pass = 0; pass < 2; ++pass) {
-      launch_vdv(dj.as<VdvJob>(), Lt_, N_, K_, 0, stream_);
-      SLAPH_HIP(hipMemcpyAsync(hG.data(), G.as<void>(), sizeof(cplx) * hG.size(), hipMemcpyDeviceToHost, stream_));
-      SLAPH_HIP(hipStreamSynchronize(stream_));
-      for (int t = 0; t < Lt_; ++t) {
-        std::vector<cplx> Gt(hG.begin() + (size_t)t * N_ * N_, hG.begin() + (size_t)(t + 1) * N_ * N_);
-        auto Rinv = chol_inverse_upper(Gt, N_);
-        double2 *Vt = V_.as<double2>() + (size_t)t * K_ * N_;
-        SLAPH_HIP(hipMemcpy(R.as<void>(), Rinv.data(), sizeof(cplx) * Rinv.size(), hipMemcpyHostToDevice));
-        launch_gemm_nn_small(Vt, R.as<double2>(), tmp.as<double2>(), K_, N_, stream_);
-        SLAPH_HIP(hipMemcpyAsync(Vt, tmp.as<void>(), sizeof(double2) * K_ * N_, hipMemcpyDeviceToDevice, stream_));
-        SLAPH_HIP(hipStreamSynchronize(stream_));
-      }
-    }
-  }
+  ev_seed_ = seed;
+  ev_ortho_ = orthonormalise;
+  ev_source_ = 2;
+  if (!streaming_)  // streaming: generated timeslice by timeslice in create_operators
+    for (int t = 0; t < Lt_; ++t) synthetic_ev(t, V_.as<double2>() + (size_t)t * K_ * N_);
   SLAPH_HIP(hipStreamSynchronize(stream_));
   have_ev_ = have_peram_ = have_rnd_ = true;
 }
 
-void Engine::get_input(int kind, int index, cplx *out) const {
+// V(t) of the seeded generator: complex Gaussian (stream seed_of(seed, 3, t)),
+// orthonormalised when asked by two passes of Cholesky-QR (the Gram matrix on
+// the device, its Cholesky factor on the host, V <- V R^{-1} on the device).
+void Engine::synthetic_ev(int t, double2 *dst) {
+  launch_fill_gauss(dst, K_ * N_, seed_of(ev_seed_, 3, t), stream_);
+  if (!ev_ortho_) return;
+  vtmp_.ensure(sizeof(double2) * K_ * N_);
+  gram_.ensure(sizeof(double2) * N_ * N_);
+  rinv_.ensure(sizeof(double2) * N_ * N_);
+  VdvJob job{};
+  job.V = dst;
+  job.np = 1;
+  job.C[0] = gram_.as<double2>();
+  upload(gjob_, std::vector<VdvJob>{job});
+  std::vector<cplx> G((size_t)N_ * N_);
+  for (int pass = 0; pass < 2; ++pass) {
+    launch_vdv(gjob_.as<VdvJob>(), 1, N_, K_, 0, stream_);
+    SLAPH_HIP(hipMemcpyAsync(G.data(), gram_.as<void>(), sizeof(cplx) * G.size(), hipMemcpyDeviceToHost, stream_));
+    SLAPH_HIP(hipStreamSynchronize(stream_));
+    auto Rinv = chol_inverse_upper(G, N_);
+    SLAPH_HIP(hipMemcpy(rinv_.as<void>(), Rinv.data(), sizeof(cplx) * Rinv.size(), hipMemcpyHostToDevice));
+    launch_gemm_nn_small(dst, rinv_.as<double2>(), vtmp_.as<double2>(), K_, N_, stream_);
+    SLAPH_HIP(hipMemcpyAsync(dst, vtmp_.as<void>(), sizeof(double2) * K_ * N_, hipMemcpyDeviceToDevice, stream_));
+    SLAPH_HIP(hipStreamSynchronize(stream_));
+  }
+}
+
+void Engine::get_input(int kind, int index, cplx *out) {
   require_inputs();
   if (kind == 0 && !have_ev_) throw std::runtime_error("inputs are not set");
   const double2 *src = nullptr;
   size_t n = 0;
-  if (kind == 0 && index >= 0 && index < Lt_) {
+  DevBuf one;
+  SLAPH_HIP(hipSetDevice(device_));
+  if (kind == 0 && index >= 0 && index < Lt_ && streaming_) {
+    n = (size_t)K_ * N_;
+    one.alloc(sizeof(double2) * n);
+    load_eigenvectors(index, one.as<double2>());
+    SLAPH_HIP(hipStreamSynchronize(stream_));
+    src = one.as<double2>();
+  } else if (kind == 0 && index >= 0 && index < Lt_) {
     n = (size_t)K_ * N_;
     src = V_.as<double2>() + index * n;
   } else if (kind == 1 && index >= 0 && index < nrnd_) {
@@ -356,21 +442,43 @@ void Engine::create_operators(int config) {
     SLAPH_HIP(hipMemcpy(phase_.as<void>(), ph.data(), sizeof(cplx) * ph.size(), hipMemcpyHostToDevice));
     phases_ready_ = true;
   }
-  // Non-trivial momenta in balanced groups of at most kVdvGroup: each
-  // workgroup of the kernel serves a whole group from one V(t) panel load.
-  std::vector<int> ids;
+  // Undisplaced non-trivial momenta by the Fourier (or direct) form;
+  // displaced operators as V^+ (D V) without momentum phase, as the reference
+  // builds them (OperatorFactory.cpp:287-300).
+  std::vector<int> ids, disp_ids;
   for (auto const &op : gd_.vdaggerv_lookup) {
     if (op.id == gd_.index_of_unity)
       launch_identity(vdv_.as<double2>() + (size_t)op.id * Lt_ * N_ * N_, N_, Lt_, stream_);
+    else if (!op.disp.empty())
+      disp_ids.push_back(op.id);
     else
       ids.push_back(op.id);
   }
+  if (!disp_ids.empty() && !have_gauge_) throw std::runtime_error("gauge links are not set (displaced operators)");
   const char *direct = std::getenv("SLAPH_VDV_DIRECT");
-  if (prec_vdv_ == 0 && !ids.empty() && gd_.Lz <= kFtMaxLz && !(direct && *direct && *direct != '0')) {
-    vdaggerv_fourier(ids);
+  const bool fourier =
+      prec_vdv_ == 0 && !ids.empty() && gd_.Lz <= kFtMaxLz && !(direct && *direct && *direct != '0');
+  EventTimer tm(stream_);
+  int launches = 0;
+  auto plain = [&](int t0, int nt, const double2 *V) {
+    if (ids.empty()) return;
+    launches += fourier ? vdaggerv_fourier(ids, t0, nt, V) : vdaggerv_direct(ids, t0, nt, V);
+  };
+  if (!streaming_) {
+    plain(0, Lt_, V_.as<double2>());
+    if (!disp_ids.empty())
+      for (int t = 0; t < Lt_; ++t) launches += vdaggerv_displaced(disp_ids, t, V_.as<double2>() + (size_t)t * K_ * N_);
   } else {
-    vdaggerv_direct(ids);
+    // one timeslice of eigenvectors resident at a time
+    for (int t = 0; t < Lt_; ++t) {
+      load_eigenvectors(t, V_.as<double2>());
+      plain(t, 1, V_.as<double2>());
+      if (!disp_ids.empty()) launches += vdaggerv_displaced(disp_ids, t, V_.as<double2>());
+    }
   }
+  times_.vdaggerv_kernel_ms = tm.stop();
+  times_.vdaggerv_launches = launches;
+  times_.vdaggerv_ms = times_.vdaggerv_kernel_ms;
   if (mode == "write") write_vdaggerv_files(config);
   }
   // Daggered copies of the operators that some factor uses with -p
@@ -391,16 +499,16 @@ void Engine::create_operators(int config) {
 
 // Direct form: one V(t) panel pass per group of up to kVdvGroup momenta
 // (FP64 or the single-pass FP32 precision form).
-void Engine::vdaggerv_direct(std::vector<int> const &ids) {
+int Engine::vdaggerv_direct(std::vector<int> const &ids, int t0, int nt, const double2 *V) {
   int Vs = gd_.Lx * gd_.Ly * gd_.Lz;
   int ngroups = int((ids.size() + kVdvGroup - 1) / kVdvGroup);
   std::vector<VdvJob> jobs;
-  for (int t = 0; t < Lt_; ++t) {
+  for (int t = t0; t < t0 + nt; ++t) {
     size_t at = 0;
     for (int g = 0; g < ngroups; ++g) {
       size_t len = ids.size() / ngroups + (size_t(g) < ids.size() % ngroups ? 1 : 0);
       VdvJob j{};
-      j.V = V_.as<double2>() + (size_t)t * K_ * N_;
+      j.V = V + (size_t)(t - t0) * K_ * N_;
       j.np = int(len);
       for (size_t k = 0; k < len; ++k, ++at) {
         int id = ids[at];
@@ -412,12 +520,47 @@ void Engine::vdaggerv_direct(std::vector<int> const &ids) {
   }
   DevBuf dj;
   upload(dj, jobs);
-  EventTimer tm(stream_);
   for (size_t i = 0; i < jobs.size(); i += 65535)
     launch_vdv(dj.as<VdvJob>() + i, int(std::min<size_t>(65535, jobs.size() - i)), N_, K_, prec_vdv_, stream_);
-  times_.vdaggerv_kernel_ms = tm.stop();
-  times_.vdaggerv_launches = int((jobs.size() + 65534) / 65535);
-  times_.vdaggerv_ms = times_.vdaggerv_kernel_ms;
+  SLAPH_HIP(hipStreamSynchronize(stream_));  // before the job table is freed
+  return int((jobs.size() + 65534) / 65535);
+}
+
+// Displaced operators at timeslice t: W = D V by the one-link displacements
+// of the operator in order (each '>' forward / '<' backward along x, y or
+// z), then VdaggerV = V^+ W on FP64 MFMA (OperatorFactory.cpp:287-300).  The
+// displacement is the covariant one, U_mu(x) V(x + mu) / U_mu(x - mu)^+ V(x -
+// mu), computed from the unchanged input of each step (the reference updates
+// its result in place while reading it, displace_eigenvectors,
+// OperatorFactory.cpp:33-55, so that some sites read already-displaced
+// neighbours).
+int Engine::vdaggerv_displaced(std::vector<int> const &ids, int t, const double2 *V) {
+  const size_t kn = (size_t)K_ * N_;
+  vdisp_.ensure(sizeof(double2) * kn);
+  vtmp_.ensure(sizeof(double2) * kn);
+  const double2 *U = U_.as<double2>() + (size_t)t * 27 * (K_ / 3);
+  int launches = 0;
+  DevBuf dj;
+  for (int id : ids) {
+    auto const &disp = gd_.vdaggerv_lookup.at(id).disp;
+    const double2 *cur = V;
+    for (size_t k = 0; k < disp.size(); ++k) {
+      const char axis = disp[k].second;
+      const int mu = axis == 'x' ? 0 : (axis == 'y' ? 1 : (axis == 'z' ? 2 : -1));
+      if (mu < 0 || (disp[k].first != '>' && disp[k].first != '<'))
+        throw std::runtime_error("Something wrong with the displacement in the operator definition!");
+      double2 *out = (k % 2 == 0) ? vdisp_.as<double2>() : vtmp_.as<double2>();
+      launch_displace(U, cur, out, gd_.Lx, gd_.Ly, gd_.Lz, N_, mu, disp[k].first == '>' ? 1 : 0, stream_);
+      cur = out;
+      ++launches;
+    }
+    AhBJob j{V, cur, vdv_.as<double2>() + ((size_t)id * Lt_ + t) * N_ * N_, nullptr, nullptr, K_, K_, N_};
+    upload(dj, std::vector<AhBJob>{j});
+    launch_ahb(dj.as<AhBJob>(), 1, N_, N_, int(K_), stream_);
+    ++launches;
+    SLAPH_HIP(hipStreamSynchronize(stream_));  // the job table and work space are reused
+  }
+  return launches;
 }
 
 // Sitewise Fourier form (FP64): W_site = V_site^H V_site per 16 x 16 tile
@@ -440,7 +583,7 @@ struct Engine::FtPlan {
   int max_nc = 0;
 };
 
-void Engine::vdaggerv_fourier(std::vector<int> const &ids) {
+int Engine::vdaggerv_fourier(std::vector<int> const &ids, int t_first, int nt_all, const double2 *V) {
   const int Lx = gd_.Lx, Ly = gd_.Ly, Lz = gd_.Lz;
   if (!ft_) {
     auto F = std::make_shared<FtPlan>();
@@ -542,24 +685,21 @@ void Engine::vdaggerv_fourier(std::vector<int> const &ids) {
   }
   vr_.ensure(sizeof(double2) * (size_t)F.TC * K_ * F.Npad);
   ftu_.ensure(sizeof(double2) * (size_t)F.TC * Lx * F.max_nc * F.ntp * 256);
-  EventTimer tm(stream_);
   int launches = 0;
-  for (int t0 = 0; t0 < Lt_; t0 += F.TC) {
-    int nt = std::min(F.TC, Lt_ - t0);
-    launch_vrow(V_.as<double2>() + (size_t)t0 * K_ * N_, vr_.as<double2>(), K_, N_, F.Npad, nt, stream_);
+  for (int c0 = 0; c0 < nt_all; c0 += F.TC) {
+    const int nt = std::min(F.TC, nt_all - c0);
+    launch_vrow(V + (size_t)c0 * K_ * N_, vr_.as<double2>(), K_, N_, F.Npad, nt, stream_);
     ++launches;
     for (auto &P : F.passes) {
       launch_vdv_ft(vr_.as<double2>(), K_ * F.Npad, FtLayout{F.Npad, 1, F.Npad}, Lx, Ly, Lz, F.tiles.as<int2>(), F.ntp,
                     nt, P.p,
                     P.ey.as<double2>(), P.zcs.as<double2>(), ftu_.as<double2>(), stream_);
       launch_vdv_xsum(ftu_.as<double2>(), P.p.nc, F.ntp, nt, Lx, F.tiles.as<int2>(), P.tg.as<FtTarget>(),
-                      int(P.targets.size()), P.ex.as<double2>(), N_, t0, (long long)N_ * N_, stream_);
+                      int(P.targets.size()), P.ex.as<double2>(), N_, t_first + c0, (long long)N_ * N_, stream_);
       launches += 2;
     }
   }
-  times_.vdaggerv_kernel_ms = tm.stop();
-  times_.vdaggerv_launches = launches;
-  times_.vdaggerv_ms = times_.vdaggerv_kernel_ms;
+  return launches;
 }
 
 // write_vdaggerv (OperatorFactory.cpp:144-168, 303-313): every non-trivial

@@ -102,12 +102,29 @@ class Engine {
   void set_random_vector(int r, const cplx *rho);  // Lt * N * 4, entry 4(N t + v) + d
   void read_inputs(std::string const &base_dir, int config);
   void synthetic_inputs(unsigned long long seed, bool orthonormalise);
-  void get_input(int kind, int index, cplx *out) const;
+  void get_input(int kind, int index, cplx *out);  // streaming: V(t) is produced anew
 
   // OperatorFactory::create_operators / return_vdaggerv; config names the
   // VdaggerV files of handling_vdaggerv = write / read / liuming.
   void create_operators(int config);
   int last_config() const { return config_; }
+
+  // Timeslice streaming: only one timeslice of eigenvectors is resident;
+  // create_operators produces V(t) timeslice by timeslice from the eigenvector
+  // files (read_inputs) or the seeded generator (synthetic_inputs), as the
+  // reference reads them per timeslice (OperatorFactory.cpp:263-277).  Set
+  // before the inputs; set_eigenvectors is not available in this mode.
+  void set_streaming(bool on);
+  bool streaming() const { return streaming_; }
+
+  // Gauge links for displaced operators (GaugeField, src/GaugeField.cpp):
+  // one timeslice of Lx Ly Lz sites x 3 directions x 3 x 3 (column-major).
+  void set_gauge_links(int t, const cplx *U);
+  void get_gauge_links(int t, cplx *U) const;
+  // seeded SU(3) links exp(i eps H), H traceless Hermitian (Gaussian)
+  void synthetic_gauge(unsigned long long seed, double eps);
+  // `iterations` stout steps of every timeslice (smearing_stout, GaugeField.cpp:228-274)
+  void smear_stout(double rho, int iterations);
   void return_vdaggerv(int id, int t, cplx *out);
 
   // DilutedFactorFactory<Q>::operator[](key) for one operator index: copies
@@ -168,8 +185,16 @@ class Engine {
   const double2 *vdv_ptr(int id, bool dagger, int t) const;
   void compute_q_offsets();
   void build_factors(std::vector<std::vector<int>> const &q1keys, std::vector<std::vector<int>> const &q2keys);
-  void vdaggerv_fourier(std::vector<int> const &ids);
-  void vdaggerv_direct(std::vector<int> const &ids);
+  // VdaggerV of timeslices t0 .. t0 + nt - 1 whose eigenvectors start at V
+  // (consecutive timeslices), for the given undisplaced / displaced ids
+  // (each returns its number of kernel launches)
+  int vdaggerv_fourier(std::vector<int> const &ids, int t0, int nt, const double2 *V);
+  int vdaggerv_direct(std::vector<int> const &ids, int t0, int nt, const double2 *V);
+  int vdaggerv_displaced(std::vector<int> const &ids, int t, const double2 *V);
+  // V(t) of the configured source into dst (K x N): the seeded generator or
+  // the eigenvector file
+  void load_eigenvectors(int t, double2 *dst);
+  void synthetic_ev(int t, double2 *dst);
   void write_vdaggerv_files(int config);
   void read_vdaggerv_files(int config, bool liuming);
   struct ContractScope;  // contract.cpp
@@ -184,6 +209,16 @@ class Engine {
   bool configured_ = false;
   bool have_ev_ = false, have_peram_ = false, have_rnd_ = false, have_ops_ = false;
   int config_ = 0;  // gauge configuration of the last read_inputs
+  // eigenvector source: 0 = set by the caller (resident), 1 = files with
+  // prefix ev_prefix_, 2 = seeded generator (ev_seed_, ev_ortho_)
+  bool streaming_ = false;
+  int ev_source_ = 0;
+  std::string ev_prefix_;
+  unsigned long long ev_seed_ = 0;
+  bool ev_ortho_ = false;
+  DevBuf U_;  // gauge links, Lt x sites x 3 x 9 (allocated when displaced operators exist)
+  bool have_gauge_ = false;
+  DevBuf vdisp_, vtmp_, gram_, rinv_, gjob_;  // displacement and orthonormalisation work space
   long long K_ = 0;
   int N_ = 0, D_ = 0, Lt_ = 0, nb_ = 0, nrnd_ = 0;
 

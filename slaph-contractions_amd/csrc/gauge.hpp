@@ -1,0 +1,28 @@
+// Gauge links of one timeslice and the covariant displacement of eigenvectors
+// (reference: src/GaugeField.cpp, src/OperatorFactory.cpp:33-55).
+//
+// Layout: U[(site * 3 + mu) * 9 + c * 3 + r] = element (r, c) of the 3 x 3
+// link U_mu(site) (column-major, the element order of Eigen::Matrix3cd),
+// site = (x Ly + y) Lz + z, mu = 0, 1, 2 for x, y, z.  Eigenvector matrices
+// are K x N column-major with row 3 site + colour.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace slaph {
+
+// Seeded SU(3) links U = exp(i eps H), H a traceless Hermitian matrix from
+// complex Gaussian entries (stream `seed`, link index as counter).
+void launch_su3_random(double2 *U, long long nlinks, unsigned long long seed, double eps, hipStream_t s);
+// One stout step of smearing_stout (GaugeField.cpp:228-274) on a timeslice:
+// staple C = sum_{nu != mu} U_nu(x) U_mu(x+nu) U_nu(x+mu)^+ +
+// U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu), Omega = (rho / 4) C U_mu(x)^+,
+// q = (Omega^+ - Omega) / 2 - tr(Omega^+ - Omega) / 6, U' = exp(-q) U.
+void launch_stout_step(const double2 *Uin, double2 *Uout, int Lx, int Ly, int Lz, double rho, hipStream_t s);
+// One one-link displacement of the N columns of V (K = 3 Lx Ly Lz rows):
+// forward (the reference's '>'): out(x) = U_mu(x) V(x + mu);
+// backward ('<'): out(x) = U_mu(x - mu)^+ V(x - mu).
+void launch_displace(const double2 *U, const double2 *Vin, double2 *Vout, int Lx, int Ly, int Lz, int N, int mu,
+                     int forward, hipStream_t s);
+
+}  // namespace slaph

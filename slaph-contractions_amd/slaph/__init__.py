@@ -49,6 +49,11 @@ _SIGS = {
     "slc_get_input": (_c.c_int, [_P, _c.c_int, _c.c_int, _D]),
     "slc_create_operators": (_c.c_int, [_P]),
     "slc_create_operators_config": (_c.c_int, [_P, _c.c_int]),
+    "slc_set_timeslice_streaming": (_c.c_int, [_P, _c.c_int]),
+    "slc_set_gauge_links": (_c.c_int, [_P, _c.c_int, _D]),
+    "slc_get_gauge_links": (_c.c_int, [_P, _c.c_int, _D]),
+    "slc_synthetic_gauge": (_c.c_int, [_P, _c.c_ulonglong, _c.c_double]),
+    "slc_smear_stout": (_c.c_int, [_P, _c.c_double, _c.c_int]),
     "slc_vdaggerv_filename": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_int, _I, _c.c_char_p, _c.c_int, _c.c_char_p,
                                          _c.c_int]),
     "slc_query_string": (_c.c_int, [_P, _c.c_char_p, _c.c_char_p, _c.c_int]),
@@ -199,6 +204,31 @@ class Context:
 
     def synthetic_inputs(self, seed: int, orthonormalise: bool = True):
         self._check(_lib.slc_synthetic_inputs(self._h, seed, int(orthonormalise)))
+
+    def set_timeslice_streaming(self, on: bool = True):
+        """One timeslice of eigenvectors resident; V(t) produced per timeslice."""
+        self._check(_lib.slc_set_timeslice_streaming(self._h, int(on)))
+
+    def _gauge_shape(self):
+        return (self.query("Lx") * self.query("Ly") * self.query("Lz"), 3, 3, 3)
+
+    def set_gauge_links(self, t: int, U: np.ndarray):
+        """U[site, mu] = 3 x 3 link matrix (numpy, row/column as usual)."""
+        U = np.asarray(U, dtype=np.complex128)
+        self._expect_shape("gauge links (sites, 3, 3, 3)", U, self._gauge_shape())
+        flat = np.ascontiguousarray(U.transpose(0, 1, 3, 2))  # column-major 3 x 3 per link
+        self._check(_lib.slc_set_gauge_links(self._h, int(t), _dptr(flat.reshape(-1))))
+
+    def get_gauge_links(self, t: int) -> np.ndarray:
+        out = np.zeros(self._gauge_shape(), dtype=np.complex128)
+        self._check(_lib.slc_get_gauge_links(self._h, int(t), _dptr(out.reshape(-1))))
+        return out.transpose(0, 1, 3, 2).copy()  # back to [site, mu, row, col]
+
+    def synthetic_gauge(self, seed: int, eps: float):
+        self._check(_lib.slc_synthetic_gauge(self._h, int(seed), float(eps)))
+
+    def smear_stout(self, rho: float, iterations: int):
+        self._check(_lib.slc_smear_stout(self._h, float(rho), int(iterations)))
 
     def get_input(self, kind: str, index: int) -> np.ndarray:
         """Resident input back on the host: 'eigenvectors' (K x N), 'perambulator'
