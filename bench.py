@@ -5,16 +5,20 @@ One step = VdaggerV for every momentum and timeslice + all pipi I=2 diagrams
 (C2c, C4cD, C4cB, C4cC over |P|^2 <= 4 and C6cD at P = 0) over all dilution
 block pairs, on synthetic inputs already resident in HBM:
 L = 32, T = 64, N_ev = 220, 27 momenta |p|^2 <= 3, 5 random vectors,
-dilution TB16 / EI4 / DF4.
+dilution TB16 / EI4 / DF4.  Other workloads (--workload): c2 (24^3 x 48,
+N_ev = 120, D = 32), disp48 (48^3 x 96, N_ev = 400, displaced operators, one
+eigenvector timeslice resident) and small (4^3 x 8).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pipi32|pipi24|small]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload pipi32|c2|disp48|small]
 
-Prints one JSON line.  The roofline object is the largest kernel of the step,
-the queued left-stationary trace kernel k_trace_lstat_q (C4cC traces),
-timed with HIP events around each of its launches on the stream it runs on;
-its algorithmic bytes per launch are the distinct product matrices the launch
-reads plus the partial sums it writes.  cpu_baseline is the numpy oracle timed
-on a bounded sample on this host and extrapolated.
+Prints one JSON line.  `roofline` is the largest kernel of the step, the
+queued left-stationary trace kernel k_trace_lstat_q (C4cC traces), timed with
+HIP events around each of its launches on the stream it runs on; its
+algorithmic bytes per launch are the distinct product matrices the launch
+reads plus the partial sums it writes.  `roofline_vdaggerv` is the VdaggerV
+kernel k_vdv_ft on its own operation count against the FP64 peak (the
+roofline object itself for workloads without C4cC).  cpu_baseline is the
+numpy oracle timed on a bounded sample on this host and extrapolated.
 """
 import argparse
 import json
@@ -34,6 +38,12 @@ WORKLOADS = {
     # plain FP64 at production size: 24^3 x 48, N_ev = 120, 33 momenta
     # |p|^2 <= 4, TB4 / EI8 / DF4 (D = 32), C2c C3c C4cD C4cB
     "c2": dict(L=24, T=48, N=120, nrnd=5, dilT=4, dilE=8, pmax="0,1,2,3,4", diagrams=("C2c", "C3c", "C4cD", "C4cB")),
+    # large volume with displaced operators: 48^3 x 96, N_ev = 400, 33 momenta
+    # |p|^2 <= 4 undisplaced and one-link displaced in x, y, z, TB24 / EI4 /
+    # DF4, C2c (both operator lists) and C4cD; one eigenvector timeslice
+    # resident, seeded SU(3) links smeared by 3 stout steps at rho = 0.1
+    "disp48": dict(L=48, T=96, N=400, nrnd=4, dilT=24, dilE=4, pmax="0,1,2,3,4", diagrams=("C2c", "C4cD"),
+                   displaced=">x,>y,>z", streaming=True),
     # tiny CPU-checkable case
     "small": dict(L=4, T=8, N=8, nrnd=4, dilT=2, dilE=4, pmax="0,1", diagrams=PIPI),
 }
@@ -55,6 +65,10 @@ def infile(w, pmax=None, diagrams=None, total_momenta=None):
     """Input file of a workload; pmax / diagrams / total_momenta override the
     operator momenta, the diagram list and the total momenta of every list."""
     lines = ["correlator_list = " + _CORR[d] for d in (diagrams or w["diagrams"])]
+    ops = "operator_list = g5.d0.p" + (pmax or w["pmax"])
+    if w.get("displaced"):  # second operator list: displaced operators, two-point functions
+        ops += "\noperator_list = g5.d%s.p%s" % (w["displaced"], pmax or w["pmax"])
+        lines.append("correlator_list = C2c:Q0:Op1:Q0:Op1:P0,1,2,3,4")
     if total_momenta is not None:
         lines = [ln.rsplit(":P", 1)[0] + ":P" + total_momenta for ln in lines]
     return """
@@ -66,10 +80,10 @@ number_of_eigen_vec = {N}
 [quarks]
 quark = u:{nrnd}:TB:{dilT}:EI:{dilE}:DF:4:synthetic
 [operator_lists]
-operator_list = g5.d0.p{pm}
+{ops}
 [correlator_lists]
 {corr}
-""".format(pm=pmax or w["pmax"], corr="\n".join(lines), **w)
+""".format(ops=ops, corr="\n".join(lines), **w)
 
 
 # the kernel of the roofline object (the block form runs under SLAPH_LSTAT_BLOCKS=1)
@@ -161,6 +175,23 @@ def cpu_baseline(w, text, max_seconds=30.0):
             "note": "VdaggerV leg uses BLAS with the listed threads; the contraction leg is single-threaded Python"}
 
 
+def _describe(name, w):
+    corr = {"pipi32": "C2c C4cD C4cB C4cC (P^2<=4) + C6cD (P=0)", "small": "C2c C4cD C4cB C4cC (P^2<=4) + C6cD (P=0)",
+            "c2": "C2c C3c C4cD C4cB (P^2<=4)",
+            "disp48": "C2c (undisplaced and displaced >x,>y,>z) + C4cD (P^2<=4), one eigenvector timeslice "
+                      "resident, SU(3) links + 3 stout steps rho=0.1 per config"}[name]
+    return "%s: L=%d T=%d Nev=%d, %d rnd vecs, TB%d/EI%d/DF4, g5 p^2<=%s, %s, plain FP64 path" % (
+        name, w["L"], w["T"], w["N"], w["nrnd"], w["dilT"], w["dilE"], w["pmax"][-1], corr)
+
+
+def _kstat(ctx, kernel):
+    """(ms, launches, bytes, flops) of `kernel` in the last step, or None."""
+    try:
+        return tuple(ctx.kernel_stat(kernel, k) for k in ("ms", "launches", "bytes", "flops"))
+    except Exception:  # the kernel did not run in this workload
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,7 +200,7 @@ def main():
     ap.add_argument("--workload", default="pipi32", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-precision-forms", action="store_true",
-                    help="skip timing the FP32 precision forms against the FP64 path")
+                    help="skip timing the precision forms against the FP64 path")
     ap.add_argument("--chunk-gib", type=float, default=0.0, help="chunk memory budget (0: automatic)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -180,36 +211,49 @@ def main():
     w = WORKLOADS[args.workload]
     text = infile(w)
     ctx = slaph.Context(0)
+    if w.get("streaming"):
+        ctx.set_timeslice_streaming(True)  # before configure: V of one timeslice is allocated
     ctx.configure(text)
     ctx.set_chunk_budget(int(args.chunk_gib * (1 << 30)))
     t0 = time.perf_counter()
-    ctx.synthetic_inputs(20261203, True)
+    # streamed eigenvectors are generated per timeslice inside each step (in
+    # place of the reference's per-timeslice file read), without the
+    # orthonormalisation pass
+    ctx.synthetic_inputs(20261203, not w.get("streaming"))
     print("# inputs generated in %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
     ops = slaph.OperatorFactory(ctx)
-    for i in range(args.warmup):
+
+    def step():
+        if w.get("displaced"):
+            ctx.synthetic_gauge(20261204, 0.2)
+            ctx.smear_stout(0.1, 3)
         ops.create_operators()
         slaph.contract(ctx)
+
+    for i in range(args.warmup):
+        step()
         print("# warmup %d done" % i, file=sys.stderr, flush=True)
     stage = {"vdaggerv": 0.0, "factors": 0.0, "contract": 0.0}
-    ls = {"ms": 0.0, "launches": 0.0, "bytes": 0.0}
+    stats = {LSTAT_KERNEL: [0.0] * 4, "k_vdv_ft": [0.0] * 4}
     t0 = time.perf_counter()
     for i in range(args.steps):
-        ops.create_operators()
-        slaph.contract(ctx)
+        step()
         for k in stage:
             stage[k] += ctx.stage_time(k) / args.steps
-        for k in ls:
-            ls[k] += ctx.kernel_stat(LSTAT_KERNEL, k)
+        for kern in stats:
+            st = _kstat(ctx, kern)
+            if st is None:
+                stats[kern] = None
+            elif stats[kern] is not None:
+                stats[kern] = [a + b for a, b in zip(stats[kern], st)]
         print("# step %d done" % i, file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     ms = elapsed / args.steps * 1000.0
     L, T, N = w["L"], w["T"], w["N"]
-    launches = max(ls["launches"], 1.0)
-    kern_ms = ls["ms"] / launches
-    bytes_per_launch = ls["bytes"] / launches
-    achieved = bytes_per_launch / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
+    headline = args.workload == "pipi32"
     line = {
-        "metric": "s per gauge config (VdV+all diagrams) L=32 T=64 Nev=220: mixed vs FP64 @1e-10",
+        "metric": ("s per gauge config (VdV+all diagrams) L=32 T=64 Nev=220: mixed vs FP64 @1e-10" if headline else
+                   "s per gauge config (VdV+all diagrams) L=%d T=%d Nev=%d" % (L, T, N)),
         "value": ms / 1000.0,
         "unit": "s",
         "n_gpus": 1,
@@ -220,22 +264,42 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded, device-generated; orthonormalised eigenvectors)",
-        "config": {"workload": "%s: L=%d T=%d Nev=%d, %d rnd vecs, TB%d/EI%d/DF4, g5 p^2<=%s, "
-                               "C2c C4cD C4cB C4cC (P^2<=4) + C6cD (P=0), plain FP64 path"
-                               % (args.workload, L, T, N, w["nrnd"], w["dilT"], w["dilE"], w["pmax"][-1]),
-                   "precision": "fp64"},
+        "data": "synthetic (seeded, device-generated; %s)" % (
+            "eigenvectors generated per timeslice, not orthonormalised" if w.get("streaming")
+            else "orthonormalised eigenvectors"),
+        "config": {"workload": _describe(args.workload, w), "precision": "fp64"},
         "stages_ms": {"vdaggerv": stage["vdaggerv"], "factors": stage["factors"],
                       "contract_incl_factors": stage["contract"]},
-        "roofline": {"bound": "hbm", "kernel": LSTAT_KERNEL + " (C4cC traces)", "achieved": achieved,
-                     "peak": PEAK_HBM, "unit": "GB/s", "frac": achieved / PEAK_HBM,
-                     "traffic": kernel_traffic(LSTAT_KERNEL, args.workload),
-                     "traffic_unit": "bytes per launch (HBM, rocprofv3 --pmc)",
-                     "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": kern_ms,
-                     "launches_per_step": ls["launches"] / args.steps},
     }
+    ls = stats[LSTAT_KERNEL]
+    if ls is not None:
+        launches = max(ls[1], 1.0)
+        kern_ms, bytes_per_launch = ls[0] / launches, ls[2] / launches
+        achieved = bytes_per_launch / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
+        line["roofline"] = {"bound": "hbm", "kernel": LSTAT_KERNEL + " (non-separable traces: C4cC, C4cD)", "achieved": achieved,
+                            "peak": PEAK_HBM, "unit": "GB/s", "frac": achieved / PEAK_HBM,
+                            "traffic": kernel_traffic(LSTAT_KERNEL, args.workload),
+                            "traffic_unit": "bytes per launch (HBM, rocprofv3 --pmc)",
+                            "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": kern_ms,
+                            "launches_per_step": ls[1] / args.steps}
+    vf = stats["k_vdv_ft"]
+    if vf is not None:
+        launches = max(vf[1], 1.0)
+        kern_ms, flops_per_launch = vf[0] / launches, vf[3] / launches
+        tf = flops_per_launch / (kern_ms / 1e3) / 1e12 if kern_ms > 0 else 0.0
+        n_vdv = ctx.query("n_vdaggerv") - (1 if ctx.query("index_of_unity") >= 0 else 0)
+        obj = {"bound": "fp64", "kernel": "k_vdv_ft (VdaggerV, sitewise Fourier form)", "achieved": tf,
+               "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": tf / PEAK_FP64,
+               "flops_per_launch": flops_per_launch, "avg_launch_ms": kern_ms,
+               "launches_per_step": vf[1] / args.steps,
+               "direct_product_equivalent_tflops": 8.0 * N * N * 3 * L ** 3 * n_vdv * T / (vf[0] / args.steps / 1e3)
+               / 1e12 if vf[0] > 0 else None}
+        if "roofline" in line:
+            line["roofline_vdaggerv"] = obj
+        else:
+            line["roofline"] = obj
     if not args.no_precision_forms:
-        pf = precision_forms(ctx, ops, ms / 1000.0)
+        pf = precision_forms(ctx, ops, ms / 1000.0, step)
         line["precision_forms"] = pf
         ok = [f for f in pf if f["within_1e-10"]]
         # `value` is the plain FP64 path; a mixed form would replace it only if
@@ -247,7 +311,7 @@ def main():
     return 0
 
 
-def precision_forms(ctx, ops, fp64_s):
+def precision_forms(ctx, ops, fp64_s, step):
     """Time each precision form (one warm-up and one timed configuration on the
     inputs of the FP64 run) and state its per-correlator relative error against
     the FP64 path: err = max_t |C(t) - C_fp64(t)| / max_t |C_fp64(t)|."""
@@ -259,13 +323,11 @@ def precision_forms(ctx, ops, fp64_s):
 
     ref = correlators()
     out = []
-    for vdv, fac in (("fp32", "fp64"), ("fp64", "fp32"), ("fp32", "fp32")):
+    for vdv, fac in (("fp32", "fp64"), ("fp64", "fp32"), ("fp64", "fp48"), ("fp32", "fp32")):
         ctx.set_precision(vdv, fac)
-        ops.create_operators()
-        slaph.contract(ctx)
+        step()
         t0 = time.perf_counter()
-        ops.create_operators()
-        slaph.contract(ctx)
+        step()
         s = time.perf_counter() - t0
         got = correlators()
         err = np.array([np.max(np.abs(got[d][n] - v)) / max(np.max(np.abs(v)), 1e-300)

@@ -85,8 +85,10 @@ int slc_synthetic_inputs(slc_ctx *ctx, unsigned long long seed, int orthonormali
  * on the device; slc_create_operators produces V(t) timeslice by timeslice
  * from the eigenvector files of slc_read_inputs or from the seeded generator
  * of slc_synthetic_inputs, as the reference reads them per timeslice while it
- * builds VdaggerV (src/OperatorFactory.cpp:263-277).  Call after
- * slc_configure and before the inputs; slc_set_eigenvectors is then refused.
+ * builds VdaggerV (src/OperatorFactory.cpp:263-277).  Call before the
+ * inputs (before slc_configure as well, so that the full eigenvector array
+ * is never allocated); slc_set_eigenvectors is then refused.  The mode stays
+ * set across later slc_configure calls.
  * No reference counterpart as a switch (the reference always streams). */
 int slc_set_timeslice_streaming(slc_ctx *ctx, int on);
 
@@ -198,11 +200,15 @@ int slc_fetch_correlators(const slc_ctx *ctx, const char *diagram, double *out);
  * events): what = "vdaggerv", "vdaggerv_kernel", "factors", "contract". */
 int slc_stage_time(const slc_ctx *ctx, const char *what, double *ms);
 
-/* Launch statistics of one kernel in the last contract (no reference
- * counterpart; for roofline measurements): kernel = "k_trace_lstat";
- * what = "ms" (device time summed over its launches, HIP events on the
- * engine stream), "launches", or "bytes" (algorithmic bytes summed over the
- * launches: each operand matrix once, every partial sum written once). */
+/* Launch statistics of one kernel in the last contract or create_operators
+ * (no reference counterpart; for roofline measurements): kernel =
+ * "k_trace_lstat_q" / "k_trace_lstat" (non-separable traces) or "k_vdv_ft"
+ * (VdaggerV, sitewise Fourier form); what = "ms" (device time summed over its
+ * launches, HIP events on the engine stream), "launches", "bytes"
+ * (algorithmic bytes summed over the launches: each operand matrix once,
+ * every output written once) or "flops" (operations of the kernel's own
+ * algorithm: for k_vdv_ft 24 per (element pair, site) for V^H V over three
+ * colours plus 5 per |pz| value of the pass for the z sums). */
 int slc_kernel_stat(const slc_ctx *ctx, const char *kernel, const char *what, double *value);
 
 /* Upper bound on the device memory the contraction may use for per-chunk

@@ -322,6 +322,7 @@ int slc_kernel_stat(const slc_ctx *cctx, const char *kernel, const char *what, d
     if (w == "ms") *value = ks.ms;
     else if (w == "launches") *value = ks.launches;
     else if (w == "bytes") *value = ks.bytes;
+    else if (w == "flops") *value = ks.flops;
     else throw std::out_of_range("unknown kernel statistic " + w);
   });
 }

@@ -120,6 +120,7 @@ void Engine::require_inputs() const {
 
 void Engine::alloc_inputs() {
   // streaming: one timeslice of eigenvectors resident
+  V_.reset();
   V_.alloc(sizeof(double2) * K_ * N_ * (streaming_ ? 1 : Lt_));
   rho_.alloc(sizeof(double2) * (size_t)nrnd_ * Lt_ * N_ * 4);
   peram_.alloc(sizeof(double2) * (size_t)nrnd_ * gd_.peram_rows() * gd_.peram_cols());
@@ -127,13 +128,14 @@ void Engine::alloc_inputs() {
 }
 
 void Engine::set_streaming(bool on) {
-  require_configured();
   if (on == streaming_) return;
   streaming_ = on;
+  if (!configured_) return;  // configure() allocates for the mode set
   have_ev_ = have_ops_ = false;
   ev_source_ = 0;
   ft_.reset();
   SLAPH_HIP(hipSetDevice(device_));
+  V_.reset();
   V_.alloc(sizeof(double2) * K_ * N_ * (streaming_ ? 1 : Lt_));
 }
 
@@ -460,6 +462,7 @@ void Engine::create_operators(int config) {
       prec_vdv_ == 0 && !ids.empty() && gd_.Lz <= kFtMaxLz && !(direct && *direct && *direct != '0');
   EventTimer tm(stream_);
   int launches = 0;
+  kstats_.erase("k_vdv_ft");
   auto plain = [&](int t0, int nt, const double2 *V) {
     if (ids.empty()) return;
     launches += fourier ? vdaggerv_fourier(ids, t0, nt, V) : vdaggerv_direct(ids, t0, nt, V);
@@ -528,37 +531,61 @@ int Engine::vdaggerv_direct(std::vector<int> const &ids, int t0, int nt, const d
 
 // Displaced operators at timeslice t: W = D V by the one-link displacements
 // of the operator in order (each '>' forward / '<' backward along x, y or
-// z), then VdaggerV = V^+ W on FP64 MFMA (OperatorFactory.cpp:287-300).  The
-// displacement is the covariant one, U_mu(x) V(x + mu) / U_mu(x - mu)^+ V(x -
-// mu), computed from the unchanged input of each step (the reference updates
-// its result in place while reading it, displace_eigenvectors,
-// OperatorFactory.cpp:33-55, so that some sites read already-displaced
-// neighbours).
+// z), then VdaggerV = V^+ W on MFMA (OperatorFactory.cpp:287-300), the
+// distinct displacement paths of a batch in one launch of the VdaggerV
+// kernel (A panel V, B panel the path's W).  The displacement is the
+// covariant one, U_mu(x) V(x + mu) / U_mu(x - mu)^+ V(x - mu), computed from
+// the unchanged input of each step (the reference updates its result in
+// place while reading it, displace_eigenvectors, OperatorFactory.cpp:33-55,
+// so that some sites read already-displaced neighbours).  The momentum does
+// not enter a displaced operator, so operators with the same displacement
+// path are equal: each path is computed once and copied.
 int Engine::vdaggerv_displaced(std::vector<int> const &ids, int t, const double2 *V) {
   const size_t kn = (size_t)K_ * N_;
-  vdisp_.ensure(sizeof(double2) * kn);
-  vtmp_.ensure(sizeof(double2) * kn);
   const double2 *U = U_.as<double2>() + (size_t)t * 27 * (K_ / 3);
+  std::vector<int> paths;  // first id of each distinct path
+  std::map<Displacement, int> first;
+  for (int id : ids)
+    if (first.emplace(gd_.vdaggerv_lookup.at(id).disp, id).second) paths.push_back(id);
+  const int batch = std::min<int>(int(paths.size()), kDispBatch);
+  vdisp_.ensure(sizeof(double2) * kn * batch);
+  vtmp_.ensure(sizeof(double2) * kn);
   int launches = 0;
-  DevBuf dj;
-  for (int id : ids) {
-    auto const &disp = gd_.vdaggerv_lookup.at(id).disp;
-    const double2 *cur = V;
-    for (size_t k = 0; k < disp.size(); ++k) {
-      const char axis = disp[k].second;
-      const int mu = axis == 'x' ? 0 : (axis == 'y' ? 1 : (axis == 'z' ? 2 : -1));
-      if (mu < 0 || (disp[k].first != '>' && disp[k].first != '<'))
-        throw std::runtime_error("Something wrong with the displacement in the operator definition!");
-      double2 *out = (k % 2 == 0) ? vdisp_.as<double2>() : vtmp_.as<double2>();
-      launch_displace(U, cur, out, gd_.Lx, gd_.Ly, gd_.Lz, N_, mu, disp[k].first == '>' ? 1 : 0, stream_);
-      cur = out;
-      ++launches;
+  auto out_of = [&](int id) { return vdv_.as<double2>() + ((size_t)id * Lt_ + t) * N_ * N_; };
+  for (size_t b0 = 0; b0 < paths.size(); b0 += batch) {
+    std::vector<VdvJob> jobs;
+    for (size_t b = b0; b < std::min(paths.size(), b0 + batch); ++b) {
+      auto const &disp = gd_.vdaggerv_lookup.at(paths[b]).disp;
+      double2 *W = vdisp_.as<double2>() + (b - b0) * kn;
+      const double2 *cur = V;
+      const size_t steps = disp.size();
+      for (size_t k = 0; k < steps; ++k) {
+        const char axis = disp[k].second;
+        const int mu = axis == 'x' ? 0 : (axis == 'y' ? 1 : (axis == 'z' ? 2 : -1));
+        if (mu < 0 || (disp[k].first != '>' && disp[k].first != '<'))
+          throw std::runtime_error("Something wrong with the displacement in the operator definition!");
+        // alternate so that the last step lands in W
+        double2 *out = ((steps - 1 - k) % 2 == 0) ? W : vtmp_.as<double2>();
+        launch_displace(U, cur, out, gd_.Lx, gd_.Ly, gd_.Lz, N_, mu, disp[k].first == '>' ? 1 : 0, stream_);
+        cur = out;
+        ++launches;
+      }
+      VdvJob j{};
+      j.V = V;
+      j.W = W;
+      j.np = 1;
+      j.C[0] = out_of(paths[b]);
+      jobs.push_back(j);
     }
-    AhBJob j{V, cur, vdv_.as<double2>() + ((size_t)id * Lt_ + t) * N_ * N_, nullptr, nullptr, K_, K_, N_};
-    upload(dj, std::vector<AhBJob>{j});
-    launch_ahb(dj.as<AhBJob>(), 1, N_, N_, int(K_), stream_);
+    upload(gjob_, jobs);
+    launch_vdv(gjob_.as<VdvJob>(), int(jobs.size()), N_, K_, prec_vdv_, stream_);
     ++launches;
     SLAPH_HIP(hipStreamSynchronize(stream_));  // the job table and work space are reused
+  }
+  for (int id : ids) {
+    const int src = first.at(gd_.vdaggerv_lookup.at(id).disp);
+    if (src != id)
+      SLAPH_HIP(hipMemcpyAsync(out_of(id), out_of(src), sizeof(double2) * N_ * N_, hipMemcpyDeviceToDevice, stream_));
   }
   return launches;
 }
@@ -691,9 +718,18 @@ int Engine::vdaggerv_fourier(std::vector<int> const &ids, int t_first, int nt_al
     launch_vrow(V + (size_t)c0 * K_ * N_, vr_.as<double2>(), K_, N_, F.Npad, nt, stream_);
     ++launches;
     for (auto &P : F.passes) {
+      EventTimer tm(stream_);
       launch_vdv_ft(vr_.as<double2>(), K_ * F.Npad, FtLayout{F.Npad, 1, F.Npad}, Lx, Ly, Lz, F.tiles.as<int2>(), F.ntp,
                     nt, P.p,
                     P.ey.as<double2>(), P.zcs.as<double2>(), ftu_.as<double2>(), stream_);
+      // own operation count: 24 per (element pair, site) for W = V^H V over
+      // three colours, 5 per |pz| value for the z sums; bytes: V read once,
+      // the per-(x, py, pz) sums written once
+      KernelStat &ks = kstats_["k_vdv_ft"];
+      ks.ms += tm.stop();
+      ks.flops += (24.0 + 5.0 * std::max(1, P.p.nk)) * F.ntp * 256.0 * (K_ / 3) * nt;
+      ks.bytes += 16.0 * K_ * N_ * nt + 16.0 * Lx * P.p.nc * F.ntp * 256.0 * nt;
+      ++ks.launches;
       launch_vdv_xsum(ftu_.as<double2>(), P.p.nc, F.ntp, nt, Lx, F.tiles.as<int2>(), P.tg.as<FtTarget>(),
                       int(P.targets.size()), P.ex.as<double2>(), N_, t_first + c0, (long long)N_ * N_, stream_);
       launches += 2;

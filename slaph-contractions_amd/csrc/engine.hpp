@@ -73,11 +73,12 @@ struct StageTimes {
 };
 
 // Launches of one kernel in the last create_operators / contract: summed
-// device time (HIP events around each launch on the engine stream) and the
+// device time (HIP events around each launch on the engine stream), the
 // summed algorithmic bytes the launches must move (each operand matrix read
-// once, every output written once).
+// once, every output written once) and the summed floating-point operations
+// the kernel's own algorithm performs.
 struct KernelStat {
-  double ms = 0, bytes = 0;
+  double ms = 0, bytes = 0, flops = 0;
   int launches = 0;
 };
 
@@ -113,7 +114,8 @@ class Engine {
   // create_operators produces V(t) timeslice by timeslice from the eigenvector
   // files (read_inputs) or the seeded generator (synthetic_inputs), as the
   // reference reads them per timeslice (OperatorFactory.cpp:263-277).  Set
-  // before the inputs; set_eigenvectors is not available in this mode.
+  // before the inputs (before configure() too: it allocates for the mode);
+  // set_eigenvectors is not available in this mode.
   void set_streaming(bool on);
   bool streaming() const { return streaming_; }
 
@@ -219,6 +221,7 @@ class Engine {
   DevBuf U_;  // gauge links, Lt x sites x 3 x 9 (allocated when displaced operators exist)
   bool have_gauge_ = false;
   DevBuf vdisp_, vtmp_, gram_, rinv_, gjob_;  // displacement and orthonormalisation work space
+  static constexpr int kDispBatch = 4;        // displacement paths (W = D V, K x N each) per launch
   long long K_ = 0;
   int N_ = 0, D_ = 0, Lt_ = 0, nb_ = 0, nrnd_ = 0;
 

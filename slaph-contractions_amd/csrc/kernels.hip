@@ -261,7 +261,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // One workgroup = one 32 x 32 complex output tile of every momentum of the
 // group; 4 waves, each a 16 x 16 block.  V is streamed in K-chunks of 32 rows
 // through double-buffered LDS (the A and B panels are columns of the same
-// V(t)); the momentum phases of the chunk's sites are staged next to them and
+// V(t), or of V and the displaced W = D V for a displaced operator, whose
+// job has one entry and no phase); the momentum phases of the chunk's sites are staged next to them and
 // applied to the B fragment in registers, so every V element loaded feeds
 // 3 * np MFMAs.  The complex product uses three real products (Gauss):
 // conj(a) b = (ax bx + ay by) + i ((ax - ay)(bx + by) - ax bx + ay by), so the
@@ -313,6 +314,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const double2 *__restrict__ V = J.V;
+  const double2 *__restrict__ Vb = J.W ? J.W : J.V;
   Acc p1[kVdvGroup], p2[kVdvGroup], p3[kVdvGroup];
 #pragma unroll
   for (int p = 0; p < kVdvGroup; ++p) {
@@ -330,7 +332,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       const int kk = idx % VDV_KC, c = idx / VDV_KC;
       const long long k = k0 + kk;
       ra[r] = (k < K && m0 + c < N) ? V[(long long)(m0 + c) * K + k] : make_double2(0.0, 0.0);
-      rb[r] = (k < K && n0 + c < N) ? V[(long long)(n0 + c) * K + k] : make_double2(0.0, 0.0);
+      rb[r] = (k < K && n0 + c < N) ? Vb[(long long)(n0 + c) * K + k] : make_double2(0.0, 0.0);
     }
     if (tid < kVdvGroup * VDV_SITES) {
       const int p = tid / VDV_SITES, j = tid % VDV_SITES;

@@ -60,6 +60,7 @@ struct VdvJob {
   const double2 *phase[kVdvGroup];
   double2 *C[kVdvGroup];
   int np;
+  const double2 *W;  // right operand (same shape as V); nullptr: V itself
 };
 // f32 != 0: single-pass FP32 MFMA (V and phases rounded to FP32, FP32
 // accumulation), results widened to FP64.

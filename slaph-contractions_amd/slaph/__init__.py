@@ -278,8 +278,10 @@ class Context:
         return v.value
 
     def kernel_stat(self, kernel: str, what: str) -> float:
-        """Launch statistics of one kernel in the last contract: what = "ms"
-        (summed device time), "launches" or "bytes" (summed algorithmic bytes)."""
+        """Launch statistics of one kernel in the last contract or
+        create_operators: what = "ms" (summed device time), "launches",
+        "bytes" (summed algorithmic bytes) or "flops" (summed operations of
+        the kernel's own algorithm)."""
         v = _c.c_double()
         self._check(_lib.slc_kernel_stat(self._h, kernel.encode(), what.encode(), _c.byref(v)))
         return v.value

@@ -203,3 +203,72 @@ def test_single_block_pair_reduced_operators_against_oracle(slaph, workload, mon
     assert n > 0
     say("  %d correlators, worst relative error %.2e" % (n, worst))
     ctx.close()
+
+
+def test_displaced_large_volume_streamed_against_oracle(slaph):
+    """disp48 -- 48^3 x 96, N_ev = 400, one eigenvector timeslice resident,
+    seeded SU(3) links smeared by 3 stout steps (rho = 0.1), 33 momenta
+    undisplaced and displaced >x, >y, >z: the smeared links of sampled
+    timeslices in SU(3); every displaced VdaggerV of sampled timeslices
+    against numpy V^+ (D V) on the downloaded eigenvectors and links (1e-12
+    relative), operators of one displacement path equal whatever their
+    momentum, sampled undisplaced ones against numpy; then one diagonal block
+    pair of C2c (displaced and |p|^2 <= 1 undisplaced operators, |P|^2 <= 1)
+    against the oracle's contract fed the device VdaggerV, 1e-12 norm-wise."""
+    w = dict(bench.WORKLOADS["disp48"])
+    text = bench.infile(w, pmax="0,1", diagrams=("C2c",), total_momenta="0,1")
+    cfg = o.parse_infile(text)
+    lut = o.init_lookup_tables(cfg)
+    ctx = slaph.Context(0)
+    ctx.set_timeslice_streaming(True)
+    ctx.configure(text)
+    ctx.synthetic_inputs(SEED, False)
+    ctx.synthetic_gauge(SEED + 2, 0.2)
+    ctx.smear_stout(0.1, 3)
+    ops = slaph.OperatorFactory(ctx)
+    say("disp48 create_operators (streamed)")
+    ops.create_operators()
+    L, Lt = w["L"], w["T"]
+    by_path = {}
+    for vid, mom, disp in lut["vdv_lookup"]:
+        by_path.setdefault(disp, []).append((vid, mom))
+    for t in (0, Lt // 2 + 3):
+        say("disp48 sampled timeslice", t)
+        U = ctx.get_gauge_links(t)
+        m = U.reshape(-1, 3, 3)[::97]
+        assert np.max(np.abs(m @ m.conj().swapaxes(1, 2) - np.eye(3))) < 1e-13
+        assert np.max(np.abs(np.linalg.det(m) - 1.0)) < 1e-13
+        V = ctx.get_input("eigenvectors", t)
+        for disp, members in by_path.items():
+            if disp:
+                want = V.conj().T @ o.displace(V, U, disp, (L, L, L))
+                first = ops.return_vdaggerv(members[0][0], t)
+                err = np.max(np.abs(first - want)) / np.max(np.abs(want))
+                assert err <= 1e-12, (t, disp, err)
+                for vid, _ in members[1:]:
+                    assert np.array_equal(ops.return_vdaggerv(vid, t), first), (t, disp, vid)
+            else:
+                for vid, mom in members[:: max(1, len(members) // 2)]:
+                    if vid == lut["index_of_unity"]:
+                        continue
+                    want = o.vdaggerv(V, o.create_momenta(L, L, L, mom))
+                    err = np.max(np.abs(ops.return_vdaggerv(vid, t) - want)) / np.max(np.abs(want))
+                    assert err <= 1e-12, (t, mom, err)
+        del V, U
+    say("disp48 single block pair")
+    slaph.contract(ctx, single_time_slice_combination=0)
+    nrnd = w["nrnd"]
+    perams = [ctx.get_input("perambulator", r) for r in range(nrnd)]
+    rnds = [ctx.get_input("random_vector", r) for r in range(nrnd)]
+    vdv = {(vid, t): ops.return_vdaggerv(vid, t) for vid, _, _ in lut["vdv_lookup"] for t in range(Lt)}
+    want = o.contract(cfg, lut, vdv, perams, rnds, single_block_pair=0)
+    n = 0
+    for d, lst in want.items():
+        got = ctx.correlators(d)
+        for name, vals in lst:
+            err = np.max(np.abs(got[name] - vals)) / max(np.max(np.abs(vals)), 1e-300)
+            assert err <= 1e-12, (d, name, err)
+            n += 1
+    assert n > 0
+    say("  %d correlators" % n)
+    ctx.close()
