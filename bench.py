@@ -323,7 +323,8 @@ def precision_forms(ctx, ops, fp64_s, step):
 
     ref = correlators()
     out = []
-    for vdv, fac in (("fp32", "fp64"), ("fp64", "fp32"), ("fp64", "fp48"), ("fp32", "fp32")):
+    for vdv, fac in (("fp32", "fp64"), ("bf16x3", "fp64"), ("ozaki", "fp64"), ("fp64", "fp32"), ("fp64", "fp48"),
+                     ("ozaki", "fp48"), ("fp32", "fp32")):
         ctx.set_precision(vdv, fac)
         step()
         t0 = time.perf_counter()

@@ -221,7 +221,11 @@ int slc_set_chunk_budget(slc_ctx *ctx, long long bytes);
  * complex<double> arithmetic, OperatorFactory.cpp:294-300 and
  * DilutedFactorFactory.cpp:33-253, defines; no reference counterpart):
  *   vdaggerv: 0 = FP64 MFMA (default), 1 = single-pass FP32 MFMA (V and the
- *             phases rounded to FP32, FP32 accumulation);
+ *             phases rounded to FP32, FP32 accumulation), 2 = three-term
+ *             split BF16 on the BF16 MFMA (six products per real product,
+ *             FP32 accumulation), 3 = Ozaki int8: columns scaled to |x| < 1
+ *             and cut into 7 slices of 7 bits, slice pairs i + j <= 8
+ *             multiplied exactly on the int8 MFMA, combined in FP64;
  *   factors:  0 = FP64 (default), 1 = diluted factors and product tables
  *             stored in FP32, 2 = stored in a 48-bit form (each FP64 part
  *             rounded to its upper 48 bits: 36 mantissa bits, relative

@@ -501,11 +501,34 @@ void Engine::create_operators(int config) {
 }
 
 // Direct form: one V(t) panel pass per group of up to kVdvGroup momenta
-// (FP64 or the single-pass FP32 precision form).
+// (FP64 or the single-pass FP32 precision form), or one pass per momentum
+// (the split-BF16 and Ozaki precision forms).
 int Engine::vdaggerv_direct(std::vector<int> const &ids, int t0, int nt, const double2 *V) {
   int Vs = gd_.Lx * gd_.Ly * gd_.Lz;
-  int ngroups = int((ids.size() + kVdvGroup - 1) / kVdvGroup);
   std::vector<VdvJob> jobs;
+  if (prec_vdv_ >= 2) {
+    if (prec_vdv_ == 3) {  // column exponents of every timeslice
+      colexp_.ensure(sizeof(int) * (size_t)nt * N_);
+      for (int t = 0; t < nt; ++t) launch_colexp(V + (size_t)t * K_ * N_, K_, N_, colexp_.as<int>() + (size_t)t * N_, stream_);
+    }
+    for (int t = t0; t < t0 + nt; ++t)
+      for (int id : ids) {
+        VdvJob j{};
+        j.V = V + (size_t)(t - t0) * K_ * N_;
+        j.np = 1;
+        j.phase[0] = phase_.as<double2>() + (size_t)id * Vs;
+        j.C[0] = vdv_.as<double2>() + ((size_t)id * Lt_ + t) * N_ * N_;
+        j.ex[0] = j.ex[1] = colexp_.as<int>() + (size_t)(t - t0) * N_;
+        jobs.push_back(j);
+      }
+    DevBuf dj;
+    upload(dj, jobs);
+    for (size_t i = 0; i < jobs.size(); i += 65535)
+      launch_vdv_split(dj.as<VdvJob>() + i, int(std::min<size_t>(65535, jobs.size() - i)), N_, K_, prec_vdv_, stream_);
+    SLAPH_HIP(hipStreamSynchronize(stream_));  // before the job table is freed
+    return int((jobs.size() + 65534) / 65535) + (prec_vdv_ == 3 ? nt : 0);
+  }
+  int ngroups = int((ids.size() + kVdvGroup - 1) / kVdvGroup);
   for (int t = t0; t < t0 + nt; ++t) {
     size_t at = 0;
     for (int g = 0; g < ngroups; ++g) {
@@ -575,10 +598,21 @@ int Engine::vdaggerv_displaced(std::vector<int> const &ids, int t, const double2
       j.W = W;
       j.np = 1;
       j.C[0] = out_of(paths[b]);
+      if (prec_vdv_ == 3) {  // column exponents of V (slot 0) and of each W
+        colexp_.ensure(sizeof(int) * (size_t)(1 + batch) * N_);
+        if (b == b0) launch_colexp(V, K_, N_, colexp_.as<int>(), stream_);
+        int *e = colexp_.as<int>() + (size_t)(1 + b - b0) * N_;
+        launch_colexp(W, K_, N_, e, stream_);
+        j.ex[0] = colexp_.as<int>();
+        j.ex[1] = e;
+      }
       jobs.push_back(j);
     }
     upload(gjob_, jobs);
-    launch_vdv(gjob_.as<VdvJob>(), int(jobs.size()), N_, K_, prec_vdv_, stream_);
+    if (prec_vdv_ >= 2)
+      launch_vdv_split(gjob_.as<VdvJob>(), int(jobs.size()), N_, K_, prec_vdv_, stream_);
+    else
+      launch_vdv(gjob_.as<VdvJob>(), int(jobs.size()), N_, K_, prec_vdv_, stream_);
     ++launches;
     SLAPH_HIP(hipStreamSynchronize(stream_));  // the job table and work space are reused
   }

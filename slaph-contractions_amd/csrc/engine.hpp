@@ -154,12 +154,14 @@ class Engine {
     chunk_budget_ = bytes;
     plan_.reset();  // chunk tables depend on the budget
   }
-  // Precision forms: vdaggerv 0 = FP64, 1 = single-pass FP32 MFMA; factors
-  // 0 = FP64, 1 / 2 = factor slabs and product tables stored in FP32 / in the
-  // 48-bit form (arithmetic and every accumulation in FP64).
+  // Precision forms: vdaggerv 0 = FP64, 1 = single-pass FP32 MFMA, 2 =
+  // three-term split BF16 (FP32 accumulation), 3 = Ozaki int8 slices (FP64
+  // combination); factors 0 = FP64, 1 / 2 = factor slabs and product tables
+  // stored in FP32 / in the 48-bit form (arithmetic and every accumulation in
+  // FP64).
   void set_precision(int vdaggerv, int factors) {
-    if (vdaggerv < 0 || vdaggerv > 1 || factors < 0 || factors > 2)
-      throw std::out_of_range("precision: vdaggerv must be 0 or 1, factors 0, 1 or 2");
+    if (vdaggerv < 0 || vdaggerv > 3 || factors < 0 || factors > 2)
+      throw std::out_of_range("precision: vdaggerv must be 0, 1, 2 or 3, factors 0, 1 or 2");
     prec_vdv_ = vdaggerv;
     if (factors != prec_factors_) plan_.reset();
     prec_factors_ = factors;
@@ -221,6 +223,7 @@ class Engine {
   DevBuf U_;  // gauge links, Lt x sites x 3 x 9 (allocated when displaced operators exist)
   bool have_gauge_ = false;
   DevBuf vdisp_, vtmp_, gram_, rinv_, gjob_;  // displacement and orthonormalisation work space
+  DevBuf colexp_;                             // column exponents of the Ozaki VdaggerV form
   static constexpr int kDispBatch = 4;        // displacement paths (W = D V, K x N each) per launch
   long long K_ = 0;
   int N_ = 0, D_ = 0, Lt_ = 0, nb_ = 0, nrnd_ = 0;

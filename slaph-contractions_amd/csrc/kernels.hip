@@ -394,6 +394,249 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 }
 
 // ---------------------------------------------------------------------------
+// VdaggerV precision forms on the low-precision matrix cores, one momentum (or
+// one displaced operator) per job, same tiling and LDS staging as k_vdv (the
+// phase applied to the B panel as it is staged, in FP64).
+//  * kForm 0, split BF16: every FP64 operand element x = hi + mid + lo with
+//    three BF16 terms (each the RNE rounding of the remainder), products
+//    hi.hi + hi.mid + mid.hi + hi.lo + lo.hi + mid.mid on the BF16 MFMA with
+//    FP32 accumulation (about 24 significant bits per operand; the K-long
+//    FP32 sums bound the accuracy).  Four real products per complex one.
+//  * kForm 1, Ozaki int8: each operand column scaled by 2^-e (e from the
+//    column's largest modulus, so that |x| < 1) and cut into kOzakiSlices
+//    int8 slices of 7 bits, x = sum_i s_i 2^-7i; every slice pair with
+//    i + j <= S + 1 multiplied exactly on the int8 MFMA into int32 sums per
+//    diagonal d = i + j, which are moved into FP64 accumulators (times
+//    2^-7d) every kOzakiFlush K-chunks, before an int32 sum can overflow
+//    (2 S 127^2 32 kOzakiFlush < 2^31); the result is scaled back by
+//    2^(eA + eB).  Only the dropped slice pairs and the truncation of the
+//    slicing limit the accuracy: about 2^-7(S+1) of max|A| max|B| K.
+// Lane layouts (checked on the device): bf16 16x16x16 A lane l row l%16,
+// k = 4(l/16) + j; i8 16x16x32 k = 8(l/16) + j; results row 4(l/16) + i,
+// column l%16.
+// ---------------------------------------------------------------------------
+typedef int i4v __attribute__((ext_vector_type(4)));
+typedef short s4v __attribute__((ext_vector_type(4)));
+constexpr int kOzakiFlush = 256;
+
+__device__ __forceinline__ short bf16_rne(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return short(u >> 16);
+}
+__device__ __forceinline__ double bf16_value(short h) { return double(__uint_as_float(unsigned(static_cast<unsigned short>(h)) << 16)); }
+
+// three-term BF16 split of x: x = hi + mid + lo up to the last term's rounding
+__device__ __forceinline__ void split3(double x, short &hi, short &mid, short &lo) {
+  hi = bf16_rne(float(x));
+  const double r1 = x - bf16_value(hi);
+  mid = bf16_rne(float(r1));
+  const double r2 = r1 - bf16_value(mid);
+  lo = bf16_rne(float(r2));
+}
+
+template <int S>
+__device__ __forceinline__ void ozaki_slices(const double (&x)[8], long long (&out)[S]) {
+#pragma unroll
+  for (int i = 0; i < S; ++i) out[i] = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    double v = x[j];
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const double y = v * 128.0;
+      const double t = trunc(y);  // |t| <= 127 since |v| < 1
+      v = y - t;                 // exact
+      out[i] |= (long long)(unsigned char)(signed char)int(t) << (8 * j);
+    }
+  }
+}
+
+template <int kForm>
+__global__ __launch_bounds__(256) void k_vdv_split(const VdvJob *__restrict__ jobs, int N, long long K, int tiles_n) {
+  constexpr int S = kOzakiSlices;
+  const VdvJob &J = jobs[blockIdx.y];
+  const int m0 = (blockIdx.x / tiles_n) * VDV_T, n0 = (blockIdx.x % tiles_n) * VDV_T;
+  __shared__ double2 As[2][VDV_T][VDV_KC + 1];
+  __shared__ double2 Bs[2][VDV_T][VDV_KC + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const double2 *__restrict__ V = J.V;
+  const double2 *__restrict__ Vb = J.W ? J.W : J.V;
+  const double2 *__restrict__ ph = J.phase[0];
+  double2 ra[4], rb[4];
+  const int nk = int((K + VDV_KC - 1) / VDV_KC);
+  auto load = [&](int kc) {
+    const long long k0 = (long long)kc * VDV_KC;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int idx = tid + r * 256;
+      const int kk = idx % VDV_KC, c = idx / VDV_KC;
+      const long long k = k0 + kk;
+      ra[r] = (k < K && m0 + c < N) ? V[(long long)(m0 + c) * K + k] : make_double2(0.0, 0.0);
+      double2 b = (k < K && n0 + c < N) ? Vb[(long long)(n0 + c) * K + k] : make_double2(0.0, 0.0);
+      if (ph && k < K) {
+        const double2 p = ph[k / 3];
+        b = make_double2(p.x * b.x - p.y * b.y, p.x * b.y + p.y * b.x);
+      }
+      rb[r] = b;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int idx = tid + r * 256;
+      As[buf][idx / VDV_KC][idx % VDV_KC] = ra[r];
+      Bs[buf][idx / VDV_KC][idx % VDV_KC] = rb[r];
+    }
+  };
+  const int arow = wm * 16 + (lane & 15), brow = wn * 16 + (lane & 15);
+  // per-column scales of the Ozaki form (2^-e of this lane's A row / B column)
+  double sa = 1.0, sb = 1.0;
+  int ea = 0, eb = 0;
+  if (kForm == 1) {
+    if (m0 + arow < N) ea = J.ex[0][m0 + arow];
+    if (n0 + brow < N) eb = J.ex[1][n0 + brow];
+    sa = ldexp(1.0, -ea);
+    sb = ldexp(1.0, -eb);
+  }
+  f4 facc_re = {0, 0, 0, 0}, facc_im = {0, 0, 0, 0};  // split BF16
+  i4v iacc_re[S], iacc_im[S];                          // Ozaki, diagonal d = 2 .. S + 1
+  double dre[4] = {0, 0, 0, 0}, dim[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < S; ++d) {
+    iacc_re[d] = i4v{0, 0, 0, 0};
+    iacc_im[d] = i4v{0, 0, 0, 0};
+  }
+  auto flush = [&]() {
+#pragma unroll
+    for (int d = 0; d < S; ++d) {
+      const double sc = ldexp(1.0, -7 * (d + 2));
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        dre[i] += sc * double(iacc_re[d][i]);
+        dim[i] += sc * double(iacc_im[d][i]);
+      }
+      iacc_re[d] = i4v{0, 0, 0, 0};
+      iacc_im[d] = i4v{0, 0, 0, 0};
+    }
+  };
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kc = 0; kc < nk; ++kc) {
+    const int buf = kc & 1;
+    if (kc + 1 < nk) load(kc + 1);
+    if (kForm == 0) {
+#pragma unroll
+      for (int kq = 0; kq < VDV_KC; kq += 16) {
+        s4v axh, axm, axl, ayh, aym, ayl, bxh, bxm, bxl, byh, bym, byl;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int k = kq + 4 * (lane >> 4) + j;
+          const double2 a = As[buf][arow][k], b = Bs[buf][brow][k];
+          short h, m, l;
+          split3(a.x, h, m, l);
+          axh[j] = h, axm[j] = m, axl[j] = l;
+          split3(a.y, h, m, l);
+          ayh[j] = h, aym[j] = m, ayl[j] = l;
+          split3(b.x, h, m, l);
+          bxh[j] = h, bxm[j] = m, bxl[j] = l;
+          split3(b.y, h, m, l);
+          byh[j] = h, bym[j] = m, byl[j] = l;
+        }
+        // conj(a) b: Re = ax bx + ay by, Im = ax by - ay bx
+        auto prod = [&](f4 c, s4v xh, s4v xm, s4v xl, s4v yh, s4v ym, s4v yl) {
+          c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xl, yh, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xh, yl, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xm, ym, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xm, yh, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xh, ym, c, 0, 0, 0);
+          return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xh, yh, c, 0, 0, 0);
+        };
+        facc_re = prod(facc_re, axh, axm, axl, bxh, bxm, bxl);
+        facc_re = prod(facc_re, ayh, aym, ayl, byh, bym, byl);
+        facc_im = prod(facc_im, axh, axm, axl, byh, bym, byl);
+        const s4v sg = {short(0x8000), short(0x8000), short(0x8000), short(0x8000)};
+        facc_im = prod(facc_im, ayh ^ sg, aym ^ sg, ayl ^ sg, bxh, bxm, bxl);  // -ay: sign bits flipped
+      }
+    } else {
+      double ax[8], ay[8], bx[8], by[8], nbx[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = 8 * (lane >> 4) + j;
+        const double2 a = As[buf][arow][k], b = Bs[buf][brow][k];
+        ax[j] = a.x * sa;
+        ay[j] = a.y * sa;
+        bx[j] = b.x * sb;
+        by[j] = b.y * sb;
+        nbx[j] = -bx[j];
+      }
+      long long sx[S], sy[S], tx[S], ty[S], tn[S];
+      ozaki_slices<S>(ax, sx);
+      ozaki_slices<S>(ay, sy);
+      ozaki_slices<S>(bx, tx);
+      ozaki_slices<S>(by, ty);
+      ozaki_slices<S>(nbx, tn);
+#pragma unroll
+      for (int d = 0; d < S; ++d)      // diagonal i + j = d + 2 (slices numbered from 1)
+#pragma unroll
+        for (int i = 0; i <= d; ++i) {  // slice i + 1 of A with slice d - i + 1 of B
+          const int j = d - i;
+          iacc_re[d] = __builtin_amdgcn_mfma_i32_16x16x32_i8(sx[i], tx[j], iacc_re[d], 0, 0, 0);
+          iacc_re[d] = __builtin_amdgcn_mfma_i32_16x16x32_i8(sy[i], ty[j], iacc_re[d], 0, 0, 0);
+          iacc_im[d] = __builtin_amdgcn_mfma_i32_16x16x32_i8(sx[i], ty[j], iacc_im[d], 0, 0, 0);
+          iacc_im[d] = __builtin_amdgcn_mfma_i32_16x16x32_i8(sy[i], tn[j], iacc_im[d], 0, 0, 0);
+        }
+      if ((kc + 1) % kOzakiFlush == 0) flush();
+    }
+    if (kc + 1 < nk) store(buf ^ 1);
+    __syncthreads();
+  }
+  if (kForm == 1) flush();
+  double2 *C = J.C[0];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int row = m0 + wm * 16 + 4 * (lane >> 4) + i;
+    const int col = n0 + wn * 16 + (lane & 15);
+    if (row < N && col < N) {
+      double2 v;
+      if (kForm == 0) {
+        v = make_double2(double(facc_re[i]), double(facc_im[i]));
+      } else {
+        // this lane's A scale belongs to row wm*16 + lane%16, not to `row`
+        const int er = J.ex[0][row], ec = J.ex[1][col];
+        v = make_double2(ldexp(dre[i], er + ec), ldexp(dim[i], er + ec));
+      }
+      C[(long long)col * N + row] = v;
+    }
+  }
+}
+
+// e[n] with max_k |V[k, n]| (1 + 2^-40) < 2^e per column of a K x N matrix
+// (one workgroup per column); 0 for a zero column.
+__global__ __launch_bounds__(256) void k_colexp(const double2 *__restrict__ V, long long K, int *__restrict__ e) {
+  const int n = blockIdx.x;
+  double m = 0.0;
+  for (long long k = threadIdx.x; k < K; k += 256) {
+    const double2 v = V[(long long)n * K + k];
+    m = fmax(m, hypot(v.x, v.y));
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int ex = 0;
+    if (red[0] > 0.0) frexp(red[0] * (1.0 + 0x1p-40), &ex);
+    e[n] = ex;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // VdaggerV by the sitewise Fourier factorisation (see kernels.hpp).  One
 // single-wave workgroup = one (timeslice, x, tile pair I <= J).  Per site the
 // wave forms W = V_I^H V_J on the 16 x 16 tile by four accumulating MFMAs
@@ -2049,6 +2292,24 @@ void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, int f32, hi
     hipLaunchKernelGGL(k_vdv<true>, dim3(tiles * tiles, njobs), dim3(256), 0, s, d_jobs, N, K, tiles);
   else
     hipLaunchKernelGGL(k_vdv<false>, dim3(tiles * tiles, njobs), dim3(256), 0, s, d_jobs, N, K, tiles);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_vdv_split(const VdvJob *d_jobs, int njobs, int N, long long K, int form, hipStream_t s) {
+  if (njobs <= 0) return;
+  if (njobs > 65535) throw std::runtime_error("launch_vdv_split: too many jobs in one launch");
+  if (form != 2 && form != 3) throw std::runtime_error("launch_vdv_split: form must be 2 (split BF16) or 3 (Ozaki)");
+  const int tiles = (N + VDV_T - 1) / VDV_T;
+  if (form == 2)
+    hipLaunchKernelGGL(k_vdv_split<0>, dim3(tiles * tiles, njobs), dim3(256), 0, s, d_jobs, N, K, tiles);
+  else
+    hipLaunchKernelGGL(k_vdv_split<1>, dim3(tiles * tiles, njobs), dim3(256), 0, s, d_jobs, N, K, tiles);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_colexp(const double2 *V, long long K, int N, int *e, hipStream_t s) {
+  if (N <= 0) return;
+  hipLaunchKernelGGL(k_colexp, dim3(N), dim3(256), 0, s, V, K, e);
   SLAPH_HIP(hipGetLastError());
 }
 

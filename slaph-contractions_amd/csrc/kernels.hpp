@@ -60,11 +60,21 @@ struct VdvJob {
   const double2 *phase[kVdvGroup];
   double2 *C[kVdvGroup];
   int np;
-  const double2 *W;  // right operand (same shape as V); nullptr: V itself
+  const double2 *W;     // right operand (same shape as V); nullptr: V itself
+  const int *ex[2];     // Ozaki form: column exponents of V and of the right operand
 };
 // f32 != 0: single-pass FP32 MFMA (V and phases rounded to FP32, FP32
 // accumulation), results widened to FP64.
 void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, int f32, hipStream_t s);
+// VdaggerV precision forms, one entry per job (np = 1; phase[0] may be
+// null): form 2 = three-term split BF16 on the BF16 MFMA with FP32
+// accumulation, form 3 = Ozaki int8 (kOzakiSlices slices of 7 bits, exact
+// int32 slice products, FP64 combination; needs ex[0] / ex[1] from
+// launch_colexp of V and of the right operand).
+constexpr int kOzakiSlices = 7;
+void launch_vdv_split(const VdvJob *d_jobs, int njobs, int N, long long K, int form, hipStream_t s);
+// e[n] with max_k |V[k, n]| < 2^e for each column of a K x N matrix
+void launch_colexp(const double2 *V, long long K, int N, int *e, hipStream_t s);
 
 // VdaggerV by a sitewise Fourier factorisation.  The momentum phase of site
 // (x, y, z) factorises, exp(-i 2 pi (px x / Lx + py y / Ly + pz z / Lz)), so

@@ -290,11 +290,12 @@ class Context:
         self._check(_lib.slc_set_chunk_budget(self._h, int(nbytes)))
 
     def set_precision(self, vdaggerv: str = "fp64", factors: str = "fp64"):
-        """Precision forms: vdaggerv "fp64" | "fp32" (single-pass FP32 MFMA);
-        factors "fp64" | "fp32" | "fp48" (factor slabs and product tables
-        stored in FP32 or in 48-bit FP64 -- 36 mantissa bits --, arithmetic and
-        accumulation FP64)."""
-        vmodes = {"fp64": 0, "fp32": 1}
+        """Precision forms: vdaggerv "fp64" | "fp32" (single-pass FP32 MFMA) |
+        "bf16x3" (three-term split BF16, FP32 accumulation) | "ozaki" (int8
+        slices, exact slice products, FP64 combination); factors "fp64" |
+        "fp32" | "fp48" (factor slabs and product tables stored in FP32 or in
+        48-bit FP64 -- 36 mantissa bits --, arithmetic and accumulation FP64)."""
+        vmodes = {"fp64": 0, "fp32": 1, "bf16x3": 2, "ozaki": 3}
         fmodes = {"fp64": 0, "fp32": 1, "fp48": 2}
         if vdaggerv not in vmodes or factors not in fmodes:
             raise ValueError("unknown precision form %r / %r" % (vdaggerv, factors))

@@ -147,6 +147,21 @@ def test_displaced_vdaggerv_matches_oracle(slaph, inputs):
         assert np.max(np.abs(got - W)) < 1e-13, (vid, t, lut["vdv_lookup"][vid])
 
 
+@pytest.mark.parametrize("form,tol", [("bf16x3", 1e-5), ("ozaki", 1e-13)])
+def test_displaced_vdaggerv_split_forms(slaph, inputs, form, tol):
+    """Displaced operators through the split-BF16 and Ozaki forms (right
+    operand W = D V with its own column exponents) against the oracle."""
+    say("displaced vdaggerv", form)
+    cfg, lut, ev, U = inputs[:4]
+    ctx = _context(slaph, inputs)
+    ctx.set_precision(form, "fp64")
+    slaph.OperatorFactory(ctx).create_operators()
+    want = o.build_vdaggerv(cfg, lut, ev, links=U)
+    for (vid, t), W in want.items():
+        got = slaph.OperatorFactory(ctx).return_vdaggerv(vid, t)
+        assert np.max(np.abs(got - W)) <= tol * np.max(np.abs(W)), (form, vid, t)
+
+
 def test_displaced_contraction_matches_oracle(slaph, inputs):
     say("displaced contraction")
     cfg, lut, ev, U, perams, rnds = inputs

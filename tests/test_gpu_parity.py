@@ -423,6 +423,44 @@ def test_precision_forms_against_fp64(slaph):
     assert np.all(_rel_errors(again, ref) == 0.0)
 
 
+@pytest.mark.parametrize("form,vdv_tol,corr_tol", [("bf16x3", 1e-5, 1e-4), ("ozaki", 1e-13, 1e-12)])
+def test_split_vdaggerv_forms_against_fp64(slaph, form, vdv_tol, corr_tol):
+    """VdaggerV on the low-precision matrix cores: three-term split BF16
+    (FP32 accumulation; about FP32 accuracy) and the Ozaki int8 split (7
+    slices of 7 bits, exact slice products, FP64 combination; FP64-level
+    accuracy).  Every VdaggerV (p, t) against the FP64 path, relative to the
+    largest entry, and every correlator norm-wise; the p = 0 operator stays
+    the exact identity."""
+    say("split vdaggerv", form)
+    cfg = o.parse_infile(WIDE)
+    lut = o.init_lookup_tables(cfg)
+    ctx = slaph.Context(0)
+    ctx.configure(WIDE)
+    ctx.synthetic_inputs(77, True)
+    ops = slaph.OperatorFactory(ctx)
+
+    def run(vdv):
+        ctx.set_precision(vdv, "fp64")
+        ops.create_operators()
+        slaph.contract(ctx)
+        mats = {(vid, t): ops.return_vdaggerv(vid, t) for vid, _, _ in lut["vdv_lookup"] for t in range(cfg["Lt"])}
+        return mats, {d: ctx.correlators(d) for d in ctx.diagrams()}
+
+    ref_m, ref_c = run("fp64")
+    got_m, got_c = run(form)
+    worst = 0.0
+    for key, w in ref_m.items():
+        e = np.max(np.abs(got_m[key] - w)) / np.max(np.abs(w))
+        worst = max(worst, e)
+        assert e <= vdv_tol, (form, key, e)
+    assert np.array_equal(got_m[(lut["index_of_unity"], 0)], np.eye(cfg["number_of_eigen_vec"]))
+    err = _rel_errors(got_c, ref_c)
+    say("  vdaggerv max %.2e; correlators max %.2e median %.2e" % (worst, err.max(), np.median(err)))
+    assert err.max() <= corr_tol
+    assert worst > 0.0  # the split form did run
+    ctx.set_precision("fp64", "fp64")
+
+
 @pytest.mark.parametrize("infile", ["wide", "EI6-D24", "EI8-D32"])
 def test_fp48_storage_form_within_1e_10(slaph, infile):
     """Factor slabs and product tables in the 48-bit form (36 mantissa bits,
