@@ -202,6 +202,7 @@ def main():
     ap.add_argument("--no-precision-forms", action="store_true",
                     help="skip timing the precision forms against the FP64 path")
     ap.add_argument("--chunk-gib", type=float, default=0.0, help="chunk memory budget (0: automatic)")
+    ap.add_argument("--no-graphs", action="store_true", help="launch the contraction directly (no HIP graph replay)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: there is no multi-GPU path "
@@ -215,6 +216,7 @@ def main():
         ctx.set_timeslice_streaming(True)  # before configure: V of one timeslice is allocated
     ctx.configure(text)
     ctx.set_chunk_budget(int(args.chunk_gib * (1 << 30)))
+    ctx.set_graphs(not args.no_graphs)
     t0 = time.perf_counter()
     # streamed eigenvectors are generated per timeslice inside each step (in
     # place of the reference's per-timeslice file read), without the
@@ -267,7 +269,8 @@ def main():
         "data": "synthetic (seeded, device-generated; %s)" % (
             "eigenvectors generated per timeslice, not orthonormalised" if w.get("streaming")
             else "orthonormalised eigenvectors"),
-        "config": {"workload": _describe(args.workload, w), "precision": "fp64"},
+        "config": {"workload": _describe(args.workload, w), "precision": "fp64",
+                   "hip_graphs": not args.no_graphs},
         "stages_ms": {"vdaggerv": stage["vdaggerv"], "factors": stage["factors"],
                       "contract_incl_factors": stage["contract"]},
     }

@@ -233,6 +233,15 @@ int slc_set_chunk_budget(slc_ctx *ctx, long long bytes);
  * Correlator accumulation is FP64 (optionally Kahan) in every form. */
 int slc_set_precision(slc_ctx *ctx, int vdaggerv, int factors);
 
+/* HIP graphs for slc_contract (no reference counterpart; on by default):
+ * with on != 0 the second slc_contract of a plan captures the launch
+ * sequence of one configuration (every chunk's products and traces, the
+ * resolves, accumulations and the finalisation; kernels only) into a HIP
+ * graph and later calls replay it.  The statistics of slc_kernel_stat for the
+ * trace kernels are then those of the last call that launched directly.
+ * Changing the setting drops the plan. */
+int slc_set_graphs(slc_ctx *ctx, int on);
+
 /* C = A^H diag(phase[k/3]) B on the device (batch of `batch` products with
  * element strides sA, sB, sC): the kernel behind VdaggerV
  * (OperatorFactory.cpp:294-300).  A, B, C, phase are device pointers; phase

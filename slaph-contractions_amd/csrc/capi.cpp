@@ -327,6 +327,10 @@ int slc_kernel_stat(const slc_ctx *cctx, const char *kernel, const char *what, d
   });
 }
 
+int slc_set_graphs(slc_ctx *ctx, int on) {
+  return guarded(ctx, [&] { ctx->eng->set_graphs(on != 0); });
+}
+
 int slc_set_chunk_budget(slc_ctx *ctx, long long bytes) {
   return guarded(ctx, [&] {
     if (bytes < 0) throw std::out_of_range("budget must not be negative");

@@ -912,7 +912,14 @@ struct ContractCache {
   std::vector<ChunkPlan> chunks;
   DevBuf d_prod, d_gv, d_cval, d_zv, d_res, d_pv;
   std::vector<hipEvent_t> ev;  // event pairs around the left-stationary trace launches
+  // the launch sequence of one configuration (every chunk, resolve,
+  // accumulation and finalisation), captured on the second call of the plan
+  // and replayed by later calls
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
   ~ContractCache() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     for (auto e : ev) (void)hipEventDestroy(e);
   }
   std::vector<DevBuf> d_sum;  // per diagram: Lt x n_corr accumulators
@@ -1660,11 +1667,7 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
   // ---- accumulators and chunk buffers (kept with the plan) ----
   auto &d_sum = C.d_sum;
   d_sum.resize(corrs.size());
-  for (size_t c = 0; c < corrs.size(); ++c) {
-    size_t n = (size_t)Lt_ * corrs[c].reqs.size();
-    d_sum[c].alloc(sizeof(double2) * n);
-    SLAPH_HIP(hipMemsetAsync(d_sum[c].as<void>(), 0, sizeof(double2) * n, stream_));
-  }
+  for (size_t c = 0; c < corrs.size(); ++c) d_sum[c].alloc(sizeof(double2) * (size_t)Lt_ * corrs[c].reqs.size());
   {
     long long np_max = 1, ng_max = 1, nc_max = 1;
     for (auto const &ch : C.chunks) {
@@ -1683,125 +1686,159 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
   }
 
   // ---- launches: per chunk, products in topological order, then traces,
-  // then resolve + accumulate per diagram; no host work between chunks ----
+  // then resolve + accumulate per diagram; no host work between chunks.
+  // With graphs on, the second call of a plan captures the whole sequence
+  // (everything it launches is a kernel: no memset, copy or host work) and
+  // later calls replay it; the statistics of the left-stationary traces are
+  // then those of the last call that ran the launches directly. ----
   EventTimer ctimer(stream_);
+  const bool graph = graphs_ && !fresh && !scope && !progress;
   size_t ls_ev = 0;  // events used around left-stationary trace launches
   double ls_bytes = 0;
   bool ls_queued = false;
-  size_t last_bp = size_t(-1);
-  for (auto const &ch : C.chunks) {
-    if (progress && ch.bp != last_bp) {
-      // like the reference's per-block-pair message (Correlators.cpp:92-97)
-      std::fprintf(stderr, "[slaph] block pair %zu of %zu (%zu slice pairs, %d per launch)\n", ch.bp,
-                   bp_slices.size(), bp_slices[ch.bp].size(), ch.S);
-      std::fflush(stderr);
-    }
-    last_bp = ch.bp;
-    const int S = ch.S;
-    for (size_t fi = 0; fi < fams.size(); ++fi) {
-      Family &f = fams[fi];
-      for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
-        auto const &ns = f.shape.nodes[n];
-        NodePlan &np = f.nodes[n];
-        NodeChunk const &nc = ch.nodes[fi][n];
-        if (ns.kind == kLeaf || np.owner != int(n)) continue;
-        const bool root = n + 1 == f.shape.nodes.size();
-        auto src = [&](int child) -> const void * {
-          if (child < 0) return nullptr;
-          auto const &cs = f.shape.nodes[child];
-          if (cs.kind == kLeaf) return slab_of(f.shape.leaves[cs.leaf].q);
-          return C.d_prod.as<void>();
-        };
-        // operands stored transposed: only a trace's right operand may be
-        auto is_t = [&](int child) {
-          return child >= 0 && f.shape.nodes[child].kind == kProd && f.nodes[f.nodes[child].owner].transposed;
-        };
-        const int rt = is_t(ns.right) ? 1 : 0;
-        if (is_t(ns.left) || (!root && rt)) throw std::logic_error("transposed product table read as a factor");
-        const long long gb = ch.gbase[fi];
-        if (!root) {
-          launch_prod(src(ns.left), nc.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
-                      nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), C.d_prod.as<void>(), nc.base, np.stride,
-                      np.d_combos.as<ComboProd>(), int(np.combos.size()), nc.ntime, D_, np.transposed ? 1 : 0, f32,
-                      stream_);
-        } else if (nc.grid && !f.gg_tiles.empty()) {
-          const int ntile = int(f.gg_tiles.size()), nrun = int(f.gg_runs.size() / 3);
-          const int kp = trace_gemm_parts(ntile, nrun);
-          if (kp > 1) f.d_ggp.ensure(sizeof(double2) * (size_t)ntile * kp * 4096);
-          launch_trace_gemm(src(ns.left), src(ns.right), f.d_ggt.as<GridGTile>(), ntile, f.d_ggr.as<int>(), nrun,
-                            nc.d_tile.as<GridTile>(), C.d_gv.as<double2>(), gb, f.gstride, D_, rt, f32, kp,
-                            kp > 1 ? f.d_ggp.as<double2>() : nullptr, stream_);
-        } else if (nc.grid && !f.gr_groups.empty()) {
-          launch_trace_grid_r(src(ns.left), src(ns.right), f.d_grg.as<GridRGroup>(), int(f.gr_groups.size()),
-                              f.d_grr.as<int>(), nc.d_tile.as<GridTile>(), 1, C.d_gv.as<double2>(), gb, f.gstride,
-                              D_, rt, f32, stream_);
-        } else if (nc.grid) {
-          launch_trace_grid(src(ns.left), np.d_Xq.as<long long>(), src(ns.right), np.d_Yq.as<long long>(),
-                            f.d_gitems.as<GroupItem>(), int(f.gitems.size()), f.d_gpairs.as<int2>(),
-                            nc.d_tile.as<GridTile>(), 1, C.d_gv.as<double2>(), gb, f.gstride, D_, rt, f32, stream_);
-        } else if (ns.kind == kTrace2 && !f.ls_blocks.empty() && !lstat_off) {
-          if (ch.pbase[fi] + (long long)nc.ntime * f.ls_npart > ch.p_total)
-            throw std::logic_error("trace plan indexes outside the partial-sum buffer");
-          if (ls_ev + 2 > C.ev.size()) {
-            C.ev.resize(ls_ev + 2, nullptr);
-            SLAPH_HIP(hipEventCreate(&C.ev[ls_ev]));
-            SLAPH_HIP(hipEventCreate(&C.ev[ls_ev + 1]));
-          }
-          SLAPH_HIP(hipEventRecord(C.ev[ls_ev], stream_));
-          ls_bytes += f.ls_bytes * nc.ntime;
-          ls_queued = f.ls_queue;
-          if (f.ls_queue)
-            launch_trace_lstat_q(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
-                                 f.d_lsu.as<int>(), int(f.ls_blocks.size()), f.d_lst.as<LsTask>(),
-                                 f.d_lsd.as<long long>(), C.d_pv.as<double2>(), ch.pbase[fi], f.ls_npart, nc.ntime, D_,
-                                 rt, f32, f.d_lsq.as<int>(), stream_);
-          else
-            launch_trace_lstat(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
-                               f.d_lsb.as<LsBlock>(), int(f.ls_blocks.size()), f.d_lsc.as<LsChunk>(),
-                               f.d_lsl.as<int>(), f.d_lst.as<LsTask>(), f.d_lsd.as<long long>(), C.d_pv.as<double2>(),
-                               ch.pbase[fi], f.ls_npart, nc.ntime, D_, rt, f32, stream_);
-          SLAPH_HIP(hipEventRecord(C.ev[ls_ev + 1], stream_));
-          ls_ev += 2;
-          launch_ls_reduce(C.d_pv.as<double2>() + ch.pbase[fi], f.ls_npart, f.d_lsr.as<GroupItem>(),
-                           int(f.ls_red.size()), f.d_lsri.as<int>(), C.d_gv.as<double2>(), gb, f.gstride, nc.ntime,
-                           stream_);
-        } else {
-          launch_trace_groups(src(ns.left), nc.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
-                              nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), f.d_gitems.as<GroupItem>(),
-                              int(f.gitems.size()), f.d_gpairs.as<int2>(), C.d_gv.as<double2>(), gb, f.gstride,
-                              nc.ntime, D_, rt, f32, stream_);
+  if (graph && C.gexec) {
+    SLAPH_HIP(hipGraphLaunch(C.gexec, stream_));
+  } else {
+    struct CaptureGuard {  // ends a capture that a throwing launch left open
+      hipStream_t s;
+      bool on;
+      ~CaptureGuard() {
+        if (on) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
         }
       }
-      // subset sums of this family's trace vectors for the Moebius form
-      launch_zeta(C.d_gv.as<double2>(), ch.gbase[fi], f.gstride, C.d_gmasks.as<uint16_t>(), f.d_zitems.as<ZItem>(),
-                  int(f.zitems.size()), C.d_zmask.as<uint16_t>(), int(C.zmask.size()), C.d_zv.as<double2>(),
-                  ch.zbase[fi], f.zstride, S, stream_);
+    } capture{stream_, false};
+    if (graph) {
+      SLAPH_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
+      capture.on = true;
     }
-    for (size_t c = 0; c < corrs.size(); ++c) {
-      int ncorr = int(corrs[c].reqs.size());
-      launch_resolve(C.d_gv.as<double2>(), C.d_gmasks.as<uint16_t>(), ch.d_jobs[c].as<CorrJob>(), ncorr, S,
-                     C.d_zv.as<double2>(), C.d_zmask.as<uint16_t>(), int(C.zmask.size()), C.d_cval.as<double2>(),
-                     per_element ? 1 : 0, stream_);
-      launch_accumulate(C.d_cval.as<double2>(), ch.d_tds.as<int>(), ch.d_tdg.as<int4>(), ch.ntdg, ncorr,
-                        d_sum[c].as<double2>(), stream_);
+    for (size_t c = 0; c < corrs.size(); ++c)
+      launch_zero(d_sum[c].as<double2>(), (long long)Lt_ * corrs[c].reqs.size(), stream_);
+    size_t last_bp = size_t(-1);
+    for (auto const &ch : C.chunks) {
+      if (progress && ch.bp != last_bp) {
+        // like the reference's per-block-pair message (Correlators.cpp:92-97)
+        std::fprintf(stderr, "[slaph] block pair %zu of %zu (%zu slice pairs, %d per launch)\n", ch.bp,
+                     bp_slices.size(), bp_slices[ch.bp].size(), ch.S);
+        std::fflush(stderr);
+      }
+      last_bp = ch.bp;
+      const int S = ch.S;
+      for (size_t fi = 0; fi < fams.size(); ++fi) {
+        Family &f = fams[fi];
+        for (size_t n = 0; n < f.shape.nodes.size(); ++n) {
+          auto const &ns = f.shape.nodes[n];
+          NodePlan &np = f.nodes[n];
+          NodeChunk const &nc = ch.nodes[fi][n];
+          if (ns.kind == kLeaf || np.owner != int(n)) continue;
+          const bool root = n + 1 == f.shape.nodes.size();
+          auto src = [&](int child) -> const void * {
+            if (child < 0) return nullptr;
+            auto const &cs = f.shape.nodes[child];
+            if (cs.kind == kLeaf) return slab_of(f.shape.leaves[cs.leaf].q);
+            return C.d_prod.as<void>();
+          };
+          // operands stored transposed: only a trace's right operand may be
+          auto is_t = [&](int child) {
+            return child >= 0 && f.shape.nodes[child].kind == kProd && f.nodes[f.nodes[child].owner].transposed;
+          };
+          const int rt = is_t(ns.right) ? 1 : 0;
+          if (is_t(ns.left) || (!root && rt)) throw std::logic_error("transposed product table read as a factor");
+          const long long gb = ch.gbase[fi];
+          if (!root) {
+            launch_prod(src(ns.left), nc.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
+                        nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), C.d_prod.as<void>(), nc.base, np.stride,
+                        np.d_combos.as<ComboProd>(), int(np.combos.size()), nc.ntime, D_, np.transposed ? 1 : 0, f32,
+                        stream_);
+          } else if (nc.grid && !f.gg_tiles.empty()) {
+            const int ntile = int(f.gg_tiles.size()), nrun = int(f.gg_runs.size() / 3);
+            const int kp = trace_gemm_parts(ntile, nrun);
+            if (kp > 1) f.d_ggp.ensure(sizeof(double2) * (size_t)ntile * kp * 4096);
+            launch_trace_gemm(src(ns.left), src(ns.right), f.d_ggt.as<GridGTile>(), ntile, f.d_ggr.as<int>(), nrun,
+                              nc.d_tile.as<GridTile>(), C.d_gv.as<double2>(), gb, f.gstride, D_, rt, f32, kp,
+                              kp > 1 ? f.d_ggp.as<double2>() : nullptr, stream_);
+          } else if (nc.grid && !f.gr_groups.empty()) {
+            launch_trace_grid_r(src(ns.left), src(ns.right), f.d_grg.as<GridRGroup>(), int(f.gr_groups.size()),
+                                f.d_grr.as<int>(), nc.d_tile.as<GridTile>(), 1, C.d_gv.as<double2>(), gb, f.gstride,
+                                D_, rt, f32, stream_);
+          } else if (nc.grid) {
+            launch_trace_grid(src(ns.left), np.d_Xq.as<long long>(), src(ns.right), np.d_Yq.as<long long>(),
+                              f.d_gitems.as<GroupItem>(), int(f.gitems.size()), f.d_gpairs.as<int2>(),
+                              nc.d_tile.as<GridTile>(), 1, C.d_gv.as<double2>(), gb, f.gstride, D_, rt, f32, stream_);
+          } else if (ns.kind == kTrace2 && !f.ls_blocks.empty() && !lstat_off) {
+            if (ch.pbase[fi] + (long long)nc.ntime * f.ls_npart > ch.p_total)
+              throw std::logic_error("trace plan indexes outside the partial-sum buffer");
+            if (ls_ev + 2 > C.ev.size()) {
+              C.ev.resize(ls_ev + 2, nullptr);
+              SLAPH_HIP(hipEventCreate(&C.ev[ls_ev]));
+              SLAPH_HIP(hipEventCreate(&C.ev[ls_ev + 1]));
+            }
+            if (!graph) SLAPH_HIP(hipEventRecord(C.ev[ls_ev], stream_));
+            ls_bytes += f.ls_bytes * nc.ntime;
+            ls_queued = f.ls_queue;
+            if (f.ls_queue)
+              launch_trace_lstat_q(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
+                                   f.d_lsu.as<int>(), int(f.ls_blocks.size()), f.d_lst.as<LsTask>(),
+                                   f.d_lsd.as<long long>(), C.d_pv.as<double2>(), ch.pbase[fi], f.ls_npart, nc.ntime, D_,
+                                   rt, f32, f.d_lsq.as<int>(), stream_);
+            else
+              launch_trace_lstat(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
+                                 f.d_lsb.as<LsBlock>(), int(f.ls_blocks.size()), f.d_lsc.as<LsChunk>(),
+                                 f.d_lsl.as<int>(), f.d_lst.as<LsTask>(), f.d_lsd.as<long long>(), C.d_pv.as<double2>(),
+                                 ch.pbase[fi], f.ls_npart, nc.ntime, D_, rt, f32, stream_);
+            if (!graph) {
+              SLAPH_HIP(hipEventRecord(C.ev[ls_ev + 1], stream_));
+              ls_ev += 2;
+            }
+            launch_ls_reduce(C.d_pv.as<double2>() + ch.pbase[fi], f.ls_npart, f.d_lsr.as<GroupItem>(),
+                             int(f.ls_red.size()), f.d_lsri.as<int>(), C.d_gv.as<double2>(), gb, f.gstride, nc.ntime,
+                             stream_);
+          } else {
+            launch_trace_groups(src(ns.left), nc.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
+                                nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), f.d_gitems.as<GroupItem>(),
+                                int(f.gitems.size()), f.d_gpairs.as<int2>(), C.d_gv.as<double2>(), gb, f.gstride,
+                                nc.ntime, D_, rt, f32, stream_);
+          }
+        }
+        // subset sums of this family's trace vectors for the Moebius form
+        launch_zeta(C.d_gv.as<double2>(), ch.gbase[fi], f.gstride, C.d_gmasks.as<uint16_t>(), f.d_zitems.as<ZItem>(),
+                    int(f.zitems.size()), C.d_zmask.as<uint16_t>(), int(C.zmask.size()), C.d_zv.as<double2>(),
+                    ch.zbase[fi], f.zstride, S, stream_);
+      }
+      for (size_t c = 0; c < corrs.size(); ++c) {
+        int ncorr = int(corrs[c].reqs.size());
+        launch_resolve(C.d_gv.as<double2>(), C.d_gmasks.as<uint16_t>(), ch.d_jobs[c].as<CorrJob>(), ncorr, S,
+                       C.d_zv.as<double2>(), C.d_zmask.as<uint16_t>(), int(C.zmask.size()), C.d_cval.as<double2>(),
+                       per_element ? 1 : 0, stream_);
+        launch_accumulate(C.d_cval.as<double2>(), ch.d_tds.as<int>(), ch.d_tdg.as<int4>(), ch.ntdg, ncorr,
+                          d_sum[c].as<double2>(), stream_);
+      }
+    }
+    if (scope) {  // the caller reads the trace values of the plan
+      SLAPH_HIP(hipStreamSynchronize(stream_));
+      guard.done = true;
+      return;
+    }
+    // ---- results: correlator_[t][i].value() / Lt (Diagram.cpp:132-147),
+    // formed on the device as [correlator][t] and copied once ----
+    size_t res_total = 0;
+    for (auto const &cp : corrs) res_total += (size_t)Lt_ * cp.reqs.size();
+    C.d_res.ensure(sizeof(double2) * std::max<size_t>(1, res_total));
+    for (size_t c = 0, off = 0; c < corrs.size(); off += (size_t)Lt_ * corrs[c].reqs.size(), ++c)
+      launch_finalize(d_sum[c].as<double2>(), int(corrs[c].reqs.size()), Lt_, C.d_res.as<double2>() + off, stream_);
+    if (graph) {
+      capture.on = false;
+      SLAPH_HIP(hipStreamEndCapture(stream_, &C.graph));
+      SLAPH_HIP(hipGraphInstantiate(&C.gexec, C.graph, nullptr, nullptr, 0));
+      SLAPH_HIP(hipGraphLaunch(C.gexec, stream_));
     }
   }
-  if (scope) {  // the caller reads the trace values of the plan
-    SLAPH_HIP(hipStreamSynchronize(stream_));
-    guard.done = true;
-    return;
-  }
-  // ---- results: correlator_[t][i].value() / Lt (Diagram.cpp:132-147),
-  // formed on the device as [correlator][t] and copied once ----
   results_.resize(corrs.size());
-  size_t res_total = 0;
-  for (auto const &cp : corrs) res_total += (size_t)Lt_ * cp.reqs.size();
-  C.d_res.ensure(sizeof(double2) * std::max<size_t>(1, res_total));
-  for (size_t c = 0, off = 0; c < corrs.size(); off += (size_t)Lt_ * corrs[c].reqs.size(), ++c)
-    launch_finalize(d_sum[c].as<double2>(), int(corrs[c].reqs.size()), Lt_, C.d_res.as<double2>() + off, stream_);
   double kms = ctimer.stop();
   (void)kms;
-  {
+  if (ls_ev > 0) {
     KernelStat ks;
     for (size_t e = 0; e < ls_ev; e += 2) {
       float ms = 0;

@@ -166,6 +166,13 @@ class Engine {
     if (factors != prec_factors_) plan_.reset();
     prec_factors_ = factors;
   }
+  // HIP graphs for contract(): on (default), the second call of a plan
+  // captures the configuration's launch sequence and later calls replay it.
+  void set_graphs(bool on) {
+    graphs_ = on;
+    plan_.reset();
+  }
+  bool graphs() const { return graphs_; }
   StageTimes const &times() const { return times_; }
   KernelStat kernel_stat(std::string const &name) const {
     auto it = kstats_.find(name);
@@ -247,6 +254,7 @@ class Engine {
 
   size_t chunk_budget_ = 0;  // 0: from the free device memory
   int prec_vdv_ = 0, prec_factors_ = 0;
+  bool graphs_ = true;
   DevBuf q0f_, q1f_, q2f_;  // FP32 copies of the factor slabs (factors precision 1)
   struct FactorJobs;        // build_factors' job tables (engine.cpp)
   std::shared_ptr<FactorJobs> fjobs_;

@@ -1067,6 +1067,11 @@ __global__ void k_dagger(const double2 *__restrict__ in, double2 *__restrict__ o
   }
 }
 
+__global__ void k_zero2(double2 *p, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = make_double2(0.0, 0.0);
+}
+
 __global__ void k_identity(double2 *out, int N, int count) {
   const long long total = (long long)N * N * count;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
@@ -2392,6 +2397,12 @@ void launch_q2b(const Q2BJob *d_jobs, int njobs, int N, int dilE, long long prow
 void launch_dagger(const double2 *in, double2 *out, int N, int count, hipStream_t s) {
   if (count <= 0) return;
   hipLaunchKernelGGL(k_dagger, grid1d((long long)N * N * count, 256), dim3(256), 0, s, in, out, N, count);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_zero(double2 *p, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero2, grid1d(n, 256), dim3(256), 0, s, p, n);
   SLAPH_HIP(hipGetLastError());
 }
 

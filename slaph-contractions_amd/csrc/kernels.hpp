@@ -161,6 +161,8 @@ void launch_q2b(const Q2BJob *d_jobs, int njobs, int N, int dilE, long long pera
 
 void launch_dagger(const double2 *in, double2 *out, int N, int count, hipStream_t s);
 void launch_identity(double2 *out, int N, int count, hipStream_t s);
+// n zero complex numbers by a kernel (a fill that a HIP graph may hold)
+void launch_zero(double2 *p, long long n, hipStream_t s);
 
 // Storage forms of factor slabs and product tables (the arithmetic is FP64 in
 // every form): 0 = double2; 1 = float2 (FP32); 2 = d48c, each FP64 part cut to

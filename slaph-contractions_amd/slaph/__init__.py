@@ -50,6 +50,7 @@ _SIGS = {
     "slc_create_operators": (_c.c_int, [_P]),
     "slc_create_operators_config": (_c.c_int, [_P, _c.c_int]),
     "slc_set_timeslice_streaming": (_c.c_int, [_P, _c.c_int]),
+    "slc_set_graphs": (_c.c_int, [_P, _c.c_int]),
     "slc_set_gauge_links": (_c.c_int, [_P, _c.c_int, _D]),
     "slc_get_gauge_links": (_c.c_int, [_P, _c.c_int, _D]),
     "slc_synthetic_gauge": (_c.c_int, [_P, _c.c_ulonglong, _c.c_double]),
@@ -285,6 +286,10 @@ class Context:
         v = _c.c_double()
         self._check(_lib.slc_kernel_stat(self._h, kernel.encode(), what.encode(), _c.byref(v)))
         return v.value
+
+    def set_graphs(self, on: bool = True):
+        """HIP graph capture and replay of contract's launch sequence (default on)."""
+        self._check(_lib.slc_set_graphs(self._h, int(on)))
 
     def set_chunk_budget(self, nbytes: int):
         self._check(_lib.slc_set_chunk_budget(self._h, int(nbytes)))

@@ -488,6 +488,37 @@ def test_fp48_storage_form_within_1e_10(slaph, infile):
     assert np.all(_rel_errors(run("fp64"), ref) == 0.0)
 
 
+@pytest.mark.parametrize("kahan", [False, True])
+def test_graph_replays_match_direct_launches(slaph, kahan):
+    """contract() with HIP graphs: the second call of a plan captures the
+    launch sequence, later calls replay it.  Every replay -- with eager
+    VdaggerV work, host synchronisations and new inputs between them -- gives
+    correlators bit-identical to the directly launched contraction of a
+    context without graphs on the same inputs."""
+    say("graph replays", "kahan" if kahan else "")
+
+    def context(graphs):
+        ctx = slaph.Context(0)
+        ctx.configure(WIDE)
+        ctx.set_graphs(graphs)
+        return ctx
+
+    def run(ctx, seed):
+        ctx.synthetic_inputs(seed, True)
+        slaph.OperatorFactory(ctx).create_operators()
+        slaph.contract(ctx, kahan=kahan)
+        return {d: ctx.correlators(d) for d in ctx.diagrams()}
+
+    eager = context(False)
+    graphed = context(True)
+    for call, seed in enumerate((5, 5, 5, 6, 5, 7)):
+        want = run(eager, seed)
+        got = run(graphed, seed)
+        for d in want:
+            for name, v in want[d].items():
+                assert np.array_equal(got[d][name], v), (call, seed, d, name)
+
+
 def test_plan_reuse_is_deterministic(slaph):
     """The contraction plan is built on the first call and reused: a second
     call on new operators gives bit-identical correlators, a budget change
