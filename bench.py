@@ -203,6 +203,8 @@ def main():
                     help="skip timing the precision forms against the FP64 path")
     ap.add_argument("--chunk-gib", type=float, default=0.0, help="chunk memory budget (0: automatic)")
     ap.add_argument("--no-graphs", action="store_true", help="launch the contraction directly (no HIP graph replay)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default schedules (otherwise the autotuner picks them before the timed steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: there is no multi-GPU path "
@@ -235,6 +237,19 @@ def main():
     for i in range(args.warmup):
         step()
         print("# warmup %d done" % i, file=sys.stderr, flush=True)
+    tune = None
+    if not args.no_autotune:
+        # default schedules first (the "before"), then the autotuner's choice
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            step()
+        before = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        report = ctx.autotune(2)
+        tune = {"default_s_per_config": before, "chosen": report["chosen"],
+                "tuning_s": time.perf_counter() - t0, "knobs": report["knobs"]}
+        print("# autotuned in %.1f s: %s" % (tune["tuning_s"], report["chosen"]), file=sys.stderr, flush=True)
+        step()
     stage = {"vdaggerv": 0.0, "factors": 0.0, "contract": 0.0}
     stats = {LSTAT_KERNEL: [0.0] * 4, "k_vdv_ft": [0.0] * 4}
     t0 = time.perf_counter()
@@ -270,7 +285,7 @@ def main():
             "eigenvectors generated per timeslice, not orthonormalised" if w.get("streaming")
             else "orthonormalised eigenvectors"),
         "config": {"workload": _describe(args.workload, w), "precision": "fp64",
-                   "hip_graphs": not args.no_graphs},
+                   "hip_graphs": not args.no_graphs, "tuning": ctx.tuning()},
         "stages_ms": {"vdaggerv": stage["vdaggerv"], "factors": stage["factors"],
                       "contract_incl_factors": stage["contract"]},
     }
@@ -301,6 +316,8 @@ def main():
             line["roofline_vdaggerv"] = obj
         else:
             line["roofline"] = obj
+    if tune is not None:
+        line["autotune"] = tune
     if not args.no_precision_forms:
         pf = precision_forms(ctx, ops, ms / 1000.0, step)
         line["precision_forms"] = pf

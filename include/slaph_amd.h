@@ -233,6 +233,26 @@ int slc_set_chunk_budget(slc_ctx *ctx, long long bytes);
  * Correlator accumulation is FP64 (optionally Kahan) in every form. */
 int slc_set_precision(slc_ctx *ctx, int vdaggerv, int factors);
 
+/* Autotuner (no reference counterpart): times slc_contract (and, when the
+ * operators are built here, slc_create_operators for the VdaggerV form) on
+ * the context's current inputs for each candidate of each schedule / launch
+ * shape in turn -- the persistent grid of the left-stationary traces
+ * (workgroups per CU), the K split of the GEMM-form traces, MFMA or generic
+ * D = 16 products, Moebius or direct masked inner products, Fourier or
+ * direct VdaggerV -- `repeats` timed calls each, keeping a candidate only when
+ * it is faster by more than 0.5 %.  The choice is kept in the context for
+ * this input file (a later slc_configure with the same text reuses it); the
+ * correlators stay equal up to rounding.  Needs inputs and operators. */
+int slc_autotune(slc_ctx *ctx, int repeats);
+/* JSON report of the last slc_autotune (every candidate's time in ms and the
+ * choice); copies at most len bytes, returns the bytes needed incl. NUL. */
+int slc_autotune_report(const slc_ctx *ctx, char *buf, int len);
+/* The current choice as text "key=value ..." (slc_get_tuning, same copy
+ * convention), and setting it from such text (unknown keys are an error);
+ * lets a caller persist a tuning across processes. */
+int slc_get_tuning(const slc_ctx *ctx, char *buf, int len);
+int slc_set_tuning(slc_ctx *ctx, const char *text);
+
 /* HIP graphs for slc_contract (no reference counterpart; on by default):
  * with on != 0 the second slc_contract of a plan captures the launch
  * sequence of one configuration (every chunk's products and traces, the

@@ -12,6 +12,7 @@
 struct slc_ctx {
   std::unique_ptr<slaph::Engine> eng;
   std::string err;
+  std::string tune_report;  // JSON report of the last slc_autotune
 };
 
 namespace {
@@ -324,6 +325,31 @@ int slc_kernel_stat(const slc_ctx *cctx, const char *kernel, const char *what, d
     else if (w == "bytes") *value = ks.bytes;
     else if (w == "flops") *value = ks.flops;
     else throw std::out_of_range("unknown kernel statistic " + w);
+  });
+}
+
+int slc_autotune(slc_ctx *ctx, int repeats) {
+  return guarded(ctx, [&] { ctx->tune_report = ctx->eng->autotune(repeats); });
+}
+
+int slc_autotune_report(const slc_ctx *cctx, char *buf, int len) {
+  slc_ctx *ctx = const_cast<slc_ctx *>(cctx);
+  int need = 0;
+  const int rc = guarded(ctx, [&] { need = copy_string(ctx->tune_report, buf, len); });
+  return rc < 0 ? rc : need;
+}
+
+int slc_get_tuning(const slc_ctx *cctx, char *buf, int len) {
+  slc_ctx *ctx = const_cast<slc_ctx *>(cctx);
+  int need = 0;
+  const int rc = guarded(ctx, [&] { need = copy_string(ctx->eng->tuning_text(), buf, len); });
+  return rc < 0 ? rc : need;
+}
+
+int slc_set_tuning(slc_ctx *ctx, const char *text) {
+  return guarded(ctx, [&] {
+    if (!text) throw std::out_of_range("null tuning text");
+    ctx->eng->set_tuning(text);
   });
 }
 

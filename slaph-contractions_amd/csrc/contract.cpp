@@ -1381,7 +1381,8 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
           }
         }
       C.zmask.clear();
-      if (zs.size() > 1 && zs.size() <= 4096 && !std::getenv("SLAPH_RESOLVE_GROUPS") && !per_element)
+      if (zs.size() > 1 && zs.size() <= 4096 && !std::getenv("SLAPH_RESOLVE_GROUPS") && !tune_.resolve_groups &&
+          !per_element)
         C.zmask.assign(zs.begin(), zs.end());
       const long long nz = (long long)C.zmask.size();
       for (size_t fi = 0; fi < fams.size(); ++fi) {
@@ -1751,10 +1752,10 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
             launch_prod(src(ns.left), nc.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
                         nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), C.d_prod.as<void>(), nc.base, np.stride,
                         np.d_combos.as<ComboProd>(), int(np.combos.size()), nc.ntime, D_, np.transposed ? 1 : 0, f32,
-                        stream_);
+                        tune_.launch.prod_generic, stream_);
           } else if (nc.grid && !f.gg_tiles.empty()) {
             const int ntile = int(f.gg_tiles.size()), nrun = int(f.gg_runs.size() / 3);
-            const int kp = trace_gemm_parts(ntile, nrun);
+            const int kp = trace_gemm_parts(ntile, nrun, tune_.launch.gemm_wg_per_cu);
             if (kp > 1) f.d_ggp.ensure(sizeof(double2) * (size_t)ntile * kp * 4096);
             launch_trace_gemm(src(ns.left), src(ns.right), f.d_ggt.as<GridGTile>(), ntile, f.d_ggr.as<int>(), nrun,
                               nc.d_tile.as<GridTile>(), C.d_gv.as<double2>(), gb, f.gstride, D_, rt, f32, kp,
@@ -1782,7 +1783,7 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
               launch_trace_lstat_q(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
                                    f.d_lsu.as<int>(), int(f.ls_blocks.size()), f.d_lst.as<LsTask>(),
                                    f.d_lsd.as<long long>(), C.d_pv.as<double2>(), ch.pbase[fi], f.ls_npart, nc.ntime, D_,
-                                   rt, f32, f.d_lsq.as<int>(), stream_);
+                                   rt, f32, f.d_lsq.as<int>(), tune_.launch.lstat_wg_per_cu, stream_);
             else
               launch_trace_lstat(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
                                  f.d_lsb.as<LsBlock>(), int(f.ls_blocks.size()), f.d_lsc.as<LsChunk>(),

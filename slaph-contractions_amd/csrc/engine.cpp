@@ -88,6 +88,9 @@ void Engine::configure(std::string const &text) {
     throw std::runtime_error("HYP smearing of the gauge links is not implemented on this path (stout smearing is)");
   gd_ = std::move(gd);
   configured_ = true;
+  infile_ = text;
+  auto tuned = tuned_.find(text);
+  tune_ = tuned == tuned_.end() ? Tuning{} : tuned->second;
   K_ = 3LL * gd_.Lx * gd_.Ly * gd_.Lz;
   N_ = gd_.nev;
   D_ = 4 * gd_.dilE();
@@ -458,8 +461,8 @@ void Engine::create_operators(int config) {
   }
   if (!disp_ids.empty() && !have_gauge_) throw std::runtime_error("gauge links are not set (displaced operators)");
   const char *direct = std::getenv("SLAPH_VDV_DIRECT");
-  const bool fourier =
-      prec_vdv_ == 0 && !ids.empty() && gd_.Lz <= kFtMaxLz && !(direct && *direct && *direct != '0');
+  const bool fourier = prec_vdv_ == 0 && !ids.empty() && gd_.Lz <= kFtMaxLz && !tune_.vdv_direct &&
+                       !(direct && *direct && *direct != '0');
   EventTimer tm(stream_);
   int launches = 0;
   kstats_.erase("k_vdv_ft");
@@ -1091,6 +1094,117 @@ void Engine::fetch_correlators(std::string const &d, cplx *out) const {
       return;
     }
   throw std::runtime_error("no results for diagram " + d);
+}
+
+// ---------------------------------------------------------------------------
+// Tuning: text form and the autotuner
+// ---------------------------------------------------------------------------
+namespace {
+struct TuneKnob {
+  const char *name;
+  int Engine::Tuning::*field = nullptr;
+  int LaunchTuning::*lfield = nullptr;
+  std::vector<int> candidates;
+  bool vdaggerv = false;  // timed on create_operators instead of contract
+};
+std::vector<TuneKnob> tune_knobs() {
+  std::vector<TuneKnob> k(5);
+  k[0] = {"vdaggerv_direct", &Engine::Tuning::vdv_direct, nullptr, {0, 1}, true};
+  k[1] = {"lstat_wg_per_cu", nullptr, &LaunchTuning::lstat_wg_per_cu, {2, 1, 3}, false};
+  k[2] = {"gemm_wg_per_cu", nullptr, &LaunchTuning::gemm_wg_per_cu, {16, 8, 32}, false};
+  k[3] = {"prod_generic", nullptr, &LaunchTuning::prod_generic, {0, 1}, false};
+  k[4] = {"resolve_groups", &Engine::Tuning::resolve_groups, nullptr, {0, 1}, false};
+  return k;
+}
+int &knob_ref(Engine::Tuning &t, TuneKnob const &k) { return k.field ? t.*(k.field) : t.launch.*(k.lfield); }
+}  // namespace
+
+std::string Engine::tuning_text() const {
+  std::string out;
+  Tuning t = tune_;
+  for (auto const &k : tune_knobs()) {
+    if (!out.empty()) out += ' ';
+    out += std::string(k.name) + "=" + std::to_string(knob_ref(t, k));
+  }
+  return out;
+}
+
+void Engine::set_tuning(std::string const &text) {
+  Tuning t = tune_;
+  auto knobs = tune_knobs();
+  size_t pos = 0;
+  while (pos < text.size()) {
+    while (pos < text.size() && (text[pos] == ' ' || text[pos] == '\t' || text[pos] == '\n')) ++pos;
+    if (pos >= text.size()) break;
+    size_t end = text.find_first_of(" \t\n", pos);
+    if (end == std::string::npos) end = text.size();
+    const std::string item = text.substr(pos, end - pos);
+    pos = end;
+    const size_t eq = item.find('=');
+    if (eq == std::string::npos) throw std::runtime_error("tuning: expected key=value, got '" + item + "'");
+    const std::string key = item.substr(0, eq), val = item.substr(eq + 1);
+    auto it = std::find_if(knobs.begin(), knobs.end(), [&](TuneKnob const &k) { return key == k.name; });
+    if (it == knobs.end()) throw std::runtime_error("tuning: unknown key '" + key + "'");
+    char *stop = nullptr;
+    const long v = std::strtol(val.c_str(), &stop, 10);
+    if (val.empty() || *stop != '\0' || v < 0 || v > 64) throw std::runtime_error("tuning: bad value for " + key);
+    knob_ref(t, *it) = int(v);
+  }
+  tune_ = t;
+  if (configured_) tuned_[infile_] = t;
+  plan_.reset();
+  ft_.reset();
+}
+
+std::string Engine::autotune(int repeats) {
+  require_inputs();
+  if (!have_ops_) throw std::runtime_error("autotune: create the operators first (the contraction is timed on them)");
+  if (repeats < 1 || repeats > 100) throw std::out_of_range("autotune: repeats must be in 1 .. 100");
+  auto timed = [&](bool vdv) {
+    double best = 1e300;
+    // untimed first call(s): the plan (and the graph capture) of this choice
+    for (int i = 0; i < (vdv ? 1 : 2); ++i) vdv ? create_operators(config_) : contract(-1, 0);
+    for (int r = 0; r < repeats; ++r) {
+      auto t0 = std::chrono::steady_clock::now();
+      vdv ? create_operators(config_) : contract(-1, 0);
+      best = std::min(best, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    return best;
+  };
+  std::string report = "{\"knobs\": [";
+  Tuning best = tune_;
+  bool first_knob = true;
+  for (auto const &k : tune_knobs()) {
+    // the VdaggerV form only matters when the operators are computed here
+    if (k.vdaggerv && gd_.handling_vdaggerv != "build" && gd_.handling_vdaggerv != "write") continue;
+    double best_ms = -1.0;
+    int best_v = knob_ref(best, k);
+    std::string meas;
+    for (int v : k.candidates) {
+      tune_ = best;
+      knob_ref(tune_, k) = v;
+      plan_.reset();
+      ft_.reset();
+      const double ms = timed(k.vdaggerv);
+      if (!meas.empty()) meas += ", ";
+      meas += "\"" + std::to_string(v) + "\": " + std::to_string(ms);
+      if (best_ms < 0 || ms < best_ms * 0.995) {
+        best_ms = ms;
+        best_v = v;
+      }
+    }
+    knob_ref(best, k) = best_v;
+    report += std::string(first_knob ? "" : ", ") + "{\"knob\": \"" + k.name + "\", \"ms\": {" + meas +
+              "}, \"chosen\": " + std::to_string(best_v) + (k.vdaggerv ? ", \"stage\": \"vdaggerv\"" : "") + "}";
+    first_knob = false;
+  }
+  tune_ = best;
+  tuned_[infile_] = best;
+  plan_.reset();
+  ft_.reset();
+  create_operators(config_);  // operators of the chosen form
+  report += "], \"chosen\": \"" + tuning_text() + "\"}";
+  return report;
 }
 
 }  // namespace slaph

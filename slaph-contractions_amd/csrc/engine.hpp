@@ -166,6 +166,25 @@ class Engine {
     if (factors != prec_factors_) plan_.reset();
     prec_factors_ = factors;
   }
+  // Schedules and launch shapes (autotune() picks them by timing the
+  // current workload; set_tuning / tuning() read and write them as text
+  // "key=value ...").  Every choice computes the same correlators up to
+  // rounding (the orders of some sums differ).
+  struct Tuning {
+    LaunchTuning launch;
+    int vdv_direct = 0;      // VdaggerV by the direct product instead of the sitewise Fourier form
+    int resolve_groups = 0;  // masked inner products by direct group loops instead of the Moebius form
+  };
+  Tuning const &tuning() const { return tune_; }
+  std::string tuning_text() const;
+  void set_tuning(std::string const &text);
+  // Times contract() (and create_operators() for the VdaggerV form) on the
+  // current inputs for each candidate of each knob in turn, keeping a
+  // candidate only if it is faster by more than 0.5 %; `repeats` timed calls
+  // per candidate (the fastest counts).  The choice is kept in the context
+  // for this input file (configure() with the same text reuses it).
+  // Returns a JSON report of every measurement.
+  std::string autotune(int repeats);
   // HIP graphs for contract(): on (default), the second call of a plan
   // captures the configuration's launch sequence and later calls replay it.
   void set_graphs(bool on) {
@@ -255,6 +274,9 @@ class Engine {
   size_t chunk_budget_ = 0;  // 0: from the free device memory
   int prec_vdv_ = 0, prec_factors_ = 0;
   bool graphs_ = true;
+  Tuning tune_;
+  std::map<std::string, Tuning> tuned_;  // input file text -> autotuned choice
+  std::string infile_;                   // text of the last configure()
   DevBuf q0f_, q1f_, q2f_;  // FP32 copies of the factor slabs (factors precision 1)
   struct FactorJobs;        // build_factors' job tables (engine.cpp)
   std::shared_ptr<FactorJobs> fjobs_;

@@ -2416,10 +2416,10 @@ namespace {
 template <typename T>
 void launch_prod_t(const void *sX, const long long *Xt, const long long *Xq, const void *sY, const long long *Yt,
                    const long long *Yq, void *out, long long obase, long long ostride, const ComboProd *combos,
-                   int ncombo, int ntime, int D, int transposed, hipStream_t s) {
+                   int ncombo, int ntime, int D, int transposed, int generic, hipStream_t s) {
   const T *x = static_cast<const T *>(sX), *y = static_cast<const T *>(sY);
   T *o = static_cast<T *>(out);
-  if (D == 16 && !std::getenv("SLAPH_PROD_GENERIC")) {
+  if (D == 16 && !generic && !std::getenv("SLAPH_PROD_GENERIC")) {
     const int per_block = 4 * kProdPerWave;
     const dim3 grid((ncombo + per_block - 1) / per_block, ntime);
     if ((long long)grid.x * grid.y > 0x7fffffffLL) throw std::runtime_error("launch_prod: grid too large");
@@ -2466,21 +2466,21 @@ void launch_trace_lstat_t(const void *sX, const long long *Xt, const void *sY, c
 template <int D, typename T>
 void launch_lstat_q_d(const T *x, const long long *Xt, const T *y, const long long *Yt, const int *units, int nunit,
                       const LsTask *tasks, const long long *slot_dst, double2 *out, long long obase, long long ostride,
-                      int ntime, int rt, int *qcount, hipStream_t s) {
-  // Two workgroups per compute unit (fewer if fewer fit), at most one per
-  // unit of work.  With two the units in flight on one XCD span few
+                      int ntime, int rt, int *qcount, int wg_per_cu, hipStream_t s) {
+  // wg_per_cu workgroups per compute unit (fewer if fewer fit), at most one
+  // per unit of work.  With two the units in flight on one XCD span few
   // component-pieces and more of their right matrices stay in its L2: on the
   // 32^3 x 64 workload (ring of 4 at three waves per SIMD) 512 workgroups ran
   // the launch in 14.5 ms, 768 in 16.6 ms, 384 in 16.5 ms.  At two waves per
   // SIMD the registers allow a ring of 6: 14.2 ms (8: 14.8 ms).
-  static int slots = 0;
-  if (slots == 0) {
-    int dev = 0, ncu = 0, per_cu = 0;
+  static int ncu = 0, per_cu = 0;
+  if (ncu == 0) {
+    int dev = 0;
     SLAPH_HIP(hipGetDevice(&dev));
     SLAPH_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     SLAPH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_lstat_q<D, T>, 64 * kLsWaves, 0));
-    slots = std::max(1, ncu * std::max(1, std::min(per_cu, 2)));
   }
+  const int slots = std::max(1, ncu * std::max(1, std::min(per_cu, std::max(1, wg_per_cu))));
   const long long total = (long long)nunit * ntime;
   const unsigned grid = unsigned(std::min<long long>(total, slots));
   // descriptor window: the largest whole number of rounds that keeps a
@@ -2498,20 +2498,25 @@ void launch_lstat_q_d(const T *x, const long long *Xt, const T *y, const long lo
 template <typename T>
 void launch_trace_lstat_q_t(const void *sX, const long long *Xt, const void *sY, const long long *Yt, const int *units,
                             int nunit, const LsTask *tasks, const long long *slot_dst, double2 *out, long long obase,
-                            long long ostride, int ntime, int D, int rt, int *qcount, hipStream_t s) {
+                            long long ostride, int ntime, int D, int rt, int *qcount, int wg_per_cu,
+                            hipStream_t s) {
   const T *x = static_cast<const T *>(sX), *y = static_cast<const T *>(sY);
   switch (D) {
     case 8:
-      launch_lstat_q_d<8>(x, Xt, y, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, rt, qcount, s);
+      launch_lstat_q_d<8>(x, Xt, y, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, rt, qcount,
+                          wg_per_cu, s);
       break;
     case 16:
-      launch_lstat_q_d<16>(x, Xt, y, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, rt, qcount, s);
+      launch_lstat_q_d<16>(x, Xt, y, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, rt, qcount,
+                          wg_per_cu, s);
       break;
     case 24:
-      launch_lstat_q_d<24>(x, Xt, y, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, rt, qcount, s);
+      launch_lstat_q_d<24>(x, Xt, y, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, rt, qcount,
+                          wg_per_cu, s);
       break;
     case 32:
-      launch_lstat_q_d<32>(x, Xt, y, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, rt, qcount, s);
+      launch_lstat_q_d<32>(x, Xt, y, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime, rt, qcount,
+                          wg_per_cu, s);
       break;
     default:
       throw std::runtime_error("launch_trace_lstat_q: unsupported factor dimension");
@@ -2521,24 +2526,26 @@ void launch_trace_lstat_q_t(const void *sX, const long long *Xt, const void *sY,
 
 void launch_trace_lstat_q(const void *sX, const long long *Xt, const void *sY, const long long *Yt, const int *units,
                           int nunit, const LsTask *tasks, const long long *slot_dst, double2 *out, long long obase,
-                          long long ostride, int ntime, int D, int rt, int f32, int *qcount, hipStream_t s) {
+                          long long ostride, int ntime, int D, int rt, int f32, int *qcount, int wg_per_cu,
+                          hipStream_t s) {
   if (nunit <= 0 || ntime <= 0) return;
   if ((long long)nunit * ntime > 0x7fffffffLL) throw std::runtime_error("launch_trace_lstat_q: too many units");
   with_storage(f32, [&](auto tag) {
     launch_trace_lstat_q_t<decltype(tag)>(sX, Xt, sY, Yt, units, nunit, tasks, slot_dst, out, obase, ostride, ntime,
-                                          D, rt, qcount, s);
+                                          D, rt, qcount, wg_per_cu, s);
   });
   SLAPH_HIP(hipGetLastError());
 }
 
 void launch_prod(const void *sX, const long long *Xt, const long long *Xq, const void *sY, const long long *Yt,
                  const long long *Yq, void *out, long long obase, long long ostride, const ComboProd *combos,
-                 int ncombo, int ntime, int D, int transposed, int f32, hipStream_t s) {
+                 int ncombo, int ntime, int D, int transposed, int f32, int generic, hipStream_t s) {
   if (ncombo <= 0 || ntime <= 0) return;
   if (D > kMaxD) throw std::runtime_error("factor dimension above 32 is not supported");
   if (ntime > 65535) throw std::runtime_error("launch_prod: too many time keys");
   with_storage(f32, [&](auto tag) {
-    launch_prod_t<decltype(tag)>(sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos, ncombo, ntime, D, transposed, s);
+    launch_prod_t<decltype(tag)>(sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos, ncombo, ntime, D, transposed,
+                                 generic, s);
   });
   SLAPH_HIP(hipGetLastError());
 }
@@ -2614,10 +2621,10 @@ void launch_trace_groups(const void *sX, const long long *Xt, const long long *X
   SLAPH_HIP(hipGetLastError());
 }
 
-int trace_gemm_parts(int ntiles, int nrun) {
-  // At least 16 workgroups per compute unit, raised to a part count that
-  // divides the runs evenly when one is near (parts of unequal length leave
-  // workgroups waiting on the longest), at most 32 parts.
+int trace_gemm_parts(int ntiles, int nrun, int wg_per_cu) {
+  // At least wg_per_cu workgroups per compute unit, raised to a part count
+  // that divides the runs evenly when one is near (parts of unequal length
+  // leave workgroups waiting on the longest), at most 32 parts.
   static int ncu = 0;
   if (ncu == 0) {
     int dev = 0;
@@ -2625,7 +2632,7 @@ int trace_gemm_parts(int ntiles, int nrun) {
     SLAPH_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   }
   if (ntiles <= 0 || nrun <= 0) return 1;
-  const int want = std::max(1, std::min({(16 * ncu + ntiles - 1) / ntiles, 32, nrun}));
+  const int want = std::max(1, std::min({(std::max(1, wg_per_cu) * ncu + ntiles - 1) / ntiles, 32, nrun}));
   for (int kp = want; kp <= std::min({2 * want, 32, nrun}); ++kp)
     if (nrun % kp == 0) return kp;
   return want;

@@ -186,9 +186,11 @@ struct ComboProd {
   int qn, i, j;
   long long o;
 };
+// generic != 0 (or SLAPH_PROD_GENERIC set): D = 16 products by the generic
+// LDS kernel instead of the MFMA tiles
 void launch_prod(const void *slabX, const long long *Xt, const long long *Xq, const void *slabY, const long long *Yt,
                  const long long *Yq, void *out, long long obase, long long ostride, const ComboProd *combos,
-                 int ncombo, int ntime, int D, int transposed, int f32, hipStream_t s);
+                 int ncombo, int ntime, int D, int transposed, int f32, int generic, hipStream_t s);
 // In the trace launchers below, rt != 0 means the right matrices are stored
 // transposed (Y^T), so that tr(X Y) = sum_e X[e] Y^T[e]; the operand slabs
 // are double2 or (f32 != 0) float2 arrays, the group sums always double2.
@@ -284,7 +286,7 @@ struct GridGTile {
 // K (the runs) is split into kparts (trace_gemm_parts) parts; with more than
 // one, part holds ntiles * kparts * 4096 double2 partial tiles that a second
 // kernel sums in part order.
-int trace_gemm_parts(int ntiles, int nrun);
+int trace_gemm_parts(int ntiles, int nrun, int wg_per_cu);
 void launch_trace_gemm(const void *slabX, const void *slabY, const GridGTile *tiles, int ntiles, const int *runs,
                        int nrun, const GridTile *times, double2 *out, long long obase, long long ostride, int D, int rt,
                        int f32, int kparts, double2 *part, hipStream_t s);
@@ -331,10 +333,20 @@ void launch_trace_lstat(const void *slabX, const long long *Xt, const void *slab
 constexpr int kLsUnitHdr = 12;
 int trace_lstat_qchunk(int D);
 int trace_lstat_qring(int D);
+// wg_per_cu: workgroups per compute unit of the persistent grid (at most
+// what fits)
 void launch_trace_lstat_q(const void *slabX, const long long *Xt, const void *slabY, const long long *Yt,
                           const int *units, int nunit, const LsTask *tasks, const long long *slot_dst, double2 *out,
                           long long obase, long long ostride, int ntime, int D, int rt, int f32, int *qcount,
-                          hipStream_t s);
+                          int wg_per_cu, hipStream_t s);
+
+// Launch shapes chosen by the autotuner (Engine::autotune); the defaults
+// are the values measured best on the 32^3 x 64 workload.
+struct LaunchTuning {
+  int lstat_wg_per_cu = 2;  // persistent grid of the queued left-stationary traces
+  int gemm_wg_per_cu = 16;  // workgroups per CU the K split of the GEMM-form traces aims at
+  int prod_generic = 0;     // D = 16 products by the generic kernel instead of MFMA tiles
+};
 // Sums of partial slots: out[obase + ti*ostride + items[k].dst] =
 // sum_{p in pidx[items[k].list_off .. + n)} part[ti*pstride + p], in list order.
 void launch_ls_reduce(const double2 *part, long long pstride, const GroupItem *items, int nitems, const int *pidx,

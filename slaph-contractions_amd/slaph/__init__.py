@@ -51,6 +51,10 @@ _SIGS = {
     "slc_create_operators_config": (_c.c_int, [_P, _c.c_int]),
     "slc_set_timeslice_streaming": (_c.c_int, [_P, _c.c_int]),
     "slc_set_graphs": (_c.c_int, [_P, _c.c_int]),
+    "slc_autotune": (_c.c_int, [_P, _c.c_int]),
+    "slc_autotune_report": (_c.c_int, [_P, _c.c_char_p, _c.c_int]),
+    "slc_get_tuning": (_c.c_int, [_P, _c.c_char_p, _c.c_int]),
+    "slc_set_tuning": (_c.c_int, [_P, _c.c_char_p]),
     "slc_set_gauge_links": (_c.c_int, [_P, _c.c_int, _D]),
     "slc_get_gauge_links": (_c.c_int, [_P, _c.c_int, _D]),
     "slc_synthetic_gauge": (_c.c_int, [_P, _c.c_ulonglong, _c.c_double]),
@@ -286,6 +290,25 @@ class Context:
         v = _c.c_double()
         self._check(_lib.slc_kernel_stat(self._h, kernel.encode(), what.encode(), _c.byref(v)))
         return v.value
+
+    def _string(self, fn):
+        need = self._check(fn(self._h, None, 0))
+        buf = _c.create_string_buffer(max(1, need))
+        self._check(fn(self._h, buf, need))
+        return buf.value.decode()
+
+    def autotune(self, repeats: int = 2) -> dict:
+        """Time the schedule / launch-shape candidates on the current inputs
+        and keep the fastest; returns the report (every candidate's ms)."""
+        import json
+        self._check(_lib.slc_autotune(self._h, int(repeats)))
+        return json.loads(self._string(_lib.slc_autotune_report))
+
+    def tuning(self) -> str:
+        return self._string(_lib.slc_get_tuning)
+
+    def set_tuning(self, text: str):
+        self._check(_lib.slc_set_tuning(self._h, text.encode()))
 
     def set_graphs(self, on: bool = True):
         """HIP graph capture and replay of contract's launch sequence (default on)."""

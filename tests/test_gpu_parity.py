@@ -519,6 +519,46 @@ def test_graph_replays_match_direct_launches(slaph, kahan):
                 assert np.array_equal(got[d][name], v), (call, seed, d, name)
 
 
+def test_autotune_keeps_correlators_and_tuning_text(slaph):
+    """The autotuner times every candidate on the current inputs; whatever it
+    picks (and every single non-default choice set by text) gives the
+    correlators of the default schedule within 1e-12 norm-wise.  The choice
+    is kept for the input file across configure(); bad tuning text raises."""
+    say("autotune")
+    ctx = slaph.Context(0)
+    ctx.configure(WIDE)
+    ctx.synthetic_inputs(21, True)
+    ops = slaph.OperatorFactory(ctx)
+    ops.create_operators()
+    slaph.contract(ctx)
+    ref = {d: ctx.correlators(d) for d in ctx.diagrams()}
+    default = ctx.tuning()
+    assert "lstat_wg_per_cu=2" in default and "vdaggerv_direct=0" in default
+    rep = ctx.autotune(1)
+    knobs = {k["knob"] for k in rep["knobs"]}
+    assert knobs == {"vdaggerv_direct", "lstat_wg_per_cu", "gemm_wg_per_cu", "prod_generic", "resolve_groups"}
+    assert all(len(k["ms"]) >= 2 and all(v > 0 for v in k["ms"].values()) for k in rep["knobs"])
+    assert rep["chosen"] == ctx.tuning()
+    slaph.contract(ctx)
+    assert _rel_errors({d: ctx.correlators(d) for d in ctx.diagrams()}, ref).max() <= 1e-12
+    for text in ("vdaggerv_direct=1", "lstat_wg_per_cu=1", "lstat_wg_per_cu=3", "gemm_wg_per_cu=8",
+                 "prod_generic=1", "resolve_groups=1"):
+        ctx.set_tuning(default)
+        ctx.set_tuning(text)
+        ops.create_operators()
+        slaph.contract(ctx)
+        err = _rel_errors({d: ctx.correlators(d) for d in ctx.diagrams()}, ref).max()
+        assert err <= 1e-12, (text, err)
+    ctx.set_tuning("gemm_wg_per_cu=32")
+    kept = ctx.tuning()
+    ctx.configure(WIDE)
+    assert ctx.tuning() == kept
+    with pytest.raises(slaph.SlaphError, match="unknown key"):
+        ctx.set_tuning("no_such_knob=1")
+    with pytest.raises(slaph.SlaphError, match="bad value"):
+        ctx.set_tuning("lstat_wg_per_cu=x")
+
+
 def test_plan_reuse_is_deterministic(slaph):
     """The contraction plan is built on the first call and reused: a second
     call on new operators gives bit-identical correlators, a budget change
