@@ -170,27 +170,34 @@ __global__ void k_stout_step(const double2 *__restrict__ Uin, double2 *__restric
   m_store(Uout + ((long long)site * 3 + mu) * 9, m_mul(m_exp(m_scale(q, -1.0)), u));
 }
 
-__global__ void k_displace(const double2 *__restrict__ U, const double2 *__restrict__ Vin, double2 *__restrict__ Vout,
-                           int Lx, int Ly, int Lz, int N, int mu, int forward) {
+// One thread per (site, group of kDispN eigenvectors): the link matrix is
+// loaded once for the group; the 64 threads of a wave take consecutive sites
+// of one eigenvector, so their loads and stores of the 3-colour vectors are
+// contiguous.
+constexpr int kDispN = 16;
+
+__global__ __launch_bounds__(256) void k_displace(const double2 *__restrict__ U, const double2 *__restrict__ Vin,
+                                                  double2 *__restrict__ Vout, int Lx, int Ly, int Lz, int N, int mu,
+                                                  int forward) {
   const int V3 = Lx * Ly * Lz;
-  const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (id >= (long long)V3 * N) return;
-  const int site = int(id % V3), n = int(id / V3);
+  const int site = blockIdx.x * blockDim.x + threadIdx.x;
+  if (site >= V3) return;
   const long long K = 3LL * V3;
   const int other = nbr(site, mu, forward ? 1 : -1, Lx, Ly, Lz);
   // forward: U_mu(x) v(x + mu); backward: U_mu(x - mu)^+ v(x - mu)
   const double2 *u = U + ((long long)(forward ? site : other) * 3 + mu) * 9;
-  const double2 *v = Vin + n * K + 3LL * other;
-  const double2 v0 = v[0], v1 = v[1], v2 = v[2];
-  double2 *o = Vout + n * K + 3LL * site;
+  double2 m[9];  // the applied 3 x 3 matrix, column-major
 #pragma unroll
-  for (int r = 0; r < 3; ++r) {
-    double2 s;
-    if (forward)
-      s = c_add(c_add(c_mul(u[r], v0), c_mul(u[3 + r], v1)), c_mul(u[6 + r], v2));
-    else  // (U^+)(r, c) = conj(U(c, r))
-      s = c_add(c_add(c_mul(c_conj(u[r * 3]), v0), c_mul(c_conj(u[r * 3 + 1]), v1)), c_mul(c_conj(u[r * 3 + 2]), v2));
-    o[r] = s;
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m[c * 3 + r] = forward ? u[c * 3 + r] : c_conj(u[r * 3 + c]);
+  const int n0 = blockIdx.y * kDispN, n1 = min(N, n0 + kDispN);
+  for (int n = n0; n < n1; ++n) {
+    const double2 *v = Vin + n * K + 3LL * other;
+    const double2 v0 = v[0], v1 = v[1], v2 = v[2];
+    double2 *o = Vout + n * K + 3LL * site;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) o[r] = c_add(c_add(c_mul(m[r], v0), c_mul(m[3 + r], v1)), c_mul(m[6 + r], v2));
   }
 }
 
@@ -219,8 +226,11 @@ void launch_displace(const double2 *U, const double2 *Vin, double2 *Vout, int Lx
   if (n <= 0) return;
   if (mu < 0 || mu > 2) throw std::runtime_error("launch_displace: direction must be 0, 1 or 2");
   if (Vin == Vout) throw std::runtime_error("launch_displace: the displacement needs separate input and output");
-  if ((n + 255) / 256 > 0x7fffffffLL) throw std::runtime_error("launch_displace: grid too large");
-  hipLaunchKernelGGL(k_displace, blocks_for(n), dim3(256), 0, s, U, Vin, Vout, Lx, Ly, Lz, N, mu, forward);
+  const long long V3 = (long long)Lx * Ly * Lz;
+  if ((V3 + 255) / 256 > 0x7fffffffLL || (N + kDispN - 1) / kDispN > 65535)
+    throw std::runtime_error("launch_displace: grid too large");
+  hipLaunchKernelGGL(k_displace, dim3(unsigned((V3 + 255) / 256), unsigned((N + kDispN - 1) / kDispN)), dim3(256), 0, s,
+                     U, Vin, Vout, Lx, Ly, Lz, N, mu, forward);
   SLAPH_HIP(hipGetLastError());
 }
 
