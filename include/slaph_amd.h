@@ -107,8 +107,19 @@ int slc_synthetic_gauge(slc_ctx *ctx, unsigned long long seed, double eps);
 /* `iterations` steps of GaugeField::smearing_stout (GaugeField.cpp:228-274)
  * on every timeslice: three-dimensional staples, Omega = (rho / 4) C U^+,
  * U' = exp(-q) U with q the traceless anti-Hermitian part of Omega^+ - Omega
- * over 2 (the reference's factor 1/4 on rho kept). */
+ * over 2 (the reference's factor 1/4 on rho kept).  The negative staple is
+ * U_nu(x-nu)^+ U_mu(x-nu) U_nu(x-nu+mu), the gauge-covariant form; the
+ * reference's smearing_stout multiplies U_mu(x-nu+mu) there (lines 255-256),
+ * which breaks gauge covariance (its HYP staples, lines 358-359, use U_nu). */
 int slc_smear_stout(slc_ctx *ctx, double rho, int iterations);
+/* `iterations` HYP steps of every timeslice's links with the reference's
+ * parameters (GaugeField::smearing_hyp, src/GaugeField.cpp:322-431: alpha2 the
+ * decorated level, alpha1 the outer level, each level projected back to SU(3)
+ * by proj_to_su3_imp).  slc_create_operators applies the input file's
+ * HypSmearing parameters to a copy of the links by itself
+ * (OperatorFactory.cpp:249-260), so this entry is for callers that smear
+ * links they keep. */
+int slc_smear_hyp(slc_ctx *ctx, double alpha1, double alpha2, int iterations);
 
 /* Copy one resident input back to the host in the layouts above:
  * kind 0 = eigenvectors of timeslice `index`, 1 = perambulator `index`,

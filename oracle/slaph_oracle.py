@@ -26,8 +26,10 @@ Pins (each checked in tests/test_oracle.py):
                        tests/test_oracle_gauge.py (unit links -> lattice shift,
                        forward then backward = identity, gauge invariance of
                        displaced VdaggerV, gauge covariance / SU(3) / rho = 0 of
-                       the stout step, the diagonal-link phase update); the
-                       reference's tests hold no displaced data
+                       the stout step, the diagonal-link phase update; HYP:
+                       alpha = 0 identity, SU(3), gauge covariance, the
+                       diagonal-link closed form); the reference's tests hold
+                       no displaced data
 """
 from __future__ import annotations
 
@@ -592,6 +594,50 @@ def displace(V, links, disp, L):
         else:
             raise RuntimeError("Something wrong with the displacement in the operator definition!")
     return W
+
+
+def proj_su3(m):
+    """proj_to_su3_imp (GaugeField.cpp:203-210) for a stack of 3 x 3
+    matrices: the unitary polar factor A (A^+ A)^(-1/2) (from the SVD,
+    A = X S Y^+ -> X Y^+) times det^(-1/3) with the principal cube root."""
+    x, _, yh = np.linalg.svd(m)
+    w = x @ yh
+    d = np.linalg.det(w)
+    return w / (d ** (1.0 / 3.0))[..., None, None]
+
+
+def hyp_step(links, alpha1, alpha2, L):
+    """One step of GaugeField::smearing_hyp (GaugeField.cpp:322-431): links
+    decorated in each of the two other directions with alpha2 (slot mu for
+    the first other direction, mu + 3 for the second: the decor map,
+    :167-187), then the outer level with the links decorated in the third
+    direction and alpha1 / 4, each level projected by proj_su3."""
+    adj = lambda m: m.conj().swapaxes(-1, -2)  # noqa: E731
+    nsite = links.shape[0]
+    others = {0: (1, 2), 1: (0, 2), 2: (0, 1)}
+    dec = np.empty((nsite, 6, 3, 3), complex)
+    for mu in range(3):
+        xm = neighbour(L, mu, 1)
+        for k, rho in enumerate(others[mu]):
+            xr, xd = neighbour(L, rho, 1), neighbour(L, rho, -1)
+            st = links[:, rho] @ links[xr, mu] @ adj(links[xm, rho])
+            st = st + adj(links[xd, rho]) @ links[xd, mu] @ links[xm[xd], rho]
+            dec[:, mu + 3 * k] = proj_su3((1.0 - alpha2) * links[:, mu] + alpha2 / 2.0 * st)
+    slot = lambda d, p: d if p == others[d][0] else d + 3  # noqa: E731
+    out = np.empty_like(links)
+    for mu in range(3):
+        xm = neighbour(L, mu, 1)
+        st = np.zeros((nsite, 3, 3), complex)
+        for nu in range(3):
+            if nu == mu:
+                continue
+            plane = 3 - mu - nu
+            a, b = slot(nu, plane), slot(mu, plane)
+            xn, xd = neighbour(L, nu, 1), neighbour(L, nu, -1)
+            st = st + dec[:, a] @ dec[xn, b] @ adj(dec[xm, a])
+            st = st + adj(dec[xd, a]) @ dec[xd, b] @ dec[xm[xd], a]
+        out[:, mu] = proj_su3((1.0 - alpha1) * links[:, mu] + alpha1 / 4.0 * st)
+    return out
 
 
 def stout_step(links, rho, L):

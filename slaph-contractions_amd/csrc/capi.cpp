@@ -182,6 +182,10 @@ int slc_smear_stout(slc_ctx *ctx, double rho, int iterations) {
   return guarded(ctx, [&] { ctx->eng->smear_stout(rho, iterations); });
 }
 
+int slc_smear_hyp(slc_ctx *ctx, double alpha1, double alpha2, int iterations) {
+  return guarded(ctx, [&] { ctx->eng->smear_hyp(alpha1, alpha2, iterations); });
+}
+
 int slc_create_operators(slc_ctx *ctx) {
   return guarded(ctx, [&] { ctx->eng->create_operators(ctx->eng->last_config()); });
 }

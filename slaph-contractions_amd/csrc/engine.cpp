@@ -84,8 +84,6 @@ void Engine::configure(std::string const &text) {
     if (q.dilT != gd.quarks[0].dilT || q.dilE != gd.quarks[0].dilE || q.dilD != 4)
       throw std::runtime_error("all quarks must share the dilution scheme of quark 0 with full Dirac dilution");
   if (gd.Lt % gd.quarks[0].dilT != 0) throw std::runtime_error("Lt must be a multiple of the time dilution block");
-  if (gd.need_gaugefield && gd.hyp_iterations > 0)
-    throw std::runtime_error("HYP smearing of the gauge links is not implemented on this path (stout smearing is)");
   gd_ = std::move(gd);
   configured_ = true;
   infile_ = text;
@@ -187,6 +185,37 @@ void Engine::smear_stout(double rho, int iterations) {
     }
   }
   SLAPH_HIP(hipStreamSynchronize(stream_));
+  have_ops_ = false;
+}
+
+// HYP steps of every timeslice of `in` into `out` (in == out allowed): per
+// step the decorated links of a timeslice into hyp_dec_, then the outer level
+// into hyp_tmp_ and back.
+void Engine::hyp_links(const double2 *in, double2 *out, double alpha1, double alpha2, int iterations) {
+  const size_t n = 27 * (size_t)(K_ / 3);
+  hyp_dec_.ensure(sizeof(double2) * 2 * n);
+  hyp_tmp_.ensure(sizeof(double2) * n);
+  for (int t = 0; t < Lt_; ++t) {
+    const double2 *src = in + (size_t)t * n;
+    double2 *dst = out + (size_t)t * n;
+    if (iterations == 0 && src != dst)
+      SLAPH_HIP(hipMemcpyAsync(dst, src, sizeof(double2) * n, hipMemcpyDeviceToDevice, stream_));
+    for (int i = 0; i < iterations; ++i) {
+      launch_hyp_decorate(i == 0 ? src : dst, hyp_dec_.as<double2>(), gd_.Lx, gd_.Ly, gd_.Lz, alpha2, stream_);
+      launch_hyp_outer(i == 0 ? src : dst, hyp_dec_.as<double2>(), hyp_tmp_.as<double2>(), gd_.Lx, gd_.Ly, gd_.Lz,
+                       alpha1, stream_);
+      SLAPH_HIP(hipMemcpyAsync(dst, hyp_tmp_.as<void>(), sizeof(double2) * n, hipMemcpyDeviceToDevice, stream_));
+    }
+  }
+  SLAPH_HIP(hipStreamSynchronize(stream_));
+}
+
+void Engine::smear_hyp(double alpha1, double alpha2, int iterations) {
+  require_configured();
+  if (!have_gauge_) throw std::runtime_error("gauge links are not set");
+  if (iterations < 0) throw std::out_of_range("HYP smearing: negative iteration count");
+  SLAPH_HIP(hipSetDevice(device_));
+  hyp_links(U_.as<double2>(), U_.as<double2>(), alpha1, alpha2, iterations);
   have_ops_ = false;
 }
 
@@ -460,6 +489,15 @@ void Engine::create_operators(int config) {
       ids.push_back(op.id);
   }
   if (!disp_ids.empty() && !have_gauge_) throw std::runtime_error("gauge links are not set (displaced operators)");
+  // HYP smearing of the input file, applied to a copy of the links for this
+  // call, as the reference smears the freshly read configuration in
+  // create_operators (OperatorFactory.cpp:249-260)
+  disp_links_ = U_.as<double2>();
+  if (!disp_ids.empty() && gd_.hyp_iterations > 0) {
+    Us_.ensure(U_.bytes());
+    hyp_links(U_.as<double2>(), Us_.as<double2>(), gd_.hyp_alpha1, gd_.hyp_alpha2, gd_.hyp_iterations);
+    disp_links_ = Us_.as<double2>();
+  }
   const char *direct = std::getenv("SLAPH_VDV_DIRECT");
   const bool fourier = prec_vdv_ == 0 && !ids.empty() && gd_.Lz <= kFtMaxLz && !tune_.vdv_direct &&
                        !(direct && *direct && *direct != '0');
@@ -568,7 +606,7 @@ int Engine::vdaggerv_direct(std::vector<int> const &ids, int t0, int nt, const d
 // path are equal: each path is computed once and copied.
 int Engine::vdaggerv_displaced(std::vector<int> const &ids, int t, const double2 *V) {
   const size_t kn = (size_t)K_ * N_;
-  const double2 *U = U_.as<double2>() + (size_t)t * 27 * (K_ / 3);
+  const double2 *U = disp_links_ + (size_t)t * 27 * (K_ / 3);
   std::vector<int> paths;  // first id of each distinct path
   std::map<Displacement, int> first;
   for (int id : ids)

@@ -127,6 +127,10 @@ class Engine {
   void synthetic_gauge(unsigned long long seed, double eps);
   // `iterations` stout steps of every timeslice (smearing_stout, GaugeField.cpp:228-274)
   void smear_stout(double rho, int iterations);
+  // `iterations` HYP steps of every timeslice (smearing_hyp, GaugeField.cpp:322-431);
+  // create_operators applies the input file's HYP parameters to a copy of
+  // the links by itself, as the reference does
+  void smear_hyp(double alpha1, double alpha2, int iterations);
   void return_vdaggerv(int id, int t, cplx *out);
 
   // DilutedFactorFactory<Q>::operator[](key) for one operator index: copies
@@ -248,6 +252,9 @@ class Engine {
   bool ev_ortho_ = false;
   DevBuf U_;  // gauge links, Lt x sites x 3 x 9 (allocated when displaced operators exist)
   bool have_gauge_ = false;
+  DevBuf Us_, hyp_dec_, hyp_tmp_;         // HYP-smeared copy of the links and its work space
+  const double2 *disp_links_ = nullptr;   // links the displacements of this create_operators use
+  void hyp_links(const double2 *in, double2 *out, double alpha1, double alpha2, int iterations);
   DevBuf vdisp_, vtmp_, gram_, rinv_, gjob_;  // displacement and orthonormalisation work space
   DevBuf colexp_;                             // column exponents of the Ozaki VdaggerV form
   static constexpr int kDispBatch = 4;        // displacement paths (W = D V, K x N each) per launch

@@ -91,6 +91,76 @@ __device__ M3 m_exp(M3 const &a) {
   return sum;
 }
 
+__device__ __forceinline__ double2 c_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 c_scale(double2 a, double f) { return make_double2(a.x * f, a.y * f); }
+__device__ M3 m_add(M3 a, M3 const &b) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) a.e[i] = c_add(a.e[i], b.e[i]);
+  return a;
+}
+__device__ double m_fro2(M3 const &a) {
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) s += a.e[i].x * a.e[i].x + a.e[i].y * a.e[i].y;
+  return s;
+}
+__device__ __forceinline__ double2 m_at(M3 const &a, int r, int c) { return a.e[c * 3 + r]; }
+__device__ double2 c_div(double2 a, double2 b) {
+  const double d = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+// determinant and inverse (adjugate over determinant) of a 3 x 3 matrix
+__device__ double2 m_det(M3 const &a) {
+  auto m = [&](int r, int c) { return m_at(a, r, c); };
+  const double2 c0 = c_sub(c_mul(m(1, 1), m(2, 2)), c_mul(m(1, 2), m(2, 1)));
+  const double2 c1 = c_sub(c_mul(m(1, 2), m(2, 0)), c_mul(m(1, 0), m(2, 2)));
+  const double2 c2 = c_sub(c_mul(m(1, 0), m(2, 1)), c_mul(m(1, 1), m(2, 0)));
+  return c_add(c_add(c_mul(m(0, 0), c0), c_mul(m(0, 1), c1)), c_mul(m(0, 2), c2));
+}
+__device__ M3 m_inv(M3 const &a) {
+  auto m = [&](int r, int c) { return m_at(a, r, c); };
+  M3 adj;  // adj(r, c) = cofactor(c, r)
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int r1 = (c + 1) % 3, r2 = (c + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+      adj.e[c * 3 + r] = c_sub(c_mul(m(r1, c1), m(r2, c2)), c_mul(m(r1, c2), m(r2, c1)));
+    }
+  const double2 d = m_det(a);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) adj.e[i] = c_div(adj.e[i], d);
+  return adj;
+}
+// proj_to_su3_imp (GaugeField.cpp:203-210): the unitary polar factor
+// W = A (A^+ A)^(-1/2) by the Newton iteration X <- (g X + X^-+ / g) / 2 with
+// g = (|X^-1|_F / |X|_F)^(1/2) (quadratic convergence from any nonsingular A;
+// the loop ends when a step changes X by less than 1e-15 of its norm, at most
+// 60 steps), then W det(W)^(-1/3) with the principal cube root.
+__device__ M3 m_proj_su3(M3 x) {
+  for (int it = 0; it < 60; ++it) {
+    const M3 xi = m_inv(x);
+    const double g = sqrt(sqrt(m_fro2(xi) / m_fro2(x)));
+    M3 nx;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)  // (X^-+)(r, c) = conj(X^-1(c, r))
+        nx.e[c * 3 + r] = c_add(c_scale(x.e[c * 3 + r], 0.5 * g), c_scale(c_conj(xi.e[r * 3 + c]), 0.5 / g));
+    M3 dx = nx;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dx.e[i] = c_sub(nx.e[i], x.e[i]);
+    x = nx;
+    if (m_fro2(dx) <= 1e-30 * m_fro2(x)) break;
+  }
+  const double2 d = m_det(x);
+  const double ang = atan2(d.y, d.x) / 3.0, mag = cbrt(hypot(d.x, d.y));
+  const double2 f = make_double2(cos(ang) / mag, -sin(ang) / mag);  // 1 / (principal cube root of d)
+#pragma unroll
+  for (int i = 0; i < 9; ++i) x.e[i] = c_mul(x.e[i], f);
+  return x;
+}
+
 __device__ __forceinline__ unsigned long long mix(unsigned long long z) {
   z += 0x9E3779B97F4A7C15ULL;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -170,6 +240,59 @@ __global__ void k_stout_step(const double2 *__restrict__ Uin, double2 *__restric
   m_store(Uout + ((long long)site * 3 + mu) * 9, m_mul(m_exp(m_scale(q, -1.0)), u));
 }
 
+// HYP step, level 1: thread per (site, mu); the two decorated links of U_mu(x)
+__global__ void k_hyp_decorate(const double2 *__restrict__ Uin, double2 *__restrict__ dec, int Lx, int Ly, int Lz,
+                               double alpha2) {
+  const int V3 = Lx * Ly * Lz;
+  const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (id >= 3LL * V3) return;
+  const int site = int(id / 3), mu = int(id % 3);
+  auto link = [&](int s, int d) { return m_load(Uin + ((long long)s * 3 + d) * 9); };
+  const M3 u = link(site, mu);
+  for (int k = 0; k < 2; ++k) {
+    const int rho = (mu == 0) ? 1 + k : (mu == 1 ? 2 * k : k);  // the other directions in increasing order
+    const int xr = nbr(site, rho, 1, Lx, Ly, Lz), xm = nbr(site, mu, 1, Lx, Ly, Lz);
+    M3 st = m_mul(link(site, rho), m_mul(link(xr, mu), m_adj(link(xm, rho))));
+    const int xd = nbr(site, rho, -1, Lx, Ly, Lz), xdm = nbr(xd, mu, 1, Lx, Ly, Lz);
+    st = m_add(st, m_mul(m_adj(link(xd, rho)), m_mul(link(xd, mu), link(xdm, rho))));
+    const M3 a = m_add(m_scale(u, 1.0 - alpha2), m_scale(st, alpha2 / 2.0));
+    m_store(dec + ((long long)site * 6 + mu + 3 * k) * 9, m_proj_su3(a));
+  }
+}
+
+// slot of the link in direction d decorated in direction p (the `decor` map
+// of GaugeField.cpp:167-187): d for the first other direction, d + 3 for the
+// second
+__device__ __forceinline__ int hyp_slot(int d, int p) {
+  const int first = d == 0 ? 1 : 0;
+  return p == first ? d : d + 3;
+}
+
+// HYP step, level 2: thread per (site, mu)
+__global__ void k_hyp_outer(const double2 *__restrict__ Uin, const double2 *__restrict__ dec,
+                            double2 *__restrict__ Uout, int Lx, int Ly, int Lz, double alpha1) {
+  const int V3 = Lx * Ly * Lz;
+  const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (id >= 3LL * V3) return;
+  const int site = int(id / 3), mu = int(id % 3);
+  auto dl = [&](int s, int slot) { return m_load(dec + ((long long)s * 6 + slot) * 9); };
+  M3 st;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) st.e[k] = make_double2(0.0, 0.0);
+  for (int nu = 0; nu < 3; ++nu) {
+    if (nu == mu) continue;
+    const int plane = 3 - mu - nu;  // the third direction
+    const int a = hyp_slot(nu, plane), b = hyp_slot(mu, plane);
+    const int xn = nbr(site, nu, 1, Lx, Ly, Lz), xm = nbr(site, mu, 1, Lx, Ly, Lz);
+    st = m_add(st, m_mul(dl(site, a), m_mul(dl(xn, b), m_adj(dl(xm, a)))));
+    const int xd = nbr(site, nu, -1, Lx, Ly, Lz), xdm = nbr(xd, mu, 1, Lx, Ly, Lz);
+    st = m_add(st, m_mul(m_adj(dl(xd, a)), m_mul(dl(xd, b), dl(xdm, a))));
+  }
+  const M3 u = m_load(Uin + ((long long)site * 3 + mu) * 9);
+  m_store(Uout + ((long long)site * 3 + mu) * 9,
+          m_proj_su3(m_add(m_scale(u, 1.0 - alpha1), m_scale(st, alpha1 / 4.0))));
+}
+
 // One thread per (site, group of kDispN eigenvectors): the link matrix is
 // loaded once for the group; the 64 threads of a wave take consecutive sites
 // of one eigenvector, so their loads and stores of the 3-colour vectors are
@@ -217,6 +340,22 @@ void launch_stout_step(const double2 *Uin, double2 *Uout, int Lx, int Ly, int Lz
   if (n <= 0) return;
   if (Uin == Uout) throw std::runtime_error("launch_stout_step: the step needs separate input and output links");
   hipLaunchKernelGGL(k_stout_step, blocks_for(n), dim3(256), 0, s, Uin, Uout, Lx, Ly, Lz, rho);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_hyp_decorate(const double2 *U, double2 *dec, int Lx, int Ly, int Lz, double alpha2, hipStream_t s) {
+  const long long n = 3LL * Lx * Ly * Lz;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_hyp_decorate, blocks_for(n), dim3(256), 0, s, U, dec, Lx, Ly, Lz, alpha2);
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_hyp_outer(const double2 *U, const double2 *dec, double2 *Uout, int Lx, int Ly, int Lz, double alpha1,
+                      hipStream_t s) {
+  const long long n = 3LL * Lx * Ly * Lz;
+  if (n <= 0) return;
+  if (U == Uout) throw std::runtime_error("launch_hyp_outer: the step needs separate input and output links");
+  hipLaunchKernelGGL(k_hyp_outer, blocks_for(n), dim3(256), 0, s, U, dec, Uout, Lx, Ly, Lz, alpha1);
   SLAPH_HIP(hipGetLastError());
 }
 

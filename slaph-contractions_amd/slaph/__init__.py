@@ -59,6 +59,7 @@ _SIGS = {
     "slc_get_gauge_links": (_c.c_int, [_P, _c.c_int, _D]),
     "slc_synthetic_gauge": (_c.c_int, [_P, _c.c_ulonglong, _c.c_double]),
     "slc_smear_stout": (_c.c_int, [_P, _c.c_double, _c.c_int]),
+    "slc_smear_hyp": (_c.c_int, [_P, _c.c_double, _c.c_double, _c.c_int]),
     "slc_vdaggerv_filename": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_int, _I, _c.c_char_p, _c.c_int, _c.c_char_p,
                                          _c.c_int]),
     "slc_query_string": (_c.c_int, [_P, _c.c_char_p, _c.c_char_p, _c.c_int]),
@@ -234,6 +235,9 @@ class Context:
 
     def smear_stout(self, rho: float, iterations: int):
         self._check(_lib.slc_smear_stout(self._h, float(rho), int(iterations)))
+
+    def smear_hyp(self, alpha1: float, alpha2: float, iterations: int):
+        self._check(_lib.slc_smear_hyp(self._h, float(alpha1), float(alpha2), int(iterations)))
 
     def get_input(self, kind: str, index: int) -> np.ndarray:
         """Resident input back on the host: 'eigenvectors' (K x N), 'perambulator'

@@ -136,6 +136,48 @@ def test_stout_smearing_matches_oracle(slaph, inputs):
         ctx.smear_stout(0.1, -1)
 
 
+def test_hyp_smearing_matches_oracle(slaph, inputs):
+    """Two HYP steps (alpha1 = 0.75, alpha2 = 0.6) of every timeslice against
+    the oracle's hyp_step (SVD polar factor; the device uses the Newton
+    iteration), and zero parameters as the identity."""
+    say("hyp smearing")
+    U = inputs[3]
+    ctx = slaph.Context(0)
+    ctx.configure(DISP)
+    for t in range(4):
+        ctx.set_gauge_links(t, U[t])
+    ctx.smear_hyp(0.0, 0.0, 1)
+    assert max(np.max(np.abs(ctx.get_gauge_links(t) - U[t])) for t in range(4)) < 1e-14
+    ctx.smear_hyp(0.75, 0.6, 2)
+    for t in range(4):
+        want = o.hyp_step(o.hyp_step(U[t], 0.75, 0.6, L), 0.75, 0.6, L)
+        assert np.max(np.abs(ctx.get_gauge_links(t) - want)) < 1e-13, t
+
+
+def test_infile_hyp_parameters_smear_a_copy_for_the_displacements(slaph, inputs):
+    """alpha1 / alpha2 / iterations in the input file: create_operators smears
+    a copy of the links (OperatorFactory.cpp:249-260) and displaces with it;
+    the stored links stay as set."""
+    say("infile hyp")
+    cfg, lut, ev, U = inputs[:4]
+    text = DISP.replace("number_of_eigen_vec = 8", "number_of_eigen_vec = 8\nalpha1 = 0.75\nalpha2 = 0.6\niterations = 2")
+    ctx = slaph.Context(0)
+    ctx.configure(text)
+    for t in range(cfg["Lt"]):
+        ctx.set_eigenvectors(t, ev[t])
+        ctx.set_gauge_links(t, U[t])
+    for r in range(4):
+        ctx.set_perambulator(r, inputs[4][r])
+        ctx.set_random_vector(r, inputs[5][r])
+    of = slaph.OperatorFactory(ctx)
+    of.create_operators()
+    smeared = [o.hyp_step(o.hyp_step(u, 0.75, 0.6, L), 0.75, 0.6, L) for u in U]
+    want = o.build_vdaggerv(cfg, lut, ev, links=smeared)
+    for (vid, t), W in want.items():
+        assert np.max(np.abs(of.return_vdaggerv(vid, t) - W)) < 1e-13, (vid, t)
+    assert all(np.array_equal(ctx.get_gauge_links(t), U[t]) for t in range(4))
+
+
 def test_displaced_vdaggerv_matches_oracle(slaph, inputs):
     say("displaced vdaggerv")
     cfg, lut, ev, U = inputs[:4]

@@ -9,6 +9,9 @@ invariant, stout smearing is gauge covariant, keeps links in SU(3), is the
 identity at rho = 0, and on commuting (diagonal) links reduces to the phase
 update theta' = theta + Im w - (1/3) sum Im w with w = (rho / 4) sum of the
 staple phases e^{i(phi - theta)}, computed here by explicit coordinate loops.
+The SU(3) projection and the HYP step (hyp_step) likewise: fixed points,
+unitarity and determinant 1, gauge covariance, zero parameters as the
+identity, and the per-component phase form on diagonal links.
 """
 import itertools
 
@@ -182,3 +185,89 @@ def test_stout_on_diagonal_links_matches_the_phase_update():
     for k in range(3):
         off[:, :, k, k] = 0
     assert np.max(np.abs(off)) < 1e-15
+
+
+def test_proj_su3_closed_forms():
+    """proj_su3: an SU(3) matrix is fixed; c U (c > 0) -> U; the result is
+    unitary with determinant 1; diagonal diag(z) -> diag(z/|z|) / (prod z/|z|)^(1/3)."""
+    U = _links(15).reshape(-1, 3, 3)[:20]
+    assert np.max(np.abs(o.proj_su3(U) - U)) < 1e-14
+    assert np.max(np.abs(o.proj_su3(2.5 * U) - U)) < 1e-14
+    rng = np.random.default_rng(16)
+    A = rng.standard_normal((20, 3, 3)) + 1j * rng.standard_normal((20, 3, 3))
+    P = o.proj_su3(A)
+    assert np.max(np.abs(P @ P.conj().swapaxes(1, 2) - np.eye(3))) < 1e-13
+    assert np.max(np.abs(np.linalg.det(P) - 1.0)) < 1e-13
+    z = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+    ph = z / np.abs(z)
+    want = np.diag(ph) / np.prod(ph) ** (1.0 / 3.0)
+    assert np.max(np.abs(o.proj_su3(np.diag(z)[None])[0] - want)) < 1e-14
+
+
+def test_hyp_zero_alphas_identity_su3_and_covariance():
+    U = _links(17)
+    assert np.max(np.abs(o.hyp_step(U, 0.0, 0.0, L) - U)) < 1e-13
+    S = o.hyp_step(U, 0.75, 0.6, L)
+    m = S.reshape(-1, 3, 3)
+    assert np.max(np.abs(m @ m.conj().swapaxes(1, 2) - np.eye(3))) < 1e-13
+    assert np.max(np.abs(np.linalg.det(m) - 1.0)) < 1e-13
+    assert np.max(np.abs(S - U)) > 1e-2
+    g = _su3(np.random.default_rng(18), SITES)
+    a = _gauge_transform(o.hyp_step(U, 0.75, 0.6, L), g)
+    b = o.hyp_step(_gauge_transform(U, g), 0.75, 0.6, L)
+    assert np.max(np.abs(a - b)) < 1e-12
+
+
+def test_hyp_on_diagonal_links_matches_the_phase_form():
+    """Diagonal links diag(e^{i theta_k}): every product is diagonal, so each
+    level acts per component k: w_k = (1 - alpha) e^{i theta_k} + (alpha / n)
+    (sum of the staple phases), projected to w_k / |w_k| times the common
+    factor (prod_k w_k / |w_k|)^(-1/3).  Explicit loops over coordinates."""
+    rng = np.random.default_rng(19)
+    th = rng.uniform(-np.pi, np.pi, (SITES, 3, 2))
+    theta = np.concatenate([th, -th.sum(axis=2, keepdims=True)], axis=2)
+    U = np.zeros((SITES, 3, 3, 3), complex)
+    for k in range(3):
+        U[:, :, k, k] = np.exp(1j * theta[:, :, k])
+    a1, a2 = 0.7, 0.5
+    got = o.hyp_step(U, a1, a2, L)
+
+    def unit_and_fix(w):
+        ph = w / np.abs(w)
+        return ph / np.prod(ph) ** (1.0 / 3.0)
+
+    def site(c):
+        return _site(*[c[i] % L[i] for i in range(3)])
+
+    def shift(c, d, s):
+        c = list(c)
+        c[d] += s
+        return c
+
+    coords = list(itertools.product(range(L[0]), range(L[1]), range(L[2])))
+    others = {0: (1, 2), 1: (0, 2), 2: (0, 1)}
+    dec = {}  # (site, mu, rho) -> diagonal (3,)
+    for c in coords:
+        for mu in range(3):
+            for rho in others[mu]:
+                up = (np.exp(1j * theta[site(c), rho]) * np.exp(1j * theta[site(shift(c, rho, 1)), mu])
+                      * np.exp(-1j * theta[site(shift(c, mu, 1)), rho]))
+                cd = shift(c, rho, -1)
+                dn = (np.exp(-1j * theta[site(cd), rho]) * np.exp(1j * theta[site(cd), mu])
+                      * np.exp(1j * theta[site(shift(cd, mu, 1)), rho]))
+                w = (1 - a2) * np.exp(1j * theta[site(c), mu]) + a2 / 2 * (up + dn)
+                dec[(site(c), mu, rho)] = unit_and_fix(w)
+    for c in coords:
+        for mu in range(3):
+            st = np.zeros(3, complex)
+            for nu in range(3):
+                if nu == mu:
+                    continue
+                p = 3 - mu - nu
+                cd = shift(c, nu, -1)
+                st += (dec[(site(c), nu, p)] * dec[(site(shift(c, nu, 1)), mu, p)]
+                       * np.conj(dec[(site(shift(c, mu, 1)), nu, p)]))
+                st += (np.conj(dec[(site(cd), nu, p)]) * dec[(site(cd), mu, p)]
+                       * dec[(site(shift(cd, mu, 1)), nu, p)])
+            want = unit_and_fix((1 - a1) * np.exp(1j * theta[site(c), mu]) + a1 / 4 * st)
+            assert np.max(np.abs(np.diag(got[site(c), mu]) - want)) < 1e-13
