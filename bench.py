@@ -18,7 +18,8 @@ algorithmic bytes per launch are the distinct product matrices the launch
 reads plus the partial sums it writes.  `roofline_vdaggerv` is the VdaggerV
 kernel k_vdv_ft on its own operation count against the FP64 peak (the
 roofline object itself for workloads without C4cC).  cpu_baseline is the
-numpy oracle timed on a bounded sample on this host and extrapolated.
+C++ / OpenMP restatement of the reference's algorithm (oracle/cpu_ref.cpp)
+timed on a bounded sample on this host and extrapolated (see cpu_baseline).
 """
 import argparse
 import json
@@ -103,76 +104,98 @@ def kernel_traffic(kernel, workload):
     return None
 
 
-def cpu_baseline(w, text, max_seconds=30.0):
-    """Numpy oracle on a bounded sample of the same workload, extrapolated to
-    one configuration: one VdaggerV product, Q0 of one timeslice, Q2 of one
-    key, and one slice pair of every diagram on a subset of its correlators."""
+def cpu_baseline(w, text, sample_slice_pairs=2):
+    """The reference's algorithm as the C++ / OpenMP restatement
+    (oracle/cpu_ref.cpp, checked against the reference's golden files) on this
+    host, on a bounded sample of the same workload, extrapolated to one
+    configuration:
+
+    * contraction: every dilution block pair runs on its own thread (the
+      reference's `omp for schedule(dynamic)` over block pairs,
+      Correlators.cpp:69-116) for its first `sample_slice_pairs` slice pairs,
+      so the factor builds of the keys they need and their traces are timed
+      under the real concurrency; per block pair the time is then (factor keys
+      it needs, by type) x (measured seconds per key of that type) + (its slice
+      pairs) x (measured trace seconds per slice pair), and the block pairs are
+      dealt to the threads in order, each to the first free thread;
+    * VdaggerV: one (p, t) product timed on a K-slab of the lattice and scaled
+      by K, times the momenta (and displacement paths) per timeslice, with the
+      timeslices dealt to the threads (OperatorFactory.cpp:263).
+    Inputs are random of the workload's shapes (values do not change the
+    work)."""
     import numpy as np
+    from oracle import cpu_ref
     from oracle import slaph_oracle as o
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())
     rng = np.random.default_rng(7)
     cfg = o.parse_infile(text)
     lut = o.init_lookup_tables(cfg)
-    L, T, N = w["L"], w["T"], w["N"]
+    L, T, N, dilT = w["L"], w["T"], w["N"], w["dilT"]
     K = 3 * L ** 3
-    V = np.linalg.qr(rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N)))[0]
-    ph = o.create_momenta(L, L, L, (1, 0, 0))
-    t0 = time.perf_counter()
-    o.vdaggerv(V, ph)
-    t_vdv = time.perf_counter() - t0
-    n_vdv = sum(1 for vid, _, _ in lut["vdv_lookup"] if vid != lut["index_of_unity"])
-    # factors and one slice pair on random operators (values do not change the cost)
-    nb = T // w["dilT"]
+    nb = T // dilT
     cols = nb * w["dilE"] * 4
     perams = [rng.standard_normal((4 * N * T, cols)) + 1j * rng.standard_normal((4 * N * T, cols))
               for _ in range(w["nrnd"])]
     rnds = [(rng.integers(0, 2, T * N * 4) * 2 - 1 + 1j * (rng.integers(0, 2, T * N * 4) * 2 - 1)) / np.sqrt(2)
             for _ in range(w["nrnd"])]
+    M = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    ref = cpu_ref.CpuRef(cfg, lut, lambda i, t: M, perams, rnds)
+    bps = o.block_pairs(T, dilT)
+    used = min(threads, len(bps))
+    _, wall = ref.contract(threads=used, first_block_pair=0, block_pairs=len(bps), slice_pairs=sample_slice_pairs)
+    st = ref.last_stats()
+    per_key = {q: st[q][0] / st[q][1] for q in ("Q0", "Q1", "Q2") if st[q][1] > 0}
+    n_sampled = sum(min(sample_slice_pairs, len(o.slice_pairs(a, b, T, nb))) for a, b in bps)
+    factor_s = sum(st[q][0] for q in ("Q0", "Q1", "Q2"))
+    per_pair = (st["busy"] - factor_s) / n_sampled  # thread-seconds of traces and resolves per slice pair
+    times = []
+    for a, b in bps:
+        sp = o.slice_pairs(a, b, T, nb)
+        keys = cpu_ref.factor_keys(lut, dilT, sp)
+        tb = len(sp) * per_pair
+        for q in ("Q0", "Q1", "Q2"):
+            nq = sum(1 for k in keys if k[0] == q)
+            if nq:
+                tb += nq * per_key[q]
+        times.append(tb)
+    free = [0.0] * used
+    for tb in times:  # schedule(dynamic): the next block pair to the first free thread
+        i = free.index(min(free))
+        free[i] += tb
+    contract_s = max(free)
+    # VdaggerV: one product on a K-slab, scaled by K
+    slab = max(3, (K // 8) // 3 * 3)
+    V = rng.standard_normal((slab, N)) + 1j * rng.standard_normal((slab, N))
+    ph = np.exp(-2j * np.pi * rng.random(slab // 3))
+    _, t_slab = cpu_ref.vdaggerv(V, ph)
+    t_prod = t_slab * K / slab
+    n_ops = sum(1 for vid, _, _ in lut["vdv_lookup"] if vid != lut["index_of_unity"])
+    free = [0.0] * min(threads, T)
+    for _ in range(T):
+        i = free.index(min(free))
+        free[i] += n_ops * t_prod
+    vdv_s = max(free)
+    total = vdv_s + contract_s
+    sample = ("C++/OpenMP restatement of the reference (oracle/cpu_ref.cpp), random inputs of the workload's shapes: "
+              "the first %d slice pairs of each of the %d block pairs, one block pair per thread on %d threads "
+              "(%.1f s wall; factor keys %s s each; %.3f thread-s of traces per slice pair), extrapolated per "
+              "block pair by its keys and slice pairs and scheduled over %d threads -> contraction %.0f s; "
+              "VdaggerV: one product on %d of %d rows (%.2f s), x %d operators x %d timeslices over %d threads "
+              "-> %.0f s" % (sample_slice_pairs, len(bps), used, wall,
+                             json.dumps({q: round(v, 3) for q, v in per_key.items()}), per_pair, used, contract_s,
+                             slab, K, t_slab, n_ops, T, min(threads, T), vdv_s))
+    return {"value": total, "unit": "s", "cores": used, "kind": "port", "sample": sample,
+            "cpu": _cpu_model(), "threads_available": threads}
 
-    class LazyVdv(dict):
-        def __missing__(self, key):
-            m = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
-            self[key] = m
-            return m
-    fac = o.Factories(cfg, lut, LazyVdv(), perams, rnds, w["dilT"], w["dilE"])
-    tr = o.Traces(fac, lut, w["dilT"])
-    t_so, t_si = 0, w["dilT"]  # an off-diagonal block pair
-    t0 = time.perf_counter()
-    fac.get("Q0", (t_si,))
-    t_q0 = time.perf_counter() - t0
-    b_so, b_si = t_so // w["dilT"], t_si // w["dilT"]
-    t0 = time.perf_counter()
-    fac.get("Q2", (b_si, t_so, b_si))
-    t_q2 = time.perf_counter() - t0
-    # the remaining keys this slice pair needs are built untimed
-    fac.get("Q0", (t_so,))
-    for key in ((b_so, t_so, b_si), (b_si, t_si, b_so), (b_si, t_si, b_si), (b_so, t_so, b_so)):
-        fac.get("Q2", key)
-    q2keys = set()
-    for a in range(nb):
-        for t in range(T):
-            q2keys.add((a, t, a))
-            q2keys.add((t // w["dilT"], t, a))
-    per_pair = 0.0
-    budget = max_seconds
-    subset = {}
-    for d, reqs in lut["requests"].items():
-        n_sub = 0
-        t0 = time.perf_counter()
-        for trs, _ in reqs:
-            o.resolve_request(trs, tr, t_so, t_si)
-            n_sub += 1
-            if time.perf_counter() - t0 > budget / max(1, len(lut["requests"])):
-                break
-        dt = time.perf_counter() - t0
-        per_pair += dt * len(reqs) / n_sub
-        subset[d] = "%d/%d" % (n_sub, len(reqs))
-    total = t_vdv * n_vdv * T + t_q0 * T + t_q2 * len(q2keys) + per_pair * T * T
-    sample = ("numpy oracle, extrapolated: VdaggerV 1 of %d (p,t) products [%.2f s each]; Q0 of 1 of %d "
-              "timeslices; Q2 of 1 of %d keys; 1 of %d slice pairs with correlator subsets %s" %
-              (n_vdv * T, t_vdv, T, len(q2keys), T * T, json.dumps(subset)))
-    threads = os.environ.get("OMP_NUM_THREADS") or str(os.cpu_count())
-    return {"value": total, "unit": "s", "cores": int(threads), "kind": "port", "sample": sample,
-            "note": "VdaggerV leg uses BLAS with the listed threads; the contraction leg is single-threaded Python"}
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _describe(name, w):
