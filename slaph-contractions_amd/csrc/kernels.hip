@@ -1056,6 +1056,59 @@ __global__ __launch_bounds__(256) void k_q2b16(const Q2BJob *__restrict__ jobs, 
     J.out[(lane >> 4) + 4 * i + 16 * col] = make_double2(p1[i] - p2[i], p3[i] - p1[i] - p2[i]);
 }
 
+// D = 24 / 32 (N a multiple of 4): the same product as 2 x 2 MFMA tiles of
+// 16 x 16 per job (rows and columns past D are zero operands, not stored);
+// lane l supplies M'[16 ta + l%16][k] and P2[k][16 tb + l%16] at
+// k = bd*N + w0 + l/16.  One wavefront per job, 4 jobs per workgroup.
+template <int D>
+__global__ __launch_bounds__(256) void k_q2b_mfma(const Q2BJob *__restrict__ jobs, int njobs, int N, long long prow) {
+  const int job = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (job >= njobs) return;
+  const Q2BJob J = jobs[job];
+  const int lane = threadIdx.x & 63, kk = lane >> 4, m = lane & 15;
+  d4 acc[2][2][3];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc[a][b][k] = d4{0.0, 0.0, 0.0, 0.0};
+  const bool in1 = 16 + m < D;  // the second tile's row / column exists
+  for (int bd = 0; bd < 4; ++bd) {
+    const int g = c_gamma_row[J.gamma][bd];
+    const double2 *p0 = J.peram2 + (long long)(J.b2 * D + m) * prow + (long long)(J.t * 4 + g) * N + kk;
+    const double2 *p1 = p0 + 16LL * prow;
+    const double2 *mr = J.Mp + ((long long)bd * N + kk) * D + m;
+    for (int w0 = 0; w0 < N; w0 += 4) {
+      const double2 zero = make_double2(0.0, 0.0);
+      const double2 a0 = mr[(long long)w0 * D], a1 = in1 ? mr[(long long)w0 * D + 16] : zero;
+      const double2 b0 = p0[w0], b1 = in1 ? p1[w0] : zero;
+      const double2 av[2] = {a0, a1}, bv[2] = {b0, b1};
+#pragma unroll
+      for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb) {
+          acc[ta][tb][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ta].x, bv[tb].x, acc[ta][tb][0], 0, 0, 0);
+          acc[ta][tb][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ta].y, bv[tb].y, acc[ta][tb][1], 0, 0, 0);
+          acc[ta][tb][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ta].x + av[ta].y, bv[tb].x + bv[tb].y,
+                                                                acc[ta][tb][2], 0, 0, 0);
+        }
+    }
+  }
+#pragma unroll
+  for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = 16 * ta + kk + 4 * i, col = 16 * tb + m;
+        if (row < D && col < D) {
+          const d4 &q1 = acc[ta][tb][0], &q2 = acc[ta][tb][1], &q3 = acc[ta][tb][2];
+          J.out[(long long)col * D + row] = make_double2(q1[i] - q2[i], q3[i] - q1[i] - q2[i]);
+        }
+      }
+}
+
 __global__ void k_dagger(const double2 *__restrict__ in, double2 *__restrict__ out, int N, int count) {
   const long long total = (long long)N * N * count;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
@@ -1195,6 +1248,74 @@ __global__ __launch_bounds__(256) void k_prod16(const T *__restrict__ sX, const 
 #pragma unroll
     for (int i = 0; i < 4; ++i) st2_nt(o + lane + 64 * i, make_double2(p1[i] - p2[i], p3[i] - p1[i] - p2[i]));
   }
+}
+
+// D = 24 and D = 32 factor products on FP64 MFMA: one wavefront per
+// product, the D x D output as 2 x 2 tiles of 16 x 16 (rows and columns past
+// D of the D = 24 case are zero operands and are not stored), D / 4 k-steps of
+// 16x16x4 MFMAs, three real products per complex one (Gauss).  As in k_prod16
+// the stored layout decides the operand roles: C (column-major) is computed
+// as C^T = Y^T X^T with Y as the A operand, so that a lane's results are 16
+// consecutive rows of one column; C^T (kTransposed) as C = X Y.
+template <typename T, int D, bool kTransposed>
+__global__ __launch_bounds__(256) void k_prod_mfma(const T *__restrict__ sX, const long long *__restrict__ Xt,
+                                                   const long long *__restrict__ Xq, const T *__restrict__ sY,
+                                                   const long long *__restrict__ Yt, const long long *__restrict__ Yq,
+                                                   T *__restrict__ out, long long obase, long long ostride,
+                                                   const ComboProd *__restrict__ combos, int ncombo) {
+  static_assert(D == 24 || D == 32, "k_prod_mfma is for D = 24 and 32");
+  constexpr int KS = D / 4;  // k-steps of 4
+  // XCD-contiguous mapping of the (time, combination block) sequence, as k_prod16
+  const int NB = gridDim.x * gridDim.y, lin = blockIdx.y * gridDim.x + blockIdx.x;
+  const int per = NB / 8, rem = NB % 8, xcd = lin % 8, jx = lin / 8;
+  const int lb = xcd < rem ? xcd * (per + 1) + jx : rem * (per + 1) + (xcd - rem) * per + jx;
+  const int ti = lb / gridDim.x, bx = lb % gridDim.x;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = bx * 4 + wave;
+  if (c >= ncombo) return;
+  const ComboProd cb = combos[c];
+  const T *x = sX + (Xt[ti] + Xq[cb.qn] + cb.i) * (long long)(D * D);
+  const T *y = sY + (Yt[ti] + Yq[cb.qn] + cb.j) * (long long)(D * D);
+  const int m = lane & 15, kk = lane >> 4;
+  // A[t][s]: element (16 t + m, 4 s + kk) of the A matrix, B[t][s]: element
+  // (4 s + kk, 16 t + m) of the B matrix, zero past D
+  double2 A[2][KS], B[2][KS];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const int r = 16 * t + m, k = 4 * s + kk;
+      const bool in = r < D;
+      if (kTransposed) {  // A = X (r, k), B = Y (k, r)
+        A[t][s] = in ? ld2(x + k * D + r) : make_double2(0.0, 0.0);
+        B[t][s] = in ? ld2(y + r * D + k) : make_double2(0.0, 0.0);
+      } else {  // A = Y^T (r, k) = Y (k, r), B = X^T (k, r) = X (r, k)
+        A[t][s] = in ? ld2(y + r * D + k) : make_double2(0.0, 0.0);
+        B[t][s] = in ? ld2(x + k * D + r) : make_double2(0.0, 0.0);
+      }
+    }
+  T *o = out + (obase + ti * ostride + cb.o) * (long long)(D * D);
+#pragma unroll
+  for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb) {
+      d4 p1 = d4{0.0, 0.0, 0.0, 0.0}, p2 = p1, p3 = p1;
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const double2 a = A[ta][s], b = B[tb][s];
+        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, p1, 0, 0, 0);
+        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, p2, 0, 0, 0);
+        p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x + a.y, b.x + b.y, p3, 0, 0, 0);
+      }
+      // result element i of lane l: row 16 ta + l/16 + 4 i, column 16 tb + l%16
+      // of A B; stored at (column 16 ta + ..) x D + row 16 tb + l%16 of the
+      // output (C when A = Y^T, C^T when A = X), i.e. 16 consecutive elements
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int ar = 16 * ta + kk + 4 * i, bc = 16 * tb + m;
+        if (ar < D && bc < D) st2(o + (long long)ar * D + bc, make_double2(p1[i] - p2[i], p3[i] - p1[i] - p2[i]));
+      }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2389,6 +2510,10 @@ void launch_q2b(const Q2BJob *d_jobs, int njobs, int N, int dilE, long long prow
   upload_gamma_constants();
   if (dilE == 4 && N % 4 == 0)
     hipLaunchKernelGGL(k_q2b16, dim3((njobs + 3) / 4), dim3(256), 0, s, d_jobs, njobs, N, prow);
+  else if (dilE == 6 && N % 4 == 0)
+    hipLaunchKernelGGL(k_q2b_mfma<24>, dim3((njobs + 3) / 4), dim3(256), 0, s, d_jobs, njobs, N, prow);
+  else if (dilE == 8 && N % 4 == 0)
+    hipLaunchKernelGGL(k_q2b_mfma<32>, dim3((njobs + 3) / 4), dim3(256), 0, s, d_jobs, njobs, N, prow);
   else
     hipLaunchKernelGGL(k_q2b, dim3(njobs), dim3(256), 0, s, d_jobs, N, dilE, prow);
   SLAPH_HIP(hipGetLastError());
@@ -2419,6 +2544,24 @@ void launch_prod_t(const void *sX, const long long *Xt, const long long *Xq, con
                    int ncombo, int ntime, int D, int transposed, int generic, hipStream_t s) {
   const T *x = static_cast<const T *>(sX), *y = static_cast<const T *>(sY);
   T *o = static_cast<T *>(out);
+  if ((D == 24 || D == 32) && !generic && !std::getenv("SLAPH_PROD_GENERIC")) {
+    const dim3 grid((ncombo + 3) / 4, ntime);
+    if ((long long)grid.x * grid.y > 0x7fffffffLL) throw std::runtime_error("launch_prod: grid too large");
+#define SLAPH_PROD_MFMA(DD)                                                                                       \
+  if (transposed)                                                                                                 \
+    hipLaunchKernelGGL((k_prod_mfma<T, DD, true>), grid, dim3(256), 0, s, x, Xt, Xq, y, Yt, Yq, o, obase, ostride, \
+                       combos, ncombo);                                                                           \
+  else                                                                                                            \
+    hipLaunchKernelGGL((k_prod_mfma<T, DD, false>), grid, dim3(256), 0, s, x, Xt, Xq, y, Yt, Yq, o, obase, ostride, \
+                       combos, ncombo);
+    if (D == 24) {
+      SLAPH_PROD_MFMA(24)
+    } else {
+      SLAPH_PROD_MFMA(32)
+    }
+#undef SLAPH_PROD_MFMA
+    return;
+  }
   if (D == 16 && !generic && !std::getenv("SLAPH_PROD_GENERIC")) {
     const int per_block = 4 * kProdPerWave;
     const dim3 grid((ncombo + per_block - 1) / per_block, ntime);

@@ -294,15 +294,16 @@ KERNEL_SWITCHES = ("SLAPH_TRACE_GROUPS", "SLAPH_PROD_GENERIC", "SLAPH_RESOLVE_GR
                    "SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS", "SLAPH_LSTAT_WINDOW", "SLAPH_GRID_GROUPS")
 
 
-@pytest.mark.parametrize("dilE,N", [(6, 14), (8, 20)], ids=["EI6-D24", "EI8-D32"])
+@pytest.mark.parametrize("dilE,N", [(6, 14), (6, 16), (8, 20)], ids=["EI6-D24", "EI6-D24-N16", "EI8-D32"])
 @pytest.mark.parametrize("kernels", ["production", "blocks+items", "window1+gridgroups", "groups+generic"])
 def test_dilution_d24_d32_matches_oracle(slaph, dilE, N, kernels, monkeypatch):
     """D = 24 and D = 32 (the EI6 of the reference's infiles/pipi_I2.in and the
     EI8 of the 24^3 x 48 production workload) with N not a multiple of dilE:
-    Q0 / Q1 / Q2 at dilE 6 and 8, the generic product kernel, and the
-    left-stationary trace kernels at D = 24 / 32 (queued and block form,
-    one-round descriptor windows) against the oracle at 1e-12 norm-wise, on
-    device-generated inputs downloaded for the oracle."""
+    Q0 / Q1 / Q2 at dilE 6 and 8 (the MFMA Q2 stage when N is a multiple of
+    4, the generic one otherwise), the MFMA and the generic product kernels,
+    and the left-stationary trace kernels at D = 24 / 32 (queued and block
+    form, one-round descriptor windows) against the oracle at 1e-12
+    norm-wise, on device-generated inputs downloaded for the oracle."""
     say("dilution D=%d N=%d" % (4 * dilE, N), kernels)
     on = {"production": (), "blocks+items": ("SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS"),
           "window1+gridgroups": ("SLAPH_LSTAT_WINDOW", "SLAPH_GRID_GROUPS"),
