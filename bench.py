@@ -91,17 +91,67 @@ quark = u:{nrnd}:TB:{dilT}:EI:{dilE}:DF:4:synthetic
 LSTAT_KERNEL = "k_trace_lstat" if os.environ.get("SLAPH_LSTAT_BLOCKS") else "k_trace_lstat_q"
 
 
+def kernel_source_hash(kernel):
+    """sha256 (12 hex digits) of the source text of `kernel`'s definition in
+    kernels.hip (from its __global__ line to the closing brace at column 0):
+    a traffic figure measured for another version of the kernel is stale."""
+    import hashlib
+    src = open(os.path.join(ROOT, "slaph-contractions_amd", "csrc", "kernels.hip")).read().splitlines()
+    start = next((i for i, ln in enumerate(src) if "__global__" in ln and (" %s(" % kernel) in ln + src[min(i + 1, len(src) - 1)]),
+                 None)
+    if start is None:
+        return None
+    end = next(i for i in range(start, len(src)) if src[i] == "}")
+    return hashlib.sha256("\n".join(src[start:end + 1]).encode()).hexdigest()[:12]
+
+
 def kernel_traffic(kernel, workload):
     """HBM bytes of one launch of `kernel`, measured with two rocprofv3 --pmc
-    runs (FETCH_SIZE, WRITE_SIZE, copy-calibrated) and kept under profiles/;
-    None for a workload without such a measurement."""
+    runs (FETCH_SIZE, WRITE_SIZE, copy-calibrated) and kept under profiles/,
+    with the kernel source hash it was measured on: (bytes or None, note)."""
     path = os.path.join(ROOT, "profiles", "%s_%s_traffic.txt" % (kernel, workload))
     if not os.path.exists(path):
-        return None
+        return None, "no measurement for this workload"
+    vals = {}
     for line in open(path):
-        if line.startswith("bytes_per_launch"):
-            return float(line.split()[1])
-    return None
+        if not line.startswith("#") and line.strip():
+            k, v = line.split(None, 1)
+            vals[k] = v.strip()
+    now = kernel_source_hash(kernel)
+    if vals.get("kernel_sha") != now:
+        return None, "stale: measured on kernel source %s, now %s" % (vals.get("kernel_sha"), now)
+    return float(vals["bytes_per_launch"]), "profiles/%s_%s_traffic.txt (kernel source %s)" % (kernel, workload, now)
+
+
+def measure_traffic(kernel, workload, copy_kib):
+    """HBM bytes per launch of `kernel` in this run's configuration, from two
+    child runs of this bench under rocprofv3 --pmc (FETCH_SIZE, then
+    WRITE_SIZE), each scaled by what the counter reads for the device-to-device
+    copy of one eigenvector timeslice (copy_kib KiB) in the same run."""
+    import csv
+    import subprocess
+    import tempfile
+    out = tempfile.mkdtemp(prefix="slaph_traffic_", dir=os.environ.get("TMPDIR", "/tmp"))
+    child = [sys.executable, os.path.abspath(__file__), "--workload", workload, "--steps", "1", "--warmup", "0",
+             "--no-cpu-baseline", "--no-precision-forms", "--no-autotune", "--no-graphs"]
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", out, "-o", counter, "--"] + child
+        print("# traffic: %s" % " ".join(cmd), file=sys.stderr, flush=True)
+        subprocess.run(cmd, check=True, timeout=600, stdout=subprocess.DEVNULL)
+        kern, copy = [], []
+        for r in csv.DictReader(open(os.path.join(out, counter + "_counter_collection.csv"))):
+            v = float(r["Counter_Value"])
+            if kernel + "<" in r["Kernel_Name"] or kernel + "(" in r["Kernel_Name"]:
+                kern.append(v)
+            elif "copyBuffer" in r["Kernel_Name"] and abs(v - copy_kib) < 0.75 * copy_kib:
+                copy.append(v)
+        if not kern or not copy:
+            raise RuntimeError("traffic: no %s dispatches of %s or of the calibration copy" % (counter, kernel))
+        res[counter] = (sum(kern) / len(kern), copy_kib / (sum(copy) / len(copy)), len(kern))
+    (f, sf, n), (w, sw, _) = res["FETCH_SIZE"], res["WRITE_SIZE"]
+    return {"bytes_per_launch": (sf * f + sw * w) * 1024.0, "fetch_kib": f, "write_kib": w, "fetch_scale": sf,
+            "write_scale": sw, "launches": n, "kernel_sha": kernel_source_hash(kernel)}
 
 
 def cpu_baseline(w, text, sample_slice_pairs=2):
@@ -228,13 +278,18 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch the contraction directly (no HIP graph replay)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default schedules (otherwise the autotuner picks them before the timed steps)")
+    ap.add_argument("--measure-traffic", action="store_true",
+                    help="measure roofline.traffic in this run: two child runs under rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: there is no multi-GPU path "
               "(block pairs of slice pairs would be split across GPUs)", file=sys.stderr)
         return 2
-    import slaph
     w = WORKLOADS[args.workload]
+    traffic_run = None
+    if args.measure_traffic:  # before this process touches the GPU
+        traffic_run = measure_traffic(LSTAT_KERNEL, args.workload, 3 * w["L"] ** 3 * w["N"] * 16 / 1024.0)
+    import slaph
     text = infile(w)
     ctx = slaph.Context(0)
     if w.get("streaming"):
@@ -317,9 +372,13 @@ def main():
         launches = max(ls[1], 1.0)
         kern_ms, bytes_per_launch = ls[0] / launches, ls[2] / launches
         achieved = bytes_per_launch / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
-        line["roofline"] = {"bound": "hbm", "kernel": LSTAT_KERNEL + " (non-separable traces: C4cC, C4cD)", "achieved": achieved,
+        if traffic_run is not None:
+            traffic, tnote = traffic_run["bytes_per_launch"], "measured in this run: %s" % json.dumps(traffic_run)
+        else:
+            traffic, tnote = kernel_traffic(LSTAT_KERNEL, args.workload)
+        line["roofline"] = {"bound": "hbm", "kernel": LSTAT_KERNEL + " (non-separable traces)", "achieved": achieved,
                             "peak": PEAK_HBM, "unit": "GB/s", "frac": achieved / PEAK_HBM,
-                            "traffic": kernel_traffic(LSTAT_KERNEL, args.workload),
+                            "traffic": traffic, "traffic_source": tnote,
                             "traffic_unit": "bytes per launch (HBM, rocprofv3 --pmc)",
                             "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": kern_ms,
                             "launches_per_step": ls[1] / args.steps}
