@@ -278,6 +278,8 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch the contraction directly (no HIP graph replay)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default schedules (otherwise the autotuner picks them before the timed steps)")
+    ap.add_argument("--precision", default="fp64,fp64",
+                    help="VdaggerV and factor forms of the timed run (reported in config; the FP64 path by default)")
     ap.add_argument("--measure-traffic", action="store_true",
                     help="measure roofline.traffic in this run: two child runs under rocprofv3 --pmc")
     args = ap.parse_args()
@@ -297,6 +299,8 @@ def main():
     ctx.configure(text)
     ctx.set_chunk_budget(int(args.chunk_gib * (1 << 30)))
     ctx.set_graphs(not args.no_graphs)
+    pv, pf = args.precision.split(",")
+    ctx.set_precision(pv, pf)
     t0 = time.perf_counter()
     # streamed eigenvectors are generated per timeslice inside each step (in
     # place of the reference's per-timeslice file read), without the
@@ -358,11 +362,11 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64,fp64" else "mixed (%s)" % args.precision,
         "data": "synthetic (seeded, device-generated; %s)" % (
             "eigenvectors generated per timeslice, not orthonormalised" if w.get("streaming")
             else "orthonormalised eigenvectors"),
-        "config": {"workload": _describe(args.workload, w), "precision": "fp64",
+        "config": {"workload": _describe(args.workload, w), "precision": args.precision,
                    "hip_graphs": not args.no_graphs, "tuning": ctx.tuning()},
         "stages_ms": {"vdaggerv": stage["vdaggerv"], "factors": stage["factors"],
                       "contract_incl_factors": stage["contract"]},
@@ -400,7 +404,7 @@ def main():
             line["roofline"] = obj
     if tune is not None:
         line["autotune"] = tune
-    if not args.no_precision_forms:
+    if not args.no_precision_forms and args.precision == "fp64,fp64":
         pf = precision_forms(ctx, ops, ms / 1000.0, step)
         line["precision_forms"] = pf
         ok = [f for f in pf if f["within_1e-10"]]

@@ -596,8 +596,9 @@ int Engine::vdaggerv_direct(std::vector<int> const &ids, int t0, int nt, const d
 // Displaced operators at timeslice t: W = D V by the one-link displacements
 // of the operator in order (each '>' forward / '<' backward along x, y or
 // z), then VdaggerV = V^+ W on MFMA (OperatorFactory.cpp:287-300), the
-// distinct displacement paths of a batch in one launch of the VdaggerV
-// kernel (A panel V, B panel the path's W).  The displacement is the
+// distinct displacement paths of a batch in one launch: in FP64 the split-K
+// tall product (64 x 64 tiles), in the precision forms their VdaggerV
+// kernels (A panel V, B panel the path's W).  The displacement is the
 // covariant one, U_mu(x) V(x + mu) / U_mu(x - mu)^+ V(x - mu), computed from
 // the unchanged input of each step (the reference updates its result in
 // place while reading it, displace_eigenvectors, OperatorFactory.cpp:33-55,
@@ -649,11 +650,19 @@ int Engine::vdaggerv_displaced(std::vector<int> const &ids, int t, const double2
       }
       jobs.push_back(j);
     }
-    upload(gjob_, jobs);
-    if (prec_vdv_ >= 2)
-      launch_vdv_split(gjob_.as<VdvJob>(), int(jobs.size()), N_, K_, prec_vdv_, stream_);
-    else
-      launch_vdv(gjob_.as<VdvJob>(), int(jobs.size()), N_, K_, prec_vdv_, stream_);
+    if (prec_vdv_ == 0) {  // FP64: the split-K tall product
+      std::vector<AhbBigJob> big;
+      for (auto const &j : jobs) big.push_back(AhbBigJob{j.V, j.W, j.C[0]});
+      upload(bigjob_, big);
+      ahb_part_.ensure(sizeof(double2) * (size_t)ahb_big_part_elems(int(big.size()), N_, K_));
+      launch_ahb_big(bigjob_.as<AhbBigJob>(), int(big.size()), N_, K_, ahb_part_.as<double2>(), stream_);
+    } else {
+      upload(gjob_, jobs);
+      if (prec_vdv_ >= 2)
+        launch_vdv_split(gjob_.as<VdvJob>(), int(jobs.size()), N_, K_, prec_vdv_, stream_);
+      else
+        launch_vdv(gjob_.as<VdvJob>(), int(jobs.size()), N_, K_, prec_vdv_, stream_);
+    }
     ++launches;
     SLAPH_HIP(hipStreamSynchronize(stream_));  // the job table and work space are reused
   }

@@ -257,6 +257,7 @@ class Engine {
   void hyp_links(const double2 *in, double2 *out, double alpha1, double alpha2, int iterations);
   DevBuf vdisp_, vtmp_, gram_, rinv_, gjob_;  // displacement and orthonormalisation work space
   DevBuf colexp_;                             // column exponents of the Ozaki VdaggerV form
+  DevBuf bigjob_, ahb_part_;                  // displaced products: job table, split-K partial tiles
   static constexpr int kDispBatch = 4;        // displacement paths (W = D V, K x N each) per launch
   long long K_ = 0;
   int N_ = 0, D_ = 0, Lt_ = 0, nb_ = 0, nrnd_ = 0;

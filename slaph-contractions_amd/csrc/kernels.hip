@@ -8,6 +8,7 @@
 // that the host planner builds.
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 #include <mutex>
 
@@ -610,6 +611,112 @@ __global__ __launch_bounds__(256) void k_vdv_split(const VdvJob *__restrict__ jo
       }
       C[(long long)col * N + row] = v;
     }
+  }
+}
+
+// C = A^H B for tall K (displaced VdaggerV: A = V(t), B = D V(t), K = 3 L^3):
+// 64 x 64 output tile per workgroup, 4 waves of 32 x 32 (2 x 2 MFMA tiles,
+// three real products per complex one: Re = p1 + p2, Im = p3 - p1 + p2 for
+// conj(a) b), K split into `parts` ranges of whole 16-row chunks staged
+// through double-buffered LDS; each part writes its partial tile, and
+// k_ahb_big_reduce sums the parts in order (no atomics: the result does not
+// depend on the schedule).
+constexpr int kBigT = 64, kBigKC = 16;
+
+__global__ __launch_bounds__(256) void k_ahb_big(const AhbBigJob *__restrict__ jobs, int N, long long K, int tiles,
+                                                 long long kper, double2 *__restrict__ part) {
+  const AhbBigJob J = jobs[blockIdx.z];
+  const int tile = blockIdx.x, p = blockIdx.y, parts = gridDim.y;
+  const int m0 = (tile / tiles) * kBigT, n0 = (tile % tiles) * kBigT;
+  const long long kb = (long long)p * kper, ke = min(K, kb + kper);
+  __shared__ double2 As[2][kBigT][kBigKC + 1];
+  __shared__ double2 Bs[2][kBigT][kBigKC + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  d4 acc[2][2][3];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc[a][b][k] = d4{0.0, 0.0, 0.0, 0.0};
+  double2 ra[4], rb[4];
+  const int nk = int((ke - kb + kBigKC - 1) / kBigKC);
+  auto load = [&](int kc) {
+    const long long k0 = kb + (long long)kc * kBigKC;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int idx = tid + r * 256, kk = idx % kBigKC, c = idx / kBigKC;
+      const long long k = k0 + kk;
+      ra[r] = (k < ke && m0 + c < N) ? J.A[(long long)(m0 + c) * K + k] : make_double2(0.0, 0.0);
+      rb[r] = (k < ke && n0 + c < N) ? J.B[(long long)(n0 + c) * K + k] : make_double2(0.0, 0.0);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int idx = tid + r * 256;
+      As[buf][idx / kBigKC][idx % kBigKC] = ra[r];
+      Bs[buf][idx / kBigKC][idx % kBigKC] = rb[r];
+    }
+  };
+  if (nk > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int kc = 0; kc < nk; ++kc) {
+    const int buf = kc & 1;
+    if (kc + 1 < nk) load(kc + 1);
+#pragma unroll
+    for (int kq = 0; kq < kBigKC; kq += 4) {
+      const int kr = kq + (lane >> 4), l16 = lane & 15;
+      const double2 a[2] = {As[buf][wm * 32 + l16][kr], As[buf][wm * 32 + 16 + l16][kr]};
+      const double2 b[2] = {Bs[buf][wn * 32 + l16][kr], Bs[buf][wn * 32 + 16 + l16][kr]};
+#pragma unroll
+      for (int ta = 0; ta < 2; ++ta) {
+        const double ad = a[ta].x - a[ta].y;
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb) {
+          acc[ta][tb][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta].x, b[tb].x, acc[ta][tb][0], 0, 0, 0);
+          acc[ta][tb][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ta].y, b[tb].y, acc[ta][tb][1], 0, 0, 0);
+          acc[ta][tb][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad, b[tb].x + b[tb].y, acc[ta][tb][2], 0, 0, 0);
+        }
+      }
+    }
+    if (kc + 1 < nk) store(buf ^ 1);
+    __syncthreads();
+  }
+  // partial tile, column-major 64 x 64: [job][part][tile]
+  double2 *o = part + (((long long)blockIdx.z * parts + p) * gridDim.x + tile) * (kBigT * kBigT);
+#pragma unroll
+  for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = wm * 32 + ta * 16 + (lane >> 4) + 4 * i, col = wn * 32 + tb * 16 + (lane & 15);
+        const d4 &q1 = acc[ta][tb][0], &q2 = acc[ta][tb][1], &q3 = acc[ta][tb][2];
+        o[col * kBigT + row] = make_double2(q1[i] + q2[i], q3[i] - q1[i] + q2[i]);
+      }
+}
+
+__global__ void k_ahb_big_reduce(const AhbBigJob *__restrict__ jobs, int njobs, int N, int tiles, int parts,
+                                 const double2 *__restrict__ part) {
+  const long long total = (long long)njobs * N * N;
+  for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+       id += (long long)gridDim.x * blockDim.x) {
+    const int job = int(id / ((long long)N * N));
+    const int e = int(id % ((long long)N * N)), row = e % N, col = e / N;
+    const int tile = (row / kBigT) * tiles + col / kBigT;
+    const int local = (col % kBigT) * kBigT + row % kBigT;
+    double2 s = make_double2(0.0, 0.0);
+    for (int p = 0; p < parts; ++p) {
+      const double2 v = part[(((long long)job * parts + p) * tiles * tiles + tile) * (kBigT * kBigT) + local];
+      s.x += v.x;
+      s.y += v.y;
+    }
+    jobs[job].C[(long long)col * N + row] = s;
   }
 }
 
@@ -2430,6 +2537,33 @@ void launch_vdv_split(const VdvJob *d_jobs, int njobs, int N, long long K, int f
     hipLaunchKernelGGL(k_vdv_split<0>, dim3(tiles * tiles, njobs), dim3(256), 0, s, d_jobs, N, K, tiles);
   else
     hipLaunchKernelGGL(k_vdv_split<1>, dim3(tiles * tiles, njobs), dim3(256), 0, s, d_jobs, N, K, tiles);
+  SLAPH_HIP(hipGetLastError());
+}
+
+long long ahb_big_part_elems(int njobs, int N, long long K) {
+  const int tiles = (N + kBigT - 1) / kBigT;
+  return (long long)njobs * ahb_big_parts(njobs, N, K) * tiles * tiles * kBigT * kBigT;
+}
+
+int ahb_big_parts(int njobs, int N, long long K) {
+  // about 8 workgroups per compute unit of 256, each part at least 64 chunks
+  const int tiles = (N + kBigT - 1) / kBigT;
+  const long long want = (8LL * 256 + (long long)tiles * tiles * njobs - 1) / ((long long)tiles * tiles * njobs);
+  const long long most = std::max(1LL, K / (64LL * kBigKC));
+  return int(std::max(1LL, std::min({want, most, 64LL})));
+}
+
+void launch_ahb_big(const AhbBigJob *d_jobs, int njobs, int N, long long K, double2 *part, hipStream_t s) {
+  if (njobs <= 0 || N <= 0 || K <= 0) return;
+  if (njobs > 65535) throw std::runtime_error("launch_ahb_big: too many jobs");
+  const int tiles = (N + kBigT - 1) / kBigT, parts = ahb_big_parts(njobs, N, K);
+  const long long chunks = (K + kBigKC - 1) / kBigKC;
+  const long long kper = ((chunks + parts - 1) / parts) * kBigKC;
+  hipLaunchKernelGGL(k_ahb_big, dim3(unsigned(tiles * tiles), unsigned(parts), unsigned(njobs)), dim3(256), 0, s,
+                     d_jobs, N, K, tiles, kper, part);
+  SLAPH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_ahb_big_reduce, grid1d((long long)njobs * N * N, 256), dim3(256), 0, s, d_jobs, njobs, N,
+                     tiles, parts, part);
   SLAPH_HIP(hipGetLastError());
 }
 

@@ -73,6 +73,16 @@ void launch_vdv(const VdvJob *d_jobs, int njobs, int N, long long K, int f32, hi
 // launch_colexp of V and of the right operand).
 constexpr int kOzakiSlices = 7;
 void launch_vdv_split(const VdvJob *d_jobs, int njobs, int N, long long K, int form, hipStream_t s);
+// C = A^H B (N x N, column-major) for K x N column-major A and B with tall
+// K, FP64 MFMA, K split into parts summed in order; `part` holds
+// ahb_big_part_elems(njobs, N, K) complex numbers.
+struct AhbBigJob {
+  const double2 *A, *B;
+  double2 *C;
+};
+int ahb_big_parts(int njobs, int N, long long K);
+long long ahb_big_part_elems(int njobs, int N, long long K);
+void launch_ahb_big(const AhbBigJob *d_jobs, int njobs, int N, long long K, double2 *part, hipStream_t s);
 // e[n] with max_k |V[k, n]| < 2^e for each column of a K x N matrix
 void launch_colexp(const double2 *V, long long K, int N, int *e, hipStream_t s);
 
