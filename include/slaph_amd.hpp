@@ -109,8 +109,37 @@ class Context {
   void set_random_vector(int entity, std::vector<Complex> const &rho) {
     check(slc_set_random_vector(get(), entity, reinterpret_cast<const double *>(rho.data())));
   }
+  // GaugeField time slices (after read_gauge_field): Lx Ly Lz sites x 3
+  // directions x Eigen::Matrix3cd (column-major), and its smearings
+  void set_gauge_links(int t, std::vector<Complex> const &U) {
+    check(slc_set_gauge_links(get(), t, reinterpret_cast<const double *>(U.data())));
+  }
+  void smearing_stout(double rho, int iterations) { check(slc_smear_stout(get(), rho, iterations)); }
+  void smearing_hyp(double alpha1, double alpha2, int iterations) {
+    check(slc_smear_hyp(get(), alpha1, alpha2, iterations));
+  }
+  // one eigenvector timeslice on the device (call before configure)
+  void set_timeslice_streaming(bool on) { check(slc_set_timeslice_streaming(get(), on ? 1 : 0)); }
+  // beside the reference: precision forms, HIP graphs, autotuning
+  void set_precision(int vdaggerv, int factors) { check(slc_set_precision(get(), vdaggerv, factors)); }
+  void set_graphs(bool on) { check(slc_set_graphs(get(), on ? 1 : 0)); }
+  std::string autotune(int repeats) {
+    check(slc_autotune(get(), repeats));
+    return text_of(slc_autotune_report);
+  }
+  std::string tuning() const { return text_of(slc_get_tuning); }
+  void set_tuning(std::string const &text) { check(slc_set_tuning(get(), text.c_str())); }
 
  private:
+  std::string text_of(int (*fn)(const slc_ctx *, char *, int)) const {
+    const int need = fn(get(), nullptr, 0);
+    check(need);
+    std::string out(size_t(need), '\0');
+    check(fn(get(), &out[0], need));
+    out.resize(size_t(need) - 1);
+    return out;
+  }
+
   struct Del {
     void operator()(slc_ctx *c) const { slc_ctx_destroy(c); }
   };

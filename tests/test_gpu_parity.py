@@ -164,6 +164,32 @@ def test_l4_golden_correlators(slaph, l4_ctx, kahan):
     say("l4 worst abs error", worst)
 
 
+def test_l4_golden_with_eigenvectors_streamed_from_files(slaph):
+    """Timeslice streaming with the eigenvectors read file by file inside
+    create_operators (as the reference reads them, OperatorFactory.cpp:263-277):
+    the same 2,307 golden datasets at 1e-12, the full V never resident, and
+    get_input reading V(t) back from its file."""
+    say("l4: streamed eigenvector files")
+    ctx = slaph.Context(0)
+    ctx.set_timeslice_streaming(True)
+    ctx.configure(open(os.path.join(L4, "test_all.ini")).read())
+    ctx.read_inputs(L4, 1000)
+    slaph.OperatorFactory(ctx).create_operators()
+    slaph.contract(ctx)
+    gold = np.load(os.path.join(GOLD, "l4_correlators.npz"))
+    seen = set()
+    for d in ctx.diagrams():
+        for name, vals in ctx.correlators(d).items():
+            seen.add(d + "/" + name)
+            assert np.max(np.abs(vals - gold[d + "/" + name])) <= 1e-12, (d, name)
+    assert seen == set(gold.files)
+    cfg = o.parse_infile(open(os.path.join(L4, "test_all.ini")).read())
+    _, _, ev = o.read_inputs(cfg, L4, 1000)
+    assert np.array_equal(ctx.get_input("eigenvectors", 2), ev[2])
+    with pytest.raises(slaph.SlaphError, match="streaming"):
+        ctx.set_eigenvectors(0, ev[0])
+
+
 def test_l4_single_block_pair_and_chunking(slaph, l4_ctx, l4_oracle):
     """single_time_slice_combination and a tiny chunk budget (one slice pair
     per launch) give the oracle's numbers."""
