@@ -37,7 +37,7 @@ enum {
 int slc_ctx_create(int device, slc_ctx **out);
 void slc_ctx_destroy(slc_ctx *ctx);
 /* Message of the last failure on ctx; with ctx == NULL, of the last failed
- * slc_ctx_create on this thread. */
+ * slc_ctx_create or slc_read_lime_gauge on this thread. */
 const char *slc_last_error(const slc_ctx *ctx);
 
 /* read_parameters + init_lookup_tables from the text of an infile
@@ -48,7 +48,8 @@ int slc_configure(slc_ctx *ctx, const char *infile_text);
 /* Sizes after slc_configure.  `what` is one of "Lt", "Lx", "Ly", "Lz", "nev",
  * "dilT", "dilE", "n_rnd", "peram_rows", "peram_cols", "n_vdaggerv",
  * "index_of_unity", "n_Q0", "n_Q1", "n_Q2", "start_config", "end_config",
- * "delta_config". */
+ * "delta_config", "need_gaugefield" (1 when an operator is displaced:
+ * OperatorLookup::need_gaugefield, init_lookup_tables.cpp). */
 int slc_query(const slc_ctx *ctx, const char *what, long long *out);
 
 /* String settings of the infile after slc_configure (GlobalData members,
@@ -95,11 +96,40 @@ int slc_set_timeslice_streaming(slc_ctx *ctx, int on);
 /* Gauge links of displaced operators (GaugeField, src/GaugeField.cpp): one
  * timeslice t of Lx Ly Lz sites (site = (x Ly + y) Lz + z) x 3 directions
  * (x, y, z) x one 3 x 3 SU(3) matrix, column-major (the element order of
- * Eigen::Matrix3cd; GaugeField::tslices[t][site][dir]).  Reading ILDG/lime
- * configurations (GaugeField.cpp:781-928) is not part of this path: links
- * are handed over or generated. */
+ * Eigen::Matrix3cd; GaugeField::tslices[t][site][dir]).  Links are handed
+ * over here, read from lime files (slc_read_gauge_field) or generated
+ * (slc_synthetic_gauge). */
 int slc_set_gauge_links(slc_ctx *ctx, int t, const double *U);
 int slc_get_gauge_links(slc_ctx *ctx, int t, double *U);
+/* GaugeField::read_gauge_field (src/GaugeField.cpp:762-779) with
+ * read_lime_gauge_field_doubleprec_timeslices (:781-928) and
+ * map_timeslice_to_eigen (:111-149): timeslices slice_i..slice_f of the lime
+ * file <config_path>/conf.%04d (config_path NULL or "" = path_config of the
+ * input file; a relative path is taken below base_dir, NULL = ".") into the
+ * links above; U_mu(x, y, z, t) of the file's ILDG binary record goes to
+ * site (x Ly + y) Lz + z, direction mu, as is.  For Lx = Ly = Lz (the lattices
+ * the reference reads correctly: for other shapes its reader and timeslice
+ * map index sites differently) this is the reference's tslices.  Errors as
+ * the reference's: "Could not open file ...", "no ildg-binary-data record
+ * found in file ...", "Probably wrong lattice size or precision ...",
+ * "Single precision read!"; beyond it, an ildg-format record whose lattice
+ * differs from the input file, and a scidac-checksum record that does not
+ * match the data, are errors too (the links are then marked not set).
+ * slc_create_operators applies the input file's HYP smearing to a copy. */
+int slc_read_gauge_field(slc_ctx *ctx, const char *base_dir, const char *config_path, int config, int slice_i,
+                         int slice_f);
+/* GaugeField::plaque_ts (src/GaugeField.cpp:742-760) of the links of
+ * timeslice t: the mean over sites and ordered pairs mu != nu of the spatial
+ * directions of Re tr(U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+) / 3. */
+int slc_plaquette(slc_ctx *ctx, int t, double *plaquette);
+/* The reader of slc_read_gauge_field into host memory, without a context or
+ * a device: timeslices slice_i..slice_f of the lime file `path` on an
+ * Lx x Ly x Lz x Lt lattice into U, (slice_f - slice_i + 1) x 27 Lx Ly Lz
+ * complex numbers in the layout of slc_set_gauge_links, timeslice after
+ * timeslice (GaugeField::read_lime_gauge_field_doubleprec_timeslices with
+ * map_timeslice_to_eigen, src/GaugeField.cpp:781-928, :111-149).  The
+ * message of a failure is slc_last_error(NULL). */
+int slc_read_lime_gauge(const char *path, int Lx, int Ly, int Lz, int Lt, int slice_i, int slice_f, double *U);
 /* Seeded SU(3) links exp(i eps H), H traceless Hermitian from complex
  * Gaussian entries, on every timeslice (no reference counterpart: synthetic
  * gauge configurations in place of the lime files). */

@@ -109,8 +109,19 @@ class Context {
   void set_random_vector(int entity, std::vector<Complex> const &rho) {
     check(slc_set_random_vector(get(), entity, reinterpret_cast<const double *>(rho.data())));
   }
-  // GaugeField time slices (after read_gauge_field): Lx Ly Lz sites x 3
-  // directions x Eigen::Matrix3cd (column-major), and its smearings
+  // GaugeField time slices: Lx Ly Lz sites x 3 directions x
+  // Eigen::Matrix3cd (column-major), read from <config_path>/conf.%04d
+  // (GaugeField::read_gauge_field; "" = path_config of the infile) or handed
+  // over, and their smearings and plaquette
+  void read_gauge_field(int config, int slice_i, int slice_f, std::string const &config_path = "",
+                        std::string const &base_dir = ".") {
+    check(slc_read_gauge_field(get(), base_dir.c_str(), config_path.c_str(), config, slice_i, slice_f));
+  }
+  double plaque_ts(int t) const {
+    double p = 0.0;
+    check(slc_plaquette(get(), t, &p));
+    return p;
+  }
   void set_gauge_links(int t, std::vector<Complex> const &U) {
     check(slc_set_gauge_links(get(), t, reinterpret_cast<const double *>(U.data())));
   }
@@ -160,7 +171,10 @@ class OperatorFactory {
   }
   // create_operators(filename, rnd_vec, config): the eigenvectors come from
   // Context::read_inputs (or set_eigenvectors); config names the VdaggerV
-  // files of handling_vdaggerv = write / read / liuming.
+  // files of handling_vdaggerv = write / read / liuming.  Displaced
+  // operators take the links of Context::read_gauge_field (the file the
+  // reference's build_vdaggerv reads itself) or set_gauge_links; the
+  // infile's HYP smearing is applied here, as there.
   void create_operators(int config) {
     cache_.clear();
     ctx_.check(slc_create_operators_config(ctx_.get(), config));

@@ -30,6 +30,14 @@ Pins (each checked in tests/test_oracle.py):
                        alpha = 0 identity, SU(3), gauge covariance, the
                        diagonal-link closed form); the reference's tests hold
                        no displaced data
+  * lime / ILDG reader - tests/test_lime_gauge.py: the gauge file of
+                       tests/integration-L4 (gauge/conf.1000) against its own
+                       SciDAC checksum record and the plaquette of its
+                       xlf-info record (0.604379109408), which fix the byte
+                       order, the site and direction order and the matrix
+                       orientation; plaque_ts against the spatial planes of
+                       the independent 4D plaquette, gauge invariance, unit
+                       links -> 1
 """
 from __future__ import annotations
 
@@ -662,6 +670,109 @@ def stout_step(links, rho, L):
         q = 0.5 * a - (np.trace(a, axis1=1, axis2=2) / 6.0)[:, None, None] * np.eye(3)
         out[:, mu] = np.stack([scipy.linalg.expm(-qs) for qs in q]) @ links[:, mu]
     return out
+
+
+# ---------------------------------------------------------------------------
+# Gauge configurations in lime / ILDG files (GaugeField.cpp:762-928, the
+# timeslice map :111-149) and the spatial plaquette (:722-760)
+# ---------------------------------------------------------------------------
+
+
+def lime_records(data):
+    """{record type: payload} of a lime file: a 144-byte big-endian header
+    per record (magic 0x456789ab, version, flags, payload length, 128-byte
+    type string), the payload padded to a multiple of 8 bytes."""
+    import struct
+    recs, pos = {}, 0
+    while pos + 144 <= len(data):
+        magic, _, _, length = struct.unpack(">IHHQ", data[pos:pos + 16])
+        if magic != 0x456789AB:
+            raise RuntimeError("not a lime record at byte %d" % pos)
+        typ = data[pos + 16:pos + 144].split(b"\0")[0].decode()
+        recs.setdefault(typ, data[pos + 144:pos + 144 + length])
+        pos += 144 + ((length + 7) // 8) * 8
+    return recs
+
+
+def scidac_checksum(binary, site_bytes):
+    """SciDAC checksum of an ILDG binary record: per site of lexicographic
+    rank r (x fastest, t slowest) the CRC-32 c of its bytes as stored;
+    suma = xor of rotl(c, r mod 29), sumb = xor of rotl(c, r mod 31)."""
+    import zlib
+    rot = lambda c, n: ((c << n) | (c >> (32 - n))) & 0xFFFFFFFF if n else c  # noqa: E731
+    suma = sumb = 0
+    for r in range(len(binary) // site_bytes):
+        c = zlib.crc32(binary[r * site_bytes:(r + 1) * site_bytes]) & 0xFFFFFFFF
+        suma ^= rot(c, r % 29)
+        sumb ^= rot(c, r % 31)
+    return suma, sumb
+
+
+def ildg_links(binary, L, Lt):
+    """The 4D links of an FP64 ILDG record as [t][z][y][x][mu][row][col]
+    (mu = x, y, z, t; big-endian doubles, the record's order), and the spatial
+    links per timeslice as [t][site][mu][row][col], site = (x Ly + y) Lz + z,
+    mu = x, y, z: the reference's reader + timeslice map for Lx = Ly = Lz
+    (GaugeField.cpp:879-905 stores U_mu at mu * 18 of the site,
+    map_timeslice_to_eigen :111-149 maps them to tslices[t][site][mu - 1])."""
+    Lx, Ly, Lz = L
+    a = np.frombuffer(binary, dtype=">f8").astype(np.float64).reshape(Lt, Lz, Ly, Lx, 4, 3, 3, 2)
+    full = a[..., 0] + 1j * a[..., 1]
+    spatial = np.ascontiguousarray(np.transpose(full, (0, 3, 2, 1, 4, 5, 6))[:, :, :, :, :3])
+    return full, spatial.reshape(Lt, Lx * Ly * Lz, 3, 3, 3)
+
+
+def read_lime_gauge(path, L, Lt):
+    """GaugeField::read_gauge_field (GaugeField.cpp:762-779) for a
+    double-precision file: (spatial links [t][site][mu][3][3], records).
+    A single-precision record raises "Single precision read!" as :830-833."""
+    data = open(path, "rb").read()
+    recs = lime_records(data)
+    if "ildg-binary-data" not in recs:
+        raise RuntimeError("no ildg-binary-data record found in file " + path)
+    b = recs["ildg-binary-data"]
+    nd = L[0] * L[1] * L[2] * Lt * 72
+    if len(b) != nd * 8:
+        if len(b) == nd * 4:
+            raise RuntimeError("Single precision read!")
+        raise RuntimeError("Probably wrong lattice size or precision (bytes=%d) in file %s" % (len(b), path))
+    return ildg_links(b, L, Lt)[1], recs
+
+
+def plaquette_4d(full, planes=None):
+    """Mean of Re tr(U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+) / 3 over the
+    planes mu < nu (default all six) and all sites of the 4D links
+    [t][z][y][x][mu][3][3], mu = x, y, z, t (over all six: the value tmLQCD
+    writes to a file's xlf-info record)."""
+    axis = {0: 3, 1: 2, 2: 1, 3: 0}
+    adj = lambda m: np.conj(np.swapaxes(m, -1, -2))  # noqa: E731
+    if planes is None:
+        planes = [(mu, nu) for mu in range(4) for nu in range(mu + 1, 4)]
+    tot, n = 0.0, 0
+    for mu, nu in planes:
+        um, un = full[..., mu, :, :], full[..., nu, :, :]
+        p = um @ np.roll(un, -1, axis=axis[mu]) @ adj(np.roll(um, -1, axis=axis[nu])) @ adj(un)
+        tot += np.trace(p, axis1=-2, axis2=-1).real.sum() / 3.0
+        n += p[..., 0, 0].size
+    return tot / n
+
+
+def plaquette_ts(links, L):
+    """GaugeField::plaque_ts (GaugeField.cpp:722-760): the mean over sites and
+    ordered pairs mu != nu of the three spatial directions of
+    Re tr(U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+) / 3 for one timeslice's
+    links [site][mu][3][3]."""
+    adj = lambda m: m.conj().swapaxes(-1, -2)  # noqa: E731
+    tot = 0.0
+    for mu in range(3):
+        for nu in range(3):
+            if mu == nu:
+                continue
+            j = neighbour(L, mu, 1)
+            l_ = neighbour(L, nu, 1)
+            p = links[:, mu] @ links[j, nu] @ adj(links[l_, mu]) @ adj(links[:, nu])
+            tot += np.trace(p, axis1=1, axis2=2).real.sum() / 3.0
+    return tot / (6 * links.shape[0])
 
 
 # ---------------------------------------------------------------------------

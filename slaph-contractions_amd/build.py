@@ -26,7 +26,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-I" + 
 # Packed FP32 instructions stay off in device code: some of their forms have
 # given wrong results on this hardware without any error.
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["kernels.hip", "gauge.hip", "host.cpp", "engine.cpp", "contract.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "gauge.hip", "host.cpp", "lime.cpp", "engine.cpp", "contract.cpp", "capi.cpp"]
 
 
 def compile_one(src):
@@ -72,7 +72,7 @@ def build(verbose=True):
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, "link")
+    run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"], "link")
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -4,6 +4,7 @@
 #include "../../include/slaph_amd.h"
 
 #include "engine.hpp"
+#include "lime.hpp"
 
 #include <cstring>
 #include <new>
@@ -17,8 +18,8 @@ struct slc_ctx {
 
 namespace {
 
-// Message of the last failed slc_ctx_create on this thread (no context exists
-// to hold it); returned by slc_last_error(NULL).
+// Message of the last failed slc_ctx_create or context-free call on this
+// thread (no context exists to hold it); returned by slc_last_error(NULL).
 thread_local std::string g_create_error;
 
 template <typename F>
@@ -105,6 +106,7 @@ int slc_query(const slc_ctx *cctx, const char *what, long long *out) {
     else if (w == "start_config") *out = gd.start_config;
     else if (w == "end_config") *out = gd.end_config;
     else if (w == "delta_config") *out = gd.delta_config;
+    else if (w == "need_gaugefield") *out = gd.need_gaugefield ? 1 : 0;
     else throw std::out_of_range("unknown query " + w);
   });
 }
@@ -171,6 +173,37 @@ int slc_get_gauge_links(slc_ctx *ctx, int t, double *U) {
   return guarded(ctx, [&] {
     if (!U) throw std::out_of_range("null output");
     ctx->eng->get_gauge_links(t, reinterpret_cast<slaph::cplx *>(U));
+  });
+}
+
+int slc_read_gauge_field(slc_ctx *ctx, const char *base, const char *config_path, int config, int slice_i,
+                         int slice_f) {
+  return guarded(ctx, [&] {
+    ctx->eng->read_gauge_field(base ? base : ".", config_path ? config_path : "", config, slice_i, slice_f);
+  });
+}
+
+int slc_read_lime_gauge(const char *path, int Lx, int Ly, int Lz, int Lt, int slice_i, int slice_f, double *U) {
+  try {
+    if (!path || !U) throw std::out_of_range("null argument");
+    const size_t n = 27 * (size_t)Lx * Ly * Lz;
+    slaph::read_ildg_gauge(path, Lx, Ly, Lz, Lt, slice_i, slice_f, [&](int t, const slaph::cplx *links) {
+      std::memcpy(U + 2 * n * size_t(t - slice_i), links, sizeof(slaph::cplx) * n);
+    });
+    return SLC_OK;
+  } catch (std::out_of_range const &e) {
+    g_create_error = e.what();
+    return SLC_ERR_ARG;
+  } catch (std::exception const &e) {
+    g_create_error = e.what();
+    return SLC_ERR_RUNTIME;
+  }
+}
+
+int slc_plaquette(slc_ctx *ctx, int t, double *plaquette) {
+  return guarded(ctx, [&] {
+    if (!plaquette) throw std::out_of_range("null output");
+    *plaquette = ctx->eng->plaquette(t);
   });
 }
 

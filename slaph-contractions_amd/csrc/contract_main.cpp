@@ -137,9 +137,22 @@ int main(int ac, char **av) {
   check(ctx, slc_query_string(ctx, "output_path", opath.data(), need), "query");
   const std::string output_path(opath.data());
 
+  long long need_gauge = 0, Lt_all = 0;
+  check(ctx, slc_query(ctx, "need_gaugefield", &need_gauge), "query");
+  check(ctx, slc_query(ctx, "Lt", &Lt_all), "query");
+  const int hneed = slc_query_string(ctx, "handling_vdaggerv", nullptr, 0);
+  check(ctx, hneed, "query");
+  std::vector<char> hbuf(size_t(hneed) + 1, 0);
+  check(ctx, slc_query_string(ctx, "handling_vdaggerv", hbuf.data(), hneed), "query");
+  const std::string handling(hbuf.data());
+
   for (int cfg = start; cfg <= end; cfg += delta) {
     std::cout << "\nprocessing configuration: " << cfg << "\n\n" << std::endl;
     check(ctx, slc_read_inputs(ctx, ".", cfg), "reading inputs");
+    // build_vdaggerv reads the gauge configuration when an operator is
+    // displaced (OperatorFactory.cpp:241-261)
+    if (need_gauge && (handling == "build" || handling == "write"))
+      check(ctx, slc_read_gauge_field(ctx, ".", nullptr, cfg, 0, int(Lt_all) - 1), "reading the gauge field");
     check(ctx, slc_create_operators_config(ctx, cfg), "create_operators");
     check(ctx, slc_contract(ctx, single, kahan), "contract");
     char ending[64];

@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include "gauge.hpp"
+#include "lime.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -157,6 +158,45 @@ void Engine::get_gauge_links(int t, cplx *U) const {
   const size_t n = 27 * (size_t)(K_ / 3);
   SLAPH_HIP(hipSetDevice(device_));
   SLAPH_HIP(hipMemcpy(U, U_.as<double2>() + (size_t)t * n, sizeof(double2) * n, hipMemcpyDeviceToHost));
+}
+
+void Engine::read_gauge_field(std::string const &base, std::string const &config_path, int config, int slice_i,
+                              int slice_f) {
+  require_configured();
+  if (!gd_.need_gaugefield) throw std::runtime_error("no displaced operator needs gauge links");
+  std::string dir = config_path.empty() ? gd_.path_config : config_path;
+  if (!dir.empty() && dir[0] != '/' && !base.empty() && base != ".") dir = base + "/" + dir;
+  char name[32];
+  std::snprintf(name, sizeof name, "/conf.%04d", config);
+  const size_t n = 27 * (size_t)(K_ / 3);
+  SLAPH_HIP(hipSetDevice(device_));
+  have_ops_ = false;
+  try {
+    read_ildg_gauge(dir + name, gd_.Lx, gd_.Ly, gd_.Lz, Lt_, slice_i, slice_f, [&](int t, const cplx *U) {
+      SLAPH_HIP(hipMemcpy(U_.as<double2>() + (size_t)t * n, U, sizeof(double2) * n, hipMemcpyHostToDevice));
+    });
+  } catch (...) {
+    have_gauge_ = false;  // links of a file that failed its checks are not used
+    throw;
+  }
+  have_gauge_ = true;
+}
+
+double Engine::plaquette(int t) {
+  require_configured();
+  if (!have_gauge_) throw std::runtime_error("gauge links are not set");
+  if (t < 0 || t >= Lt_) throw std::runtime_error("timeslice out of range");
+  const int nb = plaquette_blocks(gd_.Lx, gd_.Ly, gd_.Lz);
+  SLAPH_HIP(hipSetDevice(device_));
+  DevBuf part;
+  part.alloc(sizeof(double) * nb);
+  launch_plaquette(U_.as<double2>() + (size_t)t * 27 * (K_ / 3), gd_.Lx, gd_.Ly, gd_.Lz, part.as<double>(), stream_);
+  std::vector<double> h(nb);
+  SLAPH_HIP(hipMemcpyAsync(h.data(), part.as<void>(), sizeof(double) * nb, hipMemcpyDeviceToHost, stream_));
+  SLAPH_HIP(hipStreamSynchronize(stream_));
+  double s = 0.0;
+  for (double v : h) s += v;
+  return s / (6.0 * (K_ / 3));
 }
 
 void Engine::synthetic_gauge(unsigned long long seed, double eps) {

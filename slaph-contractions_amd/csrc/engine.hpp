@@ -123,6 +123,13 @@ class Engine {
   // one timeslice of Lx Ly Lz sites x 3 directions x 3 x 3 (column-major).
   void set_gauge_links(int t, const cplx *U);
   void get_gauge_links(int t, cplx *U) const;
+  // GaugeField::read_gauge_field (GaugeField.cpp:762-779): timeslices
+  // slice_i..slice_f of the lime file <config_path>/conf.%04d (config_path
+  // "" = path_config of the input file, relative paths below base_dir)
+  void read_gauge_field(std::string const &base_dir, std::string const &config_path, int config, int slice_i,
+                        int slice_f);
+  // GaugeField::plaque_ts (GaugeField.cpp:742-760) of the stored links
+  double plaquette(int t);
   // seeded SU(3) links exp(i eps H), H traceless Hermitian (Gaussian)
   void synthetic_gauge(unsigned long long seed, double eps);
   // `iterations` stout steps of every timeslice (smearing_stout, GaugeField.cpp:228-274)

@@ -324,9 +324,54 @@ __global__ __launch_bounds__(256) void k_displace(const double2 *__restrict__ U,
   }
 }
 
+// Spatial plaquette sum: thread per site, the six ordered planes (mu, nu),
+// Re tr(U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+) / 3 summed per site, then
+// per block (fixed tree order) into partial[blockIdx.x].
+__global__ __launch_bounds__(256) void k_plaquette(const double2 *__restrict__ U, int Lx, int Ly, int Lz,
+                                                   double *__restrict__ partial) {
+  __shared__ double red[256];
+  const int V3 = Lx * Ly * Lz;
+  const int site = blockIdx.x * 256 + threadIdx.x;
+  double p = 0.0;
+  if (site < V3) {
+    auto link = [&](int s, int d) { return m_load(U + ((long long)s * 3 + d) * 9); };
+    for (int mu = 0; mu < 3; ++mu)
+      for (int nu = 0; nu < 3; ++nu) {
+        if (mu == nu) continue;
+        const int j = nbr(site, mu, 1, Lx, Ly, Lz), l = nbr(site, nu, 1, Lx, Ly, Lz);
+        const M3 a = m_mul(link(site, mu), link(j, nu));
+        const M3 b = m_mul(m_adj(link(l, mu)), m_adj(link(site, nu)));
+        // Re tr(a b) = sum_{r,c} Re(a[r][c] b[c][r])
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) {
+            const double2 x = m_at(a, r, c), y = m_at(b, c, r);
+            p += x.x * y.x - x.y * y.y;
+          }
+      }
+    p /= 3.0;
+  }
+  red[threadIdx.x] = p;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 dim3 blocks_for(long long n) { return dim3(unsigned((n + 255) / 256)); }
 
 }  // namespace
+
+int plaquette_blocks(int Lx, int Ly, int Lz) { return int(((long long)Lx * Ly * Lz + 255) / 256); }
+
+void launch_plaquette(const double2 *U, int Lx, int Ly, int Lz, double *partial, hipStream_t s) {
+  const long long V3 = (long long)Lx * Ly * Lz;
+  if (V3 <= 0) return;
+  if ((V3 + 255) / 256 > 0x7fffffffLL) throw std::runtime_error("launch_plaquette: grid too large");
+  hipLaunchKernelGGL(k_plaquette, blocks_for(V3), dim3(256), 0, s, U, Lx, Ly, Lz, partial);
+  SLAPH_HIP(hipGetLastError());
+}
 
 void launch_su3_random(double2 *U, long long nlinks, unsigned long long seed, double eps, hipStream_t s) {
   if (nlinks <= 0) return;

@@ -37,5 +37,12 @@ void launch_hyp_outer(const double2 *U, const double2 *dec, double2 *Uout, int L
 // backward ('<'): out(x) = U_mu(x - mu)^+ V(x - mu).
 void launch_displace(const double2 *U, const double2 *Vin, double2 *Vout, int Lx, int Ly, int Lz, int N, int mu,
                      int forward, hipStream_t s);
+// Spatial plaquette of a timeslice (GaugeField::plaque_ts, GaugeField.cpp:
+// 722-760): per 256-site block the sum over its sites and the six ordered
+// planes mu != nu of Re tr(U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+) / 3
+// into partial[block], plaquette_blocks(...) entries; the plaquette is their
+// sum / (6 Lx Ly Lz).
+int plaquette_blocks(int Lx, int Ly, int Lz);
+void launch_plaquette(const double2 *U, int Lx, int Ly, int Lz, double *partial, hipStream_t s);
 
 }  // namespace slaph

@@ -60,6 +60,9 @@ _SIGS = {
     "slc_synthetic_gauge": (_c.c_int, [_P, _c.c_ulonglong, _c.c_double]),
     "slc_smear_stout": (_c.c_int, [_P, _c.c_double, _c.c_int]),
     "slc_smear_hyp": (_c.c_int, [_P, _c.c_double, _c.c_double, _c.c_int]),
+    "slc_read_gauge_field": (_c.c_int, [_P, _c.c_char_p, _c.c_char_p, _c.c_int, _c.c_int, _c.c_int]),
+    "slc_plaquette": (_c.c_int, [_P, _c.c_int, _D]),
+    "slc_read_lime_gauge": (_c.c_int, [_c.c_char_p] + [_c.c_int] * 6 + [_D]),
     "slc_vdaggerv_filename": (_c.c_int, [_c.c_int, _c.c_char_p, _c.c_int, _I, _c.c_char_p, _c.c_int, _c.c_char_p,
                                          _c.c_int]),
     "slc_query_string": (_c.c_int, [_P, _c.c_char_p, _c.c_char_p, _c.c_int]),
@@ -128,6 +131,21 @@ def vdaggerv_filename(kind: str, path: str, config: int, mom, displacement: str 
     buf = _c.create_string_buffer(need)
     _lib.slc_vdaggerv_filename(*args, buf, need)
     return buf.value.decode()
+
+
+def read_lime_gauge(path: str, L, Lt: int, slice_i: int = 0, slice_f: int = None) -> np.ndarray:
+    """Links of timeslices slice_i..slice_f of a lime / ILDG file on the host,
+    [t - slice_i, site, mu, row, col] (site = (x Ly + y) Lz + z, mu = x, y, z),
+    by the library's reader (no context or device needed)."""
+    Lx, Ly, Lz = (int(v) for v in L)
+    if slice_f is None:
+        slice_f = Lt - 1
+    nt = max(0, int(slice_f) - int(slice_i) + 1)
+    out = np.zeros((max(1, nt), Lx * Ly * Lz, 3, 3, 3), dtype=np.complex128)
+    rc = _lib.slc_read_lime_gauge(path.encode(), Lx, Ly, Lz, int(Lt), int(slice_i), int(slice_f), _dptr(out.reshape(-1)))
+    if rc != 0:
+        raise SlaphError(_lib.slc_last_error(None).decode())
+    return out.transpose(0, 1, 2, 4, 3).copy()
 
 
 def dilution_text(num_slice: int, num_block: int, interlace: bool) -> str:
@@ -232,6 +250,22 @@ class Context:
 
     def synthetic_gauge(self, seed: int, eps: float):
         self._check(_lib.slc_synthetic_gauge(self._h, int(seed), float(eps)))
+
+    def read_gauge_field(self, config: int, slice_i: int = 0, slice_f: int = None, config_path: str = None,
+                         base_dir: str = "."):
+        """GaugeField::read_gauge_field: timeslices slice_i..slice_f (default
+        all) of <config_path>/conf.%04d (default: path_config of the infile)."""
+        if slice_f is None:
+            slice_f = self.query("Lt") - 1
+        self._check(_lib.slc_read_gauge_field(self._h, base_dir.encode(),
+                                              None if config_path is None else config_path.encode(),
+                                              int(config), int(slice_i), int(slice_f)))
+
+    def plaquette(self, t: int) -> float:
+        """GaugeField::plaque_ts of the stored links of timeslice t."""
+        out = _c.c_double()
+        self._check(_lib.slc_plaquette(self._h, int(t), _c.byref(out)))
+        return out.value
 
     def smear_stout(self, rho: float, iterations: int):
         self._check(_lib.slc_smear_stout(self._h, float(rho), int(iterations)))

@@ -6,6 +6,7 @@ Tolerances: gauge links and VdaggerV 1e-13 absolute (unitary links, unit
 columns); correlators 1e-12 relative to the largest magnitude of each
 correlator, the bar of the undisplaced tests.
 """
+import os
 import sys
 
 import numpy as np
@@ -280,3 +281,38 @@ def test_displaced_operators_without_links_raise(slaph, inputs):
     ctx = _context(slaph, inputs, links=False)
     with pytest.raises(slaph.SlaphError, match="gauge links"):
         slaph.OperatorFactory(ctx).create_operators()
+
+
+GAUGE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "l4", "gauge")
+
+
+def test_read_gauge_field_and_plaquette(slaph, tmp_path):
+    """read_gauge_field of the reference's lime file (tests/integration-L4/
+    gauge/conf.1000): the device links equal the oracle's reading bit for bit,
+    plaque_ts on the device matches the oracle's per timeslice (1e-14), a
+    partial slice range reads those timeslices only, and a file that fails
+    its checksum leaves the links unset."""
+    import shutil
+    say("read_gauge_field + plaquette")
+    want, _ = o.read_lime_gauge(os.path.join(GAUGE_DIR, "conf.1000"), L, 4)
+    ctx = slaph.Context(0)
+    ctx.configure(DISP.replace("Lz = 4", "Lz = 4\npath_config = " + GAUGE_DIR))
+    ctx.read_gauge_field(1000)
+    for t in range(4):
+        assert np.array_equal(ctx.get_gauge_links(t), want[t]), t
+        assert abs(ctx.plaquette(t) - o.plaquette_ts(want[t], L)) < 1e-14, t
+    ctx.set_gauge_links(2, np.broadcast_to(np.eye(3), (SITES, 3, 3, 3)))
+    assert abs(ctx.plaquette(2) - 1.0) < 1e-15
+    ctx.read_gauge_field(1000, slice_i=2, slice_f=2, config_path=GAUGE_DIR)
+    assert np.array_equal(ctx.get_gauge_links(2), want[2])
+    # a corrupted copy fails its SciDAC checksum
+    data = bytearray(open(os.path.join(GAUGE_DIR, "conf.1000"), "rb").read())
+    data[data.index(b"ildg-binary-data") + 128 + 5000] ^= 0x10  # payload = type string + 128 B
+    (tmp_path / "conf.1000").write_bytes(bytes(data))
+    with pytest.raises(slaph.SlaphError, match="checksum mismatch"):
+        ctx.read_gauge_field(1000, config_path=str(tmp_path))
+    with pytest.raises(slaph.SlaphError, match="gauge links are not set"):
+        ctx.plaquette(0)
+    shutil.copy(os.path.join(GAUGE_DIR, "conf.1000"), str(tmp_path / "conf.1001"))
+    ctx.read_gauge_field(1001, config_path=str(tmp_path))
+    assert np.array_equal(ctx.get_gauge_links(3), want[3])
