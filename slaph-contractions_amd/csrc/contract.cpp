@@ -275,6 +275,15 @@ struct Family {
   std::vector<GridGTile> gg_tiles;
   std::vector<int> gg_runs;
   DevBuf d_ggt, d_ggr, d_ggp;  // d_ggp: partial tiles of the K split
+  // Summed-product form of the crossed trace (see plan_xform): the product
+  // node's table holds the X and W matrices, the trace items are sub-items of
+  // the trace ids (several per trace id, one reduction each), xroot their
+  // operand offsets.
+  bool xform = false;
+  std::vector<STerm> xterms;
+  std::vector<SpUnit> xunits;
+  DevBuf d_xterms, d_xunits;
+  NodePlan xroot;
 };
 
 struct CorrPlan {
@@ -621,11 +630,27 @@ void plan_lstat(Family &f, NodePlan const &root, int D) {
           throw std::logic_error("queued trace plan: partial sums must follow slot order");
     }
   }
-  // (4) per item: its partial sums, in piece order
-  for (size_t k = 0; k < f.gitems.size(); ++k) {
-    GroupItem r{0, int(f.ls_ridx.size()), int(item_parts[k].size()), f.gitems[k].dst};
-    f.ls_ridx.insert(f.ls_ridx.end(), item_parts[k].begin(), item_parts[k].end());
-    f.ls_red.push_back(r);
+  // (4) per item: its partial sums, in piece order (in the summed-product
+  // form the items of one trace id share one reduction, in item order)
+  if (f.xform) {
+    std::map<long long, int> red_of;
+    std::vector<std::vector<int>> parts;
+    for (size_t k = 0; k < f.gitems.size(); ++k) {
+      auto it = red_of.emplace(f.gitems[k].dst, int(parts.size())).first;
+      if (it->second == int(parts.size())) parts.emplace_back();
+      parts[it->second].insert(parts[it->second].end(), item_parts[k].begin(), item_parts[k].end());
+    }
+    for (auto const &kv : red_of) {
+      auto const &pl = parts[kv.second];
+      f.ls_red.push_back(GroupItem{0, int(f.ls_ridx.size()), int(pl.size()), kv.first});
+      f.ls_ridx.insert(f.ls_ridx.end(), pl.begin(), pl.end());
+    }
+  } else {
+    for (size_t k = 0; k < f.gitems.size(); ++k) {
+      GroupItem r{0, int(f.ls_ridx.size()), int(item_parts[k].size()), f.gitems[k].dst};
+      f.ls_ridx.insert(f.ls_ridx.end(), item_parts[k].begin(), item_parts[k].end());
+      f.ls_red.push_back(r);
+    }
   }
   if (f.ls_npart > INT32_MAX) throw std::runtime_error("trace plan: too many partial sums");
   {
@@ -857,6 +882,193 @@ void plan_gridg(Family &f, NodePlan const &root) {
   f.gg_runs = runs;
   upload(f.d_ggt, f.gg_tiles);
   upload(f.d_ggr, f.gg_runs);
+}
+
+// Summed-product form of the crossed trace trQ2Q0Q2Q0 (DilutedTraceFactory.cpp:
+// 161-190) when its two products read the same table.  A trace id
+// (a, b, c, d) sums tr(Q0_a(r1 r2) Q2_b(r2 r3) Q0_c(r3 r4) Q2_d(r4 r1)) over
+// the random-vector chains that operator* and factor_to_trace admit
+// (DilutedFactor.cpp:6-51, DilutedTrace.cpp:84-138): factors exist for their
+// (first, second) vector pairs, r1 != r3 (both products) and r2 != r4 (the
+// used masks).  Summing r2 and r4 inside the products and removing r2 = r4 by
+// inclusion and exclusion,
+//   T = sum_{r1 != r3} tr(X_ab(r1 r3) X_cd(r3 r1))
+//       - sum_{r2} tr(Y_bc(r2) Y_da(r2)) + sum_{r2, r1} tr(V_bc(r2 r1) V_da(r2 r1))
+// with X_ab(r1 r3) = sum_{r2} Q0_a(r1 r2) Q2_b(r2 r3) over r1 != r3,
+// V_bc(r2 r1) = Q2_b(r2 r1) Q0_c(r1 r2) and Y_bc(r2) = sum_{r1} V_bc(r2 r1):
+// the second sum is every r2 = r4 chain (r1 = r3 included), the third puts
+// back those with r1 = r3.  The product node's table holds, per time key, the X
+// matrices of every product entry (a, b) and, per crossed pair (b, c) the
+// trace ids read, the V matrices of each r2 followed by i Y(r2), both
+// transposed (tr(A^T B^T) = tr(A B); the transpose puts Q0 on the left of
+// every product, see launch_sprod): i Y is stored on both sides, tr(iY iY) =
+// -tr(Y Y), so that every term is a plain trace of two table matrices.  The
+// X matrices are sums over r2, so the table has no per-r2 products to
+// write.  Each trace id becomes two items of the left-stationary
+// plan (X pairs, W pairs) that reduce into its one group sum.  The trace
+// reads 20 + 25 matrices per trace id instead of 60 (five random vectors), the
+// table 45 per entry instead of 60.
+void plan_xform(Family &f, std::vector<FactorIndex> const &q0l, std::vector<FactorIndex> const &q2l,
+                std::vector<long long> const &off0, std::vector<long long> const &off2) {
+  NodePlan &p = f.nodes[2];
+  NodePlan &root = f.nodes.back();
+  using Lab = std::pair<int, int>;
+  f.xterms.clear();
+  f.xunits.clear();
+  long long tot = 0;
+  long long maxq0 = 0, maxq2 = 0;
+  // Dirac block pattern of a Q0 entry: its gamma matrix's row of each column
+  auto perm_of = [&](int e) {
+    auto const &g = gamma_table().at(q0l.at(e).gamma.at(0));
+    int m = 0, seen = 0;
+    for (int d = 0; d < 4; ++d) {
+      m |= (g.row[d] & 3) << (2 * d);
+      seen |= 1 << (g.row[d] & 3);
+    }
+    if (seen != 15) throw std::logic_error("summed-product form: gamma rows are not a permutation");
+    return m;
+  };
+  // Runs of terms per Q2 operator entry (a unit of launch_sprod): kind 0 sums
+  // Q0[xo + x] Q2_b[y] into output o; kind 1 stores (Q2_b[x] Q0[xo + y])^T at
+  // o, o + 1, .. and i times their sum after them.
+  struct Run {
+    int kind, perm;
+    long long xo, o;
+    std::vector<int2> t;
+  };
+  std::map<int, std::vector<Run>> runs;  // Q2 entry -> its runs
+  auto add_job = [&](long long xo, int b, long long o, std::vector<int2> const &t, int kind, int perm) {
+    runs[b].push_back({kind, perm, xo, o, t});
+  };
+  // X part: one summed output per (r1, r3) of each product entry
+  std::vector<long long> xoff(p.qn_key.size());
+  std::vector<std::map<Lab, int>> xlab(p.qn_key.size());
+  for (size_t q = 0; q < p.qn_key.size(); ++q) {
+    const int a = p.qn_key[q].at(0), b = p.qn_key[q].at(1);
+    auto const &ra = q0l.at(a).rnd, &rb = q2l.at(b).rnd;
+    std::vector<Lab> order;
+    std::map<Lab, std::vector<int2>> terms;
+    for (size_t i = 0; i < ra.size(); ++i)
+      for (size_t j = 0; j < rb.size(); ++j)
+        if (ra[i].second == rb[j].first && ra[i].first != rb[j].second) {
+          const Lab l{ra[i].first, rb[j].second};
+          auto &t = terms[l];
+          if (t.empty()) order.push_back(l);
+          t.push_back(make_int2(int(i), int(j)));
+        }
+    xoff[q] = tot;
+    for (size_t k = 0; k < order.size(); ++k) {
+      add_job(off0.at(a), b, tot + (long long)k, terms[order[k]], 0, perm_of(a));
+      xlab[q][order[k]] = int(k);
+    }
+    tot += (long long)order.size();
+  }
+  // W part, per crossed pair (b, c): labels (r2, r1) for V, (r2, -1) for i Y
+  std::map<Lab, int> wid;
+  std::vector<long long> woff;
+  std::vector<std::map<Lab, int>> wlab;
+  auto wentry = [&](int b, int c) -> int {
+    auto it = wid.find({b, c});
+    if (it != wid.end()) return it->second;
+    auto const &rb = q2l.at(b).rnd, &rc = q0l.at(c).rnd;
+    std::vector<int> r2s;
+    std::map<int, std::vector<std::pair<int, int2>>> byr2;  // r2 -> (r1, (Q2 pair, Q0 pair))
+    for (size_t i = 0; i < rb.size(); ++i)
+      for (size_t j = 0; j < rc.size(); ++j)
+        if (rb[i].second == rc[j].first && rb[i].first == rc[j].second) {
+          auto &v = byr2[rb[i].first];
+          if (v.empty()) r2s.push_back(rb[i].first);
+          v.push_back({rb[i].second, make_int2(int(i), int(j))});
+        }
+    std::map<Lab, int> lab;
+    int k = 0;
+    for (int r2 : r2s) {
+      std::vector<int2> t;
+      const long long o = tot + k;
+      for (auto const &e : byr2[r2]) {
+        lab[{r2, e.first}] = k++;
+        t.push_back(e.second);
+      }
+      lab[{r2, -1}] = k++;
+      add_job(off0.at(c), b, o, t, 1, perm_of(c));
+    }
+    woff.push_back(tot);
+    wlab.push_back(lab);
+    tot += k;
+    return wid[{b, c}] = int(woff.size()) - 1;
+  };
+  // items: per trace id (mirrors reuse their partner's sum) an X item and a W item
+  std::vector<GroupItem> items;
+  std::vector<int2> pairs;
+  f.xroot = NodePlan();
+  auto item = [&](long long lo, long long ro, std::map<Lab, int> const &L, std::map<Lab, int> const &R, bool swap,
+                  long long dst) {
+    const int off = int(pairs.size());
+    for (auto const &kv : L) {
+      auto it = R.find(swap ? Lab{kv.first.second, kv.first.first} : kv.first);
+      if (it != R.end()) pairs.push_back(make_int2(kv.second, it->second));
+    }
+    const int n = int(pairs.size()) - off;
+    if (n == 0) return;
+    items.push_back({int(f.xroot.h_Xq.size()), off, n, dst});
+    f.xroot.h_Xq.push_back(lo);
+    f.xroot.h_Yq.push_back(ro);
+  };
+  for (size_t trl = 0; trl < f.tr_ids.size(); ++trl) {
+    if (f.mirror_of[trl] >= 0) continue;
+    if (f.ngroup[trl] != 1) throw std::logic_error("summed-product form: a trace with several mask groups");
+    const int q = root.trl_qn[trl], ql = root.qn_left[q], qr = root.qn_right[q];
+    const long long dst = f.goff[trl];
+    item(xoff[ql], xoff[qr], xlab[ql], xlab[qr], true, dst);
+    const int a = p.qn_key[ql].at(0), b = p.qn_key[ql].at(1), c = p.qn_key[qr].at(0), d = p.qn_key[qr].at(1);
+    const int wl = wentry(b, c), wr = wentry(d, a);
+    item(woff[wl], woff[wr], wlab[wl], wlab[wr], false, dst);
+  }
+  // units: per Q2 entry its runs, dealt to the four waves in about equal term counts
+  for (auto &kv : runs) {
+    const int b = kv.first;
+    SpUnit un;
+    un.s0 = int(off2.at(b));
+    un.np = int(q2l.at(b).rnd.size());
+    if (un.np > kSpStage) throw std::logic_error("summed-product form: Q2 entry with too many random-vector pairs");
+    maxq2 = std::max(maxq2, off2.at(b) + un.np - 1);
+    long long nterm = 0;
+    for (auto const &rn : kv.second) nterm += (long long)rn.t.size();
+    long long done = 0;
+    int w = 0;
+    un.tb[0] = int(f.xterms.size());
+    for (auto const &rn : kv.second) {
+      while (w < 3 && done * 4 >= nterm * (w + 1)) un.tb[++w] = int(f.xterms.size());
+      for (size_t k = 0; k < rn.t.size(); ++k) {
+        const long long q0i = rn.xo + (rn.kind ? rn.t[k].y : rn.t[k].x);
+        const int q2i = rn.kind ? rn.t[k].x : rn.t[k].y;
+        const long long oo = rn.kind ? rn.o + (long long)k : rn.o;
+        if (q0i > INT32_MAX || oo + 1 > INT32_MAX) throw std::runtime_error("summed-product plan: index overflows");
+        if (q2i < 0 || q2i >= un.np) throw std::logic_error("summed-product plan: Q2 pair out of range");
+        maxq0 = std::max(maxq0, q0i);
+        STerm st;
+        st.l = int(q0i);
+        st.r = q2i;
+        st.o = int(oo);
+        st.flags = (k + 1 == rn.t.size() ? 1 : 0) | (rn.kind ? 2 : 0) | (rn.perm << 8);
+        f.xterms.push_back(st);
+      }
+      done += (long long)rn.t.size();
+    }
+    while (w < 4) un.tb[++w] = int(f.xterms.size());
+    f.xunits.push_back(un);
+  }
+  if (tot == 0 || items.empty() || f.xterms.empty()) throw std::logic_error("summed-product form: empty plan");
+  f.gitems = std::move(items);
+  f.gpairs = std::move(pairs);
+  // the product node's table is now the X and W matrices of a time key
+  p.stride = tot;
+  p.cmaxO = tot - 1;
+  p.cmaxX = maxq0;
+  p.cmaxY = maxq2;
+  root.cmaxX = root.cmaxY = tot - 1;
+  upload(f.d_xterms, f.xterms);
+  upload(f.d_xunits, f.xunits);
 }
 
 // Every (factor type, time key) a leaf of this family reads at slice pair
@@ -1326,6 +1538,24 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
         off += (long long)gm.size();
       }
       f.gstride = off;
+      // the summed-product form (plan_xform): D = 16, the crossed trace with
+      // one table for both products, every trace id read by single-trace
+      // correlators only, the left-stationary kernel (not the per-element,
+      // group-kernel or restricted forms); SLAPH_XFORM_OFF=1 (or the tuning
+      // knob) keeps the product-per-combination form, a cross-check
+      f.xform = f.name == "trQ2Q0Q2Q0" && f.coincide && D_ == 16 && !per_element && !scope &&
+                f.shape.nodes.size() == 7 && f.nodes[5].owner == 2 && !std::getenv("SLAPH_XFORM_OFF") &&
+                !tune_.xform_off && !std::getenv("SLAPH_TRACE_GROUPS") && trace_lstat_qchunk(D_) > 0 &&
+                std::none_of(multi[fi].begin(), multi[fi].end(), [](char m) { return m != 0; });
+      if (f.xform) {
+        plan_xform(f, gd_.quarkline_lookuptable.at("Q0"), gd_.quarkline_lookuptable.at("Q2"), q_off_[0], q_off_[2]);
+        order_items_by_component(f, f.xroot);
+        upload(f.d_gitems, f.gitems);
+        upload(f.d_gpairs, f.gpairs);
+        plan_lstat(f, f.xroot, D_);
+        if (f.ls_blocks.empty()) throw std::logic_error("summed-product form without a left-stationary trace plan");
+        continue;
+      }
       order_items_by_component(f, root);
       upload(f.d_gitems, f.gitems);
       upload(f.d_gpairs, f.gpairs);
@@ -1748,23 +1978,29 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
           const int rt = is_t(ns.right) ? 1 : 0;
           if (is_t(ns.left) || (!root && rt)) throw std::logic_error("transposed product table read as a factor");
           const long long gb = ch.gbase[fi];
-          if (!root) {
+          // the summed-product form never takes the slice-pair tile kernels
+          const bool grid = nc.grid && !f.xform;
+          if (!root && f.xform) {
+            launch_sprod(slab_of(0), nc.d_Xt.as<long long>(), slab_of(2), nc.d_Yt.as<long long>(), C.d_prod.as<void>(),
+                         nc.base, np.stride, f.d_xterms.as<STerm>(), f.d_xunits.as<SpUnit>(), int(f.xunits.size()),
+                         nc.ntime, f32, stream_);
+          } else if (!root) {
             launch_prod(src(ns.left), nc.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
                         nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), C.d_prod.as<void>(), nc.base, np.stride,
                         np.d_combos.as<ComboProd>(), int(np.combos.size()), nc.ntime, D_, np.transposed ? 1 : 0, f32,
                         tune_.launch.prod_generic, stream_);
-          } else if (nc.grid && !f.gg_tiles.empty()) {
+          } else if (grid && !f.gg_tiles.empty()) {
             const int ntile = int(f.gg_tiles.size()), nrun = int(f.gg_runs.size() / 3);
             const int kp = trace_gemm_parts(ntile, nrun, tune_.launch.gemm_wg_per_cu);
             if (kp > 1) f.d_ggp.ensure(sizeof(double2) * (size_t)ntile * kp * 4096);
             launch_trace_gemm(src(ns.left), src(ns.right), f.d_ggt.as<GridGTile>(), ntile, f.d_ggr.as<int>(), nrun,
                               nc.d_tile.as<GridTile>(), C.d_gv.as<double2>(), gb, f.gstride, D_, rt, f32, kp,
                               kp > 1 ? f.d_ggp.as<double2>() : nullptr, stream_);
-          } else if (nc.grid && !f.gr_groups.empty()) {
+          } else if (grid && !f.gr_groups.empty()) {
             launch_trace_grid_r(src(ns.left), src(ns.right), f.d_grg.as<GridRGroup>(), int(f.gr_groups.size()),
                                 f.d_grr.as<int>(), nc.d_tile.as<GridTile>(), 1, C.d_gv.as<double2>(), gb, f.gstride,
                                 D_, rt, f32, stream_);
-          } else if (nc.grid) {
+          } else if (grid) {
             launch_trace_grid(src(ns.left), np.d_Xq.as<long long>(), src(ns.right), np.d_Yq.as<long long>(),
                               f.d_gitems.as<GroupItem>(), int(f.gitems.size()), f.d_gpairs.as<int2>(),
                               nc.d_tile.as<GridTile>(), 1, C.d_gv.as<double2>(), gb, f.gstride, D_, rt, f32, stream_);
@@ -1797,6 +2033,7 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
                              int(f.ls_red.size()), f.d_lsri.as<int>(), C.d_gv.as<double2>(), gb, f.gstride, nc.ntime,
                              stream_);
           } else {
+            if (f.xform) throw std::logic_error("summed-product form needs the left-stationary traces");
             launch_trace_groups(src(ns.left), nc.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
                                 nc.d_Yt.as<long long>(), np.d_Yq.as<long long>(), f.d_gitems.as<GroupItem>(),
                                 int(f.gitems.size()), f.d_gpairs.as<int2>(), C.d_gv.as<double2>(), gb, f.gstride,

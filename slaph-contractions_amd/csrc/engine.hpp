@@ -185,6 +185,7 @@ class Engine {
     LaunchTuning launch;
     int vdv_direct = 0;      // VdaggerV by the direct product instead of the sitewise Fourier form
     int resolve_groups = 0;  // masked inner products by direct group loops instead of the Moebius form
+    int xform_off = 0;       // crossed four-factor traces from one product per combination, not summed products
   };
   Tuning const &tuning() const { return tune_; }
   std::string tuning_text() const;

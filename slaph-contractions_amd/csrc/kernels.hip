@@ -1359,6 +1359,140 @@ __global__ __launch_bounds__(256) void k_prod16(const T *__restrict__ sX, const 
   }
 }
 
+// Summed products of the crossed four-factor trace (D = 16).  Q0 has one
+// nonzero 4 x 4 Dirac block per block row and column, so a product with Q0
+// on the left is sixteen 4 x 4 x 4 block products: four v_mfma_f64_4x4x4f64
+// (four independent blocks each, the FP64 rate of the 16 x 16 x 4 form) per
+// real product instead of four 16 x 16 x 4, three real products per complex
+// one (Gauss).  Both kinds have Q0 on the left (kind 1 forms (Q2 Q0)^T =
+// Q0^T Q2^T).  Lane l = 16 q + 4 g + r runs block slot g of every instruction
+// (operand elements A[i][k] = lane i + 16 k + 4 g, B[k][j] = lane 16 k + j +
+// 4 g, D[i][j] = lane 16 i + j + 4 g); instruction n computes output column
+// block n from output row block g: C[g][n] = F[g][c] S[c][n] with F the Q0
+// side (one nonzero block per row block, in column block c) and S the Q2 side.
+// The lane's F element is the same for all n, so each lane loads one of the
+// 64 nonzeros per term (a ring of kSpRing terms in flight), and four S
+// elements from the unit's Q2 matrices in LDS (staged once per unit: every
+// product of the unit reads them).  Instruction n's results are elements
+// 64 n + (4 g + q + 16 r); before storing, lane l takes element 64 n + l from
+// lane 16 (l & 3) + 4 ((l >> 2) & 3) + (l >> 4) (a swap of the 2-bit fields q
+// and r), so that every store writes one KiB in lane order.  XCD x runs the
+// x-th contiguous eighth of the (time, unit) sequence, so that the units of
+// one time key read its Q0 slab through one L2.
+constexpr int kSpRing = 4;
+// LDS slot of element e of a staged matrix: the low nibble XORed with the
+// column's low two bits times 4, which spreads the four lanes r of one (q, g)
+// over distinct banks
+__device__ __forceinline__ int sp_swz(int e) { return e ^ (((e >> 4) & 3) << 2); }
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sprod16(
+    const T *__restrict__ sQ0, const long long *__restrict__ Xt, const T *__restrict__ sQ2,
+    const long long *__restrict__ Yt, T *__restrict__ out, long long obase, long long ostride,
+    const STerm *__restrict__ terms, const SpUnit *__restrict__ units, int nunit) {
+  __shared__ double2 S[kSpStage * 256];
+  const int NB = gridDim.x * gridDim.y, lin = blockIdx.y * gridDim.x + blockIdx.x;
+  const int per = NB / 8, rem = NB % 8, xcd = lin % 8, jx = lin / 8;
+  const int lb = xcd < rem ? xcd * (per + 1) + jx : rem * (per + 1) + (xcd - rem) * per + jx;
+  const int ti = lb / gridDim.x, u = lb % gridDim.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const SpUnit U = units[u];
+  {
+    const T *sb = sQ2 + (Yt[ti] + U.s0) * 256;
+    for (int e = tid; e < U.np * 256; e += 256) S[(e & ~255) + sp_swz(e & 255)] = ld2(sb + e);
+  }
+  __syncthreads();
+  const int t0 = U.tb[wave], t1 = U.tb[wave + 1];
+  const T *q0 = sQ0 + Xt[ti] * 256;
+  T *ob = out + (obase + ti * ostride) * 256;
+  const int q = lane >> 4, g = (lane >> 2) & 3, r = lane & 3;
+  // the column block c of F's nonzero block in row block g
+  auto fblock = [&](int flags) {
+    const int perm = flags >> 8;
+    if (flags & 2) return (perm >> (2 * g)) & 3;  // F = Q0^T: Q0's block in column block g
+    int c = 0;
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+      if (((perm >> (2 * d)) & 3) == g) c = d;
+    return c;
+  };
+  double2 fa[kSpRing];
+  auto load = [&](int k, int t) {
+    const STerm tm = terms[t];
+    const int c = fblock(tm.flags);
+    const int e = tm.flags & 2 ? (4 * c + q) + 16 * (4 * g + r) : (4 * g + r) + 16 * (4 * c + q);
+    fa[k] = ld2(q0 + (long long)tm.l * 256 + e);
+  };
+#pragma unroll
+  for (int k = 0; k < kSpRing; ++k)
+    if (t0 + k < t1) load(k, t0 + k);
+  double p1[4], p2[4], p3[4];
+  double2 sum[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    p1[n] = p2[n] = p3[n] = 0.0;
+    sum[n] = make_double2(0.0, 0.0);
+  }
+  const int src = 4 * (16 * (lane & 3) + 4 * ((lane >> 2) & 3) + (lane >> 4));
+  auto perm2 = [&](double2 v) {
+    const int a = __builtin_amdgcn_ds_bpermute(src, __double2loint(v.x));
+    const int b = __builtin_amdgcn_ds_bpermute(src, __double2hiint(v.x));
+    const int c = __builtin_amdgcn_ds_bpermute(src, __double2loint(v.y));
+    const int d = __builtin_amdgcn_ds_bpermute(src, __double2hiint(v.y));
+    return make_double2(__hiloint2double(b, a), __hiloint2double(d, c));
+  };
+  for (int t = t0; t < t1; t += kSpRing) {
+#pragma unroll
+    for (int k = 0; k < kSpRing; ++k) {
+      const int tt = t + k;
+      if (tt < t1) {
+        const STerm tm = terms[tt];
+        const double2 fc = fa[k];
+        if (tt + kSpRing < t1) load(k, tt + kSpRing);
+        const bool k1 = tm.flags & 2, last = tm.flags & 1;
+        const int c = fblock(tm.flags);
+        const double2 *sm = S + tm.r * 256;
+        double2 sc[4];
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+          sc[n] = sm[sp_swz(k1 ? (4 * n + r) + 16 * (4 * c + q) : (4 * c + q) + 16 * (4 * n + r))];
+        if (k1) {
+#pragma unroll
+          for (int n = 0; n < 4; ++n) p1[n] = p2[n] = p3[n] = 0.0;
+        }
+        const double fs = fc.x + fc.y;
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+          p1[n] = __builtin_amdgcn_mfma_f64_4x4x4f64(fc.x, sc[n].x, p1[n], 0, 0, 0);
+          p2[n] = __builtin_amdgcn_mfma_f64_4x4x4f64(fc.y, sc[n].y, p2[n], 0, 0, 0);
+          p3[n] = __builtin_amdgcn_mfma_f64_4x4x4f64(fs, sc[n].x + sc[n].y, p3[n], 0, 0, 0);
+        }
+        T *o = ob + (long long)tm.o * 256 + lane;
+        if (k1) {
+#pragma unroll
+          for (int n = 0; n < 4; ++n) {
+            const double2 v = make_double2(p1[n] - p2[n], p3[n] - p1[n] - p2[n]);
+            st2_nt(o + 64 * n, perm2(v));
+            sum[n] = cadd(sum[n], v);
+          }
+          if (last) {
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+              st2_nt(o + 256 + 64 * n, perm2(make_double2(-sum[n].y, sum[n].x)));
+              sum[n] = make_double2(0.0, 0.0);
+            }
+          }
+        } else if (last) {
+#pragma unroll
+          for (int n = 0; n < 4; ++n) {
+            st2_nt(o + 64 * n, perm2(make_double2(p1[n] - p2[n], p3[n] - p1[n] - p2[n])));
+            p1[n] = p2[n] = p3[n] = 0.0;
+          }
+        }
+      }
+    }
+  }
+}
+
 // D = 24 and D = 32 factor products on FP64 MFMA: one wavefront per
 // product, the D x D output as 2 x 2 tiles of 16 x 16 (rows and columns past
 // D of the D = 24 case are zero operands and are not stored), D / 4 k-steps of
@@ -2825,6 +2959,20 @@ void launch_prod(const void *sX, const long long *Xt, const long long *Xq, const
   with_storage(f32, [&](auto tag) {
     launch_prod_t<decltype(tag)>(sX, Xt, Xq, sY, Yt, Yq, out, obase, ostride, combos, ncombo, ntime, D, transposed,
                                  generic, s);
+  });
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_sprod(const void *sQ0, const long long *Xt, const void *sQ2, const long long *Yt, void *out,
+                  long long obase, long long ostride, const STerm *terms, const SpUnit *units, int nunit, int ntime,
+                  int f32, hipStream_t s) {
+  if (nunit <= 0 || ntime <= 0) return;
+  if (ntime > 65535) throw std::runtime_error("launch_sprod: too many time keys");
+  if ((long long)nunit * ntime > 0x7fffffffLL) throw std::runtime_error("launch_sprod: grid too large");
+  with_storage(f32, [&](auto tag) {
+    using T = decltype(tag);
+    hipLaunchKernelGGL(k_sprod16<T>, dim3(nunit, ntime), dim3(256), 0, s, static_cast<const T *>(sQ0), Xt,
+                       static_cast<const T *>(sQ2), Yt, static_cast<T *>(out), obase, ostride, terms, units, nunit);
   });
   SLAPH_HIP(hipGetLastError());
 }

@@ -201,6 +201,31 @@ struct ComboProd {
 void launch_prod(const void *slabX, const long long *Xt, const long long *Xq, const void *slabY, const long long *Yt,
                  const long long *Yq, void *out, long long obase, long long ostride, const ComboProd *combos,
                  int ncombo, int ntime, int D, int transposed, int f32, int generic, hipStream_t s);
+// Summed products of the crossed four-factor trace tr(Q0 Q2 Q0 Q2) (D = 16).
+// A unit is (time key, one Q2 operator entry): its np pair matrices
+// Q2[Yt + s0 .. + np) are staged in LDS, and its terms multiply one of them
+// with a Q0 matrix:
+//  kind 0: C = Q0[Xt + l] Q2[s0 + r], summed over the consecutive terms of
+//          one output o (the last has bit 0 of flags set) and stored once;
+//  kind 1: C = (Q2[s0 + r] Q0[Xt + l])^T = Q0^T Q2^T, stored at o; after the
+//          last term of a run (bit 0) i times the run's sum is stored at o + 1;
+// with Q0 / Q2 the factor slabs at the time bases Xt[ti] / Yt[ti], output
+// matrix index obase + ti * ostride + o, column-major like launch_prod's.
+// A Q0 factor is nonzero only in its Dirac blocks (row block perm(d), column
+// block d) (DilutedFactorFactory.cpp:70-90): bits 8 + 2d of flags hold
+// perm(d), the gamma matrix's row of column d; bit 1 is the kind.  Wave w of a
+// unit runs terms [tb[w], tb[w + 1]), whole runs each.
+constexpr int kSpStage = 20;  // largest np (80 KiB of LDS)
+struct STerm {
+  int l, r, o, flags;
+};
+struct SpUnit {
+  int s0, np;
+  int tb[5];
+};
+void launch_sprod(const void *slabQ0, const long long *Xt, const void *slabQ2, const long long *Yt, void *out,
+                  long long obase, long long ostride, const STerm *terms, const SpUnit *units, int nunit, int ntime,
+                  int f32, hipStream_t s);
 // In the trace launchers below, rt != 0 means the right matrices are stored
 // transposed (Y^T), so that tr(X Y) = sum_e X[e] Y^T[e]; the operand slabs
 // are double2 or (f32 != 0) float2 arrays, the group sums always double2.

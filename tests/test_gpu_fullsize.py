@@ -14,8 +14,9 @@ Correlators.cpp:87-90) with a reduced operator list against the oracle's
 contract.  The full correlator sets of the production kernels are compared
 with those of the alternative kernels (block-form and item-wise traces;
 direct group loops for the masked inner products, generic products and the
-direct VdaggerV), which share no trace, product, resolve or VdaggerV kernel
-with them, at 1e-12 relative."""
+direct VdaggerV, one product per combination for the crossed traces instead
+of the summed-product form), which share no trace, product, resolve or
+VdaggerV kernel with them, at 1e-12 relative."""
 import sys
 
 import numpy as np
@@ -28,7 +29,7 @@ pytestmark = pytest.mark.gpu
 
 SEED = 20261203
 SWITCHES = ("SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS", "SLAPH_RESOLVE_GROUPS", "SLAPH_PROD_GENERIC",
-            "SLAPH_VDV_DIRECT", "SLAPH_TRACE_GROUPS", "SLAPH_LSTAT_WINDOW", "SLAPH_GRID_GROUPS")
+            "SLAPH_VDV_DIRECT", "SLAPH_TRACE_GROUPS", "SLAPH_LSTAT_WINDOW", "SLAPH_GRID_GROUPS", "SLAPH_XFORM_OFF")
 COUNTS = {"pipi32": 58489}
 
 
@@ -83,13 +84,15 @@ def test_production_and_alternative_kernels_agree(slaph, workload, monkeypatch):
     (a) the block-form left-stationary and item-wise tile trace kernels and
     (b) direct group loops for the masked inner products (no Moebius
     inversion: at pipi32 this checks the 16-term subset sums of all 43,561
-    C6cD correlators), the generic product kernel and the direct VdaggerV."""
+    C6cD correlators), the generic product kernel, the direct VdaggerV and
+    the crossed traces (C4cC) from one product per combination instead of
+    the summed products with inclusion and exclusion."""
     text = bench.infile(bench.WORKLOADS[workload])
     _set_switches(monkeypatch, ())
     say(workload, "production kernels")
     prod = _correlators(slaph, text)
     for alt in (("SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS"),
-                ("SLAPH_RESOLVE_GROUPS", "SLAPH_PROD_GENERIC", "SLAPH_VDV_DIRECT")):
+                ("SLAPH_RESOLVE_GROUPS", "SLAPH_PROD_GENERIC", "SLAPH_VDV_DIRECT", "SLAPH_XFORM_OFF")):
         _set_switches(monkeypatch, alt)
         say(workload, "alternative kernels", "+".join(alt))
         n, worst = _agree(prod, _correlators(slaph, text))

@@ -232,8 +232,10 @@ WIDE = SMALL.replace("C4cB:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0\n", "C4cB:Q0:Op0:Q0:Op
 
 
 @pytest.mark.parametrize("infile,kernels", [(SMALL, "lstat+mfma"), (WIDE, "lstat+mfma"), (WIDE, "groups+generic"),
-                                            (WIDE, "blocks+items"), (WIDE, "lstat+mfma+window1")],
-                         ids=["small", "wide", "wide-alt-kernels", "wide-block-traces", "wide-one-round-windows"])
+                                            (WIDE, "blocks+items"), (WIDE, "lstat+mfma+window1"),
+                                            (WIDE, "lstat+mfma+crossed_direct")],
+                         ids=["small", "wide", "wide-alt-kernels", "wide-block-traces", "wide-one-round-windows",
+                              "wide-crossed-direct"])
 def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypatch):
     """Config-1-like lattice (4^3 x 8, N=8, dilE 4, 4 random vectors) with
     seeded host inputs: every correlator against the oracle at 1e-12
@@ -248,12 +250,17 @@ def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypat
     queued form's task descriptors at every round of its ring (the largest
     units of the 32^3 x 64 workload fit one window, so only this variant
     runs that reload) and runs the slice-pair tiles grouped by right operand
-    instead of as GEMMs."""
+    instead of as GEMMs.  The crossed traces (C4cC) run in the summed-product
+    form (products summed over the middle random vector, the coincident
+    chains removed by inclusion and exclusion) except in the crossed-direct
+    variant (one product per combination) and the alternative-kernels one
+    (one wave per group, which the summed form does not use)."""
     say("small pipi", "wide" if infile is WIDE else "", kernels)
     for var, on in (("SLAPH_TRACE_GROUPS", "groups" in kernels), ("SLAPH_PROD_GENERIC", "generic" in kernels),
                     ("SLAPH_RESOLVE_GROUPS", "groups" in kernels), ("SLAPH_VDV_DIRECT", "groups" in kernels),
                     ("SLAPH_LSTAT_BLOCKS", "blocks" in kernels), ("SLAPH_GRID_ITEMS", "items" in kernels),
-                    ("SLAPH_LSTAT_WINDOW", "window1" in kernels), ("SLAPH_GRID_GROUPS", "window1" in kernels)):
+                    ("SLAPH_LSTAT_WINDOW", "window1" in kernels), ("SLAPH_GRID_GROUPS", "window1" in kernels),
+                    ("SLAPH_XFORM_OFF", "crossed_direct" in kernels)):
         if on:
             monkeypatch.setenv(var, "1")
         else:
@@ -317,7 +324,8 @@ correlator_list = C4cC:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0,1
 _DIL_WANT = {}  # (dilE, N) -> (oracle correlators, inputs)
 
 KERNEL_SWITCHES = ("SLAPH_TRACE_GROUPS", "SLAPH_PROD_GENERIC", "SLAPH_RESOLVE_GROUPS", "SLAPH_VDV_DIRECT",
-                   "SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS", "SLAPH_LSTAT_WINDOW", "SLAPH_GRID_GROUPS")
+                   "SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS", "SLAPH_LSTAT_WINDOW", "SLAPH_GRID_GROUPS",
+                   "SLAPH_XFORM_OFF")
 
 
 @pytest.mark.parametrize("dilE,N", [(6, 14), (6, 16), (8, 20)], ids=["EI6-D24", "EI6-D24-N16", "EI8-D32"])
@@ -563,13 +571,14 @@ def test_autotune_keeps_correlators_and_tuning_text(slaph):
     assert "lstat_wg_per_cu=2" in default and "vdaggerv_direct=0" in default
     rep = ctx.autotune(1)
     knobs = {k["knob"] for k in rep["knobs"]}
-    assert knobs == {"vdaggerv_direct", "lstat_wg_per_cu", "gemm_wg_per_cu", "prod_generic", "resolve_groups"}
+    assert knobs == {"vdaggerv_direct", "lstat_wg_per_cu", "gemm_wg_per_cu", "prod_generic", "resolve_groups",
+                     "crossed_direct"}
     assert all(len(k["ms"]) >= 2 and all(v > 0 for v in k["ms"].values()) for k in rep["knobs"])
     assert rep["chosen"] == ctx.tuning()
     slaph.contract(ctx)
     assert _rel_errors({d: ctx.correlators(d) for d in ctx.diagrams()}, ref).max() <= 1e-12
     for text in ("vdaggerv_direct=1", "lstat_wg_per_cu=1", "lstat_wg_per_cu=3", "gemm_wg_per_cu=8",
-                 "prod_generic=1", "resolve_groups=1"):
+                 "prod_generic=1", "resolve_groups=1", "crossed_direct=1"):
         ctx.set_tuning(default)
         ctx.set_tuning(text)
         ops.create_operators()
