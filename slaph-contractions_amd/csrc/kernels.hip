@@ -1575,7 +1575,7 @@ __global__ __launch_bounds__(256) void k_prod_mfma(const T *__restrict__ sX, con
 // of the run's slot; the copies are summed in wave order at the end, so the
 // summation order is fixed by the plan.
 // ---------------------------------------------------------------------------
-constexpr int kLsStageBytes = 48 * 1024;
+constexpr int kLsStageBytes = 52 * 1024;
 // LDS slot of element e of a staged matrix: the low four bits of the 16-byte
 // slot index are XORed with the next four, so that both the lane-contiguous
 // reads and the transposing writes (stride D slots) hit distinct banks.
