@@ -1314,17 +1314,17 @@ __global__ __launch_bounds__(256) void k_prod16(const T *__restrict__ sX, const 
   // the right operand of the next combination is loaded while the current
   // one is multiplied and stored
   ComboProd cn = combos[c0];
-  double2 yn[4];
+  T yn[4];  // in the storage form, unpacked at use
   {
     const T *y = sY + (yt + Yq[cn.qn] + cn.j) * 256 + yoff;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) yn[s] = ld2(y + 4 * s);
+    for (int s = 0; s < 4; ++s) yn[s] = y[4 * s];
   }
   for (int c = c0; c < c1; ++c) {
     const ComboProd cb = cn;
     double2 yb[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) yb[s] = yn[s];
+    for (int s = 0; s < 4; ++s) yb[s] = ld2(&yn[s]);
     const long long xi = xt + Xq[cb.qn] + cb.i;
     if (xi != xcur) {
 #pragma unroll
@@ -1335,7 +1335,7 @@ __global__ __launch_bounds__(256) void k_prod16(const T *__restrict__ sX, const 
       cn = combos[c + 1];
       const T *y = sY + (yt + Yq[cn.qn] + cn.j) * 256 + yoff;
 #pragma unroll
-      for (int s = 0; s < 4; ++s) yn[s] = ld2(y + 4 * s);
+      for (int s = 0; s < 4; ++s) yn[s] = y[4 * s];
     }
     // C[r][c] = sum_k X[r][k] Y[k][c] from three real products:
     // P1 = Xr Yr, P2 = Xi Yi, P3 = (Xr + Xi)(Yr + Yi); re = P1 - P2, im = P3 - P1 - P2
@@ -1415,12 +1415,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       if (((perm >> (2 * d)) & 3) == g) c = d;
     return c;
   };
-  double2 fa[kSpRing];
+  T fa[kSpRing];  // in the storage form, unpacked at use
   auto load = [&](int k, int t) {
     const STerm tm = terms[t];
     const int c = fblock(tm.flags);
     const int e = tm.flags & 2 ? (4 * c + q) + 16 * (4 * g + r) : (4 * g + r) + 16 * (4 * c + q);
-    fa[k] = ld2(q0 + (long long)tm.l * 256 + e);
+    fa[k] = q0[(long long)tm.l * 256 + e];
   };
 #pragma unroll
   for (int k = 0; k < kSpRing; ++k)
@@ -1446,7 +1446,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       const int tt = t + k;
       if (tt < t1) {
         const STerm tm = terms[tt];
-        const double2 fc = fa[k];
+        const double2 fc = ld2(&fa[k]);
         if (tt + kSpRing < t1) load(k, tt + kSpRing);
         const bool k1 = tm.flags & 2, last = tm.flags & 1;
         const int c = fblock(tm.flags);
@@ -1655,14 +1655,16 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3
     }
   };
   fetch(0);
-  double2 r[PF][EPL];
+  // ring of right matrices in the storage form, unpacked at use (an unpack
+  // right after the load would wait for it there)
+  TS r[PF][EPL];
   if (ns > 0) {
     batch();
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       const long long y = __builtin_amdgcn_readlane(qy, min(u, ns - 1));
 #pragma unroll
-      for (int k = 0; k < EPL; ++k) r[u][k] = ld2(yb + y * DD + lane + 64 * k);
+      for (int k = 0; k < EPL; ++k) r[u][k] = yb[y * DD + lane + 64 * k];
     }
   }
   double re = 0.0, im = 0.0;
@@ -1698,7 +1700,7 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3
           }
 #pragma unroll
           for (int k = 0; k < EPL; ++k) {
-            const double2 v = r[u][k];
+            const double2 v = ld2(&r[u][k]);
             re = fma(w[k].x, v.x, re);
             re = fma(-w[k].y, v.y, re);
             im = fma(w[k].x, v.y, im);
@@ -1717,7 +1719,7 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(3
         // stream's end: its last task again, so that every load is issued)
         const long long y = __builtin_amdgcn_readlane(qy, min(pos + u + PF, ns - 1) - bb);
 #pragma unroll
-        for (int k = 0; k < EPL; ++k) r[u][k] = ld2(yb + y * DD + lane + 64 * k);
+        for (int k = 0; k < EPL; ++k) r[u][k] = yb[y * DD + lane + 64 * k];
       }
     }
   }
@@ -1848,14 +1850,16 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(2
   fetch(ud, xt);
   int qy, qab, qs;
   batch(ud, 0, qy, qab, qs);
-  double2 r[PF][EPL];
+  // ring of right matrices in the storage form, unpacked at use (an unpack
+  // right after the load would wait for it there)
+  TS r[PF][EPL];
   {
     const TS *yb = ybase(yt);
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       const long long y = __builtin_amdgcn_readlane(qy, u);
 #pragma unroll
-      for (int k = 0; k < EPL; ++k) r[u][k] = ld2(yb + y * DD + lane + 64 * k);
+      for (int k = 0; k < EPL; ++k) r[u][k] = yb[y * DD + lane + 64 * k];
     }
   }
   int pti = 0, pso = 0, pns = 0;  // the previous unit's slots, summed at the next unit's start
@@ -1925,7 +1929,7 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(2
           }
 #pragma unroll
           for (int k = 0; k < EPL; ++k) {
-            const double2 v = r[u][k];
+            const double2 v = ld2(&r[u][k]);
             re = fma(w[k].x, v.x, re);
             re = fma(-w[k].y, v.y, re);
             im = fma(w[k].x, v.y, im);
@@ -1948,7 +1952,7 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(2
             p < ns ? __builtin_amdgcn_readlane(qy, p - bb)
                    : (p - ns < nsn ? __builtin_amdgcn_readlane(udn, kLsUnitHdr + wave * PF + min(p - ns, PF - 1)) : 0);
 #pragma unroll
-        for (int k = 0; k < EPL; ++k) r[u][k] = ld2(base + y * DD + lane + 64 * k);
+        for (int k = 0; k < EPL; ++k) r[u][k] = base[y * DD + lane + 64 * k];
       }
       }
     }
@@ -1958,7 +1962,7 @@ __global__ __launch_bounds__(64 * kLsWaves) __attribute__((amdgpu_waves_per_eu(2
         const TS *base = u < nsn ? ybn : yb;
         const long long y = u < nsn ? __builtin_amdgcn_readlane(udn, kLsUnitHdr + wave * PF + u) : 0;
 #pragma unroll
-        for (int k = 0; k < EPL; ++k) r[u][k] = ld2(base + y * DD + lane + 64 * k);
+        for (int k = 0; k < EPL; ++k) r[u][k] = base[y * DD + lane + 64 * k];
       }
     }
     // the claim after next, then the next unit's left matrices
