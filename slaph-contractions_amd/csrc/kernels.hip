@@ -1380,10 +1380,12 @@ __global__ __launch_bounds__(256) void k_prod16(const T *__restrict__ sX, const 
 // x-th contiguous eighth of the (time, unit) sequence, so that the units of
 // one time key read its Q0 slab through one L2.
 constexpr int kSpRing = 4;
-// LDS slot of element e of a staged matrix: the low nibble XORed with the
-// column's low two bits times 4, which spreads the four lanes r of one (q, g)
-// over distinct banks
-__device__ __forceinline__ int sp_swz(int e) { return e ^ (((e >> 4) & 3) << 2); }
+// LDS slot of element e = i + 16 j of a staged matrix: (i ^ j) + 16 j.  A
+// wave's read of S[4 c + q][4 n + r] (kind 0) or S[4 n + r][4 c + q] (kind 1,
+// transposed) then lands 16 consecutive lanes (fixed q, all g and r) on
+// distinct 16-byte bank groups: i ^ j = 4 (c ^ n) + (q ^ r), with c running
+// over all four blocks as g does (gamma permutations are XORs by a constant).
+__device__ __forceinline__ int sp_swz(int e) { return e ^ ((e >> 4) & 15); }
 template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sprod16(
     const T *__restrict__ sQ0, const long long *__restrict__ Xt, const T *__restrict__ sQ2,
