@@ -279,7 +279,8 @@ int slc_set_precision(slc_ctx *ctx, int vdaggerv, int factors);
  * the context's current inputs for each candidate of each schedule / launch
  * shape in turn -- the persistent grid of the left-stationary traces
  * (workgroups per CU), the K split of the GEMM-form traces, MFMA or generic
- * D = 16 products, Moebius or direct masked inner products, Fourier or
+ * D = 16 products, Moebius or direct masked inner products, summed or
+ * per-combination products of the crossed traces (crossed_direct), Fourier or
  * direct VdaggerV -- `repeats` timed calls each, keeping a candidate only when
  * it is faster by more than 0.5 %.  The choice is kept in the context for
  * this input file (a later slc_configure with the same text reuses it); the
