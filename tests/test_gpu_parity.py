@@ -375,6 +375,62 @@ def test_dilution_d24_d32_matches_oracle(slaph, dilE, N, kernels, monkeypatch):
     say("  worst relative error %.2e" % worst)
 
 
+TWO_FLAVOURS = """
+Lt = 8
+Lx = 4
+Ly = 4
+Lz = 4
+number_of_eigen_vec = 8
+[quarks]
+quark = u:4:TB:2:EI:4:DF:4:synthetic
+quark = s:2:TB:2:EI:4:DF:4:synthetic
+[operator_lists]
+operator_list = g5.d0.p0,1
+operator_list = g2.d0.p0,1
+[correlator_lists]
+correlator_list = C4cC:Q0:Op0:Q1:Op0:Q0:Op0:Q1:Op0:P0,1
+correlator_list = C4cC:Q0:Op0:Q0:Op1:Q0:Op0:Q0:Op1:P0
+correlator_list = C2c:Q0:Op0:Q1:Op0:P0,1
+"""
+_TWO_WANT = {}
+
+
+@pytest.mark.parametrize("form", ["summed", "per-combination"])
+def test_crossed_traces_two_flavours_mixed_gammas(slaph, form, monkeypatch):
+    """D = 16 crossed traces (C4cC) with two quark flavours (u u / s s random
+    vector pairs of different sets, so the products' and traces' admitted
+    chains differ per factor) and a gamma_2 operator beside gamma_5 (Q0's
+    nonzero Dirac blocks off the diagonal): the summed-product form and one
+    product per combination against the oracle at 1e-12 norm-wise."""
+    say("two flavours, mixed gammas", form)
+    for var in KERNEL_SWITCHES:
+        monkeypatch.delenv(var, raising=False)
+    if form == "per-combination":
+        monkeypatch.setenv("SLAPH_XFORM_OFF", "1")
+    ctx = slaph.Context(0)
+    ctx.configure(TWO_FLAVOURS)
+    ctx.synthetic_inputs(4242, True)
+    slaph.OperatorFactory(ctx).create_operators()
+    slaph.contract(ctx)
+    if not _TWO_WANT:
+        cfg = o.parse_infile(TWO_FLAVOURS)
+        lut = o.init_lookup_tables(cfg)
+        ev = [ctx.get_input("eigenvectors", t) for t in range(cfg["Lt"])]
+        perams = [ctx.get_input("perambulator", r) for r in range(6)]
+        rnds = [ctx.get_input("random_vector", r) for r in range(6)]
+        _TWO_WANT.update(o.contract(cfg, lut, o.build_vdaggerv(cfg, lut, ev), perams, rnds))
+    assert set(_TWO_WANT) == set(ctx.diagrams()) == {"C2c", "C4cC"}
+    worst = 0.0
+    for d, lst in _TWO_WANT.items():
+        got = ctx.correlators(d)
+        assert len(got) == len(lst)
+        for name, vals in lst:
+            err = np.max(np.abs(got[name] - vals)) / max(np.max(np.abs(vals)), 1e-300)
+            worst = max(worst, err)
+            assert err <= 1e-12, (d, name, err)
+    say("  %d C4cC correlators, worst relative error %.2e" % (len(_TWO_WANT["C4cC"]), worst))
+
+
 def _rel_errors(got, want):
     """Per-correlator norm-wise relative error max_t |C - C_ref| / max_t |C_ref|."""
     out = []
