@@ -907,11 +907,15 @@ void plan_gridg(Family &f, NodePlan const &root) {
 // write.  Each trace id becomes two items of the left-stationary
 // plan (X pairs, W pairs) that reduce into its one group sum.  The trace
 // reads 20 + 25 matrices per trace id instead of 60 (five random vectors), the
-// table 45 per entry instead of 60.
-void plan_xform(Family &f, std::vector<FactorIndex> const &q0l, std::vector<FactorIndex> const &q2l,
+// table 45 per entry instead of 60.  Returns false, with the family unchanged,
+// when a Q2 entry has more random-vector pairs than the product kernel stages
+// (kSpStage); the family then keeps one product per combination.
+bool plan_xform(Family &f, std::vector<FactorIndex> const &q0l, std::vector<FactorIndex> const &q2l,
                 std::vector<long long> const &off0, std::vector<long long> const &off2) {
   NodePlan &p = f.nodes[2];
   NodePlan &root = f.nodes.back();
+  for (auto const &key : p.qn_key)
+    if (q2l.at(key.at(1)).rnd.size() > size_t(kSpStage)) return false;
   using Lab = std::pair<int, int>;
   f.xterms.clear();
   f.xunits.clear();
@@ -1069,6 +1073,7 @@ void plan_xform(Family &f, std::vector<FactorIndex> const &q0l, std::vector<Fact
   root.cmaxX = root.cmaxY = tot - 1;
   upload(f.d_xterms, f.xterms);
   upload(f.d_xunits, f.xunits);
+  return true;
 }
 
 // Every (factor type, time key) a leaf of this family reads at slice pair
@@ -1547,8 +1552,10 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
                 f.shape.nodes.size() == 7 && f.nodes[5].owner == 2 && !std::getenv("SLAPH_XFORM_OFF") &&
                 !tune_.xform_off && !std::getenv("SLAPH_TRACE_GROUPS") && trace_lstat_qchunk(D_) > 0 &&
                 std::none_of(multi[fi].begin(), multi[fi].end(), [](char m) { return m != 0; });
+      if (f.xform)
+        f.xform = plan_xform(f, gd_.quarkline_lookuptable.at("Q0"), gd_.quarkline_lookuptable.at("Q2"), q_off_[0],
+                             q_off_[2]);
       if (f.xform) {
-        plan_xform(f, gd_.quarkline_lookuptable.at("Q0"), gd_.quarkline_lookuptable.at("Q2"), q_off_[0], q_off_[2]);
         order_items_by_component(f, f.xroot);
         upload(f.d_gitems, f.gitems);
         upload(f.d_gpairs, f.gpairs);

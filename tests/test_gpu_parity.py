@@ -431,6 +431,44 @@ def test_crossed_traces_two_flavours_mixed_gammas(slaph, form, monkeypatch):
     say("  %d C4cC correlators, worst relative error %.2e" % (len(_TWO_WANT["C4cC"]), worst))
 
 
+SIX_RANDOM = """
+Lt = 4
+Lx = 4
+Ly = 4
+Lz = 4
+number_of_eigen_vec = 8
+[quarks]
+quark = u:6:TB:2:EI:4:DF:4:synthetic
+[operator_lists]
+operator_list = g5.d0.p0,1
+[correlator_lists]
+correlator_list = C4cC:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0
+"""
+
+
+def test_crossed_traces_six_random_vectors(slaph):
+    """Six random vectors: a Q2 entry has 30 random-vector pairs, more than the
+    summed-product kernel stages in LDS, so the crossed traces fall back to one
+    product per combination instead of failing; against the oracle at 1e-12."""
+    say("crossed traces, six random vectors")
+    ctx = slaph.Context(0)
+    ctx.configure(SIX_RANDOM)
+    ctx.synthetic_inputs(606, True)
+    slaph.OperatorFactory(ctx).create_operators()
+    slaph.contract(ctx)
+    cfg = o.parse_infile(SIX_RANDOM)
+    lut = o.init_lookup_tables(cfg)
+    ev = [ctx.get_input("eigenvectors", t) for t in range(cfg["Lt"])]
+    perams = [ctx.get_input("perambulator", r) for r in range(6)]
+    rnds = [ctx.get_input("random_vector", r) for r in range(6)]
+    want = o.contract(cfg, lut, o.build_vdaggerv(cfg, lut, ev), perams, rnds)
+    got = ctx.correlators("C4cC")
+    assert len(got) == len(want["C4cC"]) > 0
+    for name, vals in want["C4cC"]:
+        err = np.max(np.abs(got[name] - vals)) / max(np.max(np.abs(vals)), 1e-300)
+        assert err <= 1e-12, (name, err)
+
+
 def _rel_errors(got, want):
     """Per-correlator norm-wise relative error max_t |C - C_ref| / max_t |C_ref|."""
     out = []
