@@ -1028,7 +1028,7 @@ bool plan_xform(Family &f, std::vector<FactorIndex> const &q0l, std::vector<Fact
     const int wl = wentry(b, c), wr = wentry(d, a);
     item(woff[wl], woff[wr], wlab[wl], wlab[wr], false, dst);
   }
-  // units: per Q2 entry its runs, dealt to the four waves in about equal term counts
+  // units: per Q2 entry its runs, dealt to the kSpWaves waves in about equal term counts
   for (auto &kv : runs) {
     const int b = kv.first;
     SpUnit un;
@@ -1042,7 +1042,7 @@ bool plan_xform(Family &f, std::vector<FactorIndex> const &q0l, std::vector<Fact
     int w = 0;
     un.tb[0] = int(f.xterms.size());
     for (auto const &rn : kv.second) {
-      while (w < 3 && done * 4 >= nterm * (w + 1)) un.tb[++w] = int(f.xterms.size());
+      while (w < kSpWaves - 1 && done * kSpWaves >= nterm * (w + 1)) un.tb[++w] = int(f.xterms.size());
       for (size_t k = 0; k < rn.t.size(); ++k) {
         const long long q0i = rn.xo + (rn.kind ? rn.t[k].y : rn.t[k].x);
         const int q2i = rn.kind ? rn.t[k].x : rn.t[k].y;
@@ -1059,7 +1059,7 @@ bool plan_xform(Family &f, std::vector<FactorIndex> const &q0l, std::vector<Fact
       }
       done += (long long)rn.t.size();
     }
-    while (w < 4) un.tb[++w] = int(f.xterms.size());
+    while (w < kSpWaves) un.tb[++w] = int(f.xterms.size());
     f.xunits.push_back(un);
   }
   if (tot == 0 || items.empty() || f.xterms.empty()) throw std::logic_error("summed-product form: empty plan");

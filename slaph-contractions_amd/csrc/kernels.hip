@@ -1387,7 +1387,7 @@ constexpr int kSpRing = 4;
 // over all four blocks as g does (gamma permutations are XORs by a constant).
 __device__ __forceinline__ int sp_swz(int e) { return e ^ ((e >> 4) & 15); }
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_sprod16(
+__global__ __launch_bounds__(64 * kSpWaves) void k_sprod16(
     const T *__restrict__ sQ0, const long long *__restrict__ Xt, const T *__restrict__ sQ2,
     const long long *__restrict__ Yt, T *__restrict__ out, long long obase, long long ostride,
     const STerm *__restrict__ terms, const SpUnit *__restrict__ units, int nunit) {
@@ -1400,7 +1400,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const SpUnit U = units[u];
   {
     const T *sb = sQ2 + (Yt[ti] + U.s0) * 256;
-    for (int e = tid; e < U.np * 256; e += 256) S[(e & ~255) + sp_swz(e & 255)] = ld2(sb + e);
+    for (int e = tid; e < U.np * 256; e += 64 * kSpWaves) S[(e & ~255) + sp_swz(e & 255)] = ld2(sb + e);
   }
   __syncthreads();
   const int t0 = U.tb[wave], t1 = U.tb[wave + 1];
@@ -2977,7 +2977,7 @@ void launch_sprod(const void *sQ0, const long long *Xt, const void *sQ2, const l
   if ((long long)nunit * ntime > 0x7fffffffLL) throw std::runtime_error("launch_sprod: grid too large");
   with_storage(f32, [&](auto tag) {
     using T = decltype(tag);
-    hipLaunchKernelGGL(k_sprod16<T>, dim3(nunit, ntime), dim3(256), 0, s, static_cast<const T *>(sQ0), Xt,
+    hipLaunchKernelGGL(k_sprod16<T>, dim3(nunit, ntime), dim3(64 * kSpWaves), 0, s, static_cast<const T *>(sQ0), Xt,
                        static_cast<const T *>(sQ2), Yt, static_cast<T *>(out), obase, ostride, terms, units, nunit);
   });
   SLAPH_HIP(hipGetLastError());

@@ -216,12 +216,13 @@ void launch_prod(const void *slabX, const long long *Xt, const long long *Xq, co
 // perm(d), the gamma matrix's row of column d; bit 1 is the kind.  Wave w of a
 // unit runs terms [tb[w], tb[w + 1]), whole runs each.
 constexpr int kSpStage = 20;  // largest np (80 KiB of LDS)
+constexpr int kSpWaves = 8;   // wavefronts per unit
 struct STerm {
   int l, r, o, flags;
 };
 struct SpUnit {
   int s0, np;
-  int tb[5];
+  int tb[kSpWaves + 1];
 };
 void launch_sprod(const void *slabQ0, const long long *Xt, const void *slabQ2, const long long *Yt, void *out,
                   long long obase, long long ostride, const STerm *terms, const SpUnit *units, int nunit, int ntime,
