@@ -87,8 +87,18 @@ quark = u:{nrnd}:TB:{dilT}:EI:{dilE}:DF:4:synthetic
 """.format(ops=ops, corr="\n".join(lines), **w)
 
 
-# the kernel of the roofline object (the block form runs under SLAPH_LSTAT_BLOCKS=1)
-LSTAT_KERNEL = "k_trace_lstat" if os.environ.get("SLAPH_LSTAT_BLOCKS") else "k_trace_lstat_q"
+def trace_kernel(tuning=""):
+    """The kernel of the roofline object: the crossed (C4cC) traces as
+    symmetric products, or on the left-stationary kernel when SLAPH_SYM_OFF
+    or the tuning (crossed_lstat=1) selects it (its block form under
+    SLAPH_LSTAT_BLOCKS=1)."""
+    if os.environ.get("SLAPH_SYM_OFF") or "crossed_lstat=1" in tuning.split():
+        return "k_trace_lstat" if os.environ.get("SLAPH_LSTAT_BLOCKS") else "k_trace_lstat_q"
+    return "k_trace_sym"
+
+
+KERNEL_LABEL = {"k_trace_sym": "crossed traces as symmetric products",
+                "k_trace_lstat_q": "non-separable traces", "k_trace_lstat": "non-separable traces"}
 
 
 def kernel_source_hash(kernel):
@@ -151,7 +161,7 @@ def measure_traffic(kernel, workload, copy_kib):
         res[counter] = (sum(kern) / len(kern), copy_kib / (sum(copy) / len(copy)), len(kern))
     (f, sf, n), (w, sw, _) = res["FETCH_SIZE"], res["WRITE_SIZE"]
     return {"bytes_per_launch": (sf * f + sw * w) * 1024.0, "fetch_kib": f, "write_kib": w, "fetch_scale": sf,
-            "write_scale": sw, "launches": n, "kernel_sha": kernel_source_hash(kernel)}
+            "write_scale": sw, "launches": n, "kernel": kernel, "kernel_sha": kernel_source_hash(kernel)}
 
 
 def cpu_baseline(w, text, sample_slice_pairs=2):
@@ -290,7 +300,7 @@ def main():
     w = WORKLOADS[args.workload]
     traffic_run = None
     if args.measure_traffic:  # before this process touches the GPU
-        traffic_run = measure_traffic(LSTAT_KERNEL, args.workload, 3 * w["L"] ** 3 * w["N"] * 16 / 1024.0)
+        traffic_run = measure_traffic(trace_kernel(), args.workload, 3 * w["L"] ** 3 * w["N"] * 16 / 1024.0)
     import slaph
     text = infile(w)
     ctx = slaph.Context(0)
@@ -333,7 +343,8 @@ def main():
         print("# autotuned in %.1f s: %s" % (tune["tuning_s"], report["chosen"]), file=sys.stderr, flush=True)
         step()
     stage = {"vdaggerv": 0.0, "factors": 0.0, "contract": 0.0}
-    stats = {LSTAT_KERNEL: [0.0] * 4, "k_vdv_ft": [0.0] * 4}
+    tkern = trace_kernel(ctx.tuning())
+    stats = {tkern: [0.0] * 4, "k_vdv_ft": [0.0] * 4}
     t0 = time.perf_counter()
     for i in range(args.steps):
         step()
@@ -371,16 +382,18 @@ def main():
         "stages_ms": {"vdaggerv": stage["vdaggerv"], "factors": stage["factors"],
                       "contract_incl_factors": stage["contract"]},
     }
-    ls = stats[LSTAT_KERNEL]
+    ls = stats[tkern]
     if ls is not None:
         launches = max(ls[1], 1.0)
         kern_ms, bytes_per_launch = ls[0] / launches, ls[2] / launches
         achieved = bytes_per_launch / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
-        if traffic_run is not None:
+        if traffic_run is not None and traffic_run.get("kernel", tkern) == tkern:
             traffic, tnote = traffic_run["bytes_per_launch"], "measured in this run: %s" % json.dumps(traffic_run)
+        elif traffic_run is not None:
+            traffic, tnote = None, "measured for %s, but the tuning ran %s" % (traffic_run["kernel"], tkern)
         else:
-            traffic, tnote = kernel_traffic(LSTAT_KERNEL, args.workload)
-        line["roofline"] = {"bound": "hbm", "kernel": LSTAT_KERNEL + " (non-separable traces)", "achieved": achieved,
+            traffic, tnote = kernel_traffic(tkern, args.workload)
+        line["roofline"] = {"bound": "hbm", "kernel": "%s (%s)" % (tkern, KERNEL_LABEL[tkern]), "achieved": achieved,
                             "peak": PEAK_HBM, "unit": "GB/s", "frac": achieved / PEAK_HBM,
                             "traffic": traffic, "traffic_source": tnote,
                             "traffic_unit": "bytes per launch (HBM, rocprofv3 --pmc)",

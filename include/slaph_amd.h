@@ -243,8 +243,10 @@ int slc_stage_time(const slc_ctx *ctx, const char *what, double *ms);
 
 /* Launch statistics of one kernel in the last contract or create_operators
  * (no reference counterpart; for roofline measurements): kernel =
- * "k_trace_lstat_q" / "k_trace_lstat" (non-separable traces) or "k_vdv_ft"
- * (VdaggerV, sitewise Fourier form); what = "ms" (device time summed over its
+ * "k_trace_sym" (crossed traces of the summed-product form as symmetric
+ * products), "k_trace_lstat_q" / "k_trace_lstat" (non-separable traces on
+ * the left-stationary kernel) or "k_vdv_ft" (VdaggerV, sitewise Fourier
+ * form); what = "ms" (device time summed over its
  * launches, HIP events on the engine stream), "launches", "bytes"
  * (algorithmic bytes summed over the launches: each operand matrix once,
  * every output written once) or "flops" (operations of the kernel's own
@@ -280,8 +282,10 @@ int slc_set_precision(slc_ctx *ctx, int vdaggerv, int factors);
  * shape in turn -- the persistent grid of the left-stationary traces
  * (workgroups per CU), the K split of the GEMM-form traces, MFMA or generic
  * D = 16 products, Moebius or direct masked inner products, summed or
- * per-combination products of the crossed traces (crossed_direct), Fourier or
- * direct VdaggerV -- `repeats` timed calls each, keeping a candidate only when
+ * per-combination products of the crossed traces (crossed_direct), their
+ * traces as symmetric products or on the left-stationary kernel
+ * (crossed_lstat), Fourier or direct VdaggerV -- `repeats` timed calls each,
+ * keeping a candidate only when
  * it is faster by more than 0.5 %.  The choice is kept in the context for
  * this input file (a later slc_configure with the same text reuses it); the
  * correlators stay equal up to rounding.  Needs inputs and operators. */

@@ -284,6 +284,19 @@ struct Family {
   std::vector<SpUnit> xunits;
   DevBuf d_xterms, d_xunits;
   NodePlan xroot;
+  // the table entries its items read: offset -> (label pairing: 1 swaps the
+  // two random vectors of a label, 0 keeps it; label -> matrix index)
+  std::map<long long, std::pair<int, std::map<std::pair<int, int>, int>>> xent;
+  // symmetric-product plan of its traces (see plan_sym): units, stage and
+  // tile tables, per trace id its partial sums (one per item)
+  std::vector<SymUnit> sym_units;
+  std::vector<int> sym_stages, sym_tiles;
+  std::vector<GroupItem> sym_red;
+  std::vector<int> sym_ridx;
+  long long sym_npart = 0;
+  double sym_bytes = 0;  // bytes per slice pair: staged matrices + partial sums
+  DevBuf d_symu, d_syms, d_symt, d_symr, d_symri;
+  long long npart() const { return std::max(ls_npart, sym_npart); }
 };
 
 struct CorrPlan {
@@ -919,6 +932,7 @@ bool plan_xform(Family &f, std::vector<FactorIndex> const &q0l, std::vector<Fact
   using Lab = std::pair<int, int>;
   f.xterms.clear();
   f.xunits.clear();
+  f.xent.clear();
   long long tot = 0;
   long long maxq0 = 0, maxq2 = 0;
   // Dirac block pattern of a Q0 entry: its gamma matrix's row of each column
@@ -965,6 +979,7 @@ bool plan_xform(Family &f, std::vector<FactorIndex> const &q0l, std::vector<Fact
       add_job(off0.at(a), b, tot + (long long)k, terms[order[k]], 0, perm_of(a));
       xlab[q][order[k]] = int(k);
     }
+    if (!order.empty()) f.xent[tot] = {1, xlab[q]};
     tot += (long long)order.size();
   }
   // W part, per crossed pair (b, c): labels (r2, r1) for V, (r2, -1) for i Y
@@ -998,6 +1013,7 @@ bool plan_xform(Family &f, std::vector<FactorIndex> const &q0l, std::vector<Fact
     }
     woff.push_back(tot);
     wlab.push_back(lab);
+    if (k > 0) f.xent[tot] = {0, lab};
     tot += k;
     return wid[{b, c}] = int(woff.size()) - 1;
   };
@@ -1073,6 +1089,231 @@ bool plan_xform(Family &f, std::vector<FactorIndex> const &q0l, std::vector<Fact
   root.cmaxX = root.cmaxY = tot - 1;
   upload(f.d_xterms, f.xterms);
   upload(f.d_xunits, f.xunits);
+  return true;
+}
+
+// Symmetric-product plan of the summed-product traces (launch_trace_sym).
+// The family's items form two trace graphs, one per label pairing (entries:
+// table offsets, edges: items): the X items pair label (r1, r3) of the left
+// entry with (r3, r1) of the right one, the W items equal labels, so that
+// within a connected component every item value is an element of the
+// symmetric matrix T[e1][e2] = sum_l tr(M_e1(l) M_e2(pi l)).  A component of
+// at most `maxe` entries (kSymEntries, or SLAPH_SYM_ENTRIES for tests) is one
+// piece, its tiles on and above the diagonal; a larger one is cut into chunks
+// of `chunk` entries, one piece per chunk pair (a pair of distinct chunks:
+// rows from the first, columns from the second).  Pieces are packed into
+// units of at most kSymEntries positions, each piece (and its column part) at
+// a multiple of four.  A unit's stages cover every label of its entries whose
+// partner label occurs too: four cross stages (block (I, J) of l with block
+// (J, I) of pi l) per label pair, one cross stage (blocks (0, 1), (1, 0)) and
+// one self stage ((0, 0), (1, 1)) per label with pi l = l.  Each item has a
+// partial sum of its own (its index in gitems); the reduction adds the items
+// of a trace id in item order.  Returns false, leaving no plan, when an
+// index does not fit the kernel's tables.
+bool plan_sym(Family &f) {
+  auto reset = [&]() {
+    f.sym_units.clear();
+    f.sym_stages.clear();
+    f.sym_tiles.clear();
+    f.sym_red.clear();
+    f.sym_ridx.clear();
+    f.sym_npart = 0;
+    f.sym_bytes = 0;
+  };
+  reset();
+  using Lab = std::pair<int, int>;
+  int maxe = kSymEntries;
+  if (const char *e = std::getenv("SLAPH_SYM_ENTRIES")) maxe = std::max(4, std::min(kSymEntries, std::atoi(e)));
+  const int chunk = std::max(4, std::min(16, (maxe / 2) / 4 * 4));
+  auto ceil4 = [](size_t n) { return int((n + 3) / 4 * 4); };
+  // (1) components of both graphs (entry offsets are distinct across them)
+  std::map<long long, long long> par;
+  std::function<long long(long long)> find = [&](long long x) -> long long {
+    auto it = par.emplace(x, x).first;
+    if (it->second == x) return x;
+    return it->second = find(it->second);
+  };
+  std::vector<std::array<long long, 2>> ends(f.gitems.size());
+  for (size_t k = 0; k < f.gitems.size(); ++k) {
+    const long long l = f.xroot.h_Xq.at(f.gitems[k].qn), r = f.xroot.h_Yq.at(f.gitems[k].qn);
+    auto il = f.xent.find(l), ir = f.xent.find(r);
+    if (il == f.xent.end() || ir == f.xent.end() || il->second.first != ir->second.first) return false;
+    ends[k] = {l, r};
+    const long long a = find(l), b = find(r);
+    if (a != b) par[std::max(a, b)] = std::min(a, b);
+  }
+  std::map<long long, std::set<long long>> comp;
+  std::map<long long, std::vector<int>> comp_items;
+  for (size_t k = 0; k < f.gitems.size(); ++k) {
+    const long long root = find(ends[k][0]);
+    comp[root].insert(ends[k][0]);
+    comp[root].insert(ends[k][1]);
+    comp_items[root].push_back(int(k));
+  }
+  // (2) pieces
+  struct Piece {
+    std::vector<long long> rows, cols;  // cols empty: the symmetric piece of rows
+    std::vector<int> items;
+    int width() const { return int((rows.size() + 3) / 4 * 4 + (cols.size() + 3) / 4 * 4); }
+  };
+  std::vector<Piece> pieces[2];
+  for (auto const &kv : comp) {
+    const std::vector<long long> ent(kv.second.begin(), kv.second.end());
+    const int kind = f.xent.at(ent[0]).first;
+    auto const &items = comp_items[kv.first];
+    if (int(ent.size()) <= maxe) {
+      pieces[kind].push_back({ent, {}, items});
+      continue;
+    }
+    std::map<long long, int> chunk_of;
+    for (size_t i = 0; i < ent.size(); ++i) chunk_of[ent[i]] = int(i) / chunk;
+    auto members = [&](int c) {
+      return std::vector<long long>(ent.begin() + size_t(c) * chunk,
+                                    ent.begin() + std::min(ent.size(), size_t(c + 1) * chunk));
+    };
+    std::map<std::pair<int, int>, Piece> pc;
+    for (int k : items) {
+      int a = chunk_of.at(ends[k][0]), b = chunk_of.at(ends[k][1]);
+      if (a > b) std::swap(a, b);
+      Piece &P = pc[{a, b}];
+      if (P.rows.empty()) {
+        P.rows = members(a);
+        if (a != b) P.cols = members(b);
+      }
+      P.items.push_back(k);
+    }
+    for (auto &x : pc) pieces[kind].push_back(std::move(x.second));
+  }
+  long long nblocks = 0;
+  for (int kind = 0; kind < 2; ++kind) {
+    auto &pcs = pieces[kind];
+    for (auto const &P : pcs)
+      if (P.width() > kSymEntries) throw std::logic_error("symmetric-product plan: piece wider than a unit");
+    std::stable_sort(pcs.begin(), pcs.end(), [](Piece const &x, Piece const &y) { return x.width() > y.width(); });
+    // (3) first fit into units
+    std::vector<std::vector<int>> bins;
+    std::vector<int> used;
+    for (size_t i = 0; i < pcs.size(); ++i) {
+      size_t b = 0;
+      while (b < bins.size() && used[b] + pcs[i].width() > kSymEntries) ++b;
+      if (b == bins.size()) {
+        bins.emplace_back();
+        used.push_back(0);
+      }
+      bins[b].push_back(int(i));
+      used[b] += pcs[i].width();
+    }
+    // (4) per unit: positions, stages, tiles
+    const bool swap = kind == 1;
+    for (size_t b = 0; b < bins.size(); ++b) {
+      const int ns = used[b];
+      std::vector<long long> at(ns, -1);  // entry at each position
+      std::set<Lab> labels;
+      std::map<std::pair<int, int>, std::array<int, 16>> tiles;
+      int base = 0;
+      for (int pi : bins[b]) {
+        Piece const &P = pcs[size_t(pi)];
+        const int cbase = base + ceil4(P.rows.size());
+        std::map<long long, int> rpos, cpos;
+        for (size_t i = 0; i < P.rows.size(); ++i) at[size_t(base) + i] = P.rows[i], rpos[P.rows[i]] = base + int(i);
+        for (size_t i = 0; i < P.cols.size(); ++i) at[size_t(cbase) + i] = P.cols[i], cpos[P.cols[i]] = cbase + int(i);
+        for (long long e : P.rows)
+          for (auto const &lk : f.xent.at(e).second) labels.insert(lk.first);
+        for (long long e : P.cols)
+          for (auto const &lk : f.xent.at(e).second) labels.insert(lk.first);
+        for (int k : P.items) {
+          int pl, pr;
+          if (P.cols.empty()) {
+            pl = rpos.at(ends[k][0]);
+            pr = rpos.at(ends[k][1]);
+            if (pl / 4 > pr / 4) std::swap(pl, pr);
+          } else if (rpos.count(ends[k][0])) {
+            pl = rpos.at(ends[k][0]);
+            pr = cpos.at(ends[k][1]);
+          } else {
+            pl = rpos.at(ends[k][1]);
+            pr = cpos.at(ends[k][0]);
+          }
+          auto it = tiles.find({pl / 4, pr / 4});
+          if (it == tiles.end()) {
+            it = tiles.emplace(std::make_pair(pl / 4, pr / 4), std::array<int, 16>{}).first;
+            it->second.fill(-1);
+          }
+          int idx = 4 * (pl & 3) + (pr & 3);
+          if (it->second[idx] >= 0 && pl / 4 == pr / 4) idx = 4 * (pr & 3) + (pl & 3);
+          if (it->second[idx] >= 0) return reset(), false;  // two items on one element of T
+          it->second[idx] = k;
+        }
+        base = cbase + ceil4(P.cols.size());
+      }
+      SymUnit un{};
+      if (f.sym_stages.size() > size_t(INT32_MAX) / 2) return reset(), false;
+      un.st_off = int(f.sym_stages.size());
+      un.ns = ns;
+      auto emit = [&](int flags, Lab l0, int I0, int J0, Lab l1, int I1, int J1) -> bool {
+        f.sym_stages.push_back(flags);
+        for (int p = 0; p < ns; ++p)
+          for (int slot = 0; slot < 2; ++slot) {
+            int off = -1;
+            if (at[size_t(p)] >= 0) {
+              auto const &lab = f.xent.at(at[size_t(p)]).second;
+              auto it = lab.find(slot ? l1 : l0);
+              if (it != lab.end()) {
+                const long long o = (at[size_t(p)] + it->second) * 256 + 128 * (slot ? I1 : I0) + 8 * (slot ? J1 : J0);
+                if (o > INT32_MAX - 256) return false;
+                off = int(o);
+                ++nblocks;
+              }
+            }
+            f.sym_stages.push_back(off);
+          }
+        ++un.nst;
+        return true;
+      };
+      for (Lab const &l : labels) {
+        const Lab pl = swap ? Lab{l.second, l.first} : l;
+        bool ok = true;
+        if (pl == l) {
+          ok = emit(0, l, 0, 1, l, 1, 0) && emit(1, l, 0, 0, l, 1, 1);
+        } else if (l < pl && labels.count(pl)) {
+          for (int I = 0; I < 2 && ok; ++I)
+            for (int J = 0; J < 2 && ok; ++J) ok = emit(0, l, I, J, pl, J, I);
+        }
+        if (!ok) return reset(), false;
+      }
+      if (un.nst % 2 && !emit(0, Lab{-2, -2}, 0, 0, Lab{-2, -2}, 0, 0)) return reset(), false;  // a zero stage
+      // tiles in (row, column) order, cut into kSymWaves contiguous ranges
+      const int n = int(tiles.size());
+      if (n > kSymWaves * kSymWaveTiles) return reset(), false;
+      un.tile_off = int(f.sym_tiles.size() / 17);
+      for (auto const &t : tiles) {
+        f.sym_tiles.push_back(t.first.first | (t.first.second << 8));
+        f.sym_tiles.insert(f.sym_tiles.end(), t.second.begin(), t.second.end());
+      }
+      for (int w = 0; w <= kSymWaves; ++w) un.wt[w] = int((long long)n * w / kSymWaves);
+      if (un.nst > 0 && n > 0) f.sym_units.push_back(un);
+    }
+  }
+  // larger units first (the grid runs unit-major)
+  std::stable_sort(f.sym_units.begin(), f.sym_units.end(), [](SymUnit const &x, SymUnit const &y) {
+    return (long long)x.nst * x.ns > (long long)y.nst * y.ns;
+  });
+  std::map<long long, std::vector<int>> by_dst;
+  for (size_t k = 0; k < f.gitems.size(); ++k) by_dst[f.gitems[k].dst].push_back(int(k));
+  for (auto const &kv : by_dst) {
+    f.sym_red.push_back(GroupItem{0, int(f.sym_ridx.size()), int(kv.second.size()), kv.first});
+    f.sym_ridx.insert(f.sym_ridx.end(), kv.second.begin(), kv.second.end());
+  }
+  f.sym_npart = (long long)f.gitems.size();
+  f.sym_bytes = double(nblocks) * 64 * sizeof(double2) + double(f.sym_npart) * sizeof(double2);
+  if (std::getenv("SLAPH_PROGRESS"))
+    std::fprintf(stderr, "symmetric-product trace plan: %zu units, %zu stage entries, %zu tiles, %lld staged blocks\n",
+                 f.sym_units.size(), f.sym_stages.size(), f.sym_tiles.size() / 17, nblocks);
+  upload(f.d_symu, f.sym_units);
+  upload(f.d_syms, f.sym_stages);
+  upload(f.d_symt, f.sym_tiles);
+  upload(f.d_symr, f.sym_red);
+  upload(f.d_symri, f.sym_ridx);
   return true;
 }
 
@@ -1561,6 +1802,9 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
         upload(f.d_gpairs, f.gpairs);
         plan_lstat(f, f.xroot, D_);
         if (f.ls_blocks.empty()) throw std::logic_error("summed-product form without a left-stationary trace plan");
+        // the symmetric-product traces unless SLAPH_SYM_OFF=1 (or the tuning
+        // knob) keeps the left-stationary kernel, a cross-check
+        if (!std::getenv("SLAPH_SYM_OFF") && !tune_.sym_off) plan_sym(f);
         continue;
       }
       order_items_by_component(f, root);
@@ -1650,7 +1894,7 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
   long long gstride_all = 0, npart_all = 0;
   for (auto const &f : fams) gstride_all += f.gstride;
   if (!lstat_off)
-    for (auto const &f : fams) npart_all += f.ls_npart;
+    for (auto const &f : fams) npart_all += f.npart();
   size_t max_corr = 1;
   for (auto const &c : corrs) max_corr = std::max(max_corr, c.reqs.size());
   const bool progress = std::getenv("SLAPH_PROGRESS") != nullptr;
@@ -1810,7 +2054,7 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
         ch.pbase.resize(fams.size());
         for (size_t fi = 0; fi < fams.size(); ++fi) {
           ch.pbase[fi] = ch.p_total;
-          if (!lstat_off) ch.p_total += (long long)S * fams[fi].ls_npart;
+          if (!lstat_off) ch.p_total += (long long)S * fams[fi].npart();
         }
         ch.nodes.resize(fams.size());
         for (size_t fi = 0; fi < fams.size(); ++fi) {
@@ -1931,9 +2175,25 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
   // then those of the last call that ran the launches directly. ----
   EventTimer ctimer(stream_);
   const bool graph = graphs_ && !fresh && !scope && !progress;
-  size_t ls_ev = 0;  // events used around left-stationary trace launches
-  double ls_bytes = 0;
-  bool ls_queued = false;
+  // HIP events around the trace launches timed for the kernel statistics:
+  // kernel -> (first event of each launch's pair, algorithmic bytes)
+  size_t ls_ev = 0;
+  std::map<std::string, std::pair<std::vector<size_t>, double>> ev_stat;
+  auto ev_begin = [&]() {
+    if (ls_ev + 2 > C.ev.size()) {
+      C.ev.resize(ls_ev + 2, nullptr);
+      SLAPH_HIP(hipEventCreate(&C.ev[ls_ev]));
+      SLAPH_HIP(hipEventCreate(&C.ev[ls_ev + 1]));
+    }
+    if (!graph) SLAPH_HIP(hipEventRecord(C.ev[ls_ev], stream_));
+  };
+  auto ev_end = [&](const char *kernel, double bytes) {
+    if (graph) return;
+    SLAPH_HIP(hipEventRecord(C.ev[ls_ev + 1], stream_));
+    ev_stat[kernel].first.push_back(ls_ev);
+    ev_stat[kernel].second += bytes;
+    ls_ev += 2;
+  };
   if (graph && C.gexec) {
     SLAPH_HIP(hipGraphLaunch(C.gexec, stream_));
   } else {
@@ -2012,17 +2272,19 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
                               f.d_gitems.as<GroupItem>(), int(f.gitems.size()), f.d_gpairs.as<int2>(),
                               nc.d_tile.as<GridTile>(), 1, C.d_gv.as<double2>(), gb, f.gstride, D_, rt, f32, stream_);
           } else if (ns.kind == kTrace2 && !f.ls_blocks.empty() && !lstat_off) {
-            if (ch.pbase[fi] + (long long)nc.ntime * f.ls_npart > ch.p_total)
+            // the summed-product traces as symmetric products when planned
+            const bool sym = f.xform && !f.sym_units.empty();
+            const long long npart = sym ? f.sym_npart : f.ls_npart;
+            if (ch.pbase[fi] + (long long)nc.ntime * npart > ch.p_total)
               throw std::logic_error("trace plan indexes outside the partial-sum buffer");
-            if (ls_ev + 2 > C.ev.size()) {
-              C.ev.resize(ls_ev + 2, nullptr);
-              SLAPH_HIP(hipEventCreate(&C.ev[ls_ev]));
-              SLAPH_HIP(hipEventCreate(&C.ev[ls_ev + 1]));
-            }
-            if (!graph) SLAPH_HIP(hipEventRecord(C.ev[ls_ev], stream_));
-            ls_bytes += f.ls_bytes * nc.ntime;
-            ls_queued = f.ls_queue;
-            if (f.ls_queue)
+            if (sym && (rt || nc.h_Xt != nc.h_Yt))
+              throw std::logic_error("symmetric-product traces need both operands in one table at one time base");
+            ev_begin();
+            if (sym)
+              launch_trace_sym(src(ns.left), nc.d_Xt.as<long long>(), f.d_symu.as<SymUnit>(), int(f.sym_units.size()),
+                               f.d_syms.as<int>(), f.d_symt.as<int>(), C.d_pv.as<double2>() + ch.pbase[fi], npart,
+                               nc.ntime, f32, stream_);
+            else if (f.ls_queue)
               launch_trace_lstat_q(src(ns.left), nc.d_Xt.as<long long>(), src(ns.right), nc.d_Yt.as<long long>(),
                                    f.d_lsu.as<int>(), int(f.ls_blocks.size()), f.d_lst.as<LsTask>(),
                                    f.d_lsd.as<long long>(), C.d_pv.as<double2>(), ch.pbase[fi], f.ls_npart, nc.ntime, D_,
@@ -2032,13 +2294,15 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
                                  f.d_lsb.as<LsBlock>(), int(f.ls_blocks.size()), f.d_lsc.as<LsChunk>(),
                                  f.d_lsl.as<int>(), f.d_lst.as<LsTask>(), f.d_lsd.as<long long>(), C.d_pv.as<double2>(),
                                  ch.pbase[fi], f.ls_npart, nc.ntime, D_, rt, f32, stream_);
-            if (!graph) {
-              SLAPH_HIP(hipEventRecord(C.ev[ls_ev + 1], stream_));
-              ls_ev += 2;
-            }
-            launch_ls_reduce(C.d_pv.as<double2>() + ch.pbase[fi], f.ls_npart, f.d_lsr.as<GroupItem>(),
-                             int(f.ls_red.size()), f.d_lsri.as<int>(), C.d_gv.as<double2>(), gb, f.gstride, nc.ntime,
-                             stream_);
+            // statistics keyed by the kernel that ran (the queued
+            // left-stationary form unless SLAPH_LSTAT_BLOCKS selected blocks)
+            ev_end(sym ? "k_trace_sym" : f.ls_queue ? "k_trace_lstat_q" : "k_trace_lstat",
+                   (sym ? f.sym_bytes : f.ls_bytes) * nc.ntime);
+            launch_ls_reduce(C.d_pv.as<double2>() + ch.pbase[fi], npart,
+                             sym ? f.d_symr.as<GroupItem>() : f.d_lsr.as<GroupItem>(),
+                             int(sym ? f.sym_red.size() : f.ls_red.size()),
+                             sym ? f.d_symri.as<int>() : f.d_lsri.as<int>(), C.d_gv.as<double2>(), gb, f.gstride,
+                             nc.ntime, stream_);
           } else {
             if (f.xform) throw std::logic_error("summed-product form needs the left-stationary traces");
             launch_trace_groups(src(ns.left), nc.d_Xt.as<long long>(), np.d_Xq.as<long long>(), src(ns.right),
@@ -2083,18 +2347,16 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
   results_.resize(corrs.size());
   double kms = ctimer.stop();
   (void)kms;
-  if (ls_ev > 0) {
+  for (auto const &kv : ev_stat) {
     KernelStat ks;
-    for (size_t e = 0; e < ls_ev; e += 2) {
+    for (size_t e : kv.second.first) {
       float ms = 0;
       SLAPH_HIP(hipEventElapsedTime(&ms, C.ev[e], C.ev[e + 1]));
       ks.ms += ms;
     }
-    ks.launches = int(ls_ev / 2);
-    ks.bytes = ls_bytes;
-    // keyed by the kernel the left-stationary traces ran on (the queued form
-    // unless SLAPH_LSTAT_BLOCKS selected blocks)
-    kstats_[ls_queued ? "k_trace_lstat_q" : "k_trace_lstat"] = ks;
+    ks.launches = int(kv.second.first.size());
+    ks.bytes = kv.second.second;
+    kstats_[kv.first] = ks;
   }
   for (size_t c = 0, off = 0; c < corrs.size(); off += (size_t)Lt_ * corrs[c].reqs.size(), ++c) {
     DiagramResult &r = results_[c];

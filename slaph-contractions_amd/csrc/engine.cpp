@@ -1195,13 +1195,14 @@ struct TuneKnob {
   bool vdaggerv = false;  // timed on create_operators instead of contract
 };
 std::vector<TuneKnob> tune_knobs() {
-  std::vector<TuneKnob> k(6);
+  std::vector<TuneKnob> k(7);
   k[0] = {"vdaggerv_direct", &Engine::Tuning::vdv_direct, nullptr, {0, 1}, true};
   k[1] = {"lstat_wg_per_cu", nullptr, &LaunchTuning::lstat_wg_per_cu, {2, 1, 3}, false};
   k[2] = {"gemm_wg_per_cu", nullptr, &LaunchTuning::gemm_wg_per_cu, {16, 8, 32}, false};
   k[3] = {"prod_generic", nullptr, &LaunchTuning::prod_generic, {0, 1}, false};
   k[4] = {"resolve_groups", &Engine::Tuning::resolve_groups, nullptr, {0, 1}, false};
   k[5] = {"crossed_direct", &Engine::Tuning::xform_off, nullptr, {0, 1}, false};
+  k[6] = {"crossed_lstat", &Engine::Tuning::sym_off, nullptr, {0, 1}, false};
   return k;
 }
 int &knob_ref(Engine::Tuning &t, TuneKnob const &k) { return k.field ? t.*(k.field) : t.launch.*(k.lfield); }

@@ -186,6 +186,7 @@ class Engine {
     int vdv_direct = 0;      // VdaggerV by the direct product instead of the sitewise Fourier form
     int resolve_groups = 0;  // masked inner products by direct group loops instead of the Moebius form
     int xform_off = 0;       // crossed four-factor traces from one product per combination, not summed products
+    int sym_off = 0;         // summed-product traces by the left-stationary kernel, not symmetric products
   };
   Tuning const &tuning() const { return tune_; }
   std::string tuning_text() const;

@@ -1495,6 +1495,162 @@ __global__ __launch_bounds__(64 * kSpWaves) void k_sprod16(
   }
 }
 
+// Traces of the summed-product tables as symmetric products per component of
+// the trace graph (the trace ids of DilutedTraceFactory.cpp:161-190, each a sum
+// of tr(X Y) = sum_{a,b} X[a,b] Y[b,a] over its matrix pairs, factor_to_trace
+// in DilutedTrace.cpp:84-138).  One workgroup per (unit, slice pair): the
+// unit's table matrices are read from memory once, stage by stage (two 8 x 8
+// blocks per staged entry), and T = A B^T over the stage's K range runs on
+// the 4 x 4 x 4 FP64 MFMA, one 4 x 4 tile of T (four row entries, four
+// column entries) per instruction, its four blocks g splitting K, three real
+// products per complex one (Gauss).  Lane l = i + 4 g + 16 k supplies entry i
+// of the tile's rows (A) and of its columns (B) at K element (row 2 s +
+// (k >> 1), column 4 (k & 1) + g) of the stage block for sub-step s: A reads
+// that element of its slot, B the transposed element of the paired slot.
+// LDS position of element (r, c) of staged position p: 64 p + ((8 r + (c ^ r))
+// ^ 4 (p & 3)); the 16 lanes of one k (four entries, four consecutive
+// elements of a block row for A, of a block column for B) then fall on
+// sixteen distinct 16-byte bank groups.  Two stage buffers: while stage k
+// is multiplied, stage k + 1 waits in registers for its buffer and stage
+// k + 2's loads are in flight (two register sets); one barrier per stage.
+// Lanes 16 i + j hold T[i][j] of a tile after the sum over the blocks g (row
+// rotations by 8 and 4).
+constexpr int kSymPass = 2 * kSymEntries / kSymWaves;  // staged blocks per wave and stage
+__device__ __forceinline__ int sym_pos(int p, int r, int c) { return 64 * p + ((8 * r + (c ^ r)) ^ ((p & 3) << 2)); }
+template <typename T>
+__global__ __launch_bounds__(64 * kSymWaves) __attribute__((amdgpu_waves_per_eu(2))) void k_trace_sym(
+    const T *__restrict__ slab, const long long *__restrict__ Tt, const SymUnit *__restrict__ units, int ntime,
+    const int *__restrict__ stages, const int *__restrict__ tiles, double2 *__restrict__ part, long long pstride) {
+  // two stage buffers (stage k in buffer k % 2), two 8 x 8 blocks per position
+  __shared__ double2 Ls[2][2][kSymEntries * 64];
+  const int u = blockIdx.x / ntime, ti = blockIdx.x % ntime;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // unit fields, table entries and tile headers are wave-uniform: through
+  // readfirstlane the branches on them are scalar
+  auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+  const SymUnit *U = units + u;
+  const int ns = uni(U->ns), nst = uni(U->nst), rowlen = 1 + 2 * ns;
+  const int *st = stages + uni(U->st_off);
+  // staging: wave w loads blocks b = w + kSymWaves q (position b / 2, slot
+  // b % 2), lane (r, c) = (lane / 8, lane % 8) of the 8 x 8 block
+  const int cr = lane >> 3, cc = lane & 7;
+  const T *tb = slab + Tt[ti] * 256 + cr * 16 + cc;
+  // stage k's loads go to register set k % 2: set (k + 1) % 2 holds stage
+  // k + 1 (issued one stage earlier) while stage k is multiplied.  All
+  // offsets of a stage are read first (scalar loads), then its block loads
+  // go out back to back.  Every wave issues the same kSymPass loads per
+  // stage, whatever the unit (blocks past it and zero blocks, offset -1,
+  // load block 0 and are not used: bit q of the set's mask), and the stages
+  // past the last reload the last one, so that the compiler's count of
+  // loads in flight is exact: the write of stage k + 1 to LDS then waits
+  // only for its own loads, not for stage k + 2's.
+  T pre0[kSymPass], pre1[kSymPass];
+  unsigned zero0 = 0, zero1 = 0;
+  const int nb = 2 * ns;
+  auto fetch = [&](int k, T(&pre)[kSymPass], unsigned &zero) {
+    const int *row = st + min(k, nst - 1) * rowlen + 1;
+    int off[kSymPass];
+#pragma unroll
+    for (int q = 0; q < kSymPass; ++q) off[q] = kSymWaves * q + wave < nb ? uni(row[kSymWaves * q + wave]) : -1;
+    zero = 0;
+#pragma unroll
+    for (int q = 0; q < kSymPass; ++q) {
+      zero |= off[q] < 0 ? 1u << q : 0u;
+      pre[q] = tb[max(off[q], 0)];
+    }
+  };
+  auto commit = [&](T(&pre)[kSymPass], unsigned zero, int buf) {
+#pragma unroll
+    for (int q = 0; q < kSymPass; ++q) {
+      const int b = kSymWaves * q + wave;
+      if (b < nb) Ls[buf][b & 1][sym_pos(b >> 1, cr, cc)] = (zero >> q) & 1 ? make_double2(0.0, 0.0) : ld2(&pre[q]);
+    }
+  };
+  // this wave's tiles
+  const int t0 = uni(U->wt[wave]), nt = uni(U->wt[wave + 1]) - t0;
+  const int *tl = tiles + 17LL * (uni(U->tile_off) + t0);
+  int th[kSymWaveTiles];
+#pragma unroll
+  for (int t = 0; t < kSymWaveTiles; ++t) th[t] = t < nt ? uni(tl[17 * t]) : 0;
+  double p1[kSymWaveTiles], p2[kSymWaveTiles], p3[kSymWaveTiles];
+#pragma unroll
+  for (int t = 0; t < kSymWaveTiles; ++t) p1[t] = p2[t] = p3[t] = 0.0;
+  // per sub-step s: the lane's A and B element within a group of four positions
+  const int li = lane & 3, lg = (lane >> 2) & 3, lk = lane >> 4;
+  int offA[4], offB[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int r = 2 * s + (lk >> 1), c = 4 * (lk & 1) + lg;
+    offA[s] = sym_pos(li, r, c);
+    offB[s] = sym_pos(li, c, r);
+  }
+  auto compute = [&](int buf, bool self) {
+    int curR = -1;
+    double2 a[2][4];
+#pragma unroll
+    for (int t = 0; t < kSymWaveTiles; ++t) {
+      if (t < nt) {
+        const int R = th[t] & 255, C = th[t] >> 8;
+        if (R != curR) {
+          curR = R;
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) a[h][s] = Ls[buf][h][256 * R + offA[s]];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const double2 *bp = Ls[buf][self ? h : 1 - h] + 256 * C;
+          double2 b[4];
+#pragma unroll
+          for (int s = 0; s < 4; ++s) b[s] = bp[offB[s]];
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const double2 x = a[h][s], y = b[s];
+            p1[t] = __builtin_amdgcn_mfma_f64_4x4x4f64(x.x, y.x, p1[t], 0, 0, 0);
+            p2[t] = __builtin_amdgcn_mfma_f64_4x4x4f64(x.y, y.y, p2[t], 0, 0, 0);
+            p3[t] = __builtin_amdgcn_mfma_f64_4x4x4f64(x.x + x.y, y.x + y.y, p3[t], 0, 0, 0);
+          }
+        }
+      }
+    }
+  };
+  // one stage: the loads of stage k + 2 go out into the set stage k left,
+  // stage k is multiplied, stage k + 1 is written to the other buffer (whose
+  // stage k - 1 every wave finished before the last barrier)
+  // (nst is even; the last step's write goes to a buffer nobody reads)
+  auto step = [&](int k, T(&cur)[kSymPass], unsigned &zc, T(&nxt)[kSymPass], unsigned zn, int buf) {
+    const int flags = uni(st[k * rowlen]);
+    fetch(k + 2, cur, zc);
+    compute(buf, flags & 1);
+    commit(nxt, zn, buf ^ 1);
+    __syncthreads();
+  };
+  fetch(0, pre0, zero0);
+  fetch(1, pre1, zero1);
+  commit(pre0, zero0, 0);
+  __syncthreads();
+  for (int k = 0; k < nst; k += 2) {
+    step(k, pre0, zero0, pre1, zero1, 0);
+    step(k + 1, pre1, zero1, pre0, zero0, 1);
+  }
+  double2 *pb = part + (long long)ti * pstride;
+#pragma unroll
+  for (int t = 0; t < kSymWaveTiles; ++t) {
+    if (t < nt) {
+      double re = p1[t] - p2[t], im = p3[t] - p1[t] - p2[t];
+      re += dpp_d<0x128>(re);  // row_ror:8
+      im += dpp_d<0x128>(im);
+      re += dpp_d<0x124>(re);  // row_ror:4
+      im += dpp_d<0x124>(im);
+      if (lg == 0) {
+        const int slot = tl[17 * t + 1 + 4 * lk + li];
+        if (slot >= 0) pb[slot] = make_double2(re, im);
+      }
+    }
+  }
+}
+
 // D = 24 and D = 32 factor products on FP64 MFMA: one wavefront per
 // product, the D x D output as 2 x 2 tiles of 16 x 16 (rows and columns past
 // D of the D = 24 case are zero operands and are not stored), D / 4 k-steps of
@@ -2979,6 +3135,18 @@ void launch_sprod(const void *sQ0, const long long *Xt, const void *sQ2, const l
     using T = decltype(tag);
     hipLaunchKernelGGL(k_sprod16<T>, dim3(nunit, ntime), dim3(64 * kSpWaves), 0, s, static_cast<const T *>(sQ0), Xt,
                        static_cast<const T *>(sQ2), Yt, static_cast<T *>(out), obase, ostride, terms, units, nunit);
+  });
+  SLAPH_HIP(hipGetLastError());
+}
+
+void launch_trace_sym(const void *slab, const long long *Tt, const SymUnit *units, int nunit, const int *stages,
+                      const int *tiles, double2 *part, long long pstride, int ntime, int f32, hipStream_t s) {
+  if (nunit <= 0 || ntime <= 0) return;
+  if ((long long)nunit * ntime > 0x7fffffffLL) throw std::runtime_error("launch_trace_sym: grid too large");
+  with_storage(f32, [&](auto tag) {
+    using T = decltype(tag);
+    hipLaunchKernelGGL(k_trace_sym<T>, dim3(unsigned(nunit * ntime)), dim3(64 * kSymWaves), 0, s,
+                       static_cast<const T *>(slab), Tt, units, ntime, stages, tiles, part, pstride);
   });
   SLAPH_HIP(hipGetLastError());
 }

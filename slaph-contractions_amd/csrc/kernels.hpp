@@ -227,6 +227,34 @@ struct SpUnit {
 void launch_sprod(const void *slabQ0, const long long *Xt, const void *slabQ2, const long long *Yt, void *out,
                   long long obase, long long ostride, const STerm *terms, const SpUnit *units, int nunit, int ntime,
                   int f32, hipStream_t s);
+// Traces of the summed-product tables (D = 16) as symmetric products.  The
+// items of one connected component of a trace graph read the table matrices
+// M_e(l) of their two entries (e1, e2) as
+//   T[e1][e2] = sum_l tr(M_e1(l) M_e2(pi l)),
+// pi a pairing of the labels (involution), so that T is symmetric.  A unit
+// stages up to kSymEntries entries (positions p) and runs nst stages (an
+// even number); stage k holds two 8 x 8 blocks (slot 0, slot 1) of a matrix
+// of every staged entry and contributes two K steps of T = A B^T: "cross"
+// (flags bit 0 clear) pairs
+// slot 0 as A with slot 1 read transposed as B, then slot 1 with slot 0;
+// "self" (bit 0 set) pairs each slot with its own transpose.  The stage table
+// (stages + st_off) holds per stage 1 + 2 ns ints: the flags, then per
+// position p the element offsets of its slot 0 and slot 1 blocks (first
+// element) from the slice pair's time base, -1 for a zero block.  Tiles: per
+// tile 17 ints from tiles + 17 tile_off: row group | column group << 8 (four
+// positions each), then the partial-sum index of T[4 row + i][4 col + j] at
+// 1 + 4 i + j, -1 where none is needed; wave w of a unit runs tiles
+// [wt[w], wt[w + 1]) (at most kSymWaveTiles).  Output: part[ti * pstride +
+// index] for the slice pairs ti < ntime, table base Tt[ti] (matrices).
+constexpr int kSymEntries = 36;   // staged positions per unit (2 KiB of LDS each, two stage buffers)
+constexpr int kSymWaves = 8;      // wavefronts per unit
+constexpr int kSymWaveTiles = 6;  // tiles per wavefront
+struct SymUnit {
+  int st_off, nst, ns, tile_off;
+  int wt[kSymWaves + 1];
+};
+void launch_trace_sym(const void *slab, const long long *Tt, const SymUnit *units, int nunit, const int *stages,
+                      const int *tiles, double2 *part, long long pstride, int ntime, int f32, hipStream_t s);
 // In the trace launchers below, rt != 0 means the right matrices are stored
 // transposed (Y^T), so that tr(X Y) = sum_e X[e] Y^T[e]; the operand slabs
 // are double2 or (f32 != 0) float2 arrays, the group sums always double2.

@@ -29,7 +29,8 @@ pytestmark = pytest.mark.gpu
 
 SEED = 20261203
 SWITCHES = ("SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS", "SLAPH_RESOLVE_GROUPS", "SLAPH_PROD_GENERIC",
-            "SLAPH_VDV_DIRECT", "SLAPH_TRACE_GROUPS", "SLAPH_LSTAT_WINDOW", "SLAPH_GRID_GROUPS", "SLAPH_XFORM_OFF")
+            "SLAPH_VDV_DIRECT", "SLAPH_TRACE_GROUPS", "SLAPH_LSTAT_WINDOW", "SLAPH_GRID_GROUPS", "SLAPH_XFORM_OFF",
+            "SLAPH_SYM_OFF")
 COUNTS = {"pipi32": 58489}
 
 
@@ -81,7 +82,9 @@ def _agree(a, b, tol=1e-12):
 @pytest.mark.parametrize("workload", ["pipi32", "c2"])
 def test_production_and_alternative_kernels_agree(slaph, workload, monkeypatch):
     """Every correlator of the workload from the production kernels against
-    (a) the block-form left-stationary and item-wise tile trace kernels and
+    (a) the block-form left-stationary and item-wise tile trace kernels (the
+    summed-product traces of C4cC on the left-stationary kernel instead of
+    as symmetric products) and
     (b) direct group loops for the masked inner products (no Moebius
     inversion: at pipi32 this checks the 16-term subset sums of all 43,561
     C6cD correlators), the generic product kernel, the direct VdaggerV and
@@ -91,7 +94,7 @@ def test_production_and_alternative_kernels_agree(slaph, workload, monkeypatch):
     _set_switches(monkeypatch, ())
     say(workload, "production kernels")
     prod = _correlators(slaph, text)
-    for alt in (("SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS"),
+    for alt in (("SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS", "SLAPH_SYM_OFF"),
                 ("SLAPH_RESOLVE_GROUPS", "SLAPH_PROD_GENERIC", "SLAPH_VDV_DIRECT", "SLAPH_XFORM_OFF")):
         _set_switches(monkeypatch, alt)
         say(workload, "alternative kernels", "+".join(alt))

@@ -233,9 +233,10 @@ WIDE = SMALL.replace("C4cB:Q0:Op0:Q0:Op0:Q0:Op0:Q0:Op0:P0\n", "C4cB:Q0:Op0:Q0:Op
 
 @pytest.mark.parametrize("infile,kernels", [(SMALL, "lstat+mfma"), (WIDE, "lstat+mfma"), (WIDE, "groups+generic"),
                                             (WIDE, "blocks+items"), (WIDE, "lstat+mfma+window1"),
-                                            (WIDE, "lstat+mfma+crossed_direct")],
+                                            (WIDE, "lstat+mfma+crossed_direct"), (WIDE, "lstat+mfma+crossed_lstat"),
+                                            (WIDE, "lstat+mfma+sym_split")],
                          ids=["small", "wide", "wide-alt-kernels", "wide-block-traces", "wide-one-round-windows",
-                              "wide-crossed-direct"])
+                              "wide-crossed-direct", "wide-crossed-lstat", "wide-crossed-sym-split"])
 def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypatch):
     """Config-1-like lattice (4^3 x 8, N=8, dilE 4, 4 random vectors) with
     seeded host inputs: every correlator against the oracle at 1e-12
@@ -254,15 +255,20 @@ def test_host_inputs_small_pipi_matches_oracle(slaph, infile, kernels, monkeypat
     form (products summed over the middle random vector, the coincident
     chains removed by inclusion and exclusion) except in the crossed-direct
     variant (one product per combination) and the alternative-kernels one
-    (one wave per group, which the summed form does not use)."""
+    (one wave per group, which the summed form does not use); their traces
+    run as symmetric products per trace-graph component, in the crossed-lstat
+    variant on the left-stationary kernel, in the sym-split variant with
+    every component of more than four entries cut into chunks of four (pieces
+    of two chunks, rows from one and columns from the other)."""
     say("small pipi", "wide" if infile is WIDE else "", kernels)
     for var, on in (("SLAPH_TRACE_GROUPS", "groups" in kernels), ("SLAPH_PROD_GENERIC", "generic" in kernels),
                     ("SLAPH_RESOLVE_GROUPS", "groups" in kernels), ("SLAPH_VDV_DIRECT", "groups" in kernels),
                     ("SLAPH_LSTAT_BLOCKS", "blocks" in kernels), ("SLAPH_GRID_ITEMS", "items" in kernels),
                     ("SLAPH_LSTAT_WINDOW", "window1" in kernels), ("SLAPH_GRID_GROUPS", "window1" in kernels),
-                    ("SLAPH_XFORM_OFF", "crossed_direct" in kernels)):
+                    ("SLAPH_XFORM_OFF", "crossed_direct" in kernels), ("SLAPH_SYM_OFF", "crossed_lstat" in kernels),
+                    ("SLAPH_SYM_ENTRIES", "sym_split" in kernels)):
         if on:
-            monkeypatch.setenv(var, "1")
+            monkeypatch.setenv(var, "4" if var == "SLAPH_SYM_ENTRIES" else "1")
         else:
             monkeypatch.delenv(var, raising=False)
     cfg = o.parse_infile(infile)
@@ -325,7 +331,7 @@ _DIL_WANT = {}  # (dilE, N) -> (oracle correlators, inputs)
 
 KERNEL_SWITCHES = ("SLAPH_TRACE_GROUPS", "SLAPH_PROD_GENERIC", "SLAPH_RESOLVE_GROUPS", "SLAPH_VDV_DIRECT",
                    "SLAPH_LSTAT_BLOCKS", "SLAPH_GRID_ITEMS", "SLAPH_LSTAT_WINDOW", "SLAPH_GRID_GROUPS",
-                   "SLAPH_XFORM_OFF")
+                   "SLAPH_XFORM_OFF", "SLAPH_SYM_OFF", "SLAPH_SYM_ENTRIES")
 
 
 @pytest.mark.parametrize("dilE,N", [(6, 14), (6, 16), (8, 20)], ids=["EI6-D24", "EI6-D24-N16", "EI8-D32"])
@@ -395,18 +401,24 @@ correlator_list = C2c:Q0:Op0:Q1:Op0:P0,1
 _TWO_WANT = {}
 
 
-@pytest.mark.parametrize("form", ["summed", "per-combination"])
+@pytest.mark.parametrize("form", ["summed", "summed-lstat", "summed-split", "per-combination"])
 def test_crossed_traces_two_flavours_mixed_gammas(slaph, form, monkeypatch):
     """D = 16 crossed traces (C4cC) with two quark flavours (u u / s s random
     vector pairs of different sets, so the products' and traces' admitted
     chains differ per factor) and a gamma_2 operator beside gamma_5 (Q0's
-    nonzero Dirac blocks off the diagonal): the summed-product form and one
-    product per combination against the oracle at 1e-12 norm-wise."""
+    nonzero Dirac blocks off the diagonal): the summed-product form (its
+    traces as symmetric products, also with components cut into chunks of
+    four entries, and on the left-stationary kernel) and one product per
+    combination against the oracle at 1e-12 norm-wise."""
     say("two flavours, mixed gammas", form)
     for var in KERNEL_SWITCHES:
         monkeypatch.delenv(var, raising=False)
     if form == "per-combination":
         monkeypatch.setenv("SLAPH_XFORM_OFF", "1")
+    elif form == "summed-lstat":
+        monkeypatch.setenv("SLAPH_SYM_OFF", "1")
+    elif form == "summed-split":
+        monkeypatch.setenv("SLAPH_SYM_ENTRIES", "4")
     ctx = slaph.Context(0)
     ctx.configure(TWO_FLAVOURS)
     ctx.synthetic_inputs(4242, True)
@@ -666,13 +678,13 @@ def test_autotune_keeps_correlators_and_tuning_text(slaph):
     rep = ctx.autotune(1)
     knobs = {k["knob"] for k in rep["knobs"]}
     assert knobs == {"vdaggerv_direct", "lstat_wg_per_cu", "gemm_wg_per_cu", "prod_generic", "resolve_groups",
-                     "crossed_direct"}
+                     "crossed_direct", "crossed_lstat"}
     assert all(len(k["ms"]) >= 2 and all(v > 0 for v in k["ms"].values()) for k in rep["knobs"])
     assert rep["chosen"] == ctx.tuning()
     slaph.contract(ctx)
     assert _rel_errors({d: ctx.correlators(d) for d in ctx.diagrams()}, ref).max() <= 1e-12
     for text in ("vdaggerv_direct=1", "lstat_wg_per_cu=1", "lstat_wg_per_cu=3", "gemm_wg_per_cu=8",
-                 "prod_generic=1", "resolve_groups=1", "crossed_direct=1"):
+                 "prod_generic=1", "resolve_groups=1", "crossed_direct=1", "crossed_lstat=1"):
         ctx.set_tuning(default)
         ctx.set_tuning(text)
         ops.create_operators()
