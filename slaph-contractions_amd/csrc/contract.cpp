@@ -301,8 +301,12 @@ struct Family {
 
 struct CorrPlan {
   std::string name;
-  std::vector<std::vector<std::pair<int, int>>> reqs;  // correlator -> [(family, trl)]
-  std::vector<std::string> datasets;
+  // correlators in the device order (ordered by their traces, so that the
+  // lanes of a resolve wave share traces): position -> [(family, trl)]
+  std::vector<std::vector<std::pair<int, int>>> reqs;
+  std::vector<int> orig;              // position -> index in the request list (and in datasets)
+  std::vector<std::string> datasets;  // in request order
+  DevBuf d_orig;
   std::vector<double> n_adm;  // admitted combinations per correlator
 };
 
@@ -1460,6 +1464,16 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
         cp.reqs.push_back(refs);
         cp.datasets.push_back(req.dataset);
       }
+      // device order: by traces (family, trace id), ties in request order
+      cp.orig.resize(cp.reqs.size());
+      for (size_t i = 0; i < cp.orig.size(); ++i) cp.orig[i] = int(i);
+      std::stable_sort(cp.orig.begin(), cp.orig.end(), [&](int x, int y) { return cp.reqs[x] < cp.reqs[y]; });
+      {
+        std::vector<std::vector<std::pair<int, int>>> sorted(cp.reqs.size());
+        for (size_t i = 0; i < cp.orig.size(); ++i) sorted[i] = cp.reqs[cp.orig[i]];
+        cp.reqs = std::move(sorted);
+      }
+      upload(cp.d_orig, cp.orig);
       corrs.push_back(std::move(cp));
     }
 
@@ -1874,7 +1888,7 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
           for (size_t trl = 0; trl < f.tr_ids.size(); ++trl) {
             if (!multi[fi][trl] || f.mirror_of[trl] >= 0) continue;
             f.zslot[trl] = int(f.zitems.size());
-            f.zitems.push_back({(long long)f.goff[trl], f.ngroup[trl], f.gmask_off[trl], (long long)f.zitems.size() * nz});
+            f.zitems.push_back({(long long)f.goff[trl], f.ngroup[trl], f.gmask_off[trl]});
           }
         for (size_t trl = 0; trl < f.tr_ids.size(); ++trl)
           if (f.mirror_of[trl] >= 0) f.zslot[trl] = f.zslot[f.mirror_of[trl]];
@@ -2110,6 +2124,7 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
           std::vector<CorrJob> jobs(cp.reqs.size());
           for (size_t i = 0; i < cp.reqs.size(); ++i) {
             jobs[i].ntr = int(cp.reqs[i].size());
+            jobs[i].out = int(i);
             jobs[i].n = cp.n_adm[i];
             for (size_t k = 0; k < cp.reqs[i].size(); ++k) {
               auto const &f = fams[cp.reqs[i][k].first];
@@ -2129,13 +2144,17 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
             for (size_t k = 0; k < 3; ++k) {
               jobs[i].tr[k].zbase = -1;
               jobs[i].tr[k].zstride = 0;
+              jobs[i].tr[k].zq = 0;
             }
             for (size_t k = 0; z_ok && k < cp.reqs[i].size(); ++k) {
               auto const &f = fams[cp.reqs[i][k].first];
-              const long long nz = (long long)C.zmask.size();
-              jobs[i].tr[k].zbase = ch.zbase[cp.reqs[i][k].first] + f.zslot[cp.reqs[i][k].second] * nz;
+              const long long nz = (long long)C.zmask.size(), nit = (long long)f.zitems.size();
+              // subset-major: Z(S_q) of item i at zbase + q * nit + i
+              jobs[i].tr[k].zbase = ch.zbase[cp.reqs[i][k].first] + f.zslot[cp.reqs[i][k].second];
               jobs[i].tr[k].zstride = f.zstride;
-              if (jobs[i].tr[k].zbase + (long long)(S - 1) * f.zstride + nz > ch.z_total)
+              jobs[i].tr[k].zq = int(nit);
+              if (nit > INT32_MAX ||
+                  jobs[i].tr[k].zbase + (long long)(S - 1) * f.zstride + (nz - 1) * nit + 1 > ch.z_total)
                 throw std::logic_error("correlator request indexes outside the subset-sum buffer");
             }
           }
@@ -2336,7 +2355,8 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
     for (auto const &cp : corrs) res_total += (size_t)Lt_ * cp.reqs.size();
     C.d_res.ensure(sizeof(double2) * std::max<size_t>(1, res_total));
     for (size_t c = 0, off = 0; c < corrs.size(); off += (size_t)Lt_ * corrs[c].reqs.size(), ++c)
-      launch_finalize(d_sum[c].as<double2>(), int(corrs[c].reqs.size()), Lt_, C.d_res.as<double2>() + off, stream_);
+      launch_finalize(d_sum[c].as<double2>(), int(corrs[c].reqs.size()), Lt_, corrs[c].d_orig.as<int>(),
+                      C.d_res.as<double2>() + off, stream_);
     if (graph) {
       capture.on = false;
       SLAPH_HIP(hipStreamEndCapture(stream_, &C.graph));
@@ -2364,6 +2384,7 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
       r.name = corrs[c].name;
       r.names = corrs[c].datasets;
     }
+    // k_finalize wrote the values in request order
     r.values.resize((size_t)Lt_ * corrs[c].reqs.size());
     SLAPH_HIP(hipMemcpy(r.values.data(), C.d_res.as<double2>() + off, sizeof(cplx) * r.values.size(),
                         hipMemcpyDeviceToHost));

@@ -1515,10 +1515,10 @@ __global__ __launch_bounds__(64 * kSpWaves) void k_sprod16(
 // k + 2's loads are in flight (two register sets); one barrier per stage.
 // Lanes 16 i + j hold T[i][j] of a tile after the sum over the blocks g (row
 // rotations by 8 and 4).
-constexpr int kSymPass = 2 * kSymEntries / kSymWaves;  // staged blocks per wave and stage
+constexpr int kSymPass = (2 * kSymEntries + kSymWaves - 1) / kSymWaves;  // staged blocks per wave and stage
 __device__ __forceinline__ int sym_pos(int p, int r, int c) { return 64 * p + ((8 * r + (c ^ r)) ^ ((p & 3) << 2)); }
 template <typename T>
-__global__ __launch_bounds__(64 * kSymWaves) __attribute__((amdgpu_waves_per_eu(2))) void k_trace_sym(
+__global__ __launch_bounds__(64 * kSymWaves) __attribute__((amdgpu_waves_per_eu(kSymWaves / 4))) void k_trace_sym(
     const T *__restrict__ slab, const long long *__restrict__ Tt, const SymUnit *__restrict__ units, int ntime,
     const int *__restrict__ stages, const int *__restrict__ tiles, double2 *__restrict__ part, long long pstride) {
   // two stage buffers (stage k in buffer k % 2), two 8 x 8 blocks per position
@@ -2522,17 +2522,22 @@ __global__ void k_zeta(const double2 *__restrict__ gv, long long gbase, long lon
                        const uint16_t *__restrict__ gm, const ZItem *__restrict__ items, int nitems,
                        const uint16_t *__restrict__ zmask, int nz, double2 *__restrict__ zv, long long zbase,
                        long long zstride, int nsp) {
+  // one thread per (slice pair, item), every subset in turn: subset-major
+  // output, so consecutive threads write consecutive items of one subset
   const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (id >= (long long)nsp * nitems * nz) return;
-  const int q = int(id % nz), it = int((id / nz) % nitems), s = int(id / ((long long)nz * nitems));
+  if (id >= (long long)nsp * nitems) return;
+  const int it = int(id % nitems), s = int(id / nitems);
   const ZItem I = items[it];
   const double2 *v = gv + gbase + s * gstride + I.goff;
   const uint16_t *m = gm + I.gmask_off;
-  const uint16_t S = zmask[q];
-  double2 z = make_double2(0.0, 0.0);
-  for (int g = 0; g < I.ngroup; ++g)
-    if ((m[g] & S) == S) z = cadd(z, v[g]);
-  zv[zbase + s * zstride + I.zoff + q] = z;
+  double2 *out = zv + zbase + s * zstride + it;
+  for (int q = 0; q < nz; ++q) {
+    const uint16_t S = zmask[q];
+    double2 z = make_double2(0.0, 0.0);
+    for (int g = 0; g < I.ngroup; ++g)
+      if ((m[g] & S) == S) z = cadd(z, v[g]);
+    out[(long long)q * nitems] = z;
+  }
 }
 
 // Neumaier-Kahan accumulator of KahanAccumulator<Complex>::operator+=
@@ -2607,12 +2612,15 @@ __global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__rest
   } else if (J.ntr == 1) {
     for (int g = 0; g < J.tr[0].ngroup; ++g) res = cadd(res, v0[g]);
   } else if (J.tr[0].zbase >= 0) {
+    // subset-major sums: the lanes of a wave (jobs ordered by their traces)
+    // share their first traces and read neighbouring items of the last
     const double2 *z0 = zv + J.tr[0].zbase + s * J.tr[0].zstride;
     const double2 *z1 = zv + J.tr[1].zbase + s * J.tr[1].zstride;
     const double2 *z2 = J.ntr == 3 ? zv + J.tr[2].zbase + s * J.tr[2].zstride : nullptr;
+    const int q0 = J.tr[0].zq, q1 = J.tr[1].zq, q2 = J.ntr == 3 ? J.tr[2].zq : 0;
     for (int q = 0; q < nz; ++q) {
-      double2 t = cmul(z0[q], z1[q]);
-      if (z2) t = cmul(t, z2[q]);
+      double2 t = cmul(z0[q * q0], z1[q * q1]);
+      if (z2) t = cmul(t, z2[q * q2]);
       if (__popc(zmask[q]) & 1) {
         res.x -= t.x;
         res.y -= t.y;
@@ -2642,7 +2650,7 @@ __global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__rest
         res = cadd(res, cmul(cmul(v0[g], v1[h]), rs));
       }
   }
-  cval[(long long)s * ncorr + corr] = make_double2(res.x / J.n, res.y / J.n);
+  cval[(long long)s * ncorr + J.out] = make_double2(res.x / J.n, res.y / J.n);
 }
 
 // correlator_[t][i] += c (reference: Diagram::assemble, Diagram.cpp:96-114).
@@ -2665,12 +2673,13 @@ __global__ void k_accumulate(const double2 *__restrict__ cval, const int *__rest
   sum[at] = acc;
 }
 
-__global__ void k_finalize(const double2 *__restrict__ sum, int ncorr, int Lt, double2 *__restrict__ out) {
+__global__ void k_finalize(const double2 *__restrict__ sum, int ncorr, int Lt, const int *__restrict__ orig,
+                           double2 *__restrict__ out) {
   const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (id >= (long long)ncorr * Lt) return;
   const int t = int(id % Lt), i = int(id / Lt);
   const long long at = (long long)t * ncorr + i;
-  out[id] = make_double2(sum[at].x / double(Lt), sum[at].y / double(Lt));
+  out[(long long)(orig ? orig[i] : i) * Lt + t] = make_double2(sum[at].x / double(Lt), sum[at].y / double(Lt));
 }
 
 template <typename T>
@@ -3298,18 +3307,18 @@ void launch_to_storage(const double2 *in, void *out, long long n, int form, hipS
   SLAPH_HIP(hipGetLastError());
 }
 
-void launch_finalize(const double2 *sum, int ncorr, int Lt, double2 *out, hipStream_t s) {
+void launch_finalize(const double2 *sum, int ncorr, int Lt, const int *orig, double2 *out, hipStream_t s) {
   const long long n = (long long)ncorr * Lt;
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_finalize, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, sum, ncorr, Lt, out);
+  hipLaunchKernelGGL(k_finalize, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, sum, ncorr, Lt, orig, out);
   SLAPH_HIP(hipGetLastError());
 }
 
 void launch_zeta(const double2 *gv, long long gbase, long long gstride, const uint16_t *gm, const ZItem *items,
                  int nitems, const uint16_t *zmask, int nz, double2 *zv, long long zbase, long long zstride, int nsp,
                  hipStream_t s) {
-  long long n = (long long)nsp * nitems * nz;
-  if (n <= 0) return;
+  long long n = (long long)nsp * nitems;
+  if (n <= 0 || nz <= 0) return;
   hipLaunchKernelGGL(k_zeta, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, gv, gbase, gstride, gm, items, nitems,
                      zmask, nz, zv, zbase, zstride, nsp);
   SLAPH_HIP(hipGetLastError());

@@ -270,24 +270,26 @@ struct TraceRef {
   long long stride;  // group-sum stride per slice pair
   int ngroup;        // number of mask groups
   int gmask_off;     // into group masks[]
-  long long zbase;   // offset of the subset sums Z(S) at slice pair 0, or -1
-  long long zstride;
+  long long zbase;   // offset of the subset sum Z(S_0) at slice pair 0, or -1
+  long long zstride;  // subset-sum stride per slice pair
+  int zq;             // stride between consecutive subsets S_q (the family's subset-sum items)
 };
 // Subset sums of one trace vector: Z(S) = sum of the group sums whose mask
-// contains S, for every S of the subset list zmask[0..nz).  The masked inner
+// contains S, for every S of the subset list zmask[0..nz), stored subset-major
+// per slice pair: Z(S_q) of item i at zbase + s * zstride + q * nitems + i.  The masked inner
 // products then follow from Moebius inversion on the subset lattice,
 // [m0 & m1 (& m2) == 0] = sum_{S subset of m0 & m1 (& m2)} (-1)^|S|, i.e.
 // sum_S (-1)^|S| Z0(S) Z1(S) (Z2(S)).
 struct ZItem {
   long long goff;  // the trace's group sums within one slice pair
   int ngroup, gmask_off;
-  long long zoff;  // its subset sums within one slice pair
 };
 void launch_zeta(const double2 *gvals, long long gbase, long long gstride, const uint16_t *gmasks, const ZItem *items,
                  int nitems, const uint16_t *zmask, int nz, double2 *zv, long long zbase, long long zstride, int nsp,
                  hipStream_t s);
 struct CorrJob {
   int ntr;
+  int out;   // the correlator's index in the output (jobs are ordered by their traces)
   double n;  // number of admitted combinations (host-computed, exact); the value is divided by it
   TraceRef tr[3];
 };
@@ -425,8 +427,9 @@ void launch_resolve(const double2 *gvals, const uint16_t *gmasks, const CorrJob 
 void launch_accumulate(const double2 *cval, const int *sidx, const int4 *tgroups, int ngroups, int ncorr,
                        double2 *sum, hipStream_t s);
 void launch_scale(double2 *x, long long n, double f, hipStream_t s);
-// Correlator values (Diagram.cpp:132-147): out[i*Lt + t] = sum[t*ncorr + i] / Lt.
-void launch_finalize(const double2 *sum, int ncorr, int Lt, double2 *out, hipStream_t s);
+// Correlator values (Diagram.cpp:132-147): out[o*Lt + t] = sum[t*ncorr + i] / Lt
+// with o = orig[i] (the correlator's request index; i when orig is null).
+void launch_finalize(const double2 *sum, int ncorr, int Lt, const int *orig, double2 *out, hipStream_t s);
 
 // Synthetic inputs: seeded counter-based Gaussian / Z2xZ2 fills.
 void launch_fill_gauss(double2 *x, long long n, unsigned long long seed, hipStream_t s);
