@@ -2513,7 +2513,7 @@ __global__ void k_gemm_reduce(const double2 *__restrict__ part, int KP, const Gr
 // resolve_request{1,2,3} (reference: Diagram.cpp:9-94, DilutedTrace.cpp:5-82)
 // on mask-group sums: mean, sum_g L_g sum_{h: m_g & m_h = 0} R_h, and the
 // 3-way form that rejects only masks shared by all three traces
-// (DilutedTrace.cpp:63).  One thread per (slice pair, correlator).  When the
+// (DilutedTrace.cpp:63).  One thread per (correlator, up to 8 slice pairs).  When the
 // traces carry subset sums (zbase >= 0), the 2- and 3-way forms are evaluated
 // by Moebius inversion: sum_S (-1)^|S| Z0(S) Z1(S) (Z2(S)) over the subset
 // list, the same sum re-associated (nz terms instead of ngroup^ntr).
@@ -2570,16 +2570,21 @@ struct KahanC {
 // admitted combinations, as the reference does.
 __global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__restrict__ gm,
                           const CorrJob *__restrict__ jobs, int ncorr, int nsp, const double2 *__restrict__ zv,
-                          const uint16_t *__restrict__ zmask, int nz, double2 *__restrict__ cval, int kahan) {
+                          const uint16_t *__restrict__ zmask, int nz, double2 *__restrict__ cval, int kahan, int spt) {
   // XCD x (workgroups are dealt to the XCDs round-robin) runs the x-th
   // contiguous eighth of the (slice pair, correlator) sequence, so each XCD's
   // L2 holds the trace values of its own slice pairs only.
   const int NB = gridDim.x, per = NB / 8, rem = NB % 8, xcd = blockIdx.x % 8, jx = blockIdx.x / 8;
   const int lb = xcd < rem ? xcd * (per + 1) + jx : rem * (per + 1) + (xcd - rem) * per + jx;
+  // one thread per (correlator, spt slice pairs): the job record is read
+  // once for all of them, and for each slice pair the lanes of a wave write
+  // consecutive correlators
   const long long id = lb * (long long)blockDim.x + threadIdx.x;
-  if (id >= (long long)ncorr * nsp) return;
-  const int corr = int(id % ncorr), s = int(id / ncorr);
+  const int nsb = (nsp + spt - 1) / spt;
+  if (id >= (long long)ncorr * nsb) return;
+  const int corr = int(id % ncorr), sb = int(id / ncorr);
   const CorrJob J = jobs[corr];
+  for (int s = sb * spt; s < min(nsp, (sb + 1) * spt); ++s) {
   const double2 *v0 = gv + J.tr[0].base + s * J.tr[0].stride;
   double2 res = make_double2(0.0, 0.0);
   if (kahan) {
@@ -2613,13 +2618,18 @@ __global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__rest
     for (int g = 0; g < J.tr[0].ngroup; ++g) res = cadd(res, v0[g]);
   } else if (J.tr[0].zbase >= 0) {
     // subset-major sums: the lanes of a wave (jobs ordered by their traces)
-    // share their first traces and read neighbouring items of the last
-    const double2 *z0 = zv + J.tr[0].zbase + s * J.tr[0].zstride;
-    const double2 *z1 = zv + J.tr[1].zbase + s * J.tr[1].zstride;
+    // mostly share their first two traces, read then through scalar loads,
+    // and read neighbouring items of the last
+    const long long b0 = J.tr[0].zbase + s * J.tr[0].zstride, b1 = J.tr[1].zbase + s * J.tr[1].zstride;
     const double2 *z2 = J.ntr == 3 ? zv + J.tr[2].zbase + s * J.tr[2].zstride : nullptr;
     const int q0 = J.tr[0].zq, q1 = J.tr[1].zq, q2 = J.ntr == 3 ? J.tr[2].zq : 0;
-    for (int q = 0; q < nz; ++q) {
-      double2 t = cmul(z0[q * q0], z1[q * q1]);
+    auto rfl = [](long long v) {
+      return (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(int(v >> 32)) << 32) |
+                         (unsigned)__builtin_amdgcn_readfirstlane(int(v)));
+    };
+    const long long u0 = rfl(b0), u1 = rfl(b1);
+    const int uq0 = __builtin_amdgcn_readfirstlane(q0), uq1 = __builtin_amdgcn_readfirstlane(q1);
+    auto term = [&](double2 t, int q) {
       if (z2) t = cmul(t, z2[q * q2]);
       if (__popc(zmask[q]) & 1) {
         res.x -= t.x;
@@ -2627,6 +2637,15 @@ __global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__rest
       } else {
         res = cadd(res, t);
       }
+    };
+    if (__all(b0 == u0 && b1 == u1 && q0 == uq0 && q1 == uq1)) {
+      const double2 *z0 = zv + u0, *z1 = zv + u1;
+#pragma unroll 4
+      for (int q = 0; q < nz; ++q) term(cmul(z0[q * uq0], z1[q * uq1]), q);
+    } else {
+      const double2 *z0 = zv + b0, *z1 = zv + b1;
+#pragma unroll 4
+      for (int q = 0; q < nz; ++q) term(cmul(z0[q * q0], z1[q * q1]), q);
     }
   } else if (J.ntr == 2) {
     const double2 *v1 = gv + J.tr[1].base + s * J.tr[1].stride;
@@ -2651,6 +2670,7 @@ __global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__rest
       }
   }
   cval[(long long)s * ncorr + J.out] = make_double2(res.x / J.n, res.y / J.n);
+  }
 }
 
 // correlator_[t][i] += c (reference: Diagram::assemble, Diagram.cpp:96-114).
@@ -3326,11 +3346,14 @@ void launch_zeta(const double2 *gv, long long gbase, long long gstride, const ui
 
 void launch_resolve(const double2 *gv, const uint16_t *gm, const CorrJob *jobs, int ncorr, int nsp,
                     const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, int kahan, hipStream_t s) {
-  long long n = (long long)ncorr * nsp;
+  // slice pairs per thread: up to 8 while at least 2^18 threads remain
+  int spt = 1;
+  while (spt < 8 && (long long)ncorr * ((nsp + 2 * spt - 1) / (2 * spt)) >= (1LL << 18)) spt *= 2;
+  long long n = (long long)ncorr * ((nsp + spt - 1) / spt);
   if (n <= 0) return;
   if ((n + 255) / 256 > 0x7fffffffLL) throw std::runtime_error("launch_resolve: grid too large");
   hipLaunchKernelGGL(k_resolve, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, gv, gm, jobs, ncorr, nsp, zv, zmask,
-                     nz, cval, kahan);
+                     nz, cval, kahan, spt);
   SLAPH_HIP(hipGetLastError());
 }
 
