@@ -1353,6 +1353,7 @@ struct ChunkPlan {
   DevBuf d_tds, d_tdg;                        // slice pairs grouped by time separation (ascending in a group)
   int ntdg = 0;
   std::vector<DevBuf> d_jobs;                 // per diagram: resolve jobs
+  std::vector<char> zonly;                    // per diagram: every job in the Moebius form
 };
 
 }  // namespace
@@ -2119,6 +2120,7 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
           ch.ntdg = int(tdg.size());
         }
         ch.d_jobs.resize(corrs.size());
+        ch.zonly.assign(corrs.size(), 0);
         for (size_t c = 0; c < corrs.size(); ++c) {
           auto const &cp = corrs[c];
           std::vector<CorrJob> jobs(cp.reqs.size());
@@ -2158,6 +2160,9 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
                 throw std::logic_error("correlator request indexes outside the subset-sum buffer");
             }
           }
+          ch.zonly[c] = !jobs.empty() && std::all_of(jobs.begin(), jobs.end(), [](CorrJob const &j) {
+            return j.ntr > 1 && j.tr[0].zbase >= 0;
+          });
           upload(ch.d_jobs[c], jobs);
         }
         C.chunks.push_back(std::move(ch));
@@ -2339,7 +2344,7 @@ void Engine::run_contract(std::shared_ptr<ContractCache> &plan_, int single_bloc
         int ncorr = int(corrs[c].reqs.size());
         launch_resolve(C.d_gv.as<double2>(), C.d_gmasks.as<uint16_t>(), ch.d_jobs[c].as<CorrJob>(), ncorr, S,
                        C.d_zv.as<double2>(), C.d_zmask.as<uint16_t>(), int(C.zmask.size()), C.d_cval.as<double2>(),
-                       per_element ? 1 : 0, stream_);
+                       per_element ? 1 : 0, ch.zonly[c], stream_);
         launch_accumulate(C.d_cval.as<double2>(), ch.d_tds.as<int>(), ch.d_tdg.as<int4>(), ch.ntdg, ncorr,
                           d_sum[c].as<double2>(), stream_);
       }
@@ -2472,7 +2477,7 @@ cplx Engine::inner_product(int ntr, const cplx *const *vals, const uint16_t *con
   upload(dj, std::vector<CorrJob>{job});
   dout.alloc(sizeof(double2));
   launch_resolve(dv.as<double2>(), dm.as<uint16_t>(), dj.as<CorrJob>(), 1, 1, nullptr, nullptr, 0, dout.as<double2>(),
-                 kahan ? 1 : 0, stream_);
+                 kahan ? 1 : 0, 0, stream_);
   cplx r;
   SLAPH_HIP(hipMemcpyAsync(&r, dout.as<void>(), sizeof(cplx), hipMemcpyDeviceToHost, stream_));
   SLAPH_HIP(hipStreamSynchronize(stream_));

@@ -2673,6 +2673,54 @@ __global__ void k_resolve(const double2 *__restrict__ gv, const uint16_t *__rest
   }
 }
 
+// The Moebius form alone (every job of the launch has subset sums and no
+// Kahan form): the same sums as k_resolve's with a fraction of its registers,
+// so that more waves hide the load latency.
+__global__ void k_resolve_z(const CorrJob *__restrict__ jobs, int ncorr, int nsp, const double2 *__restrict__ zv,
+                            const uint16_t *__restrict__ zmask, int nz, double2 *__restrict__ cval, int spt) {
+  const int NB = gridDim.x, per = NB / 8, rem = NB % 8, xcd = blockIdx.x % 8, jx = blockIdx.x / 8;
+  const int lb = xcd < rem ? xcd * (per + 1) + jx : rem * (per + 1) + (xcd - rem) * per + jx;
+  const long long id = lb * (long long)blockDim.x + threadIdx.x;
+  const int nsb = (nsp + spt - 1) / spt;
+  if (id >= (long long)ncorr * nsb) return;
+  const int corr = int(id % ncorr), sb = int(id / ncorr);
+  const CorrJob J = jobs[corr];
+  for (int s = sb * spt; s < min(nsp, (sb + 1) * spt); ++s) {
+  double2 res = make_double2(0.0, 0.0);
+    // subset-major sums: the lanes of a wave (jobs ordered by their traces)
+    // mostly share their first two traces, read then through scalar loads,
+    // and read neighbouring items of the last
+    const long long b0 = J.tr[0].zbase + s * J.tr[0].zstride, b1 = J.tr[1].zbase + s * J.tr[1].zstride;
+    const double2 *z2 = J.ntr == 3 ? zv + J.tr[2].zbase + s * J.tr[2].zstride : nullptr;
+    const int q0 = J.tr[0].zq, q1 = J.tr[1].zq, q2 = J.ntr == 3 ? J.tr[2].zq : 0;
+    auto rfl = [](long long v) {
+      return (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(int(v >> 32)) << 32) |
+                         (unsigned)__builtin_amdgcn_readfirstlane(int(v)));
+    };
+    const long long u0 = rfl(b0), u1 = rfl(b1);
+    const int uq0 = __builtin_amdgcn_readfirstlane(q0), uq1 = __builtin_amdgcn_readfirstlane(q1);
+    auto term = [&](double2 t, int q) {
+      if (z2) t = cmul(t, z2[q * q2]);
+      if (__popc(zmask[q]) & 1) {
+        res.x -= t.x;
+        res.y -= t.y;
+      } else {
+        res = cadd(res, t);
+      }
+    };
+    if (__all(b0 == u0 && b1 == u1 && q0 == uq0 && q1 == uq1)) {
+      const double2 *z0 = zv + u0, *z1 = zv + u1;
+#pragma unroll 4
+      for (int q = 0; q < nz; ++q) term(cmul(z0[q * uq0], z1[q * uq1]), q);
+    } else {
+      const double2 *z0 = zv + b0, *z1 = zv + b1;
+#pragma unroll 4
+      for (int q = 0; q < nz; ++q) term(cmul(z0[q * q0], z1[q * q1]), q);
+    }
+  cval[(long long)s * ncorr + J.out] = make_double2(res.x / J.n, res.y / J.n);
+  }
+}
+
 // correlator_[t][i] += c (reference: Diagram::assemble, Diagram.cpp:96-114).
 // The reference gives every slice pair a fresh accumulator holding its one
 // value and merges it with KahanAccumulator::operator+=(KahanAccumulator)
@@ -3345,15 +3393,20 @@ void launch_zeta(const double2 *gv, long long gbase, long long gstride, const ui
 }
 
 void launch_resolve(const double2 *gv, const uint16_t *gm, const CorrJob *jobs, int ncorr, int nsp,
-                    const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, int kahan, hipStream_t s) {
+                    const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, int kahan, int zonly,
+                    hipStream_t s) {
   // slice pairs per thread: up to 8 while at least 2^18 threads remain
   int spt = 1;
   while (spt < 8 && (long long)ncorr * ((nsp + 2 * spt - 1) / (2 * spt)) >= (1LL << 18)) spt *= 2;
   long long n = (long long)ncorr * ((nsp + spt - 1) / spt);
   if (n <= 0) return;
   if ((n + 255) / 256 > 0x7fffffffLL) throw std::runtime_error("launch_resolve: grid too large");
-  hipLaunchKernelGGL(k_resolve, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, gv, gm, jobs, ncorr, nsp, zv, zmask,
-                     nz, cval, kahan, spt);
+  if (zonly && !kahan)
+    hipLaunchKernelGGL(k_resolve_z, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, jobs, ncorr, nsp, zv, zmask, nz,
+                       cval, spt);
+  else
+    hipLaunchKernelGGL(k_resolve, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, gv, gm, jobs, ncorr, nsp, zv,
+                       zmask, nz, cval, kahan, spt);
   SLAPH_HIP(hipGetLastError());
 }
 

@@ -419,8 +419,11 @@ void launch_ls_reduce(const double2 *part, long long pstride, const GroupItem *i
                       double2 *out, long long obase, long long ostride, int ntime, hipStream_t s);
 // kahan != 0: Kahan-Neumaier sums inside each (slice pair, correlator) in the
 // reference's element order (every group one trace value, no subset sums).
+// zonly != 0: every job has subset sums (tr[0].zbase >= 0, ntr > 1), so the
+// Moebius form's own kernel may run.
 void launch_resolve(const double2 *gvals, const uint16_t *gmasks, const CorrJob *jobs, int ncorr, int nsp,
-                    const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, int kahan, hipStream_t s);
+                    const double2 *zv, const uint16_t *zmask, int nz, double2 *cval, int kahan, int zonly,
+                    hipStream_t s);
 // sum[t*ncorr + c] += cval[s*ncorr + c] for the slice pairs s of each time
 // separation t, in the order listed: tgroups[g] = (t, offset into sidx,
 // count), sidx ascending within a group.
