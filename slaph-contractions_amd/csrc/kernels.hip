@@ -2685,39 +2685,43 @@ __global__ void k_resolve_z(const CorrJob *__restrict__ jobs, int ncorr, int nsp
   if (id >= (long long)ncorr * nsb) return;
   const int corr = int(id % ncorr), sb = int(id / ncorr);
   const CorrJob J = jobs[corr];
+  auto rfl = [](long long v) {
+    return (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(int(v >> 32)) << 32) |
+                       (unsigned)__builtin_amdgcn_readfirstlane(int(v)));
+  };
+  const int q0 = J.tr[0].zq, q1 = J.tr[1].zq, q2 = J.ntr == 3 ? J.tr[2].zq : 0;
+  const int uq0 = __builtin_amdgcn_readfirstlane(q0), uq1 = __builtin_amdgcn_readfirstlane(q1);
   for (int s = sb * spt; s < min(nsp, (sb + 1) * spt); ++s) {
-  double2 res = make_double2(0.0, 0.0);
     // subset-major sums: the lanes of a wave (jobs ordered by their traces)
-    // mostly share their first two traces, read then through scalar loads,
-    // and read neighbouring items of the last
+    // mostly share their first trace, read then through scalar loads (the
+    // second too when the whole wave shares it), and read neighbouring
+    // items of the last
     const long long b0 = J.tr[0].zbase + s * J.tr[0].zstride, b1 = J.tr[1].zbase + s * J.tr[1].zstride;
     const double2 *z2 = J.ntr == 3 ? zv + J.tr[2].zbase + s * J.tr[2].zstride : nullptr;
-    const int q0 = J.tr[0].zq, q1 = J.tr[1].zq, q2 = J.ntr == 3 ? J.tr[2].zq : 0;
-    auto rfl = [](long long v) {
-      return (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(int(v >> 32)) << 32) |
-                         (unsigned)__builtin_amdgcn_readfirstlane(int(v)));
-    };
     const long long u0 = rfl(b0), u1 = rfl(b1);
-    const int uq0 = __builtin_amdgcn_readfirstlane(q0), uq1 = __builtin_amdgcn_readfirstlane(q1);
-    auto term = [&](double2 t, int q) {
-      if (z2) t = cmul(t, z2[q * q2]);
-      if (__popc(zmask[q]) & 1) {
-        res.x -= t.x;
-        res.y -= t.y;
-      } else {
-        res = cadd(res, t);
+    double2 res = make_double2(0.0, 0.0);
+    auto sum = [&](const double2 *z0, int s0, const double2 *z1, int s1) {
+#pragma unroll 4
+      for (int q = 0; q < nz; ++q) {
+        double2 t = cmul(z0[q * s0], z1[q * s1]);
+        if (z2) t = cmul(t, z2[q * q2]);
+        if (__popc(zmask[q]) & 1) {
+          res.x -= t.x;
+          res.y -= t.y;
+        } else {
+          res = cadd(res, t);
+        }
       }
     };
-    if (__all(b0 == u0 && b1 == u1 && q0 == uq0 && q1 == uq1)) {
-      const double2 *z0 = zv + u0, *z1 = zv + u1;
-#pragma unroll 4
-      for (int q = 0; q < nz; ++q) term(cmul(z0[q * uq0], z1[q * uq1]), q);
+    if (__all(b0 == u0 && q0 == uq0)) {
+      if (__all(b1 == u1 && q1 == uq1))
+        sum(zv + u0, uq0, zv + u1, uq1);
+      else
+        sum(zv + u0, uq0, zv + b1, q1);
     } else {
-      const double2 *z0 = zv + b0, *z1 = zv + b1;
-#pragma unroll 4
-      for (int q = 0; q < nz; ++q) term(cmul(z0[q * q0], z1[q * q1]), q);
+      sum(zv + b0, q0, zv + b1, q1);
     }
-  cval[(long long)s * ncorr + J.out] = make_double2(res.x / J.n, res.y / J.n);
+    cval[(long long)s * ncorr + J.out] = make_double2(res.x / J.n, res.y / J.n);
   }
 }
 
