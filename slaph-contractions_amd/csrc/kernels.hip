@@ -204,24 +204,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   __shared__ double2 Bs[AHB_TN][AHB_KC + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
+  // this thread's four (row, k) slots of a chunk: the row pointers are
+  // resolved once, and the next chunk's elements are loaded into registers
+  // while the current one is multiplied
+  const double2 *arow[4], *brow[4];
+  int kk[4], mm[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int idx = tid + r * 256;
+    kk[r] = idx % AHB_KC;
+    mm[r] = idx / AHB_KC;
+    const int m = m0 + mm[r], n = n0 + mm[r];
+    arow[r] = m < M ? rows[G.job0 + m / rpj].A + (long long)(m % rpj) * lda : nullptr;
+    brow[r] = n < N ? G.B + (long long)n * ldb : nullptr;
+  }
+  double2 an[4], bn[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = k0 + kk[r];
+      an[r] = k < K && arow[r] ? arow[r][k] : make_double2(0.0, 0.0);
+      bn[r] = k < K && brow[r] ? brow[r][k] : make_double2(0.0, 0.0);
+    }
+  };
   // conj(a) b by three real products (Gauss): P1 = ax bx, P2 = ay by,
   // P3 = (ax - ay)(bx + by); re = P1 + P2, im = P3 - P1 + P2
   d4 p1[2][2], p2[2][2], p3[2][2];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) p1[a][b] = p2[a][b] = p3[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+  load(0);
   for (int k0 = 0; k0 < K; k0 += AHB_KC) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int idx = tid + r * 256;
-      const int kk = idx % AHB_KC, mm = idx / AHB_KC;
-      const int k = k0 + kk, m = m0 + mm, n = n0 + mm;
-      double2 av = make_double2(0.0, 0.0), bv = make_double2(0.0, 0.0);
-      if (k < K && m < M) av = rows[G.job0 + m / rpj].A[(long long)(m % rpj) * lda + k];
-      if (k < K && n < N) bv = G.B[(long long)n * ldb + k];
-      As[mm][kk] = av;
-      Bs[mm][kk] = bv;
+      As[mm[r]][kk[r]] = an[r];
+      Bs[mm[r]][kk[r]] = bn[r];
     }
     __syncthreads();
+    if (k0 + AHB_KC < K) load(k0 + AHB_KC);
 #pragma unroll
     for (int kq = 0; kq < AHB_KC; kq += 4) {
       const int kr = kq + (lane >> 4);
