@@ -252,9 +252,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
         for (int ni = 0; ni < 2; ++ni) {
-          p1[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi].x, bv[ni].x, p1[mi][ni], 0, 0, 0);
-          p2[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi].y, bv[ni].y, p2[mi][ni], 0, 0, 0);
-          p3[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(am[mi], bp[ni], p3[mi][ni], 0, 0, 0);
+          // operands swapped: the result block is (C column, C row), so that
+          // lanes 0..15 of a row group hold 16 consecutive rows of C
+          p1[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[ni].x, av[mi].x, p1[mi][ni], 0, 0, 0);
+          p2[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[ni].y, av[mi].y, p2[mi][ni], 0, 0, 0);
+          p3[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(bp[ni], am[mi], p3[mi][ni], 0, 0, 0);
         }
     }
     __syncthreads();
@@ -265,8 +267,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int row = m0 + wm * 32 + mi * 16 + (lane >> 4) + 4 * i;
-        const int col = n0 + wn * 32 + ni * 16 + (lane & 15);
+        const int row = m0 + wm * 32 + mi * 16 + (lane & 15);
+        const int col = n0 + wn * 32 + ni * 16 + (lane >> 4) + 4 * i;
         if (row < M && col < N) {
           const AhBRow R = rows[G.job0 + row / rpj];
           const double q1 = p1[mi][ni][i], q2 = p2[mi][ni][i];
