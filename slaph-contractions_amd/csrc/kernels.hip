@@ -69,11 +69,14 @@ __device__ __forceinline__ void st2_nt(float2 *p, double2 v) {
   f2v w = {float(v.x), float(v.y)};
   __builtin_nontemporal_store(w, reinterpret_cast<f2v *>(p));
 }
+// one 12-byte store per element, so that 64 lanes writing consecutive
+// elements write 768 contiguous bytes (three 4-byte stores each wrote every
+// third word of them)
+typedef unsigned u3v __attribute__((ext_vector_type(3)));
 __device__ __forceinline__ void st2_nt(d48c *p, double2 v) {
   const d48c o = to_d48(v);
-  __builtin_nontemporal_store(o.w[0], &p->w[0]);
-  __builtin_nontemporal_store(o.w[1], &p->w[1]);
-  __builtin_nontemporal_store(o.w[2], &p->w[2]);
+  const u3v w = {o.w[0], o.w[1], o.w[2]};
+  __builtin_nontemporal_store(w, reinterpret_cast<u3v *>(p));
 }
 __device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b) {
   acc.x = fma(a.x, b.x, acc.x);
