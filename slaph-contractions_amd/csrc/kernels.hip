@@ -2542,6 +2542,7 @@ __global__ void k_gemm_reduce(const double2 *__restrict__ part, int KP, const Gr
 // by Moebius inversion: sum_S (-1)^|S| Z0(S) Z1(S) (Z2(S)) over the subset
 // list, the same sum re-associated (nz terms instead of ngroup^ntr).
 // ---------------------------------------------------------------------------
+constexpr int kZetaRegs = 16;  // subset lists up to this length keep their sums in registers
 __global__ void k_zeta(const double2 *__restrict__ gv, long long gbase, long long gstride,
                        const uint16_t *__restrict__ gm, const ZItem *__restrict__ items, int nitems,
                        const uint16_t *__restrict__ zmask, int nz, double2 *__restrict__ zv, long long zbase,
@@ -2555,6 +2556,24 @@ __global__ void k_zeta(const double2 *__restrict__ gv, long long gbase, long lon
   const double2 *v = gv + gbase + s * gstride + I.goff;
   const uint16_t *m = gm + I.gmask_off;
   double2 *out = zv + zbase + s * zstride + it;
+  if (nz <= kZetaRegs) {
+    // every group sum read once, the subset sums in registers (each in the
+    // same order of groups as below)
+    double2 z[kZetaRegs];
+#pragma unroll
+    for (int q = 0; q < kZetaRegs; ++q) z[q] = make_double2(0.0, 0.0);
+    for (int g = 0; g < I.ngroup; ++g) {
+      const double2 x = v[g];
+      const uint16_t mg = m[g];
+#pragma unroll
+      for (int q = 0; q < kZetaRegs; ++q)
+        if (q < nz && (mg & zmask[q]) == zmask[q]) z[q] = cadd(z[q], x);
+    }
+#pragma unroll
+    for (int q = 0; q < kZetaRegs; ++q)
+      if (q < nz) out[(long long)q * nitems] = z[q];
+    return;
+  }
   for (int q = 0; q < nz; ++q) {
     const uint16_t S = zmask[q];
     double2 z = make_double2(0.0, 0.0);
